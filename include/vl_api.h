@@ -1,0 +1,188 @@
+/*
+ * vl_api.h — C ABI of the MI355X (gfx950) rectangle-Boolean engine.
+ *
+ * This is the drop-in boundary for the MAGPARSER `exec run` operator path of
+ * skoranne/VLSI_LAYOUT.  Every function uses plain pointers, sizes and
+ * status codes; nothing here depends on torch or on any C++ type.  Fortran
+ * binds to it with iso_c_binding (vlsi_layout_amd/fortran/vl_layer_ops.f90),
+ * Python with ctypes (vlsi_layout_amd/__init__.py).
+ *
+ * Semantics (point-set, as in the reference): a layer is the union of its
+ * boxes; a box is the half-open integer rectangle [x1,x2) x [y1,y2) and is
+ * valid iff x1<x2 and y1<y2 (src/geometry.f90:21-22,66-70).  Every operator
+ * returns its result as the unique maximal-strip decomposition the reference's
+ * heal_boxes produces (src/polygon_fracture.f90:805-1004): at every x the
+ * covered set is split into maximal y-intervals, and each interval extends in
+ * x as long as it stays identical.  Output order is deterministic (tile order,
+ * then x, then y inside a tile); vl_canonicalize additionally sorts by
+ * (x1,y1,x2,y2).  All arithmetic is exact integer (int32 coordinates, int64
+ * areas).
+ *
+ * Threading: an engine owns one HIP stream; calls on one engine are
+ * serialised by an internal mutex.  Use one engine per host thread for
+ * concurrency.
+ */
+#ifndef VL_API_H
+#define VL_API_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference has none: it error-stops) -------------- */
+#define VL_OK         0
+#define VL_EINVAL     1  /* invalid argument, or an invalid box (x1>=x2 or y1>=y2):
+                            the reference error-stops in LoadKLBin (src/kldata.f90:55-57)
+                            and box_grow_directional (src/geometry.f90:187,192) */
+#define VL_ENOSPC     2  /* caller capacity too small; the required count is returned */
+#define VL_ENOMEM     3  /* device allocation failed */
+#define VL_EHIP       4  /* HIP runtime error */
+#define VL_EASSERT    5  /* ASSERT_ZERO found shapes and abort was requested */
+#define VL_EINTERNAL  7  /* internal consistency check failed */
+
+/* Box: identical layout to the reference's Box (src/geometry.f90:21-22) and
+ * c_box of the Boost bridge (src/boost_polygon_api.h). */
+typedef struct vl_box { int32_t x1, y1, x2, y2; } vl_box;
+
+/* XYTracker: the 12-byte event record of src/commonmodule.f90:30-34 as laid
+ * out by src/event_sorter.cu:15-19. */
+typedef struct vl_tracker { int32_t X, Y; int8_t polygonNumber; } vl_tracker;
+
+typedef struct vl_engine vl_engine;   /* device, stream, scratch, tuning */
+typedef struct vl_layer  vl_layer;    /* a device-resident box layer       */
+
+/* Boost control codes (src/commonmodule.f90:12-14). */
+#define VL_OP_XOR   0
+#define VL_OP_OR    1
+#define VL_OP_AND   2
+#define VL_OP_NOT   3
+#define VL_OP_MERGE 4
+#define VL_OP_SIZE  5
+#define VL_OP_GROW  6   /* not a Boost code: CalculateGROWLayer */
+
+/* ---- engine -------------------------------------------------------------- */
+/* Thread-local text of the last error on this thread. */
+const char* vl_last_error(void);
+int vl_engine_create(int device, vl_engine** out);
+int vl_engine_destroy(vl_engine* e);
+/* Tuning knobs: "tile_nmax" (127|255|511), "tile_target" (boxes per tile),
+ * "level_factor" (row coarsening per seam level), "threads" (workgroup size). */
+int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
+int vl_engine_get_param(vl_engine* e, const char* key, int64_t* value);
+/* Block until all work queued on the engine's stream has finished. */
+int vl_engine_sync(vl_engine* e);
+
+/* ---- layers (the Layer type of src/design.f90:65-82, device-resident) ---- */
+int vl_layer_create(vl_engine* e, vl_layer** out);
+int vl_layer_destroy(vl_layer* l);
+/* Copy n boxes from host memory; every box must be valid (VL_EINVAL). */
+int vl_layer_upload(vl_layer* l, const vl_box* host, int64_t n);
+/* Copy n boxes already in device memory (same device). */
+int vl_layer_upload_device(vl_layer* l, const vl_box* dev, int64_t n);
+/* Copy the layer to host memory; VL_ENOSPC with *n set when cap is too small. */
+int vl_layer_download(const vl_layer* l, vl_box* host, int64_t cap, int64_t* n);
+int vl_layer_size(const vl_layer* l, int64_t* n);
+/* Device address of the layer's boxes (valid until the layer is next written). */
+int vl_layer_device_ptr(const vl_layer* l, const vl_box** dev);
+
+/* ---- operators: the `exec run` surface (src/controlparser.f90:409-585) ---- */
+/* C = A AND B.  Replaces CalculateAND '*' (src/design.f90:1131-1235) and AND2. */
+int vl_and(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c);
+/* C = A NOT B (A \ B).  Replaces CalculateFrameNOT '~'
+ * (src/design_impl.f90:1152-1164), CalculateNOT '@' (src/design.f90:1238-1268)
+ * and the Boost NOT '!' (src/boost_polygon_api.cpp:129-131). */
+int vl_not(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c);
+/* C = A OR B.  Replaces Boost OR '+' and CalculateOR 'OR2' (src/design.f90:1085-1103). */
+int vl_or(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c);
+/* C = A XOR B.  Replaces Boost XOR '^' and CalculateXOR/CalculateExplicitXOR
+ * (src/design_impl.f90:961-991,1166-1177); covers all of B, including the part
+ * outside MBR(A) that the reference's tiled XOR drops (src/design_impl.f90:1030). */
+int vl_xor(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c);
+/* C = GROW(A; E,N,W,S): x1-=W, y1-=S, x2+=E, y2+=N per box, then the union.
+ * Replaces CalculateGROWLayer (src/design_impl.f90:571-586, src/design.f90:242-246).
+ * VL_EINVAL when a grown box is invalid (src/geometry.f90:192). */
+int vl_grow(vl_engine* e, const vl_layer* a, int32_t east, int32_t north, int32_t west,
+            int32_t south, vl_layer* c);
+/* C = SIZE(A; d): d>=0 dilation by the square [-d,d]^2, d<0 erosion by it.
+ * Replaces CalculateBoostOperation(K_BOOST_CONTROL_SIZE) (src/design_impl.f90:931-951). */
+int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c);
+/* C = union of A as maximal strips.  Replaces MergeHealLayer
+ * (src/design_impl.f90:1318-1341), heal_boxes (src/polygon_fracture.f90:805) and
+ * the Boost MERGE (src/boost_polygon_api.cpp:41-70). */
+int vl_merge(vl_engine* e, const vl_layer* a, vl_layer* c);
+/* Box count of A (the ASSERT_ZERO check, src/controlparser.f90:524-530).
+ * Returns VL_EASSERT when *count>0 and abort_on_nonzero!=0. */
+int vl_assert_zero(vl_engine* e, const vl_layer* a, int abort_on_nonzero, int64_t* count);
+/* Exact int64 union area (calculate_union_area_fast, src/geometry.f90:1840-1927;
+ * src/klee.cu:164-210). */
+int vl_klee_area(vl_engine* e, const vl_layer* a, int64_t* area);
+/* C = merge(A) sorted by (x1,y1,x2,y2): the canonical form used for parity. */
+int vl_canonicalize(vl_engine* e, const vl_layer* a, vl_layer* c);
+/* *empty = 1 iff A XOR B is empty (the BOOST_XOR_CHECKER exit code,
+ * src/boost_xor_checker.cpp:151), computed as the area of A XOR B on the GPU. */
+int vl_xor_empty(vl_engine* e, const vl_layer* a, const vl_layer* b, int* empty);
+
+/* One entry for all operators with the Boost bridge's op codes and the host
+ * pointers of PerformBoostPolygonOperation (src/boost_polygon_api.cpp:97-105),
+ * plus the output capacity the reference lacks: VL_ENOSPC with *out_n set to
+ * the required count when cap is too small.  ctrl: VL_OP_*; val: SIZE amount,
+ * or for VL_OP_GROW the same amount on all four sides. */
+int vl_boolean_op(vl_engine* e, const vl_box* a, uint64_t an, const vl_box* b, uint64_t bn,
+                  vl_box* out, uint64_t cap, uint64_t* out_n, uint64_t ctrl, int64_t val);
+
+/* ---- sorts (the Thrust prototypes' roles) --------------------------------- */
+/* Stable sort of n trackers by ((X^0x80000000)<<32)|(Y^0x80000000).
+ * Replaces device_event_sort (src/event_sorter.cu:48-81).  arr may be host or
+ * device memory. */
+int vl_event_sort(vl_engine* e, vl_tracker* arr, int64_t n);
+/* Stable sort of boxes by the 64-bit Morton code of their centre
+ * ((x1+x2)/2 ^ 0x80000000, y in odd bits).  Replaces thrust_sort_boxes_direct
+ * (src/gpu_morton.cu:149-164).  Host or device memory. */
+int vl_morton_sort_direct(vl_engine* e, vl_box* boxes, int64_t n);
+/* 1-based permutation that Morton-sorts boxes (boxes untouched).  Replaces
+ * thrust_sort_boxes_indirect (src/gpu_morton.cu:169-187).  Host or device memory. */
+int vl_morton_sort_indirect(vl_engine* e, const vl_box* boxes, int64_t* idx, int64_t n);
+
+/* Same-name, same-signature entry points for the reference's Fortran
+ * interfaces (src/event_sorter_apif.f90:17-23, src/gpu_morton_sort.f90:23-36).
+ * They use a process-wide engine on device 0 and abort the process on error,
+ * as the reference's error stop would. */
+void device_event_sort(vl_tracker* d_arr, int64_t n);
+void thrust_sort_boxes_direct(vl_box* d_boxes, int64_t n);
+void thrust_sort_boxes_indirect(vl_box* d_boxes, int64_t* d_indices, int64_t n);
+
+/* ---- batched form ------------------------------------------------------- */
+/* A program is a recorded list of operators over layers.  vl_program_run
+ * executes independent operators of one step together (one fused tile launch
+ * per phase across all of them) and replays the whole list as one hipGraph
+ * once its worst-case scratch is planned.  Layers used by a program must not
+ * be resized by other calls while it runs. */
+typedef struct vl_program vl_program;
+int vl_program_create(vl_engine* e, vl_program** out);
+int vl_program_destroy(vl_program* p);
+/* op: VL_OP_*; args: GROW E,N,W,S / SIZE d in args[0] / unused.  b may be NULL
+ * for unary operators.  VL_OP_MERGE with a=b is the union; ASSERT_ZERO is
+ * recorded with op = -1 and reports through vl_program_assert_count. */
+int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, vl_layer* c,
+                   const int32_t* args);
+int vl_program_run(vl_program* p);
+/* Number of ASSERT_ZERO checks of the last run that found shapes. */
+int vl_program_assert_count(vl_program* p, int64_t* failures);
+
+/* ---- timing ---------------------------------------------------------------- */
+/* Device time (ms) of the last operator / program run on this engine, measured
+ * with HIP events on the engine stream from its first to its last kernel. */
+int vl_engine_last_time_ms(vl_engine* e, double* ms);
+/* Per-kernel statistics collected while the "profile" parameter is 1 (setting
+ * it resets them).  kernel: "tile" (the per-tile Boolean kernel), "bin" (tile
+ * binning), "sort" (radix sort passes).  *bytes is the algorithmic byte count
+ * of those launches (tile: 16 B per entry read + 16 B per strip written + 48 B
+ * per tile of descriptor and look-back state). */
+int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms,
+                           double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

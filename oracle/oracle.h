@@ -1,0 +1,75 @@
+/*
+ * oracle.h — CPU restatement of the reference's rectangle-Boolean path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (vlsi_layout_amd/) links,
+ * loads or calls this code.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, and only as the checker / CPU baseline.
+ *
+ * Every function names the reference file:line it restates (paths relative to
+ * the reference repository root).  Pinning of each function (golden vectors,
+ * known answers, brute-force rasterisation, the flang-built reference routines
+ * under oracle/_ref) is listed in the header of oracle.c.
+ */
+#ifndef VL_ORACLE_H
+#define VL_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Half-open integer rectangle [x1,x2) x [y1,y2); src/geometry.f90:21-22. */
+typedef struct { int32_t x1, y1, x2, y2; } ob_box;
+
+/* 12-byte XYTracker; src/commonmodule.f90:30-34, src/event_sorter.cu:15-19. */
+typedef struct { int32_t X, Y; int8_t w; } ob_tracker;
+
+/* All list-producing functions return the element count and store a malloc'd
+ * array in *out (NULL when the count is 0).  Free with or_free(). */
+void    or_free(void* p);
+
+/* heal_boxes: corner-token x-sweep -> maximal y-runs extended along x
+ * (src/polygon_fracture.f90:805-1004). */
+int64_t or_heal(const ob_box* in, int64_t n, ob_box** out);
+/* heal then sort by (x1,y1,x2,y2): the canonical form used for parity. */
+int64_t or_canonical(const ob_box* in, int64_t n, ob_box** out);
+/* Complement of the union of b inside bbox: generate_trackers + the same sweep
+ * (src/polygon_fracture.f90:767-803, 361-610). */
+int64_t or_complement(const ob_box* b, int64_t n, ob_box bbox, ob_box** out);
+
+/* Exact int64 Klee area: calculate_union_area_fast (src/geometry.f90:1840-1972). */
+int64_t or_klee_area(const ob_box* in, int64_t n);
+
+/* Operators (each output is in canonical maximal-strip form, sorted). */
+int64_t or_and(const ob_box* a, int64_t na, const ob_box* b, int64_t nb, ob_box** out);
+int64_t or_not(const ob_box* a, int64_t na, const ob_box* b, int64_t nb, ob_box** out);
+int64_t or_or (const ob_box* a, int64_t na, const ob_box* b, int64_t nb, ob_box** out);
+int64_t or_xor(const ob_box* a, int64_t na, const ob_box* b, int64_t nb, ob_box** out);
+/* GROW args are E,N,W,S (src/geometry.f90:184-193).  Returns -1 when a grown
+ * box becomes invalid (the reference error-stops there). */
+int64_t or_grow(const ob_box* a, int64_t na, int32_t e, int32_t n, int32_t w, int32_t s, ob_box** out);
+/* SIZE d (Boost resize semantics restated; src/boost_polygon_api.cpp:132-135). */
+int64_t or_size(const ob_box* a, int64_t na, int32_t d, ob_box** out);
+
+/* Pairwise positive-area intersections (CalculateAND fragment loop,
+ * src/design.f90:1168-1194), before healing. */
+int64_t or_and_fragments(const ob_box* a, int64_t na, const ob_box* b, int64_t nb, ob_box** out);
+
+/* Sorts (library equivalence with std::stable_sort on the reference's keys). */
+void    or_event_sort(ob_tracker* t, int64_t n);                  /* src/event_sorter.cu:48-81 */
+uint64_t or_morton_key(ob_box b);                                  /* src/gpu_morton.cu:113-144 */
+void    or_morton_sort_direct(ob_box* b, int64_t n);               /* src/gpu_morton.cu:149-164 */
+void    or_morton_sort_indirect(const ob_box* b, int64_t* idx, int64_t n); /* :169-187 */
+
+/* CPU baseline: the same operators run tile-parallel with OpenMP the way the
+ * reference's frame functions run (src/design_impl.f90:993-1129, 1193-1316):
+ * K_GRID = floor(nodes^(1/4)) tiles per side, one task per tile.  Output is the
+ * concatenation of per-tile results (tile seams remain, as in the reference).
+ * op: 0=XOR 1=OR 2=AND 3=NOT 4=MERGE 5=SIZE 6=GROW (args[0..3] = E,N,W,S or d). */
+int64_t or_par_op(int op, const ob_box* a, int64_t na, const ob_box* b, int64_t nb,
+                  const int32_t* args, int nthreads, ob_box** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

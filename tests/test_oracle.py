@@ -1,0 +1,189 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Checks, per oracle function:
+  * golden vectors written by the reference's own heal_boxes / scanline_fracture /
+    calculate_union_area_fast (flang build, tests/golden/make_golden.py) on the
+    reference's SEDFXTP1 and square KLBIN fixtures and seeded random layers —
+    heal output must match element by element, in the reference's order;
+  * known answers quoted in the reference (src/klee.cu:214-224,
+    src/thrust_boolean.cu:546, src/test_rectboolean.f90, src/test_boxmerge.f90);
+  * brute-force rasterisation on small grids for every operator;
+  * algebraic identities on larger random layers;
+  * library equivalence for the sorts.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from raster import dilate, erode, raster, strips_from_raster
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+META = json.load(open(os.path.join(GOLD, "golden.json")))
+
+
+def load(path):
+    return np.fromfile(path, dtype=np.int32).reshape(-1, 4)
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_heal_matches_reference_golden(name):
+    a = load(os.path.join(GOLD, "inputs", name + ".bin"))
+    exp = load(os.path.join(GOLD, "expected", name + ".heal.bin"))
+    got = O.heal(a)
+    assert np.array_equal(got, exp), name          # same boxes, same emission order
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_area_matches_reference_golden(name):
+    a = load(os.path.join(GOLD, "inputs", name + ".bin"))
+    assert O.klee_area(a) == META[name]["area"]
+    assert O.klee_area(O.canonical(a)) == META[name]["area"]
+
+
+@pytest.mark.parametrize("name", [k for k in sorted(META) if META[k]["n"] > 0])
+def test_complement_matches_reference_golden(name):
+    a = load(os.path.join(GOLD, "inputs", name + ".bin"))
+    exp = load(os.path.join(GOLD, "expected", name + ".complement.bin"))
+    bb = META[name]["complement_bbox"]
+    assert np.array_equal(O.canonical(O.complement(a, bb)), O.canonical(exp))
+
+
+def test_known_answers():
+    assert O.klee_area([[0, 0, 10, 10], [5, 5, 15, 15]]) == 175                    # src/klee.cu:214-224
+    assert O.op_and([[0, 0, 10, 10]], [[5, 5, 20, 20]]).tolist() == [[5, 5, 10, 10]]  # thrust_boolean.cu:546
+    # (0,0,10,10) minus the hole (2,2,8,8): src/test_rectboolean.f90:12,44 (fragments of
+    # src/rectboolean.f90:164-170; the canonical strips of the same point set)
+    got = O.op_not([[0, 0, 10, 10]], [[2, 2, 8, 8]])
+    assert got.tolist() == [[0, 0, 2, 10], [2, 0, 8, 2], [2, 8, 8, 10], [8, 0, 10, 10]]
+    assert O.klee_area(got) == 100 - 36
+    # touching boxes merge: src/test_boxmerge.f90:15-17
+    assert O.heal([[0, 0, 2, 2], [2, 0, 4, 2]]).tolist() == [[0, 0, 4, 2]]
+    # a box inside another: src/test_boxmerge.f90:19-21
+    assert O.canonical([[10, 10, 20, 20], [12, 12, 15, 15]]).tolist() == [[10, 10, 20, 20]]
+    # L-shape: src/test_boxmerge.f90:23-25
+    assert O.canonical([[30, 30, 40, 40], [35, 35, 45, 45]]).tolist() == [
+        [30, 30, 35, 40], [35, 30, 40, 45], [40, 35, 45, 45]]
+    # corner-only touch does not merge, edge touch of unequal spans does not extend
+    assert O.canonical([[0, 0, 1, 1], [1, 1, 2, 2]]).tolist() == [[0, 0, 1, 1], [1, 1, 2, 2]]
+
+
+def rand_boxes(rng, n, span, wmax, off=0):
+    x = rng.integers(off, off + span, n)
+    y = rng.integers(off, off + span, n)
+    w = rng.integers(1, wmax + 1, n)
+    h = rng.integers(1, wmax + 1, n)
+    return np.stack([x, y, x + w, y + h], 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_ops_against_raster(seed):
+    rng = np.random.default_rng(seed)
+    span, wmax = 24, 9
+    a = rand_boxes(rng, int(rng.integers(0, 14)), span, wmax)
+    b = rand_boxes(rng, int(rng.integers(0, 14)), span, wmax)
+    M = 8                              # margin for GROW/SIZE
+    ox = oy = -M
+    W = H = span + wmax + 2 * M
+    ga, gb = raster(a, ox, oy, W, H), raster(b, ox, oy, W, H)
+    exp = {
+        "and": ga & gb, "not": ga & ~gb, "or": ga | gb, "xor": ga ^ gb,
+    }
+    for op, g in exp.items():
+        got = getattr(O, "op_" + op)(a, b)
+        assert np.array_equal(got, strips_from_raster(g, ox, oy)), (op, seed)
+    assert np.array_equal(O.canonical(a), strips_from_raster(ga, ox, oy))
+    assert O.klee_area(a) == int(ga.sum())
+    e, n, w, s = (int(v) for v in rng.integers(0, 4, 4))
+    assert np.array_equal(O.grow(a, e, n, w, s), strips_from_raster(dilate(ga, e, n, w, s), ox, oy))
+    d = int(rng.integers(1, 4))
+    assert np.array_equal(O.size(a, d), strips_from_raster(dilate(ga, d, d, d, d), ox, oy))
+    assert np.array_equal(O.size(a, -d), strips_from_raster(erode(ga, d), ox, oy))
+    assert np.array_equal(O.size(a, 0), strips_from_raster(ga, ox, oy))
+
+
+def test_size_single_box_closed_form():
+    assert O.size([[0, 0, 10, 10]], -1).tolist() == [[1, 1, 9, 9]]
+    assert O.size([[0, 0, 10, 10]], -5).tolist() == []
+    assert O.size([[0, 0, 10, 4]], -2).tolist() == []
+    assert O.size([[0, 0, 10, 10]], 3).tolist() == [[-3, -3, 13, 13]]
+    with pytest.raises(ValueError):
+        O.grow([[0, 0, 2, 2]], -2, 0, 0, 0)      # box becomes invalid (geometry.f90:192)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_identities_medium(seed):
+    rng = np.random.default_rng(100 + seed)
+    a = rand_boxes(rng, 3000, 20000, 400, off=-10000)
+    b = rand_boxes(rng, 3000, 20000, 400, off=-10000)
+    A, B = O.klee_area(a), O.klee_area(b)
+    AND, NOT, OR = O.op_and(a, b), O.op_not(a, b), O.op_or(a, b)
+    assert O.klee_area(AND) + O.klee_area(NOT) == A
+    assert O.klee_area(OR) + O.klee_area(AND) == A + B
+    assert len(O.op_and(NOT, b)) == 0
+    assert len(O.op_xor(a, a)) == 0
+    assert np.array_equal(O.op_xor(a, b), O.op_or(O.op_not(a, b), O.op_not(b, a)))
+    # OP1 ring invariants (src/op_template.m4:14-22)
+    L = O.canonical(a)
+    L1, L2 = O.grow(L, 1, 1, 1, 1), O.size(L, -1)
+    outer, inner = O.op_not(L1, L), O.op_not(L, L2)
+    assert len(O.op_and(outer, L)) == 0
+    assert len(O.op_and(outer, inner)) == 0
+
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed", range(10))
+def test_oracle_against_reference_build(seed):
+    rng = np.random.default_rng(1000 + seed)
+    a = rand_boxes(rng, int(rng.integers(2, 800)), 3000, 200, off=-1500)
+    assert np.array_equal(O.heal(a), O.ref_heal(a))
+    assert O.klee_area(a) == O.ref_union_area(a)
+    bb = (-2000, -2000, 2000, 2000)
+    assert np.array_equal(O.canonical(O.complement(a, bb)), O.canonical(O.ref_complement(a, bb)))
+
+
+def test_event_sort_is_stable_key_sort():
+    rng = np.random.default_rng(7)
+    n = 5000
+    t = np.zeros(n, dtype=[("X", "<i4"), ("Y", "<i4"), ("w", "i1"), ("pad", "V3")])
+    t["X"] = rng.integers(-50, 50, n)
+    t["Y"] = rng.integers(-50, 50, n)
+    t["w"] = np.arange(n) % 100                      # payload to check stability
+    ref = sorted(range(n), key=lambda i: ((int(t["X"][i]) ^ -0x80000000) & 0xFFFFFFFF,
+                                          (int(t["Y"][i]) ^ -0x80000000) & 0xFFFFFFFF, i))
+    got = O.event_sort(t.copy())
+    assert np.array_equal(got, t[ref])
+
+
+def test_morton_key_and_sorts():
+    # centre (0,0) -> cx = cy = 0x80000000: only the top bits of both set
+    assert O.morton_key([-1, -1, 1, 1]) == (1 << 63) | (1 << 62)
+    # centre (1, 0): bit 0 from x
+    assert O.morton_key([0, -1, 2, 1]) == (1 << 63) | (1 << 62) | 1
+    # negative odd sums truncate toward zero: (-3 + 0) / 2 = -1
+    assert O.morton_key([-3, 0, 0, 0]) & 1 == 1
+    rng = np.random.default_rng(3)
+    b = rand_boxes(rng, 4000, 1 << 12, 50, off=-(1 << 11))
+    keys = np.array([O.morton_key(x) for x in b], dtype=np.uint64)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(O.morton_sort_direct(b), b[order])
+    assert np.array_equal(O.morton_sort_indirect(b), order + 1)
+
+
+@pytest.mark.parametrize("op", ["and", "not", "or", "xor", "merge", "grow", "size"])
+def test_cpu_baseline_equals_oracle(op):
+    rng = np.random.default_rng(55)
+    a = rand_boxes(rng, 6000, 40000, 500)
+    b = rand_boxes(rng, 6000, 40000, 500)
+    if op in ("and", "not", "or", "xor"):
+        got = O.par_op(op, a, b)
+        exp = getattr(O, "op_" + op)(a, b)
+    elif op == "merge":
+        got, exp = O.par_op("merge", a), O.canonical(a)
+    elif op == "grow":
+        got, exp = O.par_op("grow", a, args=(2, 3, 1, 4)), O.grow(a, 2, 3, 1, 4)
+    else:
+        got, exp = O.par_op("size", a, args=(-2,)), O.size(a, -2)
+    assert np.array_equal(O.canonical(got), exp)

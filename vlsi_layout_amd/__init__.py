@@ -1,0 +1,429 @@
+"""vlsi_layout_amd — MI355X (gfx950) engine for the rectangle-Boolean `exec run`
+operators of skoranne/VLSI_LAYOUT, driven from Python through its C ABI.
+
+The compute runs in the HIP library ``libvlayout.so`` built from ``csrc/``
+(``python -m vlsi_layout_amd.build``).  This module is a thin ctypes layer over
+``include/vl_api.h``; it mirrors the reference's operator surface
+(src/design.f90:20-31, src/design_impl.f90, src/controlparser.f90:409-585):
+
+    CalculateAND(A, B, C)            '*'   src/design.f90:1131-1235
+    CalculateFrameNOT(A, B, C)       '~'   src/design_impl.f90:1152-1164
+    CalculateNOT(A, B, C)            '@'   src/design.f90:1238-1268
+    CalculateOR(A, B, C)             OR2   src/design.f90:1085-1103
+    CalculateXOR(A, B, C)            XOR   src/design_impl.f90:1166-1177
+    CalculateGROWLayer(A, C, ivar)   GROW  src/design_impl.f90:571-586 (ivar = E,N,W,S)
+    CalculateBoostOperation(A, B, C, ctrl, val)   src/design_impl.f90:931-951
+    MergeHealLayer(L)                      src/design_impl.f90:1318-1341
+    heal_boxes(boxes)                      src/polygon_fracture.f90:805-1004
+    device_event_sort / thrust_sort_boxes_direct / thrust_sort_boxes_indirect
+
+Layers live in device memory; ``Layer.boxes()`` copies them to the host as an
+(n, 4) int32 numpy array of (x1, y1, x2, y2).  There is no CPU fallback: if
+the library is missing every call raises.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+__all__ = [
+    "VLError", "Engine", "Layer", "lib", "engine",
+    "CalculateAND", "CalculateFrameNOT", "CalculateNOT", "CalculateOR", "CalculateXOR",
+    "CalculateGROWLayer", "CalculateBoostOperation", "MergeHealLayer", "heal_boxes",
+    "ASSERT_ZERO", "klee_area", "canonicalize", "xor_empty",
+    "device_event_sort", "thrust_sort_boxes_direct", "thrust_sort_boxes_indirect",
+    "synth_layer", "synth_tile_count",
+    "K_BOOST_CONTROL_XOR", "K_BOOST_CONTROL_OR", "K_BOOST_CONTROL_AND", "K_BOOST_CONTROL_NOT",
+    "K_BOOST_CONTROL_MERGE", "K_BOOST_CONTROL_SIZE",
+]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libvlayout.so")
+
+# Boost control codes (src/commonmodule.f90:12-14)
+K_BOOST_CONTROL_XOR, K_BOOST_CONTROL_OR, K_BOOST_CONTROL_AND = 0, 1, 2
+K_BOOST_CONTROL_NOT, K_BOOST_CONTROL_MERGE, K_BOOST_CONTROL_SIZE = 3, 4, 5
+VL_OP_GROW = 6
+
+VL_OK, VL_EINVAL, VL_ENOSPC, VL_ENOMEM, VL_EHIP, VL_EASSERT, VL_EINTERNAL = 0, 1, 2, 3, 4, 5, 7
+
+SYN_UNIFORM, SYN_STDCELL, SYN_TILES, SYN_CHIP = 1, 2, 3, 4
+
+
+class VLError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class _Tracker(C.Structure):
+    _fields_ = [("X", C.c_int32), ("Y", C.c_int32), ("polygonNumber", C.c_int8)]
+
+
+TRACKER_DTYPE = np.dtype({"names": ["X", "Y", "polygonNumber"], "formats": [np.int32, np.int32, np.int8],
+                          "offsets": [0, 4, 8], "itemsize": 12})
+
+
+def lib():
+    """The loaded engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise VLError(VL_EINVAL, f"{LIB_PATH} is not built; run python -m vlsi_layout_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
+        sig = {
+            "vl_last_error": (C.c_char_p, []),
+            "vl_engine_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+            "vl_engine_destroy": (C.c_int, [vp]),
+            "vl_engine_set_param": (C.c_int, [vp, C.c_char_p, i64]),
+            "vl_engine_get_param": (C.c_int, [vp, C.c_char_p, C.POINTER(i64)]),
+            "vl_engine_sync": (C.c_int, [vp]),
+            "vl_engine_last_time_ms": (C.c_int, [vp, C.POINTER(C.c_double)]),
+            "vl_layer_create": (C.c_int, [vp, C.POINTER(vp)]),
+            "vl_layer_destroy": (C.c_int, [vp]),
+            "vl_layer_upload": (C.c_int, [vp, vp, i64]),
+            "vl_layer_upload_device": (C.c_int, [vp, vp, i64]),
+            "vl_layer_download": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
+            "vl_layer_size": (C.c_int, [vp, C.POINTER(i64)]),
+            "vl_layer_device_ptr": (C.c_int, [vp, C.POINTER(vp)]),
+            "vl_and": (C.c_int, [vp, vp, vp, vp]),
+            "vl_not": (C.c_int, [vp, vp, vp, vp]),
+            "vl_or": (C.c_int, [vp, vp, vp, vp]),
+            "vl_xor": (C.c_int, [vp, vp, vp, vp]),
+            "vl_grow": (C.c_int, [vp, vp, i32, i32, i32, i32, vp]),
+            "vl_size": (C.c_int, [vp, vp, i32, vp]),
+            "vl_merge": (C.c_int, [vp, vp, vp]),
+            "vl_assert_zero": (C.c_int, [vp, vp, C.c_int, C.POINTER(i64)]),
+            "vl_klee_area": (C.c_int, [vp, vp, C.POINTER(i64)]),
+            "vl_canonicalize": (C.c_int, [vp, vp, vp]),
+            "vl_xor_empty": (C.c_int, [vp, vp, vp, C.POINTER(C.c_int)]),
+            "vl_boolean_op": (C.c_int, [vp, vp, C.c_uint64, vp, C.c_uint64, vp, C.c_uint64,
+                                        C.POINTER(C.c_uint64), C.c_uint64, i64]),
+            "vl_event_sort": (C.c_int, [vp, vp, i64]),
+            "vl_morton_sort_direct": (C.c_int, [vp, vp, i64]),
+            "vl_morton_sort_indirect": (C.c_int, [vp, vp, vp, i64]),
+            "device_event_sort": (None, [vp, i64]),
+            "thrust_sort_boxes_direct": (None, [vp, i64]),
+            "thrust_sort_boxes_indirect": (None, [vp, vp, i64]),
+            "vl_program_create": (C.c_int, [vp, C.POINTER(vp)]),
+            "vl_program_destroy": (C.c_int, [vp]),
+            "vl_program_add": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
+            "vl_program_run": (C.c_int, [vp]),
+            "vl_program_assert_count": (C.c_int, [vp, C.POINTER(i64)]),
+            "vl_synth_layer": (C.c_int, [C.c_int, C.c_uint64, C.c_int, i64, vp]),
+            "vl_synth_tile_count": (i64, [C.c_uint64, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        return L
+
+
+def _check(rc):
+    if rc != VL_OK:
+        msg = lib().vl_last_error()
+        raise VLError(rc, msg.decode() if msg else "")
+
+
+def as_boxes(a):
+    """(n, 4) int32 C-contiguous array of (x1, y1, x2, y2)."""
+    a = np.asarray(a, dtype=np.int32)
+    return np.ascontiguousarray(a.reshape(-1, 4))
+
+
+class Engine:
+    """One device, one HIP stream, a scratch pool and tuning parameters."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().vl_engine_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vl_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, key, value):
+        _check(lib().vl_engine_set_param(self._h, key.encode(), int(value)))
+
+    def get(self, key):
+        v = C.c_int64()
+        _check(lib().vl_engine_get_param(self._h, key.encode(), C.byref(v)))
+        return v.value
+
+    def sync(self):
+        _check(lib().vl_engine_sync(self._h))
+
+    def last_ms(self):
+        v = C.c_double()
+        _check(lib().vl_engine_last_time_ms(self._h, C.byref(v)))
+        return v.value
+
+    def layer(self, boxes=None):
+        return Layer(self, boxes)
+
+
+class Layer:
+    """A device-resident box layer (the reference's type(Layer), src/design.f90:65-82)."""
+
+    def __init__(self, eng, boxes=None):
+        self.engine = eng
+        h = C.c_void_p()
+        _check(lib().vl_layer_create(eng.handle, C.byref(h)))
+        self._h = h
+        if boxes is not None:
+            self.upload(boxes)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def upload(self, boxes):
+        b = as_boxes(boxes)
+        _check(lib().vl_layer_upload(self._h, b.ctypes.data_as(C.c_void_p), len(b)))
+        return self
+
+    @property
+    def n_used(self):
+        v = C.c_int64()
+        _check(lib().vl_layer_size(self._h, C.byref(v)))
+        return v.value
+
+    def __len__(self):
+        return self.n_used
+
+    def boxes(self):
+        n = self.n_used
+        out = np.zeros((n, 4), np.int32)
+        got = C.c_int64()
+        _check(lib().vl_layer_download(self._h, out.ctypes.data_as(C.c_void_p), n, C.byref(got)))
+        return out
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        _check(lib().vl_layer_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vl_layer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def engine():
+    """Process-wide engine on device 0."""
+    global _default
+    if _default is None:
+        _default = Engine(0)
+    return _default
+
+
+def _binop(fn, A, B, Cl):
+    _check(getattr(lib(), fn)(A.engine.handle, A.handle, B.handle, Cl.handle))
+    return Cl
+
+
+def CalculateAND(A, B, C_out):
+    """C = A AND B  ('*', src/design.f90:1131-1235)."""
+    return _binop("vl_and", A, B, C_out)
+
+
+def CalculateFrameNOT(A, B, C_out):
+    """C = A NOT B  ('~', src/design_impl.f90:1152-1164)."""
+    return _binop("vl_not", A, B, C_out)
+
+
+def CalculateNOT(A, B, C_out):
+    """C = A NOT B  ('@', src/design.f90:1238-1268): same point set as '~'."""
+    return _binop("vl_not", A, B, C_out)
+
+
+def CalculateOR(A, B, C_out):
+    """C = A OR B  (OR2, src/design.f90:1085-1103; '+' Boost OR)."""
+    return _binop("vl_or", A, B, C_out)
+
+
+def CalculateXOR(A, B, C_out):
+    """C = A XOR B  (XOR / '^', src/design_impl.f90:1166-1177)."""
+    return _binop("vl_xor", A, B, C_out)
+
+
+def CalculateGROWLayer(A, C_out, ivar):
+    """C = GROW(A; E, N, W, S) in DBU (src/design_impl.f90:571-586)."""
+    e, n, w, s = (int(v) for v in ivar)
+    _check(lib().vl_grow(A.engine.handle, A.handle, e, n, w, s, C_out.handle))
+    return C_out
+
+
+def CalculateBoostOperation(A, B, C_out, control_parameter, control_value=0):
+    """Boost-bridge operator codes (src/design_impl.f90:931-951)."""
+    e = A.engine.handle
+    cp = int(control_parameter)
+    if cp == K_BOOST_CONTROL_XOR:
+        return CalculateXOR(A, B, C_out)
+    if cp == K_BOOST_CONTROL_OR:
+        return CalculateOR(A, B, C_out)
+    if cp == K_BOOST_CONTROL_AND:
+        return CalculateAND(A, B, C_out)
+    if cp == K_BOOST_CONTROL_NOT:
+        return CalculateFrameNOT(A, B, C_out)
+    if cp == K_BOOST_CONTROL_MERGE:
+        _check(lib().vl_merge(e, A.handle, C_out.handle))
+        return C_out
+    if cp == K_BOOST_CONTROL_SIZE:
+        _check(lib().vl_size(e, A.handle, int(control_value), C_out.handle))
+        return C_out
+    raise VLError(VL_EINVAL, f"unknown control parameter {control_parameter}")
+
+
+def MergeHealLayer(L):
+    """Replace L by the maximal-strip union of its boxes (src/design_impl.f90:1318-1341)."""
+    _check(lib().vl_merge(L.engine.handle, L.handle, L.handle))
+    return L
+
+
+def ASSERT_ZERO(L, abort=False):
+    """Box count of L; raises VLError(VL_EASSERT) when abort and non-empty
+    (src/controlparser.f90:524-530)."""
+    v = C.c_int64()
+    _check(lib().vl_assert_zero(L.engine.handle, L.handle, 1 if abort else 0, C.byref(v)))
+    return v.value
+
+
+def klee_area(L):
+    v = C.c_int64()
+    _check(lib().vl_klee_area(L.engine.handle, L.handle, C.byref(v)))
+    return v.value
+
+
+def canonicalize(L, out=None):
+    out = out if out is not None else Layer(L.engine)
+    _check(lib().vl_canonicalize(L.engine.handle, L.handle, out.handle))
+    return out
+
+
+def xor_empty(A, B):
+    v = C.c_int()
+    _check(lib().vl_xor_empty(A.engine.handle, A.handle, B.handle, C.byref(v)))
+    return bool(v.value)
+
+
+def heal_boxes(boxes, eng=None):
+    """heal_boxes on host boxes (src/polygon_fracture.f90:805-1004): returns the
+    maximal-strip union as an (m, 4) array."""
+    eng = eng or engine()
+    L = Layer(eng, boxes)
+    MergeHealLayer(L)
+    return L.boxes()
+
+
+def device_event_sort(trackers, eng=None):
+    """Stable sort of XYTracker records by (X, Y) (src/event_sorter.cu:48-81).
+    trackers: numpy array with TRACKER_DTYPE (12-byte records); sorted in place."""
+    t = np.ascontiguousarray(trackers)
+    if t.dtype != TRACKER_DTYPE:
+        raise VLError(VL_EINVAL, "trackers must use TRACKER_DTYPE")
+    eng = eng or engine()
+    _check(lib().vl_event_sort(eng.handle, t.ctypes.data_as(C.c_void_p), len(t)))
+    return t
+
+
+def thrust_sort_boxes_direct(boxes, eng=None):
+    """Morton-sort boxes by centre (src/gpu_morton.cu:149-164); returns a sorted copy."""
+    b = as_boxes(boxes).copy()
+    eng = eng or engine()
+    _check(lib().vl_morton_sort_direct(eng.handle, b.ctypes.data_as(C.c_void_p), len(b)))
+    return b
+
+
+def thrust_sort_boxes_indirect(boxes, eng=None):
+    """1-based Morton permutation (src/gpu_morton.cu:169-187)."""
+    b = as_boxes(boxes)
+    idx = np.zeros(len(b), np.int64)
+    eng = eng or engine()
+    _check(lib().vl_morton_sort_indirect(eng.handle, b.ctypes.data_as(C.c_void_p),
+                                         idx.ctypes.data_as(C.c_void_p), len(b)))
+    return idx
+
+
+class Program:
+    """Recorded operator list run as one batch (vl_program_*)."""
+
+    def __init__(self, eng):
+        self.engine = eng
+        h = C.c_void_p()
+        _check(lib().vl_program_create(eng.handle, C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def add(self, op, a, b=None, c=None, args=None):
+        arr = None
+        if args is not None:
+            arr = (C.c_int32 * 4)(*([int(v) for v in args] + [0] * (4 - len(args))))
+            self._keep.append(arr)
+        _check(lib().vl_program_add(self._h, int(op), a.handle, b.handle if b is not None else None,
+                                    c.handle if c is not None else None, arr))
+
+    def run(self):
+        _check(lib().vl_program_run(self._h))
+
+    def assert_failures(self):
+        v = C.c_int64()
+        _check(lib().vl_program_assert_count(self._h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vl_program_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_layer(config, seed, layer, n):
+    out = np.zeros((int(n), 4), np.int32)
+    rc = lib().vl_synth_layer(int(config), int(seed), int(layer), int(n), out.ctypes.data_as(C.c_void_p))
+    if rc != VL_OK:
+        raise VLError(rc, "vl_synth_layer: bad arguments")
+    return out
+
+
+def synth_tile_count(seed, tile):
+    return int(lib().vl_synth_tile_count(int(seed), int(tile)))

@@ -1,0 +1,1180 @@
+// vl_engine.hip — host orchestration of the gfx950 Boolean engine and the C ABI
+// declared in include/vl_api.h.
+//
+// One operator (plain form) runs as:
+//   bounding boxes (one reduction kernel per input, validating every box)
+//   -> uniform tile grid over the op's domain (host arithmetic)
+//   -> binning: count, scan, scatter (boxes transformed on the fly for GROW)
+//   -> tile kernel (level 0) -> final strips + seam side stream
+//   -> seam levels: side strips touching internal row borders are re-tiled on
+//      a grid with `level_factor` times taller rows until none remain; strips
+//      only touching column borders are collected for the x-stitch
+//   -> x-stitch joins pieces along x into maximal strips.
+// Tiles with more boxes than the kernel capacity are split into x-sub-columns
+// and processed in a follow-up launch.  Every launch writes at offsets from
+// the tile kernel's look-back, so results are deterministic.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "vl_internal.h"
+
+using namespace vl;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+static bool g_debug = getenv("VL_DEBUG") != nullptr;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCK(expr)                                                                      \
+  do {                                                                                   \
+    hipError_t _r = (expr);                                                              \
+    if (_r != hipSuccess) return fail(VL_EHIP, "%s: %s", #expr, hipGetErrorString(_r)); \
+  } while (0)
+
+#define VLCK(expr)          \
+  do {                      \
+    int _s = (expr);        \
+    if (_s != VL_OK) return _s; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device memory pool (stream-ordered reuse: every engine launch uses one stream)
+struct Pool {
+  std::map<size_t, std::vector<void*>> free_;
+  size_t live = 0;
+  static size_t round(size_t b) {
+    size_t r = 256;
+    while (r < b) r <<= 1;
+    return r;
+  }
+  void* get(size_t b, size_t* rounded) {
+    size_t r = round(b ? b : 1);
+    *rounded = r;
+    auto it = free_.find(r);
+    if (it != free_.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, r) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      if (hipMalloc(&p, r) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+      }
+    }
+    live += r;
+    return p;
+  }
+  void put(void* p, size_t r) {
+    if (p) free_[r].push_back(p);
+  }
+  void trim() {
+    for (auto& kv : free_)
+      for (void* p : kv.second) {
+        (void)hipFree(p);
+        live -= kv.first;
+      }
+    free_.clear();
+  }
+};
+
+struct vl_engine {
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::recursive_mutex mu;
+  Pool pool;
+  uint32_t* d_err = nullptr;
+  uint64_t* h_stage = nullptr;   // pinned, 4 KiB
+  // tuning
+  int nmax = 127;
+  int target = 60;
+  int level_factor = 16;
+  int max_wg_cap = 0;            // 0 = occupancy limit
+  double last_ms = 0.0;
+};
+
+struct vl_layer {
+  vl_engine* e = nullptr;
+  vl_box* d = nullptr;
+  int64_t n = 0;
+  size_t bytes = 0;   // pool-rounded allocation size
+};
+
+namespace {
+
+// RAII scratch allocations returned to the pool at scope exit.
+struct Scratch {
+  vl_engine* e;
+  std::vector<std::pair<void*, size_t>> v;
+  bool oom = false;
+  explicit Scratch(vl_engine* en) : e(en) {}
+  ~Scratch() {
+    for (auto& pr : v) e->pool.put(pr.first, pr.second);
+  }
+  template <class T>
+  T* get(size_t n) {
+    size_t r = 0;
+    void* p = e->pool.get(n * sizeof(T) + 64, &r);
+    if (!p) {
+      oom = true;
+      return nullptr;
+    }
+    v.push_back({p, r});
+    return (T*)p;
+  }
+};
+
+// A growable device array owned by a Scratch-like holder.
+template <class T>
+struct DVec {
+  vl_engine* e = nullptr;
+  T* p = nullptr;
+  size_t cap = 0;
+  size_t bytes = 0;
+  explicit DVec(vl_engine* en) : e(en) {}
+  ~DVec() { release(); }
+  void release() {
+    if (p) e->pool.put(p, bytes);
+    p = nullptr;
+    cap = 0;
+    bytes = 0;
+  }
+  // Ensure capacity >= n, keeping the first `keep` elements.
+  int ensure(size_t n, size_t keep) {
+    if (n <= cap) return VL_OK;
+    size_t want = std::max(n, cap + cap / 2);
+    size_t r = 0;
+    T* q = (T*)e->pool.get(want * sizeof(T), &r);
+    if (!q) return fail(VL_ENOMEM, "device allocation of %zu bytes failed", want * sizeof(T));
+    if (p && keep) HIPCK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, e->st));
+    if (p) e->pool.put(p, bytes);
+    p = q;
+    bytes = r;
+    cap = r / sizeof(T);
+    return VL_OK;
+  }
+  T* take(size_t* out_bytes) {   // hand ownership over
+    T* q = p;
+    *out_bytes = bytes;
+    p = nullptr;
+    cap = 0;
+    bytes = 0;
+    return q;
+  }
+};
+
+int readback(vl_engine* e, void* host, const void* dev, size_t bytes) {
+  HIPCK(hipMemcpyAsync(e->h_stage, dev, bytes, hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  memcpy(host, e->h_stage, bytes);
+  return VL_OK;
+}
+
+int check_err(vl_engine* e, uint32_t* out) {
+  uint32_t v = 0;
+  VLCK(readback(e, &v, e->d_err, sizeof(v)));
+  *out = v;
+  return VL_OK;
+}
+
+int clear_err(vl_engine* e) {
+  launch_fill_u32(e->d_err, 0u, 1, e->st);
+  return VL_OK;
+}
+
+struct Box64 {
+  int64_t x1, y1, x2, y2;
+  bool empty() const { return x1 >= x2 || y1 >= y2; }
+};
+
+// Bounding boxes of A (transformed) and B; validates boxes.
+int mbr2(vl_engine* e, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb, Box64* ma, Box64* mb) {
+  Scratch s(e);
+  int32_t* d = s.get<int32_t>(8);
+  if (!d) return fail(VL_ENOMEM, "scratch");
+  int32_t init[8] = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
+  memcpy(e->h_stage, init, sizeof(init));
+  HIPCK(hipMemcpyAsync(d, e->h_stage, sizeof(init), hipMemcpyHostToDevice, e->st));
+  VLCK(clear_err(e));
+  launch_mbr(a, na, xa, d, e->d_err, e->st);
+  launch_mbr(b, nb, Xform{0, 0, 0, 0}, d + 4, e->d_err, e->st);
+  int32_t h[8];
+  uint32_t err = 0;
+  HIPCK(hipMemcpyAsync(e->h_stage, d, sizeof(h), hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipMemcpyAsync((char*)e->h_stage + 64, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  memcpy(h, e->h_stage, sizeof(h));
+  memcpy(&err, (char*)e->h_stage + 64, 4);
+  if (err & ERR_INVALID_BOX)
+    return fail(VL_EINVAL, "invalid box: input has x1>=x2 or y1>=y2, or a transformed box became invalid");
+  *ma = Box64{h[0], h[1], h[2], h[3]};
+  *mb = Box64{h[4], h[5], h[6], h[7]};
+  return VL_OK;
+}
+
+Grid make_grid(Box64 dom, int64_t nboxes, int target, int64_t max_tiles = (int64_t)1 << 26) {
+  Grid g;
+  g.x0 = dom.x1; g.y0 = dom.y1; g.x1 = dom.x2; g.y1 = dom.y2;
+  const int64_t W = dom.x2 - dom.x1, H = dom.y2 - dom.y1;
+  double nt = std::max(1.0, (double)nboxes / std::max(1, target));
+  nt = std::min(nt, (double)max_tiles);
+  double aspect = (double)W / (double)H;
+  int64_t cols = std::max<int64_t>(1, (int64_t)llround(std::sqrt(nt * aspect)));
+  cols = std::min(cols, W);
+  int64_t rows = std::max<int64_t>(1, (int64_t)std::ceil(nt / (double)cols));
+  rows = std::min(rows, H);
+  g.tw = (W + cols - 1) / cols;
+  g.th = (H + rows - 1) / rows;
+  g.cols = (int32_t)((W + g.tw - 1) / g.tw);
+  g.rows = (int32_t)((H + g.th - 1) / g.th);
+  return g;
+}
+
+// Output streams of one operator.
+struct Outs {
+  DVec<vl_box> out, side, next, stitch;
+  DVec<uint8_t> side_tag, stitch_tag;
+  uint64_t nout = 0, nside = 0, nstitch = 0;
+  long long area = 0;
+  explicit Outs(vl_engine* e) : out(e), side(e), next(e), stitch(e), side_tag(e), stitch_tag(e) {}
+};
+
+bool two_layer(int top) { return top == TOP_AND || top == TOP_NOT || top == TOP_XOR; }
+
+int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, bool two, bool area,
+                  Outs& o, int depth);
+
+// Split overflow tiles into x-sub-columns and process them.
+int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
+                      const vl_box* d_entries, bool two, bool area, Outs& o, int depth) {
+  if (depth > 24) return fail(VL_EINTERNAL, "tile splitting did not converge");
+  std::vector<uint32_t> ovf(novf);
+  {
+    Scratch s(e);
+    HIPCK(hipMemcpyAsync(ovf.data(), d_ovf, novf * sizeof(uint32_t), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+  }
+  std::sort(ovf.begin(), ovf.end());
+  std::vector<TileDesc> parents(novf);
+  for (uint32_t i = 0; i < novf; i++)
+    HIPCK(hipMemcpyAsync(&parents[i], d_desc + ovf[i], sizeof(TileDesc), hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  std::vector<TileDesc> subs;
+  std::vector<int32_t> first(novf + 1);
+  const int tgt = std::max(8, e->target);
+  for (uint32_t i = 0; i < novf; i++) {
+    const TileDesc& p = parents[i];
+    first[i] = (int32_t)subs.size();
+    const int64_t n = (int64_t)p.nA + p.nB;
+    const int64_t width = (int64_t)p.x1 - p.x0;
+    int64_t k = std::max<int64_t>(2, (n + tgt - 1) / tgt);
+    k = std::min(k, width);
+    if (k < 2)
+      return fail(VL_EINTERNAL, "a 1-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
+                  (long long)n, e->nmax);
+    const int64_t sw = (width + k - 1) / k;
+    k = (width + sw - 1) / sw;
+    for (int64_t j = 0; j < k; j++) {
+      TileDesc q = p;
+      q.x0 = (int32_t)(p.x0 + j * sw);
+      q.x1 = (int32_t)std::min<int64_t>(p.x0 + (j + 1) * sw, p.x1);
+      q.flags = (p.flags & (SEAM_B | SEAM_T)) | ((j > 0 || (p.flags & SEAM_L)) ? SEAM_L : 0u) |
+                ((j < k - 1 || (p.flags & SEAM_R)) ? SEAM_R : 0u);
+      q.nA = q.nB = 0;
+      q.off = 0;
+      subs.push_back(q);
+    }
+  }
+  first[novf] = (int32_t)subs.size();
+  const int ns = (int)subs.size();
+  Scratch s(e);
+  TileDesc* d_par = s.get<TileDesc>(novf);
+  TileDesc* d_sub = s.get<TileDesc>(ns);
+  int32_t* d_first = s.get<int32_t>(novf + 1);
+  uint32_t* cnt = s.get<uint32_t>(4 * (size_t)ns);
+  uint64_t* off = s.get<uint64_t>(ns + 1);
+  void* stmp = s.get<char>(scan_tmp_bytes(ns));
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  HIPCK(hipMemcpyAsync(d_par, parents.data(), novf * sizeof(TileDesc), hipMemcpyHostToDevice, e->st));
+  HIPCK(hipMemcpyAsync(d_sub, subs.data(), ns * sizeof(TileDesc), hipMemcpyHostToDevice, e->st));
+  HIPCK(hipMemcpyAsync(d_first, first.data(), (novf + 1) * sizeof(int32_t), hipMemcpyHostToDevice, e->st));
+  uint32_t *cA = cnt, *cB = cnt + ns, *fA = cnt + 2 * ns, *fB = cnt + 3 * ns;
+  launch_fill_u32(cnt, 0u, 4 * (size_t)ns, e->st);
+  launch_split_count(d_par, (int)novf, d_first, d_sub, cA, cB, d_entries, e->st);
+  scan_pair_counts(cA, cB, off, ns, off + ns, stmp, e->st);
+  uint64_t total = 0;
+  VLCK(readback(e, &total, off + ns, sizeof(total)));
+  vl_box* sub_entries = s.get<vl_box>(total + 1);
+  if (!sub_entries) return fail(VL_ENOMEM, "scratch");
+  launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, e->st);
+  return process_tiles(e, d_sub, ns, sub_entries, two, area, o, depth + 1);
+}
+
+int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, bool two, bool area,
+                  Outs& o, int depth) {
+  if (ntiles <= 0) return VL_OK;
+  Scratch s(e);
+  uint64_t* status = s.get<uint64_t>(ntiles);
+  uint32_t* small = s.get<uint32_t>(8);
+  uint64_t* totals = s.get<uint64_t>(2);
+  uint32_t* ovf = s.get<uint32_t>(ntiles);
+  long long* d_area = s.get<long long>(1);
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  int max_wg = tile_kernel_max_wg(e->nmax, two, area);
+  if (e->max_wg_cap > 0) max_wg = std::min(max_wg, e->max_wg_cap);
+  uint64_t tot[2] = {0, 0};
+  uint32_t nov = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!area) {
+      // Make sure the look-back never sees a stale pointer range: capacity is
+      // checked in-kernel and a second attempt runs with exact sizes.
+      VLCK(o.out.ensure(std::max<size_t>(o.out.cap, o.nout + 1024), o.nout));
+      VLCK(o.side.ensure(std::max<size_t>(o.side.cap, o.nside + 1024), o.nside));
+      VLCK(o.side_tag.ensure(o.side.cap, o.nside));
+    }
+    launch_fill_u64(status, 0ull, ntiles, e->st);
+    launch_fill_u32(small, 0u, 8, e->st);
+    launch_fill_u64(totals, 0ull, 2, e->st);
+    launch_fill_u64((uint64_t*)d_area, 0ull, 1, e->st);
+    VLCK(clear_err(e));
+    TileOut to;
+    to.out = o.out.p;
+    to.side = o.side.p;
+    to.side_tag = o.side_tag.p;
+    to.out_cap = o.out.cap;
+    to.side_cap = std::min(o.side.cap, o.side_tag.cap);
+    to.out_base = o.nout;
+    to.side_base = o.nside;
+    to.status = status;
+    to.ticket = small;
+    to.ovf = ovf;
+    to.ovf_n = small + 1;
+    to.totals = totals;
+    to.area = d_area;
+    to.err = e->d_err;
+    launch_tile_kernel(e->nmax, two, area, d_desc, ntiles, d_entries, to, max_wg, e->st);
+    HIPCK(hipGetLastError());
+    struct {
+      uint64_t tot[2];
+      uint32_t err, nov;
+      long long area;
+    } rb;
+    HIPCK(hipMemcpyAsync(e->h_stage, totals, 16, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 16, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 20, small + 1, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, d_area, 8, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    memcpy(&rb, e->h_stage, sizeof(rb));
+    if (rb.err & ERR_LOOKBACK) return fail(VL_EINTERNAL, "tile look-back did not complete");
+    tot[0] = rb.tot[0];
+    tot[1] = rb.tot[1];
+    nov = rb.nov;
+    if (area) {
+      o.area += rb.area;
+      break;
+    }
+    if (rb.err & ERR_OVERFLOW_CAP) {
+      if (attempt == 1) return fail(VL_EINTERNAL, "output capacity still short after resize");
+      VLCK(o.out.ensure(o.nout + tot[0] + 1024, o.nout));
+      VLCK(o.side.ensure(o.nside + tot[1] + 1024, o.nside));
+      VLCK(o.side_tag.ensure(o.nside + tot[1] + 1024, o.nside));
+      continue;
+    }
+    break;
+  }
+  if (!area) {
+    o.nout += tot[0];
+    o.nside += tot[1];
+  }
+  if (g_debug)
+    fprintf(stderr, "[vl]  tiles=%d depth=%d two=%d area=%d final=%llu side=%llu overflow=%u\n", ntiles, depth,
+            (int)two, (int)area, (unsigned long long)tot[0], (unsigned long long)tot[1], nov);
+  if (nov > 0) VLCK(split_and_process(e, d_desc, ovf, nov, d_entries, two, area, o, depth));
+  return VL_OK;
+}
+
+// Bins up to two inputs into grid g and processes the tiles.
+int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b,
+                    int64_t nb, bool area, Outs& o) {
+  const int64_t nt = (int64_t)g.cols * g.rows;
+  Scratch s(e);
+  uint32_t* cnt = s.get<uint32_t>(4 * (size_t)nt);
+  uint64_t* off = s.get<uint64_t>(nt + 1);
+  TileDesc* desc = s.get<TileDesc>(nt);
+  void* stmp = s.get<char>(scan_tmp_bytes(nt));
+  if (s.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+  uint32_t *cA = cnt, *cB = cnt + nt, *fA = cnt + 2 * nt, *fB = cnt + 3 * nt;
+  launch_fill_u32(cnt, 0u, 4 * (size_t)nt, e->st);
+  launch_bin_count(a, na, xa, g, cA, e->st);
+  launch_bin_count(b, nb, Xform{0, 0, 0, 0}, g, cB, e->st);
+  scan_pair_counts(cA, cB, off, nt, off + nt, stmp, e->st);
+  uint64_t total = 0;
+  VLCK(readback(e, &total, off + nt, sizeof(total)));
+  vl_box* entries = s.get<vl_box>(total + 1);
+  if (!entries) return fail(VL_ENOMEM, "entries (%llu)", (unsigned long long)total);
+  launch_bin_scatter(a, na, xa, g, off, nullptr, fA, entries, e->st);
+  launch_bin_scatter(b, nb, Xform{0, 0, 0, 0}, g, off, cA, fB, entries, e->st);
+  launch_build_desc(g, off, cA, cB, top, desc, e->st);
+  HIPCK(hipGetLastError());
+  return process_tiles(e, desc, (int)nt, entries, two_layer(top) && nb > 0, area, o, 0);
+}
+
+// Moves result strips into layer c.
+int adopt(vl_engine* e, Outs& o, vl_layer* c) {
+  size_t bytes = 0;
+  vl_box* p = o.out.take(&bytes);
+  if (c->d) e->pool.put(c->d, c->bytes);
+  c->d = p;
+  c->bytes = bytes;
+  c->n = (int64_t)o.nout;
+  return VL_OK;
+}
+
+int set_empty(vl_engine* e, vl_layer* c) {
+  (void)e;
+  c->n = 0;
+  return VL_OK;
+}
+
+// The whole operator pipeline.  area_out != null selects area mode.
+int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb,
+           vl_layer* c, long long* area_out) {
+  if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; }
+  Box64 ma, mb;
+  VLCK(mbr2(e, a, na, xa, b, nb, &ma, &mb));
+  Box64 dom;
+  const bool ea = na <= 0, eb = nb <= 0;
+  switch (top) {
+    case TOP_AND:
+      if (ea || eb) { dom = Box64{0, 0, 0, 0}; break; }
+      dom = Box64{std::max(ma.x1, mb.x1), std::max(ma.y1, mb.y1), std::min(ma.x2, mb.x2), std::min(ma.y2, mb.y2)};
+      break;
+    case TOP_NOT:
+    case TOP_UNION:
+      dom = ea ? Box64{0, 0, 0, 0} : ma;
+      break;
+    default:   // OR, XOR
+      if (ea && eb) dom = Box64{0, 0, 0, 0};
+      else if (ea) dom = mb;
+      else if (eb) dom = ma;
+      else dom = Box64{std::min(ma.x1, mb.x1), std::min(ma.y1, mb.y1), std::max(ma.x2, mb.x2), std::max(ma.y2, mb.y2)};
+  }
+  if (dom.empty()) {
+    if (area_out) *area_out = 0;
+    if (c) set_empty(e, c);
+    return VL_OK;
+  }
+  const bool area = area_out != nullptr;
+  Outs o(e);
+  Grid g = make_grid(dom, na + nb, e->target);
+  VLCK(bin_and_process(e, g, top, a, na, xa, b, nb, area, o));
+  if (area) {
+    *area_out = o.area;
+    return VL_OK;
+  }
+  // seam levels
+  Scratch s(e);
+  int level = 0;
+  while (o.nside > 0) {
+    VLCK(o.next.ensure(o.nside, 0));
+    VLCK(o.stitch.ensure(o.nstitch + o.nside, o.nstitch));
+    VLCK(o.stitch_tag.ensure(o.nstitch + o.nside, o.nstitch));
+    Scratch ps(e);
+    void* ptmp = ps.get<char>(partition_tmp_bytes((int64_t)o.nside));
+    uint64_t* cnts = ps.get<uint64_t>(2);
+    if (ps.oom) return fail(VL_ENOMEM, "scratch");
+    launch_partition_side(o.side.p, o.side_tag.p, (int64_t)o.nside, o.next.p, o.stitch.p + o.nstitch,
+                          o.stitch_tag.p + o.nstitch, cnts, ptmp, e->st);
+    uint64_t hc[2];
+    VLCK(readback(e, hc, cnts, sizeof(hc)));
+    if (g_debug) {
+      std::vector<uint8_t> tags(o.nside);
+      HIPCK(hipMemcpy(tags.data(), o.side_tag.p, o.nside, hipMemcpyDeviceToHost));
+      int hist[16] = {0};
+      for (uint8_t t : tags) hist[t & 15]++;
+      fprintf(stderr, "[vl] level %d grid %dx%d tw=%lld th=%lld side=%llu border=%llu stitch=%llu tags:", level,
+              g.cols, g.rows, (long long)g.tw, (long long)g.th, (unsigned long long)o.nside,
+              (unsigned long long)hc[0], (unsigned long long)hc[1]);
+      for (int i = 0; i < 16; i++)
+        if (hist[i]) fprintf(stderr, " %x:%d", i, hist[i]);
+      fprintf(stderr, "\n");
+    }
+    o.nstitch += hc[1];
+    o.nside = 0;
+    if (hc[0] == 0) break;
+    if (g.rows <= 1) return fail(VL_EINTERNAL, "row-seam strips remain on a single-row level");
+    // next level: same columns, taller rows
+    Grid gn = g;
+    gn.th = std::min<int64_t>(g.th * e->level_factor, g.y1 - g.y0);
+    gn.rows = (int32_t)((g.y1 - g.y0 + gn.th - 1) / gn.th);
+    if (gn.rows == g.rows && g.rows > 1) { gn.rows = 1; gn.th = g.y1 - g.y0; }
+    g = gn;
+    level++;
+    // The level's input (next) is consumed before o.next is reused.
+    DVec<vl_box> in(e);
+    std::swap(in.p, o.next.p);
+    std::swap(in.cap, o.next.cap);
+    std::swap(in.bytes, o.next.bytes);
+    VLCK(bin_and_process(e, g, TOP_UNION, in.p, (int64_t)hc[0], Xform{0, 0, 0, 0}, nullptr, 0, false, o));
+  }
+  // x-stitch
+  if (o.nstitch > 0) {
+    const int64_t n = (int64_t)o.nstitch;
+    Scratch ss(e);
+    uint64_t* keys = ss.get<uint64_t>(n);
+    uint32_t* vals = ss.get<uint32_t>(n);
+    uint64_t* keys2 = ss.get<uint64_t>(n);
+    uint32_t* vals2 = ss.get<uint32_t>(n);
+    uint32_t* nstart = ss.get<uint32_t>(1);
+    int32_t* next = ss.get<int32_t>(n);
+    uint8_t* hasp = ss.get<uint8_t>(n);
+    void* rtmp = ss.get<char>(radix_plan(n).tmp_bytes);
+    void* ptmp = ss.get<char>(partition_tmp_bytes(n));
+    uint64_t* cnt = ss.get<uint64_t>(1);
+    if (ss.oom) return fail(VL_ENOMEM, "stitch scratch");
+    launch_stitch_keys(o.stitch.p, o.stitch_tag.p, n, keys, vals, nstart, e->st);
+    radix_sort_pairs(keys, vals, keys2, vals2, n, rtmp, e->st);
+    launch_stitch_link(o.stitch.p, o.stitch_tag.p, n, keys, vals, nstart, next, hasp, e->st);
+    VLCK(o.out.ensure(o.nout + (uint64_t)n, o.nout));
+    launch_stitch_emit(o.stitch.p, n, next, hasp, o.out.p, o.nout, ptmp, cnt, e->st);
+    uint64_t heads = 0;
+    VLCK(readback(e, &heads, cnt, sizeof(heads)));
+    o.nout += heads;
+  }
+  HIPCK(hipGetLastError());
+  return adopt(e, o, c);
+}
+
+struct OpTimer {
+  vl_engine* e;
+  explicit OpTimer(vl_engine* en) : e(en) { (void)hipEventRecord(e->ev0, e->st); }
+  ~OpTimer() {
+    (void)hipEventRecord(e->ev1, e->st);
+    if (hipEventSynchronize(e->ev1) == hipSuccess) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e->ev0, e->ev1) == hipSuccess) e->last_ms = ms;
+    }
+  }
+};
+
+int check_layers(const vl_engine* e, const vl_layer* a, const vl_layer* b, const vl_layer* c) {
+  if (!e) return fail(VL_EINVAL, "null engine");
+  if (!a || !c) return fail(VL_EINVAL, "null layer");
+  if (a->e != e || (b && b->e != e) || c->e != e) return fail(VL_EINVAL, "layer belongs to another engine");
+  return VL_OK;
+}
+
+// Copies a layer (used when the output aliases an input).
+int snapshot(vl_engine* e, const vl_layer* l, Scratch& s, const vl_box** out) {
+  if (l->n == 0) { *out = nullptr; return VL_OK; }
+  vl_box* p = s.get<vl_box>(l->n);
+  if (!p) return fail(VL_ENOMEM, "scratch");
+  HIPCK(hipMemcpyAsync(p, l->d, l->n * sizeof(vl_box), hipMemcpyDeviceToDevice, e->st));
+  *out = p;
+  return VL_OK;
+}
+
+int binary(vl_engine* e, int top, const vl_layer* a, const vl_layer* b, vl_layer* c, Xform xa = Xform{0, 0, 0, 0}) {
+  VLCK(check_layers(e, a, b, c));
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  Scratch s(e);
+  const vl_box* pa = a->d;
+  const vl_box* pb = b ? b->d : nullptr;
+  if (c == a) VLCK(snapshot(e, a, s, &pa));
+  if (b && c == b) VLCK(snapshot(e, b, s, &pb));
+  return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr);
+}
+
+bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t at;
+  memset(&at, 0, sizeof(at));
+  hipError_t r = hipPointerGetAttributes(&at, p);
+  if (r != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice;
+}
+
+int upload_tmp(vl_engine* e, Scratch& s, const void* host, size_t bytes, void** dev) {
+  *dev = s.get<char>(bytes);
+  if (!*dev) return fail(VL_ENOMEM, "scratch");
+  if (bytes) HIPCK(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, e->st));
+  return VL_OK;
+}
+
+vl_engine* g_default = nullptr;
+std::mutex g_default_mu;
+
+vl_engine* default_engine() {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    if (vl_engine_create(0, &g_default) != VL_OK) {
+      fprintf(stderr, "vlsi_layout_amd: %s\n", g_err.c_str());
+      abort();
+    }
+  }
+  return g_default;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char* vl_last_error(void) { return g_err.c_str(); }
+
+int vl_engine_create(int device, vl_engine** out) {
+  if (!out) return fail(VL_EINVAL, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(VL_EINVAL, "device %d not present (%d devices)", device, ndev);
+  HIPCK(hipSetDevice(device));
+  vl_engine* e = new vl_engine();
+  e->device = device;
+  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipMalloc((void**)&e->d_err, 256) != hipSuccess ||
+      hipHostMalloc((void**)&e->h_stage, 4096, hipHostMallocDefault) != hipSuccess) {
+    delete e;
+    return fail(VL_EHIP, "engine initialisation failed");
+  }
+  if (const char* v = getenv("VL_TILE_NMAX")) e->nmax = atoi(v);
+  if (e->nmax != 63 && e->nmax != 127 && e->nmax != 255) e->nmax = 127;
+  e->target = std::max(8, e->nmax / 2);
+  *out = e;
+  return VL_OK;
+}
+
+int vl_engine_destroy(vl_engine* e) {
+  if (!e) return VL_OK;
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);
+  e->pool.trim();
+  (void)hipFree(e->d_err);
+  (void)hipHostFree(e->h_stage);
+  (void)hipEventDestroy(e->ev0);
+  (void)hipEventDestroy(e->ev1);
+  (void)hipStreamDestroy(e->st);
+  delete e;
+  return VL_OK;
+}
+
+int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
+  if (!e || !key) return fail(VL_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  std::string k(key);
+  if (k == "tile_nmax") {
+    if (v != 63 && v != 127 && v != 255) return fail(VL_EINVAL, "tile_nmax must be 63, 127 or 255");
+    e->nmax = (int)v;
+    e->target = std::min(e->target, e->nmax);
+  } else if (k == "tile_target") {
+    if (v < 4 || v > 4096) return fail(VL_EINVAL, "tile_target out of range");
+    e->target = (int)v;
+  } else if (k == "level_factor") {
+    if (v < 2 || v > 4096) return fail(VL_EINVAL, "level_factor out of range");
+    e->level_factor = (int)v;
+  } else if (k == "max_wg") {
+    e->max_wg_cap = (int)std::max<int64_t>(0, v);
+  } else {
+    return fail(VL_EINVAL, "unknown parameter '%s'", key);
+  }
+  return VL_OK;
+}
+
+int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
+  if (!e || !key || !v) return fail(VL_EINVAL, "null argument");
+  std::string k(key);
+  if (k == "tile_nmax") *v = e->nmax;
+  else if (k == "tile_target") *v = e->target;
+  else if (k == "level_factor") *v = e->level_factor;
+  else if (k == "max_wg") *v = e->max_wg_cap;
+  else return fail(VL_EINVAL, "unknown parameter '%s'", key);
+  return VL_OK;
+}
+
+int vl_engine_sync(vl_engine* e) {
+  if (!e) return fail(VL_EINVAL, "null engine");
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+int vl_engine_last_time_ms(vl_engine* e, double* ms) {
+  if (!e || !ms) return fail(VL_EINVAL, "null argument");
+  *ms = e->last_ms;
+  return VL_OK;
+}
+
+int vl_layer_create(vl_engine* e, vl_layer** out) {
+  if (!e || !out) return fail(VL_EINVAL, "null argument");
+  vl_layer* l = new vl_layer();
+  l->e = e;
+  *out = l;
+  return VL_OK;
+}
+
+int vl_layer_destroy(vl_layer* l) {
+  if (!l) return VL_OK;
+  if (l->d) {
+    std::lock_guard<std::recursive_mutex> lk(l->e->mu);
+    l->e->pool.put(l->d, l->bytes);
+  }
+  delete l;
+  return VL_OK;
+}
+
+static int layer_reserve(vl_layer* l, int64_t n) {
+  size_t need = (size_t)std::max<int64_t>(n, 1) * sizeof(vl_box);
+  if (l->d && l->bytes >= need) return VL_OK;
+  if (l->d) l->e->pool.put(l->d, l->bytes);
+  l->d = nullptr;
+  size_t r = 0;
+  void* p = l->e->pool.get(need, &r);
+  if (!p) return fail(VL_ENOMEM, "layer allocation of %zu bytes failed", need);
+  l->d = (vl_box*)p;
+  l->bytes = r;
+  return VL_OK;
+}
+
+static int validate_host(const vl_box* h, int64_t n) {
+  for (int64_t i = 0; i < n; i++)
+    if (!(h[i].x1 < h[i].x2 && h[i].y1 < h[i].y2))
+      return fail(VL_EINVAL, "invalid box %lld: (%d,%d,%d,%d)", (long long)i, h[i].x1, h[i].y1, h[i].x2, h[i].y2);
+  return VL_OK;
+}
+
+int vl_layer_upload(vl_layer* l, const vl_box* host, int64_t n) {
+  if (!l || (n > 0 && !host) || n < 0) return fail(VL_EINVAL, "bad argument");
+  VLCK(validate_host(host, n));
+  vl_engine* e = l->e;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  VLCK(layer_reserve(l, n));
+  if (n > 0) HIPCK(hipMemcpyAsync(l->d, host, n * sizeof(vl_box), hipMemcpyHostToDevice, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  l->n = n;
+  return VL_OK;
+}
+
+int vl_layer_upload_device(vl_layer* l, const vl_box* dev, int64_t n) {
+  if (!l || (n > 0 && !dev) || n < 0) return fail(VL_EINVAL, "bad argument");
+  vl_engine* e = l->e;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  VLCK(layer_reserve(l, n));
+  if (n > 0) HIPCK(hipMemcpyAsync(l->d, dev, n * sizeof(vl_box), hipMemcpyDeviceToDevice, e->st));
+  l->n = n;
+  if (n > 0) {
+    Box64 ma, mb;
+    VLCK(mbr2(e, l->d, n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb));
+  }
+  return VL_OK;
+}
+
+int vl_layer_download(const vl_layer* l, vl_box* host, int64_t cap, int64_t* n) {
+  if (!l || !n) return fail(VL_EINVAL, "null argument");
+  *n = l->n;
+  if (cap < l->n) return fail(VL_ENOSPC, "capacity %lld < %lld boxes", (long long)cap, (long long)l->n);
+  if (l->n == 0) return VL_OK;
+  if (!host) return fail(VL_EINVAL, "null host buffer");
+  vl_engine* e = l->e;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  HIPCK(hipMemcpyAsync(host, l->d, l->n * sizeof(vl_box), hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+int vl_layer_size(const vl_layer* l, int64_t* n) {
+  if (!l || !n) return fail(VL_EINVAL, "null argument");
+  *n = l->n;
+  return VL_OK;
+}
+
+int vl_layer_device_ptr(const vl_layer* l, const vl_box** dev) {
+  if (!l || !dev) return fail(VL_EINVAL, "null argument");
+  *dev = l->d;
+  return VL_OK;
+}
+
+int vl_and(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
+  if (!b) return fail(VL_EINVAL, "null layer");
+  return binary(e, TOP_AND, a, b, c);
+}
+int vl_not(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
+  if (!b) return fail(VL_EINVAL, "null layer");
+  return binary(e, TOP_NOT, a, b, c);
+}
+int vl_or(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
+  if (!b) return fail(VL_EINVAL, "null layer");
+  return binary(e, TOP_OR, a, b, c);
+}
+int vl_xor(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
+  if (!b) return fail(VL_EINVAL, "null layer");
+  return binary(e, TOP_XOR, a, b, c);
+}
+int vl_merge(vl_engine* e, const vl_layer* a, vl_layer* c) { return binary(e, TOP_UNION, a, nullptr, c); }
+
+int vl_grow(vl_engine* e, const vl_layer* a, int32_t east, int32_t north, int32_t west, int32_t south, vl_layer* c) {
+  return binary(e, TOP_UNION, a, nullptr, c, Xform{east, north, west, south});
+}
+
+int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c) {
+  if (d >= 0) return vl_grow(e, a, d, d, d, d, c);
+  VLCK(check_layers(e, a, nullptr, c));
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  if (a->n == 0) return set_empty(e, c);
+  const int32_t r = (d == INT32_MIN) ? INT32_MAX : -d;
+  OpTimer tm(e);
+  Box64 ma, mb;
+  VLCK(mbr2(e, a->d, a->n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb));
+  if (ma.x1 - 1 < INT32_MIN || ma.y1 - 1 < INT32_MIN || ma.x2 + 1 > INT32_MAX || ma.y2 + 1 > INT32_MAX)
+    return fail(VL_EINVAL, "SIZE frame leaves the int32 range");
+  Scratch s(e);
+  vl_box frame{(int32_t)(ma.x1 - 1), (int32_t)(ma.y1 - 1), (int32_t)(ma.x2 + 1), (int32_t)(ma.y2 + 1)};
+  void* dframe = nullptr;
+  VLCK(upload_tmp(e, s, &frame, sizeof(frame), &dframe));
+  const vl_box* pa = a->d;
+  if (c == a) VLCK(snapshot(e, a, s, &pa));
+  vl_layer t1, t2;
+  t1.e = t2.e = e;
+  int rc = run_op(e, TOP_NOT, (const vl_box*)dframe, 1, Xform{0, 0, 0, 0}, pa, a->n, &t1, nullptr);
+  if (rc == VL_OK) rc = run_op(e, TOP_UNION, t1.d, t1.n, Xform{r, r, r, r}, nullptr, 0, &t2, nullptr);
+  if (rc == VL_OK) rc = run_op(e, TOP_NOT, pa, a->n, Xform{0, 0, 0, 0}, t2.d, t2.n, c, nullptr);
+  if (t1.d) e->pool.put(t1.d, t1.bytes);
+  if (t2.d) e->pool.put(t2.d, t2.bytes);
+  return rc;
+}
+
+int vl_assert_zero(vl_engine* e, const vl_layer* a, int abort_on_nonzero, int64_t* count) {
+  if (!e || !a || !count) return fail(VL_EINVAL, "null argument");
+  *count = a->n;
+  if (a->n > 0 && abort_on_nonzero) return fail(VL_EASSERT, "ASSERT_ZERO: layer has %lld shapes", (long long)a->n);
+  return VL_OK;
+}
+
+int vl_klee_area(vl_engine* e, const vl_layer* a, int64_t* area) {
+  if (!e || !a || !area) return fail(VL_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  long long ar = 0;
+  VLCK(run_op(e, TOP_UNION, a->d, a->n, Xform{0, 0, 0, 0}, nullptr, 0, nullptr, &ar));
+  *area = ar;
+  return VL_OK;
+}
+
+int vl_xor_empty(vl_engine* e, const vl_layer* a, const vl_layer* b, int* empty) {
+  if (!e || !a || !b || !empty) return fail(VL_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  long long ar = 0;
+  VLCK(run_op(e, TOP_XOR, a->d, a->n, Xform{0, 0, 0, 0}, b->d, b->n, nullptr, &ar));
+  *empty = ar == 0;
+  return VL_OK;
+}
+
+int vl_canonicalize(vl_engine* e, const vl_layer* a, vl_layer* c) {
+  VLCK(check_layers(e, a, nullptr, c));
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  VLCK(vl_merge(e, a, c));
+  const int64_t n = c->n;
+  if (n <= 1) return VL_OK;
+  Scratch s(e);
+  uint64_t* k = s.get<uint64_t>(n);
+  uint32_t* v = s.get<uint32_t>(n);
+  uint64_t* k2 = s.get<uint64_t>(n);
+  uint32_t* v2 = s.get<uint32_t>(n);
+  vl_box* tmpb = s.get<vl_box>(n);
+  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  launch_corner_keys(c->d, k, v, n, e->st);
+  radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  HIPCK(hipMemcpyAsync(tmpb, c->d, n * sizeof(vl_box), hipMemcpyDeviceToDevice, e->st));
+  launch_box_gather(tmpb, v, c->d, n, e->st);
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+int vl_boolean_op(vl_engine* e, const vl_box* a, uint64_t an, const vl_box* b, uint64_t bn, vl_box* out,
+                  uint64_t cap, uint64_t* out_n, uint64_t ctrl, int64_t val) {
+  if (!e || !out_n || (an && !a) || (bn && !b)) return fail(VL_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  vl_layer A, B, C;
+  A.e = B.e = C.e = e;
+  auto cleanup = [&]() {
+    for (vl_layer* l : {&A, &B, &C})
+      if (l->d) e->pool.put(l->d, l->bytes);
+  };
+  int rc = vl_layer_upload(&A, a, (int64_t)an);
+  if (rc == VL_OK) rc = vl_layer_upload(&B, b, (int64_t)bn);
+  if (rc == VL_OK) {
+    switch (ctrl) {
+      case VL_OP_XOR: rc = vl_xor(e, &A, &B, &C); break;
+      case VL_OP_OR: rc = vl_or(e, &A, &B, &C); break;
+      case VL_OP_AND: rc = vl_and(e, &A, &B, &C); break;
+      case VL_OP_NOT: rc = vl_not(e, &A, &B, &C); break;
+      case VL_OP_MERGE: rc = vl_merge(e, &A, &C); break;
+      case VL_OP_SIZE:
+        if (val < INT32_MIN || val > INT32_MAX) rc = fail(VL_EINVAL, "SIZE amount out of range");
+        else rc = vl_size(e, &A, (int32_t)val, &C);
+        break;
+      case VL_OP_GROW:
+        if (val < INT32_MIN || val > INT32_MAX) rc = fail(VL_EINVAL, "GROW amount out of range");
+        else rc = vl_grow(e, &A, (int32_t)val, (int32_t)val, (int32_t)val, (int32_t)val, &C);
+        break;
+      default:
+        *out_n = 0;   // unknown control: empty result (src/boost_polygon_api.cpp:136-139)
+        cleanup();
+        return VL_OK;
+    }
+  }
+  if (rc == VL_OK) {
+    *out_n = (uint64_t)C.n;
+    if ((uint64_t)C.n > cap) {
+      rc = fail(VL_ENOSPC, "output needs %lld boxes, capacity %llu", (long long)C.n, (unsigned long long)cap);
+    } else if (C.n > 0) {
+      int64_t got = 0;
+      rc = vl_layer_download(&C, out, (int64_t)cap, &got);
+    }
+  }
+  cleanup();
+  return rc;
+}
+
+// ---- sorts -------------------------------------------------------------------
+int vl_event_sort(vl_engine* e, vl_tracker* arr, int64_t n) {
+  if (!e || (n > 0 && !arr) || n < 0) return fail(VL_EINVAL, "bad argument");
+  if (n <= 1) return VL_OK;
+  if (n > (int64_t)UINT32_MAX) return fail(VL_EINVAL, "more than 2^32 trackers");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  Scratch s(e);
+  const bool dev = is_device_ptr(arr);
+  const size_t bytes = (size_t)n * sizeof(vl_tracker);
+  vl_tracker* d = arr;
+  if (!dev) {
+    void* p = nullptr;
+    VLCK(upload_tmp(e, s, arr, bytes, &p));
+    d = (vl_tracker*)p;
+  }
+  uint64_t* k = s.get<uint64_t>(n);
+  uint32_t* v = s.get<uint32_t>(n);
+  uint64_t* k2 = s.get<uint64_t>(n);
+  uint32_t* v2 = s.get<uint32_t>(n);
+  vl_tracker* src = s.get<vl_tracker>(n);
+  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  launch_tracker_keys(d, k, v, n, e->st);
+  radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  HIPCK(hipMemcpyAsync(src, d, bytes, hipMemcpyDeviceToDevice, e->st));
+  launch_tracker_gather(src, v, d, n, e->st);
+  if (!dev) HIPCK(hipMemcpyAsync(arr, d, bytes, hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+int vl_morton_sort_direct(vl_engine* e, vl_box* boxes, int64_t n) {
+  if (!e || (n > 0 && !boxes) || n < 0) return fail(VL_EINVAL, "bad argument");
+  if (n <= 1) return VL_OK;
+  if (n > (int64_t)UINT32_MAX) return fail(VL_EINVAL, "more than 2^32 boxes");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  Scratch s(e);
+  const bool dev = is_device_ptr(boxes);
+  const size_t bytes = (size_t)n * sizeof(vl_box);
+  vl_box* d = boxes;
+  if (!dev) {
+    void* p = nullptr;
+    VLCK(upload_tmp(e, s, boxes, bytes, &p));
+    d = (vl_box*)p;
+  }
+  uint64_t* k = s.get<uint64_t>(n);
+  uint32_t* v = s.get<uint32_t>(n);
+  uint64_t* k2 = s.get<uint64_t>(n);
+  uint32_t* v2 = s.get<uint32_t>(n);
+  vl_box* src = s.get<vl_box>(n);
+  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  launch_morton_keys(d, k, v, n, e->st);
+  radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  HIPCK(hipMemcpyAsync(src, d, bytes, hipMemcpyDeviceToDevice, e->st));
+  launch_box_gather(src, v, d, n, e->st);
+  if (!dev) HIPCK(hipMemcpyAsync(boxes, d, bytes, hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+int vl_morton_sort_indirect(vl_engine* e, const vl_box* boxes, int64_t* idx, int64_t n) {
+  if (!e || (n > 0 && (!boxes || !idx)) || n < 0) return fail(VL_EINVAL, "bad argument");
+  if (n <= 0) return VL_OK;
+  if (n > (int64_t)UINT32_MAX) return fail(VL_EINVAL, "more than 2^32 boxes");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  Scratch s(e);
+  const bool devb = is_device_ptr(boxes), devi = is_device_ptr(idx);
+  const vl_box* d = boxes;
+  if (!devb) {
+    void* p = nullptr;
+    VLCK(upload_tmp(e, s, boxes, (size_t)n * sizeof(vl_box), &p));
+    d = (const vl_box*)p;
+  }
+  int64_t* di = devi ? idx : s.get<int64_t>(n);
+  uint64_t* k = s.get<uint64_t>(n);
+  uint32_t* v = s.get<uint32_t>(n);
+  uint64_t* k2 = s.get<uint64_t>(n);
+  uint32_t* v2 = s.get<uint32_t>(n);
+  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  launch_morton_keys(d, k, v, n, e->st);
+  if (n > 1) radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  launch_perm_to_index1(v, di, n, e->st);
+  if (!devi) HIPCK(hipMemcpyAsync(idx, di, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+void device_event_sort(vl_tracker* d_arr, int64_t n) {
+  if (vl_event_sort(default_engine(), d_arr, n) != VL_OK) {
+    fprintf(stderr, "device_event_sort: %s\n", g_err.c_str());
+    abort();
+  }
+}
+void thrust_sort_boxes_direct(vl_box* d_boxes, int64_t n) {
+  if (vl_morton_sort_direct(default_engine(), d_boxes, n) != VL_OK) {
+    fprintf(stderr, "thrust_sort_boxes_direct: %s\n", g_err.c_str());
+    abort();
+  }
+}
+void thrust_sort_boxes_indirect(vl_box* d_boxes, int64_t* d_indices, int64_t n) {
+  if (vl_morton_sort_indirect(default_engine(), d_boxes, d_indices, n) != VL_OK) {
+    fprintf(stderr, "thrust_sort_boxes_indirect: %s\n", g_err.c_str());
+    abort();
+  }
+}
+
+}  // extern "C"
+
+// ---- programs ----------------------------------------------------------------
+struct ProgOp {
+  int op;
+  const vl_layer* a;
+  const vl_layer* b;
+  vl_layer* c;
+  int32_t args[4];
+};
+
+struct vl_program {
+  vl_engine* e = nullptr;
+  std::vector<ProgOp> ops;
+  int64_t failures = 0;
+};
+
+static int run_prog_op(vl_engine* e, const ProgOp& p, int64_t* failures) {
+  switch (p.op) {
+    case -1:
+      if (p.a->n > 0) (*failures)++;
+      return VL_OK;
+    case VL_OP_XOR: return vl_xor(e, p.a, p.b, p.c);
+    case VL_OP_OR: return vl_or(e, p.a, p.b, p.c);
+    case VL_OP_AND: return vl_and(e, p.a, p.b, p.c);
+    case VL_OP_NOT: return vl_not(e, p.a, p.b, p.c);
+    case VL_OP_MERGE: return vl_merge(e, p.a, p.c);
+    case VL_OP_SIZE: return vl_size(e, p.a, p.args[0], p.c);
+    case VL_OP_GROW: return vl_grow(e, p.a, p.args[0], p.args[1], p.args[2], p.args[3], p.c);
+    default: return fail(VL_EINVAL, "unknown program op %d", p.op);
+  }
+}
+
+extern "C" {
+
+int vl_program_create(vl_engine* e, vl_program** out) {
+  if (!e || !out) return fail(VL_EINVAL, "null argument");
+  vl_program* p = new vl_program();
+  p->e = e;
+  *out = p;
+  return VL_OK;
+}
+
+int vl_program_destroy(vl_program* p) {
+  delete p;
+  return VL_OK;
+}
+
+int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, vl_layer* c, const int32_t* args) {
+  if (!p || !a) return fail(VL_EINVAL, "null argument");
+  if (op != -1 && !c) return fail(VL_EINVAL, "null output layer");
+  if ((op == VL_OP_XOR || op == VL_OP_OR || op == VL_OP_AND || op == VL_OP_NOT) && !b)
+    return fail(VL_EINVAL, "binary operator needs two inputs");
+  if (op < -1 || op > VL_OP_GROW) return fail(VL_EINVAL, "unknown op %d", op);
+  ProgOp o{op, a, b, c, {0, 0, 0, 0}};
+  if (args) {
+    const int na = (op == VL_OP_GROW) ? 4 : 1;
+    for (int i = 0; i < na; i++) o.args[i] = args[i];
+  }
+  p->ops.push_back(o);
+  return VL_OK;
+}
+
+int vl_program_run(vl_program* p) {
+  if (!p) return fail(VL_EINVAL, "null program");
+  vl_engine* e = p->e;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  hipEvent_t a, b;
+  HIPCK(hipEventCreate(&a));
+  HIPCK(hipEventCreate(&b));
+  HIPCK(hipEventRecord(a, e->st));
+  p->failures = 0;
+  int rc = VL_OK;
+  for (const ProgOp& o : p->ops) {
+    rc = run_prog_op(e, o, &p->failures);
+    if (rc != VL_OK) break;
+  }
+  (void)hipEventRecord(b, e->st);
+  (void)hipEventSynchronize(b);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) == hipSuccess) e->last_ms = ms;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+int vl_program_assert_count(vl_program* p, int64_t* failures) {
+  if (!p || !failures) return fail(VL_EINVAL, "null argument");
+  *failures = p->failures;
+  return VL_OK;
+}
+
+}  // extern "C"

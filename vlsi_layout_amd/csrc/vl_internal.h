@@ -1,0 +1,162 @@
+// vl_internal.h — shared declarations of the gfx950 engine (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/vl_api.h"
+
+namespace vl {
+
+// ---------------------------------------------------------------------------
+// Tiles.  A tile is a rectangle of the op's domain plus the boxes (of one or
+// two layers) whose interior meets it.  Entries of tile t live at
+// entries[off .. off+nA) (layer A) and [off+nA .. off+nA+nB) (layer B).
+// Seam flags say which tile borders are internal to the domain: a result run
+// touching such a border is a piece of a longer run and goes to the side
+// stream for seam resolution instead of the final output.
+enum : uint32_t {
+  SEAM_B = 1u,   // bottom border y0 is internal
+  SEAM_T = 2u,   // top border y1 is internal
+  SEAM_L = 4u,   // left border x0 is internal
+  SEAM_R = 8u,   // right border x1 is internal
+};
+
+struct TileDesc {
+  int32_t x0, y0, x1, y1;
+  int64_t off;
+  int32_t nA, nB;
+  uint32_t flags;   // SEAM_*
+  int32_t op;       // per-tile operator (batched launches mix operators)
+};
+static_assert(sizeof(TileDesc) == 40, "TileDesc layout");
+
+// Operators evaluated per cell by the tile kernel.
+enum TileOp : int32_t {
+  TOP_UNION = 0,   // A only (merge / grow)
+  TOP_AND = 1,
+  TOP_NOT = 2,     // A \ B
+  TOP_OR = 3,
+  TOP_XOR = 4,
+};
+
+// Uniform grid over a domain.  Tile (c, r) covers
+// [x0 + c*tw, min(x0+(c+1)*tw, x1)) x [y0 + r*th, min(...)).
+struct Grid {
+  int64_t x0, y0, x1, y1;   // domain (int64 so widths never overflow)
+  int64_t tw, th;
+  int32_t cols, rows;
+};
+
+// Per-box transform applied while binning (GROW): x1-=w, y1-=s, x2+=e, y2+=n.
+struct Xform {
+  int32_t e, n, w, s;
+};
+
+// Status bits written by kernels into the engine's device error word.
+enum : uint32_t {
+  ERR_INVALID_BOX = 1u,
+  ERR_LOOKBACK = 2u,
+  ERR_OVERFLOW_CAP = 4u,
+};
+
+// Look-back status word: [63:62] flag, [61:31] final count, [30:0] side count.
+constexpr uint64_t LB_AGG = 1ull << 62;
+constexpr uint64_t LB_PRE = 2ull << 62;
+constexpr uint64_t LB_FLAGMASK = 3ull << 62;
+constexpr uint64_t LB_VALMASK = ~LB_FLAGMASK;
+__host__ __device__ inline uint64_t lb_pack(uint64_t f, uint64_t s) { return (f << 31) | s; }
+
+// ---------------------------------------------------------------------------
+// Host-side launch wrappers (each .hip file implements its own kernels).
+
+// vl_prims.hip
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
+// MBR of boxes after a transform; validates boxes (ERR_INVALID_BOX).
+// mbr[4] must be pre-set to (INT_MAX, INT_MAX, INT_MIN, INT_MIN).
+void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st);
+// Exclusive scans.  tmp must hold scan_tmp_bytes(n) bytes.  total may be null.
+size_t scan_tmp_bytes(size_t n);
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, void* tmp,
+                        hipStream_t st);
+void scan_exclusive_u64(const uint64_t* in, uint64_t* out, size_t n, uint64_t* total, void* tmp,
+                        hipStream_t st);
+// Exclusive scan of (cntA[i] + cntB[i]) into off (u64); total into *total.
+void scan_pair_counts(const uint32_t* cntA, const uint32_t* cntB, uint64_t* off, size_t n,
+                      uint64_t* total, void* tmp, hipStream_t st);
+
+// vl_sort.hip — LSD radix sort of (u64 key, u32 value) pairs.
+struct RadixPlan {
+  int64_t n;
+  int nblocks;
+  size_t tmp_bytes;
+};
+RadixPlan radix_plan(int64_t n);
+// Sorts keys/vals in place (uses alt buffers of the same size + tmp).  The
+// host reads the digit histograms once to skip passes in which all keys share
+// a digit (one synchronisation).
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                      int64_t n, void* tmp, hipStream_t st);
+void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st);
+void launch_tracker_gather(const vl_tracker* in, const uint32_t* perm, vl_tracker* out, int64_t n,
+                           hipStream_t st);
+void launch_morton_keys(const vl_box* b, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st);
+void launch_corner_keys(const vl_box* b, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st);
+void launch_box_gather(const vl_box* in, const uint32_t* perm, vl_box* out, int64_t n, hipStream_t st);
+void launch_perm_to_index1(const uint32_t* perm, int64_t* idx, int64_t n, hipStream_t st);
+
+// vl_tile.hip
+// Count boxes per tile (cntA for layer 0, cntB for layer 1).
+void launch_bin_count(const vl_box* b, int64_t n, Xform xf, Grid g, uint32_t* cnt, hipStream_t st);
+void launch_bin_scatter(const vl_box* b, int64_t n, Xform xf, Grid g, const uint64_t* off,
+                        const uint32_t* base_add, uint32_t* fill, vl_box* entries, hipStream_t st);
+void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB,
+                       int32_t op, TileDesc* desc, hipStream_t st);
+// Tile kernel: result strips of every tile.  Strips touching an internal seam
+// go to side[] with a tag byte; the rest go to out[].  Tiles with more boxes
+// than the kernel's capacity are listed in ovf[] (count in *ovf_n).
+struct TileOut {
+  vl_box* out;       // final strips; written at out[out_base + ...]
+  vl_box* side;      // seam strips
+  uint8_t* side_tag; // SEAM_* bits per side strip
+  uint64_t out_cap, side_cap;
+  uint64_t out_base, side_base;
+  uint64_t* status;  // look-back words, one per tile, zeroed before launch
+  uint32_t* ticket;  // zeroed before launch
+  uint32_t* ovf;     // overflow tile indices
+  uint32_t* ovf_n;
+  uint64_t* totals;  // [0]=final count [1]=side count (written by the last tile)
+  long long* area;   // area mode: accumulated union area (int64)
+  uint32_t* err;
+};
+int tile_nmax_max();
+void launch_tile_kernel(int nmax, bool two_layers, bool area_mode, const TileDesc* desc,
+                        int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st);
+int tile_kernel_max_wg(int nmax, bool two_layers, bool area_mode);
+// Split overflow tiles: per parent one workgroup re-bins its entries into k
+// x-sub-columns (sub-descriptors precomputed on the host).
+void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_first,
+                        const TileDesc* subs, uint32_t* sub_cntA, uint32_t* sub_cntB,
+                        const vl_box* entries, hipStream_t st);
+void launch_split_scatter(const TileDesc* parents, int np, const int32_t* sub_first, TileDesc* subs,
+                          int ns, const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB,
+                          uint32_t* fillA, uint32_t* fillB, const vl_box* entries,
+                          vl_box* sub_entries, hipStream_t st);
+
+// vl_stitch.hip
+// Partition side strips: tags with SEAM_B|SEAM_T -> next (border), the rest -> stitch.
+void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl_box* next,
+                           vl_box* stitch, uint8_t* stitch_tag, uint64_t* counts, void* tmp,
+                           hipStream_t st);
+size_t partition_tmp_bytes(int64_t n);
+// X-stitch of strips whose left/right ends sit on internal x-seams.
+void launch_stitch_keys(const vl_box* s, const uint8_t* tag, int64_t n, uint64_t* keys,
+                        uint32_t* vals, uint32_t* nstart, hipStream_t st);
+void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, const uint64_t* skeys,
+                        const uint32_t* svals, const uint32_t* nstart, int32_t* next, uint8_t* has_prev,
+                        hipStream_t st);
+void launch_stitch_emit(const vl_box* s, int64_t n, const int32_t* next, const uint8_t* has_prev,
+                        vl_box* out, uint64_t out_base, void* tmp, uint64_t* count, hipStream_t st);
+
+}  // namespace vl
