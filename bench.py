@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""OP1-ring throughput of the gfx950 rectangle-Boolean engine.
+
+Workload: the standard-cell-like 4-layer design (poly, li1, met1, met2; rows of
+2000 cells on a 5-DBU grid, seed 0x5EED0002) at 10M boxes per layer, resident
+in HBM.  One step is the OP1 ring (src/op_template.m4:14-22) with d = 1 DBU on
+every layer:
+    L1 = GROW(L; 1,1,1,1)   L2 = SIZE(L; -1)   outer = L1 ~ L   inner = L ~ L2
+    and1 = outer * L        ring = outer * inner   ASSERT_ZERO(and1), ASSERT_ZERO(ring)
+Throughput = input boxes of all operators of a step (N_A+N_B for binary ops,
+N for GROW/SIZE) / wall time of the step, with the engine stream synchronised
+on both sides of the timed region.  "plain" runs one operator per call;
+"batched" runs the same operators as one recorded program.
+
+The roofline figure is for the per-tile Boolean kernel (the dominant kernel):
+algorithmic bytes of its launches in the timed region (16 B per entry read,
+16 B per strip written, 48 B per tile) / their summed duration from HIP events
+on the engine stream, against the MI355X's 8 TB/s HBM peak.
+
+cpu_baseline: the same OP1 ring on a band of rows of the same layers, run by
+the C/OpenMP restatement of the reference CPU path (oracle/liboracle.so,
+frame-tiled like src/design_impl.f90:993-1316).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Boolean-op boxes/sec at 10M boxes/layer, batched vs plain; query p99 latency"
+SEED = 0x5EED0002
+LAYERS = ["poly", "li1", "met1", "met2"]
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--boxes-per-layer", type=int, default=10_000_000)
+    p.add_argument("--cpu-sample", type=int, default=1_000_000, help="boxes per layer in the CPU sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--mode", choices=["plain", "batched", "both"], default="both")
+    return p.parse_args()
+
+
+class Ring:
+    """Layers and temporaries of the OP1 ring for one input layer."""
+
+    def __init__(self, V, eng, boxes):
+        self.L = V.Layer(eng, boxes)
+        self.L1, self.L2, self.outer, self.inner, self.and1, self.ring = (V.Layer(eng) for _ in range(6))
+
+    def run_plain(self, V):
+        n = self.L.n_used
+        V.CalculateGROWLayer(self.L, self.L1, (1, 1, 1, 1))
+        V.CalculateBoostOperation(self.L, self.L, self.L2, V.K_BOOST_CONTROL_SIZE, -1)
+        V.CalculateFrameNOT(self.L1, self.L, self.outer)
+        V.CalculateFrameNOT(self.L, self.L2, self.inner)
+        V.CalculateAND(self.outer, self.L, self.and1)
+        V.CalculateAND(self.outer, self.inner, self.ring)
+        fails = int(V.ASSERT_ZERO(self.and1) > 0) + int(V.ASSERT_ZERO(self.ring) > 0)
+        boxes = (n + n + (self.L1.n_used + n) + (n + self.L2.n_used) + (self.outer.n_used + n)
+                 + (self.outer.n_used + self.inner.n_used))
+        return boxes, fails
+
+    def record(self, V, prog):
+        prog.add(V.VL_OP_GROW, self.L, None, self.L1, (1, 1, 1, 1))
+        prog.add(V.K_BOOST_CONTROL_SIZE, self.L, None, self.L2, (-1,))
+        prog.add(V.K_BOOST_CONTROL_NOT, self.L1, self.L, self.outer)
+        prog.add(V.K_BOOST_CONTROL_NOT, self.L, self.L2, self.inner)
+        prog.add(V.K_BOOST_CONTROL_AND, self.outer, self.L, self.and1)
+        prog.add(V.K_BOOST_CONTROL_AND, self.outer, self.inner, self.ring)
+        prog.add(-1, self.and1)
+        prog.add(-1, self.ring)
+
+
+def time_steps(fn, eng, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    eng.sync()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    eng.sync()
+    return time.perf_counter() - t0, out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import vlsi_layout_amd as V
+
+    npl = args.boxes_per_layer
+    t_gen = time.perf_counter()
+    layers = [V.synth_layer(V.SYN_STDCELL, SEED, k, npl) for k in range(4)]
+    t_gen = time.perf_counter() - t_gen
+
+    print(f"[bench] generated 4 x {npl} boxes in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    eng = V.Engine(0)
+    rings = [Ring(V, eng, lay) for lay in layers]
+    result = {}
+
+    def plain_step():
+        tot, fails = 0, 0
+        for r in rings:
+            b, f = r.run_plain(V)
+            tot += b
+            fails += f
+        return tot, fails
+
+    prog = V.Program(eng)
+    for r in rings:
+        r.record(V, prog)
+
+    def batched_step():
+        prog.run()
+        return prog.assert_failures()
+
+    boxes_per_step, fails = plain_step()          # also fixes the per-step box count
+    if fails:
+        raise SystemExit(f"OP1 ring assertion failed on {fails} checks")
+    print(f"[bench] first step ok: {boxes_per_step} input boxes per step", file=sys.stderr, flush=True)
+
+    modes = ["plain", "batched"] if args.mode == "both" else [args.mode]
+    for m in modes:
+        eng.set("profile", 1)
+        secs, out = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps, args.warmup)
+        fails = out[1] if m == "plain" else out
+        if fails:
+            raise SystemExit(f"OP1 ring assertion failed in {m} mode")
+        launches, ms, by = eng.kernel_stats("tile")
+        bl, bms, bby = eng.kernel_stats("bin")
+        result[m] = {
+            "boxes_per_s": boxes_per_step * args.steps / secs,
+            "ms_per_step": 1000.0 * secs / args.steps,
+            "tile_kernel": {"launches": launches, "ms": ms, "bytes": by,
+                            "gbs": (by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0},
+            "bin_kernels": {"launches": bl, "ms": bms, "bytes": bby,
+                            "gbs": (bby / (bms * 1e-3) / 1e9) if bms > 0 else 0.0},
+        }
+        eng.set("profile", 0)
+        print(f"[bench] {m}: {result[m]['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
+
+    head = "batched" if "batched" in result else "plain"
+    tk = result[head]["tile_kernel"]
+    achieved = tk["gbs"]
+    line = {
+        "metric": METRIC,
+        "value": result[head]["boxes_per_s"],
+        "unit": "boxes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": result[head]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: seeded standard-cell generator (seed 0x5EED0002), layers resident in HBM",
+        "config": {"workload": "stdcell-4layer-OP1-ring", "boxes_per_layer": npl, "layers": LAYERS,
+                   "ring_distance_dbu": 1, "operators_per_step": 24,
+                   "input_boxes_per_step": int(boxes_per_step)},
+        "plain": result.get("plain"),
+        "batched": result.get("batched"),
+        "batched_over_plain": (result["batched"]["boxes_per_s"] / result["plain"]["boxes_per_s"]
+                               if "batched" in result and "plain" in result else None),
+        "roofline": {"bound": "hbm", "kernel": "k_tile (per-tile Boolean)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
+        "generation_s": t_gen,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        samples = [O.canonical(lay[:args.cpu_sample]) for lay in layers]
+        boxes = 0
+        t0 = time.perf_counter()
+        for L in samples:
+            n = len(L)
+            L1 = O.par_op("grow", L, args=(1, 1, 1, 1), nthreads=nthreads)
+            L2 = O.par_op("size", L, args=(-1,), nthreads=nthreads)
+            outer = O.par_op("not", L1, L, nthreads=nthreads)
+            inner = O.par_op("not", L, L2, nthreads=nthreads)
+            and1 = O.par_op("and", outer, L, nthreads=nthreads)
+            ring = O.par_op("and", outer, inner, nthreads=nthreads)
+            if len(and1) or len(ring):
+                raise SystemExit("CPU baseline OP1 assertion failed")
+            boxes += n + n + (len(L1) + n) + (n + len(L2)) + (len(outer) + n) + (len(outer) + len(inner))
+        secs = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": boxes / secs, "unit": "boxes/s", "cores": nthreads, "kind": "port",
+                                "sample": f"OP1 ring on the first {args.cpu_sample} boxes (a band of rows) "
+                                          f"of each of the 4 layers, healed first; {secs:.1f} s of CPU work"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,8 @@ def lib():
             "vl_engine_get_param": (C.c_int, [vp, C.c_char_p, C.POINTER(i64)]),
             "vl_engine_sync": (C.c_int, [vp]),
             "vl_engine_last_time_ms": (C.c_int, [vp, C.POINTER(C.c_double)]),
+            "vl_engine_kernel_stats": (C.c_int, [vp, C.c_char_p, C.POINTER(i64), C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double)]),
             "vl_layer_create": (C.c_int, [vp, C.POINTER(vp)]),
             "vl_layer_destroy": (C.c_int, [vp]),
             "vl_layer_upload": (C.c_int, [vp, vp, i64]),
@@ -182,6 +184,12 @@ class Engine:
         v = C.c_double()
         _check(lib().vl_engine_last_time_ms(self._h, C.byref(v)))
         return v.value
+
+    def kernel_stats(self, kernel):
+        """(launches, total_ms, algorithmic_bytes) of a kernel while profile=1."""
+        n, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        _check(lib().vl_engine_kernel_stats(self._h, kernel.encode(), C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
 
     def layer(self, boxes=None):
         return Layer(self, boxes)

@@ -33,6 +33,11 @@ using namespace vl;
 static thread_local std::string g_err;
 static bool g_debug = getenv("VL_DEBUG") != nullptr;
 
+#include <chrono>
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 static int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -113,7 +118,18 @@ struct vl_engine {
   int level_factor = 16;
   int max_wg_cap = 0;            // 0 = occupancy limit
   double last_ms = 0.0;
+  // kernel statistics (collected while prof is set)
+  struct KStat {
+    int64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+  };
+  bool prof = false;
+  KStat kst[3];                  // 0 tile, 1 bin, 2 sort
+  hipEvent_t kev0 = nullptr, kev1 = nullptr;
 };
+
+enum { KS_TILE = 0, KS_BIN = 1, KS_SORT = 2 };
 
 struct vl_layer {
   vl_engine* e = nullptr;
@@ -263,8 +279,8 @@ struct Outs {
 
 bool two_layer(int top) { return top == TOP_AND || top == TOP_NOT || top == TOP_XOR; }
 
-int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, bool two, bool area,
-                  Outs& o, int depth);
+int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, uint64_t nentries,
+                  bool two, bool area, Outs& o, int depth);
 
 // Split overflow tiles into x-sub-columns and process them.
 int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
@@ -329,11 +345,11 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
   vl_box* sub_entries = s.get<vl_box>(total + 1);
   if (!sub_entries) return fail(VL_ENOMEM, "scratch");
   launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, e->st);
-  return process_tiles(e, d_sub, ns, sub_entries, two, area, o, depth + 1);
+  return process_tiles(e, d_sub, ns, sub_entries, total, two, area, o, depth + 1);
 }
 
-int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, bool two, bool area,
-                  Outs& o, int depth) {
+int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, uint64_t nentries,
+                  bool two, bool area, Outs& o, int depth) {
   if (ntiles <= 0) return VL_OK;
   Scratch s(e);
   uint64_t* status = s.get<uint64_t>(ntiles);
@@ -346,6 +362,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   if (e->max_wg_cap > 0) max_wg = std::min(max_wg, e->max_wg_cap);
   uint64_t tot[2] = {0, 0};
   uint32_t nov = 0;
+  const double t_pt = g_debug ? now_ms() : 0.0;
   for (int attempt = 0; attempt < 2; attempt++) {
     if (!area) {
       // Make sure the look-back never sees a stale pointer range: capacity is
@@ -374,8 +391,10 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     to.totals = totals;
     to.area = d_area;
     to.err = e->d_err;
+    if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
     launch_tile_kernel(e->nmax, two, area, d_desc, ntiles, d_entries, to, max_wg, e->st);
     HIPCK(hipGetLastError());
+    if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
     struct {
       uint64_t tot[2];
       uint32_t err, nov;
@@ -387,6 +406,14 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, d_area, 8, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     memcpy(&rb, e->h_stage, sizeof(rb));
+    if (e->prof) {
+      float ms = 0.f;
+      HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
+      auto& k = e->kst[KS_TILE];
+      k.launches++;
+      k.ms += ms;
+      k.bytes += 16.0 * (double)nentries + 16.0 * (double)(rb.tot[0] + rb.tot[1]) + 48.0 * ntiles;
+    }
     if (rb.err & ERR_LOOKBACK) return fail(VL_EINTERNAL, "tile look-back did not complete");
     tot[0] = rb.tot[0];
     tot[1] = rb.tot[1];
@@ -409,9 +436,14 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     o.nside += tot[1];
   }
   if (g_debug)
-    fprintf(stderr, "[vl]  tiles=%d depth=%d two=%d area=%d final=%llu side=%llu overflow=%u\n", ntiles, depth,
-            (int)two, (int)area, (unsigned long long)tot[0], (unsigned long long)tot[1], nov);
-  if (nov > 0) VLCK(split_and_process(e, d_desc, ovf, nov, d_entries, two, area, o, depth));
+    fprintf(stderr, "[vl]  tiles=%d entries=%llu depth=%d two=%d area=%d final=%llu side=%llu overflow=%u %.2f ms\n",
+            ntiles, (unsigned long long)nentries, depth, (int)two, (int)area, (unsigned long long)tot[0],
+            (unsigned long long)tot[1], nov, now_ms() - t_pt);
+  if (nov > 0) {
+    const double t_sp = g_debug ? now_ms() : 0.0;
+    VLCK(split_and_process(e, d_desc, ovf, nov, d_entries, two, area, o, depth));
+    if (g_debug) fprintf(stderr, "[vl]  split %u tiles (depth %d): %.2f ms\n", nov, depth, now_ms() - t_sp);
+  }
   return VL_OK;
 }
 
@@ -427,18 +459,34 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
   if (s.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
   uint32_t *cA = cnt, *cB = cnt + nt, *fA = cnt + 2 * nt, *fB = cnt + 3 * nt;
   launch_fill_u32(cnt, 0u, 4 * (size_t)nt, e->st);
+  if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
   launch_bin_count(a, na, xa, g, cA, e->st);
   launch_bin_count(b, nb, Xform{0, 0, 0, 0}, g, cB, e->st);
+  if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
   scan_pair_counts(cA, cB, off, nt, off + nt, stmp, e->st);
   uint64_t total = 0;
   VLCK(readback(e, &total, off + nt, sizeof(total)));
+  float ms_count = 0.f;
+  if (e->prof) HIPCK(hipEventElapsedTime(&ms_count, e->kev0, e->kev1));
   vl_box* entries = s.get<vl_box>(total + 1);
   if (!entries) return fail(VL_ENOMEM, "entries (%llu)", (unsigned long long)total);
+  if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
   launch_bin_scatter(a, na, xa, g, off, nullptr, fA, entries, e->st);
   launch_bin_scatter(b, nb, Xform{0, 0, 0, 0}, g, off, cA, fB, entries, e->st);
+  if (e->prof) {
+    HIPCK(hipEventRecord(e->kev1, e->st));
+    HIPCK(hipEventSynchronize(e->kev1));
+    float ms_scatter = 0.f;
+    HIPCK(hipEventElapsedTime(&ms_scatter, e->kev0, e->kev1));
+    auto& k = e->kst[KS_BIN];
+    k.launches += 2;
+    k.ms += ms_count + ms_scatter;
+    // count: read 16 B per box; scatter: read 16 B per box, write 16 B per entry
+    k.bytes += 32.0 * (double)(na + nb) + 16.0 * (double)total;
+  }
   launch_build_desc(g, off, cA, cB, top, desc, e->st);
   HIPCK(hipGetLastError());
-  return process_tiles(e, desc, (int)nt, entries, two_layer(top) && nb > 0, area, o, 0);
+  return process_tiles(e, desc, (int)nt, entries, total, two_layer(top) && nb > 0, area, o, 0);
 }
 
 // Moves result strips into layer c.
@@ -489,7 +537,10 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
   const bool area = area_out != nullptr;
   Outs o(e);
   Grid g = make_grid(dom, na + nb, e->target);
+  const double t_start = g_debug ? now_ms() : 0.0;
   VLCK(bin_and_process(e, g, top, a, na, xa, b, nb, area, o));
+  if (g_debug) fprintf(stderr, "[vl] op %d na=%lld nb=%lld level0 %.2f ms\n", top, (long long)na, (long long)nb,
+                       now_ms() - t_start);
   if (area) {
     *area_out = o.area;
     return VL_OK;
@@ -537,9 +588,13 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
     std::swap(in.p, o.next.p);
     std::swap(in.cap, o.next.cap);
     std::swap(in.bytes, o.next.bytes);
+    const double t_lv = g_debug ? now_ms() : 0.0;
     VLCK(bin_and_process(e, g, TOP_UNION, in.p, (int64_t)hc[0], Xform{0, 0, 0, 0}, nullptr, 0, false, o));
+    if (g_debug) fprintf(stderr, "[vl] level %d: %llu strips, %.2f ms\n", level, (unsigned long long)hc[0],
+                         now_ms() - t_lv);
   }
   // x-stitch
+  const double t_st = g_debug ? now_ms() : 0.0;
   if (o.nstitch > 0) {
     const int64_t n = (int64_t)o.nstitch;
     Scratch ss(e);
@@ -563,6 +618,9 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
     VLCK(readback(e, &heads, cnt, sizeof(heads)));
     o.nout += heads;
   }
+  if (g_debug)
+    fprintf(stderr, "[vl] stitch %llu strips %.2f ms; total op %.2f ms, out %llu\n", (unsigned long long)o.nstitch,
+            now_ms() - t_st, now_ms() - t_start, (unsigned long long)o.nout);
   HIPCK(hipGetLastError());
   return adopt(e, o, c);
 }
@@ -660,6 +718,7 @@ int vl_engine_create(int device, vl_engine** out) {
   e->device = device;
   if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreate(&e->kev0) != hipSuccess || hipEventCreate(&e->kev1) != hipSuccess ||
       hipMalloc((void**)&e->d_err, 256) != hipSuccess ||
       hipHostMalloc((void**)&e->h_stage, 4096, hipHostMallocDefault) != hipSuccess) {
     delete e;
@@ -681,6 +740,8 @@ int vl_engine_destroy(vl_engine* e) {
   (void)hipHostFree(e->h_stage);
   (void)hipEventDestroy(e->ev0);
   (void)hipEventDestroy(e->ev1);
+  (void)hipEventDestroy(e->kev0);
+  (void)hipEventDestroy(e->kev1);
   (void)hipStreamDestroy(e->st);
   delete e;
   return VL_OK;
@@ -700,6 +761,9 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "level_factor") {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "level_factor out of range");
     e->level_factor = (int)v;
+  } else if (k == "profile") {
+    e->prof = v != 0;
+    for (auto& s : e->kst) s = vl_engine::KStat();
   } else if (k == "max_wg") {
     e->max_wg_cap = (int)std::max<int64_t>(0, v);
   } else {
@@ -722,6 +786,17 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
 int vl_engine_sync(vl_engine* e) {
   if (!e) return fail(VL_EINVAL, "null engine");
   HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms, double* bytes) {
+  if (!e || !kernel || !launches || !total_ms || !bytes) return fail(VL_EINVAL, "null argument");
+  std::string k(kernel);
+  int i = k == "tile" ? KS_TILE : k == "bin" ? KS_BIN : k == "sort" ? KS_SORT : -1;
+  if (i < 0) return fail(VL_EINVAL, "unknown kernel '%s'", kernel);
+  *launches = e->kst[i].launches;
+  *total_ms = e->kst[i].ms;
+  *bytes = e->kst[i].bytes;
   return VL_OK;
 }
 
