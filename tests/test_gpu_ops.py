@@ -4,7 +4,8 @@ Bar: bit-exact.  The engine's output, sorted by (x1,y1,x2,y2), must equal the
 oracle's canonical maximal-strip form exactly (so the GPU output is already
 the unique heal_boxes decomposition, only in tile order), and the union area
 must match.  Small tile budgets force many tiles, seam levels and x-stitches;
-a 63-box tile capacity forces the overflow-split path.
+tile budgets above the wave kernel capacity force the
+256-thread kernel and the x-split path.
 """
 import numpy as np
 import pytest
@@ -17,8 +18,8 @@ pytestmark = pytest.mark.gpu
 CONFIGS = [
     dict(),                                                  # defaults
     dict(tile_target=6, level_factor=2),                     # many tiles, many seam levels
-    dict(tile_nmax=63, tile_target=40, level_factor=4),      # overflow splits
-    dict(tile_nmax=255, tile_target=30, level_factor=16),
+    dict(tile_nmax=63, tile_target=180, level_factor=4),     # most tiles on the 256-thread kernel, some split
+    dict(tile_nmax=127, tile_target=90, level_factor=16),
 ]
 
 

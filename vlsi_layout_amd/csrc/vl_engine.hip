@@ -14,6 +14,7 @@
 // and processed in a follow-up launch.  Every launch writes at offsets from
 // the tile kernel's look-back, so results are deterministic.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -109,12 +110,16 @@ struct vl_engine {
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::recursive_mutex mu;
+  // Lifetime: the engine handle and every layer / program created on it hold
+  // a reference; vl_engine_destroy drops the handle's reference and the
+  // engine is freed when the last reference goes, whatever the destroy order.
+  std::atomic<int> refs{1};
   Pool pool;
   uint32_t* d_err = nullptr;
   uint64_t* h_stage = nullptr;   // pinned, 4 KiB
   // tuning
-  int nmax = 127;
-  int target = 60;
+  int nmax = 63;                 // wave-kernel tile capacity (63 or 127 boxes)
+  int target = 42;               // mean boxes per tile the grid aims for
   int level_factor = 16;
   int max_wg_cap = 0;            // 0 = occupancy limit
   double last_ms = 0.0;
@@ -287,16 +292,21 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
                       const vl_box* d_entries, bool two, bool area, Outs& o, int depth) {
   if (depth > 24) return fail(VL_EINTERNAL, "tile splitting did not converge");
   std::vector<uint32_t> ovf(novf);
+  std::vector<TileDesc> parents(novf);
   {
-    Scratch s(e);
+    // overflow list -> host, sorted for a deterministic order; parents gathered on the device
     HIPCK(hipMemcpyAsync(ovf.data(), d_ovf, novf * sizeof(uint32_t), hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
+    std::sort(ovf.begin(), ovf.end());
+    Scratch gs(e);
+    uint32_t* d_idx = gs.get<uint32_t>(novf);
+    TileDesc* d_par = gs.get<TileDesc>(novf);
+    if (gs.oom) return fail(VL_ENOMEM, "scratch");
+    HIPCK(hipMemcpyAsync(d_idx, ovf.data(), novf * sizeof(uint32_t), hipMemcpyHostToDevice, e->st));
+    launch_gather_desc(d_desc, d_idx, (int)novf, d_par, e->st);
+    HIPCK(hipMemcpyAsync(parents.data(), d_par, novf * sizeof(TileDesc), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
   }
-  std::sort(ovf.begin(), ovf.end());
-  std::vector<TileDesc> parents(novf);
-  for (uint32_t i = 0; i < novf; i++)
-    HIPCK(hipMemcpyAsync(&parents[i], d_desc + ovf[i], sizeof(TileDesc), hipMemcpyDeviceToHost, e->st));
-  HIPCK(hipStreamSynchronize(e->st));
   std::vector<TileDesc> subs;
   std::vector<int32_t> first(novf + 1);
   const int tgt = std::max(8, e->target);
@@ -309,7 +319,7 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
     k = std::min(k, width);
     if (k < 2)
       return fail(VL_EINTERNAL, "a 1-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
-                  (long long)n, e->nmax);
+                  (long long)n, TILE_BLOCK_NMAX);
     const int64_t sw = (width + k - 1) / k;
     k = (width + sw - 1) / sw;
     for (int64_t j = 0; j < k; j++) {
@@ -352,57 +362,71 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
                   bool two, bool area, Outs& o, int depth) {
   if (ntiles <= 0) return VL_OK;
   Scratch s(e);
-  uint64_t* status = s.get<uint64_t>(ntiles);
-  uint32_t* small = s.get<uint32_t>(8);
-  uint64_t* totals = s.get<uint64_t>(2);
-  uint32_t* ovf = s.get<uint32_t>(ntiles);
+  uint32_t* small = s.get<uint32_t>(8);   // [0] wave ticket [1] wave overflow count [2] block ticket [3] block overflow
+  unsigned long long* cnt = s.get<unsigned long long>(2);
+  uint32_t* ovf1 = s.get<uint32_t>(ntiles);
+  uint32_t* ovf2 = s.get<uint32_t>(ntiles);
   long long* d_area = s.get<long long>(1);
+  uint64_t* tab_off = area ? nullptr : s.get<uint64_t>(2 * (size_t)ntiles);
+  uint32_t* tab_n = area ? nullptr : s.get<uint32_t>(2 * (size_t)ntiles);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
-  int max_wg = tile_kernel_max_wg(e->nmax, two, area);
-  if (e->max_wg_cap > 0) max_wg = std::min(max_wg, e->max_wg_cap);
+  int max_w = tile_wave_max_wg(e->nmax, two, area);
+  int max_b = tile_block_max_wg(two, area);
+  if (e->max_wg_cap > 0) max_w = std::min(max_w, e->max_wg_cap);
+  // tile-ordered scratch streams (atomic ranges); sized by estimate, grown on overflow
+  DVec<vl_box> scrF(e), scrS(e);
+  DVec<uint8_t> scrT(e);
   uint64_t tot[2] = {0, 0};
   uint32_t nov = 0;
   const double t_pt = g_debug ? now_ms() : 0.0;
+  size_t capF = (size_t)nentries * 2 + 4096, capS = (size_t)nentries / 2 + 4096;
   for (int attempt = 0; attempt < 2; attempt++) {
     if (!area) {
-      // Make sure the look-back never sees a stale pointer range: capacity is
-      // checked in-kernel and a second attempt runs with exact sizes.
-      VLCK(o.out.ensure(std::max<size_t>(o.out.cap, o.nout + 1024), o.nout));
-      VLCK(o.side.ensure(std::max<size_t>(o.side.cap, o.nside + 1024), o.nside));
-      VLCK(o.side_tag.ensure(o.side.cap, o.nside));
+      VLCK(scrF.ensure(capF, 0));
+      VLCK(scrS.ensure(capS, 0));
+      VLCK(scrT.ensure(scrS.cap, 0));
     }
-    launch_fill_u64(status, 0ull, ntiles, e->st);
     launch_fill_u32(small, 0u, 8, e->st);
-    launch_fill_u64(totals, 0ull, 2, e->st);
+    launch_fill_u64((uint64_t*)cnt, 0ull, 2, e->st);
     launch_fill_u64((uint64_t*)d_area, 0ull, 1, e->st);
     VLCK(clear_err(e));
     TileOut to;
-    to.out = o.out.p;
-    to.side = o.side.p;
-    to.side_tag = o.side_tag.p;
-    to.out_cap = o.out.cap;
-    to.side_cap = std::min(o.side.cap, o.side_tag.cap);
-    to.out_base = o.nout;
-    to.side_base = o.nside;
-    to.status = status;
+    to.out = scrF.p;
+    to.side = scrS.p;
+    to.side_tag = scrT.p;
+    to.out_cap = scrF.cap;
+    to.side_cap = std::min(scrS.cap, scrT.cap);
+    to.cnt = cnt;
+    to.tab_off = tab_off;
+    to.tab_n = tab_n;
+    to.ntab = ntiles;
+    to.list = nullptr;
+    to.list_n = nullptr;
     to.ticket = small;
-    to.ovf = ovf;
+    to.ovf = ovf1;
     to.ovf_n = small + 1;
-    to.totals = totals;
     to.area = d_area;
     to.err = e->d_err;
     if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
-    launch_tile_kernel(e->nmax, two, area, d_desc, ntiles, d_entries, to, max_wg, e->st);
+    launch_tile_wave(e->nmax, two, area, d_desc, ntiles, d_entries, to, max_w, e->st);
+    // tiles above the wave kernel's capacity (count known only on the device)
+    TileOut tb = to;
+    tb.list = ovf1;
+    tb.list_n = small + 1;
+    tb.ticket = small + 2;
+    tb.ovf = ovf2;
+    tb.ovf_n = small + 3;
+    launch_tile_block(two, area, d_desc, ntiles, d_entries, tb, std::min(max_b, ntiles), e->st);
     HIPCK(hipGetLastError());
     if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
     struct {
-      uint64_t tot[2];
+      unsigned long long cnt[2];
       uint32_t err, nov;
       long long area;
     } rb;
-    HIPCK(hipMemcpyAsync(e->h_stage, totals, 16, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync(e->h_stage, cnt, 16, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipMemcpyAsync((char*)e->h_stage + 16, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 20, small + 1, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 20, small + 3, 4, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, d_area, 8, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     memcpy(&rb, e->h_stage, sizeof(rb));
@@ -410,13 +434,12 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       float ms = 0.f;
       HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
       auto& k = e->kst[KS_TILE];
-      k.launches++;
+      k.launches += 2;
       k.ms += ms;
-      k.bytes += 16.0 * (double)nentries + 16.0 * (double)(rb.tot[0] + rb.tot[1]) + 48.0 * ntiles;
+      k.bytes += 16.0 * (double)nentries + 16.0 * (double)(rb.cnt[0] + rb.cnt[1]) + 48.0 * ntiles;
     }
-    if (rb.err & ERR_LOOKBACK) return fail(VL_EINTERNAL, "tile look-back did not complete");
-    tot[0] = rb.tot[0];
-    tot[1] = rb.tot[1];
+    tot[0] = rb.cnt[0];
+    tot[1] = rb.cnt[1];
     nov = rb.nov;
     if (area) {
       o.area += rb.area;
@@ -424,14 +447,25 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     }
     if (rb.err & ERR_OVERFLOW_CAP) {
       if (attempt == 1) return fail(VL_EINTERNAL, "output capacity still short after resize");
-      VLCK(o.out.ensure(o.nout + tot[0] + 1024, o.nout));
-      VLCK(o.side.ensure(o.nside + tot[1] + 1024, o.nside));
-      VLCK(o.side_tag.ensure(o.nside + tot[1] + 1024, o.nside));
+      capF = tot[0] + 1024;
+      capS = tot[1] + 1024;
       continue;
     }
     break;
   }
-  if (!area) {
+  if (!area && (tot[0] + tot[1]) > 0) {
+    // tile order: exclusive scans of the per-tile counts, then a coalesced copy
+    VLCK(o.out.ensure(o.nout + tot[0] + 1, o.nout));
+    VLCK(o.side.ensure(o.nside + tot[1] + 1, o.nside));
+    VLCK(o.side_tag.ensure(o.nside + tot[1] + 1, o.nside));
+    uint64_t* dst = s.get<uint64_t>(2 * (size_t)ntiles + 2);
+    void* stmp = s.get<char>(scan_tmp_bytes(ntiles));
+    if (s.oom) return fail(VL_ENOMEM, "scratch");
+    scan_u32_to_u64(tab_n, dst, ntiles, nullptr, stmp, e->st);
+    scan_u32_to_u64(tab_n + ntiles, dst + ntiles, ntiles, nullptr, stmp, e->st);
+    launch_tile_reorder(ntiles, tab_off, tab_n, ntiles, dst, dst + ntiles, scrF.p, scrS.p, scrT.p, o.out.p, o.nout,
+                        o.side.p, o.side_tag.p, o.nside, e->st);
+    HIPCK(hipGetLastError());
     o.nout += tot[0];
     o.nside += tot[1];
   }
@@ -441,7 +475,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
             (unsigned long long)tot[1], nov, now_ms() - t_pt);
   if (nov > 0) {
     const double t_sp = g_debug ? now_ms() : 0.0;
-    VLCK(split_and_process(e, d_desc, ovf, nov, d_entries, two, area, o, depth));
+    VLCK(split_and_process(e, d_desc, ovf2, nov, d_entries, two, area, o, depth));
     if (g_debug) fprintf(stderr, "[vl]  split %u tiles (depth %d): %.2f ms\n", nov, depth, now_ms() - t_sp);
   }
   return VL_OK;
@@ -725,14 +759,14 @@ int vl_engine_create(int device, vl_engine** out) {
     return fail(VL_EHIP, "engine initialisation failed");
   }
   if (const char* v = getenv("VL_TILE_NMAX")) e->nmax = atoi(v);
-  if (e->nmax != 63 && e->nmax != 127 && e->nmax != 255) e->nmax = 127;
-  e->target = std::max(8, e->nmax / 2);
+  if (e->nmax != 63 && e->nmax != 127) e->nmax = 63;
+  e->target = std::max(8, (e->nmax * 2) / 3);
   *out = e;
   return VL_OK;
 }
 
-int vl_engine_destroy(vl_engine* e) {
-  if (!e) return VL_OK;
+static void engine_release(vl_engine* e) {
+  if (e->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(e->device);
   (void)hipStreamSynchronize(e->st);
   e->pool.trim();
@@ -744,6 +778,11 @@ int vl_engine_destroy(vl_engine* e) {
   (void)hipEventDestroy(e->kev1);
   (void)hipStreamDestroy(e->st);
   delete e;
+}
+
+int vl_engine_destroy(vl_engine* e) {
+  if (!e) return VL_OK;
+  engine_release(e);
   return VL_OK;
 }
 
@@ -752,9 +791,8 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   std::string k(key);
   if (k == "tile_nmax") {
-    if (v != 63 && v != 127 && v != 255) return fail(VL_EINVAL, "tile_nmax must be 63, 127 or 255");
+    if (v != 63 && v != 127) return fail(VL_EINVAL, "tile_nmax must be 63 or 127");
     e->nmax = (int)v;
-    e->target = std::min(e->target, e->nmax);
   } else if (k == "tile_target") {
     if (v < 4 || v > 4096) return fail(VL_EINVAL, "tile_target out of range");
     e->target = (int)v;
@@ -810,17 +848,20 @@ int vl_layer_create(vl_engine* e, vl_layer** out) {
   if (!e || !out) return fail(VL_EINVAL, "null argument");
   vl_layer* l = new vl_layer();
   l->e = e;
+  e->refs.fetch_add(1);
   *out = l;
   return VL_OK;
 }
 
 int vl_layer_destroy(vl_layer* l) {
   if (!l) return VL_OK;
+  vl_engine* e = l->e;
   if (l->d) {
-    std::lock_guard<std::recursive_mutex> lk(l->e->mu);
-    l->e->pool.put(l->d, l->bytes);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    e->pool.put(l->d, l->bytes);
   }
   delete l;
+  engine_release(e);
   return VL_OK;
 }
 
@@ -1198,12 +1239,16 @@ int vl_program_create(vl_engine* e, vl_program** out) {
   if (!e || !out) return fail(VL_EINVAL, "null argument");
   vl_program* p = new vl_program();
   p->e = e;
+  e->refs.fetch_add(1);
   *out = p;
   return VL_OK;
 }
 
 int vl_program_destroy(vl_program* p) {
+  if (!p) return VL_OK;
+  vl_engine* e = p->e;
   delete p;
+  engine_release(e);
   return VL_OK;
 }
 

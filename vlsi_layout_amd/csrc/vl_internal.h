@@ -60,13 +60,6 @@ enum : uint32_t {
   ERR_OVERFLOW_CAP = 4u,
 };
 
-// Look-back status word: [63:62] flag, [61:31] final count, [30:0] side count.
-constexpr uint64_t LB_AGG = 1ull << 62;
-constexpr uint64_t LB_PRE = 2ull << 62;
-constexpr uint64_t LB_FLAGMASK = 3ull << 62;
-constexpr uint64_t LB_VALMASK = ~LB_FLAGMASK;
-__host__ __device__ inline uint64_t lb_pack(uint64_t f, uint64_t s) { return (f << 31) | s; }
-
 // ---------------------------------------------------------------------------
 // Host-side launch wrappers (each .hip file implements its own kernels).
 
@@ -116,24 +109,45 @@ void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const 
 // Tile kernel: result strips of every tile.  Strips touching an internal seam
 // go to side[] with a tag byte; the rest go to out[].  Tiles with more boxes
 // than the kernel's capacity are listed in ovf[] (count in *ovf_n).
+// Every tile reserves its output ranges with one atomic add per stream and
+// records them in per-tile tables; a reorder pass then copies the strips into
+// tile order, so results do not depend on scheduling.
 struct TileOut {
-  vl_box* out;       // final strips; written at out[out_base + ...]
-  vl_box* side;      // seam strips
-  uint8_t* side_tag; // SEAM_* bits per side strip
+  vl_box* out;                 // final strips (scratch, atomic ranges)
+  vl_box* side;                // seam strips (scratch, atomic ranges)
+  uint8_t* side_tag;           // SEAM_* bits per side strip
   uint64_t out_cap, side_cap;
-  uint64_t out_base, side_base;
-  uint64_t* status;  // look-back words, one per tile, zeroed before launch
-  uint32_t* ticket;  // zeroed before launch
-  uint32_t* ovf;     // overflow tile indices
+  unsigned long long* cnt;     // [0] final strips reserved, [1] side strips reserved
+  uint64_t* tab_off;           // per tile: [2t] final range start, [2t+1] side range start
+  uint32_t* tab_n;             // per tile: [t] final count; tab_n[ntab + t] side count
+  int ntab;                    // table stride (number of descriptors)
+  const uint32_t* list;        // optional: tile = list[ticket] (null = ticket)
+  const uint32_t* list_n;      // optional: number of tiles, read on the device (null = ntiles)
+  uint32_t* ticket;            // zeroed before launch
+  uint32_t* ovf;               // tiles above the kernel's capacity
   uint32_t* ovf_n;
-  uint64_t* totals;  // [0]=final count [1]=side count (written by the last tile)
-  long long* area;   // area mode: accumulated union area (int64)
+  long long* area;             // area mode: accumulated union area (int64)
   uint32_t* err;
 };
-int tile_nmax_max();
-void launch_tile_kernel(int nmax, bool two_layers, bool area_mode, const TileDesc* desc,
-                        int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st);
-int tile_kernel_max_wg(int nmax, bool two_layers, bool area_mode);
+// Wave-per-tile kernel (capacity 63 or 127 boxes).
+void launch_tile_wave(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
+                      const vl_box* entries, TileOut o, int max_wg, hipStream_t st);
+int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode);
+// Workgroup-per-tile kernel (capacity 255 boxes) for tiles above the wave kernel's capacity.
+void launch_tile_block(bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
+                       const vl_box* entries, TileOut o, int max_wg, hipStream_t st);
+int tile_block_max_wg(bool two_layers, bool area_mode);
+constexpr int TILE_BLOCK_NMAX = 255;
+// Copies the per-tile ranges into tile order: strips of tile t go to
+// out[out_base + dstF[t] ...] and side[side_base + dstS[t] ...].
+void launch_tile_reorder(int ntiles, const uint64_t* tab_off, const uint32_t* tab_n, int ntab,
+                         const uint64_t* dstF, const uint64_t* dstS, const vl_box* scrF,
+                         const vl_box* scrS, const uint8_t* scrTag, vl_box* out, uint64_t out_base,
+                         vl_box* side, uint8_t* side_tag, uint64_t side_base, hipStream_t st);
+void launch_gather_desc(const TileDesc* desc, const uint32_t* idx, int n, TileDesc* out, hipStream_t st);
+// Exclusive scan of u32 values into u64 offsets.
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, uint64_t* total, void* tmp,
+                     hipStream_t st);
 // Split overflow tiles: per parent one workgroup re-bins its entries into k
 // x-sub-columns (sub-descriptors precomputed on the host).
 void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_first,

@@ -113,6 +113,10 @@ struct LoadPair {
   const uint32_t* b;
   __device__ uint64_t operator()(size_t i) const { return (uint64_t)a[i] + (uint64_t)b[i]; }
 };
+struct LoadWiden {
+  const uint32_t* a;
+  __device__ uint64_t operator()(size_t i) const { return (uint64_t)a[i]; }
+};
 
 size_t nblk(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
@@ -180,6 +184,10 @@ void scan_exclusive_u64(const uint64_t* in, uint64_t* out, size_t n, uint64_t* t
 void scan_pair_counts(const uint32_t* cntA, const uint32_t* cntB, uint64_t* off, size_t n,
                       uint64_t* total, void* tmp, hipStream_t st) {
   scan_impl<uint64_t, LoadPair>(LoadPair{cntA, cntB}, n, off, total, (char*)tmp, st);
+}
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, uint64_t* total, void* tmp,
+                     hipStream_t st) {
+  scan_impl<uint64_t, LoadWiden>(LoadWiden{in}, n, out, total, (char*)tmp, st);
 }
 
 }  // namespace vl

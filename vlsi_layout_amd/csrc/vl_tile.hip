@@ -1,29 +1,36 @@
-// vl_tile.hip — tile binning and the per-tile Boolean kernel for gfx950.
+// vl_tile.hip — tile binning and the per-tile Boolean kernels for gfx950.
 //
 // Binning: every box whose interior meets a tile of a uniform grid is copied
 // (after the op's per-box transform) into that tile's entry range: a counting
 // pass (atomics per tile), an exclusive scan, and a scatter pass.
 //
-// Tile kernel (one workgroup per tile, persistent over an atomic ticket):
-//  1. load the tile's boxes, clip them to the tile, gather their x and y
-//     coordinates plus the tile borders, bitonic-sort both lists in LDS and
-//     remove duplicates: nx x-breakpoints give ncol = nx-1 slabs, ny
-//     y-breakpoints give nrows = ny-1 elementary y-intervals;
-//  2. map every box to compressed indices (binary search) and rasterise the
-//     layers into per-slab coverage bitmasks in LDS, one 64-bit word per 64
-//     elementary y-intervals (wave64-sized words, ds_or_b64 atomics);
-//  3. combine the layers word by word with the operator (AND, A&~B, OR, XOR);
-//  4. per slab, list the maximal runs of set bits; a run starts a strip when
-//     the change mask (this slab XOR previous slab, dilated by one row) meets
-//     it, and the strip ends at the first later slab whose change mask meets
-//     it.  This is exactly heal_boxes' maximal-strip form
-//     (src/polygon_fracture.f90:881-960) restricted to the tile;
-//  5. strips that touch an internal tile border are seam pieces and go to the
-//     side stream with SEAM_* tags; the others are final;
-//  6. output offsets come from a decoupled look-back over tile order, so the
-//     output order is deterministic (tile order, then slab, then y).
-// In area mode step 4 sums the covered run lengths times slab widths (int64)
-// instead (calculate_union_area_fast, src/geometry.f90:1840-1927).
+// Per-tile kernels.  A tile's result inside the tile is computed on the
+// compressed grid of its own coordinates:
+//  1. clip the tile's boxes to the tile; sort their x and y coordinates (plus
+//     the tile borders) and drop duplicates: nx x-breakpoints give ncol = nx-1
+//     slabs, ny y-breakpoints give nrows = ny-1 elementary y-intervals;
+//  2. rasterise each layer into per-slab coverage bitmasks (one 64-bit word
+//     per 64 elementary intervals) with LDS atomics, and combine the layers
+//     word by word with the operator (AND, A&~B, OR, XOR);
+//  3. at every slab boundary k, the change mask D = dilate1(R_k ^ R_{k-1})
+//     marks where runs differ: a run of R_k that meets D starts a strip at k
+//     (start mask S_k, bit = first row of the run), a run of R_{k-1} that meets
+//     D ends one (end mask E_k).  These are exactly the maximal strips of
+//     heal_boxes (src/polygon_fracture.f90:881-960) inside the tile;
+//  4. the strips touching an internal tile border are seam pieces (SEAM_* tag,
+//     side stream); the rest are final.  In area mode step 3 is replaced by
+//     the int64 sum of run lengths times slab widths
+//     (calculate_union_area_fast, src/geometry.f90:1840-1927).
+//
+// The main kernel runs one wave64 per tile with no workgroup barriers: the
+// coordinate sort is a bitonic network in registers (shuffles across lanes),
+// duplicates are dropped with a wave scan, and the strips are paired by a
+// sweep over slab boundaries in which each lane owns rows {lane, lane+64,
+// ...}: it reads the broadcast S_k/E_k words and closes/opens its own rows'
+// strips.  Tiles above its capacity run on the 256-thread kernel (capacity
+// 255); larger tiles are split into x-sub-columns by the host.  Each tile
+// reserves its output ranges with one atomic add; a reorder pass restores
+// tile order.
 #include "vl_internal.h"
 #include "vl_dev.h"
 
@@ -96,7 +103,7 @@ __global__ void k_build_desc(Grid g, const uint64_t* __restrict__ off, const uin
   }
 }
 
-// ---- tile kernel -------------------------------------------------------------
+// ---- shared device helpers -----------------------------------------------------
 
 __device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int32_t v) {
   int lo = 0, hi = n;
@@ -106,6 +113,348 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int32_t 
   }
   return lo;
 }
+
+__device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
+  switch (op) {
+    case TOP_AND: return a & b;
+    case TOP_NOT: return a & ~b;
+    case TOP_XOR: return a ^ b;
+    default: return a | b;
+  }
+}
+
+// Bit w of the dilated change mask of columns cur/prv (null = empty column).
+template <int NW>
+__device__ __forceinline__ void change_mask(const uint64_t* cur, const uint64_t* prv, int W, uint64_t* D) {
+  uint64_t dl[NW + 2];
+  dl[0] = 0;
+  dl[W + 1] = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++)
+    if (w < W) dl[w + 1] = (cur ? cur[w] : 0ull) ^ (prv ? prv[w] : 0ull);
+#pragma unroll
+  for (int w = 0; w < NW; w++)
+    if (w < W) D[w] = dl[w + 1] | (dl[w + 1] << 1) | (dl[w] >> 63) | (dl[w + 1] >> 1) | (dl[w + 2] << 63);
+}
+
+__device__ __forceinline__ bool meets(const uint64_t* D, int lo, int hi) {   // any bit of D in [lo, hi]
+  const int w0 = lo >> 6, w1 = hi >> 6;
+  for (int w = w0; w <= w1; w++) {
+    const int l = (w == w0) ? (lo & 63) : 0;
+    const int h = (w == w1) ? (hi & 63) : 63;
+    if (D[w] & bits_range(l, h)) return true;
+  }
+  return false;
+}
+
+// Start bits of the runs of col that meet D (col null = empty).
+template <int NW>
+__device__ __forceinline__ void starts_meeting(const uint64_t* col, int W, const uint64_t* D, uint64_t* out) {
+#pragma unroll
+  for (int w = 0; w < NW; w++) out[w] = 0;
+  if (!col) return;
+  int a = -1;
+  for (int w = 0; w < W; w++) {
+    const uint64_t v = col[w];
+    int pos = 0;
+    while (pos < 64) {
+      if (a < 0) {
+        const uint64_t rem = v >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        a = w * 64 + pos;
+      } else {
+        const uint64_t rem = (~v) >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        if (meets(D, a, w * 64 + pos - 1)) out[a >> 6] |= 1ull << (a & 63);
+        a = -1;
+      }
+    }
+  }
+  if (a >= 0 && meets(D, a, W * 64 - 1)) out[a >> 6] |= 1ull << (a & 63);
+}
+
+// First zero bit at or above row a in a column (the run end).
+__device__ __forceinline__ int run_end(const uint64_t* col, int W, int a) {
+  int w = a >> 6;
+  uint64_t m = ~col[w] & (~0ull << (a & 63));
+  while (m == 0) {
+    if (++w >= W) return W * 64;
+    m = ~col[w];
+  }
+  return w * 64 + __ffsll((unsigned long long)m) - 1;
+}
+
+__device__ __forceinline__ uint64_t wave_excl_u32(uint32_t v, uint32_t* tot) {
+  uint32_t inc = wave_incl_scan(v);
+  *tot = __shfl(inc, WAVE - 1, WAVE);
+  return inc - v;
+}
+
+// ---- wave-per-tile kernel --------------------------------------------------------
+
+template <int NMAX, bool TWO, bool AREA>
+__global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc, int ntiles,
+                                               const vl_box* __restrict__ entries, TileOut o) {
+  constexpr int P = 2 * (NMAX + 1);   // coordinate slots per axis
+  constexpr int K = P / WAVE;         // slots per lane
+  constexpr int BPL = K / 2;          // boxes per lane (blocked)
+  constexpr int NCOL = P - 1;
+  constexpr int NW = P / WAVE;        // words per column
+  __shared__ int32_t ux[P], uy[P];
+  __shared__ uint64_t MA[NCOL * NW];
+  __shared__ uint64_t MX[(NCOL + 1) * NW];   // layer B mask, then start masks
+  __shared__ uint64_t ME[AREA ? 1 : (NCOL + 1) * NW];
+
+  const int lane = threadIdx.x;
+  const uint64_t lt = lanemask_lt();
+  const int nt = o.list_n ? (int)*o.list_n : ntiles;
+  for (;;) {
+    uint32_t tk = 0;
+    if (lane == 0) tk = atomicAdd(o.ticket, 1u);
+    tk = __shfl(tk, 0, WAVE);
+    if ((int)tk >= nt) break;
+    const uint32_t ti = o.list ? o.list[tk] : tk;
+    const TileDesc d = desc[ti];
+    const int nA = d.nA;
+    const int n = d.nA + d.nB;
+    const int op = d.op;
+    const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR);
+    if (n > NMAX || n == 0) {
+      if (lane == 0) {
+        if (n > NMAX) o.ovf[atomicAdd(o.ovf_n, 1u)] = ti;
+        if (!AREA) {
+          o.tab_n[ti] = 0;
+          o.tab_n[o.ntab + ti] = 0;
+        }
+      }
+      continue;
+    }
+    // 1. clipped boxes (blocked: lane owns boxes BPL*lane .. +BPL-1) and coordinates
+    int32_t vx[K], vy[K];
+    int32_t bx1[BPL], by1[BPL], bx2[BPL], by2[BPL];
+#pragma unroll
+    for (int q = 0; q < BPL; q++) {
+      const int i = BPL * lane + q;
+      int32_t x1 = INT32_MAX, y1 = INT32_MAX, x2 = INT32_MAX, y2 = INT32_MAX;
+      if (i < n) {
+        const vl_box b = entries[d.off + i];
+        x1 = max(b.x1, d.x0); x2 = min(b.x2, d.x1);
+        y1 = max(b.y1, d.y0); y2 = min(b.y2, d.y1);
+      } else if (i == n) {
+        x1 = d.x0; x2 = d.x1; y1 = d.y0; y2 = d.y1;
+      }
+      bx1[q] = x1; bx2[q] = x2; by1[q] = y1; by2[q] = y2;
+      vx[2 * q] = x1; vx[2 * q + 1] = x2;
+      vy[2 * q] = y1; vy[2 * q + 1] = y2;
+    }
+    const int cnt = 2 * n + 2;
+    // bitonic sort of P elements, element e = lane*K + k
+#pragma unroll
+    for (int size = 2; size <= P; size <<= 1) {
+#pragma unroll
+      for (int j = size >> 1; j > 0; j >>= 1) {
+        if (j >= K) {
+          const int lj = j / K;
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            const int e = lane * K + k;
+            const int px = __shfl_xor(vx[k], lj, WAVE);
+            const int py = __shfl_xor(vy[k], lj, WAVE);
+            const bool take_min = ((e & j) == 0) == ((e & size) == 0);
+            vx[k] = take_min ? min(vx[k], px) : max(vx[k], px);
+            vy[k] = take_min ? min(vy[k], py) : max(vy[k], py);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            if ((k & j) == 0) {
+              const int k2 = k | j;
+              const bool asc = ((lane * K + k) & size) == 0;
+              int32_t a = vx[k], b = vx[k2];
+              if ((a > b) == asc) { vx[k] = b; vx[k2] = a; }
+              a = vy[k]; b = vy[k2];
+              if ((a > b) == asc) { vy[k] = b; vy[k2] = a; }
+            }
+          }
+        }
+      }
+    }
+    // drop duplicates
+    int nxu, nyu;
+    {
+      const int32_t px = __shfl_up(vx[K - 1], 1, WAVE), py = __shfl_up(vy[K - 1], 1, WAVE);
+      bool fx[K], fy[K];
+      uint32_t cx = 0, cy = 0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int e = lane * K + k;
+        const int32_t qx = k ? vx[k - 1] : px, qy = k ? vy[k - 1] : py;
+        fx[k] = e < cnt && (e == 0 || vx[k] != qx);
+        fy[k] = e < cnt && (e == 0 || vy[k] != qy);
+        cx += fx[k];
+        cy += fy[k];
+      }
+      uint32_t tx, ty;
+      uint32_t ox = (uint32_t)wave_excl_u32(cx, &tx), oy = (uint32_t)wave_excl_u32(cy, &ty);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (fx[k]) ux[ox++] = vx[k];
+        if (fy[k]) uy[oy++] = vy[k];
+      }
+      nxu = (int)tx;
+      nyu = (int)ty;
+    }
+    const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 63) >> 6;
+    const int words = ncol * W;
+    for (int k = lane; k < words; k += WAVE) {
+      MA[k] = 0ull;
+      if (TWO) MX[k] = 0ull;
+    }
+    __syncthreads();
+    // 2. compress + rasterise
+#pragma unroll
+    for (int q = 0; q < BPL; q++) {
+      const int i = BPL * lane + q;
+      if (i >= n) continue;
+      const int cx1 = lower_bound_i32(ux, nxu, bx1[q]), cx2 = lower_bound_i32(ux, nxu, bx2[q]);
+      const int cy1 = lower_bound_i32(uy, nyu, by1[q]), cy2 = lower_bound_i32(uy, nyu, by2[q]);
+      if (cx1 >= cx2 || cy1 >= cy2) continue;
+      uint64_t* M = (twol && i >= nA) ? MX : MA;
+      const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;
+      for (int c = cx1; c < cx2; c++)
+        for (int w = w1; w <= w2; w++) {
+          const int lo = (w == w1) ? (cy1 & 63) : 0;
+          const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
+          atomicOr((unsigned long long*)&M[c * W + w], (unsigned long long)bits_range(lo, hi));
+        }
+    }
+    __syncthreads();
+    if (twol) {
+      for (int k = lane; k < words; k += WAVE) MA[k] = combine(op, MA[k], MX[k]);
+      __syncthreads();
+    }
+    if (AREA) {
+      long long acc = 0;
+      for (int j = lane; j < ncol; j += WAVE) {
+        const uint64_t* col = MA + j * W;
+        long long h = 0;
+        int a = -1;
+        for (int w = 0; w < W; w++) {
+          const uint64_t v = col[w];
+          int pos = 0;
+          while (pos < 64) {
+            if (a < 0) {
+              const uint64_t rem = v >> pos;
+              if (rem == 0) break;
+              pos += __ffsll((unsigned long long)rem) - 1;
+              a = w * 64 + pos;
+            } else {
+              const uint64_t rem = (~v) >> pos;
+              if (rem == 0) break;
+              pos += __ffsll((unsigned long long)rem) - 1;
+              h += (long long)uy[w * 64 + pos] - uy[a];
+              a = -1;
+            }
+          }
+        }
+        if (a >= 0) h += (long long)uy[W * 64] - uy[a];
+        acc += h * ((long long)ux[j + 1] - ux[j]);
+      }
+      acc = wave_sum(acc);
+      if (lane == 0 && acc) atomicAdd((unsigned long long*)o.area, (unsigned long long)acc);
+      __syncthreads();
+      continue;
+    }
+    // 3. start / end masks at every slab boundary k = 0..ncol
+    for (int k = lane; k <= ncol; k += WAVE) {
+      const uint64_t* cur = (k < ncol) ? MA + k * W : nullptr;
+      const uint64_t* prv = (k > 0) ? MA + (k - 1) * W : nullptr;
+      uint64_t D[NW], S[NW], E[NW];
+      change_mask<NW>(cur, prv, W, D);
+      starts_meeting<NW>(cur, W, D, S);
+      starts_meeting<NW>(prv, W, D, E);
+      for (int w = 0; w < W; w++) {
+        MX[k * W + w] = S[w];
+        ME[k * W + w] = E[w];
+      }
+    }
+    __syncthreads();
+    // 4. pair starts and ends: lane owns rows 64*w + lane
+    const uint32_t flags = d.flags;
+    uint32_t cF = 0, cS = 0;
+    uint64_t baseF = 0, baseS = 0;
+    bool okF = true, okS = true;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+      int oj[NW], ob[NW];
+      uint32_t rF = 0, rS = 0;
+#pragma unroll
+      for (int w = 0; w < NW; w++) { oj[w] = 0; ob[w] = 0; }
+      for (int k = 0; k <= ncol; k++) {
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          if (w >= W) break;
+          const uint64_t e = ME[k * W + w], s = MX[k * W + w];
+          if ((e | s) == 0) continue;
+          const int a = w * 64 + lane;
+          const bool endm = (e >> lane) & 1ull;
+          bool side = false;
+          if (endm)
+            side = (a == 0 && (flags & SEAM_B)) || (ob[w] == nrows && (flags & SEAM_T)) ||
+                   (oj[w] == 0 && (flags & SEAM_L)) || (k == ncol && (flags & SEAM_R));
+          const uint64_t bF = __ballot(endm && !side), bS = __ballot(endm && side);
+          if (pass == 1 && endm) {
+            vl_box st;
+            st.x1 = ux[oj[w]]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[ob[w]];
+            if (side) {
+              const uint64_t p = baseS + rS + (uint32_t)__popcll(bS & lt);
+              if (okS) {
+                o.side[p] = st;
+                o.side_tag[p] = (uint8_t)(((a == 0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                                          ((ob[w] == nrows && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                                          ((oj[w] == 0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                                          ((k == ncol && (flags & SEAM_R)) ? SEAM_R : 0u));
+              }
+            } else {
+              const uint64_t p = baseF + rF + (uint32_t)__popcll(bF & lt);
+              if (okF) o.out[p] = st;
+            }
+          }
+          rF += (uint32_t)__popcll(bF);
+          rS += (uint32_t)__popcll(bS);
+          if ((s >> lane) & 1ull) {
+            oj[w] = k;
+            ob[w] = run_end(MA + k * W, W, a);
+          }
+        }
+      }
+      if (pass == 0) {
+        cF = rF;
+        cS = rS;
+        unsigned long long b0 = 0, b1 = 0;
+        if (lane == 0) {
+          b0 = cF ? atomicAdd(&o.cnt[0], (unsigned long long)cF) : 0ull;
+          b1 = cS ? atomicAdd(&o.cnt[1], (unsigned long long)cS) : 0ull;
+          o.tab_off[2 * ti] = b0;
+          o.tab_off[2 * ti + 1] = b1;
+          o.tab_n[ti] = cF;
+          o.tab_n[o.ntab + ti] = cS;
+          if (b0 + cF > o.out_cap || b1 + cS > o.side_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
+        }
+        baseF = __shfl(b0, 0, WAVE);
+        baseS = __shfl(b1, 0, WAVE);
+        okF = baseF + cF <= o.out_cap;
+        okS = baseS + cS <= o.side_cap;
+        if (cF + cS == 0) break;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- workgroup-per-tile kernel (large tiles) ------------------------------------
 
 // Calls f(a, b) for every maximal run [a, b) of set bits in a column of W words.
 template <class F>
@@ -144,20 +493,9 @@ __device__ __forceinline__ bool any_diff(const uint64_t* c, const uint64_t* p, i
   return false;
 }
 
-__device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
-  switch (op) {
-    case TOP_AND: return a & b;
-    case TOP_NOT: return a & ~b;
-    case TOP_XOR: return a ^ b;
-    default: return a | b;
-  }
-}
-
-constexpr int LOOKBACK_SPIN_LIMIT = 1 << 26;
-
 template <int NMAX, int NT, bool TWO, bool AREA>
-__global__ __launch_bounds__(NT) void k_tile(const TileDesc* __restrict__ desc, int ntiles,
-                                             const vl_box* __restrict__ entries, TileOut o) {
+__global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc, int ntiles,
+                                               const vl_box* __restrict__ entries, TileOut o) {
   constexpr int P = 2 * (NMAX + 1);
   constexpr int NCOL = 2 * NMAX + 1;
   constexpr int NW = (NCOL + 63) / 64;
@@ -170,28 +508,38 @@ __global__ __launch_bounds__(NT) void k_tile(const TileDesc* __restrict__ desc, 
   __shared__ int wsI[NT / WAVE];
   __shared__ uint64_t wsU[NT / WAVE];
   __shared__ long long wsA[NT / WAVE];
-  __shared__ int s_tile, s_nx, s_ny;
+  __shared__ uint32_t s_tile;
   __shared__ uint64_t s_offF, s_offS;
   __shared__ int s_okF, s_okS;
 
   const int tid = threadIdx.x;
+  const int nt = o.list_n ? (int)*o.list_n : ntiles;
   for (;;) {
-    if (tid == 0) s_tile = (int)atomicAdd(o.ticket, 1u);
+    if (tid == 0) s_tile = atomicAdd(o.ticket, 1u);
     __syncthreads();
-    const int t = s_tile;
-    if (t >= ntiles) break;
-    const TileDesc d = desc[t];
+    const uint32_t tk = s_tile;
+    if ((int)tk >= nt) break;
+    const uint32_t ti = o.list ? o.list[tk] : tk;
+    const TileDesc d = desc[ti];
     const int nA = d.nA;
     const int n = d.nA + d.nB;
     const int op = d.op;
     const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR);
-    const bool big = n > NMAX;
-
+    if (n > NMAX || n == 0) {
+      if (tid == 0) {
+        if (n > NMAX) o.ovf[atomicAdd(o.ovf_n, 1u)] = ti;
+        if (!AREA) {
+          o.tab_n[ti] = 0;
+          o.tab_n[o.ntab + ti] = 0;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     uint64_t F = 0, S = 0;
     int ncol = 0, nrows = 0, W = 0;
     long long my_area = 0;
-    if (!big && n > 0) {
-      // 1. coordinates
+    {
       const int cnt = 2 * n + 2;
       int Pn = 2;
       while (Pn < cnt) Pn <<= 1;
@@ -244,7 +592,6 @@ __global__ __launch_bounds__(NT) void k_tile(const TileDesc* __restrict__ desc, 
       nrows = nyu - 1;
       W = (nrows + 63) >> 6;
       __syncthreads();
-      // 2. compressed coordinates, clear masks, rasterise
       for (int i = tid; i < n; i += NT) {
         const vl_box b = entries[d.off + i];
         cb[i][0] = (int16_t)lower_bound_i32(ux, nxu, max(b.x1, d.x0));
@@ -273,12 +620,10 @@ __global__ __launch_bounds__(NT) void k_tile(const TileDesc* __restrict__ desc, 
         }
       }
       __syncthreads();
-      // 3. combine
       if (twol) {
         for (int k = tid; k < words; k += NT) MA[k] = combine(op, MA[k], MB[k]);
         __syncthreads();
       }
-      // 4. count strips per slab (or area)
       const uint32_t flags = d.flags;
       for (int j = tid; j < ncol; j += NT) {
         const uint64_t* cur = MA + j * W;
@@ -326,52 +671,27 @@ __global__ __launch_bounds__(NT) void k_tile(const TileDesc* __restrict__ desc, 
         long long s = 0;
         for (int i = 0; i < NT / WAVE; i++) s += wsA[i];
         if (s) atomicAdd((unsigned long long*)o.area, (unsigned long long)s);
-        if (big) o.ovf[atomicAdd(o.ovf_n, 1u)] = (uint32_t)t;
       }
       __syncthreads();
       continue;
     }
 
-    // 5. decoupled look-back over tile order for output offsets
     if (tid == 0) {
-      if (big) o.ovf[atomicAdd(o.ovf_n, 1u)] = (uint32_t)t;
-      const uint64_t agg = lb_pack(F, S);
-      uint64_t excl = 0;
-      if (t == 0) {
-        __hip_atomic_store(&o.status[0], LB_PRE | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(&o.status[t], LB_AGG | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        int j = t - 1;
-        int spins = 0;
-        for (;;) {
-          const uint64_t v = __hip_atomic_load(&o.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-          const uint64_t fl = v & LB_FLAGMASK;
-          if (fl == 0) {
-            if (++spins > LOOKBACK_SPIN_LIMIT) { atomicOr(o.err, ERR_LOOKBACK); break; }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          excl += v & LB_VALMASK;
-          if (fl == LB_PRE || j == 0) break;
-          j--;
-        }
-        __hip_atomic_store(&o.status[t], LB_PRE | (excl + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      const uint64_t eF = excl >> 31, eS = excl & 0x7fffffffull;
-      s_offF = o.out_base + eF;
-      s_offS = o.side_base + eS;
-      s_okF = (o.out_base + eF + F) <= o.out_cap;
-      s_okS = (o.side_base + eS + S) <= o.side_cap;
+      const unsigned long long b0 = F ? atomicAdd(&o.cnt[0], (unsigned long long)F) : 0ull;
+      const unsigned long long b1 = S ? atomicAdd(&o.cnt[1], (unsigned long long)S) : 0ull;
+      o.tab_off[2 * ti] = b0;
+      o.tab_off[2 * ti + 1] = b1;
+      o.tab_n[ti] = (uint32_t)F;
+      o.tab_n[o.ntab + ti] = (uint32_t)S;
+      s_offF = b0;
+      s_offS = b1;
+      s_okF = (b0 + F) <= o.out_cap;
+      s_okS = (b1 + S) <= o.side_cap;
       if (!s_okF || !s_okS) atomicOr(o.err, ERR_OVERFLOW_CAP);
-      if (t == ntiles - 1) {
-        o.totals[0] = eF + F;
-        o.totals[1] = eS + S;
-      }
     }
     __syncthreads();
 
-    // 6. emit
-    if (!big && n > 0 && (F + S) > 0) {
+    if ((F + S) > 0) {
       const uint64_t offF = s_offF, offS = s_offS;
       const bool okF = s_okF, okS = s_okS;
       const uint32_t flags = d.flags;
@@ -406,7 +726,39 @@ __global__ __launch_bounds__(NT) void k_tile(const TileDesc* __restrict__ desc, 
   }
 }
 
-// ---- overflow splitting ------------------------------------------------------
+// ---- reorder into tile order -------------------------------------------------------
+constexpr int REORDER_TILES = 16;
+
+__global__ __launch_bounds__(256) void k_reorder(int ntiles, const uint64_t* __restrict__ tab_off,
+                                                 const uint32_t* __restrict__ tab_n, int ntab,
+                                                 const uint64_t* __restrict__ dstF, const uint64_t* __restrict__ dstS,
+                                                 const vl_box* __restrict__ scrF, const vl_box* __restrict__ scrS,
+                                                 const uint8_t* __restrict__ scrTag, vl_box* __restrict__ out,
+                                                 uint64_t out_base, vl_box* __restrict__ side,
+                                                 uint8_t* __restrict__ side_tag, uint64_t side_base) {
+  const int t0 = blockIdx.x * REORDER_TILES;
+  for (int t = t0; t < t0 + REORDER_TILES && t < ntiles; t++) {
+    const uint32_t nf = tab_n[t], ns = tab_n[ntab + t];
+    if (nf) {
+      const uint64_t s = tab_off[2 * t], dd = out_base + dstF[t];
+      for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x) out[dd + i] = scrF[s + i];
+    }
+    if (ns) {
+      const uint64_t s = tab_off[2 * t + 1], dd = side_base + dstS[t];
+      for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        side[dd + i] = scrS[s + i];
+        side_tag[dd + i] = scrTag[s + i];
+      }
+    }
+  }
+}
+
+__global__ void k_gather_desc(const TileDesc* __restrict__ desc, const uint32_t* __restrict__ idx, int n,
+                              TileDesc* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = desc[idx[i]];
+}
+
+// ---- overflow splitting ----------------------------------------------------------
 // One workgroup per parent tile; sub-tiles are x-sub-columns of the parent.
 __device__ __forceinline__ void sub_range(const vl_box& b, const TileDesc& p, const TileDesc* subs, int first,
                                           int nsub, int& s1, int& s2) {
@@ -468,26 +820,41 @@ unsigned grid_for(int64_t n, int per = 256) {
   return (unsigned)g;
 }
 
-template <int NMAX, bool TWO, bool AREA>
-void launch_tile_t(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
-  int g = ntiles < max_wg ? ntiles : max_wg;
-  if (g < 1) g = 1;
-  hipLaunchKernelGGL((k_tile<NMAX, 256, TWO, AREA>), dim3(g), dim3(256), 0, st, desc, ntiles, entries, o);
+int num_cus() {
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (ncu < 1) ncu = 1;
+  }
+  return ncu;
+}
+
+template <class KF>
+int occupancy_wg(KF kern, int threads) {
+  int per = 0;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, threads, 0);
+  if (per < 1) per = 1;
+  return per * num_cus();
 }
 
 template <int NMAX, bool TWO, bool AREA>
-int max_wg_t() {
-  int dev = 0, ncu = 0, per = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_tile<NMAX, 256, TWO, AREA>, 256, 0);
-  if (per < 1) per = 1;
-  return ncu * per;
+void launch_w(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
+  int g = ntiles < max_wg ? ntiles : max_wg;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL((k_tile_w<NMAX, TWO, AREA>), dim3(g), dim3(WAVE), 0, st, desc, ntiles, entries, o);
+}
+
+template <bool TWO, bool AREA>
+void launch_b(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
+  int g = ntiles < max_wg ? ntiles : max_wg;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL((k_tile_b<TILE_BLOCK_NMAX, 256, TWO, AREA>), dim3(g), dim3(256), 0, st, desc, ntiles,
+                     entries, o);
 }
 
 }  // namespace
-
-int tile_nmax_max() { return 255; }
 
 void launch_bin_count(const vl_box* b, int64_t n, Xform xf, Grid g, uint32_t* cnt, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n)), dim3(256), 0, st, b, n, xf, g, cnt);
@@ -504,36 +871,81 @@ void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const 
   hipLaunchKernelGGL(k_build_desc, dim3(grid_for(nt)), dim3(256), 0, st, g, off, cntA, cntB, op, desc);
 }
 
-#define VL_TILE_DISPATCH(NM)                                                              \
-  if (nmax == NM) {                                                                       \
-    if (area_mode) {                                                                      \
-      if (two_layers) launch_tile_t<NM, true, true>(desc, ntiles, entries, o, max_wg, st); \
-      else launch_tile_t<NM, false, true>(desc, ntiles, entries, o, max_wg, st);           \
-    } else {                                                                              \
-      if (two_layers) launch_tile_t<NM, true, false>(desc, ntiles, entries, o, max_wg, st);\
-      else launch_tile_t<NM, false, false>(desc, ntiles, entries, o, max_wg, st);          \
-    }                                                                                     \
-    return;                                                                               \
+#define VL_W_DISPATCH(NM)                                                                  \
+  if (nmax == NM) {                                                                        \
+    if (area_mode) {                                                                       \
+      if (two_layers) launch_w<NM, true, true>(desc, ntiles, entries, o, max_wg, st);      \
+      else launch_w<NM, false, true>(desc, ntiles, entries, o, max_wg, st);                \
+    } else {                                                                               \
+      if (two_layers) launch_w<NM, true, false>(desc, ntiles, entries, o, max_wg, st);     \
+      else launch_w<NM, false, false>(desc, ntiles, entries, o, max_wg, st);               \
+    }                                                                                      \
+    return;                                                                                \
   }
 
-void launch_tile_kernel(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
-                        const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
-  VL_TILE_DISPATCH(63)
-  VL_TILE_DISPATCH(127)
-  VL_TILE_DISPATCH(255)
+void launch_tile_wave(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
+                      const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
+  VL_W_DISPATCH(63)
+  VL_W_DISPATCH(127)
 }
 
-#define VL_TILE_MAXWG(NM)                                                   \
-  if (nmax == NM) {                                                         \
-    if (area_mode) return two_layers ? max_wg_t<NM, true, true>() : max_wg_t<NM, false, true>(); \
-    return two_layers ? max_wg_t<NM, true, false>() : max_wg_t<NM, false, false>();              \
+#define VL_W_MAXWG(NM)                                                                                      \
+  if (nmax == NM) {                                                                                         \
+    if (area_mode)                                                                                          \
+      return two_layers ? occupancy_wg(k_tile_w<NM, true, true>, WAVE) : occupancy_wg(k_tile_w<NM, false, true>, WAVE); \
+    return two_layers ? occupancy_wg(k_tile_w<NM, true, false>, WAVE) : occupancy_wg(k_tile_w<NM, false, false>, WAVE); \
   }
 
-int tile_kernel_max_wg(int nmax, bool two_layers, bool area_mode) {
-  VL_TILE_MAXWG(63)
-  VL_TILE_MAXWG(127)
-  VL_TILE_MAXWG(255)
-  return 256;
+int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode) {
+  static int cache[2][2][2] = {};
+  const int ni = nmax == 63 ? 0 : 1;
+  int& c = cache[ni][two_layers][area_mode];
+  if (c) return c;
+  auto f = [&]() -> int {
+    VL_W_MAXWG(63)
+    VL_W_MAXWG(127)
+    return 1024;
+  };
+  c = f();
+  return c;
+}
+
+void launch_tile_block(bool two_layers, bool area_mode, const TileDesc* desc, int ntiles, const vl_box* entries,
+                       TileOut o, int max_wg, hipStream_t st) {
+  if (area_mode) {
+    if (two_layers) launch_b<true, true>(desc, ntiles, entries, o, max_wg, st);
+    else launch_b<false, true>(desc, ntiles, entries, o, max_wg, st);
+  } else {
+    if (two_layers) launch_b<true, false>(desc, ntiles, entries, o, max_wg, st);
+    else launch_b<false, false>(desc, ntiles, entries, o, max_wg, st);
+  }
+}
+
+int tile_block_max_wg(bool two_layers, bool area_mode) {
+  static int cache[2][2] = {};
+  int& c = cache[two_layers][area_mode];
+  if (c) return c;
+  if (area_mode)
+    c = two_layers ? occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, true, true>, 256)
+                   : occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, false, true>, 256);
+  else
+    c = two_layers ? occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, true, false>, 256)
+                   : occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, false, false>, 256);
+  return c;
+}
+
+void launch_tile_reorder(int ntiles, const uint64_t* tab_off, const uint32_t* tab_n, int ntab, const uint64_t* dstF,
+                         const uint64_t* dstS, const vl_box* scrF, const vl_box* scrS, const uint8_t* scrTag,
+                         vl_box* out, uint64_t out_base, vl_box* side, uint8_t* side_tag, uint64_t side_base,
+                         hipStream_t st) {
+  if (ntiles <= 0) return;
+  const int g = (ntiles + REORDER_TILES - 1) / REORDER_TILES;
+  hipLaunchKernelGGL(k_reorder, dim3(g), dim3(256), 0, st, ntiles, tab_off, tab_n, ntab, dstF, dstS, scrF, scrS,
+                     scrTag, out, out_base, side, side_tag, side_base);
+}
+
+void launch_gather_desc(const TileDesc* desc, const uint32_t* idx, int n, TileDesc* out, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_desc, dim3(grid_for(n)), dim3(256), 0, st, desc, idx, n, out);
 }
 
 void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_first, const TileDesc* subs,
@@ -544,8 +956,8 @@ void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_firs
 }
 
 void launch_split_scatter(const TileDesc* parents, int np, const int32_t* sub_first, TileDesc* subs, int ns,
-                           const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB, uint32_t* fillA,
-                           uint32_t* fillB, const vl_box* entries, vl_box* sub_entries, hipStream_t st) {
+                          const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB, uint32_t* fillA,
+                          uint32_t* fillB, const vl_box* entries, vl_box* sub_entries, hipStream_t st) {
   if (np <= 0) return;
   hipLaunchKernelGGL(k_split_scatter, dim3(np), dim3(256), 0, st, parents, sub_first, subs, off, cntA, fillA,
                      fillB, entries, sub_entries);
