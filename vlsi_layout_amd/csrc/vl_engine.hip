@@ -362,72 +362,76 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
                   bool two, bool area, Outs& o, int depth) {
   if (ntiles <= 0) return VL_OK;
   Scratch s(e);
-  uint32_t* small = s.get<uint32_t>(8);   // [0] wave ticket [1] wave overflow count [2] block ticket [3] block overflow
-  unsigned long long* cnt = s.get<unsigned long long>(2);
+  uint32_t* small = s.get<uint32_t>(4);   // [0] wave overflow count [1] block overflow count
+  unsigned long long* cnt = s.get<unsigned long long>(3);   // [0] strips [1] seam strips [2] max region fill
   uint32_t* ovf1 = s.get<uint32_t>(ntiles);
   uint32_t* ovf2 = s.get<uint32_t>(ntiles);
   long long* d_area = s.get<long long>(1);
-  uint64_t* tab_off = area ? nullptr : s.get<uint64_t>(2 * (size_t)ntiles);
+  uint64_t* tab_off = area ? nullptr : s.get<uint64_t>((size_t)ntiles);
   uint32_t* tab_n = area ? nullptr : s.get<uint32_t>(2 * (size_t)ntiles);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
   int max_w = tile_wave_max_wg(e->nmax, two, area);
   int max_b = tile_block_max_wg(two, area);
   if (e->max_wg_cap > 0) max_w = std::min(max_w, e->max_wg_cap);
-  // tile-ordered scratch streams (atomic ranges); sized by estimate, grown on overflow
-  DVec<vl_box> scrF(e), scrS(e);
+  // Exact grids: workgroup g takes tiles g, g+G, ... and owns region g of the
+  // scratch stream (wave kernel: regions 0..Gw-1, block kernel: Gw..Gw+Gb-1).
+  const int Gw = std::max(1, std::min(ntiles, max_w));
+  const int Gb = std::max(1, std::min(ntiles, max_b));
+  DVec<vl_box> scr(e);
   DVec<uint8_t> scrT(e);
-  uint64_t tot[2] = {0, 0};
+  uint64_t total = 0, nsideT = 0;
   uint32_t nov = 0;
   const double t_pt = g_debug ? now_ms() : 0.0;
-  size_t capF = (size_t)nentries * 2 + 4096, capS = (size_t)nentries / 2 + 4096;
+  // region size: a few times the mean strips per workgroup; regrown to the
+  // measured maximum fill if any region overflows
+  uint64_t region = (uint64_t)(3.0 * (double)nentries / Gw) + 512;
   for (int attempt = 0; attempt < 2; attempt++) {
     if (!area) {
-      VLCK(scrF.ensure(capF, 0));
-      VLCK(scrS.ensure(capS, 0));
-      VLCK(scrT.ensure(scrS.cap, 0));
+      const size_t need = (size_t)(Gw + Gb) * region;
+      VLCK(scr.ensure(need, 0));
+      VLCK(scrT.ensure(need, 0));
     }
-    launch_fill_u32(small, 0u, 8, e->st);
-    launch_fill_u64((uint64_t*)cnt, 0ull, 2, e->st);
+    launch_fill_u32(small, 0u, 4, e->st);
+    launch_fill_u64((uint64_t*)cnt, 0ull, 3, e->st);
     launch_fill_u64((uint64_t*)d_area, 0ull, 1, e->st);
     VLCK(clear_err(e));
     TileOut to;
-    to.out = scrF.p;
-    to.side = scrS.p;
-    to.side_tag = scrT.p;
-    to.out_cap = scrF.cap;
-    to.side_cap = std::min(scrS.cap, scrT.cap);
+    to.strips = scr.p;
+    to.tag = scrT.p;
+    to.region_cap = area ? 0 : region;
+    to.region_base = 0;
     to.cnt = cnt;
+    to.maxuse = cnt + 2;
     to.tab_off = tab_off;
     to.tab_n = tab_n;
     to.ntab = ntiles;
     to.list = nullptr;
     to.list_n = nullptr;
-    to.ticket = small;
     to.ovf = ovf1;
-    to.ovf_n = small + 1;
+    to.ovf_n = small;
     to.area = d_area;
     to.err = e->d_err;
     if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
-    launch_tile_wave(e->nmax, two, area, d_desc, ntiles, d_entries, to, max_w, e->st);
+    launch_tile_wave(e->nmax, two, area, d_desc, ntiles, d_entries, to, Gw, e->st);
     // tiles above the wave kernel's capacity (count known only on the device)
     TileOut tb = to;
+    tb.region_base = Gw;
     tb.list = ovf1;
-    tb.list_n = small + 1;
-    tb.ticket = small + 2;
+    tb.list_n = small;
     tb.ovf = ovf2;
-    tb.ovf_n = small + 3;
-    launch_tile_block(two, area, d_desc, ntiles, d_entries, tb, std::min(max_b, ntiles), e->st);
+    tb.ovf_n = small + 1;
+    launch_tile_block(two, area, d_desc, ntiles, d_entries, tb, Gb, e->st);
     HIPCK(hipGetLastError());
     if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
     struct {
-      unsigned long long cnt[2];
+      unsigned long long cnt[3];
       uint32_t err, nov;
       long long area;
     } rb;
-    HIPCK(hipMemcpyAsync(e->h_stage, cnt, 16, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 16, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 20, small + 3, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, d_area, 8, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync(e->h_stage, cnt, 24, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 28, small + 1, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 32, d_area, 8, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     memcpy(&rb, e->h_stage, sizeof(rb));
     if (e->prof) {
@@ -436,10 +440,10 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       auto& k = e->kst[KS_TILE];
       k.launches += 2;
       k.ms += ms;
-      k.bytes += 16.0 * (double)nentries + 16.0 * (double)(rb.cnt[0] + rb.cnt[1]) + 48.0 * ntiles;
+      k.bytes += 16.0 * (double)nentries + 16.0 * (double)rb.cnt[0] + 48.0 * ntiles;
     }
-    tot[0] = rb.cnt[0];
-    tot[1] = rb.cnt[1];
+    total = rb.cnt[0];
+    nsideT = rb.cnt[1];
     nov = rb.nov;
     if (area) {
       o.area += rb.area;
@@ -447,32 +451,33 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     }
     if (rb.err & ERR_OVERFLOW_CAP) {
       if (attempt == 1) return fail(VL_EINTERNAL, "output capacity still short after resize");
-      capF = tot[0] + 1024;
-      capS = tot[1] + 1024;
+      region = rb.cnt[2] + 64;
+      if (g_debug) fprintf(stderr, "[vl]  region overflow: retry with %llu strips/region\n", (unsigned long long)region);
       continue;
     }
     break;
   }
-  if (!area && (tot[0] + tot[1]) > 0) {
+  const uint64_t nfinal = total - nsideT;
+  if (!area && total > 0) {
     // tile order: exclusive scans of the per-tile counts, then a coalesced copy
-    VLCK(o.out.ensure(o.nout + tot[0] + 1, o.nout));
-    VLCK(o.side.ensure(o.nside + tot[1] + 1, o.nside));
-    VLCK(o.side_tag.ensure(o.nside + tot[1] + 1, o.nside));
+    VLCK(o.out.ensure(o.nout + nfinal + 1, o.nout));
+    VLCK(o.side.ensure(o.nside + nsideT + 1, o.nside));
+    VLCK(o.side_tag.ensure(o.nside + nsideT + 1, o.nside));
     uint64_t* dst = s.get<uint64_t>(2 * (size_t)ntiles + 2);
     void* stmp = s.get<char>(scan_tmp_bytes(ntiles));
     if (s.oom) return fail(VL_ENOMEM, "scratch");
     scan_u32_to_u64(tab_n, dst, ntiles, nullptr, stmp, e->st);
     scan_u32_to_u64(tab_n + ntiles, dst + ntiles, ntiles, nullptr, stmp, e->st);
-    launch_tile_reorder(ntiles, tab_off, tab_n, ntiles, dst, dst + ntiles, scrF.p, scrS.p, scrT.p, o.out.p, o.nout,
-                        o.side.p, o.side_tag.p, o.nside, e->st);
+    launch_tile_reorder(ntiles, tab_off, tab_n, ntiles, dst, dst + ntiles, scr.p, scrT.p, o.out.p, o.nout, o.side.p,
+                        o.side_tag.p, o.nside, e->st);
     HIPCK(hipGetLastError());
-    o.nout += tot[0];
-    o.nside += tot[1];
+    o.nout += nfinal;
+    o.nside += nsideT;
   }
   if (g_debug)
     fprintf(stderr, "[vl]  tiles=%d entries=%llu depth=%d two=%d area=%d final=%llu side=%llu overflow=%u %.2f ms\n",
-            ntiles, (unsigned long long)nentries, depth, (int)two, (int)area, (unsigned long long)tot[0],
-            (unsigned long long)tot[1], nov, now_ms() - t_pt);
+            ntiles, (unsigned long long)nentries, depth, (int)two, (int)area, (unsigned long long)nfinal,
+            (unsigned long long)nsideT, nov, now_ms() - t_pt);
   if (nov > 0) {
     const double t_sp = g_debug ? now_ms() : 0.0;
     VLCK(split_and_process(e, d_desc, ovf2, nov, d_entries, two, area, o, depth));
@@ -615,6 +620,16 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
     gn.th = std::min<int64_t>(g.th * e->level_factor, g.y1 - g.y0);
     gn.rows = (int32_t)((g.y1 - g.y0 + gn.th - 1) / gn.th);
     if (gn.rows == g.rows && g.rows > 1) { gn.rows = 1; gn.th = g.y1 - g.y0; }
+    // Columns follow this level's own strip count (x-seams are harmless: the
+    // x-stitch joins pieces at any x), so its tiles stay near the target size.
+    {
+      const int64_t W = g.x1 - g.x0;
+      const double nt = std::max(1.0, (double)hc[0] / std::max(1, e->target));
+      int64_t cols = std::max<int64_t>(1, (int64_t)llround(nt / gn.rows));
+      cols = std::min(cols, W);
+      gn.tw = (W + cols - 1) / cols;
+      gn.cols = (int32_t)((W + gn.tw - 1) / gn.tw);
+    }
     g = gn;
     level++;
     // The level's input (next) is consumed before o.next is reused.

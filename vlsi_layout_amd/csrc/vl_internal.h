@@ -109,41 +109,47 @@ void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const 
 // Tile kernel: result strips of every tile.  Strips touching an internal seam
 // go to side[] with a tag byte; the rest go to out[].  Tiles with more boxes
 // than the kernel's capacity are listed in ovf[] (count in *ovf_n).
-// Every tile reserves its output ranges with one atomic add per stream and
-// records them in per-tile tables; a reorder pass then copies the strips into
-// tile order, so results do not depend on scheduling.
+// Workgroup g of a launch takes tiles g, g+G, g+2G, ... (no shared ticket
+// counter) and writes all strips of its tiles into its own region
+// [ (region_base+g)*region_cap, ... ) of the scratch stream, so no output
+// atomics are needed per tile.  Each tile's range is recorded in per-tile
+// tables; each strip carries a tag byte (0 = final, SEAM_* bits = seam piece).
+// A reorder pass then copies the strips in tile order into the final and side
+// streams, so results do not depend on scheduling.  A region that would
+// overflow sets ERR_OVERFLOW_CAP; maxuse tells the host the size to retry with.
 struct TileOut {
-  vl_box* out;                 // final strips (scratch, atomic ranges)
-  vl_box* side;                // seam strips (scratch, atomic ranges)
-  uint8_t* side_tag;           // SEAM_* bits per side strip
-  uint64_t out_cap, side_cap;
-  unsigned long long* cnt;     // [0] final strips reserved, [1] side strips reserved
-  uint64_t* tab_off;           // per tile: [2t] final range start, [2t+1] side range start
+  vl_box* strips;              // scratch stream
+  uint8_t* tag;                // tag per strip
+  uint64_t region_cap;         // strips per workgroup region
+  int region_base;             // first region index of this launch
+  unsigned long long* cnt;     // [0] strips written, [1] seam strips (one add per workgroup)
+  unsigned long long* maxuse;  // largest region fill (atomicMax, one per workgroup)
+  uint64_t* tab_off;           // per tile: range start
   uint32_t* tab_n;             // per tile: [t] final count; tab_n[ntab + t] side count
   int ntab;                    // table stride (number of descriptors)
-  const uint32_t* list;        // optional: tile = list[ticket] (null = ticket)
+  const uint32_t* list;        // optional: tile = list[i] (null = i)
   const uint32_t* list_n;      // optional: number of tiles, read on the device (null = ntiles)
-  uint32_t* ticket;            // zeroed before launch
   uint32_t* ovf;               // tiles above the kernel's capacity
   uint32_t* ovf_n;
   long long* area;             // area mode: accumulated union area (int64)
   uint32_t* err;
 };
-// Wave-per-tile kernel (capacity 63 or 127 boxes).
+// Wave-per-tile kernel (capacity 63 or 127 boxes), launched with exactly wg
+// workgroups (wg regions of the scratch stream).
 void launch_tile_wave(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
-                      const vl_box* entries, TileOut o, int max_wg, hipStream_t st);
+                      const vl_box* entries, TileOut o, int wg, hipStream_t st);
 int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode);
 // Workgroup-per-tile kernel (capacity 255 boxes) for tiles above the wave kernel's capacity.
 void launch_tile_block(bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
-                       const vl_box* entries, TileOut o, int max_wg, hipStream_t st);
+                       const vl_box* entries, TileOut o, int wg, hipStream_t st);
 int tile_block_max_wg(bool two_layers, bool area_mode);
 constexpr int TILE_BLOCK_NMAX = 255;
-// Copies the per-tile ranges into tile order: strips of tile t go to
-// out[out_base + dstF[t] ...] and side[side_base + dstS[t] ...].
+// Copies the per-tile ranges into tile order: final strips of tile t go to
+// out[out_base + dstF[t] ...], seam strips to side[side_base + dstS[t] ...].
 void launch_tile_reorder(int ntiles, const uint64_t* tab_off, const uint32_t* tab_n, int ntab,
-                         const uint64_t* dstF, const uint64_t* dstS, const vl_box* scrF,
-                         const vl_box* scrS, const uint8_t* scrTag, vl_box* out, uint64_t out_base,
-                         vl_box* side, uint8_t* side_tag, uint64_t side_base, hipStream_t st);
+                         const uint64_t* dstF, const uint64_t* dstS, const vl_box* scr,
+                         const uint8_t* scrTag, vl_box* out, uint64_t out_base, vl_box* side,
+                         uint8_t* side_tag, uint64_t side_base, hipStream_t st);
 void launch_gather_desc(const TileDesc* desc, const uint32_t* idx, int n, TileDesc* out, hipStream_t st);
 // Exclusive scan of u32 values into u64 offsets.
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, uint64_t* total, void* tmp,

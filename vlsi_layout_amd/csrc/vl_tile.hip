@@ -57,29 +57,69 @@ __device__ __forceinline__ vl_box xform(vl_box b, const Xform& x) {
   return b;
 }
 
-__global__ void k_bin_count(const vl_box* __restrict__ b, int64_t n, Xform xf, Grid g, uint32_t* cnt) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    vl_box v = xform(b[i], xf);
-    int c1, c2, r1, r2;
-    if (!tile_range(v, g, c1, c2, r1, r2)) continue;
-    for (int r = r1; r <= r2; r++)
-      for (int c = c1; c <= c2; c++) atomicAdd(&cnt[(int64_t)r * g.cols + c], 1u);
+// Neighbouring boxes usually fall into the same tile, so the per-tile atomics
+// are aggregated per wave: the lanes whose first tile matches the leader's
+// share one atomic (ballot over the tile index), the leader broadcasts the old
+// value and every lane adds its rank among the peers.  The remaining tiles of
+// boxes spanning several tiles use plain atomics.
+__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* ctr, int32_t t, bool valid) {
+  uint32_t mine = 0;
+  uint64_t act = __ballot(valid);
+  const uint64_t lt = lanemask_lt();
+  while (act) {
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int32_t tl = __builtin_amdgcn_readlane(t, leader);
+    const uint64_t peers = __ballot(valid && t == tl);
+    uint32_t old = 0;
+    if (__lane_id() == leader) old = atomicAdd(&ctr[tl], (uint32_t)__popcll(peers));
+    old = (uint32_t)__builtin_amdgcn_readlane((int)old, leader);
+    if (valid && t == tl) mine = old + (uint32_t)__popcll(peers & lt);
+    act &= ~peers;
+  }
+  return mine;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const vl_box* __restrict__ b, int64_t n, Xform xf, Grid g,
+                                                   uint32_t* cnt) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const int64_t i = i0 + threadIdx.x;
+    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
+    bool valid = false;
+    if (i < n) valid = tile_range(xform(b[i], xf), g, c1, c2, r1, r2);
+    (void)wave_aggregated_add(cnt, valid ? r1 * g.cols + c1 : 0, valid);
+    if (valid && (c2 > c1 || r2 > r1))
+      for (int r = r1; r <= r2; r++)
+        for (int c = c1; c <= c2; c++)
+          if (r != r1 || c != c1) atomicAdd(&cnt[(int64_t)r * g.cols + c], 1u);
   }
 }
 
-__global__ void k_bin_scatter(const vl_box* __restrict__ b, int64_t n, Xform xf, Grid g,
-                              const uint64_t* __restrict__ off, const uint32_t* __restrict__ base_add,
-                              uint32_t* fill, vl_box* __restrict__ entries) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    vl_box v = xform(b[i], xf);
-    int c1, c2, r1, r2;
-    if (!tile_range(v, g, c1, c2, r1, r2)) continue;
-    for (int r = r1; r <= r2; r++)
-      for (int c = c1; c <= c2; c++) {
-        int64_t t = (int64_t)r * g.cols + c;
-        uint64_t pos = off[t] + (base_add ? base_add[t] : 0u) + atomicAdd(&fill[t], 1u);
-        entries[pos] = v;
-      }
+__global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ b, int64_t n, Xform xf, Grid g,
+                                                     const uint64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ base_add, uint32_t* fill,
+                                                     vl_box* __restrict__ entries) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const int64_t i = i0 + threadIdx.x;
+    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
+    bool valid = false;
+    vl_box v{0, 0, 0, 0};
+    if (i < n) {
+      v = xform(b[i], xf);
+      valid = tile_range(v, g, c1, c2, r1, r2);
+    }
+    const int32_t t0 = valid ? r1 * g.cols + c1 : 0;
+    const uint32_t slot = wave_aggregated_add(fill, t0, valid);
+    if (!valid) continue;
+    entries[off[t0] + (base_add ? base_add[t0] : 0u) + slot] = v;
+    if (c2 > c1 || r2 > r1)
+      for (int r = r1; r <= r2; r++)
+        for (int c = c1; c <= c2; c++) {
+          if (r == r1 && c == c1) continue;
+          const int64_t t = (int64_t)r * g.cols + c;
+          entries[off[t] + (base_add ? base_add[t] : 0u) + atomicAdd(&fill[t], 1u)] = v;
+        }
   }
 }
 
@@ -186,6 +226,13 @@ __device__ __forceinline__ int run_end(const uint64_t* col, int W, int a) {
   return w * 64 + __ffsll((unsigned long long)m) - 1;
 }
 
+// Value of v in lane l (l uniform across the wave).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t wave_excl_u32(uint32_t v, uint32_t* tot) {
   uint32_t inc = wave_incl_scan(v);
   *tot = __shfl(inc, WAVE - 1, WAVE);
@@ -210,12 +257,10 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
   const int lane = threadIdx.x;
   const uint64_t lt = lanemask_lt();
   const int nt = o.list_n ? (int)*o.list_n : ntiles;
-  for (;;) {
-    uint32_t tk = 0;
-    if (lane == 0) tk = atomicAdd(o.ticket, 1u);
-    tk = __shfl(tk, 0, WAVE);
-    if ((int)tk >= nt) break;
-    const uint32_t ti = o.list ? o.list[tk] : tk;
+  const uint64_t rbase = (uint64_t)(o.region_base + blockIdx.x) * o.region_cap;
+  uint64_t used = 0, side_total = 0;
+  for (int tk = blockIdx.x; tk < nt; tk += gridDim.x) {
+    const uint32_t ti = o.list ? o.list[tk] : (uint32_t)tk;
     const TileDesc d = desc[ti];
     const int nA = d.nA;
     const int n = d.nA + d.nB;
@@ -367,7 +412,9 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       __syncthreads();
       continue;
     }
-    // 3. start / end masks at every slab boundary k = 0..ncol
+    // 3. start / end masks at every slab boundary k = 0..ncol; every strip
+    //    ends exactly once, so the popcount of the end masks is the strip count
+    uint32_t myT = 0;
     for (int k = lane; k <= ncol; k += WAVE) {
       const uint64_t* cur = (k < ncol) ? MA + k * W : nullptr;
       const uint64_t* prv = (k > 0) ? MA + (k - 1) * W : nullptr;
@@ -378,79 +425,89 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       for (int w = 0; w < W; w++) {
         MX[k * W + w] = S[w];
         ME[k * W + w] = E[w];
+        myT += (uint32_t)__popcll(E[w]);
       }
     }
+    const uint32_t T = wave_sum(myT);
     __syncthreads();
-    // 4. pair starts and ends: lane owns rows 64*w + lane
+    if (T == 0) {
+      if (lane == 0) {
+        o.tab_n[ti] = 0;
+        o.tab_n[o.ntab + ti] = 0;
+      }
+      continue;
+    }
+    const uint64_t base = rbase + used;
+    const bool ok = used + T <= o.region_cap;
+    used += T;
+    if (lane == 0) o.tab_off[ti] = base;
+    // 4. pair starts and ends.  Lane owns rows 64*w + lane; boundaries are
+    //    visited in order, 64 per chunk (one per lane, read into registers),
+    //    skipping boundaries with no start or end.
     const uint32_t flags = d.flags;
-    uint32_t cF = 0, cS = 0;
-    uint64_t baseF = 0, baseS = 0;
-    bool okF = true, okS = true;
-#pragma unroll 1
-    for (int pass = 0; pass < 2; pass++) {
-      int oj[NW], ob[NW];
-      uint32_t rF = 0, rS = 0;
+    int oj[NW], ob[NW];
 #pragma unroll
-      for (int w = 0; w < NW; w++) { oj[w] = 0; ob[w] = 0; }
-      for (int k = 0; k <= ncol; k++) {
+    for (int w = 0; w < NW; w++) { oj[w] = 0; ob[w] = 0; }
+    uint32_t r = 0, nside = 0;
+    for (int k0 = 0; k0 <= ncol; k0 += WAVE) {
+      const int kk = k0 + lane;
+      uint64_t ev[NW], sv[NW];
+      bool any = false;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        ev[w] = 0;
+        sv[w] = 0;
+        if (w < W && kk <= ncol) {
+          ev[w] = ME[kk * W + w];
+          sv[w] = MX[kk * W + w];
+        }
+        any |= (ev[w] | sv[w]) != 0;
+      }
+      uint64_t act = __ballot(any);
+      while (act) {
+        const int l = __ffsll((unsigned long long)act) - 1;
+        act &= act - 1;
+        const int k = k0 + l;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
           if (w >= W) break;
-          const uint64_t e = ME[k * W + w], s = MX[k * W + w];
+          const uint64_t e = readlane64(ev[w], l), s = readlane64(sv[w], l);
           if ((e | s) == 0) continue;
           const int a = w * 64 + lane;
           const bool endm = (e >> lane) & 1ull;
-          bool side = false;
+          uint32_t tg = 0;
           if (endm)
-            side = (a == 0 && (flags & SEAM_B)) || (ob[w] == nrows && (flags & SEAM_T)) ||
-                   (oj[w] == 0 && (flags & SEAM_L)) || (k == ncol && (flags & SEAM_R));
-          const uint64_t bF = __ballot(endm && !side), bS = __ballot(endm && side);
-          if (pass == 1 && endm) {
+            tg = ((a == 0 && (flags & SEAM_B)) ? SEAM_B : 0u) | ((ob[w] == nrows && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                 ((oj[w] == 0 && (flags & SEAM_L)) ? SEAM_L : 0u) | ((k == ncol && (flags & SEAM_R)) ? SEAM_R : 0u);
+          const uint64_t be = __ballot(endm), bs = __ballot(endm && tg != 0);
+          if (endm && ok) {
+            const uint64_t p = base + r + (uint32_t)__popcll(be & lt);
             vl_box st;
             st.x1 = ux[oj[w]]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[ob[w]];
-            if (side) {
-              const uint64_t p = baseS + rS + (uint32_t)__popcll(bS & lt);
-              if (okS) {
-                o.side[p] = st;
-                o.side_tag[p] = (uint8_t)(((a == 0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                                          ((ob[w] == nrows && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                                          ((oj[w] == 0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                                          ((k == ncol && (flags & SEAM_R)) ? SEAM_R : 0u));
-              }
-            } else {
-              const uint64_t p = baseF + rF + (uint32_t)__popcll(bF & lt);
-              if (okF) o.out[p] = st;
-            }
+            o.strips[p] = st;
+            o.tag[p] = (uint8_t)tg;
           }
-          rF += (uint32_t)__popcll(bF);
-          rS += (uint32_t)__popcll(bS);
+          r += (uint32_t)__popcll(be);
+          nside += (uint32_t)__popcll(bs);
           if ((s >> lane) & 1ull) {
             oj[w] = k;
             ob[w] = run_end(MA + k * W, W, a);
           }
         }
       }
-      if (pass == 0) {
-        cF = rF;
-        cS = rS;
-        unsigned long long b0 = 0, b1 = 0;
-        if (lane == 0) {
-          b0 = cF ? atomicAdd(&o.cnt[0], (unsigned long long)cF) : 0ull;
-          b1 = cS ? atomicAdd(&o.cnt[1], (unsigned long long)cS) : 0ull;
-          o.tab_off[2 * ti] = b0;
-          o.tab_off[2 * ti + 1] = b1;
-          o.tab_n[ti] = cF;
-          o.tab_n[o.ntab + ti] = cS;
-          if (b0 + cF > o.out_cap || b1 + cS > o.side_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
-        }
-        baseF = __shfl(b0, 0, WAVE);
-        baseS = __shfl(b1, 0, WAVE);
-        okF = baseF + cF <= o.out_cap;
-        okS = baseS + cS <= o.side_cap;
-        if (cF + cS == 0) break;
-      }
+    }
+    side_total += nside;
+    if (lane == 0) {
+      o.tab_n[ti] = T - nside;
+      o.tab_n[o.ntab + ti] = nside;
     }
     __syncthreads();
+  }
+  if (lane == 0 && !AREA && used) {
+    atomicAdd(&o.cnt[0], (unsigned long long)used);
+    if (side_total) atomicAdd(&o.cnt[1], (unsigned long long)side_total);
+    atomicMax(o.maxuse, (unsigned long long)used);
+    if (used > o.region_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
   }
 }
 
@@ -508,18 +565,13 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
   __shared__ int wsI[NT / WAVE];
   __shared__ uint64_t wsU[NT / WAVE];
   __shared__ long long wsA[NT / WAVE];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_offF, s_offS;
-  __shared__ int s_okF, s_okS;
 
   const int tid = threadIdx.x;
   const int nt = o.list_n ? (int)*o.list_n : ntiles;
-  for (;;) {
-    if (tid == 0) s_tile = atomicAdd(o.ticket, 1u);
-    __syncthreads();
-    const uint32_t tk = s_tile;
-    if ((int)tk >= nt) break;
-    const uint32_t ti = o.list ? o.list[tk] : tk;
+  const uint64_t rbase = (uint64_t)(o.region_base + blockIdx.x) * o.region_cap;
+  uint64_t used = 0, side_total = 0;   // uniform across the workgroup
+  for (int tk = blockIdx.x; tk < nt; tk += gridDim.x) {
+    const uint32_t ti = o.list ? o.list[tk] : (uint32_t)tk;
     const TileDesc d = desc[ti];
     const int nA = d.nA;
     const int n = d.nA + d.nB;
@@ -676,24 +728,20 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
       continue;
     }
 
+    // one range per tile in the workgroup's region: final strips first, then seam strips
+    const uint64_t b0 = rbase + used;
+    const bool ok = used + F + S <= o.region_cap;
+    used += F + S;
+    side_total += S;
     if (tid == 0) {
-      const unsigned long long b0 = F ? atomicAdd(&o.cnt[0], (unsigned long long)F) : 0ull;
-      const unsigned long long b1 = S ? atomicAdd(&o.cnt[1], (unsigned long long)S) : 0ull;
-      o.tab_off[2 * ti] = b0;
-      o.tab_off[2 * ti + 1] = b1;
+      o.tab_off[ti] = b0;
       o.tab_n[ti] = (uint32_t)F;
       o.tab_n[o.ntab + ti] = (uint32_t)S;
-      s_offF = b0;
-      s_offS = b1;
-      s_okF = (b0 + F) <= o.out_cap;
-      s_okS = (b1 + S) <= o.side_cap;
-      if (!s_okF || !s_okS) atomicOr(o.err, ERR_OVERFLOW_CAP);
     }
-    __syncthreads();
 
     if ((F + S) > 0) {
-      const uint64_t offF = s_offF, offS = s_offS;
-      const bool okF = s_okF, okS = s_okS;
+      const uint64_t offF = b0, offS = b0 + F;
+      const bool okF = ok, okS = ok;
       const uint32_t flags = d.flags;
       for (int j = tid; j < ncol; j += NT) {
         const uint64_t* cur = MA + j * W;
@@ -713,16 +761,22 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
           vl_box s;
           s.x1 = ux[j]; s.y1 = uy[a]; s.x2 = ux[k]; s.y2 = uy[b];
           if (tag) {
-            if (okS) { o.side[si] = s; o.side_tag[si] = (uint8_t)tag; }
+            if (okS) { o.strips[si] = s; o.tag[si] = (uint8_t)tag; }
             si++;
           } else {
-            if (okF) o.out[fi] = s;
+            if (okF) { o.strips[fi] = s; o.tag[fi] = 0; }
             fi++;
           }
         });
       }
     }
     __syncthreads();
+  }
+  if (tid == 0 && !AREA && used) {
+    atomicAdd(&o.cnt[0], (unsigned long long)used);
+    if (side_total) atomicAdd(&o.cnt[1], (unsigned long long)side_total);
+    atomicMax(o.maxuse, (unsigned long long)used);
+    if (used > o.region_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
   }
 }
 
@@ -732,23 +786,35 @@ constexpr int REORDER_TILES = 16;
 __global__ __launch_bounds__(256) void k_reorder(int ntiles, const uint64_t* __restrict__ tab_off,
                                                  const uint32_t* __restrict__ tab_n, int ntab,
                                                  const uint64_t* __restrict__ dstF, const uint64_t* __restrict__ dstS,
-                                                 const vl_box* __restrict__ scrF, const vl_box* __restrict__ scrS,
-                                                 const uint8_t* __restrict__ scrTag, vl_box* __restrict__ out,
-                                                 uint64_t out_base, vl_box* __restrict__ side,
-                                                 uint8_t* __restrict__ side_tag, uint64_t side_base) {
+                                                 const vl_box* __restrict__ scr, const uint8_t* __restrict__ scrTag,
+                                                 vl_box* __restrict__ out, uint64_t out_base,
+                                                 vl_box* __restrict__ side, uint8_t* __restrict__ side_tag,
+                                                 uint64_t side_base) {
+  __shared__ int ws[256 / WAVE];
   const int t0 = blockIdx.x * REORDER_TILES;
   for (int t = t0; t < t0 + REORDER_TILES && t < ntiles; t++) {
-    const uint32_t nf = tab_n[t], ns = tab_n[ntab + t];
-    if (nf) {
-      const uint64_t s = tab_off[2 * t], dd = out_base + dstF[t];
-      for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x) out[dd + i] = scrF[s + i];
-    }
-    if (ns) {
-      const uint64_t s = tab_off[2 * t + 1], dd = side_base + dstS[t];
-      for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
-        side[dd + i] = scrS[s + i];
-        side_tag[dd + i] = scrTag[s + i];
+    const uint32_t T = tab_n[t] + tab_n[ntab + t];
+    if (T == 0) continue;
+    const uint64_t src = tab_off[t];
+    uint64_t dF = out_base + dstF[t], dS = side_base + dstS[t];
+    for (uint32_t c0 = 0; c0 < T; c0 += 256) {
+      const uint32_t i = c0 + threadIdx.x;
+      const bool valid = i < T;
+      const uint8_t tg = valid ? scrTag[src + i] : 0;
+      const bool isF = valid && tg == 0;
+      vl_box b;
+      if (valid) b = scr[src + i];
+      int totF, totS;
+      const int pf = block_excl_flag_scan<256>(isF, ws, &totF);
+      const int ps = block_excl_flag_scan<256>(valid && !isF, ws, &totS);
+      if (isF) {
+        out[dF + pf] = b;
+      } else if (valid) {
+        side[dS + ps] = b;
+        side_tag[dS + ps] = tg;
       }
+      dF += totF;
+      dS += totS;
     }
   }
 }
@@ -840,16 +906,14 @@ int occupancy_wg(KF kern, int threads) {
 }
 
 template <int NMAX, bool TWO, bool AREA>
-void launch_w(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
-  int g = ntiles < max_wg ? ntiles : max_wg;
-  if (g < 1) g = 1;
+void launch_w(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int wg, hipStream_t st) {
+  const int g = wg < 1 ? 1 : wg;
   hipLaunchKernelGGL((k_tile_w<NMAX, TWO, AREA>), dim3(g), dim3(WAVE), 0, st, desc, ntiles, entries, o);
 }
 
 template <bool TWO, bool AREA>
-void launch_b(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
-  int g = ntiles < max_wg ? ntiles : max_wg;
-  if (g < 1) g = 1;
+void launch_b(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int wg, hipStream_t st) {
+  const int g = wg < 1 ? 1 : wg;
   hipLaunchKernelGGL((k_tile_b<TILE_BLOCK_NMAX, 256, TWO, AREA>), dim3(g), dim3(256), 0, st, desc, ntiles,
                      entries, o);
 }
@@ -874,17 +938,17 @@ void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const 
 #define VL_W_DISPATCH(NM)                                                                  \
   if (nmax == NM) {                                                                        \
     if (area_mode) {                                                                       \
-      if (two_layers) launch_w<NM, true, true>(desc, ntiles, entries, o, max_wg, st);      \
-      else launch_w<NM, false, true>(desc, ntiles, entries, o, max_wg, st);                \
+      if (two_layers) launch_w<NM, true, true>(desc, ntiles, entries, o, wg, st);      \
+      else launch_w<NM, false, true>(desc, ntiles, entries, o, wg, st);                \
     } else {                                                                               \
-      if (two_layers) launch_w<NM, true, false>(desc, ntiles, entries, o, max_wg, st);     \
-      else launch_w<NM, false, false>(desc, ntiles, entries, o, max_wg, st);               \
+      if (two_layers) launch_w<NM, true, false>(desc, ntiles, entries, o, wg, st);     \
+      else launch_w<NM, false, false>(desc, ntiles, entries, o, wg, st);               \
     }                                                                                      \
     return;                                                                                \
   }
 
 void launch_tile_wave(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
-                      const vl_box* entries, TileOut o, int max_wg, hipStream_t st) {
+                      const vl_box* entries, TileOut o, int wg, hipStream_t st) {
   VL_W_DISPATCH(63)
   VL_W_DISPATCH(127)
 }
@@ -911,13 +975,13 @@ int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode) {
 }
 
 void launch_tile_block(bool two_layers, bool area_mode, const TileDesc* desc, int ntiles, const vl_box* entries,
-                       TileOut o, int max_wg, hipStream_t st) {
+                       TileOut o, int wg, hipStream_t st) {
   if (area_mode) {
-    if (two_layers) launch_b<true, true>(desc, ntiles, entries, o, max_wg, st);
-    else launch_b<false, true>(desc, ntiles, entries, o, max_wg, st);
+    if (two_layers) launch_b<true, true>(desc, ntiles, entries, o, wg, st);
+    else launch_b<false, true>(desc, ntiles, entries, o, wg, st);
   } else {
-    if (two_layers) launch_b<true, false>(desc, ntiles, entries, o, max_wg, st);
-    else launch_b<false, false>(desc, ntiles, entries, o, max_wg, st);
+    if (two_layers) launch_b<true, false>(desc, ntiles, entries, o, wg, st);
+    else launch_b<false, false>(desc, ntiles, entries, o, wg, st);
   }
 }
 
@@ -935,13 +999,12 @@ int tile_block_max_wg(bool two_layers, bool area_mode) {
 }
 
 void launch_tile_reorder(int ntiles, const uint64_t* tab_off, const uint32_t* tab_n, int ntab, const uint64_t* dstF,
-                         const uint64_t* dstS, const vl_box* scrF, const vl_box* scrS, const uint8_t* scrTag,
-                         vl_box* out, uint64_t out_base, vl_box* side, uint8_t* side_tag, uint64_t side_base,
-                         hipStream_t st) {
+                         const uint64_t* dstS, const vl_box* scr, const uint8_t* scrTag, vl_box* out,
+                         uint64_t out_base, vl_box* side, uint8_t* side_tag, uint64_t side_base, hipStream_t st) {
   if (ntiles <= 0) return;
   const int g = (ntiles + REORDER_TILES - 1) / REORDER_TILES;
-  hipLaunchKernelGGL(k_reorder, dim3(g), dim3(256), 0, st, ntiles, tab_off, tab_n, ntab, dstF, dstS, scrF, scrS,
-                     scrTag, out, out_base, side, side_tag, side_base);
+  hipLaunchKernelGGL(k_reorder, dim3(g), dim3(256), 0, st, ntiles, tab_off, tab_n, ntab, dstF, dstS, scr, scrTag,
+                     out, out_base, side, side_tag, side_base);
 }
 
 void launch_gather_desc(const TileDesc* desc, const uint32_t* idx, int n, TileDesc* out, hipStream_t st) {
