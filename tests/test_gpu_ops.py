@@ -250,3 +250,44 @@ def test_sorts(eng):
     b = rand_boxes(rng, 200000, 1 << 20, 2000, off=-(1 << 19))
     assert np.array_equal(V.thrust_sort_boxes_direct(b, eng), O.morton_sort_direct(b))
     assert np.array_equal(V.thrust_sort_boxes_indirect(b, eng), O.morton_sort_indirect(b))
+
+
+# SIZE(-d) runs on the per-tile erosion kernel when the grid's tiles are at
+# least 4d wide (engine parameter erode_kernel=1, the default) and otherwise as
+# complement -> grow -> difference; both must give the oracle's erosion.
+SIZE_CASES = [
+    ("random_small", lambda rng: rand_boxes(rng, 50, 200, 40), (-1, -2, -7)),
+    ("dense_grid5", lambda rng: np.maximum(rand_boxes(rng, 1500, 400, 12) // 5 * 5,
+                                           0).astype(np.int32), (-1, -5)),
+    ("wires", lambda rng: np.concatenate([
+        rand_boxes(rng, 2000, 60000, 2500, off=-30000),
+        np.asarray([[-40000, y, 40000, y + 9] for y in range(-30000, 30000, 3000)] +
+                   [[x, -40000, x + 9, 40000] for x in range(-30000, 30000, 3000)], np.int32)]), (-1, -4, -60)),
+    ("stdcell", lambda rng: V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 2, 15000), (-1, -3)),
+]
+
+
+@pytest.mark.parametrize("erode_kernel", [1, 0])
+@pytest.mark.parametrize("name,make,ds", SIZE_CASES, ids=[c[0] for c in SIZE_CASES])
+def test_size_erosion_paths(eng, erode_kernel, name, make, ds):
+    rng = np.random.default_rng(len(name))
+    a = make(rng)
+    if name == "dense_grid5":   # keep boxes valid after snapping to the 5-DBU grid
+        a[:, 2] = np.maximum(a[:, 2], a[:, 0] + 5)
+        a[:, 3] = np.maximum(a[:, 3], a[:, 1] + 5)
+    old = eng.get("erode_kernel")
+    eng.set("erode_kernel", erode_kernel)
+    try:
+        for d in ds:
+            check(eng, "size", a, args=d)
+    finally:
+        eng.set("erode_kernel", old)
+
+
+def test_size_erosion_halo_edges(eng):
+    # boxes that only touch a tile through its halo, erosions that remove
+    # everything, and d large enough to select the complement path
+    a = np.array([[0, 0, 100, 100], [100, 0, 103, 100], [103, 40, 400, 43], [0, 100, 100, 1000],
+                  [500, 500, 501, 900], [-50, -50, 0, 0]], np.int32)
+    for d in (-1, -2, -3, -20, -49, -50, -51, -200):
+        check(eng, "size", a, args=d)

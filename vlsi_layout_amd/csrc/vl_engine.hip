@@ -121,6 +121,7 @@ struct vl_engine {
   int nmax = 63;                 // wave-kernel tile capacity (63 or 127 boxes)
   int target = 42;               // mean boxes per tile the grid aims for
   int level_factor = 16;
+  int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
   double last_ms = 0.0;
   // kernel statistics (collected while prof is set)
@@ -151,8 +152,12 @@ struct Scratch {
   std::vector<std::pair<void*, size_t>> v;
   bool oom = false;
   explicit Scratch(vl_engine* en) : e(en) {}
-  ~Scratch() {
+  ~Scratch() { release(); }
+  // Returns every buffer to the pool (work already queued on the engine's
+  // stream still sees them: later users are ordered behind it on the stream).
+  void release() {
     for (auto& pr : v) e->pool.put(pr.first, pr.second);
+    v.clear();
   }
   template <class T>
   T* get(size_t n) {
@@ -285,11 +290,11 @@ struct Outs {
 bool two_layer(int top) { return top == TOP_AND || top == TOP_NOT || top == TOP_XOR; }
 
 int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, uint64_t nentries,
-                  bool two, bool area, Outs& o, int depth);
+                  bool two, bool area, int32_t erode, Outs& o, int depth);
 
 // Split overflow tiles into x-sub-columns and process them.
 int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
-                      const vl_box* d_entries, bool two, bool area, Outs& o, int depth) {
+                      const vl_box* d_entries, bool two, bool area, int32_t erode, Outs& o, int depth) {
   if (depth > 24) return fail(VL_EINTERNAL, "tile splitting did not converge");
   std::vector<uint32_t> ovf(novf);
   std::vector<TileDesc> parents(novf);
@@ -317,6 +322,8 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
     const int64_t width = (int64_t)p.x1 - p.x0;
     int64_t k = std::max<int64_t>(2, (n + tgt - 1) / tgt);
     k = std::min(k, width);
+    // erosion windows overlap by twice the halo: keep sub-columns at least that wide
+    if (erode > 0) k = std::min<int64_t>(k, width / (2 * (int64_t)erode));
     if (k < 2)
       return fail(VL_EINTERNAL, "a 1-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
                   (long long)n, TILE_BLOCK_NMAX);
@@ -348,18 +355,19 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
   HIPCK(hipMemcpyAsync(d_first, first.data(), (novf + 1) * sizeof(int32_t), hipMemcpyHostToDevice, e->st));
   uint32_t *cA = cnt, *cB = cnt + ns, *fA = cnt + 2 * ns, *fB = cnt + 3 * ns;
   launch_fill_u32(cnt, 0u, 4 * (size_t)ns, e->st);
-  launch_split_count(d_par, (int)novf, d_first, d_sub, cA, cB, d_entries, e->st);
+  launch_split_count(d_par, (int)novf, d_first, d_sub, cA, cB, d_entries, erode, e->st);
   scan_pair_counts(cA, cB, off, ns, off + ns, stmp, e->st);
   uint64_t total = 0;
   VLCK(readback(e, &total, off + ns, sizeof(total)));
   vl_box* sub_entries = s.get<vl_box>(total + 1);
   if (!sub_entries) return fail(VL_ENOMEM, "scratch");
-  launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, e->st);
-  return process_tiles(e, d_sub, ns, sub_entries, total, two, area, o, depth + 1);
+  launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, erode,
+                       e->st);
+  return process_tiles(e, d_sub, ns, sub_entries, total, two, area, erode, o, depth + 1);
 }
 
 int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, uint64_t nentries,
-                  bool two, bool area, Outs& o, int depth) {
+                  bool two, bool area, int32_t erode, Outs& o, int depth) {
   if (ntiles <= 0) return VL_OK;
   Scratch s(e);
   uint32_t* small = s.get<uint32_t>(4);   // [0] wave overflow count [1] block overflow count
@@ -370,8 +378,9 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   uint64_t* tab_off = area ? nullptr : s.get<uint64_t>((size_t)ntiles);
   uint32_t* tab_n = area ? nullptr : s.get<uint32_t>(2 * (size_t)ntiles);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
-  int max_w = tile_wave_max_wg(e->nmax, two, area);
-  int max_b = tile_block_max_wg(two, area);
+  const int nmax_w = erode > 0 ? 63 : e->nmax;
+  int max_w = tile_wave_max_wg(nmax_w, two, area, erode > 0);
+  int max_b = tile_block_max_wg(two, area, erode > 0);
   if (e->max_wg_cap > 0) max_w = std::min(max_w, e->max_wg_cap);
   // Exact grids: workgroup g takes tiles g, g+G, ... and owns region g of the
   // scratch stream (wave kernel: regions 0..Gw-1, block kernel: Gw..Gw+Gb-1).
@@ -400,6 +409,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     to.tag = scrT.p;
     to.region_cap = area ? 0 : region;
     to.region_base = 0;
+    to.erode = erode;
     to.cnt = cnt;
     to.maxuse = cnt + 2;
     to.tab_off = tab_off;
@@ -412,7 +422,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     to.area = d_area;
     to.err = e->d_err;
     if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
-    launch_tile_wave(e->nmax, two, area, d_desc, ntiles, d_entries, to, Gw, e->st);
+    launch_tile_wave(nmax_w, two, area, erode > 0, d_desc, ntiles, d_entries, to, Gw, e->st);
     // tiles above the wave kernel's capacity (count known only on the device)
     TileOut tb = to;
     tb.region_base = Gw;
@@ -420,7 +430,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     tb.list_n = small;
     tb.ovf = ovf2;
     tb.ovf_n = small + 1;
-    launch_tile_block(two, area, d_desc, ntiles, d_entries, tb, Gb, e->st);
+    launch_tile_block(two, area, erode > 0, d_desc, ntiles, d_entries, tb, Gb, e->st);
     HIPCK(hipGetLastError());
     if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
     struct {
@@ -480,52 +490,59 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
             (unsigned long long)nsideT, nov, now_ms() - t_pt);
   if (nov > 0) {
     const double t_sp = g_debug ? now_ms() : 0.0;
-    VLCK(split_and_process(e, d_desc, ovf2, nov, d_entries, two, area, o, depth));
+    VLCK(split_and_process(e, d_desc, ovf2, nov, d_entries, two, area, erode, o, depth));
     if (g_debug) fprintf(stderr, "[vl]  split %u tiles (depth %d): %.2f ms\n", nov, depth, now_ms() - t_sp);
   }
   return VL_OK;
 }
 
-// Bins up to two inputs into grid g and processes the tiles.
-int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b,
-                    int64_t nb, bool area, Outs& o) {
+// Bins up to two inputs into grid g and processes the tiles.  Layer A boxes
+// are binned by their extent grown by xr and stored grown by xs; erode > 0
+// selects the erosion tile kernel (xr is then the erosion halo, xs zero).
+int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, Xform xs, Xform xr,
+                    const vl_box* b, int64_t nb, bool area, int32_t erode, Outs& o) {
   const int64_t nt = (int64_t)g.cols * g.rows;
+  const int64_t n = na + nb;
   Scratch s(e);
-  uint32_t* cnt = s.get<uint32_t>(4 * (size_t)nt);
-  uint64_t* off = s.get<uint64_t>(nt + 1);
+  uint32_t* cnt = s.get<uint32_t>((size_t)n + 1);
+  uint64_t* pos = s.get<uint64_t>((size_t)n + 1);
+  uint64_t* start = s.get<uint64_t>(2 * (size_t)nt + 1);
   TileDesc* desc = s.get<TileDesc>(nt);
-  void* stmp = s.get<char>(scan_tmp_bytes(nt));
+  void* stmp = s.get<char>(scan_tmp_bytes((size_t)n + 1));
   if (s.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
-  uint32_t *cA = cnt, *cB = cnt + nt, *fA = cnt + 2 * nt, *fB = cnt + 3 * nt;
-  launch_fill_u32(cnt, 0u, 4 * (size_t)nt, e->st);
-  if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
-  launch_bin_count(a, na, xa, g, cA, e->st);
-  launch_bin_count(b, nb, Xform{0, 0, 0, 0}, g, cB, e->st);
-  if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
-  scan_pair_counts(cA, cB, off, nt, off + nt, stmp, e->st);
+  launch_bin_span(a, na, xr, b, nb, g, cnt, e->st);
+  scan_u32_to_u64(cnt, pos, (size_t)n, pos + n, stmp, e->st);
   uint64_t total = 0;
-  VLCK(readback(e, &total, off + nt, sizeof(total)));
-  float ms_count = 0.f;
-  if (e->prof) HIPCK(hipEventElapsedTime(&ms_count, e->kev0, e->kev1));
+  VLCK(readback(e, &total, pos + n, sizeof(total)));
+  Scratch s2(e);
+  uint64_t* keys = s2.get<uint64_t>(total + 1);
+  uint32_t* vals = s2.get<uint32_t>(total + 1);
+  uint64_t* keys2 = s2.get<uint64_t>(total + 1);
+  uint32_t* vals2 = s2.get<uint32_t>(total + 1);
+  void* rtmp = s2.get<char>(radix_plan((int64_t)total).tmp_bytes);
   vl_box* entries = s.get<vl_box>(total + 1);
-  if (!entries) return fail(VL_ENOMEM, "entries (%llu)", (unsigned long long)total);
+  if (s2.oom || !entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
   if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
-  launch_bin_scatter(a, na, xa, g, off, nullptr, fA, entries, e->st);
-  launch_bin_scatter(b, nb, Xform{0, 0, 0, 0}, g, off, cA, fB, entries, e->st);
+  launch_bin_emit(a, na, xr, b, nb, g, pos, keys, vals, e->st);
+  radix_sort_pairs(keys, vals, keys2, vals2, (int64_t)total, rtmp, e->st);
+  launch_bin_bounds(keys, total, 2 * nt, start, e->st);
+  launch_bin_gather(a, na, xs, b, vals, total, entries, e->st);
+  launch_build_desc(g, start, top, desc, e->st);
+  HIPCK(hipGetLastError());
   if (e->prof) {
     HIPCK(hipEventRecord(e->kev1, e->st));
     HIPCK(hipEventSynchronize(e->kev1));
-    float ms_scatter = 0.f;
-    HIPCK(hipEventElapsedTime(&ms_scatter, e->kev0, e->kev1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
     auto& k = e->kst[KS_BIN];
-    k.launches += 2;
-    k.ms += ms_count + ms_scatter;
-    // count: read 16 B per box; scatter: read 16 B per box, write 16 B per entry
-    k.bytes += 32.0 * (double)(na + nb) + 16.0 * (double)total;
+    k.launches += 1;
+    k.ms += ms;
+    // algorithmic bytes: read the boxes twice (span, emit), sort 12-byte pairs,
+    // gather 16 B per entry, write 16 B per entry
+    k.bytes += 32.0 * (double)n + 12.0 * 2.0 * (double)total + 32.0 * (double)total;
   }
-  launch_build_desc(g, off, cA, cB, top, desc, e->st);
-  HIPCK(hipGetLastError());
-  return process_tiles(e, desc, (int)nt, entries, total, two_layer(top) && nb > 0, area, o, 0);
+  s2.release();
+  return process_tiles(e, desc, (int)nt, entries, total, two_layer(top) && nb > 0, area, erode, o, 0);
 }
 
 // Moves result strips into layer c.
@@ -547,7 +564,7 @@ int set_empty(vl_engine* e, vl_layer* c) {
 
 // The whole operator pipeline.  area_out != null selects area mode.
 int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb,
-           vl_layer* c, long long* area_out) {
+           vl_layer* c, long long* area_out, int32_t erode = 0) {
   if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; }
   Box64 ma, mb;
   VLCK(mbr2(e, a, na, xa, b, nb, &ma, &mb));
@@ -577,7 +594,8 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
   Outs o(e);
   Grid g = make_grid(dom, na + nb, e->target);
   const double t_start = g_debug ? now_ms() : 0.0;
-  VLCK(bin_and_process(e, g, top, a, na, xa, b, nb, area, o));
+  const Xform xr = erode > 0 ? Xform{erode, erode, erode, erode} : xa;
+  VLCK(bin_and_process(e, g, top, a, na, xa, xr, b, nb, area, erode, o));
   if (g_debug) fprintf(stderr, "[vl] op %d na=%lld nb=%lld level0 %.2f ms\n", top, (long long)na, (long long)nb,
                        now_ms() - t_start);
   if (area) {
@@ -638,7 +656,8 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
     std::swap(in.cap, o.next.cap);
     std::swap(in.bytes, o.next.bytes);
     const double t_lv = g_debug ? now_ms() : 0.0;
-    VLCK(bin_and_process(e, g, TOP_UNION, in.p, (int64_t)hc[0], Xform{0, 0, 0, 0}, nullptr, 0, false, o));
+    VLCK(bin_and_process(e, g, TOP_UNION, in.p, (int64_t)hc[0], Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0,
+                         false, 0, o));
     if (g_debug) fprintf(stderr, "[vl] level %d: %llu strips, %.2f ms\n", level, (unsigned long long)hc[0],
                          now_ms() - t_lv);
   }
@@ -819,6 +838,8 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
     for (auto& s : e->kst) s = vl_engine::KStat();
   } else if (k == "max_wg") {
     e->max_wg_cap = (int)std::max<int64_t>(0, v);
+  } else if (k == "erode_kernel") {
+    e->erode_kernel = v != 0;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
   }
@@ -832,6 +853,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "tile_target") *v = e->target;
   else if (k == "level_factor") *v = e->level_factor;
   else if (k == "max_wg") *v = e->max_wg_cap;
+  else if (k == "erode_kernel") *v = e->erode_kernel;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
 }
@@ -989,11 +1011,16 @@ int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c) {
   if (ma.x1 - 1 < INT32_MIN || ma.y1 - 1 < INT32_MIN || ma.x2 + 1 > INT32_MAX || ma.y2 + 1 > INT32_MAX)
     return fail(VL_EINVAL, "SIZE frame leaves the int32 range");
   Scratch s(e);
+  const vl_box* pa = a->d;
+  if (c == a) VLCK(snapshot(e, a, s, &pa));
+  // Per-tile erosion when the tiles of the operator's grid are at least 4d
+  // wide and tall (each box then meets only a few tile windows).
+  const Grid g = make_grid(ma, a->n, e->target);
+  if (e->erode_kernel && g.tw >= 4 * (int64_t)r && g.th >= 4 * (int64_t)r)
+    return run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r);
   vl_box frame{(int32_t)(ma.x1 - 1), (int32_t)(ma.y1 - 1), (int32_t)(ma.x2 + 1), (int32_t)(ma.y2 + 1)};
   void* dframe = nullptr;
   VLCK(upload_tmp(e, s, &frame, sizeof(frame), &dframe));
-  const vl_box* pa = a->d;
-  if (c == a) VLCK(snapshot(e, a, s, &pa));
   vl_layer t1, t2;
   t1.e = t2.e = e;
   int rc = run_op(e, TOP_NOT, (const vl_box*)dframe, 1, Xform{0, 0, 0, 0}, pa, a->n, &t1, nullptr);

@@ -100,12 +100,19 @@ void launch_box_gather(const vl_box* in, const uint32_t* perm, vl_box* out, int6
 void launch_perm_to_index1(const uint32_t* perm, int64_t* idx, int64_t n, hipStream_t st);
 
 // vl_tile.hip
-// Count boxes per tile (cntA for layer 0, cntB for layer 1).
-void launch_bin_count(const vl_box* b, int64_t n, Xform xf, Grid g, uint32_t* cnt, hipStream_t st);
-void launch_bin_scatter(const vl_box* b, int64_t n, Xform xf, Grid g, const uint64_t* off,
-                        const uint32_t* base_add, uint32_t* fill, vl_box* entries, hipStream_t st);
-void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB,
-                       int32_t op, TileDesc* desc, hipStream_t st);
+// Binning by a stable sort of (tile, layer) keys.  Boxes of layer A are
+// indices 0..na-1, boxes of layer B na..na+nb-1; a layer-A box is binned by its
+// extent grown by xra (the tile range), and stored grown by xsa.
+void launch_bin_span(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
+                     hipStream_t st);
+void launch_bin_emit(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g,
+                     const uint64_t* pos, uint64_t* keys, uint32_t* vals, hipStream_t st);
+// start[k] = number of keys < k, for k = 0..nkeys.
+void launch_bin_bounds(const uint64_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st);
+void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, const uint32_t* vals,
+                       uint64_t total, vl_box* entries, hipStream_t st);
+// Tile descriptors from the key bounds (start has 2*ntiles+1 entries).
+void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st);
 // Tile kernel: result strips of every tile.  Strips touching an internal seam
 // go to side[] with a tag byte; the rest go to out[].  Tiles with more boxes
 // than the kernel's capacity are listed in ovf[] (count in *ovf_n).
@@ -122,6 +129,7 @@ struct TileOut {
   uint8_t* tag;                // tag per strip
   uint64_t region_cap;         // strips per workgroup region
   int region_base;             // first region index of this launch
+  int32_t erode;               // erosion kernels: erosion distance (tile windows grown by it)
   unsigned long long* cnt;     // [0] strips written, [1] seam strips (one add per workgroup)
   unsigned long long* maxuse;  // largest region fill (atomicMax, one per workgroup)
   uint64_t* tab_off;           // per tile: range start
@@ -134,15 +142,18 @@ struct TileOut {
   long long* area;             // area mode: accumulated union area (int64)
   uint32_t* err;
 };
-// Wave-per-tile kernel (capacity 63 or 127 boxes), launched with exactly wg
-// workgroups (wg regions of the scratch stream).
-void launch_tile_wave(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
+// Wave-per-tile kernel (capacity 63 or 127 boxes; erosion: 63), launched with
+// exactly wg workgroups (wg regions of the scratch stream).  Erosion mode
+// computes the erosion of layer A by the square of half-side o.erode inside
+// each tile from the boxes meeting the tile grown by o.erode.
+void launch_tile_wave(int nmax, bool two_layers, bool area_mode, bool erode, const TileDesc* desc, int ntiles,
                       const vl_box* entries, TileOut o, int wg, hipStream_t st);
-int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode);
-// Workgroup-per-tile kernel (capacity 255 boxes) for tiles above the wave kernel's capacity.
-void launch_tile_block(bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
+int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode, bool erode);
+// Workgroup-per-tile kernel (capacity 255 boxes; erosion: 127) for tiles above
+// the wave kernel's capacity.
+void launch_tile_block(bool two_layers, bool area_mode, bool erode, const TileDesc* desc, int ntiles,
                        const vl_box* entries, TileOut o, int wg, hipStream_t st);
-int tile_block_max_wg(bool two_layers, bool area_mode);
+int tile_block_max_wg(bool two_layers, bool area_mode, bool erode);
 constexpr int TILE_BLOCK_NMAX = 255;
 // Copies the per-tile ranges into tile order: final strips of tile t go to
 // out[out_base + dstF[t] ...], seam strips to side[side_base + dstS[t] ...].
@@ -156,13 +167,15 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, uint64_t* tota
                      hipStream_t st);
 // Split overflow tiles: per parent one workgroup re-bins its entries into k
 // x-sub-columns (sub-descriptors precomputed on the host).
+// halo > 0: erosion tiles (a box belongs to every sub-column its extent grown
+// by halo meets).
 void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_first,
                         const TileDesc* subs, uint32_t* sub_cntA, uint32_t* sub_cntB,
-                        const vl_box* entries, hipStream_t st);
+                        const vl_box* entries, int32_t halo, hipStream_t st);
 void launch_split_scatter(const TileDesc* parents, int np, const int32_t* sub_first, TileDesc* subs,
                           int ns, const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB,
                           uint32_t* fillA, uint32_t* fillB, const vl_box* entries,
-                          vl_box* sub_entries, hipStream_t st);
+                          vl_box* sub_entries, int32_t halo, hipStream_t st);
 
 // vl_stitch.hip
 // Partition side strips: tags with SEAM_B|SEAM_T -> next (border), the rest -> stitch.

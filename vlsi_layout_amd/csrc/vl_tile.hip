@@ -1,8 +1,9 @@
 // vl_tile.hip — tile binning and the per-tile Boolean kernels for gfx950.
 //
-// Binning: every box whose interior meets a tile of a uniform grid is copied
-// (after the op's per-box transform) into that tile's entry range: a counting
-// pass (atomics per tile), an exclusive scan, and a scatter pass.
+// Binning: every box whose interior meets a tile of a uniform grid (or, for
+// erosion, the tile grown by the erosion distance) is copied, after the op's
+// per-box transform, into that tile's entry range.  The ranges come from a
+// stable radix sort of (tile, layer) keys, so binning needs no atomics.
 //
 // Per-tile kernels.  A tile's result inside the tile is computed on the
 // compressed grid of its own coordinates:
@@ -38,9 +39,11 @@ namespace vl {
 
 namespace {
 
-__device__ __forceinline__ bool tile_range(const vl_box& b, const Grid& g, int& c1, int& c2, int& r1, int& r2) {
-  int64_t bx1 = max((int64_t)b.x1, g.x0), bx2 = min((int64_t)b.x2, g.x1);
-  int64_t by1 = max((int64_t)b.y1, g.y0), by2 = min((int64_t)b.y2, g.y1);
+// Tile index range of box b grown by xr (64-bit arithmetic, clipped to the grid).
+__device__ __forceinline__ bool tile_range(const vl_box& b, const Xform& xr, const Grid& g, int& c1, int& c2,
+                                           int& r1, int& r2) {
+  const int64_t bx1 = max((int64_t)b.x1 - xr.w, g.x0), bx2 = min((int64_t)b.x2 + xr.e, g.x1);
+  const int64_t by1 = max((int64_t)b.y1 - xr.s, g.y0), by2 = min((int64_t)b.y2 + xr.n, g.y1);
   if (bx1 >= bx2 || by1 >= by2) return false;
   c1 = (int)((bx1 - g.x0) / g.tw);
   c2 = (int)((bx2 - 1 - g.x0) / g.tw);
@@ -57,74 +60,74 @@ __device__ __forceinline__ vl_box xform(vl_box b, const Xform& x) {
   return b;
 }
 
-// Neighbouring boxes usually fall into the same tile, so the per-tile atomics
-// are aggregated per wave: the lanes whose first tile matches the leader's
-// share one atomic (ballot over the tile index), the leader broadcasts the old
-// value and every lane adds its rank among the peers.  The remaining tiles of
-// boxes spanning several tiles use plain atomics.
-__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* ctr, int32_t t, bool valid) {
-  uint32_t mine = 0;
-  uint64_t act = __ballot(valid);
-  const uint64_t lt = lanemask_lt();
-  while (act) {
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    const int32_t tl = __builtin_amdgcn_readlane(t, leader);
-    const uint64_t peers = __ballot(valid && t == tl);
-    uint32_t old = 0;
-    if (__lane_id() == leader) old = atomicAdd(&ctr[tl], (uint32_t)__popcll(peers));
-    old = (uint32_t)__builtin_amdgcn_readlane((int)old, leader);
-    if (valid && t == tl) mine = old + (uint32_t)__popcll(peers & lt);
-    act &= ~peers;
-  }
-  return mine;
-}
-
-__global__ __launch_bounds__(256) void k_bin_count(const vl_box* __restrict__ b, int64_t n, Xform xf, Grid g,
-                                                   uint32_t* cnt) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-    const int64_t i = i0 + threadIdx.x;
-    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
-    bool valid = false;
-    if (i < n) valid = tile_range(xform(b[i], xf), g, c1, c2, r1, r2);
-    (void)wave_aggregated_add(cnt, valid ? r1 * g.cols + c1 : 0, valid);
-    if (valid && (c2 > c1 || r2 > r1))
-      for (int r = r1; r <= r2; r++)
-        for (int c = c1; c <= c2; c++)
-          if (r != r1 || c != c1) atomicAdd(&cnt[(int64_t)r * g.cols + c], 1u);
+// Binning is a stable sort of (tile, layer) keys: every (box, tile) pair gets
+// the key 2*tile + layer, an LSD radix sort orders the pairs, and each tile's
+// entries (layer A, then layer B, each in input order) end up contiguous.  No
+// atomics, and the entry order is deterministic.
+//   span:   tiles per box (layer A boxes 0..na-1, layer B boxes na..na+nb-1)
+//   emit:   keys/values at the scanned positions
+//   bounds: start[k] = first position with key >= k
+//   gather: entries[i] = stored form of box vals[i]
+__global__ __launch_bounds__(256) void k_bin_span(const vl_box* __restrict__ a, int64_t na, Xform xra,
+                                                  const vl_box* __restrict__ b, int64_t nb, Grid g,
+                                                  uint32_t* __restrict__ cnt) {
+  const int64_t n = na + nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int c1, c2, r1, r2;
+    const bool isA = i < na;
+    const vl_box v = isA ? a[i] : b[i - na];
+    uint32_t k = 0;
+    if (tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2))
+      k = (uint32_t)(c2 - c1 + 1) * (uint32_t)(r2 - r1 + 1);
+    cnt[i] = k;
   }
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ b, int64_t n, Xform xf, Grid g,
-                                                     const uint64_t* __restrict__ off,
-                                                     const uint32_t* __restrict__ base_add, uint32_t* fill,
-                                                     vl_box* __restrict__ entries) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-    const int64_t i = i0 + threadIdx.x;
-    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
-    bool valid = false;
-    vl_box v{0, 0, 0, 0};
-    if (i < n) {
-      v = xform(b[i], xf);
-      valid = tile_range(v, g, c1, c2, r1, r2);
+__global__ __launch_bounds__(256) void k_bin_emit(const vl_box* __restrict__ a, int64_t na, Xform xra,
+                                                  const vl_box* __restrict__ b, int64_t nb, Grid g,
+                                                  const uint64_t* __restrict__ pos, uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+  const int64_t n = na + nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int c1, c2, r1, r2;
+    const bool isA = i < na;
+    const vl_box v = isA ? a[i] : b[i - na];
+    if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) continue;
+    uint64_t p = pos[i];
+    const uint64_t layer = isA ? 0ull : 1ull;
+    for (int r = r1; r <= r2; r++)
+      for (int c = c1; c <= c2; c++) {
+        keys[p] = (((uint64_t)r * (uint64_t)g.cols + (uint64_t)c) << 1) | layer;
+        vals[p] = (uint32_t)i;
+        p++;
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_bounds(const uint64_t* __restrict__ keys, uint64_t total,
+                                                    int64_t nkeys, uint64_t* __restrict__ start) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= nkeys;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t lo = 0, hi = total;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (keys[mid] < (uint64_t)k) lo = mid + 1; else hi = mid;
     }
-    const int32_t t0 = valid ? r1 * g.cols + c1 : 0;
-    const uint32_t slot = wave_aggregated_add(fill, t0, valid);
-    if (!valid) continue;
-    entries[off[t0] + (base_add ? base_add[t0] : 0u) + slot] = v;
-    if (c2 > c1 || r2 > r1)
-      for (int r = r1; r <= r2; r++)
-        for (int c = c1; c <= c2; c++) {
-          if (r == r1 && c == c1) continue;
-          const int64_t t = (int64_t)r * g.cols + c;
-          entries[off[t] + (base_add ? base_add[t] : 0u) + atomicAdd(&fill[t], 1u)] = v;
-        }
+    start[k] = lo;
   }
 }
 
-__global__ void k_build_desc(Grid g, const uint64_t* __restrict__ off, const uint32_t* __restrict__ cntA,
-                             const uint32_t* __restrict__ cntB, int32_t op, TileDesc* desc) {
+__global__ __launch_bounds__(256) void k_bin_gather(const vl_box* __restrict__ a, int64_t na, Xform xsa,
+                                                    const vl_box* __restrict__ b, const uint32_t* __restrict__ vals,
+                                                    uint64_t total, vl_box* __restrict__ entries) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int64_t v = vals[i];
+    entries[i] = v < na ? xform(a[v], xsa) : b[v - na];
+  }
+}
+
+__global__ void k_build_desc(Grid g, const uint64_t* __restrict__ start, int32_t op, TileDesc* desc) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
     int c = (int)(t % g.cols), r = (int)(t / g.cols);
@@ -133,9 +136,10 @@ __global__ void k_build_desc(Grid g, const uint64_t* __restrict__ off, const uin
     d.x1 = (int32_t)min(g.x0 + (int64_t)(c + 1) * g.tw, g.x1);
     d.y0 = (int32_t)(g.y0 + (int64_t)r * g.th);
     d.y1 = (int32_t)min(g.y0 + (int64_t)(r + 1) * g.th, g.y1);
-    d.off = (int64_t)off[t];
-    d.nA = (int32_t)cntA[t];
-    d.nB = cntB ? (int32_t)cntB[t] : 0;
+    const uint64_t s0 = start[2 * t], s1 = start[2 * t + 1], s2 = start[2 * t + 2];
+    d.off = (int64_t)s0;
+    d.nA = (int32_t)(s1 - s0);
+    d.nB = (int32_t)(s2 - s1);
     d.flags = (c > 0 ? SEAM_L : 0u) | (c < g.cols - 1 ? SEAM_R : 0u) | (r > 0 ? SEAM_B : 0u) |
               (r < g.rows - 1 ? SEAM_T : 0u);
     d.op = op;
@@ -239,14 +243,137 @@ __device__ __forceinline__ uint64_t wave_excl_u32(uint32_t v, uint32_t* tot) {
   return inc - v;
 }
 
+// Calls f(a, b) for every maximal run [a, b) of set bits in a column of W words.
+template <class F>
+__device__ __forceinline__ void for_each_run(const uint64_t* col, int W, F&& f) {
+  int a = -1;
+  for (int w = 0; w < W; w++) {
+    const uint64_t v = col[w];
+    int pos = 0;
+    while (pos < 64) {
+      if (a < 0) {
+        const uint64_t rem = v >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        a = w * 64 + pos;
+      } else {
+        const uint64_t rem = (~v) >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        f(a, w * 64 + pos);
+        a = -1;
+      }
+    }
+  }
+  if (a >= 0) f(a, W * 64);
+}
+
+// True when columns c and p differ in any bit of rows lo..hi (p may be null = empty).
+__device__ __forceinline__ bool any_diff(const uint64_t* c, const uint64_t* p, int lo, int hi) {
+  const int w0 = lo >> 6, w1 = hi >> 6;
+  for (int w = w0; w <= w1; w++) {
+    const int l = (w == w0) ? (lo & 63) : 0;
+    const int h = (w == w1) ? (hi & 63) : 63;
+    const uint64_t pv = p ? p[w] : 0ull;
+    if ((c[w] ^ pv) & bits_range(l, h)) return true;
+  }
+  return false;
+}
+
+
+// ---- erosion ------------------------------------------------------------------------
+// Erosion by the square [-R, R]^2 on unit cells: cell p survives when every
+// cell within R of p (in x and in y) is covered.  It is separable (erode in x,
+// then in y).  The compressed grid of an erosion tile holds, besides the
+// clipped box edges, every edge shifted inwards by R (clamped to the tile), so
+// both passes map runs onto existing grid lines: a run [u_a, u_b) becomes
+// [u_a + R, u_b - R).  Columns and rows outside the tile end up empty.
+
+// Number of v[0..n) below x / at most x (v sorted).
+__device__ __forceinline__ int count_below(const int32_t* v, int n, int64_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)v[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int count_at_most(const int32_t* v, int n, int64_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)v[mid] <= x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int32_t clamp64(int64_t v, int64_t lo, int64_t hi) {
+  return (int32_t)(v < lo ? lo : (v > hi ? hi : v));
+}
+
+// x pass: column c of the tile (c0 <= c < c1) keeps the rows covered in every
+// column meeting [ux[c] - R, ux[c+1] + R); windows leaving the grid keep nothing.
+template <int NW>
+__device__ __forceinline__ void erode_x(const uint64_t* M, uint64_t* out, const int32_t* ux, int nxu, int ncol,
+                                        int W, int c0, int c1, int32_t R, int t, int nt) {
+  for (int c = t; c < ncol; c += nt) {
+    uint64_t acc[NW];
+    bool keep = c >= c0 && c < c1;
+    int lo = 0, hi = -1;
+    if (keep) {
+      lo = count_at_most(ux, nxu, (int64_t)ux[c] - R) - 1;
+      hi = count_below(ux, nxu, (int64_t)ux[c + 1] + R) - 1;
+      keep = lo >= 0 && hi < ncol;
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) acc[w] = keep ? ~0ull : 0ull;
+    if (keep)
+      for (int j = lo; j <= hi; j++) {
+#pragma unroll
+        for (int w = 0; w < NW; w++)
+          if (w < W) acc[w] &= M[j * W + w];
+      }
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (w < W) out[c * W + w] = acc[w];
+  }
+}
+
+// y pass: each run [a, b) of a column becomes the rows of
+// [clamp(uy[a] + R), clamp(uy[b] - R)), clamped to the tile's [Y0, Y1].
+template <int NW>
+__device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const int32_t* uy, int nyu, int ncol,
+                                        int W, int32_t Y0, int32_t Y1, int32_t R, int t, int nt) {
+  for (int c = t; c < ncol; c += nt) {
+    uint64_t res[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) res[w] = 0ull;
+    for_each_run(M + c * W, W, [&](int a, int b) {
+      const int32_t ys = clamp64((int64_t)uy[a] + R, Y0, Y1);
+      const int32_t ye = clamp64((int64_t)uy[b] - R, Y0, Y1);
+      if (ys >= ye) return;
+      const int ia = count_below(uy, nyu, ys), ib = count_below(uy, nyu, ye);
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const int l = max(ia, 64 * w), h = min(ib, 64 * w + 64);
+        if (l < h) res[w] |= bits_range(l - 64 * w, h - 1 - 64 * w);
+      }
+    });
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (w < W) out[c * W + w] = res[w];
+  }
+}
+
 // ---- wave-per-tile kernel --------------------------------------------------------
 
-template <int NMAX, bool TWO, bool AREA>
+template <int NMAX, bool TWO, bool AREA, bool ERO>
 __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc, int ntiles,
                                                const vl_box* __restrict__ entries, TileOut o) {
-  constexpr int P = 2 * (NMAX + 1);   // coordinate slots per axis
+  constexpr int CPB = ERO ? 4 : 2;    // coordinates per box and axis (erosion adds the shifted edges)
+  constexpr int P = CPB * (NMAX + 1); // coordinate slots per axis
   constexpr int K = P / WAVE;         // slots per lane
-  constexpr int BPL = K / 2;          // boxes per lane (blocked)
+  constexpr int BPL = K / CPB;        // boxes per lane (blocked)
   constexpr int NCOL = P - 1;
   constexpr int NW = P / WAVE;        // words per column
   __shared__ int32_t ux[P], uy[P];
@@ -276,25 +403,41 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       }
       continue;
     }
-    // 1. clipped boxes (blocked: lane owns boxes BPL*lane .. +BPL-1) and coordinates
+    // 1. clipped boxes (blocked: lane owns boxes BPL*lane .. +BPL-1) and
+    //    coordinates.  The clip window is the tile, grown by R when eroding.
+    const int32_t R = ERO ? o.erode : 0;
+    const int32_t wx0 = clamp64((int64_t)d.x0 - R, INT32_MIN, INT32_MAX);
+    const int32_t wx1 = clamp64((int64_t)d.x1 + R, INT32_MIN, INT32_MAX);
+    const int32_t wy0 = clamp64((int64_t)d.y0 - R, INT32_MIN, INT32_MAX);
+    const int32_t wy1 = clamp64((int64_t)d.y1 + R, INT32_MIN, INT32_MAX);
     int32_t vx[K], vy[K];
     int32_t bx1[BPL], by1[BPL], bx2[BPL], by2[BPL];
 #pragma unroll
     for (int q = 0; q < BPL; q++) {
       const int i = BPL * lane + q;
       int32_t x1 = INT32_MAX, y1 = INT32_MAX, x2 = INT32_MAX, y2 = INT32_MAX;
+      int32_t sx1 = INT32_MAX, sy1 = INT32_MAX, sx2 = INT32_MAX, sy2 = INT32_MAX;
       if (i < n) {
         const vl_box b = entries[d.off + i];
-        x1 = max(b.x1, d.x0); x2 = min(b.x2, d.x1);
-        y1 = max(b.y1, d.y0); y2 = min(b.y2, d.y1);
+        x1 = max(b.x1, wx0); x2 = min(b.x2, wx1);
+        y1 = max(b.y1, wy0); y2 = min(b.y2, wy1);
+        if (ERO) {
+          sx1 = clamp64((int64_t)x1 + R, d.x0, d.x1); sx2 = clamp64((int64_t)x2 - R, d.x0, d.x1);
+          sy1 = clamp64((int64_t)y1 + R, d.y0, d.y1); sy2 = clamp64((int64_t)y2 - R, d.y0, d.y1);
+        }
       } else if (i == n) {
-        x1 = d.x0; x2 = d.x1; y1 = d.y0; y2 = d.y1;
+        x1 = wx0; x2 = wx1; y1 = wy0; y2 = wy1;
+        sx1 = d.x0; sx2 = d.x1; sy1 = d.y0; sy2 = d.y1;
       }
       bx1[q] = x1; bx2[q] = x2; by1[q] = y1; by2[q] = y2;
-      vx[2 * q] = x1; vx[2 * q + 1] = x2;
-      vy[2 * q] = y1; vy[2 * q + 1] = y2;
+      vx[CPB * q] = x1; vx[CPB * q + 1] = x2;
+      vy[CPB * q] = y1; vy[CPB * q + 1] = y2;
+      if (ERO) {
+        vx[CPB * q + 2] = sx1; vx[CPB * q + 3] = sx2;
+        vy[CPB * q + 2] = sy1; vy[CPB * q + 3] = sy2;
+      }
     }
-    const int cnt = 2 * n + 2;
+    const int cnt = CPB * n + CPB;
     // bitonic sort of P elements, element e = lane*K + k
 #pragma unroll
     for (int size = 2; size <= P; size <<= 1) {
@@ -378,6 +521,18 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     __syncthreads();
     if (twol) {
       for (int k = lane; k < words; k += WAVE) MA[k] = combine(op, MA[k], MX[k]);
+      __syncthreads();
+    }
+    // the tile's own columns [colT0, colT1) and rows [rowT0, rowT1)
+    int colT0 = 0, colT1 = ncol, rowT0 = 0, rowT1 = nrows;
+    if (ERO) {
+      colT0 = count_below(ux, nxu, d.x0);
+      colT1 = count_below(ux, nxu, d.x1);
+      rowT0 = count_below(uy, nyu, d.y0);
+      rowT1 = count_below(uy, nyu, d.y1);
+      erode_x<NW>(MA, MX, ux, nxu, ncol, W, colT0, colT1, R, lane, WAVE);
+      __syncthreads();
+      erode_y<NW>(MX, MA, uy, nyu, ncol, W, d.y0, d.y1, R, lane, WAVE);
       __syncthreads();
     }
     if (AREA) {
@@ -477,8 +632,10 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
           const bool endm = (e >> lane) & 1ull;
           uint32_t tg = 0;
           if (endm)
-            tg = ((a == 0 && (flags & SEAM_B)) ? SEAM_B : 0u) | ((ob[w] == nrows && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                 ((oj[w] == 0 && (flags & SEAM_L)) ? SEAM_L : 0u) | ((k == ncol && (flags & SEAM_R)) ? SEAM_R : 0u);
+            tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                 ((ob[w] == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                 ((oj[w] == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                 ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
           const uint64_t be = __ballot(endm), bs = __ballot(endm && tg != 0);
           if (endm && ok) {
             const uint64_t p = base + r + (uint32_t)__popcll(be & lt);
@@ -513,54 +670,18 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
 
 // ---- workgroup-per-tile kernel (large tiles) ------------------------------------
 
-// Calls f(a, b) for every maximal run [a, b) of set bits in a column of W words.
-template <class F>
-__device__ __forceinline__ void for_each_run(const uint64_t* col, int W, F&& f) {
-  int a = -1;
-  for (int w = 0; w < W; w++) {
-    const uint64_t v = col[w];
-    int pos = 0;
-    while (pos < 64) {
-      if (a < 0) {
-        const uint64_t rem = v >> pos;
-        if (rem == 0) break;
-        pos += __ffsll((unsigned long long)rem) - 1;
-        a = w * 64 + pos;
-      } else {
-        const uint64_t rem = (~v) >> pos;
-        if (rem == 0) break;
-        pos += __ffsll((unsigned long long)rem) - 1;
-        f(a, w * 64 + pos);
-        a = -1;
-      }
-    }
-  }
-  if (a >= 0) f(a, W * 64);
-}
-
-// True when columns c and p differ in any bit of rows lo..hi (p may be null = empty).
-__device__ __forceinline__ bool any_diff(const uint64_t* c, const uint64_t* p, int lo, int hi) {
-  const int w0 = lo >> 6, w1 = hi >> 6;
-  for (int w = w0; w <= w1; w++) {
-    const int l = (w == w0) ? (lo & 63) : 0;
-    const int h = (w == w1) ? (hi & 63) : 63;
-    const uint64_t pv = p ? p[w] : 0ull;
-    if ((c[w] ^ pv) & bits_range(l, h)) return true;
-  }
-  return false;
-}
-
-template <int NMAX, int NT, bool TWO, bool AREA>
+template <int NMAX, int NT, bool TWO, bool AREA, bool ERO>
 __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc, int ntiles,
                                                const vl_box* __restrict__ entries, TileOut o) {
-  constexpr int P = 2 * (NMAX + 1);
-  constexpr int NCOL = 2 * NMAX + 1;
+  constexpr int CPB = ERO ? 4 : 2;   // coordinates per box and axis
+  constexpr int P = CPB * (NMAX + 1);
+  constexpr int NCOL = P - 1;
   constexpr int NW = (NCOL + 63) / 64;
   __shared__ int32_t sx[P], sy[P];
   __shared__ int32_t ux[P], uy[P];
   __shared__ int16_t cb[NMAX][4];
   __shared__ uint64_t MA[NCOL * NW];
-  __shared__ uint64_t MB[TWO ? NCOL * NW : 1];
+  __shared__ uint64_t MB[(TWO || ERO) ? NCOL * NW : 1];
   __shared__ uint64_t colc[NCOL];
   __shared__ int wsI[NT / WAVE];
   __shared__ uint64_t wsU[NT / WAVE];
@@ -591,18 +712,37 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
     uint64_t F = 0, S = 0;
     int ncol = 0, nrows = 0, W = 0;
     long long my_area = 0;
+    // clip window: the tile, grown by R when eroding
+    const int32_t R = ERO ? o.erode : 0;
+    const int32_t wx0 = clamp64((int64_t)d.x0 - R, INT32_MIN, INT32_MAX);
+    const int32_t wx1 = clamp64((int64_t)d.x1 + R, INT32_MIN, INT32_MAX);
+    const int32_t wy0 = clamp64((int64_t)d.y0 - R, INT32_MIN, INT32_MAX);
+    const int32_t wy1 = clamp64((int64_t)d.y1 + R, INT32_MIN, INT32_MAX);
+    int colT0 = 0, colT1 = 0, rowT0 = 0, rowT1 = 0;
     {
-      const int cnt = 2 * n + 2;
+      const int cnt = CPB * n + CPB;
       int Pn = 2;
       while (Pn < cnt) Pn <<= 1;
       for (int i = tid; i < Pn; i += NT) {
         int32_t vx = INT32_MAX, vy = INT32_MAX;
-        if (i < 2 * n) {
-          const vl_box b = entries[d.off + (i >> 1)];
-          if (i & 1) { vx = min(b.x2, d.x1); vy = min(b.y2, d.y1); }
-          else { vx = max(b.x1, d.x0); vy = max(b.y1, d.y0); }
-        } else if (i == 2 * n) { vx = d.x0; vy = d.y0; }
-        else if (i == 2 * n + 1) { vx = d.x1; vy = d.y1; }
+        const int q = i / CPB, part = i % CPB;
+        if (q < n) {
+          const vl_box b = entries[d.off + q];
+          const int32_t x1 = max(b.x1, wx0), x2 = min(b.x2, wx1), y1 = max(b.y1, wy0), y2 = min(b.y2, wy1);
+          switch (part) {
+            case 0: vx = x1; vy = y1; break;
+            case 1: vx = x2; vy = y2; break;
+            case 2: vx = clamp64((int64_t)x1 + R, d.x0, d.x1); vy = clamp64((int64_t)y1 + R, d.y0, d.y1); break;
+            default: vx = clamp64((int64_t)x2 - R, d.x0, d.x1); vy = clamp64((int64_t)y2 - R, d.y0, d.y1); break;
+          }
+        } else if (q == n) {
+          switch (part) {
+            case 0: vx = wx0; vy = wy0; break;
+            case 1: vx = wx1; vy = wy1; break;
+            case 2: vx = d.x0; vy = d.y0; break;
+            default: vx = d.x1; vy = d.y1; break;
+          }
+        }
         sx[i] = vx;
         sy[i] = vy;
       }
@@ -646,10 +786,10 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
       __syncthreads();
       for (int i = tid; i < n; i += NT) {
         const vl_box b = entries[d.off + i];
-        cb[i][0] = (int16_t)lower_bound_i32(ux, nxu, max(b.x1, d.x0));
-        cb[i][1] = (int16_t)lower_bound_i32(ux, nxu, min(b.x2, d.x1));
-        cb[i][2] = (int16_t)lower_bound_i32(uy, nyu, max(b.y1, d.y0));
-        cb[i][3] = (int16_t)lower_bound_i32(uy, nyu, min(b.y2, d.y1));
+        cb[i][0] = (int16_t)lower_bound_i32(ux, nxu, max(b.x1, wx0));
+        cb[i][1] = (int16_t)lower_bound_i32(ux, nxu, min(b.x2, wx1));
+        cb[i][2] = (int16_t)lower_bound_i32(uy, nyu, max(b.y1, wy0));
+        cb[i][3] = (int16_t)lower_bound_i32(uy, nyu, min(b.y2, wy1));
       }
       const int words = ncol * W;
       for (int k = tid; k < words; k += NT) {
@@ -676,6 +816,17 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
         for (int k = tid; k < words; k += NT) MA[k] = combine(op, MA[k], MB[k]);
         __syncthreads();
       }
+      colT0 = 0; colT1 = ncol; rowT0 = 0; rowT1 = nrows;
+      if (ERO) {
+        colT0 = count_below(ux, nxu, d.x0);
+        colT1 = count_below(ux, nxu, d.x1);
+        rowT0 = count_below(uy, nyu, d.y0);
+        rowT1 = count_below(uy, nyu, d.y1);
+        erode_x<NW>(MA, MB, ux, nxu, ncol, W, colT0, colT1, R, tid, NT);
+        __syncthreads();
+        erode_y<NW>(MB, MA, uy, nyu, ncol, W, d.y0, d.y1, R, tid, NT);
+        __syncthreads();
+      }
       const uint32_t flags = d.flags;
       for (int j = tid; j < ncol; j += NT) {
         const uint64_t* cur = MA + j * W;
@@ -692,8 +843,8 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
             if (!any_diff(cur, prv, lo, hi)) return;
             int k = j + 1;
             while (k < ncol && !any_diff(MA + k * W, MA + (k - 1) * W, lo, hi)) k++;
-            const bool side = (a == 0 && (flags & SEAM_B)) || (b == nrows && (flags & SEAM_T)) ||
-                              (j == 0 && (flags & SEAM_L)) || (k == ncol && (flags & SEAM_R));
+            const bool side = (a == rowT0 && (flags & SEAM_B)) || (b == rowT1 && (flags & SEAM_T)) ||
+                              (j == colT0 && (flags & SEAM_L)) || (k == colT1 && (flags & SEAM_R));
             if (side) ns++; else nf++;
           });
           colc[j] = ((uint64_t)nf << 32) | ns;
@@ -754,10 +905,10 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
           if (!any_diff(cur, prv, lo, hi)) return;
           int k = j + 1;
           while (k < ncol && !any_diff(MA + k * W, MA + (k - 1) * W, lo, hi)) k++;
-          const uint32_t tag = ((a == 0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                               ((b == nrows && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                               ((j == 0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                               ((k == ncol && (flags & SEAM_R)) ? SEAM_R : 0u);
+          const uint32_t tag = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                               ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                               ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                               ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
           vl_box s;
           s.x1 = ux[j]; s.y1 = uy[a]; s.x2 = ux[k]; s.y2 = uy[b];
           if (tag) {
@@ -827,25 +978,25 @@ __global__ void k_gather_desc(const TileDesc* __restrict__ desc, const uint32_t*
 // ---- overflow splitting ----------------------------------------------------------
 // One workgroup per parent tile; sub-tiles are x-sub-columns of the parent.
 __device__ __forceinline__ void sub_range(const vl_box& b, const TileDesc& p, const TileDesc* subs, int first,
-                                          int nsub, int& s1, int& s2) {
-  const int32_t x1 = max(b.x1, p.x0), x2 = min(b.x2, p.x1);
+                                          int nsub, int32_t halo, int& s1, int& s2) {
+  const int64_t x1 = max((int64_t)b.x1 - halo, (int64_t)p.x0), x2 = min((int64_t)b.x2 + halo, (int64_t)p.x1);
   const int64_t sw = (int64_t)subs[first].x1 - subs[first].x0;   // all but the last have width sw
-  s1 = (int)(((int64_t)x1 - p.x0) / sw);
-  s2 = (int)(((int64_t)x2 - 1 - p.x0) / sw);
+  s1 = (int)((x1 - p.x0) / sw);
+  s2 = (int)((x2 - 1 - p.x0) / sw);
   if (s1 >= nsub) s1 = nsub - 1;
   if (s2 >= nsub) s2 = nsub - 1;
 }
 
 __global__ void k_split_count(const TileDesc* __restrict__ parents, const int32_t* __restrict__ sub_first,
                               const TileDesc* __restrict__ subs, uint32_t* cntA, uint32_t* cntB,
-                              const vl_box* __restrict__ entries) {
+                              const vl_box* __restrict__ entries, int32_t halo) {
   const TileDesc p = parents[blockIdx.x];
   const int first = sub_first[blockIdx.x], nsub = sub_first[blockIdx.x + 1] - first;
   const int n = p.nA + p.nB;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const vl_box b = entries[p.off + i];
     int s1, s2;
-    sub_range(b, p, subs, first, nsub, s1, s2);
+    sub_range(b, p, subs, first, nsub, halo, s1, s2);
     for (int s = s1; s <= s2; s++) atomicAdd(i < p.nA ? &cntA[first + s] : &cntB[first + s], 1u);
   }
 }
@@ -853,14 +1004,15 @@ __global__ void k_split_count(const TileDesc* __restrict__ parents, const int32_
 __global__ void k_split_scatter(const TileDesc* __restrict__ parents, const int32_t* __restrict__ sub_first,
                                 const TileDesc* __restrict__ subs, const uint64_t* __restrict__ off,
                                 const uint32_t* __restrict__ cntA, uint32_t* fillA, uint32_t* fillB,
-                                const vl_box* __restrict__ entries, vl_box* __restrict__ sub_entries) {
+                                const vl_box* __restrict__ entries, vl_box* __restrict__ sub_entries,
+                                int32_t halo) {
   const TileDesc p = parents[blockIdx.x];
   const int first = sub_first[blockIdx.x], nsub = sub_first[blockIdx.x + 1] - first;
   const int n = p.nA + p.nB;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const vl_box b = entries[p.off + i];
     int s1, s2;
-    sub_range(b, p, subs, first, nsub, s1, s2);
+    sub_range(b, p, subs, first, nsub, halo, s1, s2);
     for (int s = s1; s <= s2; s++) {
       const int q = first + s;
       uint64_t pos = (i < p.nA) ? off[q] + atomicAdd(&fillA[q], 1u)
@@ -905,67 +1057,86 @@ int occupancy_wg(KF kern, int threads) {
   return per * num_cus();
 }
 
-template <int NMAX, bool TWO, bool AREA>
+template <int NMAX, bool TWO, bool AREA, bool ERO>
 void launch_w(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int wg, hipStream_t st) {
   const int g = wg < 1 ? 1 : wg;
-  hipLaunchKernelGGL((k_tile_w<NMAX, TWO, AREA>), dim3(g), dim3(WAVE), 0, st, desc, ntiles, entries, o);
+  hipLaunchKernelGGL((k_tile_w<NMAX, TWO, AREA, ERO>), dim3(g), dim3(WAVE), 0, st, desc, ntiles, entries, o);
 }
 
-template <bool TWO, bool AREA>
+constexpr int TILE_BLOCK_NMAX_ERODE = 127;
+
+template <bool TWO, bool AREA, bool ERO>
 void launch_b(const TileDesc* desc, int ntiles, const vl_box* entries, TileOut o, int wg, hipStream_t st) {
   const int g = wg < 1 ? 1 : wg;
-  hipLaunchKernelGGL((k_tile_b<TILE_BLOCK_NMAX, 256, TWO, AREA>), dim3(g), dim3(256), 0, st, desc, ntiles,
-                     entries, o);
+  constexpr int NM = ERO ? TILE_BLOCK_NMAX_ERODE : TILE_BLOCK_NMAX;
+  hipLaunchKernelGGL((k_tile_b<NM, 256, TWO, AREA, ERO>), dim3(g), dim3(256), 0, st, desc, ntiles, entries, o);
 }
 
 }  // namespace
 
-void launch_bin_count(const vl_box* b, int64_t n, Xform xf, Grid g, uint32_t* cnt, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n)), dim3(256), 0, st, b, n, xf, g, cnt);
+void launch_bin_span(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
+                     hipStream_t st) {
+  if (na + nb > 0)
+    hipLaunchKernelGGL(k_bin_span, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, b, nb, g, cnt);
 }
-void launch_bin_scatter(const vl_box* b, int64_t n, Xform xf, Grid g, const uint64_t* off,
-                        const uint32_t* base_add, uint32_t* fill, vl_box* entries, hipStream_t st) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n)), dim3(256), 0, st, b, n, xf, g, off, base_add, fill,
+void launch_bin_emit(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g,
+                     const uint64_t* pos, uint64_t* keys, uint32_t* vals, hipStream_t st) {
+  if (na + nb > 0)
+    hipLaunchKernelGGL(k_bin_emit, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, b, nb, g, pos, keys,
+                       vals);
+}
+void launch_bin_bounds(const uint64_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st) {
+  hipLaunchKernelGGL(k_bin_bounds, dim3(grid_for(nkeys + 1)), dim3(256), 0, st, keys, total, nkeys, start);
+}
+void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, const uint32_t* vals,
+                       uint64_t total, vl_box* entries, hipStream_t st) {
+  if (total > 0)
+    hipLaunchKernelGGL(k_bin_gather, dim3(grid_for((int64_t)total)), dim3(256), 0, st, a, na, xsa, b, vals, total,
                        entries);
 }
-void launch_build_desc(Grid g, const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB, int32_t op,
-                       TileDesc* desc, hipStream_t st) {
+void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st) {
   int64_t nt = (int64_t)g.cols * g.rows;
-  hipLaunchKernelGGL(k_build_desc, dim3(grid_for(nt)), dim3(256), 0, st, g, off, cntA, cntB, op, desc);
+  hipLaunchKernelGGL(k_build_desc, dim3(grid_for(nt)), dim3(256), 0, st, g, start, op, desc);
 }
 
 #define VL_W_DISPATCH(NM)                                                                  \
   if (nmax == NM) {                                                                        \
     if (area_mode) {                                                                       \
-      if (two_layers) launch_w<NM, true, true>(desc, ntiles, entries, o, wg, st);      \
-      else launch_w<NM, false, true>(desc, ntiles, entries, o, wg, st);                \
+      if (two_layers) launch_w<NM, true, true, false>(desc, ntiles, entries, o, wg, st);   \
+      else launch_w<NM, false, true, false>(desc, ntiles, entries, o, wg, st);             \
     } else {                                                                               \
-      if (two_layers) launch_w<NM, true, false>(desc, ntiles, entries, o, wg, st);     \
-      else launch_w<NM, false, false>(desc, ntiles, entries, o, wg, st);               \
+      if (two_layers) launch_w<NM, true, false, false>(desc, ntiles, entries, o, wg, st);  \
+      else launch_w<NM, false, false, false>(desc, ntiles, entries, o, wg, st);            \
     }                                                                                      \
     return;                                                                                \
   }
 
-void launch_tile_wave(int nmax, bool two_layers, bool area_mode, const TileDesc* desc, int ntiles,
+void launch_tile_wave(int nmax, bool two_layers, bool area_mode, bool erode, const TileDesc* desc, int ntiles,
                       const vl_box* entries, TileOut o, int wg, hipStream_t st) {
+  if (erode) {
+    launch_w<63, false, false, true>(desc, ntiles, entries, o, wg, st);
+    return;
+  }
   VL_W_DISPATCH(63)
   VL_W_DISPATCH(127)
 }
 
-#define VL_W_MAXWG(NM)                                                                                      \
-  if (nmax == NM) {                                                                                         \
-    if (area_mode)                                                                                          \
-      return two_layers ? occupancy_wg(k_tile_w<NM, true, true>, WAVE) : occupancy_wg(k_tile_w<NM, false, true>, WAVE); \
-    return two_layers ? occupancy_wg(k_tile_w<NM, true, false>, WAVE) : occupancy_wg(k_tile_w<NM, false, false>, WAVE); \
+#define VL_W_MAXWG(NM)                                                                                  \
+  if (nmax == NM) {                                                                                     \
+    if (area_mode)                                                                                      \
+      return two_layers ? occupancy_wg(k_tile_w<NM, true, true, false>, WAVE)                           \
+                        : occupancy_wg(k_tile_w<NM, false, true, false>, WAVE);                         \
+    return two_layers ? occupancy_wg(k_tile_w<NM, true, false, false>, WAVE)                            \
+                      : occupancy_wg(k_tile_w<NM, false, false, false>, WAVE);                          \
   }
 
-int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode) {
-  static int cache[2][2][2] = {};
+int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode, bool erode) {
+  static int cache[2][2][2][2] = {};
   const int ni = nmax == 63 ? 0 : 1;
-  int& c = cache[ni][two_layers][area_mode];
+  int& c = cache[ni][two_layers][area_mode][erode];
   if (c) return c;
   auto f = [&]() -> int {
+    if (erode) return occupancy_wg(k_tile_w<63, false, false, true>, WAVE);
     VL_W_MAXWG(63)
     VL_W_MAXWG(127)
     return 1024;
@@ -974,27 +1145,31 @@ int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode) {
   return c;
 }
 
-void launch_tile_block(bool two_layers, bool area_mode, const TileDesc* desc, int ntiles, const vl_box* entries,
-                       TileOut o, int wg, hipStream_t st) {
-  if (area_mode) {
-    if (two_layers) launch_b<true, true>(desc, ntiles, entries, o, wg, st);
-    else launch_b<false, true>(desc, ntiles, entries, o, wg, st);
+void launch_tile_block(bool two_layers, bool area_mode, bool erode, const TileDesc* desc, int ntiles,
+                       const vl_box* entries, TileOut o, int wg, hipStream_t st) {
+  if (erode) {
+    launch_b<false, false, true>(desc, ntiles, entries, o, wg, st);
+  } else if (area_mode) {
+    if (two_layers) launch_b<true, true, false>(desc, ntiles, entries, o, wg, st);
+    else launch_b<false, true, false>(desc, ntiles, entries, o, wg, st);
   } else {
-    if (two_layers) launch_b<true, false>(desc, ntiles, entries, o, wg, st);
-    else launch_b<false, false>(desc, ntiles, entries, o, wg, st);
+    if (two_layers) launch_b<true, false, false>(desc, ntiles, entries, o, wg, st);
+    else launch_b<false, false, false>(desc, ntiles, entries, o, wg, st);
   }
 }
 
-int tile_block_max_wg(bool two_layers, bool area_mode) {
-  static int cache[2][2] = {};
-  int& c = cache[two_layers][area_mode];
+int tile_block_max_wg(bool two_layers, bool area_mode, bool erode) {
+  static int cache[2][2][2] = {};
+  int& c = cache[two_layers][area_mode][erode];
   if (c) return c;
-  if (area_mode)
-    c = two_layers ? occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, true, true>, 256)
-                   : occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, false, true>, 256);
+  if (erode)
+    c = occupancy_wg(k_tile_b<TILE_BLOCK_NMAX_ERODE, 256, false, false, true>, 256);
+  else if (area_mode)
+    c = two_layers ? occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, true, true, false>, 256)
+                   : occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, false, true, false>, 256);
   else
-    c = two_layers ? occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, true, false>, 256)
-                   : occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, false, false>, 256);
+    c = two_layers ? occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, true, false, false>, 256)
+                   : occupancy_wg(k_tile_b<TILE_BLOCK_NMAX, 256, false, false, false>, 256);
   return c;
 }
 
@@ -1012,18 +1187,20 @@ void launch_gather_desc(const TileDesc* desc, const uint32_t* idx, int n, TileDe
 }
 
 void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_first, const TileDesc* subs,
-                        uint32_t* sub_cntA, uint32_t* sub_cntB, const vl_box* entries, hipStream_t st) {
+                        uint32_t* sub_cntA, uint32_t* sub_cntB, const vl_box* entries, int32_t halo,
+                        hipStream_t st) {
   if (np > 0)
     hipLaunchKernelGGL(k_split_count, dim3(np), dim3(256), 0, st, parents, sub_first, subs, sub_cntA, sub_cntB,
-                       entries);
+                       entries, halo);
 }
 
 void launch_split_scatter(const TileDesc* parents, int np, const int32_t* sub_first, TileDesc* subs, int ns,
                           const uint64_t* off, const uint32_t* cntA, const uint32_t* cntB, uint32_t* fillA,
-                          uint32_t* fillB, const vl_box* entries, vl_box* sub_entries, hipStream_t st) {
+                          uint32_t* fillB, const vl_box* entries, vl_box* sub_entries, int32_t halo,
+                          hipStream_t st) {
   if (np <= 0) return;
   hipLaunchKernelGGL(k_split_scatter, dim3(np), dim3(256), 0, st, parents, sub_first, subs, off, cntA, fillA,
-                     fillB, entries, sub_entries);
+                     fillB, entries, sub_entries, halo);
   hipLaunchKernelGGL(k_fill_subdesc, dim3(grid_for(ns)), dim3(256), 0, st, subs, ns, off, cntA, cntB);
 }
 
