@@ -66,8 +66,15 @@ typedef struct vl_layer  vl_layer;    /* a device-resident box layer       */
 const char* vl_last_error(void);
 int vl_engine_create(int device, vl_engine** out);
 int vl_engine_destroy(vl_engine* e);
-/* Tuning knobs: "tile_nmax" (127|255|511), "tile_target" (boxes per tile),
- * "level_factor" (row coarsening per seam level), "threads" (workgroup size). */
+/* Tuning knobs (results never depend on them):
+ *   "tile_nmax"    wave-kernel tile capacity, 63 (default) or 127 boxes
+ *   "tile_target"  mean boxes per tile the operator grid aims for (16)
+ *   "erode_target" the same for SIZE(-d) erosion grids (12)
+ *   "level_factor" row coarsening per seam level (16)
+ *   "erode_kernel" 1: SIZE(-d) by the per-tile erosion kernel (default);
+ *                  0: by complement, grow and difference
+ *   "max_wg"       cap on the tile-kernel grid (0 = occupancy limit)
+ *   "profile"      1: collect per-kernel statistics (vl_engine_kernel_stats) */
 int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
 int vl_engine_get_param(vl_engine* e, const char* key, int64_t* value);
 /* Block until all work queued on the engine's stream has finished. */

@@ -119,7 +119,8 @@ struct vl_engine {
   uint64_t* h_stage = nullptr;   // pinned, 4 KiB
   // tuning
   int nmax = 63;                 // wave-kernel tile capacity (63 or 127 boxes)
-  int target = 42;               // mean boxes per tile the grid aims for
+  int target = 16;               // mean boxes per tile the grid aims for
+  int erode_target = 12;         // same for SIZE(-d) erosion grids (windows carry a halo)
   int level_factor = 16;
   int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
@@ -142,6 +143,10 @@ struct vl_layer {
   vl_box* d = nullptr;
   int64_t n = 0;
   size_t bytes = 0;   // pool-rounded allocation size
+  // Cached bounding box of the boxes (valid while mbr_ok; every write to the
+  // layer clears it).  Operators read it instead of reducing the boxes again.
+  mutable bool mbr_ok = false;
+  mutable int64_t mbr[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -370,33 +375,50 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
                   bool two, bool area, int32_t erode, Outs& o, int depth) {
   if (ntiles <= 0) return VL_OK;
   Scratch s(e);
-  uint32_t* small = s.get<uint32_t>(4);   // [0] wave overflow count [1] block overflow count
+  // Tiers: wave kernels of growing capacity, each taking the previous tier's
+  // overflow list, then the 256-thread kernel; its overflow is split.
+  int tiers[3];
+  int ntier = 0;
+  if (erode > 0) {
+    tiers[ntier++] = 15;
+    tiers[ntier++] = 31;
+  } else {
+    tiers[ntier++] = 31;
+    tiers[ntier++] = 63;
+    if (e->nmax == 127) tiers[ntier++] = 127;
+  }
+  uint32_t* small = s.get<uint32_t>(4);   // overflow count of tier t at [t], of the block kernel at [ntier]
   unsigned long long* cnt = s.get<unsigned long long>(3);   // [0] strips [1] seam strips [2] max region fill
-  uint32_t* ovf1 = s.get<uint32_t>(ntiles);
-  uint32_t* ovf2 = s.get<uint32_t>(ntiles);
+  uint32_t* ovfl[4];
+  for (int t = 0; t <= ntier; t++) ovfl[t] = s.get<uint32_t>(ntiles);
+  uint32_t* ovf2 = ovfl[ntier];
   long long* d_area = s.get<long long>(1);
   uint64_t* tab_off = area ? nullptr : s.get<uint64_t>((size_t)ntiles);
   uint32_t* tab_n = area ? nullptr : s.get<uint32_t>(2 * (size_t)ntiles);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
-  const int nmax_w = erode > 0 ? 63 : e->nmax;
-  int max_w = tile_wave_max_wg(nmax_w, two, area, erode > 0);
-  int max_b = tile_block_max_wg(two, area, erode > 0);
-  if (e->max_wg_cap > 0) max_w = std::min(max_w, e->max_wg_cap);
-  // Exact grids: workgroup g takes tiles g, g+G, ... and owns region g of the
-  // scratch stream (wave kernel: regions 0..Gw-1, block kernel: Gw..Gw+Gb-1).
-  const int Gw = std::max(1, std::min(ntiles, max_w));
-  const int Gb = std::max(1, std::min(ntiles, max_b));
+  // Exact grids: workgroup g of a launch takes tiles g, g+G, ... and owns one
+  // region of the scratch stream (regions of the launches follow each other).
+  int G[4];
+  int nregions = 0;
+  for (int t = 0; t < ntier; t++) {
+    int mw = tile_wave_max_wg(tiers[t], two, area, erode > 0);
+    if (e->max_wg_cap > 0) mw = std::min(mw, e->max_wg_cap);
+    G[t] = std::max(1, std::min(ntiles, mw));
+    nregions += G[t];
+  }
+  G[ntier] = std::max(1, std::min(ntiles, tile_block_max_wg(two, area, erode > 0)));
+  nregions += G[ntier];
   DVec<vl_box> scr(e);
   DVec<uint8_t> scrT(e);
   uint64_t total = 0, nsideT = 0;
   uint32_t nov = 0;
   const double t_pt = g_debug ? now_ms() : 0.0;
-  // region size: a few times the mean strips per workgroup; regrown to the
-  // measured maximum fill if any region overflows
-  uint64_t region = (uint64_t)(3.0 * (double)nentries / Gw) + 512;
+  // region size: a few times the mean strips per workgroup of the first tier;
+  // regrown to the measured maximum fill if any region overflows
+  uint64_t region = (uint64_t)(4.0 * (double)nentries / G[0]) + 1024;
   for (int attempt = 0; attempt < 2; attempt++) {
     if (!area) {
-      const size_t need = (size_t)(Gw + Gb) * region;
+      const size_t need = (size_t)nregions * region;
       VLCK(scr.ensure(need, 0));
       VLCK(scrT.ensure(need, 0));
     }
@@ -415,22 +437,20 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     to.tab_off = tab_off;
     to.tab_n = tab_n;
     to.ntab = ntiles;
-    to.list = nullptr;
-    to.list_n = nullptr;
-    to.ovf = ovf1;
-    to.ovf_n = small;
     to.area = d_area;
     to.err = e->d_err;
     if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
-    launch_tile_wave(nmax_w, two, area, erode > 0, d_desc, ntiles, d_entries, to, Gw, e->st);
-    // tiles above the wave kernel's capacity (count known only on the device)
-    TileOut tb = to;
-    tb.region_base = Gw;
-    tb.list = ovf1;
-    tb.list_n = small;
-    tb.ovf = ovf2;
-    tb.ovf_n = small + 1;
-    launch_tile_block(two, area, erode > 0, d_desc, ntiles, d_entries, tb, Gb, e->st);
+    int rbase = 0;
+    for (int t = 0; t <= ntier; t++) {
+      to.region_base = rbase;
+      to.list = t ? ovfl[t - 1] : nullptr;        // tiles above the previous tier's capacity
+      to.list_n = t ? small + (t - 1) : nullptr;  // (count known only on the device)
+      to.ovf = ovfl[t];
+      to.ovf_n = small + t;
+      if (t < ntier) launch_tile_wave(tiers[t], two, area, erode > 0, d_desc, ntiles, d_entries, to, G[t], e->st);
+      else launch_tile_block(two, area, erode > 0, d_desc, ntiles, d_entries, to, G[t], e->st);
+      rbase += G[t];
+    }
     HIPCK(hipGetLastError());
     if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
     struct {
@@ -440,7 +460,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     } rb;
     HIPCK(hipMemcpyAsync(e->h_stage, cnt, 24, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 28, small + 1, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + 28, small + ntier, 4, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipMemcpyAsync((char*)e->h_stage + 32, d_area, 8, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     memcpy(&rb, e->h_stage, sizeof(rb));
@@ -448,7 +468,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       float ms = 0.f;
       HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
       auto& k = e->kst[KS_TILE];
-      k.launches += 2;
+      k.launches += ntier + 1;
       k.ms += ms;
       k.bytes += 16.0 * (double)nentries + 16.0 * (double)rb.cnt[0] + 48.0 * ntiles;
     }
@@ -524,7 +544,12 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
   if (s2.oom || !entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
   if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
   launch_bin_emit(a, na, xr, b, nb, g, pos, keys, vals, e->st);
-  radix_sort_pairs(keys, vals, keys2, vals2, (int64_t)total, rtmp, e->st);
+  int key_bits = 1;
+  while (key_bits < 63 && ((uint64_t)1 << key_bits) < (uint64_t)(2 * nt)) key_bits++;
+  if (radix_sort_pairs_bits(keys, vals, keys2, vals2, (int64_t)total, key_bits, rtmp, e->st)) {
+    std::swap(keys, keys2);
+    std::swap(vals, vals2);
+  }
   launch_bin_bounds(keys, total, 2 * nt, start, e->st);
   launch_bin_gather(a, na, xs, b, vals, total, entries, e->st);
   launch_build_desc(g, start, top, desc, e->st);
@@ -553,21 +578,41 @@ int adopt(vl_engine* e, Outs& o, vl_layer* c) {
   c->d = p;
   c->bytes = bytes;
   c->n = (int64_t)o.nout;
+  c->mbr_ok = false;
   return VL_OK;
 }
 
 int set_empty(vl_engine* e, vl_layer* c) {
   (void)e;
   c->n = 0;
+  c->mbr_ok = false;
   return VL_OK;
 }
 
 // The whole operator pipeline.  area_out != null selects area mode.
+// MBRs of the inputs: from the layers' caches when the transform cannot make a
+// box invalid (all components >= 0 and the grown MBR stays in int32), else by
+// a validating reduction.
+int op_mbrs(vl_engine* e, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb, const Box64* ka,
+            const Box64* kb, Box64* ma, Box64* mb) {
+  const bool grow_ok = xa.e >= 0 && xa.n >= 0 && xa.w >= 0 && xa.s >= 0;
+  if (ka && (kb || nb <= 0) && grow_ok) {
+    Box64 g{ka->x1 - xa.w, ka->y1 - xa.s, ka->x2 + xa.e, ka->y2 + xa.n};
+    if (na <= 0 || (g.x1 >= INT32_MIN && g.y1 >= INT32_MIN && g.x2 <= INT32_MAX && g.y2 <= INT32_MAX)) {
+      *ma = na > 0 ? g : Box64{INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
+      *mb = nb > 0 ? *kb : Box64{INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
+      return VL_OK;
+    }
+  }
+  return mbr2(e, a, na, xa, b, nb, ma, mb);
+}
+
 int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb,
-           vl_layer* c, long long* area_out, int32_t erode = 0) {
-  if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; }
+           vl_layer* c, long long* area_out, int32_t erode = 0, const Box64* ka = nullptr,
+           const Box64* kb = nullptr) {
+  if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; kb = nullptr; }
   Box64 ma, mb;
-  VLCK(mbr2(e, a, na, xa, b, nb, &ma, &mb));
+  VLCK(op_mbrs(e, a, na, xa, b, nb, ka, kb, &ma, &mb));
   Box64 dom;
   const bool ea = na <= 0, eb = nb <= 0;
   switch (top) {
@@ -592,7 +637,7 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
   }
   const bool area = area_out != nullptr;
   Outs o(e);
-  Grid g = make_grid(dom, na + nb, e->target);
+  Grid g = make_grid(dom, na + nb, erode > 0 ? e->erode_target : e->target);
   const double t_start = g_debug ? now_ms() : 0.0;
   const Xform xr = erode > 0 ? Xform{erode, erode, erode, erode} : xa;
   VLCK(bin_and_process(e, g, top, a, na, xa, xr, b, nb, area, erode, o));
@@ -722,17 +767,32 @@ int snapshot(vl_engine* e, const vl_layer* l, Scratch& s, const vl_box** out) {
   return VL_OK;
 }
 
+// Bounding box of a layer (cached on the layer).
+int layer_mbr(vl_engine* e, const vl_layer* l, Box64* m) {
+  if (!l->mbr_ok) {
+    Box64 ma{INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN}, mb;
+    if (l->n > 0) VLCK(mbr2(e, l->d, l->n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb));
+    l->mbr[0] = ma.x1; l->mbr[1] = ma.y1; l->mbr[2] = ma.x2; l->mbr[3] = ma.y2;
+    l->mbr_ok = true;
+  }
+  *m = Box64{l->mbr[0], l->mbr[1], l->mbr[2], l->mbr[3]};
+  return VL_OK;
+}
+
 int binary(vl_engine* e, int top, const vl_layer* a, const vl_layer* b, vl_layer* c, Xform xa = Xform{0, 0, 0, 0}) {
   VLCK(check_layers(e, a, b, c));
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
   OpTimer tm(e);
   Scratch s(e);
+  Box64 ka, kb;
+  VLCK(layer_mbr(e, a, &ka));
+  if (b) VLCK(layer_mbr(e, b, &kb));
   const vl_box* pa = a->d;
   const vl_box* pb = b ? b->d : nullptr;
   if (c == a) VLCK(snapshot(e, a, s, &pa));
   if (b && c == b) VLCK(snapshot(e, b, s, &pb));
-  return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr);
+  return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr, 0, &ka, b ? &kb : nullptr);
 }
 
 bool is_device_ptr(const void* p) {
@@ -794,7 +854,6 @@ int vl_engine_create(int device, vl_engine** out) {
   }
   if (const char* v = getenv("VL_TILE_NMAX")) e->nmax = atoi(v);
   if (e->nmax != 63 && e->nmax != 127) e->nmax = 63;
-  e->target = std::max(8, (e->nmax * 2) / 3);
   *out = e;
   return VL_OK;
 }
@@ -830,6 +889,9 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "tile_target") {
     if (v < 4 || v > 4096) return fail(VL_EINVAL, "tile_target out of range");
     e->target = (int)v;
+  } else if (k == "erode_target") {
+    if (v < 4 || v > 4096) return fail(VL_EINVAL, "erode_target out of range");
+    e->erode_target = (int)v;
   } else if (k == "level_factor") {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "level_factor out of range");
     e->level_factor = (int)v;
@@ -851,6 +913,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   std::string k(key);
   if (k == "tile_nmax") *v = e->nmax;
   else if (k == "tile_target") *v = e->target;
+  else if (k == "erode_target") *v = e->erode_target;
   else if (k == "level_factor") *v = e->level_factor;
   else if (k == "max_wg") *v = e->max_wg_cap;
   else if (k == "erode_kernel") *v = e->erode_kernel;
@@ -932,6 +995,7 @@ int vl_layer_upload(vl_layer* l, const vl_box* host, int64_t n) {
   if (n > 0) HIPCK(hipMemcpyAsync(l->d, host, n * sizeof(vl_box), hipMemcpyHostToDevice, e->st));
   HIPCK(hipStreamSynchronize(e->st));
   l->n = n;
+  l->mbr_ok = false;
   return VL_OK;
 }
 
@@ -943,9 +1007,10 @@ int vl_layer_upload_device(vl_layer* l, const vl_box* dev, int64_t n) {
   VLCK(layer_reserve(l, n));
   if (n > 0) HIPCK(hipMemcpyAsync(l->d, dev, n * sizeof(vl_box), hipMemcpyDeviceToDevice, e->st));
   l->n = n;
+  l->mbr_ok = false;
   if (n > 0) {
-    Box64 ma, mb;
-    VLCK(mbr2(e, l->d, n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb));
+    Box64 m;
+    VLCK(layer_mbr(e, l, &m));   // validates the boxes
   }
   return VL_OK;
 }
@@ -1006,8 +1071,8 @@ int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c) {
   if (a->n == 0) return set_empty(e, c);
   const int32_t r = (d == INT32_MIN) ? INT32_MAX : -d;
   OpTimer tm(e);
-  Box64 ma, mb;
-  VLCK(mbr2(e, a->d, a->n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb));
+  Box64 ma;
+  VLCK(layer_mbr(e, a, &ma));
   if (ma.x1 - 1 < INT32_MIN || ma.y1 - 1 < INT32_MIN || ma.x2 + 1 > INT32_MAX || ma.y2 + 1 > INT32_MAX)
     return fail(VL_EINVAL, "SIZE frame leaves the int32 range");
   Scratch s(e);
@@ -1015,9 +1080,9 @@ int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c) {
   if (c == a) VLCK(snapshot(e, a, s, &pa));
   // Per-tile erosion when the tiles of the operator's grid are at least 4d
   // wide and tall (each box then meets only a few tile windows).
-  const Grid g = make_grid(ma, a->n, e->target);
+  const Grid g = make_grid(ma, a->n, e->erode_target);
   if (e->erode_kernel && g.tw >= 4 * (int64_t)r && g.th >= 4 * (int64_t)r)
-    return run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r);
+    return run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r, &ma);
   vl_box frame{(int32_t)(ma.x1 - 1), (int32_t)(ma.y1 - 1), (int32_t)(ma.x2 + 1), (int32_t)(ma.y2 + 1)};
   void* dframe = nullptr;
   VLCK(upload_tmp(e, s, &frame, sizeof(frame), &dframe));

@@ -91,6 +91,11 @@ RadixPlan radix_plan(int64_t n);
 // a digit (one synchronisation).
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                       int64_t n, void* tmp, hipStream_t st);
+// Same sort for keys below 2^key_bits: ceil(key_bits/8) passes, no histogram
+// pre-pass and no host synchronisation.  Returns true when the sorted pairs
+// are in keys_alt/vals_alt (odd number of passes), false when in keys/vals.
+bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                           int key_bits, void* tmp, hipStream_t st);
 void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st);
 void launch_tracker_gather(const vl_tracker* in, const uint32_t* perm, vl_tracker* out, int64_t n,
                            hipStream_t st);
@@ -142,8 +147,9 @@ struct TileOut {
   long long* area;             // area mode: accumulated union area (int64)
   uint32_t* err;
 };
-// Wave-per-tile kernel (capacity 63 or 127 boxes; erosion: 63), launched with
-// exactly wg workgroups (wg regions of the scratch stream).  Erosion mode
+// Wave-per-tile kernel (capacity nmax = 31, 63 or 127 boxes; erosion: 15 or
+// 31), launched with exactly wg workgroups (wg regions of the scratch stream).
+// With o.list set it takes the tiles of that list.  Erosion mode
 // computes the erosion of layer A by the square of half-side o.erode inside
 // each tile from the boxes meeting the tile grown by o.erode.
 void launch_tile_wave(int nmax, bool two_layers, bool area_mode, bool erode, const TileDesc* desc, int ntiles,

@@ -257,6 +257,32 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
   if (done & 1) hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n)), dim3(256), 0, st, ks, vs, keys, vals, n);
 }
 
+bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                           int key_bits, void* tmp, hipStream_t st) {
+  if (n <= 1 || key_bits <= 0) return false;
+  RadixPlan p = radix_plan(n);
+  const int nb = p.nblocks;
+  const size_t hb = (size_t)RS_BINS * nb;
+  uint32_t* bhist = (uint32_t*)tmp + 8 * RS_BINS;
+  uint32_t* goff = bhist + hb;
+  char* stmp = (char*)(goff + hb);
+  stmp = (char*)(((uintptr_t)stmp + 255) & ~(uintptr_t)255);
+  uint64_t* ks = keys;
+  uint32_t* vs = vals;
+  uint64_t* kd = keys_alt;
+  uint32_t* vd = vals_alt;
+  bool alt = false;
+  for (int shift = 0; shift < key_bits && shift < 64; shift += 8) {
+    hipLaunchKernelGGL(k_rs_upsweep, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
+    scan_exclusive_u32(bhist, goff, hb, nullptr, stmp, st);
+    hipLaunchKernelGGL(k_rs_downsweep, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
+    uint64_t* tk = ks; ks = kd; kd = tk;
+    uint32_t* tv = vs; vs = vd; vd = tv;
+    alt = !alt;
+  }
+  return alt;
+}
+
 void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_tracker_keys, dim3(grid_for(n)), dim3(256), 0, st, (const int32_t*)t, keys, vals, n);

@@ -237,6 +237,27 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, WAVE);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// 64x64 bit-matrix transpose across a wave: lane l holds row l (bit c = entry
+// (l, c)) and gets column l.  Six block-swap stages: at stage s, entry (l, c)
+// with bit s of l clear and bit s of c set trades places with (l^s, c^s).
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
+  constexpr uint64_t M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                             0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const int sft = 32 >> i;
+    const uint64_t y = shfl_xor64(x, sft);
+    x = (lane & sft) ? ((x & ~M[i]) | ((y >> sft) & M[i])) : ((x & M[i]) | ((y << sft) & ~M[i]));
+  }
+  return x;
+}
+
 __device__ __forceinline__ uint64_t wave_excl_u32(uint32_t v, uint32_t* tot) {
   uint32_t inc = wave_incl_scan(v);
   *tot = __shfl(inc, WAVE - 1, WAVE);
@@ -373,13 +394,11 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
   constexpr int CPB = ERO ? 4 : 2;    // coordinates per box and axis (erosion adds the shifted edges)
   constexpr int P = CPB * (NMAX + 1); // coordinate slots per axis
   constexpr int K = P / WAVE;         // slots per lane
-  constexpr int BPL = K / CPB;        // boxes per lane (blocked)
   constexpr int NCOL = P - 1;
   constexpr int NW = P / WAVE;        // words per column
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint64_t MA[NCOL * NW];
-  __shared__ uint64_t MX[(NCOL + 1) * NW];   // layer B mask, then start masks
-  __shared__ uint64_t ME[AREA ? 1 : (NCOL + 1) * NW];
+  __shared__ uint64_t MX[(TWO || ERO) ? NCOL * NW : 1];   // layer B mask / x-eroded mask
 
   const int lane = threadIdx.x;
   const uint64_t lt = lanemask_lt();
@@ -403,39 +422,37 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       }
       continue;
     }
-    // 1. clipped boxes (blocked: lane owns boxes BPL*lane .. +BPL-1) and
-    //    coordinates.  The clip window is the tile, grown by R when eroding.
+    // 1. coordinates: slot e = lane*K + k holds component e % CPB of box
+    //    e / CPB (x1, x2, and when eroding x1+R, x2-R clamped to the tile),
+    //    clipped to the window: the tile, grown by R when eroding.  Slot
+    //    group n holds the window and tile borders; the rest are padding.
     const int32_t R = ERO ? o.erode : 0;
     const int32_t wx0 = clamp64((int64_t)d.x0 - R, INT32_MIN, INT32_MAX);
     const int32_t wx1 = clamp64((int64_t)d.x1 + R, INT32_MIN, INT32_MAX);
     const int32_t wy0 = clamp64((int64_t)d.y0 - R, INT32_MIN, INT32_MAX);
     const int32_t wy1 = clamp64((int64_t)d.y1 + R, INT32_MIN, INT32_MAX);
     int32_t vx[K], vy[K];
-    int32_t bx1[BPL], by1[BPL], bx2[BPL], by2[BPL];
 #pragma unroll
-    for (int q = 0; q < BPL; q++) {
-      const int i = BPL * lane + q;
-      int32_t x1 = INT32_MAX, y1 = INT32_MAX, x2 = INT32_MAX, y2 = INT32_MAX;
-      int32_t sx1 = INT32_MAX, sy1 = INT32_MAX, sx2 = INT32_MAX, sy2 = INT32_MAX;
+    for (int k = 0; k < K; k++) {
+      const int e = lane * K + k;
+      const int i = e / CPB, comp = e % CPB;
+      int32_t cx = INT32_MAX, cy = INT32_MAX;
       if (i < n) {
         const vl_box b = entries[d.off + i];
-        x1 = max(b.x1, wx0); x2 = min(b.x2, wx1);
-        y1 = max(b.y1, wy0); y2 = min(b.y2, wy1);
-        if (ERO) {
-          sx1 = clamp64((int64_t)x1 + R, d.x0, d.x1); sx2 = clamp64((int64_t)x2 - R, d.x0, d.x1);
-          sy1 = clamp64((int64_t)y1 + R, d.y0, d.y1); sy2 = clamp64((int64_t)y2 - R, d.y0, d.y1);
-        }
+        const int32_t x1 = max(b.x1, wx0), x2 = min(b.x2, wx1);
+        const int32_t y1 = max(b.y1, wy0), y2 = min(b.y2, wy1);
+        if (comp == 0) { cx = x1; cy = y1; }
+        else if (comp == 1) { cx = x2; cy = y2; }
+        else if (comp == 2) { cx = clamp64((int64_t)x1 + R, d.x0, d.x1); cy = clamp64((int64_t)y1 + R, d.y0, d.y1); }
+        else { cx = clamp64((int64_t)x2 - R, d.x0, d.x1); cy = clamp64((int64_t)y2 - R, d.y0, d.y1); }
       } else if (i == n) {
-        x1 = wx0; x2 = wx1; y1 = wy0; y2 = wy1;
-        sx1 = d.x0; sx2 = d.x1; sy1 = d.y0; sy2 = d.y1;
+        if (comp == 0) { cx = wx0; cy = wy0; }
+        else if (comp == 1) { cx = wx1; cy = wy1; }
+        else if (comp == 2) { cx = d.x0; cy = d.y0; }
+        else { cx = d.x1; cy = d.y1; }
       }
-      bx1[q] = x1; bx2[q] = x2; by1[q] = y1; by2[q] = y2;
-      vx[CPB * q] = x1; vx[CPB * q + 1] = x2;
-      vy[CPB * q] = y1; vy[CPB * q + 1] = y2;
-      if (ERO) {
-        vx[CPB * q + 2] = sx1; vx[CPB * q + 3] = sx2;
-        vy[CPB * q + 2] = sy1; vy[CPB * q + 3] = sy2;
-      }
+      vx[k] = cx;
+      vy[k] = cy;
     }
     const int cnt = CPB * n + CPB;
     // bitonic sort of P elements, element e = lane*K + k
@@ -501,13 +518,11 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       if (TWO) MX[k] = 0ull;
     }
     __syncthreads();
-    // 2. compress + rasterise
-#pragma unroll
-    for (int q = 0; q < BPL; q++) {
-      const int i = BPL * lane + q;
-      if (i >= n) continue;
-      const int cx1 = lower_bound_i32(ux, nxu, bx1[q]), cx2 = lower_bound_i32(ux, nxu, bx2[q]);
-      const int cy1 = lower_bound_i32(uy, nyu, by1[q]), cy2 = lower_bound_i32(uy, nyu, by2[q]);
+    // 2. compress + rasterise (box-parallel: lane takes boxes lane, lane+64, ...)
+    for (int i = lane; i < n; i += WAVE) {
+      const vl_box b = entries[d.off + i];
+      const int cx1 = lower_bound_i32(ux, nxu, max(b.x1, wx0)), cx2 = lower_bound_i32(ux, nxu, min(b.x2, wx1));
+      const int cy1 = lower_bound_i32(uy, nyu, max(b.y1, wy0)), cy2 = lower_bound_i32(uy, nyu, min(b.y2, wy1));
       if (cx1 >= cx2 || cy1 >= cy2) continue;
       uint64_t* M = (twol && i >= nA) ? MX : MA;
       const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;
@@ -567,24 +582,29 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       __syncthreads();
       continue;
     }
-    // 3. start / end masks at every slab boundary k = 0..ncol; every strip
-    //    ends exactly once, so the popcount of the end masks is the strip count
+    // 3. start / end masks at every slab boundary k = 0..ncol, kept in
+    //    registers: lane owns boundaries k = 64*kb + lane (Sr/Er[kb][w], bit =
+    //    row).  Every strip ends exactly once, so the popcount of the end
+    //    masks is the strip count.
+    uint64_t Sr[NW][NW], Er[NW][NW];
     uint32_t myT = 0;
-    for (int k = lane; k <= ncol; k += WAVE) {
-      const uint64_t* cur = (k < ncol) ? MA + k * W : nullptr;
-      const uint64_t* prv = (k > 0) ? MA + (k - 1) * W : nullptr;
-      uint64_t D[NW], S[NW], E[NW];
-      change_mask<NW>(cur, prv, W, D);
-      starts_meeting<NW>(cur, W, D, S);
-      starts_meeting<NW>(prv, W, D, E);
-      for (int w = 0; w < W; w++) {
-        MX[k * W + w] = S[w];
-        ME[k * W + w] = E[w];
-        myT += (uint32_t)__popcll(E[w]);
+#pragma unroll
+    for (int kb = 0; kb < NW; kb++) {
+      const int k = kb * WAVE + lane;
+#pragma unroll
+      for (int w = 0; w < NW; w++) { Sr[kb][w] = 0ull; Er[kb][w] = 0ull; }
+      if (k <= ncol) {
+        const uint64_t* cur = (k < ncol) ? MA + k * W : nullptr;
+        const uint64_t* prv = (k > 0) ? MA + (k - 1) * W : nullptr;
+        uint64_t D[NW];
+        change_mask<NW>(cur, prv, W, D);
+        starts_meeting<NW>(cur, W, D, Sr[kb]);
+        starts_meeting<NW>(prv, W, D, Er[kb]);
+#pragma unroll
+        for (int w = 0; w < NW; w++) myT += (uint32_t)__popcll(Er[kb][w]);
       }
     }
     const uint32_t T = wave_sum(myT);
-    __syncthreads();
     if (T == 0) {
       if (lane == 0) {
         o.tab_n[ti] = 0;
@@ -596,63 +616,66 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     const bool ok = used + T <= o.region_cap;
     used += T;
     if (lane == 0) o.tab_off[ti] = base;
-    // 4. pair starts and ends.  Lane owns rows 64*w + lane; boundaries are
-    //    visited in order, 64 per chunk (one per lane, read into registers),
-    //    skipping boundaries with no start or end.
+    // 4. strips per row.  The S/E masks are transposed in 64x64 bit blocks so
+    //    that lane l holds, for row 64*wb + l, the boundaries at which a strip
+    //    of that row starts (rs) and ends (re).  A strip starting at j ends at
+    //    the first end bit after j: between the two its run is unchanged.
+    //    Output order: row block, lane, start boundary.
     const uint32_t flags = d.flags;
-    int oj[NW], ob[NW];
+    const int KW = (ncol + 1 + 63) >> 6;   // boundary words per row
+    uint32_t r = 0, my_side = 0;
 #pragma unroll
-    for (int w = 0; w < NW; w++) { oj[w] = 0; ob[w] = 0; }
-    uint32_t r = 0, nside = 0;
-    for (int k0 = 0; k0 <= ncol; k0 += WAVE) {
-      const int kk = k0 + lane;
-      uint64_t ev[NW], sv[NW];
-      bool any = false;
+    for (int wb = 0; wb < NW; wb++) {
+      if (wb >= W) break;
+      uint64_t rs[NW], re[NW];
+      uint32_t c = 0;
 #pragma unroll
-      for (int w = 0; w < NW; w++) {
-        ev[w] = 0;
-        sv[w] = 0;
-        if (w < W && kk <= ncol) {
-          ev[w] = ME[kk * W + w];
-          sv[w] = MX[kk * W + w];
+      for (int kb = 0; kb < NW; kb++) {
+        rs[kb] = 0ull;
+        re[kb] = 0ull;
+        if (kb < KW) {
+          rs[kb] = transpose64(Sr[kb][wb], lane);
+          re[kb] = transpose64(Er[kb][wb], lane);
+          c += (uint32_t)__popcll(rs[kb]);
         }
-        any |= (ev[w] | sv[w]) != 0;
       }
-      uint64_t act = __ballot(any);
-      while (act) {
-        const int l = __ffsll((unsigned long long)act) - 1;
-        act &= act - 1;
-        const int k = k0 + l;
+      uint32_t tot;
+      const uint32_t ex = (uint32_t)wave_excl_u32(c, &tot);
+      uint64_t pos = base + r + ex;
+      const int a = wb * WAVE + lane;
 #pragma unroll
-        for (int w = 0; w < NW; w++) {
-          if (w >= W) break;
-          const uint64_t e = readlane64(ev[w], l), s = readlane64(sv[w], l);
-          if ((e | s) == 0) continue;
-          const int a = w * 64 + lane;
-          const bool endm = (e >> lane) & 1ull;
-          uint32_t tg = 0;
-          if (endm)
-            tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                 ((ob[w] == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                 ((oj[w] == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                 ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
-          const uint64_t be = __ballot(endm), bs = __ballot(endm && tg != 0);
-          if (endm && ok) {
-            const uint64_t p = base + r + (uint32_t)__popcll(be & lt);
+      for (int kb = 0; kb < NW; kb++) {
+        uint64_t m = rs[kb];
+        while (m) {
+          const int j = kb * WAVE + __ffsll((unsigned long long)m) - 1;
+          m &= m - 1;
+          int k = ncol;
+#pragma unroll
+          for (int q = NW - 1; q >= 0; q--) {
+            const int t = j - q * WAVE;   // bits of word q above j
+            uint64_t mm = re[q];
+            if (t >= 63) mm = 0ull;
+            else if (t >= 0) mm &= ~0ull << (t + 1);
+            if (mm) k = q * WAVE + __ffsll((unsigned long long)mm) - 1;
+          }
+          const int b = run_end(MA + j * W, W, a);
+          const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                              ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                              ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                              ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+          if (ok) {
             vl_box st;
-            st.x1 = ux[oj[w]]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[ob[w]];
-            o.strips[p] = st;
-            o.tag[p] = (uint8_t)tg;
+            st.x1 = ux[j]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[b];
+            o.strips[pos] = st;
+            o.tag[pos] = (uint8_t)tg;
           }
-          r += (uint32_t)__popcll(be);
-          nside += (uint32_t)__popcll(bs);
-          if ((s >> lane) & 1ull) {
-            oj[w] = k;
-            ob[w] = run_end(MA + k * W, W, a);
-          }
+          pos++;
+          my_side += tg != 0u;
         }
       }
+      r += tot;
     }
+    const uint32_t nside = (uint32_t)wave_sum((long long)my_side);
     side_total += nside;
     if (lane == 0) {
       o.tab_n[ti] = T - nside;
@@ -1114,9 +1137,11 @@ void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc
 void launch_tile_wave(int nmax, bool two_layers, bool area_mode, bool erode, const TileDesc* desc, int ntiles,
                       const vl_box* entries, TileOut o, int wg, hipStream_t st) {
   if (erode) {
-    launch_w<63, false, false, true>(desc, ntiles, entries, o, wg, st);
+    if (nmax == 15) launch_w<15, false, false, true>(desc, ntiles, entries, o, wg, st);
+    else launch_w<31, false, false, true>(desc, ntiles, entries, o, wg, st);
     return;
   }
+  VL_W_DISPATCH(31)
   VL_W_DISPATCH(63)
   VL_W_DISPATCH(127)
 }
@@ -1131,12 +1156,15 @@ void launch_tile_wave(int nmax, bool two_layers, bool area_mode, bool erode, con
   }
 
 int tile_wave_max_wg(int nmax, bool two_layers, bool area_mode, bool erode) {
-  static int cache[2][2][2][2] = {};
-  const int ni = nmax == 63 ? 0 : 1;
+  static int cache[4][2][2][2] = {};
+  const int ni = nmax == 15 ? 0 : nmax == 31 ? 1 : nmax == 63 ? 2 : 3;
   int& c = cache[ni][two_layers][area_mode][erode];
   if (c) return c;
   auto f = [&]() -> int {
-    if (erode) return occupancy_wg(k_tile_w<63, false, false, true>, WAVE);
+    if (erode)
+      return nmax == 15 ? occupancy_wg(k_tile_w<15, false, false, true>, WAVE)
+                        : occupancy_wg(k_tile_w<31, false, false, true>, WAVE);
+    VL_W_MAXWG(31)
     VL_W_MAXWG(63)
     VL_W_MAXWG(127)
     return 1024;
