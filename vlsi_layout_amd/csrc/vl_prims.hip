@@ -159,7 +159,7 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st) {
 void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st) {
   if (n <= 0) return;
   int64_t g = (n + SCAN_NT - 1) / SCAN_NT;
-  if (g > 2048) g = 2048;
+  if (g > 1024) g = 1024;   // one set of 4 atomics per workgroup on shared addresses
   hipLaunchKernelGGL(k_mbr, dim3((unsigned)g), dim3(SCAN_NT), 0, st, b, n, xf, mbr, err);
 }
 

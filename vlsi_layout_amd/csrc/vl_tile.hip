@@ -387,6 +387,20 @@ __device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const 
 }
 
 // ---- wave-per-tile kernel --------------------------------------------------------
+constexpr int TILE_BATCH = 4;   // tiles per load batch of the wave kernel
+constexpr int OVF_BUF = 256;    // overflow tiles buffered in LDS per workgroup
+
+// Appends the n buffered overflow tiles to the launch's overflow list: one
+// atomic per flush of the workgroup, not one per tile (a single counter
+// address serialises its atomics across the whole device).
+__device__ __forceinline__ void flush_overflow(const uint32_t* buf, uint32_t n, const TileOut& o, int t, int nt) {
+  __shared__ uint32_t s_base;
+  if (t == 0) s_base = atomicAdd(o.ovf_n, n);
+  __syncthreads();
+  const uint32_t base = s_base;
+  for (uint32_t k = t; k < n; k += nt) o.ovf[base + k] = buf[k];
+  __syncthreads();
+}
 
 template <int NMAX, bool TWO, bool AREA, bool ERO>
 __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc, int ntiles,
@@ -399,28 +413,146 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint64_t MA[NCOL * NW];
   __shared__ uint64_t MX[(TWO || ERO) ? NCOL * NW : 1];   // layer B mask / x-eroded mask
+  constexpr int NL = (NMAX + WAVE) / WAVE;                // box loads per lane and tile
+  __shared__ vl_box sbuf[TILE_BATCH * (NMAX + 1)];        // the batch's boxes
+  __shared__ TileDesc sdesc[TILE_BATCH];                  // the batch's descriptors
+  uint32_t* const sdw = (uint32_t*)sdesc;                 // ... as 10 dwords each
+  __shared__ vl_box sk[TWO ? NMAX + 1 : 1];               // boxes kept after pruning
+  __shared__ uint32_t sovf[OVF_BUF];                      // overflow tiles not yet flushed
+  static_assert(TILE_BATCH * 10 <= WAVE, "one descriptor dword per lane");
 
   const int lane = threadIdx.x;
-  const uint64_t lt = lanemask_lt();
   const int nt = o.list_n ? (int)*o.list_n : ntiles;
   const uint64_t rbase = (uint64_t)(o.region_base + blockIdx.x) * o.region_cap;
   uint64_t used = 0, side_total = 0;
-  for (int tk = blockIdx.x; tk < nt; tk += gridDim.x) {
-    const uint32_t ti = o.list ? o.list[tk] : (uint32_t)tk;
-    const TileDesc d = desc[ti];
-    const int nA = d.nA;
-    const int n = d.nA + d.nB;
+  uint32_t novf = 0;        // entries in sovf (uniform)
+  long long area_acc = 0;   // area mode: this workgroup's share (uniform)
+  // Workgroup g takes batches of TILE_BATCH consecutive tiles (list entries)
+  // g, g+G, ...; the descriptors and boxes of a batch are loaded together, so
+  // the memory latency is paid once per batch rather than once per tile.
+  for (int b0 = blockIdx.x * TILE_BATCH; b0 < nt; b0 += gridDim.x * TILE_BATCH) {
+    const int nbt = min(TILE_BATCH, nt - b0);
+    uint32_t my_ti = 0;
+    if (lane < nbt) my_ti = o.list ? o.list[b0 + lane] : (uint32_t)(b0 + lane);
+    {
+      const int dj = lane / 10, dw = lane - 10 * (lane / 10);
+      const uint32_t tj = (uint32_t)__shfl((int)my_ti, dj < TILE_BATCH ? dj : 0, WAVE);
+      if (lane < nbt * 10) sdw[lane] = ((const uint32_t*)desc)[(size_t)tj * 10 + dw];
+    }
+    __syncthreads();
+    {
+      int4 v[TILE_BATCH * NL];
+      int nj[TILE_BATCH];
+#pragma unroll
+      for (int j = 0; j < TILE_BATCH; j++) {
+        const int64_t off = sdesc[j].off;
+        const int c = (j < nbt) ? sdesc[j].nA + sdesc[j].nB : 0;
+        nj[j] = c <= NMAX ? c : 0;
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+          const int i = lane + q * WAVE;
+          v[j * NL + q] = make_int4(0, 0, 0, 0);
+          if (i < nj[j]) v[j * NL + q] = ((const int4*)entries)[off + i];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TILE_BATCH; j++) {
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+          const int i = lane + q * WAVE;
+          if (i < nj[j]) ((int4*)sbuf)[j * (NMAX + 1) + i] = v[j * NL + q];
+        }
+      }
+    }
+    __syncthreads();
+  for (int jb = 0; jb < nbt; jb++) {
+    const uint32_t ti = (uint32_t)__shfl((int)my_ti, jb, WAVE);
+    const TileDesc d = sdesc[jb];
+    vl_box* sb = sbuf + jb * (NMAX + 1);
+    int nA = d.nA;
+    int n = d.nA + d.nB;
     const int op = d.op;
     const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR);
     if (n > NMAX || n == 0) {
       if (lane == 0) {
-        if (n > NMAX) o.ovf[atomicAdd(o.ovf_n, 1u)] = ti;
+        if (n > NMAX) sovf[novf] = ti;
         if (!AREA) {
           o.tab_n[ti] = 0;
           o.tab_n[o.ntab + ti] = 0;
         }
       }
+      if (n > NMAX && ++novf == OVF_BUF) {
+        __syncthreads();
+        flush_overflow(sovf, novf, o, lane, WAVE);
+        novf = 0;
+        __syncthreads();
+      }
       continue;
+    }
+    // 0. the boxes are in LDS.  AND and NOT prune first: a box of B that
+    //    meets no box of A inside the tile cannot change A AND B or A \ B,
+    //    and neither can a box of A that meets no box of B change A AND B.
+    //    An AND tile without any such pair has an empty result.
+    const vl_box* bx = sb;
+    if (TWO && (op == TOP_AND || op == TOP_NOT)) {
+      constexpr bool TWOREG = NMAX >= WAVE;   // boxes lane and lane+64
+      const int nB = n - nA;
+      const uint64_t lt = lanemask_lt();
+      // lane l holds A boxes l (and l+64) and B boxes l (and l+64); the A
+      // boxes are clipped to the tile for the test and broadcast by readlane
+      vl_box a1{0, 0, 0, 0}, a2{0, 0, 0, 0}, b1{0, 0, 0, 0}, b2{0, 0, 0, 0};
+      if (lane < nA) a1 = sb[lane];
+      if (lane < nB) b1 = sb[nA + lane];
+      if (TWOREG) {
+        if (lane + WAVE < nA) a2 = sb[lane + WAVE];
+        if (lane + WAVE < nB) b2 = sb[nA + lane + WAVE];
+      }
+      const int32_t c1x1 = max(a1.x1, d.x0), c1x2 = min(a1.x2, d.x1), c1y1 = max(a1.y1, d.y0), c1y2 = min(a1.y2, d.y1);
+      const int32_t c2x1 = max(a2.x1, d.x0), c2x2 = min(a2.x2, d.x1), c2y1 = max(a2.y1, d.y0), c2y2 = min(a2.y2, d.y1);
+      uint64_t hitA1 = 0ull, hitA2 = 0ull;
+      bool keepB1 = false, keepB2 = false;
+      for (int i = 0; i < nA; i++) {
+        const bool lo = i < WAVE;
+        const int src = i & (WAVE - 1);
+        const int32_t ax1 = __builtin_amdgcn_readlane(lo ? c1x1 : c2x1, src);
+        const int32_t ax2 = __builtin_amdgcn_readlane(lo ? c1x2 : c2x2, src);
+        const int32_t ay1 = __builtin_amdgcn_readlane(lo ? c1y1 : c2y1, src);
+        const int32_t ay2 = __builtin_amdgcn_readlane(lo ? c1y2 : c2y2, src);
+        bool h = lane < nB && max(ax1, b1.x1) < min(ax2, b1.x2) && max(ay1, b1.y1) < min(ay2, b1.y2);
+        keepB1 |= h;
+        if (TWOREG) {
+          const bool h2 = lane + WAVE < nB && max(ax1, b2.x1) < min(ax2, b2.x2) && max(ay1, b2.y1) < min(ay2, b2.y2);
+          keepB2 |= h2;
+          h |= h2;
+        }
+        if (__ballot(h)) {
+          if (lo) hitA1 |= 1ull << src;
+          else hitA2 |= 1ull << src;
+        }
+      }
+      // compact the kept boxes (A first, then B), keeping their order
+      const bool kA1 = lane < nA && (op == TOP_NOT || ((hitA1 >> lane) & 1ull));
+      const bool kA2 = TWOREG && lane + WAVE < nA && (op == TOP_NOT || ((hitA2 >> lane) & 1ull));
+      const uint64_t mA1 = __ballot(kA1), mA2 = __ballot(kA2);
+      const uint64_t mB1 = __ballot(keepB1), mB2 = __ballot(keepB2);
+      const int cA1 = __popcll(mA1), cA = cA1 + __popcll(mA2);
+      const int cB1 = __popcll(mB1), cB = cB1 + __popcll(mB2);
+      if (kA1) sk[__popcll(mA1 & lt)] = a1;
+      if (kA2) sk[cA1 + __popcll(mA2 & lt)] = a2;
+      if (keepB1) sk[cA + __popcll(mB1 & lt)] = b1;
+      if (keepB2) sk[cA + cB1 + __popcll(mB2 & lt)] = b2;
+      __syncthreads();
+      nA = cA;
+      n = cA + cB;
+      bx = sk;
+      if (n == 0 || (op == TOP_AND && (cA == 0 || cB == 0))) {
+        if (lane == 0) {
+          o.tab_n[ti] = 0;
+          o.tab_n[o.ntab + ti] = 0;
+        }
+        __syncthreads();
+        continue;
+      }
     }
     // 1. coordinates: slot e = lane*K + k holds component e % CPB of box
     //    e / CPB (x1, x2, and when eroding x1+R, x2-R clamped to the tile),
@@ -438,7 +570,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       const int i = e / CPB, comp = e % CPB;
       int32_t cx = INT32_MAX, cy = INT32_MAX;
       if (i < n) {
-        const vl_box b = entries[d.off + i];
+        const vl_box b = bx[i];
         const int32_t x1 = max(b.x1, wx0), x2 = min(b.x2, wx1);
         const int32_t y1 = max(b.y1, wy0), y2 = min(b.y2, wy1);
         if (comp == 0) { cx = x1; cy = y1; }
@@ -520,7 +652,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     __syncthreads();
     // 2. compress + rasterise (box-parallel: lane takes boxes lane, lane+64, ...)
     for (int i = lane; i < n; i += WAVE) {
-      const vl_box b = entries[d.off + i];
+      const vl_box b = bx[i];
       const int cx1 = lower_bound_i32(ux, nxu, max(b.x1, wx0)), cx2 = lower_bound_i32(ux, nxu, min(b.x2, wx1));
       const int cy1 = lower_bound_i32(uy, nyu, max(b.y1, wy0)), cy2 = lower_bound_i32(uy, nyu, min(b.y2, wy1));
       if (cx1 >= cx2 || cy1 >= cy2) continue;
@@ -577,8 +709,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
         if (a >= 0) h += (long long)uy[W * 64] - uy[a];
         acc += h * ((long long)ux[j + 1] - ux[j]);
       }
-      acc = wave_sum(acc);
-      if (lane == 0 && acc) atomicAdd((unsigned long long*)o.area, (unsigned long long)acc);
+      area_acc += wave_sum(acc);
       __syncthreads();
       continue;
     }
@@ -683,6 +814,13 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     }
     __syncthreads();
   }
+    __syncthreads();
+  }
+  if (novf) {
+    __syncthreads();
+    flush_overflow(sovf, novf, o, lane, WAVE);
+  }
+  if (lane == 0 && AREA && area_acc) atomicAdd((unsigned long long*)o.area, (unsigned long long)area_acc);
   if (lane == 0 && !AREA && used) {
     atomicAdd(&o.cnt[0], (unsigned long long)used);
     if (side_total) atomicAdd(&o.cnt[1], (unsigned long long)side_total);
@@ -710,10 +848,13 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
   __shared__ uint64_t wsU[NT / WAVE];
   __shared__ long long wsA[NT / WAVE];
 
+  __shared__ uint32_t sovf[OVF_BUF];
   const int tid = threadIdx.x;
   const int nt = o.list_n ? (int)*o.list_n : ntiles;
   const uint64_t rbase = (uint64_t)(o.region_base + blockIdx.x) * o.region_cap;
   uint64_t used = 0, side_total = 0;   // uniform across the workgroup
+  uint32_t novf = 0;
+  long long area_acc = 0;
   for (int tk = blockIdx.x; tk < nt; tk += gridDim.x) {
     const uint32_t ti = o.list ? o.list[tk] : (uint32_t)tk;
     const TileDesc d = desc[ti];
@@ -723,13 +864,17 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
     const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR);
     if (n > NMAX || n == 0) {
       if (tid == 0) {
-        if (n > NMAX) o.ovf[atomicAdd(o.ovf_n, 1u)] = ti;
+        if (n > NMAX) sovf[novf] = ti;
         if (!AREA) {
           o.tab_n[ti] = 0;
           o.tab_n[o.ntab + ti] = 0;
         }
       }
       __syncthreads();
+      if (n > NMAX && ++novf == OVF_BUF) {
+        flush_overflow(sovf, novf, o, tid, NT);
+        novf = 0;
+      }
       continue;
     }
     uint64_t F = 0, S = 0;
@@ -893,11 +1038,9 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
       long long a = wave_sum(my_area);
       if (__lane_id() == 0) wsA[tid / WAVE] = a;
       __syncthreads();
-      if (tid == 0) {
-        long long s = 0;
-        for (int i = 0; i < NT / WAVE; i++) s += wsA[i];
-        if (s) atomicAdd((unsigned long long*)o.area, (unsigned long long)s);
-      }
+      long long s = 0;
+      for (int i = 0; i < NT / WAVE; i++) s += wsA[i];
+      area_acc += s;
       __syncthreads();
       continue;
     }
@@ -946,6 +1089,11 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
     }
     __syncthreads();
   }
+  if (novf) {
+    __syncthreads();
+    flush_overflow(sovf, novf, o, tid, NT);
+  }
+  if (tid == 0 && AREA && area_acc) atomicAdd((unsigned long long*)o.area, (unsigned long long)area_acc);
   if (tid == 0 && !AREA && used) {
     atomicAdd(&o.cnt[0], (unsigned long long)used);
     if (side_total) atomicAdd(&o.cnt[1], (unsigned long long)side_total);
