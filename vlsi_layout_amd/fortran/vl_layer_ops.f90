@@ -1,0 +1,548 @@
+! vl_layer_ops.f90 — the reference's Fortran layer-operator surface, backed by
+! the gfx950 engine (libvlayout.so) through its C ABI (include/vl_api.h).
+!
+! Procedure names, argument order and meaning follow the reference:
+!   CalculateAND, CalculateFrameNOT, CalculateNOT, CalculateOR,
+!   CalculateGROWLayer, CalculateBoostOperation, MergeHealLayer
+!                                   src/design.f90:20-31, interfaces :154-313
+!   heal_boxes                      src/polygon_fracture.f90:805-808
+!   sort_event_trackers             src/event_sorter_apif.f90:30-43
+!   SortBoxesDirect/SortBoxesIndirect  src/gpu_morton_sort.f90:41-77
+!   K_BOOST_CONTROL_* codes         src/commonmodule.f90:12-14
+! The Box type is the reference's {x1,y1,x2,y2} int32 rectangle made
+! interoperable (bind(C)), so a Layer's box array is passed to the C ABI as is.
+!
+! Layers live on the host (layer_boxes(1:n_used)) as in the reference, and
+! additionally keep a device-resident copy: an operator uploads an input only
+! when its device copy is stale, and the result stays on the device.  With
+! vl_eager_host = .true. (the default) every result is also copied back into
+! layer_boxes, which is the reference's contract; with .false. a caller
+! running a chain of operators skips those copies and calls layer_to_host
+! when it needs the boxes.  A caller that edits layer_boxes directly calls
+! layer_boxes_changed afterwards.
+!
+! The reference stops with `error stop` on every failure; so does this module,
+! after printing the engine's error text.  Results are the canonical
+! maximal-strip decomposition (the point set the reference computes; box order
+! may differ from the reference's).
+module vl_layer_ops
+  use iso_c_binding
+  use iso_fortran_env, only: int8, int32, int64, error_unit
+  implicit none
+  private
+
+  public :: Box, XYTracker, Layer
+  public :: CalculateAND, CalculateFrameNOT, CalculateNOT, CalculateOR, CalculateXOR
+  public :: CalculateGROWLayer, CalculateBoostOperation, MergeHealLayer, heal_boxes
+  public :: sort_event_trackers, SortBoxesDirect, SortBoxesIndirect
+  public :: LayerAssertZero, LayerKleeArea, LayerXorEmpty, CopyLayer
+  public :: layer_set_boxes, layer_to_host, layer_boxes_changed, layer_release
+  public :: vl_fortran_init, vl_fortran_finalize, vl_fortran_sync, vl_engine_handle
+  public :: K_BOOST_CONTROL_XOR, K_BOOST_CONTROL_OR, K_BOOST_CONTROL_AND, K_BOOST_CONTROL_NOT
+  public :: K_BOOST_CONTROL_MERGE, K_BOOST_CONTROL_SIZE
+  public :: LAYER_STATE_NONE, LAYER_STATE_HEAL
+
+  integer(int64), parameter :: K_BOOST_CONTROL_XOR = 0, K_BOOST_CONTROL_OR = 1, &
+       K_BOOST_CONTROL_AND = 2, K_BOOST_CONTROL_NOT = 3, K_BOOST_CONTROL_MERGE = 4, &
+       K_BOOST_CONTROL_SIZE = 5
+  integer(c_int), parameter :: LAYER_STATE_NONE = 0, LAYER_STATE_HEAL = 1
+
+  type, bind(C) :: Box
+     integer(c_int32_t) :: x1 = 0, y1 = 0, x2 = 0, y2 = 0
+  end type Box
+
+  ! 12-byte event record of src/commonmodule.f90:30-34 (int32 X, Y; int8 winding)
+  type, bind(C) :: XYTracker
+     integer(c_int32_t) :: X = 0, Y = 0
+     integer(c_int8_t) :: polygonNumber = 0
+  end type XYTracker
+
+  type :: Layer
+     integer(int64) :: n_used = 0
+     integer(int64) :: n_alloc = 0
+     type(Box), allocatable :: layer_boxes(:)
+     integer(c_int) :: layerState = LAYER_STATE_NONE
+     type(c_ptr) :: dev = c_null_ptr      ! the engine's layer handle
+     logical :: dev_valid = .false.        ! the device copy holds the layer
+     logical :: host_valid = .true.        ! layer_boxes(1:n_used) holds the layer
+  end type Layer
+
+  logical, public :: vl_eager_host = .true.
+  integer(c_int), public :: vl_device = 0
+  type(c_ptr), save :: engine = c_null_ptr
+
+  ! C ABI (include/vl_api.h)
+  integer(c_int), parameter :: VL_OK = 0
+  interface
+     function c_last_error() bind(C, name="vl_last_error") result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function c_last_error
+     function c_engine_create(device, e) bind(C, name="vl_engine_create") result(rc)
+       import :: c_int, c_ptr
+       integer(c_int), value :: device
+       type(c_ptr) :: e
+       integer(c_int) :: rc
+     end function c_engine_create
+     function c_engine_destroy(e) bind(C, name="vl_engine_destroy") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e
+       integer(c_int) :: rc
+     end function c_engine_destroy
+     function c_engine_sync(e) bind(C, name="vl_engine_sync") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e
+       integer(c_int) :: rc
+     end function c_engine_sync
+     function c_layer_create(e, l) bind(C, name="vl_layer_create") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e
+       type(c_ptr) :: l
+       integer(c_int) :: rc
+     end function c_layer_create
+     function c_layer_destroy(l) bind(C, name="vl_layer_destroy") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: l
+       integer(c_int) :: rc
+     end function c_layer_destroy
+     function c_layer_upload(l, host, n) bind(C, name="vl_layer_upload") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: l, host
+       integer(c_int64_t), value :: n
+       integer(c_int) :: rc
+     end function c_layer_upload
+     function c_layer_download(l, host, cap, n) bind(C, name="vl_layer_download") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: l, host
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t) :: n
+       integer(c_int) :: rc
+     end function c_layer_download
+     function c_layer_size(l, n) bind(C, name="vl_layer_size") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: l
+       integer(c_int64_t) :: n
+       integer(c_int) :: rc
+     end function c_layer_size
+     function c_grow(e, a, east, north, west, south, c) bind(C, name="vl_grow") result(rc)
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e, a, c
+       integer(c_int32_t), value :: east, north, west, south
+       integer(c_int) :: rc
+     end function c_grow
+     function c_size(e, a, d, c) bind(C, name="vl_size") result(rc)
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e, a, c
+       integer(c_int32_t), value :: d
+       integer(c_int) :: rc
+     end function c_size
+     function c_merge(e, a, c) bind(C, name="vl_merge") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, c
+       integer(c_int) :: rc
+     end function c_merge
+     function c_canonicalize(e, a, c) bind(C, name="vl_canonicalize") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, c
+       integer(c_int) :: rc
+     end function c_canonicalize
+     function c_klee_area(e, a, area) bind(C, name="vl_klee_area") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: e, a
+       integer(c_int64_t) :: area
+       integer(c_int) :: rc
+     end function c_klee_area
+     function c_xor_empty(e, a, b, empty) bind(C, name="vl_xor_empty") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, b
+       integer(c_int) :: empty
+       integer(c_int) :: rc
+     end function c_xor_empty
+     subroutine device_event_sort(d_arr, n) bind(C, name="device_event_sort")
+       import :: c_ptr, c_int64_t
+       type(c_ptr), value :: d_arr
+       integer(c_int64_t), value :: n
+     end subroutine device_event_sort
+     subroutine thrust_sort_boxes_direct(d_boxes, n) bind(C, name="thrust_sort_boxes_direct")
+       import :: c_ptr, c_int64_t
+       type(c_ptr), value :: d_boxes
+       integer(c_int64_t), value :: n
+     end subroutine thrust_sort_boxes_direct
+     subroutine thrust_sort_boxes_indirect(d_boxes, d_indices, n) bind(C, name="thrust_sort_boxes_indirect")
+       import :: c_ptr, c_int64_t
+       type(c_ptr), value :: d_boxes, d_indices
+       integer(c_int64_t), value :: n
+     end subroutine thrust_sort_boxes_indirect
+  end interface
+
+  abstract interface
+     function binop_fn(e, a, b, c) bind(C) result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, b, c
+       integer(c_int) :: rc
+     end function binop_fn
+  end interface
+  interface
+     function c_and(e, a, b, c) bind(C, name="vl_and") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, b, c
+       integer(c_int) :: rc
+     end function c_and
+     function c_not(e, a, b, c) bind(C, name="vl_not") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, b, c
+       integer(c_int) :: rc
+     end function c_not
+     function c_or(e, a, b, c) bind(C, name="vl_or") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, b, c
+       integer(c_int) :: rc
+     end function c_or
+     function c_xor(e, a, b, c) bind(C, name="vl_xor") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, b, c
+       integer(c_int) :: rc
+     end function c_xor
+  end interface
+
+contains
+
+  ! ---- engine ---------------------------------------------------------------
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: i, n
+    if (rc == VL_OK) return
+    p = c_last_error()
+    write(error_unit, '(A,A,I0)', advance='no') trim(what), ': engine status ', rc
+    if (c_associated(p)) then
+       call c_f_pointer(p, msg, [4096])
+       n = 0
+       do i = 1, 4096
+          if (msg(i) == c_null_char) exit
+          n = i
+       end do
+       write(error_unit, '(A)', advance='no') ': '
+       do i = 1, n
+          write(error_unit, '(A)', advance='no') msg(i)
+       end do
+    end if
+    write(error_unit, *)
+    error stop "vl_layer_ops: engine call failed"
+  end subroutine check
+
+  subroutine vl_fortran_init(device)
+    integer, intent(in), optional :: device
+    if (c_associated(engine)) return
+    if (present(device)) vl_device = int(device, c_int)
+    call check(c_engine_create(vl_device, engine), "vl_engine_create")
+  end subroutine vl_fortran_init
+
+  function vl_engine_handle() result(e)
+    type(c_ptr) :: e
+    call vl_fortran_init()
+    e = engine
+  end function vl_engine_handle
+
+  subroutine vl_fortran_sync()
+    if (c_associated(engine)) call check(c_engine_sync(engine), "vl_engine_sync")
+  end subroutine vl_fortran_sync
+
+  ! Destroys the engine.  Layers still holding device copies keep the engine
+  ! alive until layer_release is called on them.
+  subroutine vl_fortran_finalize()
+    if (.not. c_associated(engine)) return
+    call check(c_engine_destroy(engine), "vl_engine_destroy")
+    engine = c_null_ptr
+  end subroutine vl_fortran_finalize
+
+  ! ---- layer residency ------------------------------------------------------
+
+  subroutine ensure_handle(L)
+    type(Layer), intent(inout) :: L
+    call vl_fortran_init()
+    if (.not. c_associated(L%dev)) call check(c_layer_create(engine, L%dev), "vl_layer_create")
+  end subroutine ensure_handle
+
+  subroutine to_device(L)
+    type(Layer), intent(inout), target :: L
+    call ensure_handle(L)
+    if (L%dev_valid) return
+    if (L%n_used > 0) then
+       if (.not. allocated(L%layer_boxes)) error stop "vl_layer_ops: layer has n_used > 0 but no boxes"
+       if (size(L%layer_boxes, kind=int64) < L%n_used) error stop "vl_layer_ops: n_used exceeds layer_boxes"
+       call check(c_layer_upload(L%dev, c_loc(L%layer_boxes(1)), L%n_used), "vl_layer_upload")
+    else
+       call check(c_layer_upload(L%dev, c_null_ptr, 0_c_int64_t), "vl_layer_upload")
+    end if
+    L%dev_valid = .true.
+  end subroutine to_device
+
+  ! Copies the device copy into layer_boxes (grown when too small).
+  subroutine layer_to_host(L)
+    type(Layer), intent(inout), target :: L
+    integer(c_int64_t) :: n
+    if (L%host_valid) return
+    if (.not. L%dev_valid) error stop "vl_layer_ops: layer has no valid copy"
+    call check(c_layer_size(L%dev, n), "vl_layer_size")
+    if (.not. allocated(L%layer_boxes)) then
+       allocate(L%layer_boxes(max(n, 1_int64)))
+    else if (size(L%layer_boxes, kind=int64) < n) then
+       deallocate(L%layer_boxes)
+       allocate(L%layer_boxes(n))
+    end if
+    L%n_alloc = size(L%layer_boxes, kind=int64)
+    if (n > 0) call check(c_layer_download(L%dev, c_loc(L%layer_boxes(1)), L%n_alloc, n), "vl_layer_download")
+    L%n_used = n
+    L%host_valid = .true.
+  end subroutine layer_to_host
+
+  ! Replaces the layer's boxes with b(1:n) (host copy; uploaded on next use).
+  subroutine layer_set_boxes(L, b, n)
+    type(Layer), intent(inout) :: L
+    type(Box), intent(in) :: b(:)
+    integer(int64), intent(in) :: n
+    if (allocated(L%layer_boxes)) deallocate(L%layer_boxes)
+    allocate(L%layer_boxes(max(n, 1_int64)))
+    if (n > 0) L%layer_boxes(1:n) = b(1:n)
+    L%n_used = n
+    L%n_alloc = size(L%layer_boxes, kind=int64)
+    L%layerState = LAYER_STATE_NONE
+    L%host_valid = .true.
+    L%dev_valid = .false.
+  end subroutine layer_set_boxes
+
+  ! Call after editing layer_boxes or n_used directly.
+  subroutine layer_boxes_changed(L)
+    type(Layer), intent(inout) :: L
+    L%host_valid = .true.
+    L%dev_valid = .false.
+    L%layerState = LAYER_STATE_NONE
+  end subroutine layer_boxes_changed
+
+  subroutine layer_release(L)
+    type(Layer), intent(inout) :: L
+    if (c_associated(L%dev)) call check(c_layer_destroy(L%dev), "vl_layer_destroy")
+    L%dev = c_null_ptr
+    L%dev_valid = .false.
+    if (.not. L%host_valid) then
+       L%n_used = 0
+       L%host_valid = .true.
+    end if
+  end subroutine layer_release
+
+  ! The result of an operator is on the device; record it (and copy it back
+  ! in eager mode).  Results are canonical, i.e. healed.
+  subroutine finish_output(C)
+    type(Layer), intent(inout) :: C
+    integer(c_int64_t) :: n
+    call check(c_layer_size(C%dev, n), "vl_layer_size")
+    C%n_used = n
+    C%dev_valid = .true.
+    C%host_valid = .false.
+    C%layerState = LAYER_STATE_HEAL
+    if (vl_eager_host) call layer_to_host(C)
+  end subroutine finish_output
+
+  subroutine binop(f, what, A, B, C)
+    procedure(binop_fn) :: f
+    character(len=*), intent(in) :: what
+    type(Layer), intent(inout) :: A, B, C
+    call to_device(A)
+    call to_device(B)
+    call ensure_handle(C)
+    call check(f(engine, A%dev, B%dev, C%dev), what)
+    call finish_output(C)
+  end subroutine binop
+
+  ! ---- operators (src/design.f90 surface) -----------------------------------
+
+  ! C = A AND B  ('*', src/design.f90:1131-1235)
+  subroutine CalculateAND(A, B, C)
+    type(Layer), intent(inout) :: A, B, C
+    call binop(c_and, "CalculateAND", A, B, C)
+  end subroutine CalculateAND
+
+  ! C = A \ B  ('~', src/design_impl.f90:1152-1164)
+  subroutine CalculateFrameNOT(A, B, C)
+    type(Layer), intent(inout) :: A, B, C
+    call binop(c_not, "CalculateFrameNOT", A, B, C)
+  end subroutine CalculateFrameNOT
+
+  ! C = A \ B  ('@', src/design.f90:1238-1268: complement of B in MBR(A), AND A)
+  subroutine CalculateNOT(A, B, C)
+    type(Layer), intent(inout) :: A, B, C
+    call binop(c_not, "CalculateNOT", A, B, C)
+  end subroutine CalculateNOT
+
+  ! C = A OR B  (OR2, src/design.f90:1085-1103)
+  subroutine CalculateOR(A, B, C)
+    type(Layer), intent(inout) :: A, B, C
+    call binop(c_or, "CalculateOR", A, B, C)
+  end subroutine CalculateOR
+
+  ! C = A XOR B  (XOR, src/design_impl.f90:961-991)
+  subroutine CalculateXOR(A, B, C)
+    type(Layer), intent(inout) :: A, B, C
+    call binop(c_xor, "CalculateXOR", A, B, C)
+  end subroutine CalculateXOR
+
+  ! C = A grown by ivar = (E, N, W, S)  (src/design_impl.f90:571-586)
+  subroutine CalculateGROWLayer(A, C, ivar)
+    type(Layer), intent(inout) :: A, C
+    integer(int32), intent(in) :: ivar(4)
+    call to_device(A)
+    call ensure_handle(C)
+    call check(c_grow(engine, A%dev, ivar(1), ivar(2), ivar(3), ivar(4), C%dev), "CalculateGROWLayer")
+    call finish_output(C)
+  end subroutine CalculateGROWLayer
+
+  ! Boost-coded operators (src/design_impl.f90:931-951, src/boost_polygon_api.cpp:97-156):
+  ! XOR=0, OR=1, AND=2, NOT=3, MERGE=4 (A only), SIZE=5 (A only, by control_value).
+  ! An unknown code gives an empty C, as the reference's C++ side does.
+  subroutine CalculateBoostOperation(A, B, C, control_parameter, control_value)
+    type(Layer), intent(inout) :: A, B, C
+    integer(int64), intent(in) :: control_parameter, control_value
+    select case (control_parameter)
+    case (K_BOOST_CONTROL_XOR)
+       call binop(c_xor, "CalculateBoostOperation(XOR)", A, B, C)
+    case (K_BOOST_CONTROL_OR)
+       call binop(c_or, "CalculateBoostOperation(OR)", A, B, C)
+    case (K_BOOST_CONTROL_AND)
+       call binop(c_and, "CalculateBoostOperation(AND)", A, B, C)
+    case (K_BOOST_CONTROL_NOT)
+       call binop(c_not, "CalculateBoostOperation(NOT)", A, B, C)
+    case (K_BOOST_CONTROL_MERGE)
+       call to_device(A)
+       call ensure_handle(C)
+       call check(c_merge(engine, A%dev, C%dev), "CalculateBoostOperation(MERGE)")
+       call finish_output(C)
+    case (K_BOOST_CONTROL_SIZE)
+       if (control_value < -huge(0_int32) - 1_int64 .or. control_value > huge(0_int32)) &
+            error stop "CalculateBoostOperation: SIZE amount out of int32 range"
+       call to_device(A)
+       call ensure_handle(C)
+       call check(c_size(engine, A%dev, int(control_value, c_int32_t), C%dev), "CalculateBoostOperation(SIZE)")
+       call finish_output(C)
+    case default
+       call ensure_handle(C)
+       call check(c_layer_upload(C%dev, c_null_ptr, 0_c_int64_t), "vl_layer_upload")
+       call finish_output(C)
+    end select
+  end subroutine CalculateBoostOperation
+
+  ! L = union of L, healed in place (src/design_impl.f90:1318-1341)
+  subroutine MergeHealLayer(L)
+    type(Layer), intent(inout) :: L
+    call to_device(L)
+    call check(c_merge(engine, L%dev, L%dev), "MergeHealLayer")
+    call finish_output(L)
+  end subroutine MergeHealLayer
+
+  ! boxes(1:n_in) -> their union as maximal strips in boxes(1:n_out), sorted by
+  ! (x1, y1, x2, y2); boxes is reallocated when the result needs more room
+  ! (src/polygon_fracture.f90:805-1004).
+  subroutine heal_boxes(n_in, boxes, n_out)
+    integer(int64), intent(in) :: n_in
+    type(Box), allocatable, intent(inout), target :: boxes(:)
+    integer(int64), intent(out) :: n_out
+    type(c_ptr) :: a, c
+    integer(c_int64_t) :: n
+    n_out = 0
+    if (n_in <= 0) return
+    call vl_fortran_init()
+    a = c_null_ptr
+    c = c_null_ptr
+    call check(c_layer_create(engine, a), "vl_layer_create")
+    call check(c_layer_create(engine, c), "vl_layer_create")
+    call check(c_layer_upload(a, c_loc(boxes(1)), n_in), "vl_layer_upload")
+    call check(c_canonicalize(engine, a, c), "heal_boxes")
+    call check(c_layer_size(c, n), "vl_layer_size")
+    if (size(boxes, kind=int64) < n) then
+       deallocate(boxes)
+       allocate(boxes(n))
+    end if
+    if (n > 0) call check(c_layer_download(c, c_loc(boxes(1)), size(boxes, kind=c_int64_t), n), "vl_layer_download")
+    n_out = n
+    call check(c_layer_destroy(a), "vl_layer_destroy")
+    call check(c_layer_destroy(c), "vl_layer_destroy")
+  end subroutine heal_boxes
+
+  ! ---- checks -----------------------------------------------------------------
+
+  ! Number of boxes of the layer (ASSERT_ZERO, src/controlparser.f90:524-530).
+  function LayerAssertZero(L) result(count)
+    type(Layer), intent(in) :: L
+    integer(int64) :: count
+    count = L%n_used
+  end function LayerAssertZero
+
+  ! Exact union area (src/geometry.f90:1840-1927).
+  function LayerKleeArea(L) result(area)
+    type(Layer), intent(inout) :: L
+    integer(int64) :: area
+    integer(c_int64_t) :: a
+    call to_device(L)
+    call check(c_klee_area(engine, L%dev, a), "LayerKleeArea")
+    area = a
+  end function LayerKleeArea
+
+  ! True when A XOR B is empty (src/boost_xor_checker.cpp:68-152).
+  function LayerXorEmpty(A, B) result(empty)
+    type(Layer), intent(inout) :: A, B
+    logical :: empty
+    integer(c_int) :: e
+    call to_device(A)
+    call to_device(B)
+    call check(c_xor_empty(engine, A%dev, B%dev, e), "LayerXorEmpty")
+    empty = e /= 0
+  end function LayerXorEmpty
+
+  ! lhs = rhs (COPY, src/controlparser.f90:555-559): the union of rhs.
+  subroutine CopyLayer(lhs, rhs)
+    type(Layer), intent(inout) :: lhs, rhs
+    call to_device(rhs)
+    call ensure_handle(lhs)
+    call check(c_merge(engine, rhs%dev, lhs%dev), "CopyLayer")
+    call finish_output(lhs)
+  end subroutine CopyLayer
+
+  ! ---- sorts (the GPU sort entry points of the reference) --------------------
+
+  ! Stable sort of event trackers by (X, Y) (src/event_sorter_apif.f90:30-43).
+  subroutine sort_event_trackers(arr)
+    type(XYTracker), intent(inout), target :: arr(:)
+    integer(int64) :: n
+    n = size(arr, kind=int64)
+    if (n <= 1) return
+    call vl_fortran_init()
+    call device_event_sort(c_loc(arr(1)), n)
+  end subroutine sort_event_trackers
+
+  ! Sort boxes by the Morton code of their centre (src/gpu_morton_sort.f90:41-55).
+  subroutine SortBoxesDirect(Boxes, N)
+    type(Box), intent(inout), target :: Boxes(:)
+    integer(int64), intent(in) :: N
+    if (N <= 1) return
+    call vl_fortran_init()
+    call thrust_sort_boxes_direct(c_loc(Boxes(1)), N)
+  end subroutine SortBoxesDirect
+
+  ! 1-based permutation that sorts Boxes by Morton code (src/gpu_morton_sort.f90:59-77).
+  subroutine SortBoxesIndirect(Boxes, N, SortedIndices)
+    type(Box), intent(in), target :: Boxes(:)
+    integer(int64), intent(in) :: N
+    integer(int64), intent(out), target :: SortedIndices(:)
+    if (N <= 0) return
+    if (N == 1) then
+       SortedIndices(1) = 1
+       return
+    end if
+    call vl_fortran_init()
+    call thrust_sort_boxes_indirect(c_loc(Boxes(1)), c_loc(SortedIndices(1)), N)
+  end subroutine SortBoxesIndirect
+
+end module vl_layer_ops
