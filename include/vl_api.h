@@ -160,11 +160,14 @@ void thrust_sort_boxes_direct(vl_box* d_boxes, int64_t n);
 void thrust_sort_boxes_indirect(vl_box* d_boxes, int64_t* d_indices, int64_t n);
 
 /* ---- batched form ------------------------------------------------------- */
-/* A program is a recorded list of operators over layers.  vl_program_run
- * executes independent operators of one step together (one fused tile launch
- * per phase across all of them) and replays the whole list as one hipGraph
- * once its worst-case scratch is planned.  Layers used by a program must not
- * be resized by other calls while it runs. */
+/* A program is a recorded list of operators over layers (no counterpart in the
+ * reference, whose control file runs one operator at a time).  vl_program_run
+ * splits the list into independent chains (operator j depends on an earlier i
+ * when it reads or writes a layer i writes, or writes a layer i reads) and runs
+ * up to four chains concurrently, each in program order on its own worker
+ * stream and host thread, after the work already queued on the engine.
+ * Results are identical to running the operators one by one.  Layers used by
+ * a program must not be touched by other calls while it runs. */
 typedef struct vl_program vl_program;
 int vl_program_create(vl_engine* e, vl_program** out);
 int vl_program_destroy(vl_program* p);
@@ -182,10 +185,12 @@ int vl_program_assert_count(vl_program* p, int64_t* failures);
  * with HIP events on the engine stream from its first to its last kernel. */
 int vl_engine_last_time_ms(vl_engine* e, double* ms);
 /* Per-kernel statistics collected while the "profile" parameter is 1 (setting
- * it resets them).  kernel: "tile" (the per-tile Boolean kernel), "bin" (tile
- * binning), "sort" (radix sort passes).  *bytes is the algorithmic byte count
- * of those launches (tile: 16 B per entry read + 16 B per strip written + 48 B
- * per tile of descriptor and look-back state). */
+ * it resets them).  kernel: "tile" (the tile stage: the wave-kernel tiers and
+ * the workgroup kernel launched back to back), "bin" (tile binning: span,
+ * emit, radix sort, bounds, gather), "sort" (reserved).  *bytes is the
+ * algorithmic byte count of those launches (tile: 16 B per entry read + 16 B
+ * per strip written + 48 B per tile of descriptor and range tables; bin: 32 B
+ * per input box + 24 B per entry for the sort + 32 B per entry gathered). */
 int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms,
                            double* bytes);
 

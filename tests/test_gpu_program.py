@@ -1,0 +1,102 @@
+"""Batched form: a recorded program (vl_program_*) runs its independent
+operator chains concurrently on worker streams; every result must equal the
+plain one-operator-per-call result bit for bit, and the oracle's canonical
+form after sorting."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import vlsi_layout_amd as V
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_boxes(rng, n, span, wmax):
+    x = rng.integers(0, span, n)
+    y = rng.integers(0, span, n)
+    w = rng.integers(1, wmax + 1, n)
+    h = rng.integers(1, wmax + 1, n)
+    return np.stack([x, y, x + w, y + h], 1).astype(np.int32)
+
+
+def op1_record(prog, L, t):
+    L1, L2, outer, inner, and1, ring = t
+    prog.add(V.VL_OP_GROW, L, None, L1, (1, 1, 1, 1))
+    prog.add(V.K_BOOST_CONTROL_SIZE, L, None, L2, (-1,))
+    prog.add(V.K_BOOST_CONTROL_NOT, L1, L, outer)
+    prog.add(V.K_BOOST_CONTROL_NOT, L, L2, inner)
+    prog.add(V.K_BOOST_CONTROL_AND, outer, L, and1)
+    prog.add(V.K_BOOST_CONTROL_AND, outer, inner, ring)
+    prog.add(-1, and1)
+    prog.add(-1, ring)
+
+
+def op1_plain(L):
+    L1, L2, outer, inner, and1, ring = (V.Layer(L.engine) for _ in range(6))
+    V.CalculateGROWLayer(L, L1, (1, 1, 1, 1))
+    V.CalculateBoostOperation(L, L, L2, V.K_BOOST_CONTROL_SIZE, -1)
+    V.CalculateFrameNOT(L1, L, outer)
+    V.CalculateFrameNOT(L, L2, inner)
+    V.CalculateAND(outer, L, and1)
+    V.CalculateAND(outer, inner, ring)
+    return [L1, L2, outer, inner, and1, ring]
+
+
+@pytest.mark.parametrize("nlayers", [1, 3, 6])
+def test_program_op1_chains_match_plain(gpu_engine, nlayers):
+    eng = V.Engine(0)
+    layers = [V.Layer(eng, V.synth_layer(V.SYN_STDCELL, 0x5EED0002, k % 4, 20000 + 3000 * k)) for k in range(nlayers)]
+    temps = [[V.Layer(eng) for _ in range(6)] for _ in range(nlayers)]
+    prog = V.Program(eng)
+    for L, t in zip(layers, temps):
+        op1_record(prog, L, t)
+    for rep in range(2):                       # the second run reuses the worker engines
+        prog.run()
+        assert prog.assert_failures() == 0
+        for L, t in zip(layers, temps):
+            ref = op1_plain(L)
+            for got, exp in zip(t, ref):
+                assert np.array_equal(got.boxes(), exp.boxes())
+    a = layers[0].boxes()
+    assert np.array_equal(O.sort_boxes(temps[0][0].boxes()), O.grow(a, 1, 1, 1, 1))
+    assert np.array_equal(O.sort_boxes(temps[0][1].boxes()), O.size(a, -1))
+    prog.close()
+    eng.close()
+
+
+def test_program_dependent_and_mixed_ops(gpu_engine):
+    eng = V.Engine(0)
+    rng = np.random.default_rng(41)
+    a, b, c = (rand_boxes(rng, 4000, 40000, 900) for _ in range(3))
+    A, B, Cl = V.Layer(eng, a), V.Layer(eng, b), V.Layer(eng, c)
+    x, y, z, u, w = (V.Layer(eng) for _ in range(5))
+    prog = V.Program(eng)
+    prog.add(V.K_BOOST_CONTROL_OR, A, B, x)          # chain 1: x = A|B, y = x ^ C
+    prog.add(V.K_BOOST_CONTROL_XOR, x, Cl, y)
+    prog.add(V.K_BOOST_CONTROL_AND, B, Cl, z)        # chain 2 (reads B, C; independent writes)
+    prog.add(V.K_BOOST_CONTROL_MERGE, A, None, u)    # chain 3
+    prog.add(V.K_BOOST_CONTROL_NOT, u, B, w)
+    prog.add(-1, z)
+    prog.run()
+    assert np.array_equal(O.sort_boxes(y.boxes()), O.op_xor(O.op_or(a, b), c))
+    assert np.array_equal(O.sort_boxes(z.boxes()), O.op_and(b, c))
+    assert np.array_equal(O.sort_boxes(u.boxes()), O.canonical(a))
+    assert np.array_equal(O.sort_boxes(w.boxes()), O.op_not(a, b))
+    assert prog.assert_failures() == (1 if len(O.op_and(b, c)) else 0)
+    prog.close()
+    eng.close()
+
+
+def test_program_error_from_a_worker(gpu_engine):
+    eng = V.Engine(0)
+    A = V.Layer(eng, [[0, 0, 2, 2]])
+    B = V.Layer(eng, [[5, 5, 9, 9]])
+    out1, out2 = V.Layer(eng), V.Layer(eng)
+    prog = V.Program(eng)
+    prog.add(V.VL_OP_GROW, A, None, out1, (-2, 0, 0, 0))    # makes the box invalid
+    prog.add(V.VL_OP_GROW, B, None, out2, (1, 1, 1, 1))
+    with pytest.raises(V.VLError) as ei:
+        prog.run()
+    assert ei.value.code == V.VL_EINVAL
+    prog.close()
+    eng.close()

@@ -20,9 +20,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "vl_internal.h"
@@ -114,6 +116,9 @@ struct vl_engine {
   // a reference; vl_engine_destroy drops the handle's reference and the
   // engine is freed when the last reference goes, whatever the destroy order.
   std::atomic<int> refs{1};
+  // Worker engines of a program run on the same device with their own stream
+  // and pool; they act on the layers of the engine that created them (root).
+  vl_engine* root = this;
   Pool pool;
   uint32_t* d_err = nullptr;
   uint64_t* h_stage = nullptr;   // pinned, 4 KiB
@@ -753,7 +758,8 @@ struct OpTimer {
 int check_layers(const vl_engine* e, const vl_layer* a, const vl_layer* b, const vl_layer* c) {
   if (!e) return fail(VL_EINVAL, "null engine");
   if (!a || !c) return fail(VL_EINVAL, "null layer");
-  if (a->e != e || (b && b->e != e) || c->e != e) return fail(VL_EINVAL, "layer belongs to another engine");
+  if (a->e != e->root || (b && b->e != e->root) || c->e != e->root)
+    return fail(VL_EINVAL, "layer belongs to another engine");
   return VL_OK;
 }
 
@@ -835,12 +841,8 @@ extern "C" {
 
 const char* vl_last_error(void) { return g_err.c_str(); }
 
-int vl_engine_create(int device, vl_engine** out) {
-  if (!out) return fail(VL_EINVAL, "null out");
-  *out = nullptr;
-  int ndev = 0;
-  HIPCK(hipGetDeviceCount(&ndev));
-  if (device < 0 || device >= ndev) return fail(VL_EINVAL, "device %d not present (%d devices)", device, ndev);
+// Creates an engine on a device (the caller has checked the device).
+static int engine_new(int device, vl_engine** out) {
   HIPCK(hipSetDevice(device));
   vl_engine* e = new vl_engine();
   e->device = device;
@@ -852,6 +854,18 @@ int vl_engine_create(int device, vl_engine** out) {
     delete e;
     return fail(VL_EHIP, "engine initialisation failed");
   }
+  *out = e;
+  return VL_OK;
+}
+
+int vl_engine_create(int device, vl_engine** out) {
+  if (!out) return fail(VL_EINVAL, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(VL_EINVAL, "device %d not present (%d devices)", device, ndev);
+  vl_engine* e = nullptr;
+  VLCK(engine_new(device, &e));
   if (const char* v = getenv("VL_TILE_NMAX")) e->nmax = atoi(v);
   if (e->nmax != 63 && e->nmax != 127) e->nmax = 63;
   *out = e;
@@ -1322,8 +1336,9 @@ struct vl_program {
   vl_engine* e = nullptr;
   std::vector<ProgOp> ops;
   int64_t failures = 0;
+  std::vector<vl_engine*> workers;   // worker engines (same device, own streams)
+  int max_workers = 4;
 };
-
 static int run_prog_op(vl_engine* e, const ProgOp& p, int64_t* failures) {
   switch (p.op) {
     case -1:
@@ -1354,6 +1369,7 @@ int vl_program_create(vl_engine* e, vl_program** out) {
 int vl_program_destroy(vl_program* p) {
   if (!p) return VL_OK;
   vl_engine* e = p->e;
+  for (vl_engine* w : p->workers) engine_release(w);
   delete p;
   engine_release(e);
   return VL_OK;
@@ -1374,27 +1390,121 @@ int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, 
   return VL_OK;
 }
 
+// Independent chains of a program.  Operator j depends on an earlier i when
+// it reads or writes a layer that i writes, or writes a layer that i reads;
+// the connected components of that relation run concurrently, each in program
+// order on its own worker engine (stream) and host thread.
+static std::vector<std::vector<int>> program_chains(const std::vector<ProgOp>& ops) {
+  const int n = (int)ops.size();
+  std::vector<int> parent(n);
+  for (int i = 0; i < n; i++) parent[i] = i;
+  std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
+  auto reads = [](const ProgOp& o, const vl_layer* l) { return o.a == l || o.b == l; };
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < j; i++) {
+      const ProgOp &a = ops[i], &b = ops[j];
+      const bool dep = (a.c && (reads(b, a.c) || b.c == a.c)) || (b.c && reads(a, b.c));
+      if (dep) parent[find(j)] = find(i);
+    }
+  std::map<int, std::vector<int>> groups;
+  for (int i = 0; i < n; i++) groups[find(i)].push_back(i);
+  std::vector<std::vector<int>> out;
+  for (auto& kv : groups) out.push_back(kv.second);
+  return out;
+}
+
 int vl_program_run(vl_program* p) {
   if (!p) return fail(VL_EINVAL, "null program");
   vl_engine* e = p->e;
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
-  hipEvent_t a, b;
-  HIPCK(hipEventCreate(&a));
-  HIPCK(hipEventCreate(&b));
-  HIPCK(hipEventRecord(a, e->st));
   p->failures = 0;
-  int rc = VL_OK;
-  for (const ProgOp& o : p->ops) {
-    rc = run_prog_op(e, o, &p->failures);
-    if (rc != VL_OK) break;
+  const std::vector<std::vector<int>> chains = program_chains(p->ops);
+  const int nw = (int)std::min<size_t>(chains.size(), (size_t)p->max_workers);
+  // the bounding boxes of the program's inputs (layers no operator writes)
+  // are cached before concurrent chains read them
+  {
+    auto written = [&](const vl_layer* l) {
+      for (const ProgOp& o : p->ops)
+        if (o.c == l) return true;
+      return false;
+    };
+    for (const ProgOp& o : p->ops) {
+      Box64 m;
+      if (o.a && !written(o.a)) VLCK(layer_mbr(e, o.a, &m));
+      if (o.b && !written(o.b)) VLCK(layer_mbr(e, o.b, &m));
+    }
   }
-  (void)hipEventRecord(b, e->st);
-  (void)hipEventSynchronize(b);
+  hipEvent_t t0, t1;
+  HIPCK(hipEventCreate(&t0));
+  HIPCK(hipEventCreate(&t1));
+  HIPCK(hipEventRecord(t0, e->st));
+  int rc = VL_OK;
+  if (nw <= 1) {
+    for (const ProgOp& o : p->ops) {
+      rc = run_prog_op(e, o, &p->failures);
+      if (rc != VL_OK) break;
+    }
+  } else {
+    while ((int)p->workers.size() < nw) {
+      vl_engine* w = nullptr;
+      VLCK(engine_new(e->device, &w));
+      w->root = e;
+      p->workers.push_back(w);
+    }
+    std::vector<int> rcs(nw, VL_OK);
+    std::vector<std::string> errs(nw);
+    std::vector<int64_t> fails(nw, 0);
+    std::vector<std::thread> th;
+    for (int k = 0; k < nw; k++) {
+      vl_engine* w = p->workers[k];
+      w->nmax = e->nmax;
+      w->target = e->target;
+      w->erode_target = e->erode_target;
+      w->level_factor = e->level_factor;
+      w->erode_kernel = e->erode_kernel;
+      w->max_wg_cap = e->max_wg_cap;
+      w->prof = e->prof;
+      for (auto& st : w->kst) st = vl_engine::KStat();
+      HIPCK(hipStreamWaitEvent(w->st, t0, 0));
+      th.emplace_back([&, k, w]() {
+        (void)hipSetDevice(w->device);
+        for (size_t c = k; c < chains.size(); c += nw) {
+          for (int i : chains[c]) {
+            rcs[k] = run_prog_op(w, p->ops[i], &fails[k]);
+            if (rcs[k] != VL_OK) {
+              errs[k] = g_err;
+              return;
+            }
+          }
+        }
+        if (hipStreamSynchronize(w->st) != hipSuccess) {
+          rcs[k] = VL_EHIP;
+          errs[k] = "worker stream failed";
+        }
+      });
+    }
+    for (auto& t : th) t.join();
+    for (int k = 0; k < nw; k++) {
+      p->failures += fails[k];
+      vl_engine* w = p->workers[k];
+      for (int s = 0; s < 3; s++) {
+        e->kst[s].launches += w->kst[s].launches;
+        e->kst[s].ms += w->kst[s].ms;
+        e->kst[s].bytes += w->kst[s].bytes;
+      }
+      if (rc == VL_OK && rcs[k] != VL_OK) {
+        rc = rcs[k];
+        g_err = errs[k];
+      }
+    }
+  }
+  (void)hipEventRecord(t1, e->st);
+  (void)hipEventSynchronize(t1);
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, a, b) == hipSuccess) e->last_ms = ms;
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess) e->last_ms = ms;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
   return rc;
 }
 
