@@ -180,6 +180,40 @@ int vl_program_run(vl_program* p);
 /* Number of ASSERT_ZERO checks of the last run that found shapes. */
 int vl_program_assert_count(vl_program* p, int64_t* failures);
 
+/* ---- serving: windowed queries against resident layers --------------------- */
+/* No counterpart in the reference (its control file runs one operator at a
+ * time).  A server indexes its layers once (boxes binned into a uniform cell
+ * grid, cell starts kept on the host too) and answers windowed DRC-style
+ * queries  R = GROW_s(A ∩ W) AND (B ∩ W)  with continuous batching: requests
+ * are released at their arrival times, admitted in arrival order whenever a
+ * slot is free and the in-flight memory estimate stays within the HBM budget
+ * (a request is always admitted when nothing else is in flight), and run
+ * concurrently, one per slot, each slot with its own stream and host thread.
+ * Every result is the same alone and in any mix. */
+typedef struct vl_server vl_server;
+typedef struct vl_query {
+  vl_box window;              /* W */
+  int32_t layer_a, layer_b;   /* indices into the server's layers */
+  int32_t spacing;            /* s >= 0 (GROW on all four sides) */
+  int32_t reserved;
+  double arrival_ms;          /* release time, relative to the start of vl_server_run */
+} vl_query;
+typedef struct vl_query_result {
+  int32_t status;             /* VL_OK or the failure code */
+  int32_t slot;               /* slot that ran it */
+  int64_t in_boxes;           /* boxes of A ∩ W plus B ∩ W (a box counts once per index cell it meets) */
+  int64_t count;              /* boxes of R (canonical maximal strips): 0 = ASSERT_ZERO holds */
+  uint64_t hash;              /* order-independent hash of R's boxes */
+  double start_ms, end_ms;    /* admission and completion, relative to the start of the run */
+} vl_query_result;
+/* slots: concurrent queries (1..16); hbm_budget: bytes (0 = 64 GiB). */
+int vl_server_create(vl_engine* e, const vl_layer* const* layers, int nlayers, int slots, uint64_t hbm_budget,
+                     vl_server** out);
+/* Runs n queries, replaying their arrival times in real time; res[i] is the
+ * result of q[i].  Returns the first failure code (all queries still run). */
+int vl_server_run(vl_server* s, const vl_query* q, int64_t n, vl_query_result* res);
+int vl_server_destroy(vl_server* s);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Device time (ms) of the last operator / program run on this engine, measured
  * with HIP events on the engine stream from its first to its last kernel. */

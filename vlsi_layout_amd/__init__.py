@@ -122,6 +122,9 @@ def lib():
             "vl_program_add": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
             "vl_program_run": (C.c_int, [vp]),
             "vl_program_assert_count": (C.c_int, [vp, C.POINTER(i64)]),
+            "vl_server_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
+            "vl_server_run": (C.c_int, [vp, vp, i64, vp]),
+            "vl_server_destroy": (C.c_int, [vp]),
             "vl_synth_layer": (C.c_int, [C.c_int, C.c_uint64, C.c_int, i64, vp]),
             "vl_synth_tile_count": (i64, [C.c_uint64, C.c_int]),
         }
@@ -416,6 +419,43 @@ class Program:
     def close(self):
         if getattr(self, "_h", None):
             lib().vl_program_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+QUERY_DTYPE = np.dtype([("x1", "<i4"), ("y1", "<i4"), ("x2", "<i4"), ("y2", "<i4"), ("layer_a", "<i4"),
+                        ("layer_b", "<i4"), ("spacing", "<i4"), ("reserved", "<i4"), ("arrival_ms", "<f8")])
+RESULT_DTYPE = np.dtype([("status", "<i4"), ("slot", "<i4"), ("in_boxes", "<i8"), ("count", "<i8"),
+                         ("hash", "<u8"), ("start_ms", "<f8"), ("end_ms", "<f8")])
+
+
+class Server:
+    """Windowed-query server over resident layers (vl_server_*): answers
+    R = GROW_s(A & W) AND (B & W) with continuous batching over `slots`
+    concurrent queries and an HBM admission budget."""
+
+    def __init__(self, eng, layers, slots=4, hbm_budget=0):
+        self.engine = eng
+        arr = (C.c_void_p * len(layers))(*[l.handle.value for l in layers])
+        h = C.c_void_p()
+        _check(lib().vl_server_create(eng.handle, arr, len(layers), int(slots), int(hbm_budget), C.byref(h)))
+        self._h = h
+
+    def run(self, queries):
+        """queries: QUERY_DTYPE array; returns (status code, RESULT_DTYPE array)."""
+        q = np.ascontiguousarray(queries, dtype=QUERY_DTYPE)
+        res = np.zeros(len(q), RESULT_DTYPE)
+        rc = lib().vl_server_run(self._h, q.ctypes.data_as(C.c_void_p), len(q), res.ctypes.data_as(C.c_void_p))
+        return rc, res
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vl_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -15,7 +15,9 @@
 // the tile kernel's look-back, so results are deterministic.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -521,20 +523,20 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   return VL_OK;
 }
 
-// Bins up to two inputs into grid g and processes the tiles.  Layer A boxes
-// are binned by their extent grown by xr and stored grown by xs; erode > 0
-// selects the erosion tile kernel (xr is then the erosion halo, xs zero).
-int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, Xform xs, Xform xr,
-                    const vl_box* b, int64_t nb, bool area, int32_t erode, Outs& o) {
+// Bins up to two inputs into grid g: entries of tile t (layer A, then B, each
+// in input order) at entries[start[2t] .. start[2t+2]).  Layer A boxes are
+// binned by their extent grown by xr and stored grown by xs.  The arrays are
+// allocated from `keep` and live as long as it does.
+int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xform xr, const vl_box* b, int64_t nb,
+                Scratch& keep, vl_box** entries_out, uint64_t** start_out, uint64_t* total_out) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   const int64_t n = na + nb;
   Scratch s(e);
   uint32_t* cnt = s.get<uint32_t>((size_t)n + 1);
   uint64_t* pos = s.get<uint64_t>((size_t)n + 1);
-  uint64_t* start = s.get<uint64_t>(2 * (size_t)nt + 1);
-  TileDesc* desc = s.get<TileDesc>(nt);
   void* stmp = s.get<char>(scan_tmp_bytes((size_t)n + 1));
-  if (s.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+  uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt + 1);
+  if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
   launch_bin_span(a, na, xr, b, nb, g, cnt, e->st);
   scan_u32_to_u64(cnt, pos, (size_t)n, pos + n, stmp, e->st);
   uint64_t total = 0;
@@ -545,7 +547,7 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
   uint64_t* keys2 = s2.get<uint64_t>(total + 1);
   uint32_t* vals2 = s2.get<uint32_t>(total + 1);
   void* rtmp = s2.get<char>(radix_plan((int64_t)total).tmp_bytes);
-  vl_box* entries = s.get<vl_box>(total + 1);
+  vl_box* entries = keep.get<vl_box>(total + 1);
   if (s2.oom || !entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
   if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
   launch_bin_emit(a, na, xr, b, nb, g, pos, keys, vals, e->st);
@@ -557,7 +559,6 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
   }
   launch_bin_bounds(keys, total, 2 * nt, start, e->st);
   launch_bin_gather(a, na, xs, b, vals, total, entries, e->st);
-  launch_build_desc(g, start, top, desc, e->st);
   HIPCK(hipGetLastError());
   if (e->prof) {
     HIPCK(hipEventRecord(e->kev1, e->st));
@@ -571,7 +572,27 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
     // gather 16 B per entry, write 16 B per entry
     k.bytes += 32.0 * (double)n + 12.0 * 2.0 * (double)total + 32.0 * (double)total;
   }
-  s2.release();
+  *entries_out = entries;
+  *start_out = start;
+  *total_out = total;
+  return VL_OK;
+}
+
+// Bins up to two inputs into grid g and processes the tiles.  Layer A boxes
+// are binned by their extent grown by xr and stored grown by xs; erode > 0
+// selects the erosion tile kernel (xr is then the erosion halo, xs zero).
+int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, Xform xs, Xform xr,
+                    const vl_box* b, int64_t nb, bool area, int32_t erode, Outs& o) {
+  const int64_t nt = (int64_t)g.cols * g.rows;
+  Scratch keep(e);
+  vl_box* entries = nullptr;
+  uint64_t* start = nullptr;
+  uint64_t total = 0;
+  VLCK(bin_entries(e, g, a, na, xs, xr, b, nb, keep, &entries, &start, &total));
+  TileDesc* desc = keep.get<TileDesc>(nt);
+  if (!desc) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+  launch_build_desc(g, start, top, desc, e->st);
+  HIPCK(hipGetLastError());
   return process_tiles(e, desc, (int)nt, entries, total, two_layer(top) && nb > 0, area, erode, o, 0);
 }
 
@@ -1512,6 +1533,293 @@ int vl_program_assert_count(vl_program* p, int64_t* failures) {
   if (!p || !failures) return fail(VL_EINVAL, "null argument");
   *failures = p->failures;
   return VL_OK;
+}
+
+}  // extern "C"
+
+// ---- query server --------------------------------------------------------------
+namespace {
+
+// A layer binned into a uniform cell grid; cell starts also on the host.
+struct CellIndex {
+  Grid g{};
+  vl_box* entries = nullptr;
+  uint64_t* d_start = nullptr;          // 2*cells+1 (layer A keys only)
+  std::vector<uint64_t> start;          // host copy
+  int64_t nboxes = 0;
+  Scratch* keep = nullptr;
+};
+
+// Entry ranges of the cell rows a window covers (one contiguous range per row).
+void window_ranges(const CellIndex& ix, const vl_box& w, std::vector<uint64_t>& rbeg, std::vector<uint64_t>& roff) {
+  rbeg.clear();
+  roff.assign(1, 0);
+  if (ix.nboxes == 0) return;
+  const Grid& g = ix.g;
+  const int64_t x1 = std::max<int64_t>(w.x1, g.x0), x2 = std::min<int64_t>(w.x2, g.x1);
+  const int64_t y1 = std::max<int64_t>(w.y1, g.y0), y2 = std::min<int64_t>(w.y2, g.y1);
+  if (x1 >= x2 || y1 >= y2) return;
+  const int c1 = (int)((x1 - g.x0) / g.tw), c2 = (int)((x2 - 1 - g.x0) / g.tw);
+  const int r1 = (int)((y1 - g.y0) / g.th), r2 = (int)((y2 - 1 - g.y0) / g.th);
+  for (int r = r1; r <= r2; r++) {
+    const int64_t t1 = (int64_t)r * g.cols + c1, t2 = (int64_t)r * g.cols + c2;
+    const uint64_t b = ix.start[2 * t1], e = ix.start[2 * t2 + 2];
+    if (e > b) {
+      rbeg.push_back(b);
+      roff.push_back(roff.back() + (e - b));
+    }
+  }
+}
+
+// Boxes of the index inside w, clipped to w, into layer out (engine e's stream).
+int window_extract(vl_engine* e, const CellIndex& ix, const vl_box& w, vl_layer* out) {
+  std::vector<uint64_t> rbeg, roff;
+  window_ranges(ix, w, rbeg, roff);
+  const uint64_t n = roff.back();
+  const int nr = (int)rbeg.size();
+  if (n == 0) return set_empty(e, out);
+  Scratch s(e);
+  uint64_t* d_rbeg = s.get<uint64_t>(nr);
+  uint64_t* d_roff = s.get<uint64_t>(nr + 1);
+  vl_box* clip = s.get<vl_box>(n);
+  uint32_t* keep = s.get<uint32_t>(n + 1);
+  uint32_t* pos = s.get<uint32_t>(n + 1);
+  void* stmp = s.get<char>(scan_tmp_bytes(n + 1));
+  if (s.oom) return fail(VL_ENOMEM, "window scratch");
+  HIPCK(hipMemcpyAsync(d_rbeg, rbeg.data(), nr * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
+  HIPCK(hipMemcpyAsync(d_roff, roff.data(), (nr + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
+  launch_window_clip(ix.entries, d_rbeg, d_roff, nr, n, w, clip, keep, e->st);
+  scan_exclusive_u32(keep, pos, n, pos + n, stmp, e->st);
+  uint32_t m = 0;
+  VLCK(readback(e, &m, pos + n, sizeof(m)));
+  if (m == 0) return set_empty(e, out);
+  size_t r = 0;
+  vl_box* d = (vl_box*)e->pool.get(m * sizeof(vl_box), &r);
+  if (!d) return fail(VL_ENOMEM, "window layer");
+  launch_window_compact(clip, keep, pos, n, d, e->st);
+  HIPCK(hipGetLastError());
+  if (out->d) e->pool.put(out->d, out->bytes);
+  out->d = d;
+  out->bytes = r;
+  out->n = m;
+  out->mbr_ok = false;
+  return VL_OK;
+}
+
+}  // namespace
+
+struct vl_server {
+  vl_engine* e = nullptr;
+  std::vector<CellIndex> index;
+  std::vector<vl_engine*> slots;
+  uint64_t budget = 0;
+};
+
+namespace {
+
+void release_layer(vl_engine* e, vl_layer& l) {
+  if (l.d) e->pool.put(l.d, l.bytes);
+  l.d = nullptr;
+  l.n = 0;
+  l.bytes = 0;
+}
+
+// One query on slot engine w: R = GROW_s(A ∩ W) AND (B ∩ W).
+int run_query(vl_server* sv, vl_engine* w, const vl_query& q, vl_query_result& r) {
+  std::lock_guard<std::recursive_mutex> lk(w->mu);
+  HIPCK(hipSetDevice(w->device));
+  if (q.layer_a < 0 || q.layer_a >= (int)sv->index.size() || q.layer_b < 0 || q.layer_b >= (int)sv->index.size())
+    return fail(VL_EINVAL, "query layer index out of range");
+  if (q.spacing < 0) return fail(VL_EINVAL, "negative spacing");
+  if (q.window.x1 >= q.window.x2 || q.window.y1 >= q.window.y2) return fail(VL_EINVAL, "invalid window");
+  vl_layer A, B, G, R;
+  A.e = B.e = G.e = R.e = sv->e;
+  int rc = window_extract(w, sv->index[q.layer_a], q.window, &A);
+  if (rc == VL_OK) rc = window_extract(w, sv->index[q.layer_b], q.window, &B);
+  r.in_boxes = A.n + B.n;
+  const int32_t s = q.spacing;
+  if (rc == VL_OK) rc = run_op(w, TOP_UNION, A.d, A.n, Xform{s, s, s, s}, nullptr, 0, &G, nullptr);
+  if (rc == VL_OK) rc = run_op(w, TOP_AND, G.d, G.n, Xform{0, 0, 0, 0}, B.d, B.n, &R, nullptr);
+  if (rc == VL_OK) {
+    r.count = R.n;
+    Scratch hs(w);
+    unsigned long long* h = hs.get<unsigned long long>(1);
+    if (!h) {
+      rc = fail(VL_ENOMEM, "hash scratch");
+    } else {
+      launch_fill_u64((uint64_t*)h, 0ull, 1, w->st);
+      launch_box_hash(R.d, R.n, h, w->st);
+      uint64_t hv = 0;
+      rc = readback(w, &hv, h, sizeof(hv));
+      r.hash = hv;
+    }
+  }
+  release_layer(w, A);
+  release_layer(w, B);
+  release_layer(w, G);
+  release_layer(w, R);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vl_server_create(vl_engine* e, const vl_layer* const* layers, int nlayers, int slots, uint64_t hbm_budget,
+                     vl_server** out) {
+  if (!e || !out || (nlayers > 0 && !layers) || nlayers < 0) return fail(VL_EINVAL, "bad argument");
+  if (slots < 1 || slots > 16) return fail(VL_EINVAL, "slots must be 1..16");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  vl_server* sv = new vl_server();
+  sv->e = e;
+  sv->budget = hbm_budget ? hbm_budget : (uint64_t)64 << 30;
+  e->refs.fetch_add(1);
+  auto cleanup = [&](int rc) {
+    vl_server_destroy(sv);
+    return rc;
+  };
+  sv->index.resize(nlayers);
+  for (int i = 0; i < nlayers; i++) {
+    const vl_layer* l = layers[i];
+    if (!l || l->e != e) return cleanup(fail(VL_EINVAL, "layer %d is null or belongs to another engine", i));
+    CellIndex& ix = sv->index[i];
+    ix.nboxes = l->n;
+    if (l->n == 0) continue;
+    Box64 m;
+    int rc = layer_mbr(e, l, &m);
+    if (rc != VL_OK) return cleanup(rc);
+    ix.g = make_grid(m, l->n, 64);   // about 64 boxes per cell
+    ix.keep = new Scratch(e);
+    uint64_t total = 0;
+    rc = bin_entries(e, ix.g, l->d, l->n, Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0, *ix.keep, &ix.entries,
+                     &ix.d_start, &total);
+    if (rc != VL_OK) return cleanup(rc);
+    const size_t ns = 2 * (size_t)ix.g.cols * ix.g.rows + 1;
+    ix.start.resize(ns);
+    if (hipMemcpyAsync(ix.start.data(), ix.d_start, ns * sizeof(uint64_t), hipMemcpyDeviceToHost, e->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(e->st) != hipSuccess)
+      return cleanup(fail(VL_EHIP, "index readback"));
+  }
+  for (int k = 0; k < slots; k++) {
+    vl_engine* w = nullptr;
+    int rc = engine_new(e->device, &w);
+    if (rc != VL_OK) return cleanup(rc);
+    w->root = e;
+    w->nmax = e->nmax;
+    w->target = e->target;
+    w->erode_target = e->erode_target;
+    w->level_factor = e->level_factor;
+    w->erode_kernel = e->erode_kernel;
+    sv->slots.push_back(w);
+  }
+  *out = sv;
+  return VL_OK;
+}
+
+int vl_server_destroy(vl_server* sv) {
+  if (!sv) return VL_OK;
+  vl_engine* e = sv->e;
+  {
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->st);
+    for (CellIndex& ix : sv->index) delete ix.keep;
+  }
+  for (vl_engine* w : sv->slots) engine_release(w);
+  delete sv;
+  engine_release(e);
+  return VL_OK;
+}
+
+int vl_server_run(vl_server* sv, const vl_query* q, int64_t n, vl_query_result* res) {
+  if (!sv || (n > 0 && (!q || !res)) || n < 0) return fail(VL_EINVAL, "bad argument");
+  using clk = std::chrono::steady_clock;
+  // in-flight memory estimate of a query: its candidate boxes times the
+  // per-box footprint of extraction, binning and tile scratch
+  std::vector<uint64_t> est(n);
+  {
+    std::vector<uint64_t> rb, ro;
+    for (int64_t i = 0; i < n; i++) {
+      uint64_t c = 0;
+      for (int32_t li : {q[i].layer_a, q[i].layer_b})
+        if (li >= 0 && li < (int)sv->index.size()) {
+          window_ranges(sv->index[li], q[i].window, rb, ro);
+          c += ro.back();
+        }
+      est[i] = 256 * c + ((uint64_t)8 << 20);
+    }
+  }
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<int64_t> pending;        // released, not yet admitted (arrival order)
+  size_t head = 0;
+  bool all_released = false;
+  uint64_t used = 0;
+  int inflight = 0;
+  int first_rc = VL_OK;
+  const clk::time_point t0 = clk::now();
+  auto ms_since = [&](clk::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); };
+  std::vector<std::thread> th;
+  for (int k = 0; k < (int)sv->slots.size(); k++) {
+    th.emplace_back([&, k]() {
+      vl_engine* w = sv->slots[k];
+      (void)hipSetDevice(w->device);
+      for (;;) {
+        int64_t i = -1;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv.wait(lk, [&]() {
+            if (head < pending.size()) {
+              const int64_t c = pending[head];
+              if (inflight == 0 || used + est[c] <= sv->budget) return true;
+              return false;
+            }
+            return all_released;
+          });
+          if (head >= pending.size()) return;   // all released and none left
+          i = pending[head++];
+          used += est[i];
+          inflight++;
+        }
+        vl_query_result& r = res[i];
+        r = vl_query_result{};
+        r.slot = k;
+        r.start_ms = ms_since(clk::now());
+        r.status = run_query(sv, w, q[i], r);
+        r.end_ms = ms_since(clk::now());
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          used -= est[i];
+          inflight--;
+          if (r.status != VL_OK && first_rc == VL_OK) first_rc = r.status;
+        }
+        cv.notify_all();
+      }
+    });
+  }
+  // release requests at their arrival times
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return q[a].arrival_ms < q[b].arrival_ms; });
+  for (int64_t i : order) {
+    const clk::time_point due = t0 + std::chrono::duration_cast<clk::duration>(
+                                         std::chrono::duration<double, std::milli>(std::max(0.0, q[i].arrival_ms)));
+    std::this_thread::sleep_until(due);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      pending.push_back(i);
+    }
+    cv.notify_all();
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    all_released = true;
+  }
+  cv.notify_all();
+  for (auto& t : th) t.join();
+  return first_rc;
 }
 
 }  // extern "C"

@@ -183,6 +183,17 @@ void launch_split_scatter(const TileDesc* parents, int np, const int32_t* sub_fi
                           uint32_t* fillA, uint32_t* fillB, const vl_box* entries,
                           vl_box* sub_entries, int32_t halo, hipStream_t st);
 
+// vl_window.hip — window extraction from a cell-indexed layer.  Candidates
+// are the concatenation of the entry ranges [rbeg[r], rbeg[r] + roff[r+1] -
+// roff[r]); each is clipped to w (clip[i]) and flagged when it keeps area.
+void launch_window_clip(const vl_box* entries, const uint64_t* rbeg, const uint64_t* roff, int nr, uint64_t n,
+                        vl_box w, vl_box* clip, uint32_t* keep, hipStream_t st);
+// out[pos[i]] = clip[i] for the flagged candidates.
+void launch_window_compact(const vl_box* clip, const uint32_t* keep, const uint32_t* pos, uint64_t n, vl_box* out,
+                           hipStream_t st);
+// *out += order-independent 64-bit hash of the box set b[0..n).
+void launch_box_hash(const vl_box* b, int64_t n, unsigned long long* out, hipStream_t st);
+
 // vl_stitch.hip
 // Partition side strips: tags with SEAM_B|SEAM_T -> next (border), the rest -> stitch.
 void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl_box* next,
