@@ -20,6 +20,11 @@ on the engine stream, against the MI355X's 8 TB/s HBM peak.
 cpu_baseline: the same OP1 ring on a band of rows of the same layers, run by
 the C/OpenMP restatement of the reference CPU path (oracle/liboracle.so,
 frame-tiled like src/design_impl.f90:993-1316).
+
+Also reported in the same line: "c2_10M_total", the same ring with 10M boxes
+over the 4 layers (2.5M per layer), plain and batched; and "serving", windowed
+query latency (p50/p99) and throughput of the query server on a resident
+50M-box 4-layer design under Poisson arrivals.
 """
 import argparse
 import json
@@ -47,6 +52,11 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1_000_000, help="boxes per layer in the CPU sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--mode", choices=["plain", "batched", "both"], default="both")
+    p.add_argument("--c2-total", type=int, default=10_000_000,
+                   help="also time the same ring at this many boxes over the 4 layers (0 = skip)")
+    p.add_argument("--serving-boxes", type=int, default=50_000_000,
+                   help="serving design size over 4 layers (0 = skip the serving measurement)")
+    p.add_argument("--serving-requests", type=int, default=400)
     return p.parse_args()
 
 
@@ -91,6 +101,53 @@ def time_steps(fn, eng, steps, warmup):
         out = fn()
     eng.sync()
     return time.perf_counter() - t0, out
+
+
+def serving(V, args):
+    """Windowed DRC-style queries R = GROW_s(A & W) AND (B & W) against a
+    resident 4-layer design: Poisson arrivals at 200 req/s, window box counts
+    log-uniform in [1k, 5M], s in {1, 2, 5} DBU, 64 GiB admission budget,
+    4 concurrent slots."""
+    rng = np.random.default_rng(0x5EED0005)
+    per = args.serving_boxes // 4
+    boxes = [V.synth_layer(V.SYN_STDCELL, 0x5EED0005, k, per) for k in range(4)]
+    eng = V.Engine(0)
+    layers = [V.Layer(eng, b) for b in boxes]
+    t = time.perf_counter()
+    srv = V.Server(eng, layers, slots=4, hbm_budget=64 << 30)
+    t_index = time.perf_counter() - t
+    mbr = [(int(b[:, 0].min()), int(b[:, 1].min()), int(b[:, 2].max()), int(b[:, 3].max())) for b in boxes]
+    dens = [len(b) / float((m[2] - m[0]) * (m[3] - m[1])) for b, m in zip(boxes, mbr)]
+    n = args.serving_requests
+    q = np.zeros(n, V.QUERY_DTYPE)
+    arrival = 0.0
+    for i in range(n):
+        la, lb = (int(v) for v in rng.choice(4, 2, replace=False))
+        target = float(np.exp(rng.uniform(np.log(1e3), np.log(5e6))))     # boxes of A and B in W
+        side = int(np.sqrt(target / (dens[la] + dens[lb])))
+        m = mbr[la]
+        x = int(rng.integers(m[0], max(m[0] + 1, m[2] - side)))
+        y = int(rng.integers(m[1], max(m[1] + 1, m[3] - side)))
+        q[i] = (x, y, x + max(side, 1), y + max(side, 1), la, lb, int(rng.choice([1, 2, 5])), 0, arrival)
+        arrival += rng.exponential(1000.0 / 200.0)
+    srv.run(q[:8])                                                           # warm the slots
+    rc, res = srv.run(q)
+    if rc != 0:
+        raise SystemExit(f"serving run failed with status {rc}")
+    lat = res["end_ms"] - q["arrival_ms"]
+    wall = (res["end_ms"].max() - q["arrival_ms"].min()) / 1000.0
+    events = sorted([(a, 1) for a in q["arrival_ms"]] + [(s_, -1) for s_ in res["start_ms"]])
+    depth = peak = 0
+    for _, d in events:
+        depth += d
+        peak = max(peak, depth)
+    srv.close()
+    eng.close()
+    return {"requests": n, "design_boxes": per * 4, "arrival_rate_per_s": 200, "slots": 4,
+            "hbm_budget_gib": 64, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(lat.max()), "boxes_per_s": float(res["in_boxes"].sum() / wall),
+            "window_boxes_mean": float(res["in_boxes"].mean()), "max_queue_depth": int(peak),
+            "assert_zero_failures": int((res["count"] > 0).sum()), "index_build_s": t_index}
 
 
 def main():
@@ -202,6 +259,35 @@ def main():
         line["cpu_baseline"] = {"value": boxes / secs, "unit": "boxes/s", "cores": nthreads, "kind": "port",
                                 "sample": f"OP1 ring on the first {args.cpu_sample} boxes (a band of rows) "
                                           f"of each of the 4 layers, healed first; {secs:.1f} s of CPU work"}
+    if args.c2_total:
+        # the same ring on the C2 design as 10M boxes in total (2.5M per layer)
+        per = args.c2_total // 4
+        eng.close()
+        eng2 = V.Engine(0)
+        rings2 = [Ring(V, eng2, lay[:per]) for lay in layers]
+        prog2 = V.Program(eng2)
+        for r in rings2:
+            r.record(V, prog2)
+        tot2, fails2 = 0, 0
+        for r in rings2:
+            b_, f_ = r.run_plain(V)
+            tot2 += b_
+            fails2 += f_
+        secs_p, _ = time_steps(lambda: [r.run_plain(V) for r in rings2], eng2, args.steps, args.warmup)
+        secs_b, _ = time_steps(lambda: prog2.run(), eng2, args.steps, args.warmup)
+        line["c2_10M_total"] = {"boxes_per_layer": per, "input_boxes_per_step": int(tot2),
+                                "plain_boxes_per_s": tot2 * args.steps / secs_p,
+                                "batched_boxes_per_s": tot2 * args.steps / secs_b,
+                                "plain_ms_per_step": 1000.0 * secs_p / args.steps,
+                                "batched_ms_per_step": 1000.0 * secs_b / args.steps}
+        print(f"[bench] c2 (2.5M/layer): plain {line['c2_10M_total']['plain_ms_per_step']:.1f} ms, "
+              f"batched {line['c2_10M_total']['batched_ms_per_step']:.1f} ms", file=sys.stderr, flush=True)
+        prog2.close()
+        eng2.close()
+    if args.serving_boxes:
+        line["serving"] = serving(V, args)
+        print(f"[bench] serving: p50 {line['serving']['p50_ms']:.2f} ms, p99 {line['serving']['p99_ms']:.2f} ms",
+              file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
