@@ -22,6 +22,7 @@ FLIB = os.path.join(PKG, "libvlayout_f.so")
 FEXE = os.path.join(PKG, "vl_magparser")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
 FLANG = os.environ.get("FLANG", shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
@@ -51,8 +52,8 @@ def build_hip(verbose=False):
     jobs = []
     for s in HIP_SOURCES:
         jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
-    for s in CPP_SOURCES:
-        jobs.append([HIPCC] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
+    for s in CPP_SOURCES:   # host-only C++: no device code
+        jobs.append([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
     workers = min(len(jobs), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
     with cf.ThreadPoolExecutor(workers) as ex:
         outs = list(ex.map(_run, jobs))

@@ -4,7 +4,7 @@
 #include <stdint.h>
 #include <stddef.h>
 
-#include "../../include/vl_api.h"
+#include "vl_api.h"
 
 namespace vl {
 

@@ -3,7 +3,7 @@
 #include <cstdint>
 #include <random>
 
-#include "../../include/vl_synth.h"
+#include "vl_synth.h"
 
 namespace {
 

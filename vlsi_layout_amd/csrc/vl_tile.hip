@@ -1103,8 +1103,11 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
 }
 
 // ---- reorder into tile order -------------------------------------------------------
-constexpr int REORDER_TILES = 16;
+constexpr int REORDER_TILES = 16;   // tiles per 256-thread workgroup (4 per wave)
 
+// One wave per tile at a time: the wave walks the tile's range in chunks of
+// 64 strips; a ballot over "final" splits each chunk between the final and
+// the seam stream, preserving order within each.
 __global__ __launch_bounds__(256) void k_reorder(int ntiles, const uint64_t* __restrict__ tab_off,
                                                  const uint32_t* __restrict__ tab_n, int ntab,
                                                  const uint64_t* __restrict__ dstF, const uint64_t* __restrict__ dstS,
@@ -1112,31 +1115,32 @@ __global__ __launch_bounds__(256) void k_reorder(int ntiles, const uint64_t* __r
                                                  vl_box* __restrict__ out, uint64_t out_base,
                                                  vl_box* __restrict__ side, uint8_t* __restrict__ side_tag,
                                                  uint64_t side_base) {
-  __shared__ int ws[256 / WAVE];
-  const int t0 = blockIdx.x * REORDER_TILES;
-  for (int t = t0; t < t0 + REORDER_TILES && t < ntiles; t++) {
+  const int lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+  const int wave = threadIdx.x / WAVE;
+  const int t0 = blockIdx.x * REORDER_TILES + wave * (REORDER_TILES / 4);
+  for (int t = t0; t < t0 + REORDER_TILES / 4 && t < ntiles; t++) {
     const uint32_t T = tab_n[t] + tab_n[ntab + t];
     if (T == 0) continue;
     const uint64_t src = tab_off[t];
     uint64_t dF = out_base + dstF[t], dS = side_base + dstS[t];
-    for (uint32_t c0 = 0; c0 < T; c0 += 256) {
-      const uint32_t i = c0 + threadIdx.x;
+    for (uint32_t c0 = 0; c0 < T; c0 += WAVE) {
+      const uint32_t i = c0 + lane;
       const bool valid = i < T;
       const uint8_t tg = valid ? scrTag[src + i] : 0;
-      const bool isF = valid && tg == 0;
       vl_box b;
       if (valid) b = scr[src + i];
-      int totF, totS;
-      const int pf = block_excl_flag_scan<256>(isF, ws, &totF);
-      const int ps = block_excl_flag_scan<256>(valid && !isF, ws, &totS);
+      const bool isF = valid && tg == 0;
+      const uint64_t mF = __ballot(isF), mS = __ballot(valid && !isF);
       if (isF) {
-        out[dF + pf] = b;
+        out[dF + __popcll(mF & lt)] = b;
       } else if (valid) {
-        side[dS + ps] = b;
-        side_tag[dS + ps] = tg;
+        const uint64_t p = dS + __popcll(mS & lt);
+        side[p] = b;
+        side_tag[p] = tg;
       }
-      dF += totF;
-      dS += totS;
+      dF += __popcll(mF);
+      dS += __popcll(mS);
     }
   }
 }
