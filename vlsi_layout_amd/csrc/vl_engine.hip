@@ -542,9 +542,9 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
   uint64_t total = 0;
   VLCK(readback(e, &total, pos + n, sizeof(total)));
   Scratch s2(e);
-  uint64_t* keys = s2.get<uint64_t>(total + 1);
+  uint32_t* keys = s2.get<uint32_t>(total + 1);
   uint32_t* vals = s2.get<uint32_t>(total + 1);
-  uint64_t* keys2 = s2.get<uint64_t>(total + 1);
+  uint32_t* keys2 = s2.get<uint32_t>(total + 1);
   uint32_t* vals2 = s2.get<uint32_t>(total + 1);
   void* rtmp = s2.get<char>(radix_plan((int64_t)total).tmp_bytes);
   vl_box* entries = keep.get<vl_box>(total + 1);
@@ -552,8 +552,8 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
   if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
   launch_bin_emit(a, na, xr, b, nb, g, pos, keys, vals, e->st);
   int key_bits = 1;
-  while (key_bits < 63 && ((uint64_t)1 << key_bits) < (uint64_t)(2 * nt)) key_bits++;
-  if (radix_sort_pairs_bits(keys, vals, keys2, vals2, (int64_t)total, key_bits, rtmp, e->st)) {
+  while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)(2 * nt)) key_bits++;
+  if (radix_sort_pairs_bits32(keys, vals, keys2, vals2, (int64_t)total, key_bits, rtmp, e->st)) {
     std::swap(keys, keys2);
     std::swap(vals, vals2);
   }
@@ -568,9 +568,9 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
     auto& k = e->kst[KS_BIN];
     k.launches += 1;
     k.ms += ms;
-    // algorithmic bytes: read the boxes twice (span, emit), sort 12-byte pairs,
-    // gather 16 B per entry, write 16 B per entry
-    k.bytes += 32.0 * (double)n + 12.0 * 2.0 * (double)total + 32.0 * (double)total;
+    // algorithmic bytes: read the boxes twice (span, emit), write and sort
+    // 8-byte pairs, gather 16 B per entry, write 16 B per entry
+    k.bytes += 32.0 * (double)n + 8.0 * 2.0 * (double)total + 32.0 * (double)total;
   }
   *entries_out = entries;
   *start_out = start;

@@ -96,6 +96,9 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
 // are in keys_alt/vals_alt (odd number of passes), false when in keys/vals.
 bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
                            int key_bits, void* tmp, hipStream_t st);
+// The same for 32-bit keys (key_bits <= 32).
+bool radix_sort_pairs_bits32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                             int key_bits, void* tmp, hipStream_t st);
 void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st);
 void launch_tracker_gather(const vl_tracker* in, const uint32_t* perm, vl_tracker* out, int64_t n,
                            hipStream_t st);
@@ -111,9 +114,9 @@ void launch_perm_to_index1(const uint32_t* perm, int64_t* idx, int64_t n, hipStr
 void launch_bin_span(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
                      hipStream_t st);
 void launch_bin_emit(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g,
-                     const uint64_t* pos, uint64_t* keys, uint32_t* vals, hipStream_t st);
+                     const uint64_t* pos, uint32_t* keys, uint32_t* vals, hipStream_t st);
 // start[k] = number of keys < k, for k = 0..nkeys.
-void launch_bin_bounds(const uint64_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st);
+void launch_bin_bounds(const uint32_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st);
 void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, const uint32_t* vals,
                        uint64_t total, vl_box* entries, hipStream_t st);
 // Tile descriptors from the key bounds (start has 2*ntiles+1 entries).

@@ -39,7 +39,8 @@ __global__ void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, uint
     if (h[i]) atomicAdd(&ghist[i], h[i]);
 }
 
-__global__ __launch_bounds__(RS_NT) void k_rs_upsweep(const uint64_t* __restrict__ keys, int64_t n,
+template <typename K>
+__global__ __launch_bounds__(RS_NT) void k_rs_upsweep(const K* __restrict__ keys, int64_t n,
                                                        int shift, uint32_t* bhist, int nb) {
   __shared__ uint32_t h[RS_BINS];
   h[threadIdx.x] = 0;
@@ -54,13 +55,14 @@ __global__ __launch_bounds__(RS_NT) void k_rs_upsweep(const uint64_t* __restrict
   bhist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const uint64_t* __restrict__ kin,
+template <typename K>
+__global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const K* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin,
-                                                         uint64_t* __restrict__ kout,
+                                                         K* __restrict__ kout,
                                                          uint32_t* __restrict__ vout, int64_t n,
                                                          int shift, const uint32_t* __restrict__ goff,
                                                          int nb) {
-  __shared__ uint64_t sk[RS_TILE];
+  __shared__ K sk[RS_TILE];
   __shared__ uint32_t sv[RS_TILE];
   __shared__ uint32_t wcnt[RS_NW][RS_BINS];
   __shared__ uint32_t dstart[RS_BINS];
@@ -71,14 +73,14 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const uint64_t* __restri
   for (int i = threadIdx.x; i < RS_NW * RS_BINS; i += RS_NT) (&wcnt[0][0])[i] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-  uint64_t k[RS_ITEMS];
+  K k[RS_ITEMS];
   uint32_t v[RS_ITEMS];
   uint32_t r[RS_ITEMS];
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; it++) {
     int64_t idx = base + (int64_t)w * (WAVE * RS_ITEMS) + it * WAVE + lane;
     bool valid = idx < n;
-    k[it] = valid ? kin[idx] : ~0ull;
+    k[it] = valid ? kin[idx] : (K)~(K)0;
     v[it] = valid ? vin[idx] : 0u;
   }
   const uint64_t lt = lanemask_lt();
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const uint64_t* __restri
   const int64_t rem = n - base;
   const int nvalid = rem < RS_TILE ? (int)rem : RS_TILE;
   for (int i = threadIdx.x; i < nvalid; i += RS_NT) {
-    const uint64_t key = sk[i];
+    const K key = sk[i];
     const uint32_t d = (uint32_t)(key >> shift) & 255u;
     const uint32_t g = gbase[d] + ((uint32_t)i - dstart[d]);
     kout[g] = key;
@@ -136,7 +138,8 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const uint64_t* __restri
   }
 }
 
-__global__ void k_copy_pairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, int64_t n) {
+template <typename K>
+__global__ void k_copy_pairs(const K* ks, const uint32_t* vs, K* kd, uint32_t* vd, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     kd[i] = ks[i];
     vd[i] = vs[i];
@@ -247,18 +250,21 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
       if ((int64_t)h[pass * RS_BINS + d] == n) trivial = true;
     if (trivial) continue;
     const int shift = 8 * pass;
-    hipLaunchKernelGGL(k_rs_upsweep, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
+    hipLaunchKernelGGL(k_rs_upsweep<uint64_t>, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
     scan_exclusive_u32(bhist, goff, hb, nullptr, stmp, st);
-    hipLaunchKernelGGL(k_rs_downsweep, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
+    hipLaunchKernelGGL(k_rs_downsweep<uint64_t>, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
     uint64_t* tk = ks; ks = kd; kd = tk;
     uint32_t* tv = vs; vs = vd; vd = tv;
     done++;
   }
-  if (done & 1) hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n)), dim3(256), 0, st, ks, vs, keys, vals, n);
+  if (done & 1)
+    hipLaunchKernelGGL(k_copy_pairs<uint64_t>, dim3(grid_for(n)), dim3(256), 0, st, ks, vs, keys, vals, n);
 }
 
-bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                           int key_bits, void* tmp, hipStream_t st) {
+namespace {
+template <typename K>
+bool sort_bits(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n, int key_bits, void* tmp,
+               hipStream_t st) {
   if (n <= 1 || key_bits <= 0) return false;
   RadixPlan p = radix_plan(n);
   const int nb = p.nblocks;
@@ -267,20 +273,31 @@ bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, u
   uint32_t* goff = bhist + hb;
   char* stmp = (char*)(goff + hb);
   stmp = (char*)(((uintptr_t)stmp + 255) & ~(uintptr_t)255);
-  uint64_t* ks = keys;
+  K* ks = keys;
   uint32_t* vs = vals;
-  uint64_t* kd = keys_alt;
+  K* kd = keys_alt;
   uint32_t* vd = vals_alt;
   bool alt = false;
-  for (int shift = 0; shift < key_bits && shift < 64; shift += 8) {
-    hipLaunchKernelGGL(k_rs_upsweep, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
+  for (int shift = 0; shift < key_bits && shift < (int)(8 * sizeof(K)); shift += 8) {
+    hipLaunchKernelGGL(k_rs_upsweep<K>, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
     scan_exclusive_u32(bhist, goff, hb, nullptr, stmp, st);
-    hipLaunchKernelGGL(k_rs_downsweep, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
-    uint64_t* tk = ks; ks = kd; kd = tk;
+    hipLaunchKernelGGL(k_rs_downsweep<K>, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
+    K* tk = ks; ks = kd; kd = tk;
     uint32_t* tv = vs; vs = vd; vd = tv;
     alt = !alt;
   }
   return alt;
+}
+}  // namespace
+
+bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                           int key_bits, void* tmp, hipStream_t st) {
+  return sort_bits<uint64_t>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
+}
+
+bool radix_sort_pairs_bits32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                             int key_bits, void* tmp, hipStream_t st) {
+  return sort_bits<uint32_t>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
 }
 
 void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st) {

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_bin_span(const vl_box* __restrict__ a, 
 
 __global__ __launch_bounds__(256) void k_bin_emit(const vl_box* __restrict__ a, int64_t na, Xform xra,
                                                   const vl_box* __restrict__ b, int64_t nb, Grid g,
-                                                  const uint64_t* __restrict__ pos, uint64_t* __restrict__ keys,
+                                                  const uint64_t* __restrict__ pos, uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ vals) {
   const int64_t n = na + nb;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -94,24 +94,24 @@ __global__ __launch_bounds__(256) void k_bin_emit(const vl_box* __restrict__ a, 
     const vl_box v = isA ? a[i] : b[i - na];
     if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) continue;
     uint64_t p = pos[i];
-    const uint64_t layer = isA ? 0ull : 1ull;
+    const uint32_t layer = isA ? 0u : 1u;
     for (int r = r1; r <= r2; r++)
       for (int c = c1; c <= c2; c++) {
-        keys[p] = (((uint64_t)r * (uint64_t)g.cols + (uint64_t)c) << 1) | layer;
+        keys[p] = (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
         vals[p] = (uint32_t)i;
         p++;
       }
   }
 }
 
-__global__ __launch_bounds__(256) void k_bin_bounds(const uint64_t* __restrict__ keys, uint64_t total,
+__global__ __launch_bounds__(256) void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t total,
                                                     int64_t nkeys, uint64_t* __restrict__ start) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= nkeys;
        k += (int64_t)gridDim.x * blockDim.x) {
     uint64_t lo = 0, hi = total;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) >> 1;
-      if (keys[mid] < (uint64_t)k) lo = mid + 1; else hi = mid;
+      if ((int64_t)keys[mid] < k) lo = mid + 1; else hi = mid;
     }
     start[k] = lo;
   }
@@ -1255,12 +1255,12 @@ void launch_bin_span(const vl_box* a, int64_t na, Xform xra, const vl_box* b, in
     hipLaunchKernelGGL(k_bin_span, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, b, nb, g, cnt);
 }
 void launch_bin_emit(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g,
-                     const uint64_t* pos, uint64_t* keys, uint32_t* vals, hipStream_t st) {
+                     const uint64_t* pos, uint32_t* keys, uint32_t* vals, hipStream_t st) {
   if (na + nb > 0)
     hipLaunchKernelGGL(k_bin_emit, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, b, nb, g, pos, keys,
                        vals);
 }
-void launch_bin_bounds(const uint64_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st) {
+void launch_bin_bounds(const uint32_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st) {
   hipLaunchKernelGGL(k_bin_bounds, dim3(grid_for(nkeys + 1)), dim3(256), 0, st, keys, total, nkeys, start);
 }
 void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, const uint32_t* vals,
