@@ -68,9 +68,9 @@ int vl_engine_create(int device, vl_engine** out);
 int vl_engine_destroy(vl_engine* e);
 /* Tuning knobs (results never depend on them):
  *   "tile_nmax"    wave-kernel tile capacity, 63 (default) or 127 boxes
- *   "tile_target"  mean boxes per tile the operator grid aims for (16)
+ *   "tile_target"  mean boxes per tile the operator grid aims for (20)
  *   "erode_target" the same for SIZE(-d) erosion grids (12)
- *   "level_factor" row coarsening per seam level (16)
+ *   "level_factor" row coarsening per seam level (1024)
  *   "erode_kernel" 1: SIZE(-d) by the per-tile erosion kernel (default);
  *                  0: by complement, grow and difference
  *   "max_wg"       cap on the tile-kernel grid (0 = occupancy limit)

@@ -126,9 +126,9 @@ struct vl_engine {
   uint64_t* h_stage = nullptr;   // pinned, 4 KiB
   // tuning
   int nmax = 63;                 // wave-kernel tile capacity (63 or 127 boxes)
-  int target = 16;               // mean boxes per tile the grid aims for
+  int target = 20;               // mean boxes per tile the grid aims for
   int erode_target = 12;         // same for SIZE(-d) erosion grids (windows carry a halo)
-  int level_factor = 16;
+  int level_factor = 1024;       // row coarsening per seam level
   int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
   double last_ms = 0.0;

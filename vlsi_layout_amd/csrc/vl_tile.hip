@@ -489,12 +489,13 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       }
       continue;
     }
-    // 0. the boxes are in LDS.  AND and NOT prune first: a box of B that
-    //    meets no box of A inside the tile cannot change A AND B or A \ B,
-    //    and neither can a box of A that meets no box of B change A AND B.
-    //    An AND tile without any such pair has an empty result.
+    // 0. the boxes are in LDS.  AND prunes first: a box of either layer that
+    //    meets no box of the other inside the tile cannot change A AND B, and
+    //    an AND tile without any such pair has an empty result.  (The same
+    //    test would let NOT drop B boxes, but measured on the OP1 ring the
+    //    test costs more than it saves there.)
     const vl_box* bx = sb;
-    if (TWO && (op == TOP_AND || op == TOP_NOT)) {
+    if (TWO && op == TOP_AND) {
       constexpr bool TWOREG = NMAX >= WAVE;   // boxes lane and lane+64
       const int nB = n - nA;
       const uint64_t lt = lanemask_lt();
@@ -531,8 +532,8 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
         }
       }
       // compact the kept boxes (A first, then B), keeping their order
-      const bool kA1 = lane < nA && (op == TOP_NOT || ((hitA1 >> lane) & 1ull));
-      const bool kA2 = TWOREG && lane + WAVE < nA && (op == TOP_NOT || ((hitA2 >> lane) & 1ull));
+      const bool kA1 = lane < nA && ((hitA1 >> lane) & 1ull);
+      const bool kA2 = TWOREG && lane + WAVE < nA && ((hitA2 >> lane) & 1ull);
       const uint64_t mA1 = __ballot(kA1), mA2 = __ballot(kA2);
       const uint64_t mB1 = __ballot(keepB1), mB2 = __ballot(keepB2);
       const int cA1 = __popcll(mA1), cA = cA1 + __popcll(mA2);
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       nA = cA;
       n = cA + cB;
       bx = sk;
-      if (n == 0 || (op == TOP_AND && (cA == 0 || cB == 0))) {
+      if (cA == 0 || cB == 0) {
         if (lane == 0) {
           o.tab_n[ti] = 0;
           o.tab_n[o.ntab + ti] = 0;
