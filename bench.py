@@ -9,8 +9,10 @@ every layer:
     and1 = outer * L        ring = outer * inner   ASSERT_ZERO(and1), ASSERT_ZERO(ring)
 Throughput = input boxes of all operators of a step (N_A+N_B for binary ops,
 N for GROW/SIZE) / wall time of the step, with the engine stream synchronised
-on both sides of the timed region.  "plain" runs one operator per call;
-"batched" runs the same operators as one recorded program.
+on both sides of the timed region.  "plain" runs one operator per call with
+the default tile budgets; "batched" runs the same operators as one recorded
+program (independent layer chains concurrently on worker streams) after the
+autotuner has picked the tile budgets.
 
 The roofline figure is for the per-tile Boolean kernel (the dominant kernel):
 algorithmic bytes of its launches in the timed region (16 B per entry read,
@@ -57,6 +59,9 @@ def parse():
     p.add_argument("--serving-boxes", type=int, default=50_000_000,
                    help="serving design size over 4 layers (0 = skip the serving measurement)")
     p.add_argument("--serving-requests", type=int, default=400)
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="first copy this many MiB device-to-device (a copy kernel of known size for "
+                        "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -101,6 +106,19 @@ def time_steps(fn, eng, steps, warmup):
         out = fn()
     eng.sync()
     return time.perf_counter() - t0, out
+
+
+def tile_stage_traffic(npl):
+    """Measured HBM bytes per tile stage (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy kernel) from the committed profile of this workload,
+    or None when the workload differs from the profiled one."""
+    path = os.path.join(ROOT, "profiles", "tile_stage_traffic_stdcell_10M_per_layer.md")
+    if npl != 10_000_000 or not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("traffic_bytes_per_stage = "):
+            return float(ln.split("=")[1])
+    return None
 
 
 def serving(V, args):
@@ -166,6 +184,10 @@ def main():
     print(f"[bench] generated 4 x {npl} boxes in {t_gen:.1f} s", file=sys.stderr, flush=True)
     eng = V.Engine(0)
     rings = [Ring(V, eng, lay) for lay in layers]
+    if args.calibrate_copy_mib:
+        cal = V.Layer(eng).copy_from(rings[0].L, args.calibrate_copy_mib * (1 << 20) // 16)
+        eng.sync()
+        del cal
     result = {}
 
     def plain_step():
@@ -190,7 +212,12 @@ def main():
     print(f"[bench] first step ok: {boxes_per_step} input boxes per step", file=sys.stderr, flush=True)
 
     modes = ["plain", "batched"] if args.mode == "both" else [args.mode]
+    tuned = None
     for m in modes:
+        if m == "batched":
+            # the batched form runs with autotuned tile budgets (timed on met1's ring inputs)
+            tuned = eng.autotune(rings[2].L1, rings[2].L)
+            print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
         eng.set("profile", 1)
         secs, out = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps, args.warmup)
         fails = out[1] if m == "plain" else out
@@ -212,6 +239,7 @@ def main():
     head = "batched" if "batched" in result else "plain"
     tk = result[head]["tile_kernel"]
     achieved = tk["gbs"]
+    traffic = tile_stage_traffic(npl)
     line = {
         "metric": METRIC,
         "value": result[head]["boxes_per_s"],
@@ -230,11 +258,12 @@ def main():
                    "input_boxes_per_step": int(boxes_per_step)},
         "plain": result.get("plain"),
         "batched": result.get("batched"),
+        "autotuned": tuned,
         "batched_over_plain": (result["batched"]["boxes_per_s"] / result["plain"]["boxes_per_s"]
                                if "batched" in result and "plain" in result else None),
-        "roofline": {"bound": "hbm", "kernel": "k_tile (per-tile Boolean)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "tile stage (k_tile_w tiers + k_tile_b, per operator level)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
         "generation_s": t_gen,
     }
     if not args.no_cpu_baseline:

@@ -79,6 +79,13 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
 int vl_engine_get_param(vl_engine* e, const char* key, int64_t* value);
 /* Block until all work queued on the engine's stream has finished. */
 int vl_engine_sync(vl_engine* e);
+/* Autotuner (no counterpart in the reference): sets "tile_target" and
+ * "erode_target" to the fastest of a few candidates, timed on A NOT B (or the
+ * merge of A when b is NULL) plus GROW(A; 1), and on SIZE(A; -1).  The chosen
+ * values are returned when the pointers are non-null.  Results of every
+ * operator are independent of these parameters. */
+int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64_t* target,
+                       int64_t* erode_target);
 
 /* ---- layers (the Layer type of src/design.f90:65-82, device-resident) ---- */
 int vl_layer_create(vl_engine* e, vl_layer** out);

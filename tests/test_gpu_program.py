@@ -100,3 +100,20 @@ def test_program_error_from_a_worker(gpu_engine):
     assert ei.value.code == V.VL_EINVAL
     prog.close()
     eng.close()
+
+
+def test_autotune_sets_parameters_and_keeps_results(gpu_engine):
+    eng = V.Engine(0)
+    lay = V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 2, 30000)
+    L = V.Layer(eng, lay)
+    L1 = V.Layer(eng)
+    V.CalculateGROWLayer(L, L1, (1, 1, 1, 1))
+    before = V.Layer(eng)
+    V.CalculateFrameNOT(L1, L, before)
+    chosen = eng.autotune(L1, L)
+    assert chosen["tile_target"] in (12, 16, 20, 24, 32) and chosen["erode_target"] in (8, 12, 16)
+    assert eng.get("tile_target") == chosen["tile_target"]
+    after = V.Layer(eng)
+    V.CalculateFrameNOT(L1, L, after)
+    assert np.array_equal(O.sort_boxes(before.boxes()), O.sort_boxes(after.boxes()))
+    eng.close()

@@ -122,6 +122,7 @@ def lib():
             "vl_program_add": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
             "vl_program_run": (C.c_int, [vp]),
             "vl_program_assert_count": (C.c_int, [vp, C.POINTER(i64)]),
+            "vl_engine_autotune": (C.c_int, [vp, vp, vp, C.POINTER(i64), C.POINTER(i64)]),
             "vl_server_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
             "vl_server_run": (C.c_int, [vp, vp, i64, vp]),
             "vl_server_destroy": (C.c_int, [vp]),
@@ -197,6 +198,13 @@ class Engine:
     def layer(self, boxes=None):
         return Layer(self, boxes)
 
+    def autotune(self, a, b=None):
+        """Picks tile_target / erode_target by timing candidates on layers a (and b)."""
+        t, et = C.c_int64(), C.c_int64()
+        _check(lib().vl_engine_autotune(self._h, a.handle, b.handle if b is not None else None, C.byref(t),
+                                        C.byref(et)))
+        return {"tile_target": t.value, "erode_target": et.value}
+
 
 class Layer:
     """A device-resident box layer (the reference's type(Layer), src/design.f90:65-82)."""
@@ -233,6 +241,12 @@ class Layer:
         got = C.c_int64()
         _check(lib().vl_layer_download(self._h, out.ctypes.data_as(C.c_void_p), n, C.byref(got)))
         return out
+
+    def copy_from(self, other, n=None):
+        """Device-to-device copy of the first n boxes of another layer of the same device."""
+        n = other.n_used if n is None else min(int(n), other.n_used)
+        _check(lib().vl_layer_upload_device(self._h, other.device_ptr(), n))
+        return self
 
     def device_ptr(self):
         p = C.c_void_p()

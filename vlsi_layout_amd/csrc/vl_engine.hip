@@ -1823,3 +1823,69 @@ int vl_server_run(vl_server* sv, const vl_query* q, int64_t n, vl_query_result* 
 }
 
 }  // extern "C"
+
+// ---- autotuner -------------------------------------------------------------------
+extern "C" {
+
+// Picks the tile box budget of the operator grid ("tile_target") and of the
+// erosion grid ("erode_target") by timing candidates on the caller's layers:
+// A NOT B (or the merge of A) plus GROW(A; 1) for the operator grid, SIZE(A; -1)
+// for the erosion grid; each candidate is timed twice (device time of the
+// operator, HIP events) and the faster run counts.  Results never depend on
+// these parameters, only run time.
+int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64_t* target, int64_t* erode_target) {
+  if (!e || !a) return fail(VL_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  vl_layer* c = nullptr;
+  VLCK(vl_layer_create(e, &c));
+  auto best_of = [&](const std::function<int()>& f, double* ms) {
+    double m = 1e30;
+    for (int rep = 0; rep < 2; rep++) {
+      int rc = f();
+      if (rc != VL_OK) return rc;
+      m = std::min(m, e->last_ms);
+    }
+    *ms = m;
+    return VL_OK;
+  };
+  int rc = VL_OK;
+  const int targets[] = {12, 16, 20, 24, 32};
+  const int etargets[] = {8, 12, 16};
+  const int t0 = e->target, et0 = e->erode_target;
+  int bt = t0, bet = et0;
+  double bms = 1e30;
+  for (int t : targets) {
+    e->target = t;
+    double m1 = 0, m2 = 0;
+    rc = best_of([&]() { return b ? vl_not(e, a, b, c) : vl_merge(e, a, c); }, &m1);
+    if (rc == VL_OK) rc = best_of([&]() { return vl_grow(e, a, 1, 1, 1, 1, c); }, &m2);
+    if (rc != VL_OK) break;
+    if (m1 + m2 < bms) {
+      bms = m1 + m2;
+      bt = t;
+    }
+  }
+  e->target = bt;
+  bms = 1e30;
+  for (int et : etargets) {
+    if (rc != VL_OK) break;
+    e->erode_target = et;
+    double m = 0;
+    rc = best_of([&]() { return vl_size(e, a, -1, c); }, &m);
+    if (rc == VL_OK && m < bms) {
+      bms = m;
+      bet = et;
+    }
+  }
+  e->erode_target = bet;
+  if (rc != VL_OK) {
+    e->target = t0;
+    e->erode_target = et0;
+  }
+  vl_layer_destroy(c);
+  if (target) *target = e->target;
+  if (erode_target) *erode_target = e->erode_target;
+  return rc;
+}
+
+}  // extern "C"
