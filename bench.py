@@ -237,7 +237,9 @@ def main():
         print(f"[bench] {m}: {result[m]['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
 
     head = "batched" if "batched" in result else "plain"
-    tk = result[head]["tile_kernel"]
+    # kernel efficiency is taken from the plain run (one stage in flight at a
+    # time); in the batched run concurrent chains stretch each stage's duration
+    tk = result["plain" if "plain" in result else head]["tile_kernel"]
     achieved = tk["gbs"]
     traffic = tile_stage_traffic(npl)
     line = {
