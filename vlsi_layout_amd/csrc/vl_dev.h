@@ -14,23 +14,108 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// Inclusive wave64 scan.
-template <typename T>
-__device__ __forceinline__ T wave_incl_scan(T v) {
-  const int l = __lane_id();
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    T o = __shfl_up(v, d, WAVE);
-    if (l >= d) v += o;
+// Lane exchange v(lane ^ M) without the LDS crossbar where the hardware has a
+// cheaper path: DPP quad permutes for 1 and 2, the DPP 16-lane row rotate by
+// 8 for 8, ds_swizzle in bit mode (no LDS access) for 4 and 16, and
+// ds_bpermute only for 32.
+template <int M>
+__device__ __forceinline__ int xor_lanes(int v) {
+  static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16 || M == 32, "xor distance");
+  if constexpr (M == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);        // quad_perm 1,0,3,2
+  else if constexpr (M == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  else if constexpr (M == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  else if constexpr (M == 4 || M == 16) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (M << 10));
+  else return __shfl_xor(v, 32, WAVE);
+}
+
+// The same with the distance known only after unrolling (the switch folds).
+__device__ __forceinline__ int xor_lanes(int v, int m) {
+  switch (m) {
+    case 1: return xor_lanes<1>(v);
+    case 2: return xor_lanes<2>(v);
+    case 4: return xor_lanes<4>(v);
+    case 8: return xor_lanes<8>(v);
+    case 16: return xor_lanes<16>(v);
+    case 32: return xor_lanes<32>(v);
+    default: return __shfl_xor(v, m, WAVE);
   }
-  return v;
+}
+
+__device__ __forceinline__ uint64_t xor_lanes64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)xor_lanes((int)(uint32_t)v, m);
+  const uint32_t hi = (uint32_t)xor_lanes((int)(uint32_t)(v >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// 32- or 64-bit value of lane ^ M / of the lane N below in the same 16-lane
+// row (0 where that lane is outside the row), through DPP or swizzle.
+template <int M, typename T>
+__device__ __forceinline__ T xor_lanes_t(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)xor_lanes<M>((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)xor_lanes<M>((int)(uint32_t)(u >> 32));
+    return (T)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (T)xor_lanes<M>((int)v);
+  }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ T row_shr(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x110 + N, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x110 + N, 0xF, 0xF, false);
+    return (T)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + N, 0xF, 0xF, false);
+  }
 }
 
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, WAVE);
+__device__ __forceinline__ T read_lane(T v, int l) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return (T)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (T)__builtin_amdgcn_readlane((int)v, l);
+  }
+}
+
+// Inclusive wave64 scan: DPP shifts within each 16-lane row, then the row
+// totals (lanes 15, 31, 47) are added to the rows above them.
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = __lane_id();
+  const int r = l & 15;
+  T t = row_shr<1>(v);
+  if (r >= 1) v += t;
+  t = row_shr<2>(v);
+  if (r >= 2) v += t;
+  t = row_shr<4>(v);
+  if (r >= 4) v += t;
+  t = row_shr<8>(v);
+  if (r >= 8) v += t;
+  const T s0 = read_lane(v, 15), s1 = read_lane(v, 31), s2 = read_lane(v, 47);
+  const int row = l >> 4;
+  if (row >= 1) v += s0;
+  if (row >= 2) v += s1;
+  if (row >= 3) v += s2;
   return v;
+}
+
+// Sum over the wave, returned in every lane.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  v += xor_lanes_t<1>(v);
+  v += xor_lanes_t<2>(v);
+  v += xor_lanes_t<4>(v);
+  v += xor_lanes_t<8>(v);
+  v += xor_lanes_t<16>(v);
+  return read_lane(v, 0) + read_lane(v, 32);
 }
 
 // Block-wide exclusive scan of one value per thread.  `ws` must hold

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const K* __restrict__ ki
       old = wcnt[w][d];
       wcnt[w][d] = old + (uint32_t)__popcll(peers);
     }
-    old = __shfl(old, leader, WAVE);
+    old = __shfl(old, leader, WAVE);   // per-lane source (each digit group has its own leader)
     r[it] = old + (uint32_t)__popcll(peers & lt);
   }
   __syncthreads();

@@ -237,12 +237,6 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, WAVE);
-  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, WAVE);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // 64x64 bit-matrix transpose across a wave: lane l holds row l (bit c = entry
 // (l, c)) and gets column l.  Six block-swap stages: at stage s, entry (l, c)
 // with bit s of l clear and bit s of c set trades places with (l^s, c^s).
@@ -252,7 +246,7 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     const int sft = 32 >> i;
-    const uint64_t y = shfl_xor64(x, sft);
+    const uint64_t y = xor_lanes64(x, sft);
     x = (lane & sft) ? ((x & ~M[i]) | ((y >> sft) & M[i])) : ((x & M[i]) | ((y << sft) & ~M[i]));
   }
   return x;
@@ -260,7 +254,7 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
 
 __device__ __forceinline__ uint64_t wave_excl_u32(uint32_t v, uint32_t* tot) {
   uint32_t inc = wave_incl_scan(v);
-  *tot = __shfl(inc, WAVE - 1, WAVE);
+  *tot = read_lane(inc, WAVE - 1);
   return inc - v;
 }
 
@@ -466,7 +460,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     }
     __syncthreads();
   for (int jb = 0; jb < nbt; jb++) {
-    const uint32_t ti = (uint32_t)__shfl((int)my_ti, jb, WAVE);
+    const uint32_t ti = read_lane(my_ti, jb);
     const TileDesc d = sdesc[jb];
     vl_box* sb = sbuf + jb * (NMAX + 1);
     int nA = d.nA;
@@ -598,8 +592,8 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
 #pragma unroll
           for (int k = 0; k < K; k++) {
             const int e = lane * K + k;
-            const int px = __shfl_xor(vx[k], lj, WAVE);
-            const int py = __shfl_xor(vy[k], lj, WAVE);
+            const int px = xor_lanes(vx[k], lj);
+            const int py = xor_lanes(vy[k], lj);
             const bool take_min = ((e & j) == 0) == ((e & size) == 0);
             vx[k] = take_min ? min(vx[k], px) : max(vx[k], px);
             vy[k] = take_min ? min(vy[k], py) : max(vy[k], py);
@@ -1225,11 +1219,18 @@ int num_cus() {
   return ncu;
 }
 
+// Workgroups of a persistent launch: the occupancy limit, but for the
+// one-wave tile kernels at most 24 waves per CU (6 per SIMD): measured on the
+// 10M-box OP1 ring, 7 or 8 resident waves per SIMD run the tile stage 7 %
+// slower than 6 (and 4 run it 14 % slower).
+constexpr int WAVE_KERNEL_WAVES_PER_CU = 24;
+
 template <class KF>
 int occupancy_wg(KF kern, int threads) {
   int per = 0;
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, threads, 0);
   if (per < 1) per = 1;
+  if (threads == WAVE && per > WAVE_KERNEL_WAVES_PER_CU) per = WAVE_KERNEL_WAVES_PER_CU;
   return per * num_cus();
 }
 
