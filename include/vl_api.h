@@ -74,7 +74,12 @@ int vl_engine_destroy(vl_engine* e);
  *   "erode_kernel" 1: SIZE(-d) by the per-tile erosion kernel (default);
  *                  0: by complement, grow and difference
  *   "max_wg"       cap on the tile-kernel grid (0 = occupancy limit)
- *   "profile"      1: collect per-kernel statistics (vl_engine_kernel_stats) */
+ *   "fuse"         programs: 1 (default) evaluates groups of operators over
+ *                  one source layer in one fused tile pass; 0 runs every
+ *                  operator on its own
+ *   "fuse_target"  mean boxes per tile of a fused group's grid (8)
+ *   "profile"      1: collect per-kernel statistics (vl_engine_kernel_stats)
+ * Read-only: "fused_groups" = groups run fused by programs on this engine. */
 int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
 int vl_engine_get_param(vl_engine* e, const char* key, int64_t* value);
 /* Block until all work queued on the engine's stream has finished. */
@@ -173,8 +178,17 @@ void thrust_sort_boxes_indirect(vl_box* d_boxes, int64_t* d_indices, int64_t n);
  * when it reads or writes a layer i writes, or writes a layer i reads) and runs
  * up to four chains concurrently, each in program order on its own worker
  * stream and host thread, after the work already queued on the engine.
- * Results are identical to running the operators one by one.  Layers used by
- * a program must not be touched by other calls while it runs. */
+ * Before that, consecutive operators that all derive from one source layer S
+ * are grouped (a GROW with non-negative arguments, SIZE or MERGE of S starts a
+ * group; it takes the following AND / NOT / OR / XOR / MERGE whose inputs are
+ * S or group outputs, GROW / SIZE of S itself (at most two distinct ones) and
+ * ASSERT_ZERO of group outputs, up to eight outputs): S is binned once and
+ * every output is computed per tile from the same masks.  A group that cannot
+ * run fused (tiles too dense to split, windows too large for the tiles) runs
+ * operator by operator.  Results are the same point sets, in canonical strips,
+ * as running the operators one by one (the order of the strips of a fused
+ * output may differ; vl_canonicalize sorts).  Layers used by a program must
+ * not be touched by other calls while it runs. */
 typedef struct vl_program vl_program;
 int vl_program_create(vl_engine* e, vl_program** out);
 int vl_program_destroy(vl_program* p);

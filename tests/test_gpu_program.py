@@ -1,7 +1,9 @@
 """Batched form: a recorded program (vl_program_*) runs its independent
-operator chains concurrently on worker streams; every result must equal the
-plain one-operator-per-call result bit for bit, and the oracle's canonical
-form after sorting."""
+operator chains concurrently on worker streams, and evaluates groups of
+operators over one source layer (the OP1 ring) in one fused tile pass.  Every
+result must equal the plain one-operator-per-call result: bit for bit in
+order without fusion, as the same canonical strip set (sorted) with it, and
+the oracle's canonical form after sorting."""
 import numpy as np
 import pytest
 
@@ -42,21 +44,28 @@ def op1_plain(L):
     return [L1, L2, outer, inner, and1, ring]
 
 
+@pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("nlayers", [1, 3, 6])
-def test_program_op1_chains_match_plain(gpu_engine, nlayers):
+def test_program_op1_chains_match_plain(gpu_engine, nlayers, fuse):
     eng = V.Engine(0)
+    eng.set("fuse", fuse)
     layers = [V.Layer(eng, V.synth_layer(V.SYN_STDCELL, 0x5EED0002, k % 4, 20000 + 3000 * k)) for k in range(nlayers)]
     temps = [[V.Layer(eng) for _ in range(6)] for _ in range(nlayers)]
     prog = V.Program(eng)
     for L, t in zip(layers, temps):
         op1_record(prog, L, t)
     for rep in range(2):                       # the second run reuses the worker engines
+        before = eng.get("fused_groups")
         prog.run()
         assert prog.assert_failures() == 0
+        assert eng.get("fused_groups") - before == (nlayers if fuse else 0)
         for L, t in zip(layers, temps):
             ref = op1_plain(L)
             for got, exp in zip(t, ref):
-                assert np.array_equal(got.boxes(), exp.boxes())
+                if fuse:   # same canonical strips, in the fused grid's order
+                    assert np.array_equal(O.sort_boxes(got.boxes()), O.sort_boxes(exp.boxes()))
+                else:
+                    assert np.array_equal(got.boxes(), exp.boxes())
     a = layers[0].boxes()
     assert np.array_equal(O.sort_boxes(temps[0][0].boxes()), O.grow(a, 1, 1, 1, 1))
     assert np.array_equal(O.sort_boxes(temps[0][1].boxes()), O.size(a, -1))
@@ -117,3 +126,138 @@ def test_autotune_sets_parameters_and_keeps_results(gpu_engine):
     V.CalculateFrameNOT(L1, L, after)
     assert np.array_equal(O.sort_boxes(before.boxes()), O.sort_boxes(after.boxes()))
     eng.close()
+
+
+def plain_apply(op, a, b, args):
+    """One operator through the one-call-per-operator entry points."""
+    eng = a.engine
+    out = V.Layer(eng)
+    if op == V.VL_OP_GROW:
+        V.CalculateGROWLayer(a, out, args)
+    elif op == V.K_BOOST_CONTROL_SIZE:
+        V.CalculateBoostOperation(a, a, out, V.K_BOOST_CONTROL_SIZE, args[0])
+    elif op == V.K_BOOST_CONTROL_MERGE:
+        V.CalculateBoostOperation(a, a, out, V.K_BOOST_CONTROL_MERGE)
+    else:
+        V.CalculateBoostOperation(a, b, out, op)
+    return out
+
+
+# Groups over one source S: (op, input a, input b, args); inputs name S or an
+# earlier step's output by index.
+GROUPS = {
+    "op1": [(V.VL_OP_GROW, "S", None, (1, 1, 1, 1)), (V.K_BOOST_CONTROL_SIZE, "S", None, (-1,)),
+            (V.K_BOOST_CONTROL_NOT, 0, "S", None), (V.K_BOOST_CONTROL_NOT, "S", 1, None),
+            (V.K_BOOST_CONTROL_AND, 2, "S", None), (V.K_BOOST_CONTROL_AND, 2, 3, None)],
+    "directional": [(V.VL_OP_GROW, "S", None, (3, 0, 1, 2)), (V.K_BOOST_CONTROL_SIZE, "S", None, (-2,)),
+                    (V.K_BOOST_CONTROL_XOR, 0, 1, None), (V.K_BOOST_CONTROL_OR, 1, "S", None),
+                    (V.K_BOOST_CONTROL_NOT, 0, 3, None)],
+    "two_grows": [(V.K_BOOST_CONTROL_SIZE, "S", None, (2,)), (V.VL_OP_GROW, "S", None, (0, 5, 0, 0)),
+                  (V.K_BOOST_CONTROL_AND, 0, 1, None), (V.K_BOOST_CONTROL_MERGE, "S", None, None),
+                  (V.K_BOOST_CONTROL_XOR, 2, 3, None), (V.K_BOOST_CONTROL_SIZE, "S", None, (0,))],
+    "two_erosions": [(V.K_BOOST_CONTROL_SIZE, "S", None, (-1,)), (V.K_BOOST_CONTROL_SIZE, "S", None, (-3,)),
+                     (V.K_BOOST_CONTROL_NOT, 0, 1, None), (V.K_BOOST_CONTROL_NOT, "S", 0, None)],
+}
+
+
+def run_group(eng, S, steps):
+    outs = [V.Layer(eng) for _ in steps]
+    prog = V.Program(eng)
+    for (op, a, b, args), o in zip(steps, outs):
+        la = S if a == "S" else outs[a]
+        lb = None if b is None else (S if b == "S" else outs[b])
+        prog.add(op, la, lb, o, args)
+    prog.add(-1, outs[-1])
+    prog.run()
+    fails = prog.assert_failures()
+    prog.close()
+    return outs, fails
+
+
+def plain_group(S, steps):
+    res = []
+    for op, a, b, args in steps:
+        la = S if a == "S" else res[a]
+        lb = None if b is None else (S if b == "S" else res[b])
+        res.append(plain_apply(op, la, lb, args))
+    return res
+
+
+@pytest.mark.parametrize("group", sorted(GROUPS))
+@pytest.mark.parametrize("layer", ["random", "stdcell", "dense"])
+def test_fused_groups_match_plain(gpu_engine, group, layer):
+    eng = V.Engine(0)
+    rng = np.random.default_rng(hash((group, layer)) & 0xFFFF)
+    if layer == "random":
+        s = rand_boxes(rng, 3000, 20000, 600)
+    elif layer == "stdcell":
+        s = V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 1, 15000)
+    else:   # 5-DBU grid, many touching and overlapping boxes
+        s = (rand_boxes(rng, 4000, 400, 6) * 5).astype(np.int32)
+    S = V.Layer(eng, s)
+    steps = GROUPS[group]
+    got, fails = run_group(eng, S, steps)
+    assert eng.get("fused_groups") == 1
+    exp = plain_group(S, steps)
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes())), (group, layer, k)
+    assert fails == (1 if len(exp[-1]) else 0)
+    if group == "op1":
+        assert len(got[4]) == 0 and len(got[5]) == 0
+        a = S.boxes()
+        assert np.array_equal(O.sort_boxes(got[0].boxes()), O.grow(a, 1, 1, 1, 1))
+        assert np.array_equal(O.sort_boxes(got[1].boxes()), O.size(a, -1))
+    eng.close()
+
+
+@pytest.mark.parametrize("target", [2, 30, 60])
+def test_fused_tiers_and_splits(gpu_engine, target):
+    """Grids far below and above the fused kernel's capacities: many empty
+    windows, second-tier tiles and x-sub-column splits."""
+    eng = V.Engine(0)
+    eng.set("fuse_target", target)
+    rng = np.random.default_rng(target)
+    s = rand_boxes(rng, 5000, 30000, 1500)
+    S = V.Layer(eng, s)
+    steps = GROUPS["op1"]
+    got, _ = run_group(eng, S, steps)
+    assert eng.get("fused_groups") == 1
+    exp = plain_group(S, steps)
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes())), (target, k)
+    eng.close()
+
+
+def test_fused_group_declined_falls_back(gpu_engine):
+    """More boxes over one spot than a split window can hold: the group runs
+    one operator at a time and still gives the plain results."""
+    eng = V.Engine(0)
+    s = np.concatenate([np.tile(np.array([[100, 100, 104, 140]], np.int32), (60, 1)),
+                        rand_boxes(np.random.default_rng(9), 200, 3000, 80)])
+    S = V.Layer(eng, s)
+    steps = GROUPS["op1"]
+    got, _ = run_group(eng, S, steps)
+    assert eng.get("fused_groups") == 0
+    exp = plain_group(S, steps)
+    for g, x in zip(got, exp):
+        assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes()))
+    eng.close()
+
+
+def test_fused_empty_source_and_unfusable_ops(gpu_engine):
+    eng = V.Engine(0)
+    E = V.Layer(eng)
+    got, fails = run_group(eng, E, GROUPS["op1"])
+    assert all(len(g) == 0 for g in got) and fails == 0
+    # GROW with a negative argument is not fused (and is an error on an invalidating box)
+    S = V.Layer(eng, [[0, 0, 10, 10], [20, 0, 22, 2]])
+    outs = [V.Layer(eng) for _ in range(3)]
+    prog = V.Program(eng)
+    prog.add(V.VL_OP_GROW, S, None, outs[0], (-1, 0, 0, 0))
+    prog.add(V.K_BOOST_CONTROL_SIZE, S, None, outs[1], (-1,))
+    prog.add(V.K_BOOST_CONTROL_OR, outs[0], outs[1], outs[2])
+    prog.run()
+    assert np.array_equal(O.sort_boxes(outs[2].boxes()), O.op_or(O.grow(S.boxes(), -1, 0, 0, 0), O.size(S.boxes(), -1)))
+    prog.close()
+    eng.close()
+

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -131,6 +132,9 @@ struct vl_engine {
   int level_factor = 1024;       // row coarsening per seam level
   int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
+  int fuse = 1;                  // programs: fuse operator groups over one source layer
+  int fuse_target = 8;           // mean boxes per tile of a fused group's grid
+  int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   double last_ms = 0.0;
   // kernel statistics (collected while prof is set)
   struct KStat {
@@ -304,9 +308,22 @@ bool two_layer(int top) { return top == TOP_AND || top == TOP_NOT || top == TOP_
 int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, uint64_t nentries,
                   bool two, bool area, int32_t erode, Outs& o, int depth);
 
-// Split overflow tiles into x-sub-columns and process them.
-int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
-                      const vl_box* d_entries, bool two, bool area, int32_t erode, Outs& o, int depth) {
+// Overflow tiles re-binned into x-sub-columns of about tgt boxes each.  halo >
+// 0: windows grown by halo (a box belongs to every sub-column its extent grown
+// by halo meets; sub-columns stay at least 2*halo wide).  Only x splits: the
+// seam levels rely on row borders shared by every tile of a grid row (a strip
+// crossing a row border is then cut in every tile it meets, and one not
+// crossing one in none), which a y split of one tile would break.  The arrays
+// live in s.
+struct SplitTiles {
+  TileDesc* desc = nullptr;
+  int n = 0;
+  vl_box* entries = nullptr;
+  uint64_t total = 0;
+};
+
+int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf, const vl_box* d_entries,
+                int32_t halo, int tgt, int cap, int depth, Scratch& s, SplitTiles* out) {
   if (depth > 24) return fail(VL_EINTERNAL, "tile splitting did not converge");
   std::vector<uint32_t> ovf(novf);
   std::vector<TileDesc> parents(novf);
@@ -326,27 +343,26 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
   }
   std::vector<TileDesc> subs;
   std::vector<int32_t> first(novf + 1);
-  const int tgt = std::max(8, e->target);
   for (uint32_t i = 0; i < novf; i++) {
     const TileDesc& p = parents[i];
     first[i] = (int32_t)subs.size();
     const int64_t n = (int64_t)p.nA + p.nB;
     const int64_t width = (int64_t)p.x1 - p.x0;
-    int64_t k = std::max<int64_t>(2, (n + tgt - 1) / tgt);
-    k = std::min(k, width);
-    // erosion windows overlap by twice the halo: keep sub-columns at least that wide
-    if (erode > 0) k = std::min<int64_t>(k, width / (2 * (int64_t)erode));
-    if (k < 2)
-      return fail(VL_EINTERNAL, "a 1-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
-                  (long long)n, TILE_BLOCK_NMAX);
-    const int64_t sw = (width + k - 1) / k;
-    k = (width + sw - 1) / sw;
-    for (int64_t j = 0; j < k; j++) {
+    int64_t kx = std::max<int64_t>(2, (n + tgt - 1) / tgt);
+    kx = std::min(kx, width);
+    // windows overlap by twice the halo: keep sub-columns at least that wide
+    if (halo > 0) kx = std::min<int64_t>(kx, width / (2 * (int64_t)halo));
+    if (kx < 2)
+      return fail(VL_EINTERNAL, "a %lld-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
+                  (long long)width, (long long)n, cap);
+    const int64_t sw = (width + kx - 1) / kx;
+    kx = (width + sw - 1) / sw;
+    for (int64_t c = 0; c < kx; c++) {
       TileDesc q = p;
-      q.x0 = (int32_t)(p.x0 + j * sw);
-      q.x1 = (int32_t)std::min<int64_t>(p.x0 + (j + 1) * sw, p.x1);
-      q.flags = (p.flags & (SEAM_B | SEAM_T)) | ((j > 0 || (p.flags & SEAM_L)) ? SEAM_L : 0u) |
-                ((j < k - 1 || (p.flags & SEAM_R)) ? SEAM_R : 0u);
+      q.x0 = (int32_t)(p.x0 + c * sw);
+      q.x1 = (int32_t)std::min<int64_t>(p.x0 + (c + 1) * sw, p.x1);
+      q.flags = (p.flags & (SEAM_B | SEAM_T)) | ((c > 0 || (p.flags & SEAM_L)) ? SEAM_L : 0u) |
+                ((c < kx - 1 || (p.flags & SEAM_R)) ? SEAM_R : 0u);
       q.nA = q.nB = 0;
       q.off = 0;
       subs.push_back(q);
@@ -354,7 +370,6 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
   }
   first[novf] = (int32_t)subs.size();
   const int ns = (int)subs.size();
-  Scratch s(e);
   TileDesc* d_par = s.get<TileDesc>(novf);
   TileDesc* d_sub = s.get<TileDesc>(ns);
   int32_t* d_first = s.get<int32_t>(novf + 1);
@@ -367,15 +382,29 @@ int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ov
   HIPCK(hipMemcpyAsync(d_first, first.data(), (novf + 1) * sizeof(int32_t), hipMemcpyHostToDevice, e->st));
   uint32_t *cA = cnt, *cB = cnt + ns, *fA = cnt + 2 * ns, *fB = cnt + 3 * ns;
   launch_fill_u32(cnt, 0u, 4 * (size_t)ns, e->st);
-  launch_split_count(d_par, (int)novf, d_first, d_sub, cA, cB, d_entries, erode, e->st);
+  launch_split_count(d_par, (int)novf, d_first, d_sub, cA, cB, d_entries, halo, e->st);
   scan_pair_counts(cA, cB, off, ns, off + ns, stmp, e->st);
   uint64_t total = 0;
   VLCK(readback(e, &total, off + ns, sizeof(total)));
   vl_box* sub_entries = s.get<vl_box>(total + 1);
   if (!sub_entries) return fail(VL_ENOMEM, "scratch");
-  launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, erode,
+  launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, halo,
                        e->st);
-  return process_tiles(e, d_sub, ns, sub_entries, total, two, area, erode, o, depth + 1);
+  out->desc = d_sub;
+  out->n = ns;
+  out->entries = sub_entries;
+  out->total = total;
+  return VL_OK;
+}
+
+// Split overflow tiles into x-sub-columns and process them.
+int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
+                      const vl_box* d_entries, bool two, bool area, int32_t erode, Outs& o, int depth) {
+  Scratch s(e);
+  SplitTiles sp;
+  VLCK(split_tiles(e, d_desc, d_ovf, novf, d_entries, erode, std::max(8, e->target), TILE_BLOCK_NMAX, depth, s,
+                   &sp));
+  return process_tiles(e, sp.desc, sp.n, sp.entries, sp.total, two, area, erode, o, depth + 1);
 }
 
 int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries, uint64_t nentries,
@@ -633,48 +662,10 @@ int op_mbrs(vl_engine* e, const vl_box* a, int64_t na, Xform xa, const vl_box* b
   return mbr2(e, a, na, xa, b, nb, ma, mb);
 }
 
-int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb,
-           vl_layer* c, long long* area_out, int32_t erode = 0, const Box64* ka = nullptr,
-           const Box64* kb = nullptr) {
-  if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; kb = nullptr; }
-  Box64 ma, mb;
-  VLCK(op_mbrs(e, a, na, xa, b, nb, ka, kb, &ma, &mb));
-  Box64 dom;
-  const bool ea = na <= 0, eb = nb <= 0;
-  switch (top) {
-    case TOP_AND:
-      if (ea || eb) { dom = Box64{0, 0, 0, 0}; break; }
-      dom = Box64{std::max(ma.x1, mb.x1), std::max(ma.y1, mb.y1), std::min(ma.x2, mb.x2), std::min(ma.y2, mb.y2)};
-      break;
-    case TOP_NOT:
-    case TOP_UNION:
-      dom = ea ? Box64{0, 0, 0, 0} : ma;
-      break;
-    default:   // OR, XOR
-      if (ea && eb) dom = Box64{0, 0, 0, 0};
-      else if (ea) dom = mb;
-      else if (eb) dom = ma;
-      else dom = Box64{std::min(ma.x1, mb.x1), std::min(ma.y1, mb.y1), std::max(ma.x2, mb.x2), std::max(ma.y2, mb.y2)};
-  }
-  if (dom.empty()) {
-    if (area_out) *area_out = 0;
-    if (c) set_empty(e, c);
-    return VL_OK;
-  }
-  const bool area = area_out != nullptr;
-  Outs o(e);
-  Grid g = make_grid(dom, na + nb, erode > 0 ? e->erode_target : e->target);
-  const double t_start = g_debug ? now_ms() : 0.0;
-  const Xform xr = erode > 0 ? Xform{erode, erode, erode, erode} : xa;
-  VLCK(bin_and_process(e, g, top, a, na, xa, xr, b, nb, area, erode, o));
-  if (g_debug) fprintf(stderr, "[vl] op %d na=%lld nb=%lld level0 %.2f ms\n", top, (long long)na, (long long)nb,
-                       now_ms() - t_start);
-  if (area) {
-    *area_out = o.area;
-    return VL_OK;
-  }
+// Seam levels and x-stitch of an operator's level-0 streams (grid g), then the
+// result into layer c.
+int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
   // seam levels
-  Scratch s(e);
   int level = 0;
   while (o.nside > 0) {
     VLCK(o.next.ensure(o.nside, 0));
@@ -764,6 +755,49 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
   return adopt(e, o, c);
 }
 
+int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb,
+           vl_layer* c, long long* area_out, int32_t erode = 0, const Box64* ka = nullptr,
+           const Box64* kb = nullptr) {
+  if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; kb = nullptr; }
+  Box64 ma, mb;
+  VLCK(op_mbrs(e, a, na, xa, b, nb, ka, kb, &ma, &mb));
+  Box64 dom;
+  const bool ea = na <= 0, eb = nb <= 0;
+  switch (top) {
+    case TOP_AND:
+      if (ea || eb) { dom = Box64{0, 0, 0, 0}; break; }
+      dom = Box64{std::max(ma.x1, mb.x1), std::max(ma.y1, mb.y1), std::min(ma.x2, mb.x2), std::min(ma.y2, mb.y2)};
+      break;
+    case TOP_NOT:
+    case TOP_UNION:
+      dom = ea ? Box64{0, 0, 0, 0} : ma;
+      break;
+    default:   // OR, XOR
+      if (ea && eb) dom = Box64{0, 0, 0, 0};
+      else if (ea) dom = mb;
+      else if (eb) dom = ma;
+      else dom = Box64{std::min(ma.x1, mb.x1), std::min(ma.y1, mb.y1), std::max(ma.x2, mb.x2), std::max(ma.y2, mb.y2)};
+  }
+  if (dom.empty()) {
+    if (area_out) *area_out = 0;
+    if (c) set_empty(e, c);
+    return VL_OK;
+  }
+  const bool area = area_out != nullptr;
+  Outs o(e);
+  Grid g = make_grid(dom, na + nb, erode > 0 ? e->erode_target : e->target);
+  const double t_start = g_debug ? now_ms() : 0.0;
+  const Xform xr = erode > 0 ? Xform{erode, erode, erode, erode} : xa;
+  VLCK(bin_and_process(e, g, top, a, na, xa, xr, b, nb, area, erode, o));
+  if (g_debug) fprintf(stderr, "[vl] op %d na=%lld nb=%lld level0 %.2f ms\n", top, (long long)na, (long long)nb,
+                       now_ms() - t_start);
+  if (area) {
+    *area_out = o.area;
+    return VL_OK;
+  }
+  return finish_op(e, g, o, c, t_start);
+}
+
 struct OpTimer {
   vl_engine* e;
   explicit OpTimer(vl_engine* en) : e(en) { (void)hipEventRecord(e->ev0, e->st); }
@@ -803,6 +837,207 @@ int layer_mbr(vl_engine* e, const vl_layer* l, Box64* m) {
     l->mbr_ok = true;
   }
   *m = Box64{l->mbr[0], l->mbr[1], l->mbr[2], l->mbr[3]};
+  return VL_OK;
+}
+
+// ---- fused operator groups ------------------------------------------------------
+// Level 0 of a fused group: the fused tile kernel's two tiers (20 and 41 boxes
+// per window), then x-sub-column splits of the tiles above the second tier's
+// capacity (windows grown by the halo), recursively.
+int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries,
+                        uint64_t nentries, const FusedSpec& fs, Outs* const* outs, int depth) {
+  if (ntiles <= 0) return VL_OK;
+  const int nout = fs.nout;
+  Scratch s(e);
+  const int tiers[2] = {FUSE_NMAX_SMALL, FUSE_NMAX_LARGE};
+  uint32_t* small = s.get<uint32_t>(4);   // overflow counts of the two tiers
+  unsigned long long* cnt = s.get<unsigned long long>(3 * (size_t)FUSE_MAX_OUT);
+  uint32_t* ovfl[2] = {s.get<uint32_t>(ntiles), s.get<uint32_t>(ntiles)};
+  uint64_t* tab_off = s.get<uint64_t>((size_t)nout * ntiles);
+  uint32_t* tab_n = s.get<uint32_t>((size_t)nout * 2 * ntiles);
+  if (s.oom) return fail(VL_ENOMEM, "scratch");
+  int G[2];
+  for (int t = 0; t < 2; t++) {
+    int mw = tile_fused_max_wg(tiers[t]);
+    if (e->max_wg_cap > 0) mw = std::min(mw, e->max_wg_cap);
+    G[t] = std::max(1, std::min(ntiles, mw));
+  }
+  const int nregions = G[0] + G[1];
+  // region size per output: a few strips per entry of a first-tier workgroup;
+  // regrown to the measured maximum fill if any region overflows
+  uint64_t region[FUSE_MAX_OUT];
+  for (int k = 0; k < nout; k++) region[k] = (uint64_t)(4.0 * (double)nentries / G[0]) + 1024;
+  std::unique_ptr<Scratch> rs;
+  vl_box* scr[FUSE_MAX_OUT] = {};
+  uint8_t* scrT[FUSE_MAX_OUT] = {};
+  unsigned long long hc[3 * FUSE_MAX_OUT];
+  uint32_t nov = 0;
+  const double t_pt = g_debug ? now_ms() : 0.0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    rs.reset(new Scratch(e));
+    for (int k = 0; k < nout; k++) {
+      const size_t need = (size_t)nregions * region[k];
+      scr[k] = rs->get<vl_box>(need);
+      scrT[k] = rs->get<uint8_t>(need);
+    }
+    if (rs->oom) return fail(VL_ENOMEM, "fused strip scratch");
+    launch_fill_u32(small, 0u, 4, e->st);
+    launch_fill_u64((uint64_t*)cnt, 0ull, 3 * (size_t)FUSE_MAX_OUT, e->st);
+    VLCK(clear_err(e));
+    FusedOut fo;
+    memset(&fo, 0, sizeof(fo));
+    for (int k = 0; k < nout; k++) {
+      fo.strips[k] = scr[k];
+      fo.tag[k] = scrT[k];
+      fo.tab_off[k] = tab_off + (size_t)k * ntiles;
+      fo.tab_n[k] = tab_n + (size_t)k * 2 * ntiles;
+      fo.cnt[k] = cnt + 3 * k;
+      fo.region_cap[k] = region[k];
+    }
+    fo.ntab = ntiles;
+    fo.err = e->d_err;
+    if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
+    int rbase = 0;
+    for (int t = 0; t < 2; t++) {
+      fo.region_base = rbase;
+      fo.list = t ? ovfl[0] : nullptr;        // tiles above the first tier's capacity
+      fo.list_n = t ? small : nullptr;
+      fo.ovf = ovfl[t];
+      fo.ovf_n = small + t;
+      launch_tile_fused(tiers[t], d_desc, ntiles, d_entries, fs, fo, G[t], e->st);
+      rbase += G[t];
+    }
+    HIPCK(hipGetLastError());
+    if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
+    constexpr size_t CB = sizeof(hc);
+    HIPCK(hipMemcpyAsync(e->h_stage, cnt, CB, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + CB, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + CB + 4, small + 1, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    uint32_t err = 0;
+    memcpy(hc, e->h_stage, CB);
+    memcpy(&err, (char*)e->h_stage + CB, 4);
+    memcpy(&nov, (char*)e->h_stage + CB + 4, 4);
+    if (e->prof) {
+      float ms = 0.f;
+      HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
+      auto& k = e->kst[KS_TILE];
+      k.launches += 2;
+      k.ms += ms;
+      double strips = 0.0;
+      for (int q = 0; q < nout; q++) strips += (double)hc[3 * q];
+      k.bytes += 16.0 * (double)nentries + 16.0 * strips + 48.0 * ntiles;
+    }
+    if (err & ERR_OVERFLOW_CAP) {
+      if (attempt == 1) return fail(VL_EINTERNAL, "fused output capacity still short after resize");
+      for (int k = 0; k < nout; k++) region[k] = std::max<uint64_t>(region[k], hc[3 * k + 2] + 64);
+      if (g_debug) fprintf(stderr, "[vl]  fused region overflow: retry\n");
+      continue;
+    }
+    break;
+  }
+  // every output into tile order (final and seam streams)
+  for (int k = 0; k < nout; k++) {
+    const uint64_t total = hc[3 * k], nsideT = hc[3 * k + 1];
+    if (total == 0) continue;
+    Outs& o = *outs[k];
+    const uint64_t nfinal = total - nsideT;
+    VLCK(o.out.ensure(o.nout + nfinal + 1, o.nout));
+    VLCK(o.side.ensure(o.nside + nsideT + 1, o.nside));
+    VLCK(o.side_tag.ensure(o.nside + nsideT + 1, o.nside));
+    Scratch ks(e);
+    uint64_t* dst = ks.get<uint64_t>(2 * (size_t)ntiles + 2);
+    void* stmp = ks.get<char>(scan_tmp_bytes(ntiles));
+    if (ks.oom) return fail(VL_ENOMEM, "scratch");
+    const uint32_t* tn = tab_n + (size_t)k * 2 * ntiles;
+    scan_u32_to_u64(tn, dst, ntiles, nullptr, stmp, e->st);
+    scan_u32_to_u64(tn + ntiles, dst + ntiles, ntiles, nullptr, stmp, e->st);
+    launch_tile_reorder(ntiles, tab_off + (size_t)k * ntiles, tn, ntiles, dst, dst + ntiles, scr[k], scrT[k],
+                        o.out.p, o.nout, o.side.p, o.side_tag.p, o.nside, e->st);
+    HIPCK(hipGetLastError());
+    o.nout += nfinal;
+    o.nside += nsideT;
+  }
+  rs.reset();
+  if (g_debug) {
+    fprintf(stderr, "[vl]  fused tiles=%d entries=%llu depth=%d overflow=%u %.2f ms; strips", ntiles,
+            (unsigned long long)nentries, depth, nov, now_ms() - t_pt);
+    for (int k = 0; k < nout; k++) fprintf(stderr, " %llu/%llu", hc[3 * k], hc[3 * k + 1]);
+    fprintf(stderr, "\n");
+  }
+  if (nov > 0) {
+    Scratch ss(e);
+    SplitTiles sp;
+    VLCK(split_tiles(e, d_desc, ovfl[1], nov, d_entries, fs.halo, FUSE_NMAX_SMALL / 2, FUSE_NMAX_LARGE, depth, ss,
+                     &sp));
+    VLCK(process_tiles_fused(e, sp.desc, sp.n, sp.entries, sp.total, fs, outs, depth + 1));
+  }
+  return VL_OK;
+}
+
+// Status of run_fused when the group cannot run fused here (nothing written;
+// the caller runs the group's operators one by one).
+constexpr int FUSE_DECLINED = 1000;
+
+// Outputs c[k] = fs.tt[k](S, morphological images of S) of a fused group.
+int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs, vl_layer* const* c) {
+  const int nout = fs.nout;
+  if (S->n == 0) {
+    for (int k = 0; k < nout; k++) set_empty(e, c[k]);
+    return VL_OK;
+  }
+  Box64 m;
+  VLCK(layer_mbr(e, S, &m));
+  // domain: MBR(S) grown by the largest GROW reach on each side (erosion and
+  // the Boolean outputs stay inside S or its grown images)
+  int64_t ge = 0, gn = 0, gw = 0, gs = 0, de = 0;
+  for (int k = 0; k < fs.nmorph; k++) {
+    if (fs.kind[k] == MORPH_GROW) {
+      ge = std::max<int64_t>(ge, fs.arg[k][0]);
+      gn = std::max<int64_t>(gn, fs.arg[k][1]);
+      gw = std::max<int64_t>(gw, fs.arg[k][2]);
+      gs = std::max<int64_t>(gs, fs.arg[k][3]);
+    } else {
+      de = std::max<int64_t>(de, fs.arg[k][0]);
+    }
+  }
+  const Box64 dom{m.x1 - gw, m.y1 - gs, m.x2 + ge, m.y2 + gn};
+  const int64_t h = fs.halo;
+  if (dom.x1 - h < INT32_MIN || dom.y1 - h < INT32_MIN || dom.x2 + h > INT32_MAX || dom.y2 + h > INT32_MAX)
+    return FUSE_DECLINED;
+  const Grid g = make_grid(dom, S->n, e->fuse_target);
+  // windows must stay small against the tiles (each box meets a few windows)
+  if (g.tw < 4 * h || g.th < 4 * h || g.tw < 4 * de || g.th < 4 * de) return FUSE_DECLINED;
+  const double t_start = g_debug ? now_ms() : 0.0;
+  std::vector<std::unique_ptr<Outs>> outs;
+  Outs* po[FUSE_MAX_OUT] = {};
+  for (int k = 0; k < nout; k++) {
+    outs.emplace_back(new Outs(e));
+    po[k] = outs.back().get();
+  }
+  {
+    const int64_t nt = (int64_t)g.cols * g.rows;
+    Scratch keep(e);
+    vl_box* entries = nullptr;
+    uint64_t* start = nullptr;
+    uint64_t total = 0;
+    const int32_t R = fs.halo;
+    VLCK(bin_entries(e, g, S->d, S->n, Xform{0, 0, 0, 0}, Xform{R, R, R, R}, nullptr, 0, keep, &entries, &start,
+                     &total));
+    TileDesc* desc = keep.get<TileDesc>(nt);
+    if (!desc) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+    launch_build_desc(g, start, TOP_UNION, desc, e->st);
+    HIPCK(hipGetLastError());
+    const int rc = process_tiles_fused(e, desc, (int)nt, entries, total, fs, po, 0);
+    if (rc == VL_EINTERNAL) {
+      if (g_debug) fprintf(stderr, "[vl] fused group declined: %s\n", g_err.c_str());
+      return FUSE_DECLINED;
+    }
+    VLCK(rc);
+  }
+  if (g_debug) fprintf(stderr, "[vl] fused level0 n=%lld outputs=%d %.2f ms\n", (long long)S->n, nout,
+                       now_ms() - t_start);
+  for (int k = 0; k < nout; k++) VLCK(finish_op(e, g, *outs[k], c[k], t_start));
   return VL_OK;
 }
 
@@ -937,6 +1172,11 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
     e->max_wg_cap = (int)std::max<int64_t>(0, v);
   } else if (k == "erode_kernel") {
     e->erode_kernel = v != 0;
+  } else if (k == "fuse") {
+    e->fuse = v != 0;
+  } else if (k == "fuse_target") {
+    if (v < 2 || v > 4096) return fail(VL_EINVAL, "fuse_target out of range");
+    e->fuse_target = (int)v;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
   }
@@ -952,6 +1192,9 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "level_factor") *v = e->level_factor;
   else if (k == "max_wg") *v = e->max_wg_cap;
   else if (k == "erode_kernel") *v = e->erode_kernel;
+  else if (k == "fuse") *v = e->fuse;
+  else if (k == "fuse_target") *v = e->fuse_target;
+  else if (k == "fused_groups") *v = e->fused_groups;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
 }
@@ -1411,20 +1654,205 @@ int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, 
   return VL_OK;
 }
 
-// Independent chains of a program.  Operator j depends on an earlier i when
-// it reads or writes a layer that i writes, or writes a layer that i reads;
-// the connected components of that relation run concurrently, each in program
-// order on its own worker engine (stream) and host thread.
-static std::vector<std::vector<int>> program_chains(const std::vector<ProgOp>& ops) {
-  const int n = (int)ops.size();
+// ---- program compilation: fused groups ----------------------------------------
+// A step of a compiled program: one operator, or a fused group (ops are the
+// group's operators in program order; run one by one if the group is declined).
+struct ProgStep {
+  std::vector<int> ops;
+  bool fused = false;
+  const vl_layer* src = nullptr;
+  FusedSpec fs{};
+  vl_layer* outs[FUSE_MAX_OUT] = {};
+};
+
+// Greedy grouping of consecutive operators over one source layer S: the group
+// starts at a GROW, SIZE or MERGE of S and takes every following operator
+// whose inputs are S or earlier group outputs (GROW with non-negative
+// arguments and SIZE only on S itself, at most two distinct morphological
+// images, at most FUSE_MAX_OUT outputs, each written once and never S) and
+// every ASSERT_ZERO of such a layer.  Each output is a truth table over
+// (S, image 0, image 1); S is 0xAA, image 0 is 0xCC, image 1 is 0xF0.
+struct GroupBuilder {
+  const vl_layer* src = nullptr;
+  FusedSpec fs{};
+  std::vector<std::pair<vl_layer*, uint32_t>> nodes;
+  std::vector<int> ops;
+  int nprod = 0;
+
+  bool tt_of(const vl_layer* l, uint32_t* tt) const {
+    if (l == src) {
+      *tt = 0xAAu;
+      return true;
+    }
+    for (const auto& nd : nodes)
+      if (nd.first == l) {
+        *tt = nd.second;
+        return true;
+      }
+    return false;
+  }
+  // Image index of a morphological operator of S (added if new), -1 if full.
+  int morph(int32_t kind, const int32_t a[4]) {
+    for (int k = 0; k < fs.nmorph; k++)
+      if (fs.kind[k] == kind && memcmp(fs.arg[k], a, 4 * sizeof(int32_t)) == 0) return k;
+    if (fs.nmorph == 2) return -1;
+    const int k = fs.nmorph++;
+    fs.kind[k] = kind;
+    memcpy(fs.arg[k], a, 4 * sizeof(int32_t));
+    const int32_t reach = kind == MORPH_GROW ? std::max(std::max(a[0], a[1]), std::max(a[2], a[3])) : a[0];
+    fs.halo = std::max(fs.halo, reach);
+    return k;
+  }
+  bool add(const ProgOp& o) {
+    if (!src) return false;
+    uint32_t ta = 0, tb = 0, tt = 0;
+    if (o.op == -1) return tt_of(o.a, &ta);
+    if (!o.c || o.c == src || (int)nodes.size() == FUSE_MAX_OUT) return false;
+    for (const auto& nd : nodes)
+      if (nd.first == o.c) return false;
+    const FusedSpec saved = fs;
+    switch (o.op) {
+      case VL_OP_GROW: {
+        if (o.a != src || o.args[0] < 0 || o.args[1] < 0 || o.args[2] < 0 || o.args[3] < 0) return false;
+        const int k = morph(MORPH_GROW, o.args);
+        if (k < 0) return false;
+        tt = k ? 0xF0u : 0xCCu;
+        break;
+      }
+      case VL_OP_SIZE: {
+        const int32_t d = o.args[0];
+        if (o.a != src || d == INT32_MIN) return false;
+        if (d == 0) {
+          tt = 0xAAu;
+          break;
+        }
+        const int32_t a[4] = {d > 0 ? d : -d, d > 0 ? d : 0, d > 0 ? d : 0, d > 0 ? d : 0};
+        const int k = morph(d > 0 ? MORPH_GROW : MORPH_ERODE, a);
+        if (k < 0) return false;
+        tt = k ? 0xF0u : 0xCCu;
+        break;
+      }
+      case VL_OP_MERGE:
+        if (!tt_of(o.a, &ta)) return false;
+        tt = ta;
+        break;
+      case VL_OP_AND:
+      case VL_OP_NOT:
+      case VL_OP_OR:
+      case VL_OP_XOR:
+        if (!tt_of(o.a, &ta) || !tt_of(o.b, &tb)) return false;
+        tt = o.op == VL_OP_AND ? (ta & tb) : o.op == VL_OP_NOT ? (ta & ~tb) : o.op == VL_OP_OR ? (ta | tb) : (ta ^ tb);
+        tt &= 0xFFu;
+        break;
+      default:
+        fs = saved;
+        return false;
+    }
+    nodes.push_back({o.c, tt});
+    nprod++;
+    return true;
+  }
+  bool start(const ProgOp& o) {
+    if (o.op != VL_OP_GROW && o.op != VL_OP_SIZE && o.op != VL_OP_MERGE) return false;
+    src = o.a;
+    if (add(o)) return true;
+    *this = GroupBuilder();
+    return false;
+  }
+};
+
+static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, bool fuse) {
+  std::vector<ProgStep> steps;
+  GroupBuilder g;
+  auto single = [&](int i) {
+    ProgStep st;
+    st.ops.push_back(i);
+    steps.push_back(st);
+  };
+  auto flush = [&]() {
+    if (g.ops.empty()) return;
+    if (g.nprod >= 2) {
+      ProgStep st;
+      st.ops = g.ops;
+      st.fused = true;
+      st.src = g.src;
+      st.fs = g.fs;
+      st.fs.nout = (int32_t)g.nodes.size();
+      for (size_t k = 0; k < g.nodes.size(); k++) {
+        st.outs[k] = g.nodes[k].first;
+        st.fs.tt[k] = g.nodes[k].second;
+      }
+      steps.push_back(st);
+    } else {
+      for (int i : g.ops) single(i);
+    }
+    g = GroupBuilder();
+  };
+  for (int i = 0; i < (int)ops.size(); i++) {
+    if (fuse && g.add(ops[i])) {
+      g.ops.push_back(i);
+      continue;
+    }
+    flush();
+    if (fuse && g.start(ops[i])) {
+      g.ops.push_back(i);
+      continue;
+    }
+    single(i);
+  }
+  flush();
+  return steps;
+}
+
+// Runs one step on engine e.
+static int run_step(vl_engine* e, const std::vector<ProgOp>& ops, const ProgStep& st, int64_t* failures) {
+  if (st.fused) {
+    int rc;
+    {
+      std::lock_guard<std::recursive_mutex> lk(e->mu);
+      HIPCK(hipSetDevice(e->device));
+      rc = run_fused(e, st.src, st.fs, st.outs);
+    }
+    if (rc == VL_OK) {
+      e->fused_groups++;
+      for (int i : st.ops)
+        if (ops[i].op == -1 && ops[i].a->n > 0) (*failures)++;
+      return VL_OK;
+    }
+    if (rc != FUSE_DECLINED) return rc;
+    g_err.clear();
+  }
+  for (int i : st.ops) VLCK(run_prog_op(e, ops[i], failures));
+  return VL_OK;
+}
+
+// Independent chains of a compiled program.  Step j depends on an earlier i
+// when it reads or writes a layer that i writes, or writes a layer that i
+// reads; the connected components of that relation run concurrently, each in
+// program order on its own worker engine (stream) and host thread.
+static std::vector<std::vector<int>> program_chains(const std::vector<ProgOp>& ops,
+                                                    const std::vector<ProgStep>& steps) {
+  const int n = (int)steps.size();
   std::vector<int> parent(n);
   for (int i = 0; i < n; i++) parent[i] = i;
   std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
-  auto reads = [](const ProgOp& o, const vl_layer* l) { return o.a == l || o.b == l; };
+  std::vector<std::vector<const vl_layer*>> rd(n), wr(n);
+  for (int i = 0; i < n; i++)
+    for (int k : steps[i].ops) {
+      const ProgOp& o = ops[k];
+      if (o.a) rd[i].push_back(o.a);
+      if (o.b) rd[i].push_back(o.b);
+      if (o.c) wr[i].push_back(o.c);
+    }
+  auto meets = [](const std::vector<const vl_layer*>& x, const std::vector<const vl_layer*>& y) {
+    for (const vl_layer* p : x)
+      for (const vl_layer* q : y)
+        if (p == q) return true;
+    return false;
+  };
   for (int j = 0; j < n; j++)
     for (int i = 0; i < j; i++) {
-      const ProgOp &a = ops[i], &b = ops[j];
-      const bool dep = (a.c && (reads(b, a.c) || b.c == a.c)) || (b.c && reads(a, b.c));
+      const bool dep = meets(wr[i], rd[j]) || meets(wr[i], wr[j]) || meets(rd[i], wr[j]);
       if (dep) parent[find(j)] = find(i);
     }
   std::map<int, std::vector<int>> groups;
@@ -1440,7 +1868,8 @@ int vl_program_run(vl_program* p) {
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
   p->failures = 0;
-  const std::vector<std::vector<int>> chains = program_chains(p->ops);
+  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse != 0);
+  const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
   const int nw = (int)std::min<size_t>(chains.size(), (size_t)p->max_workers);
   // the bounding boxes of the program's inputs (layers no operator writes)
   // are cached before concurrent chains read them
@@ -1462,8 +1891,8 @@ int vl_program_run(vl_program* p) {
   HIPCK(hipEventRecord(t0, e->st));
   int rc = VL_OK;
   if (nw <= 1) {
-    for (const ProgOp& o : p->ops) {
-      rc = run_prog_op(e, o, &p->failures);
+    for (const ProgStep& st : steps) {
+      rc = run_step(e, p->ops, st, &p->failures);
       if (rc != VL_OK) break;
     }
   } else {
@@ -1485,14 +1914,17 @@ int vl_program_run(vl_program* p) {
       w->level_factor = e->level_factor;
       w->erode_kernel = e->erode_kernel;
       w->max_wg_cap = e->max_wg_cap;
+      w->fuse = e->fuse;
+      w->fuse_target = e->fuse_target;
       w->prof = e->prof;
+      w->fused_groups = 0;
       for (auto& st : w->kst) st = vl_engine::KStat();
       HIPCK(hipStreamWaitEvent(w->st, t0, 0));
       th.emplace_back([&, k, w]() {
         (void)hipSetDevice(w->device);
         for (size_t c = k; c < chains.size(); c += nw) {
           for (int i : chains[c]) {
-            rcs[k] = run_prog_op(w, p->ops[i], &fails[k]);
+            rcs[k] = run_step(w, p->ops, steps[i], &fails[k]);
             if (rcs[k] != VL_OK) {
               errs[k] = g_err;
               return;
@@ -1509,6 +1941,7 @@ int vl_program_run(vl_program* p) {
     for (int k = 0; k < nw; k++) {
       p->failures += fails[k];
       vl_engine* w = p->workers[k];
+      e->fused_groups += w->fused_groups;
       for (int s = 0; s < 3; s++) {
         e->kst[s].launches += w->kst[s].launches;
         e->kst[s].ms += w->kst[s].ms;

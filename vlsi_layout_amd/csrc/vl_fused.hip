@@ -1,0 +1,469 @@
+// vl_fused.hip — fused per-tile evaluation of several operators over one
+// source layer.
+//
+// A recorded program often derives a whole family of layers from one source
+// S: the OP1 ring (src/op_template.m4:14-22) computes GROW(S), SIZE(S,-d),
+// GROW(S) NOT S, S NOT SIZE(S,-d) and two ANDs of those.  Evaluated one
+// operator at a time, every intermediate layer is written, re-read, re-binned
+// and re-sorted.  Here S is binned once, with tile windows grown by the
+// largest morphological reach, and each tile builds, on one compressed grid,
+//   * the coverage of S,
+//   * up to two morphological images of S: GROW by (e, n, w, s) (the grown
+//     boxes rasterised directly) or erosion by the square of half-side d (the
+//     erosion passes of the SIZE kernel),
+// and then every requested output as a Boolean function of those three masks
+// (a truth table), clipped to the tile, through the same start/end-mask strip
+// extraction as the single-operator kernel.  Each output has its own scratch
+// stream, tables and seam side stream, so it is finished (seam levels,
+// x-stitch) exactly like a single operator's result: the output is the
+// canonical maximal-strip decomposition of the same point set.
+//
+// The compressed grid holds, per box, x1 and x2 clipped to the window and the
+// two edges of each morphological image (grown edges clipped to the window,
+// eroded edges x1+d, x2-d clamped to the tile): 6 coordinates per box and
+// axis, so a 128-slot grid takes 20 boxes and a 256-slot grid 41.
+#include "vl_tile_dev.h"
+
+namespace vl {
+
+namespace {
+
+constexpr int FUSE_OVF_BUF = 64;   // overflow tiles buffered in LDS per workgroup
+
+// Value of truth table tt (bit s | m0 << 1 | m1 << 2) on base words s, m0, m1.
+__device__ __forceinline__ uint64_t tt_eval(uint32_t tt, uint64_t s, uint64_t m0, uint64_t m1) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int k = 1; k < 8; k++)
+    if ((tt >> k) & 1u) r |= ((k & 1) ? s : ~s) & ((k & 2) ? m0 : ~m0) & ((k & 4) ? m1 : ~m1);
+  return r;
+}
+
+// Bits of rows [r0, r1) in word w.
+__device__ __forceinline__ uint64_t row_bits(int w, int r0, int r1) {
+  const int lo = max(r0 - 64 * w, 0), hi = min(r1 - 64 * w, 64);
+  return lo < hi ? bits_range(lo, hi - 1) : 0ull;
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc, int ntiles,
+                                               const vl_box* __restrict__ entries, FusedSpec fs, FusedOut o) {
+  constexpr int CPB = 6;               // coordinates per box and axis
+  constexpr int NMAX = P / CPB - 1;    // the last slot group holds the window and tile borders
+  constexpr int K = P / WAVE;          // slots per lane
+  constexpr int NCOL = P - 1;
+  constexpr int NW = P / WAVE;         // words per column
+  constexpr int NL = (NMAX + WAVE) / WAVE;
+  static_assert(NMAX == (P == 128 ? FUSE_NMAX_SMALL : FUSE_NMAX_LARGE), "tier capacities");
+  __shared__ int32_t ux[P], uy[P];
+  __shared__ uint64_t MS[NCOL * NW];       // S
+  __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
+  __shared__ uint64_t MT[NCOL * NW];       // erosion temporary, then the current output
+  __shared__ vl_box sbuf[TILE_BATCH * (NMAX + 1)];
+  __shared__ TileDesc sdesc[TILE_BATCH];
+  uint32_t* const sdw = (uint32_t*)sdesc;
+  __shared__ uint32_t sovf[FUSE_OVF_BUF];
+  // per-output parameters, copied once from the kernel arguments
+  __shared__ vl_box* s_strips[FUSE_MAX_OUT];
+  __shared__ uint8_t* s_tag[FUSE_MAX_OUT];
+  __shared__ uint64_t* s_toff[FUSE_MAX_OUT];
+  __shared__ uint32_t* s_tn[FUSE_MAX_OUT];
+  __shared__ uint64_t s_cap[FUSE_MAX_OUT], s_used[FUSE_MAX_OUT], s_side[FUSE_MAX_OUT];
+  __shared__ uint32_t s_tt[FUSE_MAX_OUT];
+  static_assert(TILE_BATCH * 10 <= WAVE, "one descriptor dword per lane");
+
+  const int lane = threadIdx.x;
+  const int nt = o.list_n ? (int)*o.list_n : ntiles;
+  const int nout = fs.nout;
+  const int nmorph = fs.nmorph;
+  const int32_t R = fs.halo;
+  const bool g0 = nmorph > 0 && fs.kind[0] == MORPH_GROW, g1 = nmorph > 1 && fs.kind[1] == MORPH_GROW;
+  const bool e0 = nmorph > 0 && fs.kind[0] == MORPH_ERODE, e1 = nmorph > 1 && fs.kind[1] == MORPH_ERODE;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < FUSE_MAX_OUT; k++) {
+      s_strips[k] = o.strips[k];
+      s_tag[k] = o.tag[k];
+      s_toff[k] = o.tab_off[k];
+      s_tn[k] = o.tab_n[k];
+      s_cap[k] = o.region_cap[k];
+      s_used[k] = 0;
+      s_side[k] = 0;
+      s_tt[k] = fs.tt[k];
+    }
+  }
+  uint32_t novf = 0;   // entries in sovf (uniform)
+  for (int b0 = blockIdx.x * TILE_BATCH; b0 < nt; b0 += gridDim.x * TILE_BATCH) {
+    const int nbt = min(TILE_BATCH, nt - b0);
+    uint32_t my_ti = 0;
+    if (lane < nbt) my_ti = o.list ? o.list[b0 + lane] : (uint32_t)(b0 + lane);
+    {
+      const int dj = lane / 10, dw = lane - 10 * (lane / 10);
+      const uint32_t tj = (uint32_t)__shfl((int)my_ti, dj < TILE_BATCH ? dj : 0, WAVE);
+      if (lane < nbt * 10) sdw[lane] = ((const uint32_t*)desc)[(size_t)tj * 10 + dw];
+    }
+    __syncthreads();
+    {
+      int4 v[TILE_BATCH * NL];
+      int nj[TILE_BATCH];
+#pragma unroll
+      for (int j = 0; j < TILE_BATCH; j++) {
+        const int64_t off = sdesc[j].off;
+        const int c = (j < nbt) ? sdesc[j].nA + sdesc[j].nB : 0;
+        nj[j] = c <= NMAX ? c : 0;
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+          const int i = lane + q * WAVE;
+          v[j * NL + q] = make_int4(0, 0, 0, 0);
+          if (i < nj[j]) v[j * NL + q] = ((const int4*)entries)[off + i];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TILE_BATCH; j++) {
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+          const int i = lane + q * WAVE;
+          if (i < nj[j]) ((int4*)sbuf)[j * (NMAX + 1) + i] = v[j * NL + q];
+        }
+      }
+    }
+    __syncthreads();
+    for (int jb = 0; jb < nbt; jb++) {
+      const uint32_t ti = read_lane(my_ti, jb);
+      const TileDesc d = sdesc[jb];
+      const vl_box* bx = sbuf + jb * (NMAX + 1);
+      const int n = d.nA + d.nB;
+      if (n > NMAX || n == 0) {
+        if (lane == 0) {
+          if (n > NMAX) sovf[novf] = ti;
+          for (int k = 0; k < nout; k++) {
+            s_tn[k][ti] = 0;
+            s_tn[k][o.ntab + ti] = 0;
+          }
+        }
+        if (n > NMAX && ++novf == FUSE_OVF_BUF) {
+          __syncthreads();
+          flush_overflow(sovf, novf, o.ovf, o.ovf_n, lane, WAVE);
+          novf = 0;
+          __syncthreads();
+        }
+        continue;
+      }
+      // 1. coordinates: slot e = lane*K + k holds component e % 6 of box e / 6:
+      //    x1, x2 clipped to the window, then two per morphological image.
+      //    Slot group n holds the window and tile borders.
+      const int32_t wx0 = clamp64((int64_t)d.x0 - R, INT32_MIN, INT32_MAX);
+      const int32_t wx1 = clamp64((int64_t)d.x1 + R, INT32_MIN, INT32_MAX);
+      const int32_t wy0 = clamp64((int64_t)d.y0 - R, INT32_MIN, INT32_MAX);
+      const int32_t wy1 = clamp64((int64_t)d.y1 + R, INT32_MIN, INT32_MAX);
+      int32_t vx[K], vy[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int e = lane * K + k;
+        const int i = e / CPB, comp = e - CPB * (e / CPB);
+        int32_t cx = INT32_MAX, cy = INT32_MAX;
+        if (i < n) {
+          const vl_box b = bx[i];
+          const int32_t x1 = max(b.x1, wx0), x2 = min(b.x2, wx1);
+          const int32_t y1 = max(b.y1, wy0), y2 = min(b.y2, wy1);
+          const bool hi = comp & 1;
+          const int m = (comp - 2) >> 1;   // morph of components 2..5
+          cx = hi ? x2 : x1;
+          cy = hi ? y2 : y1;
+          if (comp >= 2 && m < nmorph) {
+            const int32_t kind = m ? fs.kind[1] : fs.kind[0];
+            const int32_t ae = m ? fs.arg[1][0] : fs.arg[0][0], an = m ? fs.arg[1][1] : fs.arg[0][1];
+            const int32_t aw = m ? fs.arg[1][2] : fs.arg[0][2], as = m ? fs.arg[1][3] : fs.arg[0][3];
+            if (kind == MORPH_GROW) {
+              cx = hi ? clamp64((int64_t)b.x2 + ae, wx0, wx1) : clamp64((int64_t)b.x1 - aw, wx0, wx1);
+              cy = hi ? clamp64((int64_t)b.y2 + an, wy0, wy1) : clamp64((int64_t)b.y1 - as, wy0, wy1);
+            } else {   // erosion by ae: edges shifted inwards, clamped to the tile
+              cx = hi ? clamp64((int64_t)x2 - ae, d.x0, d.x1) : clamp64((int64_t)x1 + ae, d.x0, d.x1);
+              cy = hi ? clamp64((int64_t)y2 - ae, d.y0, d.y1) : clamp64((int64_t)y1 + ae, d.y0, d.y1);
+            }
+          }
+        } else if (i == n) {
+          if (comp == 0 || comp == 4) { cx = wx0; cy = wy0; }
+          else if (comp == 1 || comp == 5) { cx = wx1; cy = wy1; }
+          else if (comp == 2) { cx = d.x0; cy = d.y0; }
+          else { cx = d.x1; cy = d.y1; }
+        }
+        vx[k] = cx;
+        vy[k] = cy;
+      }
+      const int cnt = CPB * n + CPB;
+      // bitonic sort of P elements, element e = lane*K + k
+#pragma unroll
+      for (int size = 2; size <= P; size <<= 1) {
+#pragma unroll
+        for (int j = size >> 1; j > 0; j >>= 1) {
+          if (j >= K) {
+            const int lj = j / K;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+              const int e = lane * K + k;
+              const int px = xor_lanes(vx[k], lj);
+              const int py = xor_lanes(vy[k], lj);
+              const bool take_min = ((e & j) == 0) == ((e & size) == 0);
+              vx[k] = take_min ? min(vx[k], px) : max(vx[k], px);
+              vy[k] = take_min ? min(vy[k], py) : max(vy[k], py);
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+              if ((k & j) == 0) {
+                const int k2 = k | j;
+                const bool asc = ((lane * K + k) & size) == 0;
+                int32_t a = vx[k], b = vx[k2];
+                if ((a > b) == asc) { vx[k] = b; vx[k2] = a; }
+                a = vy[k]; b = vy[k2];
+                if ((a > b) == asc) { vy[k] = b; vy[k2] = a; }
+              }
+            }
+          }
+        }
+      }
+      // drop duplicates
+      int nxu, nyu;
+      {
+        const int32_t px = __shfl_up(vx[K - 1], 1, WAVE), py = __shfl_up(vy[K - 1], 1, WAVE);
+        bool fx[K], fy[K];
+        uint32_t cx = 0, cy = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int e = lane * K + k;
+          const int32_t qx = k ? vx[k - 1] : px, qy = k ? vy[k - 1] : py;
+          fx[k] = e < cnt && (e == 0 || vx[k] != qx);
+          fy[k] = e < cnt && (e == 0 || vy[k] != qy);
+          cx += fx[k];
+          cy += fy[k];
+        }
+        uint32_t tx, ty;
+        uint32_t ox = (uint32_t)wave_excl_u32(cx, &tx), oy = (uint32_t)wave_excl_u32(cy, &ty);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          if (fx[k]) ux[ox++] = vx[k];
+          if (fy[k]) uy[oy++] = vy[k];
+        }
+        nxu = (int)tx;
+        nyu = (int)ty;
+      }
+      const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 63) >> 6;
+      const int words = ncol * W;
+      for (int k = lane; k < words; k += WAVE) {
+        MS[k] = 0ull;
+        if (g0) MM[0][k] = 0ull;
+        if (g1) MM[1][k] = 0ull;
+      }
+      __syncthreads();
+      // 2. rasterise S and the grown images (box-parallel)
+      for (int i = lane; i < n; i += WAVE) {
+        const vl_box b = bx[i];
+#pragma unroll
+        for (int m = -1; m < 2; m++) {
+          if (m == 0 && !g0) continue;
+          if (m == 1 && !g1) continue;
+          int32_t bx1 = b.x1, by1 = b.y1, bx2 = b.x2, by2 = b.y2;
+          if (m >= 0) {
+            bx1 = clamp64((int64_t)b.x1 - fs.arg[m][2], wx0, wx1);
+            by1 = clamp64((int64_t)b.y1 - fs.arg[m][3], wy0, wy1);
+            bx2 = clamp64((int64_t)b.x2 + fs.arg[m][0], wx0, wx1);
+            by2 = clamp64((int64_t)b.y2 + fs.arg[m][1], wy0, wy1);
+          }
+          const int cx1 = lower_bound_i32(ux, nxu, max(bx1, wx0)), cx2 = lower_bound_i32(ux, nxu, min(bx2, wx1));
+          const int cy1 = lower_bound_i32(uy, nyu, max(by1, wy0)), cy2 = lower_bound_i32(uy, nyu, min(by2, wy1));
+          if (cx1 >= cx2 || cy1 >= cy2) continue;
+          uint64_t* M = m < 0 ? MS : MM[m];
+          const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;
+          for (int c = cx1; c < cx2; c++)
+            for (int w = w1; w <= w2; w++) {
+              const int lo = (w == w1) ? (cy1 & 63) : 0;
+              const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
+              atomicOr((unsigned long long*)&M[c * W + w], (unsigned long long)bits_range(lo, hi));
+            }
+        }
+      }
+      __syncthreads();
+      // the tile's own columns [colT0, colT1) and rows [rowT0, rowT1)
+      const int colT0 = count_below(ux, nxu, d.x0), colT1 = count_below(ux, nxu, d.x1);
+      const int rowT0 = count_below(uy, nyu, d.y0), rowT1 = count_below(uy, nyu, d.y1);
+      // 3. erosion images
+      if (e0) {
+        erode_x<NW>(MS, MT, ux, nxu, ncol, W, colT0, colT1, fs.arg[0][0], lane, WAVE);
+        __syncthreads();
+        erode_y<NW>(MT, MM[0], uy, nyu, ncol, W, d.y0, d.y1, fs.arg[0][0], lane, WAVE);
+        __syncthreads();
+      }
+      if (e1) {
+        erode_x<NW>(MS, MT, ux, nxu, ncol, W, colT0, colT1, fs.arg[1][0], lane, WAVE);
+        __syncthreads();
+        erode_y<NW>(MT, MM[1], uy, nyu, ncol, W, d.y0, d.y1, fs.arg[1][0], lane, WAVE);
+        __syncthreads();
+      }
+      const uint32_t flags = d.flags;
+      const int KW = (ncol + 1 + 63) >> 6;   // boundary words per row
+      // 4. outputs, one after the other
+#pragma unroll 1
+      for (int q = 0; q < nout; q++) {
+        const uint32_t tt = s_tt[q];
+        uint64_t any = 0;
+        for (int c = lane; c < ncol; c += WAVE) {
+          const bool in = c >= colT0 && c < colT1;
+#pragma unroll
+          for (int w = 0; w < NW; w++) {
+            if (w >= W) break;
+            const int ix = c * W + w;
+            const uint64_t v = in ? tt_eval(tt, MS[ix], (nmorph > 0) ? MM[0][ix] : 0ull,
+                                            (nmorph > 1) ? MM[1][ix] : 0ull) & row_bits(w, rowT0, rowT1)
+                                  : 0ull;
+            MT[ix] = v;
+            any |= v;
+          }
+        }
+        uint32_t* const tn = s_tn[q];
+        if (__ballot(any != 0ull) == 0ull) {
+          if (lane == 0) {
+            tn[ti] = 0;
+            tn[o.ntab + ti] = 0;
+          }
+          continue;
+        }
+        __syncthreads();
+        // start / end masks at every slab boundary (lane owns boundaries 64*kb + lane)
+        uint64_t Sr[NW][NW], Er[NW][NW];
+        uint32_t myT = 0;
+#pragma unroll
+        for (int kb = 0; kb < NW; kb++) {
+          const int k = kb * WAVE + lane;
+#pragma unroll
+          for (int w = 0; w < NW; w++) { Sr[kb][w] = 0ull; Er[kb][w] = 0ull; }
+          if (k <= ncol) {
+            const uint64_t* cur = (k < ncol) ? MT + k * W : nullptr;
+            const uint64_t* prv = (k > 0) ? MT + (k - 1) * W : nullptr;
+            uint64_t D[NW];
+            change_mask<NW>(cur, prv, W, D);
+            starts_meeting<NW>(cur, W, D, Sr[kb]);
+            starts_meeting<NW>(prv, W, D, Er[kb]);
+#pragma unroll
+            for (int w = 0; w < NW; w++) myT += (uint32_t)__popcll(Er[kb][w]);
+          }
+        }
+        const uint32_t T = wave_sum(myT);
+        const uint64_t cap = s_cap[q];
+        const uint64_t used = s_used[q];
+        const uint64_t base = (uint64_t)(o.region_base + blockIdx.x) * cap + used;
+        const bool ok = used + T <= cap;
+        vl_box* const strips = s_strips[q];
+        uint8_t* const tags = s_tag[q];
+        if (lane == 0) s_toff[q][ti] = base;
+        // strips per row: transposed S/E masks, a strip starting at boundary j
+        // ends at the first end bit after j
+        uint32_t r = 0, my_side = 0;
+#pragma unroll
+        for (int wb = 0; wb < NW; wb++) {
+          if (wb >= W) break;
+          uint64_t rs[NW], re[NW];
+          uint32_t c = 0;
+#pragma unroll
+          for (int kb = 0; kb < NW; kb++) {
+            rs[kb] = 0ull;
+            re[kb] = 0ull;
+            if (kb < KW) {
+              rs[kb] = transpose64(Sr[kb][wb], lane);
+              re[kb] = transpose64(Er[kb][wb], lane);
+              c += (uint32_t)__popcll(rs[kb]);
+            }
+          }
+          uint32_t tot;
+          const uint32_t ex = (uint32_t)wave_excl_u32(c, &tot);
+          uint64_t pos = base + r + ex;
+          const int a = wb * WAVE + lane;
+#pragma unroll
+          for (int kb = 0; kb < NW; kb++) {
+            uint64_t m = rs[kb];
+            while (m) {
+              const int j = kb * WAVE + __ffsll((unsigned long long)m) - 1;
+              m &= m - 1;
+              int k = ncol;
+#pragma unroll
+              for (int qq = NW - 1; qq >= 0; qq--) {
+                const int t = j - qq * WAVE;
+                uint64_t mm = re[qq];
+                if (t >= 63) mm = 0ull;
+                else if (t >= 0) mm &= ~0ull << (t + 1);
+                if (mm) k = qq * WAVE + __ffsll((unsigned long long)mm) - 1;
+              }
+              const int b = run_end(MT + j * W, W, a);
+              const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                                  ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                                  ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                                  ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+              if (ok) {
+                vl_box st;
+                st.x1 = ux[j]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[b];
+                strips[pos] = st;
+                tags[pos] = (uint8_t)tg;
+              }
+              pos++;
+              my_side += tg != 0u;
+            }
+          }
+          r += tot;
+        }
+        const uint32_t nside = (uint32_t)wave_sum((long long)my_side);
+        __syncthreads();
+        if (lane == 0) {
+          tn[ti] = T - nside;
+          tn[o.ntab + ti] = nside;
+          s_used[q] = used + T;
+          s_side[q] += nside;
+        }
+        __syncthreads();
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (novf) {
+    __syncthreads();
+    flush_overflow(sovf, novf, o.ovf, o.ovf_n, lane, WAVE);
+  }
+  __syncthreads();
+  if (lane < nout) {
+    const uint64_t used = s_used[lane];
+    if (used) {
+      unsigned long long* cnt = nullptr;
+#pragma unroll
+      for (int k = 0; k < FUSE_MAX_OUT; k++)
+        if (k == lane) cnt = o.cnt[k];
+      atomicAdd(&cnt[0], (unsigned long long)used);
+      if (s_side[lane]) atomicAdd(&cnt[1], (unsigned long long)s_side[lane]);
+      atomicMax(&cnt[2], (unsigned long long)used);
+      if (used > s_cap[lane]) atomicOr(o.err, ERR_OVERFLOW_CAP);
+    }
+  }
+}
+
+template <int P>
+void launch_f(const TileDesc* desc, int ntiles, const vl_box* entries, const FusedSpec& fs, const FusedOut& o,
+              int wg, hipStream_t st) {
+  const int g = wg < 1 ? 1 : wg;
+  hipLaunchKernelGGL((k_tile_f<P>), dim3(g), dim3(WAVE), 0, st, desc, ntiles, entries, fs, o);
+}
+
+}  // namespace
+
+void launch_tile_fused(int nmax, const TileDesc* desc, int ntiles, const vl_box* entries, const FusedSpec& fs,
+                       const FusedOut& o, int wg, hipStream_t st) {
+  if (nmax == FUSE_NMAX_SMALL) launch_f<128>(desc, ntiles, entries, fs, o, wg, st);
+  else launch_f<256>(desc, ntiles, entries, fs, o, wg, st);
+}
+
+int tile_fused_max_wg(int nmax) {
+  static int cache[2] = {0, 0};
+  int& c = cache[nmax == FUSE_NMAX_SMALL ? 0 : 1];
+  if (!c) c = nmax == FUSE_NMAX_SMALL ? occupancy_wg(k_tile_f<128>, WAVE) : occupancy_wg(k_tile_f<256>, WAVE);
+  return c;
+}
+
+}  // namespace vl

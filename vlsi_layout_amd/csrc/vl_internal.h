@@ -164,6 +164,47 @@ void launch_tile_block(bool two_layers, bool area_mode, bool erode, const TileDe
                        const vl_box* entries, TileOut o, int wg, hipStream_t st);
 int tile_block_max_wg(bool two_layers, bool area_mode, bool erode);
 constexpr int TILE_BLOCK_NMAX = 255;
+// Fused tile kernel (vl_fused.hip): several operators over ONE source layer S
+// evaluated per tile from shared masks.  The base masks are S itself and up to
+// two morphological images of S (GROW by (e, n, w, s) or erosion by the
+// square of half-side d); every output is a Boolean function of the three,
+// given as a truth table: bit (s | m0 << 1 | m1 << 2) is the output value for
+// those base values.  Tile windows are grown by `halo` (at least every
+// morph's reach), so each output is exact inside the tile.
+constexpr int FUSE_MAX_OUT = 8;
+enum : int32_t { MORPH_GROW = 1, MORPH_ERODE = 2 };
+struct FusedSpec {
+  int32_t nmorph;                 // 0..2
+  int32_t kind[2];                // MORPH_GROW / MORPH_ERODE
+  int32_t arg[2][4];              // GROW: e, n, w, s (all >= 0); ERODE: d > 0 in arg[k][0]
+  int32_t halo;
+  int32_t nout;                   // 1..FUSE_MAX_OUT
+  uint32_t tt[FUSE_MAX_OUT];      // truth table per output (bit 0 must be 0)
+};
+// Per-output scratch streams and tables (same meaning as TileOut's, one set
+// per output; each workgroup owns region region_base + g of every output).
+struct FusedOut {
+  vl_box* strips[FUSE_MAX_OUT];
+  uint8_t* tag[FUSE_MAX_OUT];
+  uint64_t* tab_off[FUSE_MAX_OUT];
+  uint32_t* tab_n[FUSE_MAX_OUT];           // [t] final, [ntab + t] seam count
+  unsigned long long* cnt[FUSE_MAX_OUT];   // [0] strips, [1] seam strips, [2] largest region fill
+  uint64_t region_cap[FUSE_MAX_OUT];
+  int region_base;
+  int ntab;
+  const uint32_t* list;                    // optional tile list (as TileOut)
+  const uint32_t* list_n;
+  uint32_t* ovf;
+  uint32_t* ovf_n;
+  uint32_t* err;
+};
+// Capacities (boxes per tile window) of the fused kernel's two tiers.
+constexpr int FUSE_NMAX_SMALL = 20;
+constexpr int FUSE_NMAX_LARGE = 41;
+void launch_tile_fused(int nmax, const TileDesc* desc, int ntiles, const vl_box* entries, const FusedSpec& fs,
+                       const FusedOut& o, int wg, hipStream_t st);
+int tile_fused_max_wg(int nmax);
+
 // Copies the per-tile ranges into tile order: final strips of tile t go to
 // out[out_base + dstF[t] ...], seam strips to side[side_base + dstS[t] ...].
 void launch_tile_reorder(int ntiles, const uint64_t* tab_off, const uint32_t* tab_n, int ntab,
@@ -174,10 +215,12 @@ void launch_gather_desc(const TileDesc* desc, const uint32_t* idx, int n, TileDe
 // Exclusive scan of u32 values into u64 offsets.
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, uint64_t* total, void* tmp,
                      hipStream_t st);
-// Split overflow tiles: per parent one workgroup re-bins its entries into k
-// x-sub-columns (sub-descriptors precomputed on the host).
-// halo > 0: erosion tiles (a box belongs to every sub-column its extent grown
-// by halo meets).
+// Split overflow tiles: per parent one workgroup re-bins its entries into a
+// kx x ky grid of sub-tiles (row-major sub-descriptors precomputed on the
+// host, which splits in x only (ky = 1); all but the last column / row as
+// wide / tall as the first sub-tile).
+// halo > 0: windows grown by halo (a box belongs to every sub-tile its extent
+// grown by halo meets).
 void launch_split_count(const TileDesc* parents, int np, const int32_t* sub_first,
                         const TileDesc* subs, uint32_t* sub_cntA, uint32_t* sub_cntB,
                         const vl_box* entries, int32_t halo, hipStream_t st);

@@ -1,0 +1,289 @@
+// vl_tile_dev.h — device helpers of the per-tile kernels (vl_tile.hip,
+// vl_fused.hip): compressed-grid lookups, run masks, 64x64 bit transposes,
+// erosion passes, overflow lists, and the occupancy-sized persistent grids.
+#pragma once
+#include "vl_internal.h"
+#include "vl_dev.h"
+
+namespace vl {
+namespace {
+
+// ---- shared device helpers -----------------------------------------------------
+
+__device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int32_t v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
+  switch (op) {
+    case TOP_AND: return a & b;
+    case TOP_NOT: return a & ~b;
+    case TOP_XOR: return a ^ b;
+    default: return a | b;
+  }
+}
+
+// Bit w of the dilated change mask of columns cur/prv (null = empty column).
+template <int NW>
+__device__ __forceinline__ void change_mask(const uint64_t* cur, const uint64_t* prv, int W, uint64_t* D) {
+  uint64_t dl[NW + 2];
+  dl[0] = 0;
+  dl[W + 1] = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++)
+    if (w < W) dl[w + 1] = (cur ? cur[w] : 0ull) ^ (prv ? prv[w] : 0ull);
+#pragma unroll
+  for (int w = 0; w < NW; w++)
+    if (w < W) D[w] = dl[w + 1] | (dl[w + 1] << 1) | (dl[w] >> 63) | (dl[w + 1] >> 1) | (dl[w + 2] << 63);
+}
+
+__device__ __forceinline__ bool meets(const uint64_t* D, int lo, int hi) {   // any bit of D in [lo, hi]
+  const int w0 = lo >> 6, w1 = hi >> 6;
+  for (int w = w0; w <= w1; w++) {
+    const int l = (w == w0) ? (lo & 63) : 0;
+    const int h = (w == w1) ? (hi & 63) : 63;
+    if (D[w] & bits_range(l, h)) return true;
+  }
+  return false;
+}
+
+// Start bits of the runs of col that meet D (col null = empty).
+template <int NW>
+__device__ __forceinline__ void starts_meeting(const uint64_t* col, int W, const uint64_t* D, uint64_t* out) {
+#pragma unroll
+  for (int w = 0; w < NW; w++) out[w] = 0;
+  if (!col) return;
+  int a = -1;
+  for (int w = 0; w < W; w++) {
+    const uint64_t v = col[w];
+    int pos = 0;
+    while (pos < 64) {
+      if (a < 0) {
+        const uint64_t rem = v >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        a = w * 64 + pos;
+      } else {
+        const uint64_t rem = (~v) >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        if (meets(D, a, w * 64 + pos - 1)) out[a >> 6] |= 1ull << (a & 63);
+        a = -1;
+      }
+    }
+  }
+  if (a >= 0 && meets(D, a, W * 64 - 1)) out[a >> 6] |= 1ull << (a & 63);
+}
+
+// First zero bit at or above row a in a column (the run end).
+__device__ __forceinline__ int run_end(const uint64_t* col, int W, int a) {
+  int w = a >> 6;
+  uint64_t m = ~col[w] & (~0ull << (a & 63));
+  while (m == 0) {
+    if (++w >= W) return W * 64;
+    m = ~col[w];
+  }
+  return w * 64 + __ffsll((unsigned long long)m) - 1;
+}
+
+// Value of v in lane l (l uniform across the wave).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// 64x64 bit-matrix transpose across a wave: lane l holds row l (bit c = entry
+// (l, c)) and gets column l.  Six block-swap stages: at stage s, entry (l, c)
+// with bit s of l clear and bit s of c set trades places with (l^s, c^s).
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
+  constexpr uint64_t M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                             0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const int sft = 32 >> i;
+    const uint64_t y = xor_lanes64(x, sft);
+    x = (lane & sft) ? ((x & ~M[i]) | ((y >> sft) & M[i])) : ((x & M[i]) | ((y << sft) & ~M[i]));
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_excl_u32(uint32_t v, uint32_t* tot) {
+  uint32_t inc = wave_incl_scan(v);
+  *tot = read_lane(inc, WAVE - 1);
+  return inc - v;
+}
+
+// Calls f(a, b) for every maximal run [a, b) of set bits in a column of W words.
+template <class F>
+__device__ __forceinline__ void for_each_run(const uint64_t* col, int W, F&& f) {
+  int a = -1;
+  for (int w = 0; w < W; w++) {
+    const uint64_t v = col[w];
+    int pos = 0;
+    while (pos < 64) {
+      if (a < 0) {
+        const uint64_t rem = v >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        a = w * 64 + pos;
+      } else {
+        const uint64_t rem = (~v) >> pos;
+        if (rem == 0) break;
+        pos += __ffsll((unsigned long long)rem) - 1;
+        f(a, w * 64 + pos);
+        a = -1;
+      }
+    }
+  }
+  if (a >= 0) f(a, W * 64);
+}
+
+// True when columns c and p differ in any bit of rows lo..hi (p may be null = empty).
+__device__ __forceinline__ bool any_diff(const uint64_t* c, const uint64_t* p, int lo, int hi) {
+  const int w0 = lo >> 6, w1 = hi >> 6;
+  for (int w = w0; w <= w1; w++) {
+    const int l = (w == w0) ? (lo & 63) : 0;
+    const int h = (w == w1) ? (hi & 63) : 63;
+    const uint64_t pv = p ? p[w] : 0ull;
+    if ((c[w] ^ pv) & bits_range(l, h)) return true;
+  }
+  return false;
+}
+
+
+// ---- erosion ------------------------------------------------------------------------
+// Erosion by the square [-R, R]^2 on unit cells: cell p survives when every
+// cell within R of p (in x and in y) is covered.  It is separable (erode in x,
+// then in y).  The compressed grid of an erosion tile holds, besides the
+// clipped box edges, every edge shifted inwards by R (clamped to the tile), so
+// both passes map runs onto existing grid lines: a run [u_a, u_b) becomes
+// [u_a + R, u_b - R).  Columns and rows outside the tile end up empty.
+
+// Number of v[0..n) below x / at most x (v sorted).
+__device__ __forceinline__ int count_below(const int32_t* v, int n, int64_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)v[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int count_at_most(const int32_t* v, int n, int64_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)v[mid] <= x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int32_t clamp64(int64_t v, int64_t lo, int64_t hi) {
+  return (int32_t)(v < lo ? lo : (v > hi ? hi : v));
+}
+
+// x pass: column c of the tile (c0 <= c < c1) keeps the rows covered in every
+// column meeting [ux[c] - R, ux[c+1] + R); windows leaving the grid keep nothing.
+template <int NW>
+__device__ __forceinline__ void erode_x(const uint64_t* M, uint64_t* out, const int32_t* ux, int nxu, int ncol,
+                                        int W, int c0, int c1, int32_t R, int t, int nt) {
+  for (int c = t; c < ncol; c += nt) {
+    uint64_t acc[NW];
+    bool keep = c >= c0 && c < c1;
+    int lo = 0, hi = -1;
+    if (keep) {
+      lo = count_at_most(ux, nxu, (int64_t)ux[c] - R) - 1;
+      hi = count_below(ux, nxu, (int64_t)ux[c + 1] + R) - 1;
+      keep = lo >= 0 && hi < ncol;
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) acc[w] = keep ? ~0ull : 0ull;
+    if (keep)
+      for (int j = lo; j <= hi; j++) {
+#pragma unroll
+        for (int w = 0; w < NW; w++)
+          if (w < W) acc[w] &= M[j * W + w];
+      }
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (w < W) out[c * W + w] = acc[w];
+  }
+}
+
+// y pass: each run [a, b) of a column becomes the rows of
+// [clamp(uy[a] + R), clamp(uy[b] - R)), clamped to the tile's [Y0, Y1].
+template <int NW>
+__device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const int32_t* uy, int nyu, int ncol,
+                                        int W, int32_t Y0, int32_t Y1, int32_t R, int t, int nt) {
+  for (int c = t; c < ncol; c += nt) {
+    uint64_t res[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) res[w] = 0ull;
+    for_each_run(M + c * W, W, [&](int a, int b) {
+      const int32_t ys = clamp64((int64_t)uy[a] + R, Y0, Y1);
+      const int32_t ye = clamp64((int64_t)uy[b] - R, Y0, Y1);
+      if (ys >= ye) return;
+      const int ia = count_below(uy, nyu, ys), ib = count_below(uy, nyu, ye);
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const int l = max(ia, 64 * w), h = min(ib, 64 * w + 64);
+        if (l < h) res[w] |= bits_range(l - 64 * w, h - 1 - 64 * w);
+      }
+    });
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (w < W) out[c * W + w] = res[w];
+  }
+}
+
+// ---- overflow lists -------------------------------------------------------------
+constexpr int TILE_BATCH = 4;   // tiles per load batch of the wave kernel
+constexpr int OVF_BUF = 256;    // overflow tiles buffered in LDS per workgroup
+
+// Appends the n buffered overflow tiles to the launch's overflow list: one
+// atomic per flush of the workgroup, not one per tile (a single counter
+// address serialises its atomics across the whole device).
+__device__ __forceinline__ void flush_overflow(const uint32_t* buf, uint32_t n, uint32_t* ovf, uint32_t* ovf_n,
+                                               int t, int nt) {
+  __shared__ uint32_t s_base;
+  if (t == 0) s_base = atomicAdd(ovf_n, n);
+  __syncthreads();
+  const uint32_t base = s_base;
+  for (uint32_t k = t; k < n; k += nt) ovf[base + k] = buf[k];
+  __syncthreads();
+}
+
+// ---- persistent grids -------------------------------------------------------------
+int num_cus() {
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (ncu < 1) ncu = 1;
+  }
+  return ncu;
+}
+
+// Workgroups of a persistent launch: the occupancy limit, but for the
+// one-wave tile kernels at most 24 waves per CU (6 per SIMD): measured on the
+// 10M-box OP1 ring, 7 or 8 resident waves per SIMD run the tile stage 7 %
+// slower than 6 (and 4 run it 14 % slower).
+constexpr int WAVE_KERNEL_WAVES_PER_CU = 24;
+
+template <class KF>
+int occupancy_wg(KF kern, int threads) {
+  int per = 0;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, threads, 0);
+  if (per < 1) per = 1;
+  if (threads == WAVE && per > WAVE_KERNEL_WAVES_PER_CU) per = WAVE_KERNEL_WAVES_PER_CU;
+  return per * num_cus();
+}
+
+}  // namespace
+}  // namespace vl
