@@ -42,42 +42,39 @@ __device__ __forceinline__ void change_mask(const uint64_t* cur, const uint64_t*
     if (w < W) D[w] = dl[w + 1] | (dl[w + 1] << 1) | (dl[w] >> 63) | (dl[w + 1] >> 1) | (dl[w + 2] << 63);
 }
 
-__device__ __forceinline__ bool meets(const uint64_t* D, int lo, int hi) {   // any bit of D in [lo, hi]
-  const int w0 = lo >> 6, w1 = hi >> 6;
-  for (int w = w0; w <= w1; w++) {
-    const int l = (w == w0) ? (lo & 63) : 0;
-    const int h = (w == w1) ? (hi & 63) : 63;
-    if (D[w] & bits_range(l, h)) return true;
-  }
-  return false;
-}
-
-// Start bits of the runs of col that meet D (col null = empty).
+// Start bits of the runs of col (col null = empty; W <= NW words, bit i of
+// word w = row 64w+i) that meet D: bit a is set when col has a run [a, b)
+// holding a bit of D.  Branch-free, so the lanes of a wave (one column
+// boundary each) do not diverge on their run counts: the bits of D inside
+// runs are spread down to the run starts in six doubling steps per word (step
+// s moves a bit s rows down where the s+1 rows from the target up are all in
+// the run), words from the top down with the spread carried across word
+// borders; then only run starts are kept.
 template <int NW>
 __device__ __forceinline__ void starts_meeting(const uint64_t* col, int W, const uint64_t* D, uint64_t* out) {
+  uint64_t carry = 0;   // spread arriving at row 64(w+1) from above, as bit 63 of word w
 #pragma unroll
-  for (int w = 0; w < NW; w++) out[w] = 0;
-  if (!col) return;
-  int a = -1;
-  for (int w = 0; w < W; w++) {
-    const uint64_t v = col[w];
-    int pos = 0;
-    while (pos < 64) {
-      if (a < 0) {
-        const uint64_t rem = v >> pos;
-        if (rem == 0) break;
-        pos += __ffsll((unsigned long long)rem) - 1;
-        a = w * 64 + pos;
-      } else {
-        const uint64_t rem = (~v) >> pos;
-        if (rem == 0) break;
-        pos += __ffsll((unsigned long long)rem) - 1;
-        if (meets(D, a, w * 64 + pos - 1)) out[a >> 6] |= 1ull << (a & 63);
-        a = -1;
-      }
-    }
+  for (int w = NW - 1; w >= 0; w--) {
+    out[w] = 0;
+    if (w >= W || !col) continue;
+    const uint64_t r = col[w];
+    uint64_t m = (D[w] & r) | (carry & r);
+    uint64_t c = r & (r >> 1);   // bit i: rows i..i+1 in the run; then i..i+2, i..i+4, ...
+    m |= (m >> 1) & c;
+    c &= c >> 1;
+    m |= (m >> 2) & c;
+    c &= c >> 2;
+    m |= (m >> 4) & c;
+    c &= c >> 4;
+    m |= (m >> 8) & c;
+    c &= c >> 8;
+    m |= (m >> 16) & c;
+    c &= c >> 16;
+    m |= (m >> 32) & c;
+    const uint64_t below = (w > 0) ? (col[w - 1] >> 63) : 0ull;
+    out[w] = m & r & ~((r << 1) | below);
+    carry = (m & 1ull) ? (1ull << 63) : 0ull;
   }
-  if (a >= 0 && meets(D, a, W * 64 - 1)) out[a >> 6] |= 1ull << (a & 63);
 }
 
 // First zero bit at or above row a in a column (the run end).
