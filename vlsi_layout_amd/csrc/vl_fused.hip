@@ -39,12 +39,6 @@ __device__ __forceinline__ uint64_t tt_eval(uint32_t tt, uint64_t s, uint64_t m0
   return r;
 }
 
-// Bits of rows [r0, r1) in word w.
-__device__ __forceinline__ uint64_t row_bits(int w, int r0, int r1) {
-  const int lo = max(r0 - 64 * w, 0), hi = min(r1 - 64 * w, 64);
-  return lo < hi ? bits_range(lo, hi - 1) : 0ull;
-}
-
 template <int P>
 __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc, int ntiles,
                                                const vl_box* __restrict__ entries, FusedSpec fs, FusedOut o) {

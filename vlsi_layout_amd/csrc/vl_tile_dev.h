@@ -19,6 +19,12 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int32_t 
   return lo;
 }
 
+// Bits of rows [r0, r1) in word w.
+__device__ __forceinline__ uint64_t row_bits(int w, int r0, int r1) {
+  const int lo = max(r0 - 64 * w, 0), hi = min(r1 - 64 * w, 64);
+  return lo < hi ? bits_range(lo, hi - 1) : 0ull;
+}
+
 __device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
   switch (op) {
     case TOP_AND: return a & b;
