@@ -378,15 +378,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
             while (m) {
               const int j = kb * WAVE + __ffsll((unsigned long long)m) - 1;
               m &= m - 1;
-              int k = ncol;
-#pragma unroll
-              for (int qq = NW - 1; qq >= 0; qq--) {
-                const int t = j - qq * WAVE;
-                uint64_t mm = re[qq];
-                if (t >= 63) mm = 0ull;
-                else if (t >= 0) mm &= ~0ull << (t + 1);
-                if (mm) k = qq * WAVE + __ffsll((unsigned long long)mm) - 1;
-              }
+              const int k = next_bit_above<NW>(re, j, ncol);
               const int b = run_end(MT + j * W, W, a);
               const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
                                   ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |

@@ -526,15 +526,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
         while (m) {
           const int j = kb * WAVE + __ffsll((unsigned long long)m) - 1;
           m &= m - 1;
-          int k = ncol;
-#pragma unroll
-          for (int q = NW - 1; q >= 0; q--) {
-            const int t = j - q * WAVE;   // bits of word q above j
-            uint64_t mm = re[q];
-            if (t >= 63) mm = 0ull;
-            else if (t >= 0) mm &= ~0ull << (t + 1);
-            if (mm) k = q * WAVE + __ffsll((unsigned long long)mm) - 1;
-          }
+          const int k = next_bit_above<NW>(re, j, ncol);
           const int b = run_end(MA + j * W, W, a);
           const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
                               ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |

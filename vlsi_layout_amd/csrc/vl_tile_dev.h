@@ -83,6 +83,30 @@ __device__ __forceinline__ void starts_meeting(const uint64_t* col, int W, const
   }
 }
 
+// First set bit above position j of the NW-word bitset re, or `none`.  The
+// word holding j+1 is picked directly and the words above only while nothing
+// is found (an unrolled scan of every word kept far more registers live in
+// the strip loop).
+template <int NW>
+__device__ __forceinline__ int next_bit_above(const uint64_t (&re)[NW], int j, int none) {
+  const int t = j + 1;
+  const int qw = t >> 6;
+  uint64_t mm = 0ull;
+#pragma unroll
+  for (int q = 0; q < NW; q++)
+    if (q == qw) mm = re[q] & (~0ull << (t & 63));
+  int bq = qw;
+#pragma unroll
+  for (int s = 1; s < NW; s++)
+    if (mm == 0ull && qw + s < NW) {
+#pragma unroll
+      for (int q = 0; q < NW; q++)
+        if (q == qw + s) mm = re[q];
+      bq = qw + s;
+    }
+  return mm ? bq * WAVE + __ffsll((unsigned long long)mm) - 1 : none;
+}
+
 // First zero bit at or above row a in a column (the run end).
 __device__ __forceinline__ int run_end(const uint64_t* col, int W, int a) {
   int w = a >> 6;
