@@ -739,7 +739,7 @@ int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
     void* ptmp = ss.get<char>(partition_tmp_bytes(n));
     uint64_t* cnt = ss.get<uint64_t>(1);
     if (ss.oom) return fail(VL_ENOMEM, "stitch scratch");
-    launch_stitch_keys(o.stitch.p, o.stitch_tag.p, n, keys, vals, nstart, e->st);
+    launch_stitch_keys(o.stitch.p, o.stitch_tag.p, n, keys, vals, e->st);
     radix_sort_pairs(keys, vals, keys2, vals2, n, rtmp, e->st);
     launch_stitch_link(o.stitch.p, o.stitch_tag.p, n, keys, vals, nstart, next, hasp, e->st);
     VLCK(o.out.ensure(o.nout + (uint64_t)n, o.nout));
@@ -849,20 +849,23 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
   if (ntiles <= 0) return VL_OK;
   const int nout = fs.nout;
   Scratch s(e);
-  const int tiers[2] = {FUSE_NMAX_SMALL, FUSE_NMAX_LARGE};
-  uint32_t* small = s.get<uint32_t>(4);   // overflow counts of the two tiers
+  constexpr int NT = 3;
+  const int tiers[NT] = {FUSE_NMAX_TINY, FUSE_NMAX_SMALL, FUSE_NMAX_LARGE};
+  uint32_t* small = s.get<uint32_t>(4);   // overflow counts of the tiers
   unsigned long long* cnt = s.get<unsigned long long>(3 * (size_t)FUSE_MAX_OUT);
-  uint32_t* ovfl[2] = {s.get<uint32_t>(ntiles), s.get<uint32_t>(ntiles)};
+  uint32_t* ovfl[NT];
+  for (int t = 0; t < NT; t++) ovfl[t] = s.get<uint32_t>(ntiles);
   uint64_t* tab_off = s.get<uint64_t>((size_t)nout * ntiles);
   uint32_t* tab_n = s.get<uint32_t>((size_t)nout * 2 * ntiles);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
-  int G[2];
-  for (int t = 0; t < 2; t++) {
+  int G[NT];
+  int nregions = 0;
+  for (int t = 0; t < NT; t++) {
     int mw = tile_fused_max_wg(tiers[t]);
     if (e->max_wg_cap > 0) mw = std::min(mw, e->max_wg_cap);
     G[t] = std::max(1, std::min(ntiles, mw));
+    nregions += G[t];
   }
-  const int nregions = G[0] + G[1];
   // region size per output: a few strips per entry of a first-tier workgroup;
   // regrown to the measured maximum fill if any region overflows
   uint64_t region[FUSE_MAX_OUT];
@@ -898,10 +901,10 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     fo.err = e->d_err;
     if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
     int rbase = 0;
-    for (int t = 0; t < 2; t++) {
+    for (int t = 0; t < NT; t++) {
       fo.region_base = rbase;
-      fo.list = t ? ovfl[0] : nullptr;        // tiles above the first tier's capacity
-      fo.list_n = t ? small : nullptr;
+      fo.list = t ? ovfl[t - 1] : nullptr;        // tiles above the previous tier's capacity
+      fo.list_n = t ? small + (t - 1) : nullptr;
       fo.ovf = ovfl[t];
       fo.ovf_n = small + t;
       launch_tile_fused(tiers[t], d_desc, ntiles, d_entries, fs, fo, G[t], e->st);
@@ -912,7 +915,7 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     constexpr size_t CB = sizeof(hc);
     HIPCK(hipMemcpyAsync(e->h_stage, cnt, CB, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipMemcpyAsync((char*)e->h_stage + CB, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + CB + 4, small + 1, 4, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipMemcpyAsync((char*)e->h_stage + CB + 4, small + NT - 1, 4, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     uint32_t err = 0;
     memcpy(hc, e->h_stage, CB);
@@ -922,7 +925,7 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
       float ms = 0.f;
       HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
       auto& k = e->kst[KS_TILE];
-      k.launches += 2;
+      k.launches += NT;
       k.ms += ms;
       double strips = 0.0;
       for (int q = 0; q < nout; q++) strips += (double)hc[3 * q];
@@ -968,7 +971,7 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
   if (nov > 0) {
     Scratch ss(e);
     SplitTiles sp;
-    VLCK(split_tiles(e, d_desc, ovfl[1], nov, d_entries, fs.halo, FUSE_NMAX_SMALL / 2, FUSE_NMAX_LARGE, depth, ss,
+    VLCK(split_tiles(e, d_desc, ovfl[NT - 1], nov, d_entries, fs.halo, FUSE_NMAX_SMALL / 2, FUSE_NMAX_LARGE, depth, ss,
                      &sp));
     VLCK(process_tiles_fused(e, sp.desc, sp.n, sp.entries, sp.total, fs, outs, depth + 1));
   }

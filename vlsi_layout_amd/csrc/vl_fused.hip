@@ -21,7 +21,8 @@
 // The compressed grid holds, per box, x1 and x2 clipped to the window and the
 // two edges of each morphological image (grown edges clipped to the window,
 // eroded edges x1+d, x2-d clamped to the tile): 6 coordinates per box and
-// axis, so a 128-slot grid takes 20 boxes and a 256-slot grid 41.
+// axis, so a 64-slot grid takes 9 boxes, a 128-slot grid 20 and a 256-slot
+// grid 41.
 #include "vl_tile_dev.h"
 
 namespace vl {
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   constexpr int NCOL = P - 1;
   constexpr int NW = P / WAVE;         // words per column
   constexpr int NL = (NMAX + WAVE) / WAVE;
-  static_assert(NMAX == (P == 128 ? FUSE_NMAX_SMALL : FUSE_NMAX_LARGE), "tier capacities");
+  static_assert(NMAX == (P == 64 ? FUSE_NMAX_TINY : P == 128 ? FUSE_NMAX_SMALL : FUSE_NMAX_LARGE), "tier capacities");
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint64_t MS[NCOL * NW];       // S
   __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
@@ -441,14 +442,18 @@ void launch_f(const TileDesc* desc, int ntiles, const vl_box* entries, const Fus
 
 void launch_tile_fused(int nmax, const TileDesc* desc, int ntiles, const vl_box* entries, const FusedSpec& fs,
                        const FusedOut& o, int wg, hipStream_t st) {
-  if (nmax == FUSE_NMAX_SMALL) launch_f<128>(desc, ntiles, entries, fs, o, wg, st);
+  if (nmax == FUSE_NMAX_TINY) launch_f<64>(desc, ntiles, entries, fs, o, wg, st);
+  else if (nmax == FUSE_NMAX_SMALL) launch_f<128>(desc, ntiles, entries, fs, o, wg, st);
   else launch_f<256>(desc, ntiles, entries, fs, o, wg, st);
 }
 
 int tile_fused_max_wg(int nmax) {
-  static int cache[2] = {0, 0};
-  int& c = cache[nmax == FUSE_NMAX_SMALL ? 0 : 1];
-  if (!c) c = nmax == FUSE_NMAX_SMALL ? occupancy_wg(k_tile_f<128>, WAVE) : occupancy_wg(k_tile_f<256>, WAVE);
+  static int cache[3] = {0, 0, 0};
+  const int i = nmax == FUSE_NMAX_TINY ? 0 : nmax == FUSE_NMAX_SMALL ? 1 : 2;
+  int& c = cache[i];
+  if (!c)
+    c = i == 0 ? occupancy_wg(k_tile_f<64>, WAVE)
+               : i == 1 ? occupancy_wg(k_tile_f<128>, WAVE) : occupancy_wg(k_tile_f<256>, WAVE);
   return c;
 }
 

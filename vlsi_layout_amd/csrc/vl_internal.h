@@ -198,7 +198,9 @@ struct FusedOut {
   uint32_t* ovf_n;
   uint32_t* err;
 };
-// Capacities (boxes per tile window) of the fused kernel's two tiers.
+// Capacities (boxes per tile window) of the fused kernel's tiers (64-, 128-
+// and 256-slot grids of 6 coordinates per box and axis).
+constexpr int FUSE_NMAX_TINY = 9;
 constexpr int FUSE_NMAX_SMALL = 20;
 constexpr int FUSE_NMAX_LARGE = 41;
 void launch_tile_fused(int nmax, const TileDesc* desc, int ntiles, const vl_box* entries, const FusedSpec& fs,
@@ -248,9 +250,12 @@ void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl
 size_t partition_tmp_bytes(int64_t n);
 // X-stitch of strips whose left/right ends sit on internal x-seams.
 void launch_stitch_keys(const vl_box* s, const uint8_t* tag, int64_t n, uint64_t* keys,
-                        uint32_t* vals, uint32_t* nstart, hipStream_t st);
+                        uint32_t* vals, hipStream_t st);
+// Links every strip ending on a right seam to the strip starting at its end
+// corner with the same rows (skeys/svals sorted); nstart receives the number
+// of starts (scratch of one word).
 void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, const uint64_t* skeys,
-                        const uint32_t* svals, const uint32_t* nstart, int32_t* next, uint8_t* has_prev,
+                        const uint32_t* svals, uint32_t* nstart, int32_t* next, uint8_t* has_prev,
                         hipStream_t st);
 void launch_stitch_emit(const vl_box* s, int64_t n, const int32_t* next, const uint8_t* has_prev,
                         vl_box* out, uint64_t out_base, void* tmp, uint64_t* count, hipStream_t st);

@@ -54,13 +54,27 @@ __global__ void k_part_counts(const uint32_t* pre, const uint8_t* tag, int64_t n
   }
 }
 
+// Start keys: (x1, y1) of strips on a left seam, ~0 for the others (no start
+// can have that key: x1 < x2 <= INT32_MAX), so the starts sort first.
 __global__ void k_stitch_keys(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n,
-                              uint64_t* keys, uint32_t* vals, uint32_t* nstart) {
+                              uint64_t* keys, uint32_t* vals) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const bool st = tag[i] & SEAM_L;
     keys[i] = st ? corner_key(s[i].x1, s[i].y1) : ~0ull;
     vals[i] = (uint32_t)i;
-    if (st) atomicAdd(nstart, 1u);
+  }
+}
+
+// Number of starts = position of the first ~0 key in the sorted keys (a
+// counter bumped per start would serialise its atomics over the device).
+__global__ void k_count_starts(const uint64_t* __restrict__ skeys, int64_t n, uint32_t* nstart) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (skeys[mid] != ~0ull) lo = mid + 1; else hi = mid;
+    }
+    *nstart = (uint32_t)lo;
   }
 }
 
@@ -140,15 +154,15 @@ void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl
 }
 
 void launch_stitch_keys(const vl_box* s, const uint8_t* tag, int64_t n, uint64_t* keys, uint32_t* vals,
-                        uint32_t* nstart, hipStream_t st) {
-  launch_fill_u32(nstart, 0u, 1, st);
-  if (n > 0) hipLaunchKernelGGL(k_stitch_keys, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, keys, vals, nstart);
+                        hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_stitch_keys, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, keys, vals);
 }
 
 void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, const uint64_t* skeys,
-                        const uint32_t* svals, const uint32_t* nstart, int32_t* next, uint8_t* has_prev,
+                        const uint32_t* svals, uint32_t* nstart, int32_t* next, uint8_t* has_prev,
                         hipStream_t st) {
   if (n <= 0) return;
+  hipLaunchKernelGGL(k_count_starts, dim3(1), dim3(64), 0, st, skeys, n, nstart);
   hipLaunchKernelGGL(k_zero_u8, dim3(grid_for(n)), dim3(256), 0, st, has_prev, n);
   hipLaunchKernelGGL(k_stitch_link, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, skeys, svals, nstart, next,
                      has_prev);
