@@ -133,8 +133,13 @@ struct vl_engine {
   int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
   int fuse = 1;                  // programs: fuse operator groups over one source layer
-  int fuse_target = 8;           // mean boxes per tile of a fused group's grid
+  int fuse_target = 6;           // mean boxes per tile of a fused group's grid
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
+  // Strips per binned entry that a workgroup's output region is sized for
+  // (plain tile stage / each output of a fused group).  Raised when a region
+  // overflows, so later operators of this engine do not repeat the retry.
+  double region_scale = 4.0;
+  double fuse_region_scale = 4.0;
   double last_ms = 0.0;
   // kernel statistics (collected while prof is set)
   struct KStat {
@@ -449,10 +454,13 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   uint64_t total = 0, nsideT = 0;
   uint32_t nov = 0;
   const double t_pt = g_debug ? now_ms() : 0.0;
-  // region size: a few times the mean strips per workgroup of the first tier;
-  // regrown to the measured maximum fill if any region overflows
-  uint64_t region = (uint64_t)(4.0 * (double)nentries / G[0]) + 1024;
-  for (int attempt = 0; attempt < 2; attempt++) {
+  // region size: a few times the mean strips per workgroup of the first tier.
+  // A region that overflows makes the launch run again with bigger regions:
+  // the measured largest fill doubled (the tiles of the later tiers come from
+  // an overflow list whose order depends on timing, so the fills of a rerun
+  // can differ from the first run's).
+  uint64_t region = (uint64_t)(e->region_scale * (double)nentries / G[0]) + 1024;
+  for (int attempt = 0; attempt < 4; attempt++) {
     if (!area) {
       const size_t need = (size_t)nregions * region;
       VLCK(scr.ensure(need, 0));
@@ -516,8 +524,9 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       break;
     }
     if (rb.err & ERR_OVERFLOW_CAP) {
-      if (attempt == 1) return fail(VL_EINTERNAL, "output capacity still short after resize");
-      region = rb.cnt[2] + 64;
+      if (attempt == 3) return fail(VL_EINTERNAL, "output capacity still short after resize");
+      region = std::max<uint64_t>(2 * region, 2 * rb.cnt[2] + 64);
+      e->region_scale = std::max(e->region_scale, 1.5 * (double)rb.cnt[2] * G[0] / std::max<double>(1.0, nentries));
       if (g_debug) fprintf(stderr, "[vl]  region overflow: retry with %llu strips/region\n", (unsigned long long)region);
       continue;
     }
@@ -866,17 +875,17 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     G[t] = std::max(1, std::min(ntiles, mw));
     nregions += G[t];
   }
-  // region size per output: a few strips per entry of a first-tier workgroup;
-  // regrown to the measured maximum fill if any region overflows
+  // region size per output: a few strips per entry of a first-tier
+  // workgroup; regrown as in the single-operator tile stage
   uint64_t region[FUSE_MAX_OUT];
-  for (int k = 0; k < nout; k++) region[k] = (uint64_t)(4.0 * (double)nentries / G[0]) + 1024;
+  for (int k = 0; k < nout; k++) region[k] = (uint64_t)(e->fuse_region_scale * (double)nentries / G[0]) + 1024;
   std::unique_ptr<Scratch> rs;
   vl_box* scr[FUSE_MAX_OUT] = {};
   uint8_t* scrT[FUSE_MAX_OUT] = {};
   unsigned long long hc[3 * FUSE_MAX_OUT];
   uint32_t nov = 0;
   const double t_pt = g_debug ? now_ms() : 0.0;
-  for (int attempt = 0; attempt < 2; attempt++) {
+  for (int attempt = 0; attempt < 4; attempt++) {
     rs.reset(new Scratch(e));
     for (int k = 0; k < nout; k++) {
       const size_t need = (size_t)nregions * region[k];
@@ -932,8 +941,13 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
       k.bytes += 16.0 * (double)nentries + 16.0 * strips + 48.0 * ntiles;
     }
     if (err & ERR_OVERFLOW_CAP) {
-      if (attempt == 1) return fail(VL_EINTERNAL, "fused output capacity still short after resize");
-      for (int k = 0; k < nout; k++) region[k] = std::max<uint64_t>(region[k], hc[3 * k + 2] + 64);
+      if (attempt == 3) return fail(VL_EINTERNAL, "fused output capacity still short after resize");
+      for (int k = 0; k < nout; k++) {
+        if (hc[3 * k + 2] <= region[k]) continue;
+        region[k] = std::max<uint64_t>(2 * region[k], 2 * hc[3 * k + 2] + 64);
+        e->fuse_region_scale = std::max(e->fuse_region_scale,
+                                        1.5 * (double)hc[3 * k + 2] * G[0] / std::max<double>(1.0, nentries));
+      }
       if (g_debug) fprintf(stderr, "[vl]  fused region overflow: retry\n");
       continue;
     }
@@ -1919,6 +1933,8 @@ int vl_program_run(vl_program* p) {
       w->max_wg_cap = e->max_wg_cap;
       w->fuse = e->fuse;
       w->fuse_target = e->fuse_target;
+      w->region_scale = std::max(w->region_scale, e->region_scale);
+      w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
       w->prof = e->prof;
       w->fused_groups = 0;
       for (auto& st : w->kst) st = vl_engine::KStat();
