@@ -997,12 +997,25 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
 constexpr int FUSE_DECLINED = 1000;
 
 // Outputs c[k] = fs.tt[k](S, morphological images of S) of a fused group.
-int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs, vl_layer* const* c) {
-  const int nout = fs.nout;
+int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer* const* c_in) {
   if (S->n == 0) {
-    for (int k = 0; k < nout; k++) set_empty(e, c[k]);
+    for (int k = 0; k < fs_in.nout; k++) set_empty(e, c_in[k]);
     return VL_OK;
   }
+  // outputs with the truth table 0 (such as the ring's A AND B checks) are
+  // empty whatever S is: they are not evaluated per tile
+  FusedSpec fs = fs_in;
+  vl_layer* c[FUSE_MAX_OUT] = {};
+  int nout = 0;
+  for (int k = 0; k < fs_in.nout; k++) {
+    if (fs_in.tt[k] == 0u) continue;
+    fs.tt[nout] = fs_in.tt[k];
+    c[nout++] = c_in[k];
+  }
+  fs.nout = nout;
+  for (int k = 0; k < fs_in.nout; k++)
+    if (fs_in.tt[k] == 0u) set_empty(e, c_in[k]);
+  if (nout == 0) return VL_OK;
   Box64 m;
   VLCK(layer_mbr(e, S, &m));
   // domain: MBR(S) grown by the largest GROW reach on each side (erosion and
