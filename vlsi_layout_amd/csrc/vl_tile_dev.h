@@ -210,6 +210,22 @@ __device__ __forceinline__ int count_at_most(const int32_t* v, int n, int64_t x)
   return lo;
 }
 
+// count_below(v, n, x) when the answer is known to lie near index h: a few
+// steps from h (each one dependent LDS read), then a binary search if the
+// walk has not settled.  Erosion lookups land within a step or two of the
+// grid line they start from.
+__device__ __forceinline__ int count_below_near(const int32_t* v, int n, int64_t x, int h) {
+  h = h < 0 ? 0 : (h > n ? n : h);
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    if (h < n && (int64_t)v[h] < x) h++;
+    else if (h > 0 && (int64_t)v[h - 1] >= x) h--;
+    else return h;
+  }
+  if ((h == n || (int64_t)v[h] >= x) && (h == 0 || (int64_t)v[h - 1] < x)) return h;
+  return count_below(v, n, x);
+}
+
 __device__ __forceinline__ int32_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return (int32_t)(v < lo ? lo : (v > hi ? hi : v));
 }
@@ -224,8 +240,8 @@ __device__ __forceinline__ void erode_x(const uint64_t* M, uint64_t* out, const 
     bool keep = c >= c0 && c < c1;
     int lo = 0, hi = -1;
     if (keep) {
-      lo = count_at_most(ux, nxu, (int64_t)ux[c] - R) - 1;
-      hi = count_below(ux, nxu, (int64_t)ux[c + 1] + R) - 1;
+      lo = count_below_near(ux, nxu, (int64_t)ux[c] - R + 1, c) - 1;   // last line <= ux[c] - R
+      hi = count_below_near(ux, nxu, (int64_t)ux[c + 1] + R, c + 1) - 1;
       keep = lo >= 0 && hi < ncol;
     }
 #pragma unroll
@@ -255,7 +271,7 @@ __device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const 
       const int32_t ys = clamp64((int64_t)uy[a] + R, Y0, Y1);
       const int32_t ye = clamp64((int64_t)uy[b] - R, Y0, Y1);
       if (ys >= ye) return;
-      const int ia = count_below(uy, nyu, ys), ib = count_below(uy, nyu, ye);
+      const int ia = count_below_near(uy, nyu, ys, a), ib = count_below_near(uy, nyu, ye, b);
 #pragma unroll
       for (int w = 0; w < NW; w++) {
         const int l = max(ia, 64 * w), h = min(ib, 64 * w + 64);
