@@ -54,39 +54,41 @@ __global__ void k_part_counts(const uint32_t* pre, const uint8_t* tag, int64_t n
   }
 }
 
-// Start keys: (x1, y1) of strips on a left seam, ~0 for the others (no start
-// can have that key: x1 < x2 <= INT32_MAX), so the starts sort first.
-__global__ void k_stitch_keys(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n,
-                              uint64_t* keys, uint32_t* vals) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const bool st = tag[i] & SEAM_L;
-    keys[i] = st ? corner_key(s[i].x1, s[i].y1) : ~0ull;
-    vals[i] = (uint32_t)i;
-  }
+// Key of a corner relative to the domain origin: x - x0 above the low yb bits,
+// y - y0 in them (both below 2^32 for int32 coordinates).
+__device__ __forceinline__ uint64_t rel_key(int32_t x, int32_t y, int64_t x0, int64_t y0, int yb) {
+  return ((uint64_t)((int64_t)x - x0) << yb) | (uint64_t)((int64_t)y - y0);
 }
 
-// Number of starts = position of the first ~0 key in the sorted keys (a
-// counter bumped per start would serialise its atomics over the device).
-__global__ void k_count_starts(const uint64_t* __restrict__ skeys, int64_t n, uint32_t* nstart) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (skeys[mid] != ~0ull) lo = mid + 1; else hi = mid;
+// Starts are the strips on a left seam: flag, then (after a scan) their keys
+// and indices at the scanned positions, so only starts are sorted.
+__global__ void k_start_flags(const uint8_t* __restrict__ tag, int64_t n, uint32_t* flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = (tag[i] & SEAM_L) ? 1u : 0u;
+}
+
+__global__ void k_start_keys(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n,
+                             const uint32_t* __restrict__ pre, int64_t x0, int64_t y0, int yb, uint64_t* keys,
+                             uint32_t* vals) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (tag[i] & SEAM_L) {
+      keys[pre[i]] = rel_key(s[i].x1, s[i].y1, x0, y0, yb);
+      vals[pre[i]] = (uint32_t)i;
     }
-    *nstart = (uint32_t)lo;
-  }
+}
+
+__global__ void k_start_count(const uint32_t* pre, const uint8_t* tag, int64_t n, uint64_t* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = n ? (uint64_t)pre[n - 1] + ((tag[n - 1] & SEAM_L) ? 1u : 0u) : 0u;
 }
 
 __global__ void k_stitch_link(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n,
-                              const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
-                              const uint32_t* __restrict__ nstart_p, int32_t* next, uint8_t* has_prev) {
-  const int64_t ns = *nstart_p;
+                              const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t ns,
+                              int64_t x0, int64_t y0, int yb, int32_t* next, uint8_t* has_prev) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int32_t nx = -1;
     if (tag[i] & SEAM_R) {
       const vl_box b = s[i];
-      const uint64_t k = corner_key(b.x2, b.y1);
+      const uint64_t k = rel_key(b.x2, b.y1, x0, y0, yb);
       int64_t lo = 0, hi = ns;
       while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
@@ -153,19 +155,28 @@ void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl
   hipLaunchKernelGGL(k_part_counts, dim3(1), dim3(64), 0, st, pre, tag, n, counts);
 }
 
-void launch_stitch_keys(const vl_box* s, const uint8_t* tag, int64_t n, uint64_t* keys, uint32_t* vals,
-                        hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_stitch_keys, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, keys, vals);
+void launch_stitch_starts(const vl_box* s, const uint8_t* tag, int64_t n, int64_t x0, int64_t y0, int yb,
+                          uint64_t* keys, uint32_t* vals, uint64_t* count, void* tmp, hipStream_t st) {
+  if (n <= 0) {
+    launch_fill_u64(count, 0ull, 1, st);
+    return;
+  }
+  uint32_t* flag = (uint32_t*)tmp;
+  uint32_t* pre = flag + n;
+  char* stmp = (char*)(((uintptr_t)(pre + n) + 255) & ~(uintptr_t)255);
+  hipLaunchKernelGGL(k_start_flags, dim3(grid_for(n)), dim3(256), 0, st, tag, n, flag);
+  scan_exclusive_u32(flag, pre, (size_t)n, nullptr, stmp, st);
+  hipLaunchKernelGGL(k_start_keys, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, pre, x0, y0, yb, keys, vals);
+  hipLaunchKernelGGL(k_start_count, dim3(1), dim3(64), 0, st, pre, tag, n, count);
 }
 
 void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, const uint64_t* skeys,
-                        const uint32_t* svals, uint32_t* nstart, int32_t* next, uint8_t* has_prev,
-                        hipStream_t st) {
+                        const uint32_t* svals, int64_t ns, int64_t x0, int64_t y0, int yb, int32_t* next,
+                        uint8_t* has_prev, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_count_starts, dim3(1), dim3(64), 0, st, skeys, n, nstart);
   hipLaunchKernelGGL(k_zero_u8, dim3(grid_for(n)), dim3(256), 0, st, has_prev, n);
-  hipLaunchKernelGGL(k_stitch_link, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, skeys, svals, nstart, next,
-                     has_prev);
+  hipLaunchKernelGGL(k_stitch_link, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, skeys, svals, ns, x0, y0, yb,
+                     next, has_prev);
 }
 
 void launch_stitch_emit(const vl_box* s, int64_t n, const int32_t* next, const uint8_t* has_prev, vl_box* out,
