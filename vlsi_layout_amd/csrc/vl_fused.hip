@@ -51,6 +51,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   constexpr int NL = (NMAX + WAVE) / WAVE;
   static_assert(NMAX == (P == 64 ? FUSE_NMAX_TINY : P == 128 ? FUSE_NMAX_SMALL : FUSE_NMAX_LARGE), "tier capacities");
   __shared__ int32_t ux[P], uy[P];
+  __shared__ uint8_t sidx_x[P], sidx_y[P]; // column / row index of every coordinate slot
   __shared__ uint64_t MS[NCOL * NW];       // S
   __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
   __shared__ uint64_t MT[NCOL * NW];       // erosion temporary, then the current output
@@ -152,9 +153,11 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       const int32_t wy0 = clamp64((int64_t)d.y0 - R, INT32_MIN, INT32_MAX);
       const int32_t wy1 = clamp64((int64_t)d.y1 + R, INT32_MIN, INT32_MAX);
       int32_t vx[K], vy[K];
+      int ex[K], ey[K];   // slot of origin, carried through the sort
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int e = lane * K + k;
+        ex[k] = ey[k] = e;
         const int i = e / CPB, comp = e - CPB * (e / CPB);
         int32_t cx = INT32_MAX, cy = INT32_MAX;
         if (i < n) {
@@ -187,7 +190,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         vy[k] = cy;
       }
       const int cnt = CPB * n + CPB;
-      // bitonic sort of P elements, element e = lane*K + k
+      // bitonic sort of P elements, element e = lane*K + k, each with its slot
+      // of origin (of two equal values each partner keeps its own)
 #pragma unroll
       for (int size = 2; size <= P; size <<= 1) {
 #pragma unroll
@@ -197,11 +201,15 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
 #pragma unroll
             for (int k = 0; k < K; k++) {
               const int e = lane * K + k;
-              const int px = xor_lanes(vx[k], lj);
-              const int py = xor_lanes(vy[k], lj);
+              const int px = xor_lanes(vx[k], lj), pex = xor_lanes(ex[k], lj);
+              const int py = xor_lanes(vy[k], lj), pey = xor_lanes(ey[k], lj);
               const bool take_min = ((e & j) == 0) == ((e & size) == 0);
-              vx[k] = take_min ? min(vx[k], px) : max(vx[k], px);
-              vy[k] = take_min ? min(vy[k], py) : max(vy[k], py);
+              const bool kx = take_min ? vx[k] <= px : vx[k] >= px;
+              const bool ky = take_min ? vy[k] <= py : vy[k] >= py;
+              vx[k] = kx ? vx[k] : px;
+              ex[k] = kx ? ex[k] : pex;
+              vy[k] = ky ? vy[k] : py;
+              ey[k] = ky ? ey[k] : pey;
             }
           } else {
 #pragma unroll
@@ -210,9 +218,15 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
                 const int k2 = k | j;
                 const bool asc = ((lane * K + k) & size) == 0;
                 int32_t a = vx[k], b = vx[k2];
-                if ((a > b) == asc) { vx[k] = b; vx[k2] = a; }
+                if ((a > b) == asc) {
+                  vx[k] = b; vx[k2] = a;
+                  const int t = ex[k]; ex[k] = ex[k2]; ex[k2] = t;
+                }
                 a = vy[k]; b = vy[k2];
-                if ((a > b) == asc) { vy[k] = b; vy[k2] = a; }
+                if ((a > b) == asc) {
+                  vy[k] = b; vy[k2] = a;
+                  const int t = ey[k]; ey[k] = ey[k2]; ey[k2] = t;
+                }
               }
             }
           }
@@ -235,10 +249,13 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         }
         uint32_t tx, ty;
         uint32_t ox = (uint32_t)wave_excl_u32(cx, &tx), oy = (uint32_t)wave_excl_u32(cy, &ty);
+        // distinct values in order; every slot learns the index of its value
 #pragma unroll
         for (int k = 0; k < K; k++) {
           if (fx[k]) ux[ox++] = vx[k];
           if (fy[k]) uy[oy++] = vy[k];
+          sidx_x[ex[k]] = (uint8_t)(ox - 1);
+          sidx_y[ey[k]] = (uint8_t)(oy - 1);
         }
         nxu = (int)tx;
         nyu = (int)ty;
@@ -253,20 +270,14 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       __syncthreads();
       // 2. rasterise S and the grown images (box-parallel)
       for (int i = lane; i < n; i += WAVE) {
-        const vl_box b = bx[i];
 #pragma unroll
         for (int m = -1; m < 2; m++) {
           if (m == 0 && !g0) continue;
           if (m == 1 && !g1) continue;
-          int32_t bx1 = b.x1, by1 = b.y1, bx2 = b.x2, by2 = b.y2;
-          if (m >= 0) {
-            bx1 = clamp64((int64_t)b.x1 - fs.arg[m][2], wx0, wx1);
-            by1 = clamp64((int64_t)b.y1 - fs.arg[m][3], wy0, wy1);
-            bx2 = clamp64((int64_t)b.x2 + fs.arg[m][0], wx0, wx1);
-            by2 = clamp64((int64_t)b.y2 + fs.arg[m][1], wy0, wy1);
-          }
-          const int cx1 = lower_bound_i32(ux, nxu, max(bx1, wx0)), cx2 = lower_bound_i32(ux, nxu, min(bx2, wx1));
-          const int cy1 = lower_bound_i32(uy, nyu, max(by1, wy0)), cy2 = lower_bound_i32(uy, nyu, min(by2, wy1));
+          // the box's edges (S: slots 0, 1; grown image m: slots 2+2m, 3+2m)
+          const int e0 = CPB * i + (m < 0 ? 0 : 2 + 2 * m);
+          const int cx1 = sidx_x[e0], cx2 = sidx_x[e0 + 1];
+          const int cy1 = sidx_y[e0], cy2 = sidx_y[e0 + 1];
           if (cx1 >= cx2 || cy1 >= cy2) continue;
           uint64_t* M = m < 0 ? MS : MM[m];
           const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;
