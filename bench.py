@@ -11,8 +11,9 @@ Throughput = input boxes of all operators of a step (N_A+N_B for binary ops,
 N for GROW/SIZE) / wall time of the step, with the engine stream synchronised
 on both sides of the timed region.  "plain" runs one operator per call with
 the default tile budgets; "batched" runs the same operators as one recorded
-program (independent layer chains concurrently on worker streams) after the
-autotuner has picked the tile budgets.
+program after the autotuner has picked the tile budgets: each layer's ring is
+one fused group (the layer binned once, every output computed per tile from
+shared masks) and the four layer chains run concurrently on worker streams.
 
 The roofline figure is for the per-tile Boolean kernel (the dominant kernel):
 algorithmic bytes of its launches in the timed region (16 B per entry read,
@@ -215,8 +216,9 @@ def main():
     tuned = None
     for m in modes:
         if m == "batched":
-            # the batched form runs with autotuned tile budgets (timed on met1's ring inputs)
-            tuned = eng.autotune(rings[2].L1, rings[2].L)
+            # the batched form runs with autotuned tile budgets (timed on met1's ring:
+            # met1 NOT SIZE(met1, -1), GROW, SIZE and the fused ring of met1)
+            tuned = eng.autotune(rings[2].L, rings[2].L2)
             print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
         eng.set("profile", 1)
         secs, out = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps, args.warmup)

@@ -84,10 +84,12 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
 int vl_engine_get_param(vl_engine* e, const char* key, int64_t* value);
 /* Block until all work queued on the engine's stream has finished. */
 int vl_engine_sync(vl_engine* e);
-/* Autotuner (no counterpart in the reference): sets "tile_target" and
- * "erode_target" to the fastest of a few candidates, timed on A NOT B (or the
- * merge of A when b is NULL) plus GROW(A; 1), and on SIZE(A; -1).  The chosen
- * values are returned when the pointers are non-null.  Results of every
+/* Autotuner (no counterpart in the reference): sets "tile_target",
+ * "erode_target" and "fuse_target" to the fastest of a few candidates, timed
+ * on A NOT B (or the merge of A when b is NULL) plus GROW(A; 1), on
+ * SIZE(A; -1), and on the fused ring of A (GROW(A; 1), SIZE(A; -1) and their
+ * two differences with A).  The first two are returned when the pointers are
+ * non-null ("fuse_target" through vl_engine_get_param).  Results of every
  * operator are independent of these parameters. */
 int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64_t* target,
                        int64_t* erode_target);

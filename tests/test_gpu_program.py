@@ -121,6 +121,7 @@ def test_autotune_sets_parameters_and_keeps_results(gpu_engine):
     V.CalculateFrameNOT(L1, L, before)
     chosen = eng.autotune(L1, L)
     assert chosen["tile_target"] in (12, 16, 20, 24, 32) and chosen["erode_target"] in (8, 12, 16)
+    assert chosen["fuse_target"] in (4, 5, 6, 7, 8, 10) and eng.get("fuse_target") == chosen["fuse_target"]
     assert eng.get("tile_target") == chosen["tile_target"]
     after = V.Layer(eng)
     V.CalculateFrameNOT(L1, L, after)

@@ -2303,12 +2303,14 @@ int vl_server_run(vl_server* sv, const vl_query* q, int64_t n, vl_query_result* 
 // ---- autotuner -------------------------------------------------------------------
 extern "C" {
 
-// Picks the tile box budget of the operator grid ("tile_target") and of the
-// erosion grid ("erode_target") by timing candidates on the caller's layers:
-// A NOT B (or the merge of A) plus GROW(A; 1) for the operator grid, SIZE(A; -1)
-// for the erosion grid; each candidate is timed twice (device time of the
-// operator, HIP events) and the faster run counts.  Results never depend on
-// these parameters, only run time.
+// Picks the tile box budget of the operator grid ("tile_target"), of the
+// erosion grid ("erode_target") and of a fused group's grid ("fuse_target") by
+// timing candidates on the caller's layers: A NOT B (or the merge of A) plus
+// GROW(A; 1) for the operator grid, SIZE(A; -1) for the erosion grid, and the
+// fused ring of A (GROW(A; 1), SIZE(A; -1), GROW(A; 1) NOT A, A NOT
+// SIZE(A; -1)) for the fused grid; each candidate is timed twice (device time,
+// HIP events) and the faster run counts.  Results never depend on these
+// parameters, only run time.
 int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64_t* target, int64_t* erode_target) {
   if (!e || !a) return fail(VL_EINVAL, "null argument");
   std::lock_guard<std::recursive_mutex> lk(e->mu);
@@ -2354,9 +2356,54 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
     }
   }
   e->erode_target = bet;
+  // fused ring of A: four outputs over S = A and the images GROW 1, erode 1
+  // (the current value stays unless a candidate is more than 3 % faster:
+  // neighbouring budgets differ by less than the run-to-run noise)
+  const int ftargets[] = {4, 5, 6, 7, 8, 10};
+  const int ft0 = e->fuse_target;
+  int bft = ft0;
+  double ft0_ms = 1e30;
+  {
+    FusedSpec fs{};
+    fs.nmorph = 2;
+    fs.kind[0] = MORPH_GROW;
+    fs.kind[1] = MORPH_ERODE;
+    for (int k = 0; k < 4; k++) fs.arg[0][k] = 1;
+    fs.arg[1][0] = 1;
+    fs.halo = 1;
+    fs.nout = 4;
+    fs.tt[0] = 0xCCu;                  // GROW(A)
+    fs.tt[1] = 0xF0u;                  // SIZE(A, -1)
+    fs.tt[2] = 0xCCu & ~0xAAu;         // GROW(A) NOT A
+    fs.tt[3] = 0xAAu & ~0xF0u;         // A NOT SIZE(A, -1)
+    vl_layer* outs[4] = {};
+    for (int k = 0; k < 4 && rc == VL_OK; k++) rc = vl_layer_create(e, &outs[k]);
+    bms = 1e30;
+    for (int ft : ftargets) {
+      if (rc != VL_OK) break;
+      e->fuse_target = ft;
+      double m = 0;
+      rc = best_of([&]() {
+        HIPCK(hipSetDevice(e->device));
+        OpTimer tm(e);
+        const int r = run_fused(e, a, fs, outs);
+        return r == FUSE_DECLINED ? VL_OK : r;   // a declined candidate times nothing useful
+      }, &m);
+      if (rc == VL_OK && ft == ft0) ft0_ms = m;
+      if (rc == VL_OK && m < bms) {
+        bms = m;
+        bft = ft;
+      }
+    }
+    if (bms > 0.97 * ft0_ms) bft = ft0;
+    for (int k = 0; k < 4; k++)
+      if (outs[k]) vl_layer_destroy(outs[k]);
+  }
+  e->fuse_target = bft;
   if (rc != VL_OK) {
     e->target = t0;
     e->erode_target = et0;
+    e->fuse_target = ft0;
   }
   vl_layer_destroy(c);
   if (target) *target = e->target;
