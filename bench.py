@@ -55,6 +55,7 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1_000_000, help="boxes per layer in the CPU sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--mode", choices=["plain", "batched", "both"], default="both")
+    p.add_argument("--no-autotune", action="store_true", help="batched form with the default tile budgets")
     p.add_argument("--c2-total", type=int, default=10_000_000,
                    help="also time the same ring at this many boxes over the 4 layers (0 = skip)")
     p.add_argument("--serving-boxes", type=int, default=50_000_000,
@@ -215,7 +216,7 @@ def main():
     modes = ["plain", "batched"] if args.mode == "both" else [args.mode]
     tuned = None
     for m in modes:
-        if m == "batched":
+        if m == "batched" and not args.no_autotune:
             # the batched form runs with autotuned tile budgets (timed on met1's ring:
             # met1 NOT SIZE(met1, -1), GROW, SIZE and the fused ring of met1)
             tuned = eng.autotune(rings[2].L, rings[2].L2)
