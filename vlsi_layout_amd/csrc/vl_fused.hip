@@ -52,6 +52,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   static_assert(NMAX == (P == 64 ? FUSE_NMAX_TINY : P == 128 ? FUSE_NMAX_SMALL : FUSE_NMAX_LARGE), "tier capacities");
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint8_t sidx_x[P], sidx_y[P]; // column / row index of every coordinate slot
+  __shared__ uint16_t eup[P], edn[P];      // row maps of the y erosion
   __shared__ uint64_t MS[NCOL * NW];       // S
   __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
   __shared__ uint64_t MT[NCOL * NW];       // erosion temporary, then the current output
@@ -295,15 +296,17 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       const int rowT0 = count_below(uy, nyu, d.y0), rowT1 = count_below(uy, nyu, d.y1);
       // 3. erosion images
       if (e0) {
+        erosion_row_maps(uy, nyu, d.y0, d.y1, fs.arg[0][0], eup, edn, lane, WAVE);
         erode_x<NW>(MS, MT, ux, nxu, ncol, W, colT0, colT1, fs.arg[0][0], lane, WAVE);
         __syncthreads();
-        erode_y<NW>(MT, MM[0], uy, nyu, ncol, W, d.y0, d.y1, fs.arg[0][0], lane, WAVE);
+        erode_y<NW>(MT, MM[0], eup, edn, ncol, W, lane, WAVE);
         __syncthreads();
       }
       if (e1) {
+        erosion_row_maps(uy, nyu, d.y0, d.y1, fs.arg[1][0], eup, edn, lane, WAVE);
         erode_x<NW>(MS, MT, ux, nxu, ncol, W, colT0, colT1, fs.arg[1][0], lane, WAVE);
         __syncthreads();
-        erode_y<NW>(MT, MM[1], uy, nyu, ncol, W, d.y0, d.y1, fs.arg[1][0], lane, WAVE);
+        erode_y<NW>(MT, MM[1], eup, edn, ncol, W, lane, WAVE);
         __syncthreads();
       }
       const uint32_t flags = d.flags;

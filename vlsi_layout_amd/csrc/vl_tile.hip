@@ -156,6 +156,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
   constexpr int NCOL = P - 1;
   constexpr int NW = P / WAVE;        // words per column
   __shared__ int32_t ux[P], uy[P];
+  __shared__ uint16_t eup[ERO ? P : 1], edn[ERO ? P : 1];   // row maps of the y erosion
   __shared__ uint64_t MA[NCOL * NW];
   __shared__ uint64_t MX[(TWO || ERO) ? NCOL * NW : 1];   // layer B mask / x-eroded mask
   constexpr int NL = (NMAX + WAVE) / WAVE;                // box loads per lane and tile
@@ -423,9 +424,10 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       colT1 = count_below(ux, nxu, d.x1);
       rowT0 = count_below(uy, nyu, d.y0);
       rowT1 = count_below(uy, nyu, d.y1);
+      erosion_row_maps(uy, nyu, d.y0, d.y1, R, eup, edn, lane, WAVE);
       erode_x<NW>(MA, MX, ux, nxu, ncol, W, colT0, colT1, R, lane, WAVE);
       __syncthreads();
-      erode_y<NW>(MX, MA, uy, nyu, ncol, W, d.y0, d.y1, R, lane, WAVE);
+      erode_y<NW>(MX, MA, eup, edn, ncol, W, lane, WAVE);
       __syncthreads();
     }
     if (AREA) {
@@ -579,6 +581,7 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
   constexpr int NW = (NCOL + 63) / 64;
   __shared__ int32_t sx[P], sy[P];
   __shared__ int32_t ux[P], uy[P];
+  __shared__ uint16_t eup[ERO ? P : 1], edn[ERO ? P : 1];   // row maps of the y erosion
   __shared__ int16_t cb[NMAX][4];
   __shared__ uint64_t MA[NCOL * NW];
   __shared__ uint64_t MB[(TWO || ERO) ? NCOL * NW : 1];
@@ -729,9 +732,10 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
         colT1 = count_below(ux, nxu, d.x1);
         rowT0 = count_below(uy, nyu, d.y0);
         rowT1 = count_below(uy, nyu, d.y1);
+        erosion_row_maps(uy, nyu, d.y0, d.y1, R, eup, edn, tid, NT);
         erode_x<NW>(MA, MB, ux, nxu, ncol, W, colT0, colT1, R, tid, NT);
         __syncthreads();
-        erode_y<NW>(MB, MA, uy, nyu, ncol, W, d.y0, d.y1, R, tid, NT);
+        erode_y<NW>(MB, MA, eup, edn, ncol, W, tid, NT);
         __syncthreads();
       }
       const uint32_t flags = d.flags;

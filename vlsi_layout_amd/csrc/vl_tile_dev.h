@@ -258,20 +258,31 @@ __device__ __forceinline__ void erode_x(const uint64_t* M, uint64_t* out, const 
   }
 }
 
-// y pass: each run [a, b) of a column becomes the rows of
-// [clamp(uy[a] + R), clamp(uy[b] - R)), clamped to the tile's [Y0, Y1].
+// Row maps of the y pass: for every grid line i, the index of the line
+// uy[i] + R (up) and of the line uy[i] - R (dn), both clamped to the tile's
+// [Y0, Y1] (one walk per line, lane-parallel, instead of two per run).
+__device__ __forceinline__ void erosion_row_maps(const int32_t* uy, int nyu, int32_t Y0, int32_t Y1, int32_t R,
+                                                 uint16_t* up, uint16_t* dn, int t, int nt) {
+  for (int i = t; i < nyu; i += nt) {
+    up[i] = (uint16_t)count_below_near(uy, nyu, clamp64((int64_t)uy[i] + R, Y0, Y1), i);
+    dn[i] = (uint16_t)count_below_near(uy, nyu, clamp64((int64_t)uy[i] - R, Y0, Y1), i);
+  }
+}
+
+// y pass: each run [a, b) of a column becomes the rows [up[a], dn[b]), i.e.
+// [clamp(uy[a] + R), clamp(uy[b] - R)) clamped to the tile's [Y0, Y1].  Run
+// borders of the x-eroded mask are box edges or window borders, whose shifted
+// lines are on the grid.
 template <int NW>
-__device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const int32_t* uy, int nyu, int ncol,
-                                        int W, int32_t Y0, int32_t Y1, int32_t R, int t, int nt) {
+__device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const uint16_t* up, const uint16_t* dn,
+                                        int ncol, int W, int t, int nt) {
   for (int c = t; c < ncol; c += nt) {
     uint64_t res[NW];
 #pragma unroll
     for (int w = 0; w < NW; w++) res[w] = 0ull;
     for_each_run(M + c * W, W, [&](int a, int b) {
-      const int32_t ys = clamp64((int64_t)uy[a] + R, Y0, Y1);
-      const int32_t ye = clamp64((int64_t)uy[b] - R, Y0, Y1);
-      if (ys >= ye) return;
-      const int ia = count_below_near(uy, nyu, ys, a), ib = count_below_near(uy, nyu, ye, b);
+      const int ia = up[a], ib = dn[b];
+      if (ia >= ib) return;
 #pragma unroll
       for (int w = 0; w < NW; w++) {
         const int l = max(ia, 64 * w), h = min(ib, 64 * w + 64);
