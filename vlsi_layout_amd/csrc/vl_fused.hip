@@ -56,6 +56,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   __shared__ uint64_t MS[NCOL * NW];       // S
   __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
   __shared__ uint64_t MT[NCOL * NW];       // erosion temporary, then the current output
+  constexpr int NQ = NW == 1 ? 4 : 1;      // outputs evaluated in one pass (one-word grids)
+  __shared__ uint64_t MTQ[NQ][NW == 1 ? NCOL : 1];
   __shared__ vl_box sbuf[TILE_BATCH * (NMAX + 1)];
   __shared__ TileDesc sdesc[TILE_BATCH];
   uint32_t* const sdw = (uint32_t*)sdesc;
@@ -311,33 +313,67 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       }
       const uint32_t flags = d.flags;
       const int KW = (ncol + 1 + 63) >> 6;   // boundary words per row
-      // 4. outputs, one after the other
-#pragma unroll 1
-      for (int q = 0; q < nout; q++) {
-        const uint32_t tt = s_tt[q];
-        uint64_t any = 0;
-        for (int c = lane; c < ncol; c += WAVE) {
-          const bool in = c >= colT0 && c < colT1;
-#pragma unroll
-          for (int w = 0; w < NW; w++) {
-            if (w >= W) break;
-            const int ix = c * W + w;
-            const uint64_t v = in ? tt_eval(tt, MS[ix], (nmorph > 0) ? MM[0][ix] : 0ull,
-                                            (nmorph > 1) ? MM[1][ix] : 0ull) & row_bits(w, rowT0, rowT1)
-                                  : 0ull;
-            MT[ix] = v;
-            any |= v;
-          }
+      // 4. outputs.  On one-word grids up to four outputs get their column
+      //    words from one read of the base masks (bit q of anymask: output q
+      //    is not empty); otherwise each output is evaluated into MT in turn.
+      uint32_t anymask = 0;
+      const bool onepass = NW == 1 && nout <= NQ;
+      if (onepass) {
+        const int c = lane;
+        const bool in = c < ncol && c >= colT0 && c < colT1;
+        uint64_t b0 = 0ull, b1 = 0ull, b2 = 0ull;
+        if (in) {
+          b0 = MS[c];
+          if (nmorph > 0) b1 = MM[0][c];
+          if (nmorph > 1) b2 = MM[1][c];
         }
-        uint32_t* const tn = s_tn[q];
-        if (__ballot(any != 0ull) == 0ull) {
-          if (lane == 0) {
-            tn[ti] = 0;
-            tn[o.ntab + ti] = 0;
-          }
-          continue;
+        const uint64_t rb = row_bits(0, rowT0, rowT1);
+#pragma unroll 1
+        for (int q = 0; q < nout; q++) {
+          const uint64_t v = in ? tt_eval(s_tt[q], b0, b1, b2) & rb : 0ull;
+          if (c < ncol) MTQ[min(q, NQ - 1)][c] = v;
+          if (__ballot(v != 0ull)) anymask |= 1u << q;
         }
         __syncthreads();
+      }
+#pragma unroll 1
+      for (int q = 0; q < nout; q++) {
+        uint32_t* const tn = s_tn[q];
+        const uint64_t* MO = MT;
+        if (onepass) {
+          MO = MTQ[min(q, NQ - 1)];
+          if (!((anymask >> q) & 1u)) {
+            if (lane == 0) {
+              tn[ti] = 0;
+              tn[o.ntab + ti] = 0;
+            }
+            continue;
+          }
+        } else {
+          const uint32_t tt = s_tt[q];
+          uint64_t any = 0;
+          for (int c = lane; c < ncol; c += WAVE) {
+            const bool in = c >= colT0 && c < colT1;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+              if (w >= W) break;
+              const int ix = c * W + w;
+              const uint64_t v = in ? tt_eval(tt, MS[ix], (nmorph > 0) ? MM[0][ix] : 0ull,
+                                              (nmorph > 1) ? MM[1][ix] : 0ull) & row_bits(w, rowT0, rowT1)
+                                    : 0ull;
+              MT[ix] = v;
+              any |= v;
+            }
+          }
+          if (__ballot(any != 0ull) == 0ull) {
+            if (lane == 0) {
+              tn[ti] = 0;
+              tn[o.ntab + ti] = 0;
+            }
+            continue;
+          }
+          __syncthreads();
+        }
         // start / end masks at every slab boundary (lane owns boundaries 64*kb + lane)
         uint64_t Sr[NW][NW], Er[NW][NW];
         uint32_t myT = 0;
@@ -347,8 +383,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
 #pragma unroll
           for (int w = 0; w < NW; w++) { Sr[kb][w] = 0ull; Er[kb][w] = 0ull; }
           if (k <= ncol) {
-            const uint64_t* cur = (k < ncol) ? MT + k * W : nullptr;
-            const uint64_t* prv = (k > 0) ? MT + (k - 1) * W : nullptr;
+            const uint64_t* cur = (k < ncol) ? MO + k * W : nullptr;
+            const uint64_t* prv = (k > 0) ? MO + (k - 1) * W : nullptr;
             uint64_t D[NW];
             change_mask<NW>(cur, prv, W, D);
             starts_meeting<NW>(cur, W, D, Sr[kb]);
@@ -394,7 +430,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
               const int j = kb * WAVE + __ffsll((unsigned long long)m) - 1;
               m &= m - 1;
               const int k = next_bit_above<NW>(re, j, ncol);
-              const int b = run_end(MT + j * W, W, a);
+              const int b = run_end(MO + j * W, W, a);
               const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
                                   ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
                                   ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
