@@ -262,3 +262,28 @@ def test_fused_empty_source_and_unfusable_ops(gpu_engine):
     prog.close()
     eng.close()
 
+
+
+@pytest.mark.timeout(300)
+def test_fused_op1_ring_full_c1_layer_against_oracle(gpu_engine):
+    """The OP1 ring on the full C1 layer (200k boxes, ~30 % overlapping) as one
+    fused program group: every output equals the oracle's canonical result and
+    both ring checks hold."""
+    eng = V.Engine(0)
+    a = V.synth_layer(V.SYN_UNIFORM, 0x5EED0001, 0, 200_000)
+    L = V.Layer(eng, a)
+    t = [V.Layer(eng) for _ in range(6)]
+    prog = V.Program(eng)
+    op1_record(prog, L, t)
+    prog.run()
+    assert eng.get("fused_groups") == 1 and prog.assert_failures() == 0
+    L1, L2, outer, inner, and1, ring = (x.boxes() for x in t)
+    e1 = O.grow(a, 1, 1, 1, 1)
+    e2 = O.size(a, -1)
+    assert np.array_equal(O.sort_boxes(L1), e1)
+    assert np.array_equal(O.sort_boxes(L2), e2)
+    assert np.array_equal(O.sort_boxes(outer), O.op_not(e1, a))
+    assert np.array_equal(O.sort_boxes(inner), O.op_not(a, e2))
+    assert len(and1) == 0 and len(ring) == 0
+    prog.close()
+    eng.close()

@@ -846,11 +846,15 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
 }
 
 // ---- reorder into tile order -------------------------------------------------------
-constexpr int REORDER_TILES = 16;   // tiles per 256-thread workgroup (4 per wave)
+constexpr int REORDER_TILES = 256;   // tiles per 256-thread workgroup (64 per wave)
 
-// One wave per tile at a time: the wave walks the tile's range in chunks of
-// 64 strips; a ballot over "final" splits each chunk between the final and
-// the seam stream, preserving order within each.
+// A wave takes 64 consecutive tiles and copies their strips, concatenated in
+// tile order, in chunks of 64 (one strip per lane), so small tiles do not
+// leave most lanes idle.  A strip's tile is found by binary search over the
+// tiles' starts in the concatenation; its rank among its tile's final strips
+// comes from a ballot over the chunk plus the tile's finals in earlier chunks
+// (only the tile crossing a chunk border carries any).  Final strips go to
+// the output and seam strips to the side stream, each in tile order.
 __global__ __launch_bounds__(256) void k_reorder(int ntiles, const uint64_t* __restrict__ tab_off,
                                                  const uint32_t* __restrict__ tab_n, int ntab,
                                                  const uint64_t* __restrict__ dstF, const uint64_t* __restrict__ dstS,
@@ -858,33 +862,64 @@ __global__ __launch_bounds__(256) void k_reorder(int ntiles, const uint64_t* __r
                                                  vl_box* __restrict__ out, uint64_t out_base,
                                                  vl_box* __restrict__ side, uint8_t* __restrict__ side_tag,
                                                  uint64_t side_base) {
+  __shared__ uint32_t s_seg[4][WAVE];
+  __shared__ uint64_t s_src[4][WAVE], s_dF[4][WAVE], s_dS[4][WAVE];
   const int lane = __lane_id();
-  const uint64_t lt = lanemask_lt();
   const int wave = threadIdx.x / WAVE;
-  const int t0 = blockIdx.x * REORDER_TILES + wave * (REORDER_TILES / 4);
-  for (int t = t0; t < t0 + REORDER_TILES / 4 && t < ntiles; t++) {
-    const uint32_t T = tab_n[t] + tab_n[ntab + t];
-    if (T == 0) continue;
-    const uint64_t src = tab_off[t];
-    uint64_t dF = out_base + dstF[t], dS = side_base + dstS[t];
-    for (uint32_t c0 = 0; c0 < T; c0 += WAVE) {
-      const uint32_t i = c0 + lane;
-      const bool valid = i < T;
-      const uint8_t tg = valid ? scrTag[src + i] : 0;
-      vl_box b;
-      if (valid) b = scr[src + i];
-      const bool isF = valid && tg == 0;
-      const uint64_t mF = __ballot(isF), mS = __ballot(valid && !isF);
+  const uint64_t lt = lanemask_lt();
+  const int t = (blockIdx.x * 4 + wave) * WAVE + lane;
+  uint32_t T = 0;
+  uint64_t src = 0, dF = 0, dS = 0;
+  if (t < ntiles) {
+    T = tab_n[t] + tab_n[ntab + t];
+    if (T) {
+      src = tab_off[t];
+      dF = out_base + dstF[t];
+      dS = side_base + dstS[t];
+    }
+  }
+  uint32_t total;
+  s_seg[wave][lane] = (uint32_t)wave_excl_u32(T, &total);
+  s_src[wave][lane] = src;
+  s_dF[wave][lane] = dF;
+  s_dS[wave][lane] = dS;
+  __syncthreads();
+  uint32_t carry_tile = ~0u, carry_f = 0;   // the tile crossing into this chunk and its earlier finals
+  for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
+    const uint32_t f = c0 + lane;
+    const bool valid = f < total;
+    int lo = 0, hi = WAVE;   // tile of strip f: the last tile starting at or before f
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_seg[wave][mid] <= f) lo = mid + 1; else hi = mid;
+    }
+    const int own = lo - 1;
+    const uint32_t sg = s_seg[wave][own];
+    uint8_t tg = 0;
+    vl_box b;
+    if (valid) {
+      const uint64_t sp = s_src[wave][own] + (f - sg);
+      tg = scrTag[sp];
+      b = scr[sp];
+    }
+    const bool isF = valid && tg == 0;
+    const uint64_t mF = __ballot(isF);
+    const int first = sg > c0 ? (int)(sg - c0) : 0;   // first lane of this tile in the chunk
+    if (valid) {
+      const uint32_t rf = (uint32_t)__popcll(mF & lt & (~0ull << first)) + ((uint32_t)own == carry_tile ? carry_f : 0u);
       if (isF) {
-        out[dF + __popcll(mF & lt)] = b;
-      } else if (valid) {
-        const uint64_t p = dS + __popcll(mS & lt);
+        out[s_dF[wave][own] + rf] = b;
+      } else {
+        const uint64_t p = s_dS[wave][own] + ((f - sg) - rf);
         side[p] = b;
         side_tag[p] = tg;
       }
-      dF += __popcll(mF);
-      dS += __popcll(mS);
     }
+    const int own63 = __builtin_amdgcn_readlane(own, WAVE - 1);
+    const int first63 = __builtin_amdgcn_readlane(first, WAVE - 1);
+    const uint32_t fin63 = (uint32_t)__popcll(mF & (~0ull << first63));
+    carry_f = ((uint32_t)own63 == carry_tile ? carry_f : 0u) + fin63;
+    carry_tile = (uint32_t)own63;
   }
 }
 
