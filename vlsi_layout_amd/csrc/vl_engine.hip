@@ -1047,8 +1047,11 @@ int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer*
   if (dom.x1 - h < INT32_MIN || dom.y1 - h < INT32_MIN || dom.x2 + h > INT32_MAX || dom.y2 + h > INT32_MAX)
     return FUSE_DECLINED;
   const Grid g = make_grid(dom, S->n, e->fuse_target);
-  // windows must stay small against the tiles (each box meets a few windows)
+  // windows must stay small against the tiles (each box meets a few windows),
+  // and below 2^24 - 1 DBU wide and tall (the kernel sorts coordinates relative
+  // to the window corner in 24 bits)
   if (g.tw < 4 * h || g.th < 4 * h || g.tw < 4 * de || g.th < 4 * de) return FUSE_DECLINED;
+  if (std::max(g.tw, g.th) + 2 * h >= (int64_t)(1 << 24) - 1) return FUSE_DECLINED;
   const double t_start = g_debug ? now_ms() : 0.0;
   std::vector<std::unique_ptr<Outs>> outs;
   Outs* po[FUSE_MAX_OUT] = {};

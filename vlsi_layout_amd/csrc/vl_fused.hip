@@ -156,11 +156,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       const int32_t wy0 = clamp64((int64_t)d.y0 - R, INT32_MIN, INT32_MAX);
       const int32_t wy1 = clamp64((int64_t)d.y1 + R, INT32_MIN, INT32_MAX);
       int32_t vx[K], vy[K];
-      int ex[K], ey[K];   // slot of origin, carried through the sort
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int e = lane * K + k;
-        ex[k] = ey[k] = e;
         const int i = e / CPB, comp = e - CPB * (e / CPB);
         int32_t cx = INT32_MAX, cy = INT32_MAX;
         if (i < n) {
@@ -193,8 +191,20 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         vy[k] = cy;
       }
       const int cnt = CPB * n + CPB;
-      // bitonic sort of P elements, element e = lane*K + k, each with its slot
-      // of origin (of two equal values each partner keeps its own)
+      // Sort keys: the coordinate relative to the window corner (below 2^24:
+      // the host only fuses grids whose windows are narrower) above the slot
+      // number, so a slot travels with its value through the sort; padding
+      // slots get the largest key.
+      constexpr int SB = P == 64 ? 6 : P == 128 ? 7 : 8;   // slot bits
+      uint32_t kx[K], ky[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int e = lane * K + k;
+        const bool real = e / CPB <= n;
+        kx[k] = real ? (((uint32_t)vx[k] - (uint32_t)wx0) << SB) | (uint32_t)e : ~0u;
+        ky[k] = real ? (((uint32_t)vy[k] - (uint32_t)wy0) << SB) | (uint32_t)e : ~0u;
+      }
+      // bitonic sort of P keys, element e = lane*K + k
 #pragma unroll
       for (int size = 2; size <= P; size <<= 1) {
 #pragma unroll
@@ -204,15 +214,11 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
 #pragma unroll
             for (int k = 0; k < K; k++) {
               const int e = lane * K + k;
-              const int px = xor_lanes(vx[k], lj), pex = xor_lanes(ex[k], lj);
-              const int py = xor_lanes(vy[k], lj), pey = xor_lanes(ey[k], lj);
+              const uint32_t px = (uint32_t)xor_lanes((int)kx[k], lj);
+              const uint32_t py = (uint32_t)xor_lanes((int)ky[k], lj);
               const bool take_min = ((e & j) == 0) == ((e & size) == 0);
-              const bool kx = take_min ? vx[k] <= px : vx[k] >= px;
-              const bool ky = take_min ? vy[k] <= py : vy[k] >= py;
-              vx[k] = kx ? vx[k] : px;
-              ex[k] = kx ? ex[k] : pex;
-              vy[k] = ky ? vy[k] : py;
-              ey[k] = ky ? ey[k] : pey;
+              kx[k] = take_min ? min(kx[k], px) : max(kx[k], px);
+              ky[k] = take_min ? min(ky[k], py) : max(ky[k], py);
             }
           } else {
 #pragma unroll
@@ -220,20 +226,22 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
               if ((k & j) == 0) {
                 const int k2 = k | j;
                 const bool asc = ((lane * K + k) & size) == 0;
-                int32_t a = vx[k], b = vx[k2];
-                if ((a > b) == asc) {
-                  vx[k] = b; vx[k2] = a;
-                  const int t = ex[k]; ex[k] = ex[k2]; ex[k2] = t;
-                }
-                a = vy[k]; b = vy[k2];
-                if ((a > b) == asc) {
-                  vy[k] = b; vy[k2] = a;
-                  const int t = ey[k]; ey[k] = ey[k2]; ey[k2] = t;
-                }
+                uint32_t a = kx[k], b = kx[k2];
+                if ((a > b) == asc) { kx[k] = b; kx[k2] = a; }
+                a = ky[k]; b = ky[k2];
+                if ((a > b) == asc) { ky[k] = b; ky[k2] = a; }
               }
             }
           }
         }
+      }
+      int ex[K], ey[K];   // slot of origin of the sorted values
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        vx[k] = (int32_t)((kx[k] >> SB) + (uint32_t)wx0);
+        vy[k] = (int32_t)((ky[k] >> SB) + (uint32_t)wy0);
+        ex[k] = (int)(kx[k] & ((1u << SB) - 1));
+        ey[k] = (int)(ky[k] & ((1u << SB) - 1));
       }
       // drop duplicates
       int nxu, nyu;
