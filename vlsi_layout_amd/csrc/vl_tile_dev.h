@@ -125,6 +125,19 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Value of v in lane l, where l may differ per lane (every lane of the wave
+// must be active).
+__device__ __forceinline__ uint64_t lane_read64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(l << 2, (int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(l << 2, (int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Number of set bits of a wave mask below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // 64x64 bit-matrix transpose across a wave: lane l holds row l (bit c = entry
 // (l, c)) and gets column l.  Six block-swap stages: at stage s, entry (l, c)
 // with bit s of l clear and bit s of c set trades places with (l^s, c^s).
