@@ -14,7 +14,7 @@
  * heal_boxes produces (src/polygon_fracture.f90:805-1004): at every x the
  * covered set is split into maximal y-intervals, and each interval extends in
  * x as long as it stays identical.  Output order is deterministic (tile order,
- * then x, then y inside a tile); vl_canonicalize additionally sorts by
+ * then a fixed order inside a tile); vl_canonicalize additionally sorts by
  * (x1,y1,x2,y2).  All arithmetic is exact integer (int32 coordinates, int64
  * areas).
  *

@@ -495,56 +495,58 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     const bool ok = used + T <= o.region_cap;
     used += T;
     if (lane == 0) o.tab_off[ti] = base;
-    // 4. strips per row.  The S/E masks are transposed in 64x64 bit blocks so
-    //    that lane l holds, for row 64*wb + l, the boundaries at which a strip
-    //    of that row starts (rs) and ends (re).  A strip starting at j ends at
-    //    the first end bit after j: between the two its run is unchanged.
-    //    Output order: row block, lane, start boundary.
+    // 4. strips.  Lane l owns boundary j = 64*kb + l and walks the rows at
+    //    which a strip starts there (bits of Sr).  The end masks are
+    //    transposed in 64x64 bit blocks so that lane l holds, for row
+    //    64*wb + l, the boundaries at which a strip of that row ends; a strip
+    //    starting at (j, a) ends at the first end bit after j in row a, read
+    //    from lane a & 63.  Every lane stays in the loop until the whole wave
+    //    is done, so the cross-lane reads see a full wave.  Output order: row
+    //    block, boundary block, pass, boundary.
     const uint32_t flags = d.flags;
     const int KW = (ncol + 1 + 63) >> 6;   // boundary words per row
-    uint32_t r = 0, my_side = 0;
+    uint32_t my_side = 0;
+    uint64_t pos = base;
 #pragma unroll
     for (int wb = 0; wb < NW; wb++) {
       if (wb >= W) break;
-      uint64_t rs[NW], re[NW];
-      uint32_t c = 0;
+      uint64_t re[NW];
+#pragma unroll
+      for (int kb = 0; kb < NW; kb++) re[kb] = kb < KW ? transpose64(Er[kb][wb], lane) : 0ull;
 #pragma unroll
       for (int kb = 0; kb < NW; kb++) {
-        rs[kb] = 0ull;
-        re[kb] = 0ull;
-        if (kb < KW) {
-          rs[kb] = transpose64(Sr[kb][wb], lane);
-          re[kb] = transpose64(Er[kb][wb], lane);
-          c += (uint32_t)__popcll(rs[kb]);
-        }
-      }
-      uint32_t tot;
-      const uint32_t ex = (uint32_t)wave_excl_u32(c, &tot);
-      uint64_t pos = base + r + ex;
-      const int a = wb * WAVE + lane;
-#pragma unroll
-      for (int kb = 0; kb < NW; kb++) {
-        uint64_t m = rs[kb];
-        while (m) {
-          const int j = kb * WAVE + __ffsll((unsigned long long)m) - 1;
+        if (kb >= KW) break;
+        uint64_t m = Sr[kb][wb];
+        const int j = kb * WAVE + lane;
+        for (;;) {
+          const uint64_t act = __ballot(m != 0ull);
+          if (!act) break;
+          const bool has = m != 0ull;
+          const int t = has ? __ffsll((unsigned long long)m) - 1 : 0;
           m &= m - 1;
-          const int k = next_bit_above<NW>(re, j, ncol);
-          const int b = run_end(MA + j * W, W, a);
-          const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                              ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                              ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                              ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
-          if (ok) {
-            vl_box st;
-            st.x1 = ux[j]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[b];
-            o.strips[pos] = st;
-            o.tag[pos] = (uint8_t)tg;
+          uint64_t rr[NW];
+#pragma unroll
+          for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], t);
+          if (has) {
+            const int a = wb * WAVE + t;
+            const int k = next_bit_above<NW>(rr, j, ncol);
+            const int b = run_end(MA + j * W, W, a);
+            const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                                ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                                ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                                ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+            const uint64_t p = pos + lanes_below(act);
+            if (ok) {
+              vl_box st;
+              st.x1 = ux[j]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[b];
+              o.strips[p] = st;
+              o.tag[p] = (uint8_t)tg;
+            }
+            my_side += tg != 0u;
           }
-          pos++;
-          my_side += tg != 0u;
+          pos += (uint64_t)__popcll(act);
         }
       }
-      r += tot;
     }
 
     const uint32_t nside = (uint32_t)wave_sum((long long)my_side);
