@@ -14,6 +14,29 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Device memory reached through a pointer the compiler cannot trace to a
+// kernel argument (one read back from LDS, or a member of a by-value struct
+// in some kernels) is accessed with flat instructions.  Those count against
+// the LDS wait counter as well, so every later LDS wait also waits for the
+// store to reach memory.  These stores name the global address space.
+#define VL_GLOBAL __attribute__((address_space(1)))
+typedef int32_t vl_v4i32 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+  *(VL_GLOBAL T*)p = v;
+}
+
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const VL_GLOBAL T*)p;
+}
+
+// A 16-byte box record (x1, y1, x2, y2) at p.
+__device__ __forceinline__ void gstore_box(void* p, int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
+  *(VL_GLOBAL vl_v4i32*)p = vl_v4i32{x1, y1, x2, y2};
+}
+
 // Lane exchange v(lane ^ M) without the LDS crossbar where the hardware has a
 // cheaper path: DPP quad permutes for 1 and 2, the DPP 16-lane row rotate by
 // 8 for 8, ds_swizzle in bit mode (no LDS access) for 4 and 16, and

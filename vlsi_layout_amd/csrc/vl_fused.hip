@@ -31,13 +31,16 @@ namespace {
 
 constexpr int FUSE_OVF_BUF = 64;   // overflow tiles buffered in LDS per workgroup
 
-// Value of truth table tt (bit s | m0 << 1 | m1 << 2) on base words s, m0, m1.
+// Value of truth table tt (bit s | m0 << 1 | m1 << 2) on base words s, m0, m1
+// (bit 0, the value outside every mask, is taken as 0).  Shannon expansion on
+// m1, m0 and s with the table's bits as all-ones / all-zero words: seven
+// bit-selects, no branches.
 __device__ __forceinline__ uint64_t tt_eval(uint32_t tt, uint64_t s, uint64_t m0, uint64_t m1) {
-  uint64_t r = 0;
-#pragma unroll
-  for (int k = 1; k < 8; k++)
-    if ((tt >> k) & 1u) r |= ((k & 1) ? s : ~s) & ((k & 2) ? m0 : ~m0) & ((k & 4) ? m1 : ~m1);
-  return r;
+  const auto c = [tt](int k) { return ((tt >> k) & 1u) ? ~0ull : 0ull; };
+  const auto sel = [](uint64_t m, uint64_t a, uint64_t b) { return (m & a) | (~m & b); };
+  const uint64_t f00 = s & c(1), f01 = sel(s, c(3), c(2));
+  const uint64_t f10 = sel(s, c(5), c(4)), f11 = sel(s, c(7), c(6));
+  return sel(m1, sel(m0, f11, f10), sel(m0, f01, f00));
 }
 
 template <int P>
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         if (lane == 0) {
           if (n > NMAX) sovf[novf] = ti;
           for (int k = 0; k < nout; k++) {
-            s_tn[k][ti] = 0;
-            s_tn[k][o.ntab + ti] = 0;
+            gstore(s_tn[k] + ti, 0u);
+            gstore(s_tn[k] + o.ntab + ti, 0u);
           }
         }
         if (n > NMAX && ++novf == FUSE_OVF_BUF) {
@@ -352,8 +355,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           MO = MTQ[min(q, NQ - 1)];
           if (!((anymask >> q) & 1u)) {
             if (lane == 0) {
-              tn[ti] = 0;
-              tn[o.ntab + ti] = 0;
+              gstore(tn + ti, 0u);
+              gstore(tn + o.ntab + ti, 0u);
             }
             continue;
           }
@@ -375,8 +378,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           }
           if (__ballot(any != 0ull) == 0ull) {
             if (lane == 0) {
-              tn[ti] = 0;
-              tn[o.ntab + ti] = 0;
+              gstore(tn + ti, 0u);
+              gstore(tn + o.ntab + ti, 0u);
             }
             continue;
           }
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         const bool ok = used + T <= cap;
         vl_box* const strips = s_strips[q];
         uint8_t* const tags = s_tag[q];
-        if (lane == 0) s_toff[q][ti] = base;
+        if (lane == 0) gstore(s_toff[q] + ti, base);
         // strips: lane owns boundary j = 64*kb + lane and walks its start
         // rows; a strip starting at (j, a) ends at the first end bit after j
         // in row a, read from the row-major (transposed) end masks held by
@@ -447,10 +450,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
                                     ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
                 const uint64_t p = pos + lanes_below(act);
                 if (ok) {
-                  vl_box st;
-                  st.x1 = ux[j]; st.y1 = uy[a]; st.x2 = ux[k]; st.y2 = uy[b];
-                  strips[p] = st;
-                  tags[p] = (uint8_t)tg;
+                  gstore_box(strips + p, ux[j], uy[a], ux[k], uy[b]);
+                  gstore(tags + p, (uint8_t)tg);
                 }
                 my_side += tg != 0u;
               }
@@ -461,8 +462,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         const uint32_t nside = (uint32_t)wave_sum((long long)my_side);
         __syncthreads();
         if (lane == 0) {
-          tn[ti] = T - nside;
-          tn[o.ntab + ti] = nside;
+          gstore(tn + ti, T - nside);
+          gstore(tn + o.ntab + ti, nside);
           s_used[q] = used + T;
           s_side[q] += nside;
         }

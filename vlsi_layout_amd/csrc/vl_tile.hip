@@ -600,7 +600,7 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
   uint32_t novf = 0;
   long long area_acc = 0;
   for (int tk = blockIdx.x; tk < nt; tk += gridDim.x) {
-    const uint32_t ti = o.list ? o.list[tk] : (uint32_t)tk;
+    const uint32_t ti = o.list ? gload(o.list + tk) : (uint32_t)tk;
     const TileDesc d = desc[ti];
     const int nA = d.nA;
     const int n = d.nA + d.nB;
@@ -610,8 +610,8 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
       if (tid == 0) {
         if (n > NMAX) sovf[novf] = ti;
         if (!AREA) {
-          o.tab_n[ti] = 0;
-          o.tab_n[o.ntab + ti] = 0;
+          gstore(o.tab_n + ti, 0u);
+          gstore(o.tab_n + o.ntab + ti, 0u);
         }
       }
       __syncthreads();
@@ -796,9 +796,9 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
     used += F + S;
     side_total += S;
     if (tid == 0) {
-      o.tab_off[ti] = b0;
-      o.tab_n[ti] = (uint32_t)F;
-      o.tab_n[o.ntab + ti] = (uint32_t)S;
+      gstore(o.tab_off + ti, b0);
+      gstore(o.tab_n + ti, (uint32_t)F);
+      gstore(o.tab_n + o.ntab + ti, (uint32_t)S);
     }
 
     if ((F + S) > 0) {
@@ -820,13 +820,17 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
                                ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
                                ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
                                ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
-          vl_box s;
-          s.x1 = ux[j]; s.y1 = uy[a]; s.x2 = ux[k]; s.y2 = uy[b];
           if (tag) {
-            if (okS) { o.strips[si] = s; o.tag[si] = (uint8_t)tag; }
+            if (okS) {
+              gstore_box(o.strips + si, ux[j], uy[a], ux[k], uy[b]);
+              gstore(o.tag + si, (uint8_t)tag);
+            }
             si++;
           } else {
-            if (okF) { o.strips[fi] = s; o.tag[fi] = 0; }
+            if (okF) {
+              gstore_box(o.strips + fi, ux[j], uy[a], ux[k], uy[b]);
+              gstore(o.tag + fi, (uint8_t)0);
+            }
             fi++;
           }
         });
