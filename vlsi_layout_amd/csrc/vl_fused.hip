@@ -55,7 +55,10 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   static_assert(NMAX == (P == 64 ? FUSE_NMAX_TINY : P == 128 ? FUSE_NMAX_SMALL : FUSE_NMAX_LARGE), "tier capacities");
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint8_t sidx_x[P], sidx_y[P]; // column / row index of every coordinate slot
-  __shared__ uint16_t eup[P], edn[P];      // row maps of the y erosion
+  // row maps of the y erosion; during output extraction the strip-start table
+  __shared__ uint16_t erow[2 * P];
+  uint16_t* const eup = erow;
+  uint16_t* const edn = erow + P;
   __shared__ uint64_t MS[NCOL * NW];       // S
   __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
   __shared__ uint64_t MT[NCOL * NW];       // erosion temporary, then the current output
@@ -412,12 +415,17 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         vl_box* const strips = s_strips[q];
         uint8_t* const tags = s_tag[q];
         if (lane == 0) gstore(s_toff[q] + ti, base);
-        // strips: lane owns boundary j = 64*kb + lane and walks its start
-        // rows; a strip starting at (j, a) ends at the first end bit after j
-        // in row a, read from the row-major (transposed) end masks held by
-        // lane a & 63.  Every lane stays in the loop until the whole wave is
-        // done, so the cross-lane reads see a full wave, and the strips of one
-        // pass are written contiguously.
+        // strips.  Lane l owns boundary j = 64*kb + l; its start bits are the
+        // rows at which a strip starts there.  In rounds, every lane appends
+        // up to Q of its starts (j, a) to a table in LDS, then the whole wave
+        // takes the table's entries one per lane, so a round costs
+        // ceil(entries / 64) passes however the starts are spread over the
+        // boundaries.  A strip starting at (j, a) ends at the first end bit
+        // after j in row a, read from the row-major (transposed) end masks
+        // held by lane a & 63, and reaches up to the run end of column j.
+        // Strips are written in table order.
+        constexpr int Q = 2 * P / WAVE;   // table entries per lane and round
+        uint16_t* const stab = erow;      // 2P entries: (j << 8) | a
         uint32_t my_side = 0;
         uint64_t pos = base;
 #pragma unroll
@@ -430,32 +438,46 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           for (int kb = 0; kb < NW; kb++) {
             if (kb >= KW) break;
             uint64_t m = Sr[kb][wb];
-            const int j = kb * WAVE + lane;
-            for (;;) {
-              const uint64_t act = __ballot(m != 0ull);
-              if (!act) break;
-              const bool has = m != 0ull;
-              const int t = has ? __ffsll((unsigned long long)m) - 1 : 0;
-              m &= m - 1;
-              uint64_t rr[NW];
+            const uint32_t jl = (uint32_t)(kb * WAVE + lane);
+            while (__ballot(m != 0ull)) {
+              uint32_t nent = 0;
 #pragma unroll
-              for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], t);
-              if (has) {
-                const int a = wb * WAVE + t;
-                const int k = next_bit_above<NW>(rr, j, ncol);
-                const int b = run_end(MO + j * W, W, a);
-                const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                                    ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                                    ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                                    ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
-                const uint64_t p = pos + lanes_below(act);
-                if (ok) {
-                  gstore_box(strips + p, ux[j], uy[a], ux[k], uy[b]);
-                  gstore(tags + p, (uint8_t)tg);
+              for (int r = 0; r < Q; r++) {
+                const uint64_t act = __ballot(m != 0ull);
+                if (!act) break;
+                if (m) {
+                  const uint32_t t = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                  m &= m - 1;
+                  stab[nent + lanes_below(act)] = (uint16_t)((jl << 8) | ((uint32_t)wb * WAVE + t));
                 }
-                my_side += tg != 0u;
+                nent += (uint32_t)__popcll(act);
               }
-              pos += (uint64_t)__popcll(act);
+              __syncthreads();
+              for (uint32_t g0 = 0; g0 < nent; g0 += WAVE) {
+                const uint32_t g = g0 + (uint32_t)lane;
+                const bool has = g < nent;
+                const uint32_t ent = has ? (uint32_t)stab[g] : 0u;
+                const int j = (int)(ent >> 8), a = (int)(ent & 0xffu);
+                uint64_t rr[NW];
+#pragma unroll
+                for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], a & (WAVE - 1));
+                if (has) {
+                  const int k = next_bit_above<NW>(rr, j, ncol);
+                  const int b = run_end_w<NW>(MO + j * W, W, a);
+                  const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                                      ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                                      ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                                      ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+                  const uint64_t p = pos + g;
+                  if (ok) {
+                    gstore_box(strips + p, ux[j], uy[a], ux[k], uy[b]);
+                    gstore(tags + p, (uint8_t)tg);
+                  }
+                  my_side += tg != 0u;
+                }
+              }
+              pos += nent;
+              __syncthreads();
             }
           }
         }

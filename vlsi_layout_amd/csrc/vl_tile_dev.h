@@ -118,6 +118,18 @@ __device__ __forceinline__ int run_end(const uint64_t* col, int W, int a) {
   return w * 64 + __ffsll((unsigned long long)m) - 1;
 }
 
+// The same for grids of at most NW words per column (one word: no loop; the
+// bits above the grid's rows are clear, so a zero bit is always found).
+template <int NW>
+__device__ __forceinline__ int run_end_w(const uint64_t* col, int W, int a) {
+  if constexpr (NW == 1) {
+    const uint64_t m = ~col[0] & (~0ull << (a & 63));
+    return m ? __ffsll((unsigned long long)m) - 1 : 64;
+  } else {
+    return run_end(col, W, a);
+  }
+}
+
 // Value of v in lane l (l uniform across the wave).
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -141,14 +153,25 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 // 64x64 bit-matrix transpose across a wave: lane l holds row l (bit c = entry
 // (l, c)) and gets column l.  Six block-swap stages: at stage s, entry (l, c)
 // with bit s of l clear and bit s of c set trades places with (l^s, c^s).
+// The lane-dependent choices are bit selects with an all-ones/all-zeros
+// lane mask, not branches: the lower lane of a pair keeps its M bits and takes
+// the partner's bits shifted up, the upper lane keeps its ~M bits and takes
+// the partner's bits shifted down.
 __device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
   constexpr uint64_t M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
                              0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+  const auto bsel = [](uint64_t m, uint64_t a, uint64_t b) { return (a & m) | (b & ~m); };
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     const int sft = 32 >> i;
+    // all ones in the upper lane of the pair (bit 5-i of the lane); computed
+    // here rather than hoisted, which would hold 12 lane masks in registers
+    uint32_t s32;
+    asm volatile("v_bfe_i32 %0, %1, %2, 1" : "=v"(s32) : "v"(lane), "i"(5 - i));
+    const uint64_t up = ((uint64_t)s32 << 32) | s32;
     const uint64_t y = xor_lanes64(x, sft);
-    x = (lane & sft) ? ((x & ~M[i]) | ((y >> sft) & M[i])) : ((x & M[i]) | ((y << sft) & ~M[i]));
+    const uint64_t ys = bsel(up, y >> sft, y << sft);
+    x = bsel(M[i] ^ up, x, ys);
   }
   return x;
 }
@@ -223,20 +246,23 @@ __device__ __forceinline__ int count_at_most(const int32_t* v, int n, int64_t x)
   return lo;
 }
 
-// count_below(v, n, x) when the answer is known to lie near index h: a few
-// steps from h (each one dependent LDS read), then a binary search if the
-// walk has not settled.  Erosion lookups land within a step or two of the
-// grid line they start from.
+// count_below(v, n, x) (n >= 1) when the answer is known to lie near index
+// h: two branch-free steps from h (each reads v[h-1] and v[h] and moves h by
+// at most one), then a binary search for the lanes whose walk has not
+// settled.  Erosion lookups land within a step or two of the grid line they
+// start from.
 __device__ __forceinline__ int count_below_near(const int32_t* v, int n, int64_t x, int h) {
   h = h < 0 ? 0 : (h > n ? n : h);
 #pragma unroll
-  for (int s = 0; s < 3; s++) {
-    if (h < n && (int64_t)v[h] < x) h++;
-    else if (h > 0 && (int64_t)v[h - 1] >= x) h--;
-    else return h;
+  for (int s = 0; s < 2; s++) {
+    const int64_t lo = v[max(h - 1, 0)], hi = v[min(h, n - 1)];
+    const bool up = h < n && hi < x;     // v sorted: up and dn exclude each other
+    const bool dn = h > 0 && lo >= x;
+    h += (int)up - (int)dn;
   }
-  if ((h == n || (int64_t)v[h] >= x) && (h == 0 || (int64_t)v[h - 1] < x)) return h;
-  return count_below(v, n, x);
+  const int64_t lo = v[max(h - 1, 0)], hi = v[min(h, n - 1)];
+  const bool settled = (h == n || hi >= x) && (h == 0 || lo < x);
+  return settled ? h : count_below(v, n, x);
 }
 
 __device__ __forceinline__ int32_t clamp64(int64_t v, int64_t lo, int64_t hi) {
@@ -259,12 +285,16 @@ __device__ __forceinline__ void erode_x(const uint64_t* M, uint64_t* out, const 
     }
 #pragma unroll
     for (int w = 0; w < NW; w++) acc[w] = keep ? ~0ull : 0ull;
-    if (keep)
-      for (int j = lo; j <= hi; j++) {
+    // the wave runs its longest window; a lane past its own window re-reads
+    // its last column (AND is idempotent), a lane keeping nothing reads
+    // column 0 into a zero accumulator
+    const int len = keep ? hi - lo + 1 : 0;
+    for (int s = 0; __ballot(s < len); s++) {
+      const int j = keep ? min(lo + s, hi) : 0;
 #pragma unroll
-        for (int w = 0; w < NW; w++)
-          if (w < W) acc[w] &= M[j * W + w];
-      }
+      for (int w = 0; w < NW; w++)
+        if (w < W) acc[w] &= M[j * W + w];
+    }
 #pragma unroll
     for (int w = 0; w < NW; w++)
       if (w < W) out[c * W + w] = acc[w];
@@ -289,6 +319,25 @@ __device__ __forceinline__ void erosion_row_maps(const int32_t* uy, int nyu, int
 template <int NW>
 __device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const uint16_t* up, const uint16_t* dn,
                                         int ncol, int W, int t, int nt) {
+  if constexpr (NW == 1) {
+    // one word per column (bit 63 clear): the k-th run start pairs with the
+    // k-th run end; the wave runs its largest run count, lanes predicated
+    for (int c = t; c < ncol; c += nt) {
+      const uint64_t v = M[c];
+      uint64_t st = v & ~(v << 1), en = ~v & (v << 1), res = 0ull;
+      while (__ballot(st != 0ull)) {
+        const bool has = st != 0ull;
+        const int a = has ? __ffsll((unsigned long long)st) - 1 : 0;
+        const int b = has ? __ffsll((unsigned long long)en) - 1 : 0;
+        st &= st - 1;
+        en &= en - 1;
+        const int ia = up[a], ib = dn[b];
+        if (has && ia < ib) res |= bits_range(ia, ib - 1);
+      }
+      out[c] = res;
+    }
+    return;
+  }
   for (int c = t; c < ncol; c += nt) {
     uint64_t res[NW];
 #pragma unroll
