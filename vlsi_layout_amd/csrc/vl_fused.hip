@@ -278,35 +278,97 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         nyu = (int)ty;
       }
       const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 63) >> 6;
-      const int words = ncol * W;
-      for (int k = lane; k < words; k += WAVE) {
-        MS[k] = 0ull;
-        if (g0) MM[0][k] = 0ull;
-        if (g1) MM[1][k] = 0ull;
-      }
-      __syncthreads();
-      // 2. rasterise S and the grown images (box-parallel)
-      for (int i = lane; i < n; i += WAVE) {
+      if constexpr (NW <= 2) {
+        // 2. rasterise S and the grown images, column-parallel: lane l ORs,
+        //    into its columns 64*cb + l, the row words of every box image
+        //    whose column range holds the column.  The edges of a box image
+        //    are read from the lanes of its two slots (slot e is component
+        //    e % K on lane e / K), so its row words are uniform values.
+        __syncthreads();
+        uint32_t own[K];
 #pragma unroll
-        for (int m = -1; m < 2; m++) {
-          if (m == 0 && !g0) continue;
-          if (m == 1 && !g1) continue;
-          // the box's edges (S: slots 0, 1; grown image m: slots 2+2m, 3+2m)
-          const int e0 = CPB * i + (m < 0 ? 0 : 2 + 2 * m);
-          const int cx1 = sidx_x[e0], cx2 = sidx_x[e0 + 1];
-          const int cy1 = sidx_y[e0], cy2 = sidx_y[e0 + 1];
-          if (cx1 >= cx2 || cy1 >= cy2) continue;
-          uint64_t* M = m < 0 ? MS : MM[m];
-          const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;
-          for (int c = cx1; c < cx2; c++)
-            for (int w = w1; w <= w2; w++) {
-              const int lo = (w == w1) ? (cy1 & 63) : 0;
-              const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
-              atomicOr((unsigned long long*)&M[c * W + w], (unsigned long long)bits_range(lo, hi));
+        for (int k = 0; k < K; k++) own[k] = (uint32_t)sidx_x[lane * K + k] | ((uint32_t)sidx_y[lane * K + k] << 8);
+        const auto slot = [&](int e) {
+          uint32_t v = own[0];
+#pragma unroll
+          for (int k = 1; k < K; k++)
+            if (e % K == k) v = own[k];
+          return (uint32_t)__builtin_amdgcn_readlane((int)v, e / K);
+        };
+        uint64_t acc[3][NW][NW];
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+          for (int cb = 0; cb < NW; cb++)
+#pragma unroll
+            for (int w = 0; w < NW; w++) acc[m][cb][w] = 0ull;
+        for (int i = 0; i < n; i++) {
+#pragma unroll
+          for (int m = -1; m < 2; m++) {
+            if (m == 0 && !g0) continue;
+            if (m == 1 && !g1) continue;
+            const int e0 = CPB * i + (m < 0 ? 0 : 2 + 2 * m);
+            const uint32_t lo = slot(e0), hi = slot(e0 + 1);
+            const int cx1 = (int)(lo & 0xffu), cy1 = (int)(lo >> 8), cx2 = (int)(hi & 0xffu), cy2 = (int)(hi >> 8);
+            if (cx1 >= cx2 || cy1 >= cy2) continue;
+#pragma unroll
+            for (int cb = 0; cb < NW; cb++) {
+              const int c = cb * WAVE + lane;
+              const bool in = c >= cx1 && c < cx2;
+#pragma unroll
+              for (int w = 0; w < NW; w++) {
+                const uint64_t rows = NW == 1 ? bits_range(cy1, cy2 - 1) : row_bits(w, cy1, cy2);
+                if (in) acc[m + 1][cb][w] |= rows;
+              }
             }
+          }
         }
+#pragma unroll
+        for (int cb = 0; cb < NW; cb++) {
+          const int c = cb * WAVE + lane;
+          if (c < ncol) {
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+              if (w >= W) break;
+              MS[c * W + w] = acc[0][cb][w];
+              if (g0) MM[0][c * W + w] = acc[1][cb][w];
+              if (g1) MM[1][c * W + w] = acc[2][cb][w];
+            }
+          }
+        }
+        __syncthreads();
+      } else {
+        // (256-slot grids) box-parallel raster with LDS atomics
+        const int words = ncol * W;
+        for (int k = lane; k < words; k += WAVE) {
+          MS[k] = 0ull;
+          if (g0) MM[0][k] = 0ull;
+          if (g1) MM[1][k] = 0ull;
+        }
+        __syncthreads();
+        // 2. rasterise S and the grown images (box-parallel)
+        for (int i = lane; i < n; i += WAVE) {
+#pragma unroll
+          for (int m = -1; m < 2; m++) {
+            if (m == 0 && !g0) continue;
+            if (m == 1 && !g1) continue;
+            // the box's edges (S: slots 0, 1; grown image m: slots 2+2m, 3+2m)
+            const int e0 = CPB * i + (m < 0 ? 0 : 2 + 2 * m);
+            const int cx1 = sidx_x[e0], cx2 = sidx_x[e0 + 1];
+            const int cy1 = sidx_y[e0], cy2 = sidx_y[e0 + 1];
+            if (cx1 >= cx2 || cy1 >= cy2) continue;
+            uint64_t* M = m < 0 ? MS : MM[m];
+            const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;
+            for (int c = cx1; c < cx2; c++)
+              for (int w = w1; w <= w2; w++) {
+                const int lo = (w == w1) ? (cy1 & 63) : 0;
+                const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
+                atomicOr((unsigned long long*)&M[c * W + w], (unsigned long long)bits_range(lo, hi));
+              }
+          }
+        }
+        __syncthreads();
       }
-      __syncthreads();
       // the tile's own columns [colT0, colT1) and rows [rowT0, rowT1)
       const int colT0 = count_below(ux, nxu, d.x0), colT1 = count_below(ux, nxu, d.x1);
       const int rowT0 = count_below(uy, nyu, d.y0), rowT1 = count_below(uy, nyu, d.y1);
