@@ -34,6 +34,14 @@ __device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
   }
 }
 
+// a | (b & c), one gfx950 three-input bitwise instruction per 32-bit half
+// (truth table over src0 = 0xF0, src1 = 0xCC, src2 = 0xAA).
+__device__ __forceinline__ uint64_t or_and(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, 0xF8);
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32), 0xF8);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Bit w of the dilated change mask of columns cur/prv (null = empty column).
 template <int NW>
 __device__ __forceinline__ void change_mask(const uint64_t* cur, const uint64_t* prv, int W, uint64_t* D) {
@@ -66,17 +74,17 @@ __device__ __forceinline__ void starts_meeting(const uint64_t* col, int W, const
     const uint64_t r = col[w];
     uint64_t m = (D[w] & r) | (carry & r);
     uint64_t c = r & (r >> 1);   // bit i: rows i..i+1 in the run; then i..i+2, i..i+4, ...
-    m |= (m >> 1) & c;
+    m = or_and(m, m >> 1, c);
     c &= c >> 1;
-    m |= (m >> 2) & c;
+    m = or_and(m, m >> 2, c);
     c &= c >> 2;
-    m |= (m >> 4) & c;
+    m = or_and(m, m >> 4, c);
     c &= c >> 4;
-    m |= (m >> 8) & c;
+    m = or_and(m, m >> 8, c);
     c &= c >> 8;
-    m |= (m >> 16) & c;
+    m = or_and(m, m >> 16, c);
     c &= c >> 16;
-    m |= (m >> 32) & c;
+    m = or_and(m, m >> 32, c);
     const uint64_t below = (w > 0) ? (col[w - 1] >> 63) : 0ull;
     out[w] = m & r & ~((r << 1) | below);
     carry = (m & 1ull) ? (1ull << 63) : 0ull;
