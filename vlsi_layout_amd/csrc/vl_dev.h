@@ -108,6 +108,21 @@ __device__ __forceinline__ T read_lane(T v, int l) {
   }
 }
 
+// v, the same in every active lane, as a scalar (wave-uniform) value: values
+// read from LDS land in vector registers, and the compiler cannot tell that
+// they are uniform, so arithmetic on them would otherwise stay in the VALU.
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (T)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (T)__builtin_amdgcn_readfirstlane((int)v);
+  }
+}
+
 // Inclusive wave64 scan: DPP shifts within each 16-lane row, then the row
 // totals (lanes 15, 31, 47) are added to the rows above them.
 template <typename T>

@@ -135,7 +135,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
     __syncthreads();
     for (int jb = 0; jb < nbt; jb++) {
       const uint32_t ti = read_lane(my_ti, jb);
-      const TileDesc d = sdesc[jb];
+      TileDesc d;
+#pragma unroll
+      for (int k = 0; k < 10; k++) ((uint32_t*)&d)[k] = uniform(sdw[jb * 10 + k]);
       const vl_box* bx = sbuf + jb * (NMAX + 1);
       const int n = d.nA + d.nB;
       if (n > NMAX || n == 0) {
@@ -370,8 +372,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         __syncthreads();
       }
       // the tile's own columns [colT0, colT1) and rows [rowT0, rowT1)
-      const int colT0 = count_below(ux, nxu, d.x0), colT1 = count_below(ux, nxu, d.x1);
-      const int rowT0 = count_below(uy, nyu, d.y0), rowT1 = count_below(uy, nyu, d.y1);
+      const int colT0 = uniform(count_below(ux, nxu, d.x0)), colT1 = uniform(count_below(ux, nxu, d.x1));
+      const int rowT0 = uniform(count_below(uy, nyu, d.y0)), rowT1 = uniform(count_below(uy, nyu, d.y1));
       // 3. erosion images
       if (e0) {
         erosion_row_maps(uy, nyu, d.y0, d.y1, fs.arg[0][0], eup, edn, lane, WAVE);
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         const uint64_t rb = row_bits(0, rowT0, rowT1);
 #pragma unroll 1
         for (int q = 0; q < nout; q++) {
-          const uint64_t v = in ? tt_eval(s_tt[q], b0, b1, b2) & rb : 0ull;
+          const uint64_t v = in ? tt_eval(uniform(s_tt[q]), b0, b1, b2) & rb : 0ull;
           if (c < ncol) MTQ[min(q, NQ - 1)][c] = v;
           if (__ballot(v != 0ull)) anymask |= 1u << q;
         }
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       }
 #pragma unroll 1
       for (int q = 0; q < nout; q++) {
-        uint32_t* const tn = s_tn[q];
+        uint32_t* const tn = uniform(s_tn[q]);
         const uint64_t* MO = MT;
         if (onepass) {
           MO = MTQ[min(q, NQ - 1)];
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
             continue;
           }
         } else {
-          const uint32_t tt = s_tt[q];
+          const uint32_t tt = uniform(s_tt[q]);
           uint64_t any = 0;
           for (int c = lane; c < ncol; c += WAVE) {
             const bool in = c >= colT0 && c < colT1;
@@ -470,12 +472,12 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           }
         }
         const uint32_t T = wave_sum(myT);
-        const uint64_t cap = s_cap[q];
-        const uint64_t used = s_used[q];
+        const uint64_t cap = uniform(s_cap[q]);
+        const uint64_t used = uniform(s_used[q]);
         const uint64_t base = (uint64_t)(o.region_base + blockIdx.x) * cap + used;
         const bool ok = used + T <= cap;
-        vl_box* const strips = s_strips[q];
-        uint8_t* const tags = s_tag[q];
+        vl_box* const strips = uniform(s_strips[q]);
+        uint8_t* const tags = uniform(s_tag[q]);
         if (lane == 0) gstore(s_toff[q] + ti, base);
         // strips.  Lane l owns boundary j = 64*kb + l; its start bits are the
         // rows at which a strip starts there.  In rounds, every lane appends
