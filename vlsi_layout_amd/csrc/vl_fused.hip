@@ -372,8 +372,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         __syncthreads();
       }
       // the tile's own columns [colT0, colT1) and rows [rowT0, rowT1)
-      const int colT0 = uniform(count_below(ux, nxu, d.x0)), colT1 = uniform(count_below(ux, nxu, d.x1));
-      const int rowT0 = uniform(count_below(uy, nyu, d.y0)), rowT1 = uniform(count_below(uy, nyu, d.y1));
+      const int colT0 = count_below_wave<K>(ux, nxu, d.x0, lane), colT1 = count_below_wave<K>(ux, nxu, d.x1, lane);
+      const int rowT0 = count_below_wave<K>(uy, nyu, d.y0, lane), rowT1 = count_below_wave<K>(uy, nyu, d.y1, lane);
       // 3. erosion images
       if (e0) {
         erosion_row_maps(uy, nyu, d.y0, d.y1, fs.arg[0][0], eup, edn, lane, WAVE);

@@ -420,10 +420,10 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     // the tile's own columns [colT0, colT1) and rows [rowT0, rowT1)
     int colT0 = 0, colT1 = ncol, rowT0 = 0, rowT1 = nrows;
     if (ERO) {
-      colT0 = count_below(ux, nxu, d.x0);
-      colT1 = count_below(ux, nxu, d.x1);
-      rowT0 = count_below(uy, nyu, d.y0);
-      rowT1 = count_below(uy, nyu, d.y1);
+      colT0 = count_below_wave<K>(ux, nxu, d.x0, lane);
+      colT1 = count_below_wave<K>(ux, nxu, d.x1, lane);
+      rowT0 = count_below_wave<K>(uy, nyu, d.y0, lane);
+      rowT1 = count_below_wave<K>(uy, nyu, d.y1, lane);
       erosion_row_maps(uy, nyu, d.y0, d.y1, R, eup, edn, lane, WAVE);
       erode_x<NW>(MA, MX, ux, nxu, ncol, W, colT0, colT1, R, lane, WAVE);
       __syncthreads();

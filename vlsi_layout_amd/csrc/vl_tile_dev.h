@@ -254,6 +254,20 @@ __device__ __forceinline__ int count_at_most(const int32_t* v, int n, int64_t x)
   return lo;
 }
 
+// count_below(v, n, x) for x the same in every lane of the wave and n <= 64K:
+// one ballot per group of 64 entries instead of a binary search, whose
+// dependent LDS reads would run as a divergent loop.
+template <int K>
+__device__ __forceinline__ int count_below_wave(const int32_t* v, int n, int32_t x, int lane) {
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int e = k * WAVE + lane;
+    c += __popcll(__ballot(e < n && v[min(e, n - 1)] < x));
+  }
+  return c;
+}
+
 // count_below(v, n, x) (n >= 1) when the answer is known to lie near index
 // h: two branch-free steps from h (each reads v[h-1] and v[h] and moves h by
 // at most one), then a binary search for the lanes whose walk has not
