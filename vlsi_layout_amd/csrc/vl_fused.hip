@@ -279,7 +279,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         nxu = (int)tx;
         nyu = (int)ty;
       }
-      const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 63) >> 6;
+      // one row above the last is kept clear for the run end marks
+      const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 64) >> 6;
       if constexpr (NW <= 2) {
         // 2. rasterise S and the grown images, column-parallel: lane l ORs,
         //    into its columns 64*cb + l, the row words of every box image
@@ -452,7 +453,10 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           }
           __syncthreads();
         }
-        // start / end masks at every slab boundary (lane owns boundaries 64*kb + lane)
+        // start / end marks at every slab boundary (lane owns boundaries
+        // 64*kb + lane): the runs of the column right of the boundary (S) and
+        // left of it (E) that meet the change mask, each marked at the clear
+        // row just above the run
         uint64_t Sr[NW][NW], Er[NW][NW];
         uint32_t myT = 0;
 #pragma unroll
@@ -465,8 +469,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
             const uint64_t* prv = (k > 0) ? MO + (k - 1) * W : nullptr;
             uint64_t D[NW];
             change_mask<NW>(cur, prv, W, D);
-            starts_meeting<NW>(cur, W, D, Sr[kb]);
-            starts_meeting<NW>(prv, W, D, Er[kb]);
+            ends_meeting<NW>(cur, W, D, Sr[kb]);
+            ends_meeting<NW>(prv, W, D, Er[kb]);
 #pragma unroll
             for (int w = 0; w < NW; w++) myT += (uint32_t)__popcll(Er[kb][w]);
           }
@@ -479,17 +483,17 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         vl_box* const strips = uniform(s_strips[q]);
         uint8_t* const tags = uniform(s_tag[q]);
         if (lane == 0) gstore(s_toff[q] + ti, base);
-        // strips.  Lane l owns boundary j = 64*kb + l; its start bits are the
-        // rows at which a strip starts there.  In rounds, every lane appends
-        // up to Q of its starts (j, a) to a table in LDS, then the whole wave
-        // takes the table's entries one per lane, so a round costs
-        // ceil(entries / 64) passes however the starts are spread over the
-        // boundaries.  A strip starting at (j, a) ends at the first end bit
-        // after j in row a, read from the row-major (transposed) end masks
-        // held by lane a & 63, and reaches up to the run end of column j.
-        // Strips are written in table order.
+        // strips.  Lane l owns boundary j = 64*kb + l; its S marks are the
+        // rows b just above the runs [a, b) of strips starting there.  In
+        // rounds, every lane appends up to Q of its starts (j, b) to a table
+        // in LDS, then the whole wave takes the table's entries one per lane,
+        // so a round costs ceil(entries / 64) passes however the starts are
+        // spread over the boundaries.  A strip starting at (j, b) ends at the
+        // first E mark after j in row b, read from the row-major (transposed)
+        // end marks held by lane b & 63; its run starts one above the highest
+        // clear row of column j below b.  Strips are written in table order.
         constexpr int Q = 2 * P / WAVE;   // table entries per lane and round
-        uint16_t* const stab = erow;      // 2P entries: (j << 8) | a
+        uint16_t* const stab = erow;      // 2P entries: (j << 8) | b
         uint32_t my_side = 0;
         uint64_t pos = base;
 #pragma unroll
@@ -521,13 +525,13 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
                 const uint32_t g = g0 + (uint32_t)lane;
                 const bool has = g < nent;
                 const uint32_t ent = has ? (uint32_t)stab[g] : 0u;
-                const int j = (int)(ent >> 8), a = (int)(ent & 0xffu);
+                const int j = (int)(ent >> 8), b = (int)(ent & 0xffu);
                 uint64_t rr[NW];
 #pragma unroll
-                for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], a & (WAVE - 1));
+                for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], b & (WAVE - 1));
                 if (has) {
                   const int k = next_bit_above<NW>(rr, j, ncol);
-                  const int b = run_end_w<NW>(MO + j * W, W, a);
+                  const int a = run_start_w<NW>(MO + j * W, W, b);
                   const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
                                       ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
                                       ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |

@@ -15,23 +15,24 @@
 //     word by word with the operator (AND, A&~B, OR, XOR);
 //  3. at every slab boundary k, the change mask D = dilate1(R_k ^ R_{k-1})
 //     marks where runs differ: a run of R_k that meets D starts a strip at k
-//     (start mask S_k, bit = first row of the run), a run of R_{k-1} that meets
-//     D ends one (end mask E_k).  These are exactly the maximal strips of
+//     (mark S_k at the clear row just above the run), a run of R_{k-1} that
+//     meets D ends one (mark E_k).  These are exactly the maximal strips of
 //     heal_boxes (src/polygon_fracture.f90:881-960) inside the tile;
 //  4. the strips touching an internal tile border are seam pieces (SEAM_* tag,
 //     side stream); the rest are final.  In area mode step 3 is replaced by
 //     the int64 sum of run lengths times slab widths
 //     (calculate_union_area_fast, src/geometry.f90:1840-1927).
 //
-// The main kernel runs one wave64 per tile with no workgroup barriers: the
-// coordinate sort is a bitonic network in registers (shuffles across lanes),
-// duplicates are dropped with a wave scan, and the strips are paired by a
-// sweep over slab boundaries in which each lane owns rows {lane, lane+64,
-// ...}: it reads the broadcast S_k/E_k words and closes/opens its own rows'
-// strips.  Tiles above its capacity run on the 256-thread kernel (capacity
-// 255); larger tiles are split into x-sub-columns by the host.  Each tile
-// reserves its output ranges with one atomic add; a reorder pass restores
-// tile order.
+// The main kernel runs one wave64 per tile: the coordinate sort is a bitonic
+// network in registers (shuffles across lanes), duplicates are dropped with a
+// wave scan, each lane owns slab boundaries {lane, lane+64, ...} and marks
+// S_k/E_k with one multi-word addition per column (the run's D bits carry up
+// to the row above it); the E marks are transposed so that a lane holds one
+// row's end boundaries, and every S mark is paired with the first E mark to
+// its right in its row.  Tiles above its capacity run on the 256-thread
+// kernel (capacity 255); larger tiles are split into x-sub-columns by the
+// host.  Each workgroup writes its strips into its own region; a reorder
+// pass restores tile order.
 #include "vl_tile_dev.h"
 
 namespace vl {
@@ -390,7 +391,8 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       nxu = (int)tx;
       nyu = (int)ty;
     }
-    const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 63) >> 6;
+    // one row above the last is kept clear for the run end marks
+    const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 64) >> 6;
     const int words = ncol * W;
     for (int k = lane; k < words; k += WAVE) {
       MA[k] = 0ull;
@@ -477,8 +479,8 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
         const uint64_t* prv = (k > 0) ? MA + (k - 1) * W : nullptr;
         uint64_t D[NW];
         change_mask<NW>(cur, prv, W, D);
-        starts_meeting<NW>(cur, W, D, Sr[kb]);
-        starts_meeting<NW>(prv, W, D, Er[kb]);
+        ends_meeting<NW>(cur, W, D, Sr[kb]);
+        ends_meeting<NW>(prv, W, D, Er[kb]);
 #pragma unroll
         for (int w = 0; w < NW; w++) myT += (uint32_t)__popcll(Er[kb][w]);
       }
@@ -495,14 +497,15 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     const bool ok = used + T <= o.region_cap;
     used += T;
     if (lane == 0) o.tab_off[ti] = base;
-    // 4. strips.  Lane l owns boundary j = 64*kb + l and walks the rows at
-    //    which a strip starts there (bits of Sr).  The end masks are
-    //    transposed in 64x64 bit blocks so that lane l holds, for row
-    //    64*wb + l, the boundaries at which a strip of that row ends; a strip
-    //    starting at (j, a) ends at the first end bit after j in row a, read
-    //    from lane a & 63.  Every lane stays in the loop until the whole wave
-    //    is done, so the cross-lane reads see a full wave.  Output order: row
-    //    block, boundary block, pass, boundary.
+    // 4. strips.  Lane l owns boundary j = 64*kb + l and walks its S marks:
+    //    the rows b just above the runs [a, b) of strips starting there.  The
+    //    E marks are transposed in 64x64 bit blocks so that lane l holds, for
+    //    row 64*wb + l, the boundaries at which a strip marked in that row
+    //    ends; a strip starting at (j, b) ends at the first E mark after j in
+    //    row b, read from lane b & 63, and its run starts one above the
+    //    highest clear row of column j below b.  Every lane stays in the loop
+    //    until the whole wave is done, so the cross-lane reads see a full
+    //    wave.  Output order: row block, boundary block, pass, boundary.
     const uint32_t flags = d.flags;
     const int KW = (ncol + 1 + 63) >> 6;   // boundary words per row
     uint32_t my_side = 0;
@@ -528,9 +531,9 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
 #pragma unroll
           for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], t);
           if (has) {
-            const int a = wb * WAVE + t;
+            const int b = wb * WAVE + t;
             const int k = next_bit_above<NW>(rr, j, ncol);
-            const int b = run_end(MA + j * W, W, a);
+            const int a = run_start_w<NW>(MA + j * W, W, b);
             const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
                                 ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
                                 ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |

@@ -34,14 +34,6 @@ __device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
   }
 }
 
-// a | (b & c), one gfx950 three-input bitwise instruction per 32-bit half
-// (truth table over src0 = 0xF0, src1 = 0xCC, src2 = 0xAA).
-__device__ __forceinline__ uint64_t or_and(uint64_t a, uint64_t b, uint64_t c) {
-  const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, 0xF8);
-  const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32), 0xF8);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Bit w of the dilated change mask of columns cur/prv (null = empty column).
 template <int NW>
 __device__ __forceinline__ void change_mask(const uint64_t* cur, const uint64_t* prv, int W, uint64_t* D) {
@@ -54,41 +46,6 @@ __device__ __forceinline__ void change_mask(const uint64_t* cur, const uint64_t*
 #pragma unroll
   for (int w = 0; w < NW; w++)
     if (w < W) D[w] = dl[w + 1] | (dl[w + 1] << 1) | (dl[w] >> 63) | (dl[w + 1] >> 1) | (dl[w + 2] << 63);
-}
-
-// Start bits of the runs of col (col null = empty; W <= NW words, bit i of
-// word w = row 64w+i) that meet D: bit a is set when col has a run [a, b)
-// holding a bit of D.  Branch-free, so the lanes of a wave (one column
-// boundary each) do not diverge on their run counts: the bits of D inside
-// runs are spread down to the run starts in six doubling steps per word (step
-// s moves a bit s rows down where the s+1 rows from the target up are all in
-// the run), words from the top down with the spread carried across word
-// borders; then only run starts are kept.
-template <int NW>
-__device__ __forceinline__ void starts_meeting(const uint64_t* col, int W, const uint64_t* D, uint64_t* out) {
-  uint64_t carry = 0;   // spread arriving at row 64(w+1) from above, as bit 63 of word w
-#pragma unroll
-  for (int w = NW - 1; w >= 0; w--) {
-    out[w] = 0;
-    if (w >= W || !col) continue;
-    const uint64_t r = col[w];
-    uint64_t m = (D[w] & r) | (carry & r);
-    uint64_t c = r & (r >> 1);   // bit i: rows i..i+1 in the run; then i..i+2, i..i+4, ...
-    m = or_and(m, m >> 1, c);
-    c &= c >> 1;
-    m = or_and(m, m >> 2, c);
-    c &= c >> 2;
-    m = or_and(m, m >> 4, c);
-    c &= c >> 4;
-    m = or_and(m, m >> 8, c);
-    c &= c >> 8;
-    m = or_and(m, m >> 16, c);
-    c &= c >> 16;
-    m = or_and(m, m >> 32, c);
-    const uint64_t below = (w > 0) ? (col[w - 1] >> 63) : 0ull;
-    out[w] = m & r & ~((r << 1) | below);
-    carry = (m & 1ull) ? (1ull << 63) : 0ull;
-  }
 }
 
 // First set bit above position j of the NW-word bitset re, or `none`.  The
@@ -115,26 +72,42 @@ __device__ __forceinline__ int next_bit_above(const uint64_t (&re)[NW], int j, i
   return mm ? bq * WAVE + __ffsll((unsigned long long)mm) - 1 : none;
 }
 
-// First zero bit at or above row a in a column (the run end).
-__device__ __forceinline__ int run_end(const uint64_t* col, int W, int a) {
-  int w = a >> 6;
-  uint64_t m = ~col[w] & (~0ull << (a & 63));
-  while (m == 0) {
-    if (++w >= W) return W * 64;
-    m = ~col[w];
+// End marks of the runs of col (col null = empty; W <= NW words, bit i of
+// word w = row 64w+i) that meet D: bit b is set when col has a run [a, b)
+// holding a bit of D, b being the clear row just above the run (the grid
+// keeps one row above its last: W = (nrows + 64) / 64).  Adding the run's D
+// bits to the column carries through the rest of the run into row b and
+// stops there, so one multi-word addition marks every run that meets D.
+template <int NW>
+__device__ __forceinline__ void ends_meeting(const uint64_t* col, int W, const uint64_t* D, uint64_t* out) {
+  uint64_t carry = 0, below = 0;   // carry into word w; bit 63 of word w-1
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    out[w] = 0;
+    if (w >= W || !col) continue;
+    const uint64_t r = col[w];
+    const uint64_t s1 = r + (D[w] & r);
+    const uint64_t sum = s1 + carry;
+    carry = (uint64_t)(s1 < r) | (uint64_t)(sum < s1);
+    out[w] = sum & ~r & ((r << 1) | below);
+    below = r >> 63;
   }
-  return w * 64 + __ffsll((unsigned long long)m) - 1;
 }
 
-// The same for grids of at most NW words per column (one word: no loop; the
-// bits above the grid's rows are clear, so a zero bit is always found).
+// Lowest row of the run of a column whose clear row just above is b (b >= 1):
+// one above the highest clear bit below b, or 0.
 template <int NW>
-__device__ __forceinline__ int run_end_w(const uint64_t* col, int W, int a) {
+__device__ __forceinline__ int run_start_w(const uint64_t* col, int W, int b) {
+  int w = (b - 1) >> 6;
+  uint64_t m = ~col[w] & (((b >> 6) == w) ? ((1ull << (b & 63)) - 1) : ~0ull);
   if constexpr (NW == 1) {
-    const uint64_t m = ~col[0] & (~0ull << (a & 63));
-    return m ? __ffsll((unsigned long long)m) - 1 : 64;
+    return m ? 64 - __clzll((unsigned long long)m) : 0;
   } else {
-    return run_end(col, W, a);
+    while (m == 0) {
+      if (--w < 0) return 0;
+      m = ~col[w];
+    }
+    return 64 * w + 64 - __clzll((unsigned long long)m);
   }
 }
 
