@@ -288,16 +288,22 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         //    are read from the lanes of its two slots (slot e is component
         //    e % K on lane e / K), so its row words are uniform values.
         __syncthreads();
-        uint32_t own[K];
+        static_assert(K <= 2, "column-parallel raster on one- and two-word grids");
+        // Box images, one per lane in parallel: the image whose edges are
+        // slots e0, e0+1 (e0 = 6i, 6i+2, 6i+4) is prepared on the lane of
+        // slot e0: its column range packed as cx1 | cx2 << 8 (empty when the
+        // image is) and its row words.  Slot e is component e % K of lane
+        // e / K, so with K == 1 the upper edge comes from the next lane.
+        const uint32_t lo = (uint32_t)sidx_x[lane * K] | ((uint32_t)sidx_y[lane * K] << 8);
+        uint32_t hi;
+        if constexpr (K == 1) hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x101, 0xF, 0xF, false);   // row_shl:1
+        else hi = (uint32_t)sidx_x[lane * K + 1] | ((uint32_t)sidx_y[lane * K + 1] << 8);
+        const int bx1 = (int)(lo & 0xffu), by1 = (int)(lo >> 8), bx2 = (int)(hi & 0xffu), by2 = (int)(hi >> 8);
+        const bool live = bx1 < bx2 && by1 < by2;
+        const uint32_t span = live ? (uint32_t)bx1 | ((uint32_t)(bx2 - bx1) << 8) : 0u;
+        uint64_t rw[NW];
 #pragma unroll
-        for (int k = 0; k < K; k++) own[k] = (uint32_t)sidx_x[lane * K + k] | ((uint32_t)sidx_y[lane * K + k] << 8);
-        const auto slot = [&](int e) {
-          uint32_t v = own[0];
-#pragma unroll
-          for (int k = 1; k < K; k++)
-            if (e % K == k) v = own[k];
-          return (uint32_t)__builtin_amdgcn_readlane((int)v, e / K);
-        };
+        for (int w = 0; w < NW; w++) rw[w] = live ? (NW == 1 ? bits_range(by1, by2 - 1) : row_bits(w, by1, by2)) : 0ull;
         uint64_t acc[3][NW][NW];
 #pragma unroll
         for (int m = 0; m < 3; m++)
@@ -310,17 +316,15 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           for (int m = -1; m < 2; m++) {
             if (m == 0 && !g0) continue;
             if (m == 1 && !g1) continue;
-            const int e0 = CPB * i + (m < 0 ? 0 : 2 + 2 * m);
-            const uint32_t lo = slot(e0), hi = slot(e0 + 1);
-            const int cx1 = (int)(lo & 0xffu), cy1 = (int)(lo >> 8), cx2 = (int)(hi & 0xffu), cy2 = (int)(hi >> 8);
-            if (cx1 >= cx2 || cy1 >= cy2) continue;
+            const int l0 = (CPB * i + (m < 0 ? 0 : 2 + 2 * m)) / K;
+            const uint32_t sp = (uint32_t)__builtin_amdgcn_readlane((int)span, l0);
+            const uint32_t x1 = sp & 0xffu, width = sp >> 8;
 #pragma unroll
             for (int cb = 0; cb < NW; cb++) {
-              const int c = cb * WAVE + lane;
-              const bool in = c >= cx1 && c < cx2;
+              const bool in = (uint32_t)(cb * WAVE + lane) - x1 < width;
 #pragma unroll
               for (int w = 0; w < NW; w++) {
-                const uint64_t rows = NW == 1 ? bits_range(cy1, cy2 - 1) : row_bits(w, cy1, cy2);
+                const uint64_t rows = read_lane(rw[w], l0);
                 if (in) acc[m + 1][cb][w] |= rows;
               }
             }
