@@ -98,6 +98,40 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
     }
   }
   uint32_t novf = 0;   // entries in sovf (uniform)
+  // Per-slot constants of the coordinate step (slot e = lane*K + k holds
+  // component e % CPB of box e / CPB): bit 0 of cflag = upper edge, bit 1 =
+  // edge of a grown image (raw edge plus offset, clamped to the window), bit
+  // 2 = edge of an eroded image (clipped edge plus offset, clamped to the
+  // tile), bits 3-4 = which border the slot holds in the border group (window
+  // low / high, tile low / high).  Other slots hold the clipped edge.
+  uint32_t cflag[K];
+  int32_t coffx[K], coffy[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int e = lane * K + k;
+    const int comp = e % CPB;
+    const int hi = comp & 1;
+    const int m = (comp - 2) >> 1;   // morph of components 2..5
+    const uint32_t border = (comp == 0 || comp == 4) ? 0u : (comp == 1 || comp == 5) ? 1u : (comp == 2) ? 2u : 3u;
+    uint32_t f = (uint32_t)hi | (border << 3);
+    int32_t ox = 0, oy = 0;
+    if (comp >= 2 && m < nmorph) {
+      const int32_t kind = m ? fs.kind[1] : fs.kind[0];
+      const int32_t ae = m ? fs.arg[1][0] : fs.arg[0][0], an = m ? fs.arg[1][1] : fs.arg[0][1];
+      const int32_t aw = m ? fs.arg[1][2] : fs.arg[0][2], as = m ? fs.arg[1][3] : fs.arg[0][3];
+      if (kind == MORPH_GROW) {
+        f |= 2u;
+        ox = hi ? ae : -aw;
+        oy = hi ? an : -as;
+      } else {   // erosion by ae: edges shifted inwards
+        f |= 4u;
+        ox = oy = hi ? -ae : ae;
+      }
+    }
+    cflag[k] = f;
+    coffx[k] = ox;
+    coffy[k] = oy;
+  }
   for (int b0 = blockIdx.x * TILE_BATCH; b0 < nt; b0 += gridDim.x * TILE_BATCH) {
     const int nbt = min(TILE_BATCH, nt - b0);
     uint32_t my_ti = 0;
@@ -166,37 +200,20 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       int32_t vx[K], vy[K];
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int e = lane * K + k;
-        const int i = e / CPB, comp = e - CPB * (e / CPB);
-        int32_t cx = INT32_MAX, cy = INT32_MAX;
-        if (i < n) {
-          const vl_box b = bx[i];
-          const int32_t x1 = max(b.x1, wx0), x2 = min(b.x2, wx1);
-          const int32_t y1 = max(b.y1, wy0), y2 = min(b.y2, wy1);
-          const bool hi = comp & 1;
-          const int m = (comp - 2) >> 1;   // morph of components 2..5
-          cx = hi ? x2 : x1;
-          cy = hi ? y2 : y1;
-          if (comp >= 2 && m < nmorph) {
-            const int32_t kind = m ? fs.kind[1] : fs.kind[0];
-            const int32_t ae = m ? fs.arg[1][0] : fs.arg[0][0], an = m ? fs.arg[1][1] : fs.arg[0][1];
-            const int32_t aw = m ? fs.arg[1][2] : fs.arg[0][2], as = m ? fs.arg[1][3] : fs.arg[0][3];
-            if (kind == MORPH_GROW) {
-              cx = hi ? clamp64((int64_t)b.x2 + ae, wx0, wx1) : clamp64((int64_t)b.x1 - aw, wx0, wx1);
-              cy = hi ? clamp64((int64_t)b.y2 + an, wy0, wy1) : clamp64((int64_t)b.y1 - as, wy0, wy1);
-            } else {   // erosion by ae: edges shifted inwards, clamped to the tile
-              cx = hi ? clamp64((int64_t)x2 - ae, d.x0, d.x1) : clamp64((int64_t)x1 + ae, d.x0, d.x1);
-              cy = hi ? clamp64((int64_t)y2 - ae, d.y0, d.y1) : clamp64((int64_t)y1 + ae, d.y0, d.y1);
-            }
-          }
-        } else if (i == n) {
-          if (comp == 0 || comp == 4) { cx = wx0; cy = wy0; }
-          else if (comp == 1 || comp == 5) { cx = wx1; cy = wy1; }
-          else if (comp == 2) { cx = d.x0; cy = d.y0; }
-          else { cx = d.x1; cy = d.y1; }
-        }
-        vx[k] = cx;
-        vy[k] = cy;
+        const int i = (lane * K + k) / CPB;
+        const uint32_t f = cflag[k];
+        const vl_box b = bx[min(i, NMAX)];
+        const int32_t rx = (f & 1u) ? b.x2 : b.x1, ry = (f & 1u) ? b.y2 : b.y1;
+        const int32_t kx = (f & 1u) ? min(b.x2, wx1) : max(b.x1, wx0);   // clipped to the window
+        const int32_t ky = (f & 1u) ? min(b.y2, wy1) : max(b.y1, wy0);
+        const bool grown = f & 2u, eroded = f & 4u;
+        const int32_t cx = clamp64((int64_t)(grown ? rx : kx) + coffx[k], eroded ? d.x0 : wx0, eroded ? d.x1 : wx1);
+        const int32_t cy = clamp64((int64_t)(grown ? ry : ky) + coffy[k], eroded ? d.y0 : wy0, eroded ? d.y1 : wy1);
+        const uint32_t bsel = f >> 3;
+        const int32_t bxv = bsel == 0 ? wx0 : bsel == 1 ? wx1 : bsel == 2 ? d.x0 : d.x1;
+        const int32_t byv = bsel == 0 ? wy0 : bsel == 1 ? wy1 : bsel == 2 ? d.y0 : d.y1;
+        vx[k] = i < n ? cx : (i == n ? bxv : INT32_MAX);
+        vy[k] = i < n ? cy : (i == n ? byv : INT32_MAX);
       }
       const int cnt = CPB * n + CPB;
       // Sort keys: the coordinate relative to the window corner (below 2^24:
