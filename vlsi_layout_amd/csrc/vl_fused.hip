@@ -43,6 +43,19 @@ __device__ __forceinline__ uint64_t tt_eval(uint32_t tt, uint64_t s, uint64_t m0
   return sel(m1, sel(m0, f11, f10), sel(m0, f01, f00));
 }
 
+// The same, with the tables of the outputs most groups have (a morphological
+// image itself, and the ring differences m0 & ~s and s & ~m1) as plain word
+// operations ahead of the general evaluation (tt uniform).
+__device__ __forceinline__ uint64_t tt_eval_common(uint32_t tt, uint64_t s, uint64_t m0, uint64_t m1) {
+  switch (tt) {
+    case 0xCCu: return m0;
+    case 0xF0u: return m1;
+    case 0x44u: return m0 & ~s;
+    case 0x0Au: return s & ~m1;
+    default: return tt_eval(tt, s, m0, m1);
+  }
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc, int ntiles,
                                                const vl_box* __restrict__ entries, FusedSpec fs, FusedOut o) {
@@ -430,7 +443,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         const uint64_t rb = row_bits(0, rowT0, rowT1);
 #pragma unroll 1
         for (int q = 0; q < nout; q++) {
-          const uint64_t v = in ? tt_eval(uniform(s_tt[q]), b0, b1, b2) & rb : 0ull;
+          const uint64_t v = in ? tt_eval_common(uniform(s_tt[q]), b0, b1, b2) & rb : 0ull;
           if (c < ncol) MTQ[min(q, NQ - 1)][c] = v;
           if (__ballot(v != 0ull)) anymask |= 1u << q;
         }
@@ -458,8 +471,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
             for (int w = 0; w < NW; w++) {
               if (w >= W) break;
               const int ix = c * W + w;
-              const uint64_t v = in ? tt_eval(tt, MS[ix], (nmorph > 0) ? MM[0][ix] : 0ull,
-                                              (nmorph > 1) ? MM[1][ix] : 0ull) & row_bits(w, rowT0, rowT1)
+              const uint64_t v = in ? tt_eval_common(tt, MS[ix], (nmorph > 0) ? MM[0][ix] : 0ull,
+                                                     (nmorph > 1) ? MM[1][ix] : 0ull) & row_bits(w, rowT0, rowT1)
                                     : 0ull;
               MT[ix] = v;
               any |= v;
