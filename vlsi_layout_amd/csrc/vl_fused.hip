@@ -528,7 +528,7 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         // clear row of column j below b.  Strips are written in table order.
         constexpr int Q = 2 * P / WAVE;   // table entries per lane and round
         uint16_t* const stab = erow;      // 2P entries: (j << 8) | b
-        uint32_t my_side = 0;
+        uint32_t nside = 0;   // seam strips of this tile (uniform)
         uint64_t pos = base;
 #pragma unroll
         for (int wb = 0; wb < NW; wb++) {
@@ -563,27 +563,27 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
                 uint64_t rr[NW];
 #pragma unroll
                 for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], b & (WAVE - 1));
+                uint32_t tg = 0u;
                 if (has) {
                   const int k = next_bit_above<NW>(rr, j, ncol);
                   const int a = run_start_w<NW>(MO + j * W, W, b);
-                  const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                                      ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                                      ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                                      ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+                  tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                       ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                       ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                       ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
                   const uint64_t p = pos + g;
                   if (ok) {
                     gstore_box(strips + p, ux[j], uy[a], ux[k], uy[b]);
                     gstore(tags + p, (uint8_t)tg);
                   }
-                  my_side += tg != 0u;
                 }
+                nside += (uint32_t)__popcll(__ballot(tg != 0u));
               }
               pos += nent;
               __syncthreads();
             }
           }
         }
-        const uint32_t nside = (uint32_t)wave_sum((long long)my_side);
         __syncthreads();
         if (lane == 0) {
           gstore(tn + ti, T - nside);

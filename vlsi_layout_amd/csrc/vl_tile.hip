@@ -508,7 +508,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     //    wave.  Output order: row block, boundary block, pass, boundary.
     const uint32_t flags = d.flags;
     const int KW = (ncol + 1 + 63) >> 6;   // boundary words per row
-    uint32_t my_side = 0;
+    uint32_t nside = 0;   // seam strips of this tile (uniform)
     uint64_t pos = base;
 #pragma unroll
     for (int wb = 0; wb < NW; wb++) {
@@ -530,14 +530,15 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
           uint64_t rr[NW];
 #pragma unroll
           for (int kq = 0; kq < NW; kq++) rr[kq] = lane_read64(re[kq], t);
+          uint32_t tg = 0u;
           if (has) {
             const int b = wb * WAVE + t;
             const int k = next_bit_above<NW>(rr, j, ncol);
             const int a = run_start_w<NW>(MA + j * W, W, b);
-            const uint32_t tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
-                                ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
-                                ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
-                                ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+            tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                 ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                 ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                 ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
             const uint64_t p = pos + lanes_below(act);
             if (ok) {
               vl_box st;
@@ -545,14 +546,13 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
               o.strips[p] = st;
               o.tag[p] = (uint8_t)tg;
             }
-            my_side += tg != 0u;
           }
+          nside += (uint32_t)__popcll(__ballot(tg != 0u));
           pos += (uint64_t)__popcll(act);
         }
       }
     }
 
-    const uint32_t nside = (uint32_t)wave_sum((long long)my_side);
     side_total += nside;
     if (lane == 0) {
       o.tab_n[ti] = T - nside;
