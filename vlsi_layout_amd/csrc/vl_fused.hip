@@ -69,7 +69,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint8_t sidx_x[P], sidx_y[P]; // column / row index of every coordinate slot
   // row maps of the y erosion; during output extraction the strip-start table
-  __shared__ uint16_t erow[2 * P];
+  // (at least 4 entries per lane: one round takes nearly every tile's starts)
+  constexpr int EROW = 2 * P > 4 * WAVE ? 2 * P : 4 * WAVE;
+  __shared__ uint16_t erow[EROW];
   uint16_t* const eup = erow;
   uint16_t* const edn = erow + P;
   __shared__ uint64_t MS[NCOL * NW];       // S
@@ -526,8 +528,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         // first E mark after j in row b, read from the row-major (transposed)
         // end marks held by lane b & 63; its run starts one above the highest
         // clear row of column j below b.  Strips are written in table order.
-        constexpr int Q = 2 * P / WAVE;   // table entries per lane and round
-        uint16_t* const stab = erow;      // 2P entries: (j << 8) | b
+        constexpr int Q = EROW / WAVE;    // table entries per lane and round
+        uint16_t* const stab = erow;      // (j << 8) | b
         uint32_t nside = 0;   // seam strips of this tile (uniform)
         uint64_t pos = base;
 #pragma unroll
