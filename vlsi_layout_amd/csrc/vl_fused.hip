@@ -312,7 +312,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         nyu = (int)ty;
       }
       // one row above the last is kept clear for the run end marks
-      const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 64) >> 6;
+      // (a 64-slot grid has at most 63 rows: one word, W folds to 1)
+      const int ncol = nxu - 1, nrows = nyu - 1, W = NW == 1 ? 1 : (nrows + 64) >> 6;
       if constexpr (NW <= 2) {
         // 2. rasterise S and the grown images, column-parallel: lane l ORs,
         //    into its columns 64*cb + l, the row words of every box image

@@ -392,7 +392,8 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       nyu = (int)ty;
     }
     // one row above the last is kept clear for the run end marks
-    const int ncol = nxu - 1, nrows = nyu - 1, W = (nrows + 64) >> 6;
+    // (a 64-slot grid has at most 63 rows: one word, W folds to 1)
+    const int ncol = nxu - 1, nrows = nyu - 1, W = NW == 1 ? 1 : (nrows + 64) >> 6;
     const int words = ncol * W;
     for (int k = lane; k < words; k += WAVE) {
       MA[k] = 0ull;
