@@ -403,8 +403,8 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     // 2. compress + rasterise (box-parallel: lane takes boxes lane, lane+64, ...)
     for (int i = lane; i < n; i += WAVE) {
       const vl_box b = bx[i];
-      const int cx1 = lower_bound_i32(ux, nxu, max(b.x1, wx0)), cx2 = lower_bound_i32(ux, nxu, min(b.x2, wx1));
-      const int cy1 = lower_bound_i32(uy, nyu, max(b.y1, wy0)), cy2 = lower_bound_i32(uy, nyu, min(b.y2, wy1));
+      const int cx1 = lower_bound_fixed<P>(ux, nxu, max(b.x1, wx0)), cx2 = lower_bound_fixed<P>(ux, nxu, min(b.x2, wx1));
+      const int cy1 = lower_bound_fixed<P>(uy, nyu, max(b.y1, wy0)), cy2 = lower_bound_fixed<P>(uy, nyu, min(b.y2, wy1));
       if (cx1 >= cx2 || cy1 >= cy2) continue;
       uint64_t* M = (twol && i >= nA) ? MX : MA;
       const int w1 = cy1 >> 6, w2 = (cy2 - 1) >> 6;

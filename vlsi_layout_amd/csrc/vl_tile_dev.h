@@ -19,6 +19,22 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int32_t 
   return lo;
 }
 
+// lower_bound over a sorted array of n <= P entries (P a power of two) in a
+// fixed log2(P) + 1 steps: no data-dependent loop, so the lanes of a wave
+// stay converged (a step moves lo up by `step` when the entry just below
+// lo + step is still below v).
+template <int P>
+__device__ __forceinline__ int lower_bound_fixed(const int32_t* a, int n, int32_t v) {
+  int lo = 0;
+#pragma unroll
+  for (int step = P; step > 0; step >>= 1) {
+    const int t = lo + step;
+    const bool up = t <= n && a[max(min(t, n) - 1, 0)] < v;
+    lo = up ? t : lo;
+  }
+  return lo;
+}
+
 // Bits of rows [r0, r1) in word w.
 __device__ __forceinline__ uint64_t row_bits(int w, int r0, int r1) {
   const int lo = max(r0 - 64 * w, 0), hi = min(r1 - 64 * w, 64);
