@@ -116,6 +116,27 @@ def test_dense_overlaps_and_touching(eng, op, args):
     check(eng, op, a, b, args)
 
 
+def stacked_strips(rng, n, phase):
+    """n one-unit horizontal strips stacked two units apart (rows start at
+    `phase`), random x extents, plus a tall bar that meets every row: the
+    compressed grids of these tiles hold about 2n rows, so sweeping n puts
+    their row counts on both sides of the 64- and 128-row word borders, with
+    runs that end on the grid's last row."""
+    y = 2 * np.arange(n) + phase
+    x1 = rng.integers(0, 40, n)
+    x2 = x1 + rng.integers(1, 40, n)
+    strips = np.stack([x1, y, x2, y + 1], 1)
+    bar = np.array([[17, -3, 19, 2 * n + 5]])
+    return np.concatenate([strips, bar]).astype(np.int32)
+
+
+@pytest.mark.parametrize("n", [30, 31, 32, 33, 62, 63, 64, 65])
+@pytest.mark.parametrize("op,args", OPS, ids=[f"{o}{a}" for o, a in OPS])
+def test_row_word_borders(eng, op, args, n):
+    rng = np.random.default_rng(1000 + n)
+    check(eng, op, stacked_strips(rng, n, 0), stacked_strips(rng, n, 1), args)
+
+
 def test_extreme_coordinates(eng):
     rng = np.random.default_rng(9)
     big = 2 ** 31 - 200000

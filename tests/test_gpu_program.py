@@ -211,6 +211,31 @@ def test_fused_groups_match_plain(gpu_engine, group, layer):
     eng.close()
 
 
+@pytest.mark.parametrize("n", [9, 10, 11, 12, 19, 20, 21, 22])
+@pytest.mark.parametrize("group", sorted(GROUPS))
+def test_fused_row_word_borders(gpu_engine, group, n):
+    """n strips three units tall, eight apart, plus a bar meeting every row,
+    in one window: with the grown and eroded edges each strip brings about
+    six grid lines, so the windows of this sweep sit on the 64-, 128- and
+    256-slot tiers with row counts on both sides of the 64- and 128-row word
+    borders and runs ending on the grid's last row."""
+    eng = V.Engine(0)
+    eng.set("fuse_target", 60)
+    rng = np.random.default_rng(2000 + n)
+    y = 8 * np.arange(n)
+    x1 = rng.integers(0, 60, n)
+    s = np.concatenate([np.stack([x1, y, x1 + rng.integers(4, 60, n), y + 3], 1),
+                        [[30, -4, 36, 8 * n + 4]]]).astype(np.int32)
+    S = V.Layer(eng, s)
+    steps = GROUPS[group]
+    got, _ = run_group(eng, S, steps)
+    assert eng.get("fused_groups") == 1
+    exp = plain_group(S, steps)
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes())), (group, n, k)
+    eng.close()
+
+
 @pytest.mark.parametrize("target", [2, 30, 60])
 def test_fused_tiers_and_splits(gpu_engine, target):
     """Grids far below and above the fused kernel's capacities: many empty
