@@ -69,8 +69,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   __shared__ int32_t ux[P], uy[P];
   __shared__ uint8_t sidx_x[P], sidx_y[P]; // column / row index of every coordinate slot
   // row maps of the y erosion; during output extraction the strip-start table
-  // (at least 4 entries per lane: one round takes nearly every tile's starts)
-  constexpr int EROW = 2 * P > 4 * WAVE ? 2 * P : 4 * WAVE;
+  // (4 entries per lane on 64-slot grids, 8 on larger ones: one round takes
+  // nearly every tile's starts)
+  constexpr int EROW = 2 * P > (P == 64 ? 4 : 8) * WAVE ? 2 * P : (P == 64 ? 4 : 8) * WAVE;
   __shared__ uint16_t erow[EROW];
   uint16_t* const eup = erow;
   uint16_t* const edn = erow + P;
