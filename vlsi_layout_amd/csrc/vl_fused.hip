@@ -29,7 +29,8 @@ namespace vl {
 
 namespace {
 
-constexpr int FUSE_OVF_BUF = 64;   // overflow tiles buffered in LDS per workgroup
+constexpr int FUSE_OVF_BUF = 32;   // overflow tiles buffered in LDS per workgroup
+constexpr int FUSED_WAVES_PER_CU = 32;   // resident-workgroup cap of the fused grid
 
 // Value of truth table tt (bit s | m0 << 1 | m1 << 2) on base words s, m0, m1
 // (bit 0, the value outside every mask, is taken as 0).  Shannon expansion on
@@ -77,9 +78,13 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   uint16_t* const edn = erow + P;
   __shared__ uint64_t MS[NCOL * NW];       // S
   __shared__ uint64_t MM[2][NCOL * NW];    // morphological images of S
-  __shared__ uint64_t MT[NCOL * NW];       // erosion temporary, then the current output
   constexpr int NQ = NW == 1 ? 4 : 1;      // outputs evaluated in one pass (one-word grids)
   __shared__ uint64_t MTQ[NQ][NW == 1 ? NCOL : 1];
+  // erosion temporary, then the current output; on one-word grids the first
+  // slot of MTQ, which is written only after the erosion passes and unused
+  // when the outputs are evaluated one at a time
+  __shared__ uint64_t MTbuf[NW == 1 ? 1 : NCOL * NW];
+  uint64_t* const MT = NW == 1 ? &MTQ[0][0] : MTbuf;
   __shared__ vl_box sbuf[TILE_BATCH * (NMAX + 1)];
   __shared__ TileDesc sdesc[TILE_BATCH];
   uint32_t* const sdw = (uint32_t*)sdesc;
@@ -642,8 +647,9 @@ int tile_fused_max_wg(int nmax) {
   const int i = nmax == FUSE_NMAX_TINY ? 0 : nmax == FUSE_NMAX_SMALL ? 1 : 2;
   int& c = cache[i];
   if (!c)
-    c = i == 0 ? occupancy_wg(k_tile_f<64>, WAVE)
-               : i == 1 ? occupancy_wg(k_tile_f<128>, WAVE) : occupancy_wg(k_tile_f<256>, WAVE);
+    c = i == 0 ? occupancy_wg(k_tile_f<64>, WAVE, FUSED_WAVES_PER_CU)
+               : i == 1 ? occupancy_wg(k_tile_f<128>, WAVE, FUSED_WAVES_PER_CU)
+                        : occupancy_wg(k_tile_f<256>, WAVE, FUSED_WAVES_PER_CU);
   return c;
 }
 

@@ -404,11 +404,15 @@ int num_cus() {
 constexpr int WAVE_KERNEL_WAVES_PER_CU = 24;
 
 template <class KF>
-int occupancy_wg(KF kern, int threads) {
+int occupancy_wg(KF kern, int threads, int wave_cap = WAVE_KERNEL_WAVES_PER_CU) {
   int per = 0;
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, threads, 0);
   if (per < 1) per = 1;
-  if (threads == WAVE && per > WAVE_KERNEL_WAVES_PER_CU) per = WAVE_KERNEL_WAVES_PER_CU;
+  if (threads == WAVE && per > wave_cap) per = wave_cap;
+  // one-wave workgroups with a fixed share of the tiles each: a count per CU
+  // that is not a multiple of the CU's 4 SIMDs leaves some SIMDs one wave
+  // more, and their workgroups finish last
+  if (threads == WAVE && per >= 4) per &= ~3;
   return per * num_cus();
 }
 
