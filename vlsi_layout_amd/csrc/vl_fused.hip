@@ -30,7 +30,9 @@ namespace vl {
 namespace {
 
 constexpr int FUSE_OVF_BUF = 32;   // overflow tiles buffered in LDS per workgroup
-constexpr int FUSED_WAVES_PER_CU = 32;   // resident-workgroup cap of the fused grid
+// resident-workgroup cap of the fused grid: 24 one-wave workgroups per CU
+// (6 per SIMD) measured best on the 64-slot tier, ahead of 20, 28 and 32
+constexpr int FUSED_WAVES_PER_CU = 24;
 
 // Value of truth table tt (bit s | m0 << 1 | m1 << 2) on base words s, m0, m1
 // (bit 0, the value outside every mask, is taken as 0).  Shannon expansion on
