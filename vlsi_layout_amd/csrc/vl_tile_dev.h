@@ -349,6 +349,31 @@ __device__ __forceinline__ void erode_y(const uint64_t* M, uint64_t* out, const 
     }
     return;
   }
+  if constexpr (NW == 2) {
+    // two words (the grid's top row is clear, so every run ends inside the
+    // column): the same pairing on the 128-bit start and end masks
+    for (int c = t; c < ncol; c += nt) {
+      const uint64_t v0 = M[c * W], v1 = W > 1 ? M[c * W + 1] : 0ull;
+      uint64_t s0 = v0 & ~(v0 << 1), s1 = v1 & ~((v1 << 1) | (v0 >> 63));
+      uint64_t e0 = ~v0 & (v0 << 1), e1 = ~v1 & ((v1 << 1) | (v0 >> 63));
+      uint64_t r0 = 0ull, r1 = 0ull;
+      while (__ballot((s0 | s1) != 0ull)) {
+        const bool has = (s0 | s1) != 0ull;
+        const int a = s0 ? __ffsll((unsigned long long)s0) - 1 : (s1 ? 64 + __ffsll((unsigned long long)s1) - 1 : 0);
+        const int b = e0 ? __ffsll((unsigned long long)e0) - 1 : (e1 ? 64 + __ffsll((unsigned long long)e1) - 1 : 0);
+        if (s0) s0 &= s0 - 1; else s1 &= s1 - 1;
+        if (e0) e0 &= e0 - 1; else e1 &= e1 - 1;
+        const int ia = up[a], ib = dn[b];
+        if (has && ia < ib) {
+          if (ia < 64) r0 |= bits_range(ia, min(ib, 64) - 1);
+          if (ib > 64) r1 |= bits_range(max(ia, 64) - 64, ib - 65);
+        }
+      }
+      out[c * W] = r0;
+      if (W > 1) out[c * W + 1] = r1;
+    }
+    return;
+  }
   for (int c = t; c < ncol; c += nt) {
     uint64_t res[NW];
 #pragma unroll
