@@ -10,19 +10,27 @@ every layer:
 Throughput = input boxes of all operators of a step (N_A+N_B for binary ops,
 N for GROW/SIZE) / wall time of the step, with the engine stream synchronised
 on both sides of the timed region.  "plain" runs one operator per call with
-the default tile budgets; "batched" runs the same operators as one recorded
-program after the autotuner has picked the tile budgets: each layer's ring is
-one fused group (the layer binned once, every output computed per tile from
-shared masks) and the four layer chains run concurrently on worker streams.
+the default tile budgets; "batched" runs the same six operators per layer as
+one recorded program after the autotuner has picked the tile budgets: GROW,
+SIZE and the two NOTs of a layer are one fused group (the layer binned once,
+the four outputs computed per tile from shared masks), the two ANDs run on the
+materialised outer / inner layers (so the ASSERT_ZEROs check what the fused
+group produced), and the four layer chains run concurrently on worker
+streams.  Both forms evaluate every operator on the GPU and are timed with
+the same settings; the box count of each form is taken from its own run.
 
-The roofline figure is for the per-tile Boolean kernel (the dominant kernel):
-algorithmic bytes of its launches in the timed region (16 B per entry read,
-16 B per strip written, 48 B per tile) / their summed duration from HIP events
-on the engine stream, against the MI355X's 8 TB/s HBM peak.
+The roofline figure is for the kernel stage with the largest device time in
+the timed step: algorithmic bytes of its launches (tile / fused stage: 16 B
+per entry read, 16 B per strip written, 48 B per tile; sort: 16 B per entry
+per pass) / their summed duration from HIP events on the stream they run on,
+against the MI355X's 8 TB/s HBM peak.  "compulsory_io_frac" is 16 B per input
+and output box of every operator / step time, against the same peak.
 
-cpu_baseline: the same OP1 ring on a band of rows of the same layers, run by
-the C/OpenMP restatement of the reference CPU path (oracle/liboracle.so,
-frame-tiled like src/design_impl.f90:993-1316).
+cpu_baseline: the same OP1 ring on the first 2M healed boxes of met1 (a band
+of rows), run by the C/OpenMP restatement of the reference CPU path
+(oracle/liboracle.so, frame-tiled like src/design_impl.f90:993-1316) on the
+host's cores.  Its time grows faster than the box count (the frame grid has
+only nodes^(1/4) tiles per side), so a whole 10M-box layer is not sampled.
 
 Also reported in the same line: "c2_10M_total", the same ring with 10M boxes
 over the 4 layers (2.5M per layer), plain and batched; and "serving", windowed
@@ -52,7 +60,8 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--boxes-per-layer", type=int, default=10_000_000)
-    p.add_argument("--cpu-sample", type=int, default=1_000_000, help="boxes per layer in the CPU sample")
+    p.add_argument("--cpu-sample", type=int, default=2_000_000,
+                   help="healed boxes of met1 (a band of rows) in the CPU baseline's ring")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--mode", choices=["plain", "batched", "both"], default="both")
     p.add_argument("--no-autotune", action="store_true", help="batched form with the default tile budgets")
@@ -75,7 +84,6 @@ class Ring:
         self.L1, self.L2, self.outer, self.inner, self.and1, self.ring = (V.Layer(eng) for _ in range(6))
 
     def run_plain(self, V):
-        n = self.L.n_used
         V.CalculateGROWLayer(self.L, self.L1, (1, 1, 1, 1))
         V.CalculateBoostOperation(self.L, self.L, self.L2, V.K_BOOST_CONTROL_SIZE, -1)
         V.CalculateFrameNOT(self.L1, self.L, self.outer)
@@ -83,9 +91,19 @@ class Ring:
         V.CalculateAND(self.outer, self.L, self.and1)
         V.CalculateAND(self.outer, self.inner, self.ring)
         fails = int(V.ASSERT_ZERO(self.and1) > 0) + int(V.ASSERT_ZERO(self.ring) > 0)
-        boxes = (n + n + (self.L1.n_used + n) + (n + self.L2.n_used) + (self.outer.n_used + n)
-                 + (self.outer.n_used + self.inner.n_used))
-        return boxes, fails
+        return self.counts(), fails
+
+    def counts(self):
+        """(input boxes, output boxes) of the ring's six operators as last run:
+        N for GROW and SIZE, N_A + N_B for NOT and AND.  Both forms evaluate all
+        six on the GPU (the batched form fuses GROW, SIZE and the two NOTs into
+        one tile pass and runs the two ANDs on the materialised outer and inner
+        layers), so the same count holds for both."""
+        n, l1, l2, o, i = (x.n_used for x in (self.L, self.L1, self.L2, self.outer, self.inner))
+        a1, r = self.and1.n_used, self.ring.n_used
+        ins = n + n + (l1 + n) + (n + l2) + (o + n) + (o + i)
+        outs = l1 + l2 + o + i + a1 + r
+        return ins, outs
 
     def record(self, V, prog):
         prog.add(V.VL_OP_GROW, self.L, None, self.L1, (1, 1, 1, 1))
@@ -99,28 +117,62 @@ class Ring:
 
 
 def time_steps(fn, eng, steps, warmup):
+    """W untimed warm-up steps, then K timed steps with the engine stream
+    synchronised on both sides; per-stage kernel statistics (HIP events on
+    the streams the kernels run on, read after the timed region) are reset at
+    the start of the timed region."""
     for _ in range(warmup):
         fn()
     eng.sync()
+    eng.set("profile", 1)
     t0 = time.perf_counter()
     out = None
     for _ in range(steps):
         out = fn()
     eng.sync()
-    return time.perf_counter() - t0, out
+    secs = time.perf_counter() - t0
+    stats = stage_stats(eng, steps)
+    eng.set("profile", 0)
+    return secs, out, stats
 
 
-def tile_stage_traffic(npl):
-    """Measured HBM bytes per tile stage (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    calibrated on a copy kernel) from the committed profile of this workload,
-    or None when the workload differs from the profiled one."""
-    path = os.path.join(ROOT, "profiles", "tile_stage_traffic_stdcell_10M_per_layer.md")
-    if npl != 10_000_000 or not os.path.exists(path):
+STAGES = {
+    "tile": "single-operator tile stage (k_tile_w tiers + k_tile_b)",
+    "fused": "fused tile kernel of an operator group (k_tile_f tiers)",
+    "sort": "binning radix sort (k_rs_upsweep + k_rs_downsweep passes)",
+    "bin": "tile binning (span, emit, radix sort, bounds, gather)",
+}
+
+
+def stage_stats(eng, steps):
+    """Per-step launches, device ms and algorithmic bytes of each stage."""
+    out = {}
+    for k in STAGES:
+        n, ms, by = eng.kernel_stats(k)
+        out[k] = {"launches_per_step": n / steps, "ms_per_step": ms / steps, "bytes_per_step": by / steps,
+                  "gbs": (by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0}
+    return out
+
+
+def profile_figure(name, key):
+    """A figure recorded in a committed rocprofv3 summary (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
         return None
     for ln in open(path):
-        if ln.startswith("traffic_bytes_per_stage = "):
+        if ln.startswith(key + " = "):
             return float(ln.split("=")[1])
     return None
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def serving(V, args):
@@ -192,13 +244,19 @@ def main():
         del cal
     result = {}
 
-    def plain_step():
-        tot, fails = 0, 0
+    def step_counts():
+        ins = outs = 0
         for r in rings:
-            b, f = r.run_plain(V)
-            tot += b
-            fails += f
-        return tot, fails
+            i, o = r.counts()
+            ins += i
+            outs += o
+        return ins, outs
+
+    def plain_step():
+        fails = 0
+        for r in rings:
+            fails += r.run_plain(V)[1]
+        return fails
 
     prog = V.Program(eng)
     for r in rings:
@@ -208,10 +266,10 @@ def main():
         prog.run()
         return prog.assert_failures()
 
-    boxes_per_step, fails = plain_step()          # also fixes the per-step box count
-    if fails:
-        raise SystemExit(f"OP1 ring assertion failed on {fails} checks")
-    print(f"[bench] first step ok: {boxes_per_step} input boxes per step", file=sys.stderr, flush=True)
+    if plain_step():
+        raise SystemExit("OP1 ring assertion failed")
+    plain_counts = step_counts()
+    print(f"[bench] first step ok: {plain_counts[0]} input boxes per step", file=sys.stderr, flush=True)
 
     modes = ["plain", "batched"] if args.mode == "both" else [args.mode]
     tuned = None
@@ -221,40 +279,37 @@ def main():
             # met1 NOT SIZE(met1, -1), GROW, SIZE and the fused ring of met1)
             tuned = eng.autotune(rings[2].L, rings[2].L2)
             print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
-        # The plain steps are timed with the per-stage HIP events on (the
-        # roofline is read from them).  The batched steps are timed with them
-        # off: the binning stage's event wait would stall each worker chain
-        # and so the overlap being measured; their kernel figures come from
-        # one more, untimed, step with the events on.
-        eng.set("profile", 1 if m == "plain" else 0)
-        secs, out = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps, args.warmup)
-        fails = out[1] if m == "plain" else out
+        # Both forms are timed the same way: per-stage statistics on, read
+        # after the timed region (their HIP events add no waits).
+        secs, fails, stats = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps, args.warmup)
         if fails:
             raise SystemExit(f"OP1 ring assertion failed in {m} mode")
-        if m == "batched":
-            eng.set("profile", 1)
-            if batched_step():
-                raise SystemExit("OP1 ring assertion failed in batched mode")
-            eng.sync()
-        launches, ms, by = eng.kernel_stats("tile")
-        bl, bms, bby = eng.kernel_stats("bin")
+        ins, outs = step_counts()
+        if ins != plain_counts[0]:
+            raise SystemExit(f"{m}: {ins} input boxes per step, the plain form had {plain_counts[0]}")
+        ms = 1000.0 * secs / args.steps
         result[m] = {
-            "boxes_per_s": boxes_per_step * args.steps / secs,
-            "ms_per_step": 1000.0 * secs / args.steps,
-            "tile_kernel": {"launches": launches, "ms": ms, "bytes": by,
-                            "gbs": (by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0},
-            "bin_kernels": {"launches": bl, "ms": bms, "bytes": bby,
-                            "gbs": (bby / (bms * 1e-3) / 1e9) if bms > 0 else 0.0},
+            "boxes_per_s": ins * args.steps / secs,
+            "ms_per_step": ms,
+            "input_boxes_per_step": ins,
+            "output_boxes_per_step": outs,
+            "compulsory_io_gbs": 16.0 * (ins + outs) / (ms * 1e-3) / 1e9,
+            "stages": stats,
         }
-        eng.set("profile", 0)
-        print(f"[bench] {m}: {result[m]['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
+        print(f"[bench] {m}: {ms:.1f} ms/step", file=sys.stderr, flush=True)
 
     head = "batched" if "batched" in result else "plain"
-    # kernel efficiency is taken from the plain run (one stage in flight at a
-    # time); in the batched run concurrent chains stretch each stage's duration
-    tk = result["plain" if "plain" in result else head]["tile_kernel"]
-    achieved = tk["gbs"]
-    traffic = tile_stage_traffic(npl)
+    hs = result[head]["stages"]
+    # the roofline names the kernel stage with the largest device time in the
+    # timed step (the fused kernel in the batched form, the tile stage in the
+    # plain one); its achieved rate = algorithmic bytes / summed duration
+    dom = max(("fused", "tile", "sort"), key=lambda k: hs[k]["ms_per_step"])
+    d = hs[dom]
+    achieved = d["gbs"]
+    per_launch = d["bytes_per_step"] / max(d["launches_per_step"], 1e-9)
+    traffic = None
+    if npl == 10_000_000 and not args.no_autotune:
+        traffic = profile_figure(f"{dom}_traffic_stdcell_10M_per_layer.md", "traffic_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": result[head]["boxes_per_s"],
@@ -270,55 +325,63 @@ def main():
         "data": "synthetic: seeded standard-cell generator (seed 0x5EED0002), layers resident in HBM",
         "config": {"workload": "stdcell-4layer-OP1-ring", "boxes_per_layer": npl, "layers": LAYERS,
                    "ring_distance_dbu": 1, "operators_per_step": 24,
-                   "input_boxes_per_step": int(boxes_per_step)},
+                   "input_boxes_per_step": int(plain_counts[0])},
         "plain": result.get("plain"),
         "batched": result.get("batched"),
         "autotuned": tuned,
         "batched_over_plain": (result["batched"]["boxes_per_s"] / result["plain"]["boxes_per_s"]
                                if "batched" in result and "plain" in result else None),
-        "roofline": {"bound": "hbm", "kernel": "tile stage (k_tile_w tiers + k_tile_b, per operator level)",
+        "roofline": {"bound": "hbm", "kernel": STAGES[dom], "stage": dom, "form": head,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic},
+                     "algorithmic_bytes_per_launch": per_launch, "traffic": traffic,
+                     "compulsory_io_frac": result[head]["compulsory_io_gbs"] / HBM_PEAK_GBS},
         "generation_s": t_gen,
     }
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        samples = [O.canonical(lay[:args.cpu_sample]) for lay in layers]
-        boxes = 0
+        # the ring of a band of met1 rows, healed first (by the engine's
+        # merge, bit-identical to the oracle's canonical form)
+        k = 2
+        M = V.Layer(eng)
+        V.CalculateBoostOperation(rings[k].L, rings[k].L, M, V.K_BOOST_CONTROL_MERGE)
+        L = M.boxes()[: args.cpu_sample]
+        n = len(L)
         t0 = time.perf_counter()
-        for L in samples:
-            n = len(L)
-            L1 = O.par_op("grow", L, args=(1, 1, 1, 1), nthreads=nthreads)
-            L2 = O.par_op("size", L, args=(-1,), nthreads=nthreads)
-            outer = O.par_op("not", L1, L, nthreads=nthreads)
-            inner = O.par_op("not", L, L2, nthreads=nthreads)
-            and1 = O.par_op("and", outer, L, nthreads=nthreads)
-            ring = O.par_op("and", outer, inner, nthreads=nthreads)
-            if len(and1) or len(ring):
-                raise SystemExit("CPU baseline OP1 assertion failed")
-            boxes += n + n + (len(L1) + n) + (n + len(L2)) + (len(outer) + n) + (len(outer) + len(inner))
+        L1 = O.par_op("grow", L, args=(1, 1, 1, 1), nthreads=nthreads)
+        L2 = O.par_op("size", L, args=(-1,), nthreads=nthreads)
+        outer = O.par_op("not", L1, L, nthreads=nthreads)
+        inner = O.par_op("not", L, L2, nthreads=nthreads)
+        and1 = O.par_op("and", outer, L, nthreads=nthreads)
+        ring = O.par_op("and", outer, inner, nthreads=nthreads)
         secs = time.perf_counter() - t0
+        if len(and1) or len(ring):
+            raise SystemExit("CPU baseline OP1 assertion failed")
+        boxes = n + n + (len(L1) + n) + (n + len(L2)) + (len(outer) + n) + (len(outer) + len(inner))
         line["cpu_baseline"] = {"value": boxes / secs, "unit": "boxes/s", "cores": nthreads, "kind": "port",
-                                "sample": f"OP1 ring on the first {args.cpu_sample} boxes (a band of rows) "
-                                          f"of each of the 4 layers, healed first; {secs:.1f} s of CPU work"}
+                                "cpu": cpu_model(),
+                                "sample": f"OP1 ring on the first {n} healed boxes of {LAYERS[k]} "
+                                          f"(a band of rows of its {npl}), "
+                                          f"{secs:.1f} s of CPU work on {nthreads} threads"}
+        print(f"[bench] cpu baseline: {secs:.1f} s", file=sys.stderr, flush=True)
     if args.c2_total:
         # the same ring on the C2 design as 10M boxes in total (2.5M per layer)
         per = args.c2_total // 4
+        prog.close()
         eng.close()
         eng2 = V.Engine(0)
         rings2 = [Ring(V, eng2, lay[:per]) for lay in layers]
         prog2 = V.Program(eng2)
         for r in rings2:
             r.record(V, prog2)
-        tot2, fails2 = 0, 0
         for r in rings2:
-            b_, f_ = r.run_plain(V)
-            tot2 += b_
-            fails2 += f_
-        secs_p, _ = time_steps(lambda: [r.run_plain(V) for r in rings2], eng2, args.steps, args.warmup)
-        secs_b, _ = time_steps(lambda: prog2.run(), eng2, args.steps, args.warmup)
+            r.run_plain(V)
+        tot2 = sum(r.counts()[0] for r in rings2)
+        secs_p, _, _ = time_steps(lambda: [r.run_plain(V) for r in rings2], eng2, args.steps, args.warmup)
+        secs_b, _, _ = time_steps(lambda: prog2.run(), eng2, args.steps, args.warmup)
+        if sum(r.counts()[0] for r in rings2) != tot2:
+            raise SystemExit("c2: the batched form ran different operators")
         line["c2_10M_total"] = {"boxes_per_layer": per, "input_boxes_per_step": int(tot2),
                                 "plain_boxes_per_s": tot2 * args.steps / secs_p,
                                 "batched_boxes_per_s": tot2 * args.steps / secs_b,

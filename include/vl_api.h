@@ -242,12 +242,17 @@ int vl_server_destroy(vl_server* s);
  * with HIP events on the engine stream from its first to its last kernel. */
 int vl_engine_last_time_ms(vl_engine* e, double* ms);
 /* Per-kernel statistics collected while the "profile" parameter is 1 (setting
- * it resets them).  kernel: "tile" (the tile stage: the wave-kernel tiers and
- * the workgroup kernel launched back to back), "bin" (tile binning: span,
- * emit, radix sort, bounds, gather), "sort" (reserved).  *bytes is the
- * algorithmic byte count of those launches (tile: 16 B per entry read + 16 B
- * per strip written + 48 B per tile of descriptor and range tables; bin: 32 B
- * per input box + 24 B per entry for the sort + 32 B per entry gathered). */
+ * it resets them).  Durations come from HIP events on the stream the kernels
+ * run on; they are read at the next query, so collecting them adds no host
+ * waits.  kernel: "tile" (the single-operator tile stage: the wave-kernel
+ * tiers and the workgroup kernel launched back to back), "fused" (the fused
+ * tile kernel's tiers of a program's operator group), "bin" (tile binning:
+ * span, emit, radix sort, bounds, gather), "sort" (the radix passes inside
+ * the binning).  *launches counts kernel launches (sort: digit passes) and
+ * *bytes is their algorithmic byte count (tile and fused: 16 B per entry
+ * read + 16 B per strip written + 48 B per tile of descriptor and range
+ * tables; sort: 16 B per entry per pass; bin: 32 B per input box + 8 B per
+ * entry emitted + the sort + 32 B per entry gathered). */
 int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms,
                            double* bytes);
 

@@ -175,6 +175,33 @@ def run_group(eng, S, steps):
     return outs, fails
 
 
+def oracle_apply(op, a, b, args):
+    """One operator on host boxes through the oracle (canonical strips)."""
+    if op == V.VL_OP_GROW:
+        return O.grow(a, *args)
+    if op == V.K_BOOST_CONTROL_SIZE:
+        return O.size(a, args[0])
+    if op == V.K_BOOST_CONTROL_MERGE:
+        return O.canonical(a)
+    return {V.K_BOOST_CONTROL_AND: O.op_and, V.K_BOOST_CONTROL_NOT: O.op_not, V.K_BOOST_CONTROL_OR: O.op_or,
+            V.K_BOOST_CONTROL_XOR: O.op_xor}[op](a, b)
+
+
+def oracle_group(s, steps):
+    res = []
+    for op, a, b, args in steps:
+        la = s if a == "S" else res[a]
+        lb = None if b is None else (s if b == "S" else res[b])
+        res.append(oracle_apply(op, la, lb, args))
+    return res
+
+
+def check_group_against_oracle(got, s, steps, tag):
+    exp = oracle_group(s, steps)
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), x), (tag, k)
+
+
 def plain_group(S, steps):
     res = []
     for op, a, b, args in steps:
@@ -202,6 +229,7 @@ def test_fused_groups_match_plain(gpu_engine, group, layer):
     exp = plain_group(S, steps)
     for k, (g, x) in enumerate(zip(got, exp)):
         assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes())), (group, layer, k)
+    check_group_against_oracle(got, s, steps, (group, layer))
     assert fails == (1 if len(exp[-1]) else 0)
     if group == "op1":
         assert len(got[4]) == 0 and len(got[5]) == 0
@@ -233,6 +261,7 @@ def test_fused_row_word_borders(gpu_engine, group, n):
     exp = plain_group(S, steps)
     for k, (g, x) in enumerate(zip(got, exp)):
         assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes())), (group, n, k)
+    check_group_against_oracle(got, s, steps, (group, n))
     eng.close()
 
 
@@ -251,6 +280,7 @@ def test_fused_tiers_and_splits(gpu_engine, target):
     exp = plain_group(S, steps)
     for k, (g, x) in enumerate(zip(got, exp)):
         assert np.array_equal(O.sort_boxes(g.boxes()), O.sort_boxes(x.boxes())), (target, k)
+    check_group_against_oracle(got, s, steps, target)
     eng.close()
 
 

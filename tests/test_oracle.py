@@ -133,9 +133,12 @@ def test_identities_medium(seed):
     assert len(O.op_and(outer, inner)) == 0
 
 
-@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built")
 @pytest.mark.parametrize("seed", range(10))
 def test_oracle_against_reference_build(seed):
+    # the reference build is loaded only here, when this test runs (never at
+    # collection, so a GPU-only test run does not map it)
+    if not os.path.exists(O.REF_SO):
+        pytest.skip("oracle/_ref not built")
     rng = np.random.default_rng(1000 + seed)
     a = rand_boxes(rng, int(rng.integers(2, 800)), 3000, 200, off=-1500)
     assert np.array_equal(O.heal(a), O.ref_heal(a))

@@ -148,14 +148,27 @@ struct vl_engine {
     double bytes = 0.0;
   };
   bool prof = false;
-  KStat kst[3];                  // 0 tile, 1 bin, 2 sort
-  hipEvent_t kev0 = nullptr, kev1 = nullptr;
+  KStat kst[4];                  // KS_* below
+  // Event pairs of profiled stages whose durations are read later (at the
+  // next statistics query or the end of a program run), so that collecting
+  // statistics adds no host waits to the stream.
+  struct ProfRec {
+    hipEvent_t a, b;
+    int kind;
+    int64_t launches;
+    double bytes;
+  };
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> ev_free;
 };
 
-enum { KS_TILE = 0, KS_BIN = 1, KS_SORT = 2 };
+enum { KS_TILE = 0, KS_BIN = 1, KS_SORT = 2, KS_FUSED = 3, KS_N = 4 };
 
 struct vl_layer {
   vl_engine* e = nullptr;
+  // Lifetime: the handle and every program that records the layer hold a
+  // reference; the layer (and its engine reference) goes with the last one.
+  std::atomic<int> refs{1};
   vl_box* d = nullptr;
   int64_t n = 0;
   size_t bytes = 0;   // pool-rounded allocation size
@@ -249,6 +262,54 @@ int check_err(vl_engine* e, uint32_t* out) {
 int clear_err(vl_engine* e) {
   launch_fill_u32(e->d_err, 0u, 1, e->st);
   return VL_OK;
+}
+
+// Profiling marks: an event recorded on the engine stream (null when the
+// engine does not collect statistics or no event can be made).
+hipEvent_t prof_mark(vl_engine* e) {
+  if (!e->prof) return nullptr;
+  hipEvent_t ev = nullptr;
+  if (!e->ev_free.empty()) {
+    ev = e->ev_free.back();
+    e->ev_free.pop_back();
+  } else if (hipEventCreate(&ev) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipEventRecord(ev, e->st) != hipSuccess) {
+    (void)hipGetLastError();
+    e->ev_free.push_back(ev);
+    return nullptr;
+  }
+  return ev;
+}
+
+// Queues the stage between marks a and b (its duration is read later).
+void prof_add(vl_engine* e, hipEvent_t a, hipEvent_t b, int kind, int64_t launches, double bytes) {
+  if (!a || !b) {
+    if (a) e->ev_free.push_back(a);
+    if (b) e->ev_free.push_back(b);
+    return;
+  }
+  e->prof_pending.push_back({a, b, kind, launches, bytes});
+}
+
+// Adds the durations of every queued stage to the statistics.
+int prof_resolve(vl_engine* e) {
+  int rc = VL_OK;
+  for (auto& r : e->prof_pending) {
+    float ms = 0.f;
+    if (rc == VL_OK && (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess))
+      rc = fail(VL_EHIP, "profiling events: %s", hipGetErrorString(hipGetLastError()));
+    auto& k = e->kst[r.kind];
+    k.launches += r.launches;
+    k.ms += ms;
+    k.bytes += r.bytes;
+    e->ev_free.push_back(r.a);
+    e->ev_free.push_back(r.b);
+  }
+  e->prof_pending.clear();
+  return rc;
 }
 
 struct Box64 {
@@ -483,7 +544,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     to.ntab = ntiles;
     to.area = d_area;
     to.err = e->d_err;
-    if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
+    hipEvent_t pa = prof_mark(e);
     int rbase = 0;
     for (int t = 0; t <= ntier; t++) {
       to.region_base = rbase;
@@ -496,7 +557,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       rbase += G[t];
     }
     HIPCK(hipGetLastError());
-    if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
+    hipEvent_t pb = prof_mark(e);
     struct {
       unsigned long long cnt[3];
       uint32_t err, nov;
@@ -508,14 +569,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     HIPCK(hipMemcpyAsync((char*)e->h_stage + 32, d_area, 8, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     memcpy(&rb, e->h_stage, sizeof(rb));
-    if (e->prof) {
-      float ms = 0.f;
-      HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
-      auto& k = e->kst[KS_TILE];
-      k.launches += ntier + 1;
-      k.ms += ms;
-      k.bytes += 16.0 * (double)nentries + 16.0 * (double)rb.cnt[0] + 48.0 * ntiles;
-    }
+    prof_add(e, pa, pb, KS_TILE, ntier + 1, 16.0 * (double)nentries + 16.0 * (double)rb.cnt[0] + 48.0 * ntiles);
     total = rb.cnt[0];
     nsideT = rb.cnt[1];
     nov = rb.nov;
@@ -587,28 +641,28 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
   void* rtmp = s2.get<char>(radix_plan((int64_t)total).tmp_bytes);
   vl_box* entries = keep.get<vl_box>(total + 1);
   if (s2.oom || !entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
-  if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
+  hipEvent_t pa = prof_mark(e);
   launch_bin_emit(a, na, xr, b, nb, g, pos, keys, vals, e->st);
   int key_bits = 1;
   while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)(2 * nt)) key_bits++;
+  hipEvent_t ps = prof_mark(e);
   if (radix_sort_pairs_bits32(keys, vals, keys2, vals2, (int64_t)total, key_bits, rtmp, e->st)) {
     std::swap(keys, keys2);
     std::swap(vals, vals2);
   }
+  hipEvent_t pe = prof_mark(e);
   launch_bin_bounds(keys, total, 2 * nt, start, e->st);
   launch_bin_gather(a, na, xs, b, vals, total, entries, e->st);
   HIPCK(hipGetLastError());
-  if (e->prof) {
-    HIPCK(hipEventRecord(e->kev1, e->st));
-    HIPCK(hipEventSynchronize(e->kev1));
-    float ms = 0.f;
-    HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
-    auto& k = e->kst[KS_BIN];
-    k.launches += 1;
-    k.ms += ms;
-    // algorithmic bytes: read the boxes twice (span, emit), write and sort
-    // 8-byte pairs, gather 16 B per entry, write 16 B per entry
-    k.bytes += 32.0 * (double)n + 8.0 * 2.0 * (double)total + 32.0 * (double)total;
+  hipEvent_t pb = prof_mark(e);
+  {
+    // algorithmic bytes: the radix passes read and write 8-byte pairs; the
+    // binning as a whole reads the boxes twice (span, emit), writes the
+    // pairs, sorts them, gathers 16 B per entry and writes 16 B per entry
+    const int passes = (key_bits + 7) / 8;
+    const double sort_bytes = 16.0 * (double)total * passes;
+    prof_add(e, ps, pe, KS_SORT, passes, sort_bytes);
+    prof_add(e, pa, pb, KS_BIN, 1, 32.0 * (double)n + 8.0 * (double)total + sort_bytes + 32.0 * (double)total);
   }
   *entries_out = entries;
   *start_out = start;
@@ -919,7 +973,7 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     }
     fo.ntab = ntiles;
     fo.err = e->d_err;
-    if (e->prof) HIPCK(hipEventRecord(e->kev0, e->st));
+    hipEvent_t pa = prof_mark(e);
     int rbase = 0;
     for (int t = 0; t < NT; t++) {
       fo.region_base = rbase;
@@ -931,7 +985,7 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
       rbase += G[t];
     }
     HIPCK(hipGetLastError());
-    if (e->prof) HIPCK(hipEventRecord(e->kev1, e->st));
+    hipEvent_t pb = prof_mark(e);
     constexpr size_t CB = sizeof(hc);
     HIPCK(hipMemcpyAsync(e->h_stage, cnt, CB, hipMemcpyDeviceToHost, e->st));
     HIPCK(hipMemcpyAsync((char*)e->h_stage + CB, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
@@ -941,15 +995,10 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     memcpy(hc, e->h_stage, CB);
     memcpy(&err, (char*)e->h_stage + CB, 4);
     memcpy(&nov, (char*)e->h_stage + CB + 4, 4);
-    if (e->prof) {
-      float ms = 0.f;
-      HIPCK(hipEventElapsedTime(&ms, e->kev0, e->kev1));
-      auto& k = e->kst[KS_TILE];
-      k.launches += NT;
-      k.ms += ms;
+    {
       double strips = 0.0;
       for (int q = 0; q < nout; q++) strips += (double)hc[3 * q];
-      k.bytes += 16.0 * (double)nentries + 16.0 * strips + 48.0 * ntiles;
+      prof_add(e, pa, pb, KS_FUSED, NT, 16.0 * (double)nentries + 16.0 * strips + 48.0 * ntiles);
     }
     if (err & ERR_OVERFLOW_CAP) {
       if (attempt == 3) return fail(VL_EINTERNAL, "fused output capacity still short after resize");
@@ -1013,20 +1062,14 @@ int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer*
     for (int k = 0; k < fs_in.nout; k++) set_empty(e, c_in[k]);
     return VL_OK;
   }
-  // outputs with the truth table 0 (such as the ring's A AND B checks) are
-  // empty whatever S is: they are not evaluated per tile
-  FusedSpec fs = fs_in;
-  vl_layer* c[FUSE_MAX_OUT] = {};
-  int nout = 0;
-  for (int k = 0; k < fs_in.nout; k++) {
-    if (fs_in.tt[k] == 0u) continue;
-    fs.tt[nout] = fs_in.tt[k];
-    c[nout++] = c_in[k];
-  }
-  fs.nout = nout;
-  for (int k = 0; k < fs_in.nout; k++)
-    if (fs_in.tt[k] == 0u) set_empty(e, c_in[k]);
-  if (nout == 0) return VL_OK;
+  // Every output is evaluated per tile.  Programs never group an operator
+  // whose truth table is 0 (an AND of a morphological difference with its
+  // own source, say): such operators are checks, and they run on the
+  // materialised layers instead.
+  const FusedSpec& fs = fs_in;
+  vl_layer* const* c = c_in;
+  const int nout = fs.nout;
+  if (nout <= 0 || nout > FUSE_MAX_OUT) return fail(VL_EINVAL, "fused group with %d outputs", nout);
   Box64 m;
   VLCK(layer_mbr(e, S, &m));
   // domain: MBR(S) grown by the largest GROW reach on each side (erosion and
@@ -1148,7 +1191,6 @@ static int engine_new(int device, vl_engine** out) {
   e->device = device;
   if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-      hipEventCreate(&e->kev0) != hipSuccess || hipEventCreate(&e->kev1) != hipSuccess ||
       hipMalloc((void**)&e->d_err, 256) != hipSuccess ||
       hipHostMalloc((void**)&e->h_stage, 4096, hipHostMallocDefault) != hipSuccess) {
     delete e;
@@ -1181,8 +1223,8 @@ static void engine_release(vl_engine* e) {
   (void)hipHostFree(e->h_stage);
   (void)hipEventDestroy(e->ev0);
   (void)hipEventDestroy(e->ev1);
-  (void)hipEventDestroy(e->kev0);
-  (void)hipEventDestroy(e->kev1);
+  (void)prof_resolve(e);
+  for (hipEvent_t ev : e->ev_free) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(e->st);
   delete e;
 }
@@ -1210,6 +1252,7 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "level_factor out of range");
     e->level_factor = (int)v;
   } else if (k == "profile") {
+    VLCK(prof_resolve(e));
     e->prof = v != 0;
     for (auto& s : e->kst) s = vl_engine::KStat();
   } else if (k == "max_wg") {
@@ -1252,8 +1295,10 @@ int vl_engine_sync(vl_engine* e) {
 int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms, double* bytes) {
   if (!e || !kernel || !launches || !total_ms || !bytes) return fail(VL_EINVAL, "null argument");
   std::string k(kernel);
-  int i = k == "tile" ? KS_TILE : k == "bin" ? KS_BIN : k == "sort" ? KS_SORT : -1;
+  int i = k == "tile" ? KS_TILE : k == "bin" ? KS_BIN : k == "sort" ? KS_SORT : k == "fused" ? KS_FUSED : -1;
   if (i < 0) return fail(VL_EINVAL, "unknown kernel '%s'", kernel);
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  VLCK(prof_resolve(e));
   *launches = e->kst[i].launches;
   *total_ms = e->kst[i].ms;
   *bytes = e->kst[i].bytes;
@@ -1275,8 +1320,8 @@ int vl_layer_create(vl_engine* e, vl_layer** out) {
   return VL_OK;
 }
 
-int vl_layer_destroy(vl_layer* l) {
-  if (!l) return VL_OK;
+static void layer_release(vl_layer* l) {
+  if (l->refs.fetch_sub(1) != 1) return;
   vl_engine* e = l->e;
   if (l->d) {
     std::lock_guard<std::recursive_mutex> lk(e->mu);
@@ -1284,6 +1329,11 @@ int vl_layer_destroy(vl_layer* l) {
   }
   delete l;
   engine_release(e);
+}
+
+int vl_layer_destroy(vl_layer* l) {
+  if (!l) return VL_OK;
+  layer_release(l);
   return VL_OK;
 }
 
@@ -1678,6 +1728,9 @@ int vl_program_destroy(vl_program* p) {
   if (!p) return VL_OK;
   vl_engine* e = p->e;
   for (vl_engine* w : p->workers) engine_release(w);
+  for (const ProgOp& o : p->ops)
+    for (const vl_layer* l : {o.a, o.b, (const vl_layer*)o.c})
+      if (l) layer_release(const_cast<vl_layer*>(l));
   delete p;
   engine_release(e);
   return VL_OK;
@@ -1689,11 +1742,16 @@ int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, 
   if ((op == VL_OP_XOR || op == VL_OP_OR || op == VL_OP_AND || op == VL_OP_NOT) && !b)
     return fail(VL_EINVAL, "binary operator needs two inputs");
   if (op < -1 || op > VL_OP_GROW) return fail(VL_EINVAL, "unknown op %d", op);
+  for (const vl_layer* l : {a, b, (const vl_layer*)c})
+    if (l && l->e != p->e) return fail(VL_EINVAL, "layer belongs to another engine");
   ProgOp o{op, a, b, c, {0, 0, 0, 0}};
   if (args) {
     const int na = (op == VL_OP_GROW) ? 4 : 1;
     for (int i = 0; i < na; i++) o.args[i] = args[i];
   }
+  // the program keeps every layer it records alive until it is destroyed
+  for (const vl_layer* l : {a, b, (const vl_layer*)c})
+    if (l) const_cast<vl_layer*>(l)->refs.fetch_add(1);
   p->ops.push_back(o);
   return VL_OK;
 }
@@ -1787,6 +1845,14 @@ struct GroupBuilder {
         if (!tt_of(o.a, &ta) || !tt_of(o.b, &tb)) return false;
         tt = o.op == VL_OP_AND ? (ta & tb) : o.op == VL_OP_NOT ? (ta & ~tb) : o.op == VL_OP_OR ? (ta | tb) : (ta ^ tb);
         tt &= 0xFFu;
+        // An operator whose result is empty for every S (the OP1 ring's
+        // ANDs of a ring with its own source) is a consistency check of the
+        // group's outputs: it stays out of the group and runs on the
+        // materialised layers, so the check can fail when they are wrong.
+        if (tt == 0u) {
+          fs = saved;
+          return false;
+        }
         break;
       default:
         fs = saved;
@@ -1962,6 +2028,7 @@ int vl_program_run(vl_program* p) {
       w->fuse_target = e->fuse_target;
       w->region_scale = std::max(w->region_scale, e->region_scale);
       w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
+      (void)prof_resolve(w);
       w->prof = e->prof;
       w->fused_groups = 0;
       for (auto& st : w->kst) st = vl_engine::KStat();
@@ -1988,7 +2055,8 @@ int vl_program_run(vl_program* p) {
       p->failures += fails[k];
       vl_engine* w = p->workers[k];
       e->fused_groups += w->fused_groups;
-      for (int s = 0; s < 3; s++) {
+      if (rc == VL_OK && rcs[k] == VL_OK) rcs[k] = prof_resolve(w);
+      for (int s = 0; s < KS_N; s++) {
         e->kst[s].launches += w->kst[s].launches;
         e->kst[s].ms += w->kst[s].ms;
         e->kst[s].bytes += w->kst[s].bytes;
