@@ -388,25 +388,36 @@ struct SplitTiles {
   uint64_t total = 0;
 };
 
+// Overflow tiles of a launch, sorted by descriptor index (a deterministic
+// order), and their descriptors.
+int overflow_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
+                   std::vector<uint32_t>& ovf, std::vector<TileDesc>& parents) {
+  ovf.resize(novf);
+  parents.resize(novf);
+  HIPCK(hipMemcpyAsync(ovf.data(), d_ovf, novf * sizeof(uint32_t), hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  std::sort(ovf.begin(), ovf.end());
+  Scratch gs(e);
+  uint32_t* d_idx = gs.get<uint32_t>(novf);
+  TileDesc* d_par = gs.get<TileDesc>(novf);
+  if (gs.oom) return fail(VL_ENOMEM, "scratch");
+  HIPCK(hipMemcpyAsync(d_idx, ovf.data(), novf * sizeof(uint32_t), hipMemcpyHostToDevice, e->st));
+  launch_gather_desc(d_desc, d_idx, (int)novf, d_par, e->st);
+  HIPCK(hipMemcpyAsync(parents.data(), d_par, novf * sizeof(TileDesc), hipMemcpyDeviceToHost, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  return VL_OK;
+}
+
+// dense != null: tiles that no x-split can divide (too narrow) are listed
+// there (descriptor indices, with their descriptors in dense_par) instead of
+// failing.
 int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf, const vl_box* d_entries,
-                int32_t halo, int tgt, int cap, int depth, Scratch& s, SplitTiles* out) {
+                int32_t halo, int tgt, int cap, int depth, Scratch& s, SplitTiles* out,
+                std::vector<uint32_t>* dense = nullptr, std::vector<TileDesc>* dense_par = nullptr) {
   if (depth > 24) return fail(VL_EINTERNAL, "tile splitting did not converge");
-  std::vector<uint32_t> ovf(novf);
-  std::vector<TileDesc> parents(novf);
-  {
-    // overflow list -> host, sorted for a deterministic order; parents gathered on the device
-    HIPCK(hipMemcpyAsync(ovf.data(), d_ovf, novf * sizeof(uint32_t), hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
-    std::sort(ovf.begin(), ovf.end());
-    Scratch gs(e);
-    uint32_t* d_idx = gs.get<uint32_t>(novf);
-    TileDesc* d_par = gs.get<TileDesc>(novf);
-    if (gs.oom) return fail(VL_ENOMEM, "scratch");
-    HIPCK(hipMemcpyAsync(d_idx, ovf.data(), novf * sizeof(uint32_t), hipMemcpyHostToDevice, e->st));
-    launch_gather_desc(d_desc, d_idx, (int)novf, d_par, e->st);
-    HIPCK(hipMemcpyAsync(parents.data(), d_par, novf * sizeof(TileDesc), hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
-  }
+  std::vector<uint32_t> ovf;
+  std::vector<TileDesc> parents;
+  VLCK(overflow_tiles(e, d_desc, d_ovf, novf, ovf, parents));
   std::vector<TileDesc> subs;
   std::vector<int32_t> first(novf + 1);
   for (uint32_t i = 0; i < novf; i++) {
@@ -418,9 +429,14 @@ int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uin
     kx = std::min(kx, width);
     // windows overlap by twice the halo: keep sub-columns at least that wide
     if (halo > 0) kx = std::min<int64_t>(kx, width / (2 * (int64_t)halo));
-    if (kx < 2)
-      return fail(VL_EINTERNAL, "a %lld-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
-                  (long long)width, (long long)n, cap);
+    if (kx < 2) {
+      if (!dense)
+        return fail(VL_EINTERNAL, "a %lld-DBU-wide tile holds %lld boxes (more than the tile capacity %d)",
+                    (long long)width, (long long)n, cap);
+      dense->push_back(ovf[i]);
+      dense_par->push_back(p);
+      continue;
+    }
     const int64_t sw = (width + kx - 1) / kx;
     kx = (width + sw - 1) / sw;
     for (int64_t c = 0; c < kx; c++) {
@@ -436,6 +452,9 @@ int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uin
   }
   first[novf] = (int32_t)subs.size();
   const int ns = (int)subs.size();
+  out->n = 0;
+  out->total = 0;
+  if (ns == 0) return VL_OK;
   TileDesc* d_par = s.get<TileDesc>(novf);
   TileDesc* d_sub = s.get<TileDesc>(ns);
   int32_t* d_first = s.get<int32_t>(novf + 1);
@@ -463,13 +482,145 @@ int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uin
   return VL_OK;
 }
 
-// Split overflow tiles into x-sub-columns and process them.
+// Status of an erosion (SIZE(-d)) when a tile exceeds every erosion kernel
+// and no x-split divides it: nothing is written, and SIZE runs as the
+// complement composition instead, whose operators have the dense path.
+constexpr int ERODE_FALLBACK = 1001;
+
+// Tiles above every capacity-bounded kernel (list: descriptor indices in
+// tile order, par: their descriptors): vl_dense.hip, one workgroup per tile,
+// in batches whose scratch stays below a few GiB.  Their strips go to the
+// operator's streams in list order.
+int process_dense(vl_engine* e, const TileDesc* d_desc, const std::vector<uint32_t>& list,
+                  const std::vector<TileDesc>& par, const vl_box* d_entries, bool two, bool area, Outs& o) {
+  const size_t nl = list.size();
+  constexpr int64_t COORD_BUDGET = (int64_t)1 << 27;   // int32 coordinate slots per batch
+  constexpr uint64_t GRID_BUDGET = (uint64_t)1 << 28;  // grid words per launch
+  for (size_t i0 = 0; i0 < nl;) {
+    // batch [i0, i1) by coordinate scratch
+    std::vector<int64_t> coff;
+    int64_t cs_total = 0;
+    size_t i1 = i0;
+    while (i1 < nl) {
+      const int64_t c = dense_coord_slots((int64_t)par[i1].nA + par[i1].nB);
+      if (i1 > i0 && cs_total + c > COORD_BUDGET) break;
+      coff.push_back(cs_total);
+      cs_total += c;
+      i1++;
+    }
+    const int nb = (int)(i1 - i0);
+    Scratch s(e);
+    uint32_t* d_list = s.get<uint32_t>(nb);
+    int64_t* d_coff = s.get<int64_t>(nb);
+    int32_t* d_cs = s.get<int32_t>((size_t)cs_total);
+    int32_t* d_nxy = s.get<int32_t>(2 * (size_t)nb);
+    if (s.oom) return fail(VL_ENOMEM, "dense-tile coordinate scratch (%lld slots)", (long long)cs_total);
+    HIPCK(hipMemcpyAsync(d_list, list.data() + i0, nb * sizeof(uint32_t), hipMemcpyHostToDevice, e->st));
+    HIPCK(hipMemcpyAsync(d_coff, coff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, e->st));
+    launch_dense_coords(d_desc, d_list, nb, d_entries, d_coff, d_cs, d_nxy, e->st);
+    HIPCK(hipGetLastError());
+    std::vector<int32_t> nxy(2 * (size_t)nb);
+    HIPCK(hipMemcpyAsync(nxy.data(), d_nxy, nxy.size() * sizeof(int32_t), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    // launches [j0, j1) of this batch by grid size
+    for (int j0 = 0; j0 < nb;) {
+      std::vector<uint64_t> goff, cofs, soff;
+      uint64_t gw_total = 0, col_total = 0, s_total = 0;
+      int j1 = j0;
+      while (j1 < nb) {
+        const int64_t ncol = nxy[2 * j1] - 1, nrows = nxy[2 * j1 + 1] - 1;
+        const int64_t W = (nrows + 64) >> 6;
+        const uint64_t gw = (uint64_t)ncol * W * (two ? 2 : 1);
+        if (j1 > j0 && gw_total + gw > GRID_BUDGET) break;
+        goff.push_back(gw_total);
+        cofs.push_back(col_total);
+        soff.push_back(s_total);
+        gw_total += gw;
+        col_total += (uint64_t)ncol;
+        s_total += (uint64_t)ncol * (uint64_t)(nrows / 2 + 1);
+        j1++;
+      }
+      const int m = j1 - j0;
+      Scratch t(e);
+      uint64_t* d_goff = t.get<uint64_t>(m);
+      uint64_t* d_cofs = t.get<uint64_t>(m);
+      uint64_t* d_soff = t.get<uint64_t>(m);
+      uint64_t* d_grid = t.get<uint64_t>(gw_total + 1);
+      uint32_t* d_cc = t.get<uint32_t>(col_total + 1);
+      vl_box* d_str = area ? nullptr : t.get<vl_box>(s_total + 1);
+      uint8_t* d_tag = area ? nullptr : t.get<uint8_t>(s_total + 1);
+      uint64_t* d_toff = t.get<uint64_t>(m);
+      uint32_t* d_tn = t.get<uint32_t>(2 * (size_t)m);
+      unsigned long long* d_cnt = t.get<unsigned long long>(2);
+      long long* d_area = t.get<long long>(1);
+      if (t.oom) return fail(VL_ENOMEM, "dense-tile grids (%llu words)", (unsigned long long)gw_total);
+      HIPCK(hipMemcpyAsync(d_goff, goff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
+      HIPCK(hipMemcpyAsync(d_cofs, cofs.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
+      HIPCK(hipMemcpyAsync(d_soff, soff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
+      launch_fill_u64((uint64_t*)d_cnt, 0ull, 2, e->st);
+      launch_fill_u64((uint64_t*)d_area, 0ull, 1, e->st);
+      launch_dense_tile(d_desc, d_list + j0, m, d_entries, d_coff + j0, d_cs, d_nxy + 2 * j0, d_grid, d_goff, d_cc,
+                        d_cofs, d_str, d_tag, d_soff, d_toff, d_tn, two, d_cnt, area ? d_area : nullptr, e->st);
+      HIPCK(hipGetLastError());
+      struct {
+        unsigned long long cnt[2];
+        long long area;
+      } rb;
+      HIPCK(hipMemcpyAsync(e->h_stage, d_cnt, 16, hipMemcpyDeviceToHost, e->st));
+      HIPCK(hipMemcpyAsync((char*)e->h_stage + 16, d_area, 8, hipMemcpyDeviceToHost, e->st));
+      HIPCK(hipStreamSynchronize(e->st));
+      memcpy(&rb, e->h_stage, sizeof(rb));
+      if (area) {
+        o.area += rb.area;
+      } else if (rb.cnt[0] > 0) {
+        const uint64_t nside = rb.cnt[1], nfinal = rb.cnt[0] - nside;
+        VLCK(o.out.ensure(o.nout + nfinal + 1, o.nout));
+        VLCK(o.side.ensure(o.nside + nside + 1, o.nside));
+        VLCK(o.side_tag.ensure(o.nside + nside + 1, o.nside));
+        uint64_t* dst = t.get<uint64_t>(2 * (size_t)m + 2);
+        void* stmp = t.get<char>(scan_tmp_bytes(m));
+        if (t.oom) return fail(VL_ENOMEM, "scratch");
+        scan_u32_to_u64(d_tn, dst, m, nullptr, stmp, e->st);
+        scan_u32_to_u64(d_tn + m, dst + m, m, nullptr, stmp, e->st);
+        launch_tile_reorder(m, d_toff, d_tn, m, dst, dst + m, d_str, d_tag, o.out.p, o.nout, o.side.p, o.side_tag.p,
+                            o.nside, e->st);
+        HIPCK(hipGetLastError());
+        o.nout += nfinal;
+        o.nside += nside;
+      }
+      if (g_debug)
+        fprintf(stderr, "[vl]  dense tiles=%d strips=%llu seam=%llu grid words=%llu\n", m, rb.cnt[0], rb.cnt[1],
+                (unsigned long long)gw_total);
+      j0 = j1;
+    }
+    i0 = i1;
+  }
+  return VL_OK;
+}
+
+// Tiles above the workgroup kernel's capacity.  The first time (depth 0)
+// they are split into x-sub-columns of about the grid's target box count and
+// processed again; a sub-column still above capacity, and a tile too narrow
+// to split, holds more boxes over one x-position than any x-split can
+// separate, and runs on the dense kernel (erosion: ERODE_FALLBACK).
 int split_and_process(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uint32_t novf,
                       const vl_box* d_entries, bool two, bool area, int32_t erode, Outs& o, int depth) {
   Scratch s(e);
+  if (depth >= 1) {
+    if (erode > 0) return ERODE_FALLBACK;
+    std::vector<uint32_t> ovf;
+    std::vector<TileDesc> par;
+    VLCK(overflow_tiles(e, d_desc, d_ovf, novf, ovf, par));
+    return process_dense(e, d_desc, ovf, par, d_entries, two, area, o);
+  }
   SplitTiles sp;
-  VLCK(split_tiles(e, d_desc, d_ovf, novf, d_entries, erode, std::max(8, e->target), TILE_BLOCK_NMAX, depth, s,
-                   &sp));
+  std::vector<uint32_t> dense;
+  std::vector<TileDesc> dpar;
+  const int rc = split_tiles(e, d_desc, d_ovf, novf, d_entries, erode, std::max(8, e->target), TILE_BLOCK_NMAX, depth,
+                             s, &sp, erode > 0 ? nullptr : &dense, &dpar);
+  if (rc == VL_EINTERNAL && erode > 0) return ERODE_FALLBACK;
+  VLCK(rc);
+  if (!dense.empty()) VLCK(process_dense(e, d_desc, dense, dpar, d_entries, two, area, o));
   return process_tiles(e, sp.desc, sp.n, sp.entries, sp.total, two, area, erode, o, depth + 1);
 }
 
@@ -1043,6 +1194,10 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     fprintf(stderr, "\n");
   }
   if (nov > 0) {
+    // a sub-column still above the largest tier holds more boxes over one
+    // x-position than further x-splits separate: the group runs operator by
+    // operator instead (the single-operator path has the dense kernel)
+    if (depth >= 1) return fail(VL_EINTERNAL, "%u fused sub-columns above the largest tier", nov);
     Scratch ss(e);
     SplitTiles sp;
     VLCK(split_tiles(e, d_desc, ovfl[NT - 1], nov, d_entries, fs.halo, FUSE_NMAX_SMALL / 2, FUSE_NMAX_LARGE, depth, ss,
@@ -1453,8 +1608,11 @@ int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c) {
   // Per-tile erosion when the tiles of the operator's grid are at least 4d
   // wide and tall (each box then meets only a few tile windows).
   const Grid g = make_grid(ma, a->n, e->erode_target);
-  if (e->erode_kernel && g.tw >= 4 * (int64_t)r && g.th >= 4 * (int64_t)r)
-    return run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r, &ma);
+  if (e->erode_kernel && g.tw >= 4 * (int64_t)r && g.th >= 4 * (int64_t)r) {
+    const int rc = run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r, &ma);
+    if (rc != ERODE_FALLBACK) return rc;
+    g_err.clear();   // a tile no erosion kernel holds: the composition below
+  }
   vl_box frame{(int32_t)(ma.x1 - 1), (int32_t)(ma.y1 - 1), (int32_t)(ma.x2 + 1), (int32_t)(ma.y2 + 1)};
   void* dframe = nullptr;
   VLCK(upload_tmp(e, s, &frame, sizeof(frame), &dframe));

@@ -231,6 +231,24 @@ void launch_split_scatter(const TileDesc* parents, int np, const int32_t* sub_fi
                           uint32_t* fillA, uint32_t* fillB, const vl_box* entries,
                           vl_box* sub_entries, int32_t halo, hipStream_t st);
 
+// vl_dense.hip — tiles above every capacity-bounded kernel (one workgroup per
+// listed tile, global-memory grids).  Per tile i of the list: coordinate
+// scratch of dense_coord_slots(n_i) int32 at cs + coff[i]; k_dense_coords
+// writes the distinct x / y breakpoint counts to nxy[2i], nxy[2i+1].  Then,
+// with ncol = nx - 1, nrows = ny - 1, W = (nrows + 64) / 64: grid words
+// (two-layer operators: 2 *) ncol * W at grid + goff[i], ncol column
+// counters at colcnt + cofs[i], and at most ncol * (nrows / 2 + 1) strips at
+// strips + soff[i]; tab_off / tab_n as TileOut's, indexed by list position
+// (stride nlist).  Area mode (area non-null) adds the tile areas to *area.
+int64_t dense_coord_slots(int64_t nboxes);
+void launch_dense_coords(const TileDesc* desc, const uint32_t* list, int nlist, const vl_box* entries,
+                         const int64_t* coff, int32_t* cs, int32_t* nxy, hipStream_t st);
+void launch_dense_tile(const TileDesc* desc, const uint32_t* list, int nlist, const vl_box* entries,
+                       const int64_t* coff, const int32_t* cs, const int32_t* nxy, uint64_t* grid,
+                       const uint64_t* goff, uint32_t* colcnt, const uint64_t* cofs, vl_box* strips, uint8_t* tag,
+                       const uint64_t* soff, uint64_t* tab_off, uint32_t* tab_n, bool two,
+                       unsigned long long* cnt, long long* area, hipStream_t st);
+
 // vl_window.hip — window extraction from a cell-indexed layer.  Candidates
 // are the concatenation of the entry ranges [rbeg[r], rbeg[r] + roff[r+1] -
 // roff[r]); each is clipped to w (clip[i]) and flagged when it keeps area.

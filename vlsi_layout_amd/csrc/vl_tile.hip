@@ -961,6 +961,7 @@ __global__ void k_split_count(const TileDesc* __restrict__ parents, const int32_
                               const vl_box* __restrict__ entries, int32_t halo) {
   const TileDesc p = parents[blockIdx.x];
   const int first = sub_first[blockIdx.x], nsub = sub_first[blockIdx.x + 1] - first;
+  if (nsub == 0) return;   // a parent left whole (it goes to the dense kernel)
   const int n = p.nA + p.nB;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const vl_box b = entries[p.off + i];
@@ -978,6 +979,7 @@ __global__ void k_split_scatter(const TileDesc* __restrict__ parents, const int3
                                 int32_t halo) {
   const TileDesc p = parents[blockIdx.x];
   const int first = sub_first[blockIdx.x], nsub = sub_first[blockIdx.x + 1] - first;
+  if (nsub == 0) return;   // a parent left whole (it goes to the dense kernel)
   const int n = p.nA + p.nB;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const vl_box b = entries[p.off + i];
