@@ -173,6 +173,18 @@ void device_event_sort(vl_tracker* d_arr, int64_t n);
 void thrust_sort_boxes_direct(vl_box* d_boxes, int64_t n);
 void thrust_sort_boxes_indirect(vl_box* d_boxes, int64_t* d_indices, int64_t n);
 
+/* No counterpart in the reference: vl_event_sort for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same result as vl_event_sort on device
+ * memory; host memory is rejected (VL_EINVAL).  Its scratch is a static
+ * device array of VL_NOALLOC_SCRATCH_BYTES per device, held until the process
+ * ends, so n is at most VL_NOALLOC_MAX_TRACKERS (VL_EINVAL beyond).  It calls
+ * no allocator.  Every call on a device shares that device's array: calls
+ * that overlap on one device race (calls on one engine are serialised). */
+#define VL_NOALLOC_SCRATCH_BYTES 134217728
+#define VL_NOALLOC_MAX_TRACKERS 3500000
+int vl_event_sort_noalloc(vl_engine* e, vl_tracker* d_arr, int64_t n);
+
 /* ---- batched form ------------------------------------------------------- */
 /* A program is a recorded list of operators over layers (no counterpart in the
  * reference, whose control file runs one operator at a time).  vl_program_run

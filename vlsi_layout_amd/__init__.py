@@ -50,6 +50,9 @@ VL_OK, VL_EINVAL, VL_ENOSPC, VL_ENOMEM, VL_EHIP, VL_EASSERT, VL_EINTERNAL = 0, 1
 
 SYN_UNIFORM, SYN_STDCELL, SYN_TILES, SYN_CHIP = 1, 2, 3, 4
 
+# vl_event_sort_noalloc's static scratch bound (include/vl_api.h)
+VL_NOALLOC_MAX_TRACKERS = 3500000
+
 
 class VLError(RuntimeError):
     def __init__(self, code, msg):
@@ -112,6 +115,7 @@ def lib():
             "vl_boolean_op": (C.c_int, [vp, vp, C.c_uint64, vp, C.c_uint64, vp, C.c_uint64,
                                         C.POINTER(C.c_uint64), C.c_uint64, i64]),
             "vl_event_sort": (C.c_int, [vp, vp, i64]),
+            "vl_event_sort_noalloc": (C.c_int, [vp, vp, i64]),
             "vl_morton_sort_direct": (C.c_int, [vp, vp, i64]),
             "vl_morton_sort_indirect": (C.c_int, [vp, vp, vp, i64]),
             "device_event_sort": (None, [vp, i64]),
@@ -384,6 +388,13 @@ def device_event_sort(trackers, eng=None):
     eng = eng or engine()
     _check(lib().vl_event_sort(eng.handle, t.ctypes.data_as(C.c_void_p), len(t)))
     return t
+
+
+def event_sort_noalloc(dev_ptr, n, eng=None):
+    """vl_event_sort_noalloc on n trackers already in device memory at dev_ptr
+    (sorted in place; no device allocation; n <= VL_NOALLOC_MAX_TRACKERS)."""
+    eng = eng or engine()
+    _check(lib().vl_event_sort_noalloc(eng.handle, C.c_void_p(dev_ptr), int(n)))
 
 
 def thrust_sort_boxes_direct(boxes, eng=None):

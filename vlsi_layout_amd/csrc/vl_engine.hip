@@ -54,6 +54,16 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+int vl::set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
 #define HIPCK(expr)                                                                      \
   do {                                                                                   \
     hipError_t _r = (expr);                                                              \
@@ -1355,6 +1365,8 @@ static int engine_new(int device, vl_engine** out) {
   return VL_OK;
 }
 
+static void engine_release(vl_engine* e);
+
 int vl_engine_create(int device, vl_engine** out) {
   if (!out) return fail(VL_EINVAL, "null out");
   *out = nullptr;
@@ -1363,6 +1375,13 @@ int vl_engine_create(int device, vl_engine** out) {
   if (device < 0 || device >= ndev) return fail(VL_EINVAL, "device %d not present (%d devices)", device, ndev);
   vl_engine* e = nullptr;
   VLCK(engine_new(device, &e));
+  // set-up of the static scratch of vl_event_sort_noalloc on this device
+  // (the device is current after engine_new)
+  const int rs = noalloc_setup(device);
+  if (rs != VL_OK) {
+    engine_release(e);
+    return rs;
+  }
   if (const char* v = getenv("VL_TILE_NMAX")) e->nmax = atoi(v);
   if (e->nmax != 63 && e->nmax != 127) e->nmax = 63;
   *out = e;
@@ -1383,6 +1402,22 @@ static void engine_release(vl_engine* e) {
   (void)hipStreamDestroy(e->st);
   delete e;
 }
+
+vl::EngineLock::~EngineLock() {
+  if (locked_) e_->mu.unlock();
+}
+
+int vl::EngineLock::begin() {
+  if (!locked_) {
+    e_->mu.lock();
+    locked_ = true;
+  }
+  HIPCK(hipSetDevice(e_->device));
+  return VL_OK;
+}
+
+hipStream_t vl::EngineLock::stream() const { return e_->st; }
+int vl::EngineLock::device() const { return e_->device; }
 
 int vl_engine_destroy(vl_engine* e) {
   if (!e) return VL_OK;

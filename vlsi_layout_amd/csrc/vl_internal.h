@@ -61,6 +61,30 @@ enum : uint32_t {
 };
 
 // ---------------------------------------------------------------------------
+// Engine access for entry points outside vl_engine.hip.
+
+// Sets the calling thread's last error text (vl_last_error) and returns code.
+int set_error(int code, const char* fmt, ...);
+
+// Holds an engine's lock for its scope; begin() takes it and makes the
+// engine's device current.
+class EngineLock {
+ public:
+  explicit EngineLock(vl_engine* e) : e_(e) {}
+  ~EngineLock();
+  int begin();
+  hipStream_t stream() const;
+  int device() const;
+
+ private:
+  vl_engine* e_;
+  bool locked_ = false;
+};
+
+// vl_noalloc.hip: the static scratch's device address on `device` (current).
+int noalloc_setup(int device);
+
+// ---------------------------------------------------------------------------
 // Host-side launch wrappers (each .hip file implements its own kernels).
 
 // vl_prims.hip
