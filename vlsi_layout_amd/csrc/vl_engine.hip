@@ -144,6 +144,7 @@ struct vl_engine {
   int max_wg_cap = 0;            // 0 = occupancy limit
   int fuse = 1;                  // programs: fuse operator groups over one source layer
   int fuse_target = 6;           // mean boxes per tile of a fused group's grid
+  int bin_mode = 1;              // 1: atomic binning; 0: stable radix sort of (tile, layer) keys
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   // Strips per binned entry that a workgroup's output region is sized for
   // (plain tile stage / each output of a fused group).  Raised when a region
@@ -784,6 +785,32 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
                 Scratch& keep, vl_box** entries_out, uint64_t** start_out, uint64_t* total_out) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   const int64_t n = na + nb;
+  if (e->bin_mode == 1) {
+    // atomic binning: count per (tile, layer) key, scan, scatter
+    uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt + 1);
+    Scratch s(e);
+    uint32_t* cnt = s.get<uint32_t>(2 * (size_t)nt);
+    void* stmp = s.get<char>(scan_tmp_bytes(2 * (size_t)nt));
+    if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+    hipEvent_t pa = prof_mark(e);
+    launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
+    launch_bin_count(a, na, xr, b, nb, g, cnt, e->st);
+    scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
+    uint64_t total = 0;
+    VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
+    vl_box* entries = keep.get<vl_box>(total + 1);
+    if (!entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
+    launch_bin_scatter(a, na, xr, xs, b, nb, g, start, cnt, entries, e->st);
+    HIPCK(hipGetLastError());
+    hipEvent_t pb = prof_mark(e);
+    // algorithmic bytes: the boxes read twice (count, scatter), 16 B written
+    // per entry, the counters zeroed, scanned and counted down
+    prof_add(e, pa, pb, KS_BIN, 1, 32.0 * (double)n + 16.0 * (double)total + 24.0 * (double)nt);
+    *entries_out = entries;
+    *start_out = start;
+    *total_out = total;
+    return VL_OK;
+  }
   Scratch s(e);
   uint32_t* cnt = s.get<uint32_t>((size_t)n + 1);
   uint64_t* pos = s.get<uint64_t>((size_t)n + 1);
@@ -1454,6 +1481,9 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "fuse_target") {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "fuse_target out of range");
     e->fuse_target = (int)v;
+  } else if (k == "bin_mode") {
+    if (v != 0 && v != 1) return fail(VL_EINVAL, "bin_mode must be 0 or 1");
+    e->bin_mode = (int)v;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
   }
@@ -1471,6 +1501,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "erode_kernel") *v = e->erode_kernel;
   else if (k == "fuse") *v = e->fuse;
   else if (k == "fuse_target") *v = e->fuse_target;
+  else if (k == "bin_mode") *v = e->bin_mode;
   else if (k == "fused_groups") *v = e->fused_groups;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
@@ -2219,6 +2250,7 @@ int vl_program_run(vl_program* p) {
       w->max_wg_cap = e->max_wg_cap;
       w->fuse = e->fuse;
       w->fuse_target = e->fuse_target;
+      w->bin_mode = e->bin_mode;
       w->region_scale = std::max(w->region_scale, e->region_scale);
       w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
       (void)prof_resolve(w);
@@ -2452,6 +2484,7 @@ int vl_server_create(vl_engine* e, const vl_layer* const* layers, int nlayers, i
     w->erode_target = e->erode_target;
     w->level_factor = e->level_factor;
     w->erode_kernel = e->erode_kernel;
+    w->bin_mode = e->bin_mode;
     sv->slots.push_back(w);
   }
   *out = sv;

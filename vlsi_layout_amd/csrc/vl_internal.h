@@ -143,6 +143,14 @@ void launch_bin_emit(const vl_box* a, int64_t na, Xform xra, const vl_box* b, in
 void launch_bin_bounds(const uint32_t* keys, uint64_t total, int64_t nkeys, uint64_t* start, hipStream_t st);
 void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, const uint32_t* vals,
                        uint64_t total, vl_box* entries, hipStream_t st);
+// Atomic binning: cnt[2*tile + layer] (zeroed by the caller) += the boxes
+// meeting each tile; after an exclusive scan of cnt into start, the scatter
+// counts cnt back down to zero while placing every (box, tile) pair in its
+// key's range (entries of a key in no particular order).
+void launch_bin_count(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
+                      hipStream_t st);
+void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const vl_box* b, int64_t nb, Grid g,
+                        const uint64_t* start, uint32_t* cnt, vl_box* entries, hipStream_t st);
 // Tile descriptors from the key bounds (start has 2*ntiles+1 entries).
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st);
 // Tile kernel: result strips of every tile.  Strips touching an internal seam

@@ -127,6 +127,103 @@ __global__ __launch_bounds__(256) void k_bin_gather(const vl_box* __restrict__ a
   }
 }
 
+// Atomic binning (the default): a count pass and a scatter pass over the
+// boxes in input order.  The (tile, layer) key of every (box, tile) pair is
+// 2*tile + layer; lanes of a wave whose keys are equal and adjacent form a
+// segment, and only the segment's first lane touches the key's counter (one
+// atomic per segment: inputs produced by earlier operators are in tile order,
+// so neighbouring boxes usually share a tile).  The scatter pass takes slots
+// from the back of each key's range by counting the counters down to zero.
+// Entries of one tile end up in an order that depends on scheduling; every
+// tile kernel's result depends only on the set of its entries, so results
+// and their order do not.
+struct Seg {
+  int head;   // lane of this lane's segment head
+  int rank;   // position in the segment
+  int len;    // segment length (valid in the head lane)
+  bool is_head;
+};
+
+__device__ __forceinline__ Seg segment_of(bool act, uint32_t key, int lane) {
+  const uint32_t pk = (uint32_t)__shfl_up((int)key, 1, WAVE);
+  const int pa = __shfl_up((int)act, 1, WAVE);
+  Seg s;
+  s.is_head = act && (lane == 0 || !pa || pk != key);
+  const uint64_t hm = __ballot(s.is_head);
+  const uint64_t am = __ballot(act);
+  const uint64_t upto = (2ull << lane) - 1ull;   // lanes 0..lane (lane 63: all)
+  const uint64_t below = hm & upto;
+  s.head = below ? 63 - __clzll((long long)below) : lane;
+  s.rank = lane - s.head;
+  const uint64_t stop = (hm | ~am) & ~upto;      // first head or inactive lane above
+  s.len = (stop ? __ffsll((unsigned long long)stop) - 1 : WAVE) - lane;
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const vl_box* __restrict__ a, int64_t na, Xform xra,
+                                                   const vl_box* __restrict__ b, int64_t nb, Grid g,
+                                                   uint32_t* __restrict__ cnt) {
+  const int64_t n = na + nb;
+  const int lane = __lane_id();
+  const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); w0 < n; w0 += wstride) {
+    const int64_t i = w0 + lane;
+    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
+    const bool isA = i < na;
+    if (i < n) {
+      const vl_box v = isA ? a[i] : b[i - na];
+      if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) { c2 = c1 - 1; r2 = r1 - 1; }
+    }
+    const int wc = c2 - c1 + 1, m = (c2 >= c1 && r2 >= r1) ? wc * (r2 - r1 + 1) : 0;
+    const uint32_t layer = isA ? 0u : 1u;
+    for (int j = 0; __ballot(j < m); j++) {
+      const bool act = j < m;
+      uint32_t key = 0;
+      if (act) {
+        const int r = r1 + j / wc, c = c1 + j % wc;
+        key = (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
+      }
+      const Seg sg = segment_of(act, key, lane);
+      if (sg.is_head) atomicAdd(&cnt[key], (uint32_t)sg.len);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ a, int64_t na, Xform xra, Xform xsa,
+                                                     const vl_box* __restrict__ b, int64_t nb, Grid g,
+                                                     const uint64_t* __restrict__ start, uint32_t* __restrict__ cnt,
+                                                     vl_box* __restrict__ entries) {
+  const int64_t n = na + nb;
+  const int lane = __lane_id();
+  const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); w0 < n; w0 += wstride) {
+    const int64_t i = w0 + lane;
+    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
+    const bool isA = i < na;
+    vl_box v{0, 0, 0, 0};
+    if (i < n) {
+      v = isA ? a[i] : b[i - na];
+      if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) { c2 = c1 - 1; r2 = r1 - 1; }
+    }
+    const vl_box stored = isA ? xform(v, xsa) : v;
+    const int wc = c2 - c1 + 1, m = (c2 >= c1 && r2 >= r1) ? wc * (r2 - r1 + 1) : 0;
+    const uint32_t layer = isA ? 0u : 1u;
+    for (int j = 0; __ballot(j < m); j++) {
+      const bool act = j < m;
+      uint32_t key = 0;
+      if (act) {
+        const int r = r1 + j / wc, c = c1 + j % wc;
+        key = (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
+      }
+      const Seg sg = segment_of(act, key, lane);
+      uint32_t old = 0;
+      if (sg.is_head) old = atomicSub(&cnt[key], (uint32_t)sg.len);
+      old = (uint32_t)__shfl((int)old, sg.head, WAVE);
+      if (act) entries[start[key] + old - 1u - (uint32_t)sg.rank] = stored;
+    }
+  }
+}
+
 __global__ void k_build_desc(Grid g, const uint64_t* __restrict__ start, int32_t op, TileDesc* desc) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
@@ -1048,6 +1145,17 @@ void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, 
   if (total > 0)
     hipLaunchKernelGGL(k_bin_gather, dim3(grid_for((int64_t)total)), dim3(256), 0, st, a, na, xsa, b, vals, total,
                        entries);
+}
+void launch_bin_count(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
+                      hipStream_t st) {
+  if (na + nb > 0)
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, b, nb, g, cnt);
+}
+void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const vl_box* b, int64_t nb, Grid g,
+                        const uint64_t* start, uint32_t* cnt, vl_box* entries, hipStream_t st) {
+  if (na + nb > 0)
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, xsa, b, nb, g, start,
+                       cnt, entries);
 }
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st) {
   int64_t nt = (int64_t)g.cols * g.rows;
