@@ -16,15 +16,22 @@ SIZE and the two NOTs of a layer are one fused group (the layer binned once,
 the four outputs computed per tile from shared masks), the two ANDs run on the
 materialised outer / inner layers (so the ASSERT_ZEROs check what the fused
 group produced), and the four layer chains run concurrently on worker
-streams.  Both forms evaluate every operator on the GPU and are timed with
-the same settings; the box count of each form is taken from its own run.
+streams.  "graph" (the headline) is that program captured as a HIP graph
+(vl_program_set_graph): the first warm-up run records the sizes, captures
+the work and launches it, and the timed steps are replays of the graph
+(no host round trips; every replay checks its sizes on the device).  All
+forms evaluate every operator on the GPU and are timed with the same
+settings; the box count of each form is taken from its own run.
 
 The roofline figure is for the kernel stage with the largest device time in
 the timed step: algorithmic bytes of its launches (tile / fused stage: 16 B
 per entry read, 16 B per strip written, 48 B per tile; sort: 16 B per entry
 per pass) / their summed duration from HIP events on the stream they run on,
-against the MI355X's 8 TB/s HBM peak.  "compulsory_io_frac" is 16 B per input
-and output box of every operator / step time, against the same peak.
+against the MI355X's 8 TB/s HBM peak; for the graph form they are read from
+the eager batched form's timed steps (the same kernels and sizes: events
+recorded by graph replays give no usable elapsed times).
+"compulsory_io_frac" is 16 B per input and output box of every operator /
+step time, against the same peak.
 
 cpu_baseline: the same OP1 ring on the first 2M healed boxes of met1 (a band
 of rows), run by the C/OpenMP restatement of the reference CPU path
@@ -63,7 +70,11 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=2_000_000,
                    help="healed boxes of met1 (a band of rows) in the CPU baseline's ring")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--mode", choices=["plain", "batched", "both"], default="both")
+    p.add_argument("--mode", choices=["plain", "batched", "graph", "all"], default="all",
+                   help="plain: one call per operator; batched: the recorded program run eagerly; "
+                        "graph: the same program captured as a HIP graph and replayed (the headline)")
+    p.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                   help="engine parameter for every form (e.g. the autotuned values, with --no-autotune)")
     p.add_argument("--no-autotune", action="store_true", help="batched form with the default tile budgets")
     p.add_argument("--c2-total", type=int, default=10_000_000,
                    help="also time the same ring at this many boxes over the 4 layers (0 = skip)")
@@ -237,6 +248,9 @@ def main():
 
     print(f"[bench] generated 4 x {npl} boxes in {t_gen:.1f} s", file=sys.stderr, flush=True)
     eng = V.Engine(0)
+    for kv in args.set:
+        k, v = kv.split("=")
+        eng.set(k, int(v))
     rings = [Ring(V, eng, lay) for lay in layers]
     if args.calibrate_copy_mib:
         cal = V.Layer(eng).copy_from(rings[0].L, args.calibrate_copy_mib * (1 << 20) // 16)
@@ -266,22 +280,29 @@ def main():
         prog.run()
         return prog.assert_failures()
 
-    if plain_step():
-        raise SystemExit("OP1 ring assertion failed")
+    modes = ["plain", "batched", "graph"] if args.mode == "all" else [args.mode]
+    if modes[0] == "plain":
+        if plain_step():
+            raise SystemExit("OP1 ring assertion failed")
+    else:
+        if batched_step():
+            raise SystemExit("OP1 ring assertion failed")
     plain_counts = step_counts()
     print(f"[bench] first step ok: {plain_counts[0]} input boxes per step", file=sys.stderr, flush=True)
 
-    modes = ["plain", "batched"] if args.mode == "both" else [args.mode]
     tuned = None
     for m in modes:
-        if m == "batched" and not args.no_autotune:
+        if m == "graph":
+            prog.set_graph(True)     # the first warm-up run records, captures and launches the graph
+        if m in ("batched", "graph") and tuned is None and not args.no_autotune:
             # the batched form runs with autotuned tile budgets (timed on met1's ring:
             # met1 NOT SIZE(met1, -1), GROW, SIZE and the fused ring of met1)
             tuned = eng.autotune(rings[2].L, rings[2].L2)
             print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
         # Both forms are timed the same way: per-stage statistics on, read
         # after the timed region (their HIP events add no waits).
-        secs, fails, stats = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps, args.warmup)
+        secs, fails, stats = time_steps(plain_step if m == "plain" else batched_step, eng, args.steps,
+                                        max(args.warmup, 1 if m == "graph" else 0))
         if fails:
             raise SystemExit(f"OP1 ring assertion failed in {m} mode")
         ins, outs = step_counts()
@@ -296,10 +317,20 @@ def main():
             "compulsory_io_gbs": 16.0 * (ins + outs) / (ms * 1e-3) / 1e9,
             "stages": stats,
         }
+        if m == "graph":
+            gs = prog.graph_stats()
+            result[m]["graph"] = gs
+            if gs["fallbacks"] or gs["replays"] < args.steps:
+                raise SystemExit(f"graph form: the timed steps were not all replays ({gs})")
         print(f"[bench] {m}: {ms:.1f} ms/step", file=sys.stderr, flush=True)
 
-    head = "batched" if "batched" in result else "plain"
-    hs = result[head]["stages"]
+    head = "graph" if "graph" in result else "batched" if "batched" in result else "plain"
+    # Stage timings cannot be taken inside graph replays (events recorded by
+    # a replay give no usable elapsed times): the graph form's roofline is
+    # read from the eager batched form's timed steps, which launch the same
+    # kernels with the same sizes.
+    rform = "batched" if head == "graph" and "batched" in result else head
+    hs = result[rform]["stages"]
     # the roofline names the kernel stage with the largest device time in the
     # timed step (the fused kernel in the batched form, the tile stage in the
     # plain one); its achieved rate = algorithmic bytes / summed duration
@@ -328,10 +359,11 @@ def main():
                    "input_boxes_per_step": int(plain_counts[0])},
         "plain": result.get("plain"),
         "batched": result.get("batched"),
+        "graph": result.get("graph"),
         "autotuned": tuned,
-        "batched_over_plain": (result["batched"]["boxes_per_s"] / result["plain"]["boxes_per_s"]
-                               if "batched" in result and "plain" in result else None),
-        "roofline": {"bound": "hbm", "kernel": STAGES[dom], "stage": dom, "form": head,
+        "batched_over_plain": (result[head]["boxes_per_s"] / result["plain"]["boxes_per_s"]
+                               if head != "plain" and "plain" in result else None),
+        "roofline": {"bound": "hbm", "kernel": STAGES[dom], "stage": dom, "form": rform,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes_per_launch": per_launch, "traffic": traffic,
                      "compulsory_io_frac": result[head]["compulsory_io_gbs"] / HBM_PEAK_GBS},
@@ -380,15 +412,20 @@ def main():
         tot2 = sum(r.counts()[0] for r in rings2)
         secs_p, _, _ = time_steps(lambda: [r.run_plain(V) for r in rings2], eng2, args.steps, args.warmup)
         secs_b, _, _ = time_steps(lambda: prog2.run(), eng2, args.steps, args.warmup)
+        prog2.set_graph(True)
+        secs_g, _, _ = time_steps(lambda: prog2.run(), eng2, args.steps, max(args.warmup, 1))
         if sum(r.counts()[0] for r in rings2) != tot2:
             raise SystemExit("c2: the batched form ran different operators")
         line["c2_10M_total"] = {"boxes_per_layer": per, "input_boxes_per_step": int(tot2),
                                 "plain_boxes_per_s": tot2 * args.steps / secs_p,
                                 "batched_boxes_per_s": tot2 * args.steps / secs_b,
                                 "plain_ms_per_step": 1000.0 * secs_p / args.steps,
-                                "batched_ms_per_step": 1000.0 * secs_b / args.steps}
+                                "batched_ms_per_step": 1000.0 * secs_b / args.steps,
+                                "graph_boxes_per_s": tot2 * args.steps / secs_g,
+                                "graph_ms_per_step": 1000.0 * secs_g / args.steps}
         print(f"[bench] c2 (2.5M/layer): plain {line['c2_10M_total']['plain_ms_per_step']:.1f} ms, "
-              f"batched {line['c2_10M_total']['batched_ms_per_step']:.1f} ms", file=sys.stderr, flush=True)
+              f"batched {line['c2_10M_total']['batched_ms_per_step']:.1f} ms, "
+              f"graph {line['c2_10M_total']['graph_ms_per_step']:.1f} ms", file=sys.stderr, flush=True)
         prog2.close()
         eng2.close()
     if args.serving_boxes:

@@ -216,6 +216,22 @@ int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, 
 int vl_program_run(vl_program* p);
 /* Number of ASSERT_ZERO checks of the last run that found shapes. */
 int vl_program_assert_count(vl_program* p, int64_t* failures);
+/* Graph mode (no counterpart in the reference; its control file runs one
+ * operator at a time, src/controlparser.f90:308-309 notes that a graph could
+ * be built).  With enable != 0 the next vl_program_run runs the program
+ * eagerly while logging every size the host reads back, captures the same
+ * work on the program's worker streams into a HIP graph (no host waits;
+ * each logged size is checked on the device), and launches it; later runs
+ * replay the graph.  A replay is used only while every layer the program
+ * reads or writes still has the version the graph was captured with (any
+ * other write to one of them, or a new upload, makes the next run capture
+ * again), and a replay whose device checks fail is recomputed eagerly.
+ * Layers written by the program keep their storage across replays.  Results
+ * are the same as vl_program_run without graph mode. */
+int vl_program_set_graph(vl_program* p, int enable);
+/* Captures made, replays used, and replays or captures that fell back to
+ * eager execution, over the program's life. */
+int vl_program_graph_stats(vl_program* p, int64_t* captures, int64_t* replays, int64_t* fallbacks);
 
 /* ---- serving: windowed queries against resident layers --------------------- */
 /* No counterpart in the reference (its control file runs one operator at a

@@ -29,8 +29,8 @@ def test_engine_destroyed_before_its_layers_and_programs(gpu_engine):
 
 
 def test_program_keeps_dropped_layers_alive(gpu_engine):
-    """Layers passed to Program.add as temporaries and garbage-collected
-    before run() stay valid: the program holds a reference to each."""
+    """Layers recorded by a program and destroyed by the caller before run()
+    stay valid: the program holds a reference to each."""
     eng = V.Engine(0)
     rng = np.random.default_rng(5)
     a = rng.integers(0, 5000, (800, 2))
@@ -39,7 +39,11 @@ def test_program_keeps_dropped_layers_alive(gpu_engine):
     b = np.concatenate([b, b + rng.integers(1, 300, (800, 2))], 1).astype(np.int32)
     out = V.Layer(eng)
     prog = V.Program(eng)
-    prog.add(V.K_BOOST_CONTROL_NOT, V.Layer(eng, a), V.Layer(eng, b), out)
+    A, B = V.Layer(eng, a), V.Layer(eng, b)
+    prog.add(V.K_BOOST_CONTROL_NOT, A, B, out)
+    A.close()          # the handles go; the program's own references keep the layers
+    B.close()
+    del A, B
     gc.collect()
     for _ in range(2):
         prog.run()

@@ -126,6 +126,8 @@ def lib():
             "vl_program_add": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
             "vl_program_run": (C.c_int, [vp]),
             "vl_program_assert_count": (C.c_int, [vp, C.POINTER(i64)]),
+            "vl_program_set_graph": (C.c_int, [vp, C.c_int]),
+            "vl_program_graph_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
             "vl_engine_autotune": (C.c_int, [vp, vp, vp, C.POINTER(i64), C.POINTER(i64)]),
             "vl_server_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
             "vl_server_run": (C.c_int, [vp, vp, i64, vp]),
@@ -432,6 +434,9 @@ class Program:
             self._keep.append(arr)
         _check(lib().vl_program_add(self._h, int(op), a.handle, b.handle if b is not None else None,
                                     c.handle if c is not None else None, arr))
+        # the library holds its own reference to each layer; the Python
+        # objects are kept too, so their handles stay usable by the caller
+        self._keep.extend(x for x in (a, b, c) if x is not None)
 
     def run(self):
         _check(lib().vl_program_run(self._h))
@@ -440,6 +445,16 @@ class Program:
         v = C.c_int64()
         _check(lib().vl_program_assert_count(self._h, C.byref(v)))
         return v.value
+
+    def set_graph(self, enable=True):
+        """Capture the program as a HIP graph on its next run and replay it afterwards."""
+        _check(lib().vl_program_set_graph(self._h, 1 if enable else 0))
+
+    def graph_stats(self):
+        """{'captures', 'replays', 'fallbacks'} over the program's life."""
+        c, r, f = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().vl_program_graph_stats(self._h, C.byref(c), C.byref(r), C.byref(f)))
+        return {"captures": c.value, "replays": r.value, "fallbacks": f.value}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -81,6 +81,7 @@ int vl::set_error(int code, const char* fmt, ...) {
 struct Pool {
   std::map<size_t, std::vector<void*>> free_;
   size_t live = 0;
+  bool no_trim = false;
   static size_t round(size_t b) {
     size_t r = 256;
     while (r < b) r <<= 1;
@@ -98,6 +99,7 @@ struct Pool {
     void* p = nullptr;
     if (hipMalloc(&p, r) != hipSuccess) {
       (void)hipGetLastError();
+      if (no_trim) return nullptr;   // buffers may be referenced by a graph being captured
       trim();
       if (hipMalloc(&p, r) != hipSuccess) {
         (void)hipGetLastError();
@@ -171,7 +173,21 @@ struct vl_engine {
   };
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> ev_free;
+  // Graph capture of programs.  Record mode runs eagerly and logs every
+  // value the host reads back from the device; capture mode replays that
+  // log instead of reading (no host waits, so the work can be captured into
+  // a HIP graph), enqueues a check of each logged size against the device
+  // value (d_err[4] is set on a mismatch), and keeps the host sources of its
+  // uploads alive for the graph.
+  struct Capture {
+    int mode = 0;   // CAP_EAGER, CAP_RECORD, CAP_CAPTURE
+    std::vector<std::vector<uint8_t>> log;
+    size_t pos = 0;
+    std::vector<void*> pinned;
+  } cap;
 };
+
+enum { CAP_EAGER = 0, CAP_RECORD = 1, CAP_CAPTURE = 2 };
 
 enum { KS_TILE = 0, KS_BIN = 1, KS_SORT = 2, KS_FUSED = 3, KS_N = 4 };
 
@@ -183,6 +199,9 @@ struct vl_layer {
   vl_box* d = nullptr;
   int64_t n = 0;
   size_t bytes = 0;   // pool-rounded allocation size
+  // Bumped by every write (captured programs replay only while the versions
+  // of the layers they read and write are the ones they were captured with).
+  uint64_t version = 0;
   // Cached bounding box of the boxes (valid while mbr_ok; every write to the
   // layer clears it).  Operators read it instead of reducing the boxes again.
   mutable bool mbr_ok = false;
@@ -256,10 +275,87 @@ struct DVec {
   }
 };
 
+// Host reads of device values.  Each segment is copied to its host address;
+// `check`: in capture mode, a replay compares the device value with the
+// logged one (segments whose value may differ between correct runs, such as
+// the largest region fill, are not checked).  One stream wait per call.
+struct RB {
+  void* host;
+  const void* dev;
+  size_t bytes;
+  bool check;
+};
+
+uint32_t* cap_flag(vl_engine* e) { return e->d_err + 4; }
+
+// Status of a capture whose host decisions left the recorded path (the run
+// falls back to eager execution).
+constexpr int CAPTURE_DIVERGED = 1002;
+
+int sync_read(vl_engine* e, const RB* r, int n) {
+  size_t total = 0;
+  for (int i = 0; i < n; i++) total += r[i].bytes;
+  if (e->cap.mode == CAP_CAPTURE) {
+    if (e->cap.pos >= e->cap.log.size() || e->cap.log[e->cap.pos].size() != total)
+      return fail(CAPTURE_DIVERGED, "graph capture diverged from the recorded run");
+    const std::vector<uint8_t>& v = e->cap.log[e->cap.pos++];
+    size_t o = 0;
+    for (int i = 0; i < n; i++) {
+      memcpy(r[i].host, v.data() + o, r[i].bytes);
+      if (r[i].check) launch_expect(r[i].dev, v.data() + o, r[i].bytes, cap_flag(e), e->st);
+      o += r[i].bytes;
+    }
+    HIPCK(hipGetLastError());
+    return VL_OK;
+  }
+  if (total <= 4096) {
+    size_t o = 0;
+    for (int i = 0; i < n; i++) {
+      if (r[i].bytes) HIPCK(hipMemcpyAsync((char*)e->h_stage + o, r[i].dev, r[i].bytes, hipMemcpyDeviceToHost, e->st));
+      o += r[i].bytes;
+    }
+    HIPCK(hipStreamSynchronize(e->st));
+    o = 0;
+    for (int i = 0; i < n; i++) {
+      memcpy(r[i].host, (char*)e->h_stage + o, r[i].bytes);
+      o += r[i].bytes;
+    }
+  } else {
+    for (int i = 0; i < n; i++)
+      if (r[i].bytes) HIPCK(hipMemcpyAsync(r[i].host, r[i].dev, r[i].bytes, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+  }
+  if (e->cap.mode == CAP_RECORD) {
+    std::vector<uint8_t> v(total);
+    size_t o = 0;
+    for (int i = 0; i < n; i++) {
+      memcpy(v.data() + o, r[i].host, r[i].bytes);
+      o += r[i].bytes;
+    }
+    e->cap.log.push_back(std::move(v));
+  }
+  return VL_OK;
+}
+
 int readback(vl_engine* e, void* host, const void* dev, size_t bytes) {
-  HIPCK(hipMemcpyAsync(e->h_stage, dev, bytes, hipMemcpyDeviceToHost, e->st));
-  HIPCK(hipStreamSynchronize(e->st));
-  memcpy(host, e->h_stage, bytes);
+  RB r{host, dev, bytes, true};
+  return sync_read(e, &r, 1);
+}
+
+// Host-to-device copy of a small host array.  While capturing, the source is
+// kept in pinned memory owned by the capture (the graph copies from it on
+// every replay).
+int upload(vl_engine* e, void* dev, const void* host, size_t bytes) {
+  if (!bytes) return VL_OK;
+  if (e->cap.mode == CAP_CAPTURE) {
+    void* p = nullptr;
+    HIPCK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    e->cap.pinned.push_back(p);
+    memcpy(p, host, bytes);
+    HIPCK(hipMemcpyAsync(dev, p, bytes, hipMemcpyHostToDevice, e->st));
+    return VL_OK;
+  }
+  HIPCK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, e->st));
   return VL_OK;
 }
 
@@ -278,7 +374,9 @@ int clear_err(vl_engine* e) {
 // Profiling marks: an event recorded on the engine stream (null when the
 // engine does not collect statistics or no event can be made).
 hipEvent_t prof_mark(vl_engine* e) {
-  if (!e->prof) return nullptr;
+  // none inside a graph capture: events recorded by graph replays do not
+  // give usable elapsed times
+  if (!e->prof || e->cap.mode == CAP_CAPTURE) return nullptr;
   hipEvent_t ev = nullptr;
   if (!e->ev_free.empty()) {
     ev = e->ev_free.back();
@@ -333,19 +431,18 @@ int mbr2(vl_engine* e, const vl_box* a, int64_t na, Xform xa, const vl_box* b, i
   Scratch s(e);
   int32_t* d = s.get<int32_t>(8);
   if (!d) return fail(VL_ENOMEM, "scratch");
-  int32_t init[8] = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
-  memcpy(e->h_stage, init, sizeof(init));
-  HIPCK(hipMemcpyAsync(d, e->h_stage, sizeof(init), hipMemcpyHostToDevice, e->st));
+  // (INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN) twice, by fill kernels
+  for (int k = 0; k < 8; k += 4) {
+    launch_fill_u32((uint32_t*)d + k, (uint32_t)INT32_MAX, 2, e->st);
+    launch_fill_u32((uint32_t*)d + k + 2, (uint32_t)INT32_MIN, 2, e->st);
+  }
   VLCK(clear_err(e));
   launch_mbr(a, na, xa, d, e->d_err, e->st);
   launch_mbr(b, nb, Xform{0, 0, 0, 0}, d + 4, e->d_err, e->st);
   int32_t h[8];
   uint32_t err = 0;
-  HIPCK(hipMemcpyAsync(e->h_stage, d, sizeof(h), hipMemcpyDeviceToHost, e->st));
-  HIPCK(hipMemcpyAsync((char*)e->h_stage + 64, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-  HIPCK(hipStreamSynchronize(e->st));
-  memcpy(h, e->h_stage, sizeof(h));
-  memcpy(&err, (char*)e->h_stage + 64, 4);
+  const RB r[2] = {{h, d, sizeof(h), true}, {&err, e->d_err, 4, true}};
+  VLCK(sync_read(e, r, 2));
   if (err & ERR_INVALID_BOX)
     return fail(VL_EINVAL, "invalid box: input has x1>=x2 or y1>=y2, or a transformed box became invalid");
   *ma = Box64{h[0], h[1], h[2], h[3]};
@@ -405,17 +502,19 @@ int overflow_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, 
                    std::vector<uint32_t>& ovf, std::vector<TileDesc>& parents) {
   ovf.resize(novf);
   parents.resize(novf);
-  HIPCK(hipMemcpyAsync(ovf.data(), d_ovf, novf * sizeof(uint32_t), hipMemcpyDeviceToHost, e->st));
-  HIPCK(hipStreamSynchronize(e->st));
+  // the list's order depends on scheduling: not checked on replays (its
+  // length is, where the tile stage reads it)
+  RB r{ovf.data(), d_ovf, novf * sizeof(uint32_t), false};
+  VLCK(sync_read(e, &r, 1));
   std::sort(ovf.begin(), ovf.end());
   Scratch gs(e);
   uint32_t* d_idx = gs.get<uint32_t>(novf);
   TileDesc* d_par = gs.get<TileDesc>(novf);
   if (gs.oom) return fail(VL_ENOMEM, "scratch");
-  HIPCK(hipMemcpyAsync(d_idx, ovf.data(), novf * sizeof(uint32_t), hipMemcpyHostToDevice, e->st));
+  VLCK(upload(e, d_idx, ovf.data(), novf * sizeof(uint32_t)));
   launch_gather_desc(d_desc, d_idx, (int)novf, d_par, e->st);
-  HIPCK(hipMemcpyAsync(parents.data(), d_par, novf * sizeof(TileDesc), hipMemcpyDeviceToHost, e->st));
-  HIPCK(hipStreamSynchronize(e->st));
+  RB rp{parents.data(), d_par, novf * sizeof(TileDesc), false};
+  VLCK(sync_read(e, &rp, 1));
   return VL_OK;
 }
 
@@ -473,9 +572,9 @@ int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uin
   uint64_t* off = s.get<uint64_t>(ns + 1);
   void* stmp = s.get<char>(scan_tmp_bytes(ns));
   if (s.oom) return fail(VL_ENOMEM, "scratch");
-  HIPCK(hipMemcpyAsync(d_par, parents.data(), novf * sizeof(TileDesc), hipMemcpyHostToDevice, e->st));
-  HIPCK(hipMemcpyAsync(d_sub, subs.data(), ns * sizeof(TileDesc), hipMemcpyHostToDevice, e->st));
-  HIPCK(hipMemcpyAsync(d_first, first.data(), (novf + 1) * sizeof(int32_t), hipMemcpyHostToDevice, e->st));
+  VLCK(upload(e, d_par, parents.data(), novf * sizeof(TileDesc)));
+  VLCK(upload(e, d_sub, subs.data(), ns * sizeof(TileDesc)));
+  VLCK(upload(e, d_first, first.data(), (novf + 1) * sizeof(int32_t)));
   uint32_t *cA = cnt, *cB = cnt + ns, *fA = cnt + 2 * ns, *fB = cnt + 3 * ns;
   launch_fill_u32(cnt, 0u, 4 * (size_t)ns, e->st);
   launch_split_count(d_par, (int)novf, d_first, d_sub, cA, cB, d_entries, halo, e->st);
@@ -526,13 +625,15 @@ int process_dense(vl_engine* e, const TileDesc* d_desc, const std::vector<uint32
     int32_t* d_cs = s.get<int32_t>((size_t)cs_total);
     int32_t* d_nxy = s.get<int32_t>(2 * (size_t)nb);
     if (s.oom) return fail(VL_ENOMEM, "dense-tile coordinate scratch (%lld slots)", (long long)cs_total);
-    HIPCK(hipMemcpyAsync(d_list, list.data() + i0, nb * sizeof(uint32_t), hipMemcpyHostToDevice, e->st));
-    HIPCK(hipMemcpyAsync(d_coff, coff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, e->st));
+    VLCK(upload(e, d_list, list.data() + i0, nb * sizeof(uint32_t)));
+    VLCK(upload(e, d_coff, coff.data(), nb * sizeof(int64_t)));
     launch_dense_coords(d_desc, d_list, nb, d_entries, d_coff, d_cs, d_nxy, e->st);
     HIPCK(hipGetLastError());
     std::vector<int32_t> nxy(2 * (size_t)nb);
-    HIPCK(hipMemcpyAsync(nxy.data(), d_nxy, nxy.size() * sizeof(int32_t), hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
+    {
+      RB r{nxy.data(), d_nxy, nxy.size() * sizeof(int32_t), nxy.size() <= 64};
+      VLCK(sync_read(e, &r, 1));
+    }
     // launches [j0, j1) of this batch by grid size
     for (int j0 = 0; j0 < nb;) {
       std::vector<uint64_t> goff, cofs, soff;
@@ -565,9 +666,9 @@ int process_dense(vl_engine* e, const TileDesc* d_desc, const std::vector<uint32
       unsigned long long* d_cnt = t.get<unsigned long long>(2);
       long long* d_area = t.get<long long>(1);
       if (t.oom) return fail(VL_ENOMEM, "dense-tile grids (%llu words)", (unsigned long long)gw_total);
-      HIPCK(hipMemcpyAsync(d_goff, goff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
-      HIPCK(hipMemcpyAsync(d_cofs, cofs.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
-      HIPCK(hipMemcpyAsync(d_soff, soff.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
+      VLCK(upload(e, d_goff, goff.data(), m * sizeof(uint64_t)));
+      VLCK(upload(e, d_cofs, cofs.data(), m * sizeof(uint64_t)));
+      VLCK(upload(e, d_soff, soff.data(), m * sizeof(uint64_t)));
       launch_fill_u64((uint64_t*)d_cnt, 0ull, 2, e->st);
       launch_fill_u64((uint64_t*)d_area, 0ull, 1, e->st);
       launch_dense_tile(d_desc, d_list + j0, m, d_entries, d_coff + j0, d_cs, d_nxy + 2 * j0, d_grid, d_goff, d_cc,
@@ -577,10 +678,10 @@ int process_dense(vl_engine* e, const TileDesc* d_desc, const std::vector<uint32
         unsigned long long cnt[2];
         long long area;
       } rb;
-      HIPCK(hipMemcpyAsync(e->h_stage, d_cnt, 16, hipMemcpyDeviceToHost, e->st));
-      HIPCK(hipMemcpyAsync((char*)e->h_stage + 16, d_area, 8, hipMemcpyDeviceToHost, e->st));
-      HIPCK(hipStreamSynchronize(e->st));
-      memcpy(&rb, e->h_stage, sizeof(rb));
+      {
+        const RB r[2] = {{rb.cnt, d_cnt, 16, true}, {&rb.area, d_area, 8, true}};
+        VLCK(sync_read(e, r, 2));
+      }
       if (area) {
         o.area += rb.area;
       } else if (rb.cnt[0] > 0) {
@@ -725,12 +826,15 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       uint32_t err, nov;
       long long area;
     } rb;
-    HIPCK(hipMemcpyAsync(e->h_stage, cnt, 24, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 24, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 28, small + ntier, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + 32, d_area, 8, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
-    memcpy(&rb, e->h_stage, sizeof(rb));
+    {
+      // the largest region fill (cnt[2]) depends on scheduling: not checked
+      const RB r[5] = {{rb.cnt, cnt, 16, true},
+                       {rb.cnt + 2, cnt + 2, 8, false},
+                       {&rb.err, e->d_err, 4, true},
+                       {&rb.nov, small + ntier, 4, true},
+                       {&rb.area, d_area, 8, true}};
+      VLCK(sync_read(e, r, 5));
+    }
     prof_add(e, pa, pb, KS_TILE, ntier + 1, 16.0 * (double)nentries + 16.0 * (double)rb.cnt[0] + 48.0 * ntiles);
     total = rb.cnt[0];
     nsideT = rb.cnt[1];
@@ -885,6 +989,7 @@ int adopt(vl_engine* e, Outs& o, vl_layer* c) {
   c->bytes = bytes;
   c->n = (int64_t)o.nout;
   c->mbr_ok = false;
+  c->version++;
   return VL_OK;
 }
 
@@ -892,6 +997,7 @@ int set_empty(vl_engine* e, vl_layer* c) {
   (void)e;
   c->n = 0;
   c->mbr_ok = false;
+  c->version++;
   return VL_OK;
 }
 
@@ -930,7 +1036,7 @@ int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
                           o.stitch_tag.p + o.nstitch, cnts, ptmp, e->st);
     uint64_t hc[2];
     VLCK(readback(e, hc, cnts, sizeof(hc)));
-    if (g_debug) {
+    if (g_debug && e->cap.mode != CAP_CAPTURE) {
       std::vector<uint8_t> tags(o.nside);
       HIPCK(hipMemcpy(tags.data(), o.side_tag.p, o.nside, hipMemcpyDeviceToHost));
       int hist[16] = {0};
@@ -1062,8 +1168,11 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
 
 struct OpTimer {
   vl_engine* e;
-  explicit OpTimer(vl_engine* en) : e(en) { (void)hipEventRecord(e->ev0, e->st); }
+  explicit OpTimer(vl_engine* en) : e(en) {
+    if (e->cap.mode != CAP_CAPTURE) (void)hipEventRecord(e->ev0, e->st);
+  }
   ~OpTimer() {
+    if (e->cap.mode == CAP_CAPTURE) return;   // no host waits inside a capture
     (void)hipEventRecord(e->ev1, e->st);
     if (hipEventSynchronize(e->ev1) == hipSuccess) {
       float ms = 0.f;
@@ -1174,15 +1283,19 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     }
     HIPCK(hipGetLastError());
     hipEvent_t pb = prof_mark(e);
-    constexpr size_t CB = sizeof(hc);
-    HIPCK(hipMemcpyAsync(e->h_stage, cnt, CB, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + CB, e->d_err, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipMemcpyAsync((char*)e->h_stage + CB + 4, small + NT - 1, 4, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
     uint32_t err = 0;
-    memcpy(hc, e->h_stage, CB);
-    memcpy(&err, (char*)e->h_stage + CB, 4);
-    memcpy(&nov, (char*)e->h_stage + CB + 4, 4);
+    {
+      // per output: strips and seam strips checked, the largest region fill not
+      RB r[2 * FUSE_MAX_OUT + 2];
+      int nr = 0;
+      for (int k = 0; k < FUSE_MAX_OUT; k++) {
+        r[nr++] = RB{hc + 3 * k, cnt + 3 * k, 16, true};
+        r[nr++] = RB{hc + 3 * k + 2, cnt + 3 * k + 2, 8, false};
+      }
+      r[nr++] = RB{&err, e->d_err, 4, true};
+      r[nr++] = RB{&nov, small + NT - 1, 4, true};
+      VLCK(sync_read(e, r, nr));
+    }
     {
       double strips = 0.0;
       for (int q = 0; q < nout; q++) strips += (double)hc[3 * q];
@@ -1350,8 +1463,7 @@ bool is_device_ptr(const void* p) {
 int upload_tmp(vl_engine* e, Scratch& s, const void* host, size_t bytes, void** dev) {
   *dev = s.get<char>(bytes);
   if (!*dev) return fail(VL_ENOMEM, "scratch");
-  if (bytes) HIPCK(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, e->st));
-  return VL_OK;
+  return upload(e, *dev, host, bytes);
 }
 
 vl_engine* g_default = nullptr;
@@ -1589,6 +1701,7 @@ int vl_layer_upload(vl_layer* l, const vl_box* host, int64_t n) {
   HIPCK(hipStreamSynchronize(e->st));
   l->n = n;
   l->mbr_ok = false;
+  l->version++;
   return VL_OK;
 }
 
@@ -1601,6 +1714,7 @@ int vl_layer_upload_device(vl_layer* l, const vl_box* dev, int64_t n) {
   if (n > 0) HIPCK(hipMemcpyAsync(l->d, dev, n * sizeof(vl_box), hipMemcpyDeviceToDevice, e->st));
   l->n = n;
   l->mbr_ok = false;
+  l->version++;
   if (n > 0) {
     Box64 m;
     VLCK(layer_mbr(e, l, &m));   // validates the boxes
@@ -1920,6 +2034,14 @@ struct vl_program {
   int64_t failures = 0;
   std::vector<vl_engine*> workers;   // worker engines (same device, own streams)
   int max_workers = 4;
+  // graph capture (vl_program_set_graph)
+  bool graph = false;
+  hipGraph_t gr = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<void*> pinned;                                   // sources of captured uploads
+  std::vector<std::pair<const vl_layer*, uint64_t>> versions;  // layer versions the graph holds for
+  int64_t cap_failures = 0, cap_fused = 0;                     // per replay
+  int64_t captures = 0, replays = 0, fallbacks = 0;
 };
 static int run_prog_op(vl_engine* e, const ProgOp& p, int64_t* failures) {
   switch (p.op) {
@@ -1948,9 +2070,17 @@ int vl_program_create(vl_engine* e, vl_program** out) {
   return VL_OK;
 }
 
+static void graph_drop(vl_program* p);
+
 int vl_program_destroy(vl_program* p) {
   if (!p) return VL_OK;
   vl_engine* e = p->e;
+  {
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->st);
+    graph_drop(p);
+  }
   for (vl_engine* w : p->workers) engine_release(w);
   for (const ProgOp& o : p->ops)
     for (const vl_layer* l : {o.a, o.b, (const vl_layer*)o.c})
@@ -2196,6 +2326,237 @@ static std::vector<std::vector<int>> program_chains(const std::vector<ProgOp>& o
   return out;
 }
 
+// Worker engines of a program: at least n, with the root engine's settings.
+static int program_workers(vl_program* p, int n) {
+  vl_engine* e = p->e;
+  while ((int)p->workers.size() < n) {
+    vl_engine* w = nullptr;
+    VLCK(engine_new(e->device, &w));
+    w->root = e;
+    p->workers.push_back(w);
+  }
+  for (int k = 0; k < n; k++) {
+    vl_engine* w = p->workers[k];
+    w->nmax = e->nmax;
+    w->target = e->target;
+    w->erode_target = e->erode_target;
+    w->level_factor = e->level_factor;
+    w->erode_kernel = e->erode_kernel;
+    w->max_wg_cap = e->max_wg_cap;
+    w->fuse = e->fuse;
+    w->fuse_target = e->fuse_target;
+    w->bin_mode = e->bin_mode;
+    w->region_scale = std::max(w->region_scale, e->region_scale);
+    w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
+    (void)prof_resolve(w);
+    w->prof = e->prof;
+    w->fused_groups = 0;
+    for (auto& st : w->kst) st = vl_engine::KStat();
+  }
+  return VL_OK;
+}
+
+// Runs the chains on workers 0..nw-1 (chain c on worker c % nw, in order):
+// each worker on its own host thread when `threads`, else one after another
+// from this thread (capture: the graph's branches come from the streams, so
+// they still run concurrently when replayed).  With `sync` every worker
+// stream is waited for.  Adds assertion failures, fused-group counts and
+// kernel statistics to the program and its engine.
+static int run_chains(vl_program* p, const std::vector<ProgStep>& steps, const std::vector<std::vector<int>>& chains,
+                      int nw, bool threads, bool sync) {
+  vl_engine* e = p->e;
+  std::vector<int> rcs(nw, VL_OK);
+  std::vector<std::string> errs(nw);
+  std::vector<int64_t> fails(nw, 0);
+  auto work = [&](int k) {
+    vl_engine* w = p->workers[k];
+    (void)hipSetDevice(w->device);
+    for (size_t c = k; c < chains.size(); c += nw) {
+      for (int i : chains[c]) {
+        rcs[k] = run_step(w, p->ops, steps[i], &fails[k]);
+        if (rcs[k] != VL_OK) {
+          errs[k] = g_err;
+          return;
+        }
+      }
+    }
+    if (sync && hipStreamSynchronize(w->st) != hipSuccess) {
+      rcs[k] = VL_EHIP;
+      errs[k] = "worker stream failed";
+    }
+  };
+  if (threads && nw > 1) {
+    std::vector<std::thread> th;
+    for (int k = 0; k < nw; k++) th.emplace_back(work, k);
+    for (auto& t : th) t.join();
+  } else {
+    for (int k = 0; k < nw; k++) {
+      work(k);
+      if (rcs[k] != VL_OK) break;
+    }
+  }
+  int rc = VL_OK;
+  for (int k = 0; k < nw; k++) {
+    p->failures += fails[k];
+    vl_engine* w = p->workers[k];
+    e->fused_groups += w->fused_groups;
+    w->fused_groups = 0;
+    if (rc == VL_OK && rcs[k] == VL_OK && sync) rcs[k] = prof_resolve(w);
+    for (int s = 0; s < KS_N; s++) {
+      e->kst[s].launches += w->kst[s].launches;
+      e->kst[s].ms += w->kst[s].ms;
+      e->kst[s].bytes += w->kst[s].bytes;
+      w->kst[s] = vl_engine::KStat();
+    }
+    if (rc == VL_OK && rcs[k] != VL_OK) {
+      rc = rcs[k];
+      g_err = errs[k];
+    }
+  }
+  return rc;
+}
+
+static void graph_drop(vl_program* p) {
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->gr) (void)hipGraphDestroy(p->gr);
+  p->gexec = nullptr;
+  p->gr = nullptr;
+  for (void* q : p->pinned) (void)hipHostFree(q);
+  p->pinned.clear();
+  p->versions.clear();
+}
+
+static std::vector<const vl_layer*> program_layers(const vl_program* p) {
+  std::vector<const vl_layer*> v;
+  for (const ProgOp& o : p->ops)
+    for (const vl_layer* l : {o.a, o.b, (const vl_layer*)o.c})
+      if (l && std::find(v.begin(), v.end(), l) == v.end()) v.push_back(l);
+  return v;
+}
+
+static void forget_output_mbrs(vl_program* p) {
+  for (const ProgOp& o : p->ops)
+    if (o.c) o.c->mbr_ok = false;
+}
+
+// Launches the captured graph and reports whether every size it was planned
+// with held (*ok).
+static int graph_replay(vl_program* p, int nw, bool* ok) {
+  vl_engine* e = p->e;
+  HIPCK(hipGraphLaunch(p->gexec, e->st));
+  HIPCK(hipStreamSynchronize(e->st));
+  uint32_t flag = 0;
+  *ok = true;
+  for (int k = 0; k < nw; k++) {
+    HIPCK(hipMemcpy(&flag, cap_flag(p->workers[k]), 4, hipMemcpyDeviceToHost));
+    if (flag) *ok = false;
+  }
+  return VL_OK;
+}
+
+// Record run (eager, sizes logged), then capture of the same work into a
+// graph, then its first launch.  Falls back to an eager run when the capture
+// fails or the first launch's checks do not hold.
+static int graph_record_capture(vl_program* p, const std::vector<ProgStep>& steps,
+                                const std::vector<std::vector<int>>& chains, int nw) {
+  vl_engine* e = p->e;
+  graph_drop(p);
+  struct Tune {
+    double rs, frs;
+  };
+  std::vector<Tune> tune(nw);
+  for (int k = 0; k < nw; k++) {
+    vl_engine* w = p->workers[k];
+    tune[k] = Tune{w->region_scale, w->fuse_region_scale};
+    w->cap.mode = CAP_RECORD;
+    w->cap.log.clear();
+    w->cap.pos = 0;
+  }
+  forget_output_mbrs(p);
+  int rc = run_chains(p, steps, chains, nw, true, true);
+  for (int k = 0; k < nw; k++) p->workers[k]->cap.mode = CAP_EAGER;
+  if (rc != VL_OK) return rc;
+  // capture: the same decisions as the record run (same tuning state, same
+  // cached bounding boxes, logged sizes)
+  const int64_t failures = p->failures, fused = e->fused_groups;
+  forget_output_mbrs(p);
+  for (int k = 0; k < nw; k++) {
+    vl_engine* w = p->workers[k];
+    w->region_scale = tune[k].rs;
+    w->fuse_region_scale = tune[k].frs;
+    w->cap.mode = CAP_CAPTURE;
+    w->cap.pos = 0;
+    w->cap.pinned.clear();
+    w->pool.no_trim = true;
+    (void)prof_resolve(w);   // stages of the record run
+  }
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins(nw, nullptr);
+  bool ev_ok = hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < nw && ev_ok; k++) ev_ok = hipEventCreateWithFlags(&joins[k], hipEventDisableTiming) == hipSuccess;
+  hipGraph_t gr = nullptr;
+  int crc = ev_ok ? VL_OK : fail(VL_EHIP, "graph events");
+  if (crc == VL_OK && hipStreamBeginCapture(e->st, hipStreamCaptureModeRelaxed) != hipSuccess)
+    crc = fail(VL_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(hipGetLastError()));
+  const bool capturing = crc == VL_OK;
+  if (capturing) {
+    if (hipEventRecord(fork, e->st) != hipSuccess) crc = fail(VL_EHIP, "fork event");
+    for (int k = 0; k < nw && crc == VL_OK; k++) {
+      vl_engine* w = p->workers[k];
+      if (hipStreamWaitEvent(w->st, fork, 0) != hipSuccess) crc = fail(VL_EHIP, "fork wait");
+      launch_fill_u32(cap_flag(w), 0u, 1, w->st);
+    }
+    if (crc == VL_OK) crc = run_chains(p, steps, chains, nw, false, false);
+    for (int k = 0; k < nw; k++) {
+      vl_engine* w = p->workers[k];
+      if (hipEventRecord(joins[k], w->st) != hipSuccess || hipStreamWaitEvent(e->st, joins[k], 0) != hipSuccess)
+        if (crc == VL_OK) crc = fail(VL_EHIP, "join");
+    }
+    const hipError_t er = hipStreamEndCapture(e->st, &gr);
+    if (er != hipSuccess && crc == VL_OK) crc = fail(VL_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(er));
+  }
+  for (int k = 0; k < nw; k++) {
+    vl_engine* w = p->workers[k];
+    if (crc == VL_OK && w->cap.pos != w->cap.log.size()) crc = fail(VL_EINTERNAL, "capture left recorded sizes unread");
+    w->cap.mode = CAP_EAGER;
+    w->pool.no_trim = false;
+    for (void* q : w->cap.pinned) p->pinned.push_back(q);
+    w->cap.pinned.clear();
+  }
+  if (fork) (void)hipEventDestroy(fork);
+  for (hipEvent_t j : joins)
+    if (j) (void)hipEventDestroy(j);
+  // counted once (the record run counted them)
+  p->cap_failures = p->failures - failures;
+  p->cap_fused = e->fused_groups - fused;
+  e->fused_groups = fused;
+  p->failures = failures;
+  if (crc == VL_OK) {
+    p->gr = gr;
+    if (hipGraphInstantiate(&p->gexec, gr, nullptr, nullptr, 0) != hipSuccess) {
+      p->gexec = nullptr;
+      crc = fail(VL_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(hipGetLastError()));
+    }
+  } else if (gr) {
+    (void)hipGraphDestroy(gr);
+  }
+  bool ok = false;
+  if (crc == VL_OK) crc = graph_replay(p, nw, &ok);
+  if (crc == VL_OK && ok) {
+    for (const vl_layer* l : program_layers(p)) p->versions.push_back({l, l->version});
+    p->captures++;
+    return VL_OK;
+  }
+  // the capture replaced the outputs' storage: compute them again eagerly
+  if (g_debug) fprintf(stderr, "[vl] graph capture not used: %s\n", crc == VL_OK ? "first launch's checks failed" : g_err.c_str());
+  graph_drop(p);
+  p->fallbacks++;
+  g_err.clear();
+  p->failures = 0;
+  forget_output_mbrs(p);
+  return run_chains(p, steps, chains, nw, true, true);
+}
+
 int vl_program_run(vl_program* p) {
   if (!p) return fail(VL_EINVAL, "null program");
   vl_engine* e = p->e;
@@ -2204,7 +2565,7 @@ int vl_program_run(vl_program* p) {
   p->failures = 0;
   const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse != 0);
   const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
-  const int nw = (int)std::min<size_t>(chains.size(), (size_t)p->max_workers);
+  int nw = (int)std::min<size_t>(chains.size(), (size_t)p->max_workers);
   // the bounding boxes of the program's inputs (layers no operator writes)
   // are cached before concurrent chains read them
   {
@@ -2224,73 +2585,44 @@ int vl_program_run(vl_program* p) {
   HIPCK(hipEventCreate(&t1));
   HIPCK(hipEventRecord(t0, e->st));
   int rc = VL_OK;
-  if (nw <= 1) {
+  if (p->graph && !chains.empty()) {
+    // a captured program runs on worker engines only (their memory pools
+    // hold the graph's buffers), one per chain up to max_workers
+    nw = std::max(nw, 1);
+    rc = program_workers(p, nw);
+    bool replayed = false;
+    if (rc == VL_OK && p->gexec) {
+      bool same = true;
+      for (const auto& lv : p->versions)
+        if (lv.first->version != lv.second) same = false;
+      if (same) {
+        for (int k = 0; k < nw; k++) HIPCK(hipStreamWaitEvent(p->workers[k]->st, t0, 0));
+        bool ok = false;
+        rc = graph_replay(p, nw, &ok);
+        if (rc == VL_OK && ok) {
+          p->failures = p->cap_failures;
+          e->fused_groups += p->cap_fused;
+          p->replays++;
+          replayed = true;
+        } else if (rc == VL_OK) {
+          p->fallbacks++;   // sizes differed: the results are recomputed below
+        }
+      }
+      if (!replayed) graph_drop(p);
+    }
+    if (rc == VL_OK && !replayed) {
+      for (int k = 0; k < nw; k++) HIPCK(hipStreamWaitEvent(p->workers[k]->st, t0, 0));
+      rc = graph_record_capture(p, steps, chains, nw);
+    }
+  } else if (nw <= 1) {
     for (const ProgStep& st : steps) {
       rc = run_step(e, p->ops, st, &p->failures);
       if (rc != VL_OK) break;
     }
   } else {
-    while ((int)p->workers.size() < nw) {
-      vl_engine* w = nullptr;
-      VLCK(engine_new(e->device, &w));
-      w->root = e;
-      p->workers.push_back(w);
-    }
-    std::vector<int> rcs(nw, VL_OK);
-    std::vector<std::string> errs(nw);
-    std::vector<int64_t> fails(nw, 0);
-    std::vector<std::thread> th;
-    for (int k = 0; k < nw; k++) {
-      vl_engine* w = p->workers[k];
-      w->nmax = e->nmax;
-      w->target = e->target;
-      w->erode_target = e->erode_target;
-      w->level_factor = e->level_factor;
-      w->erode_kernel = e->erode_kernel;
-      w->max_wg_cap = e->max_wg_cap;
-      w->fuse = e->fuse;
-      w->fuse_target = e->fuse_target;
-      w->bin_mode = e->bin_mode;
-      w->region_scale = std::max(w->region_scale, e->region_scale);
-      w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
-      (void)prof_resolve(w);
-      w->prof = e->prof;
-      w->fused_groups = 0;
-      for (auto& st : w->kst) st = vl_engine::KStat();
-      HIPCK(hipStreamWaitEvent(w->st, t0, 0));
-      th.emplace_back([&, k, w]() {
-        (void)hipSetDevice(w->device);
-        for (size_t c = k; c < chains.size(); c += nw) {
-          for (int i : chains[c]) {
-            rcs[k] = run_step(w, p->ops, steps[i], &fails[k]);
-            if (rcs[k] != VL_OK) {
-              errs[k] = g_err;
-              return;
-            }
-          }
-        }
-        if (hipStreamSynchronize(w->st) != hipSuccess) {
-          rcs[k] = VL_EHIP;
-          errs[k] = "worker stream failed";
-        }
-      });
-    }
-    for (auto& t : th) t.join();
-    for (int k = 0; k < nw; k++) {
-      p->failures += fails[k];
-      vl_engine* w = p->workers[k];
-      e->fused_groups += w->fused_groups;
-      if (rc == VL_OK && rcs[k] == VL_OK) rcs[k] = prof_resolve(w);
-      for (int s = 0; s < KS_N; s++) {
-        e->kst[s].launches += w->kst[s].launches;
-        e->kst[s].ms += w->kst[s].ms;
-        e->kst[s].bytes += w->kst[s].bytes;
-      }
-      if (rc == VL_OK && rcs[k] != VL_OK) {
-        rc = rcs[k];
-        g_err = errs[k];
-      }
-    }
+    rc = program_workers(p, nw);
+    for (int k = 0; k < nw && rc == VL_OK; k++) HIPCK(hipStreamWaitEvent(p->workers[k]->st, t0, 0));
+    if (rc == VL_OK) rc = run_chains(p, steps, chains, nw, true, true);
   }
   (void)hipEventRecord(t1, e->st);
   (void)hipEventSynchronize(t1);
@@ -2299,6 +2631,22 @@ int vl_program_run(vl_program* p) {
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   return rc;
+}
+
+int vl_program_set_graph(vl_program* p, int enable) {
+  if (!p) return fail(VL_EINVAL, "null program");
+  std::lock_guard<std::recursive_mutex> lk(p->e->mu);
+  p->graph = enable != 0;
+  if (!p->graph) graph_drop(p);
+  return VL_OK;
+}
+
+int vl_program_graph_stats(vl_program* p, int64_t* captures, int64_t* replays, int64_t* fallbacks) {
+  if (!p || !captures || !replays || !fallbacks) return fail(VL_EINVAL, "null argument");
+  *captures = p->captures;
+  *replays = p->replays;
+  *fallbacks = p->fallbacks;
+  return VL_OK;
 }
 
 int vl_program_assert_count(vl_program* p, int64_t* failures) {
@@ -2375,6 +2723,7 @@ int window_extract(vl_engine* e, const CellIndex& ix, const vl_box& w, vl_layer*
   out->bytes = r;
   out->n = m;
   out->mbr_ok = false;
+  out->version++;
   return VL_OK;
 }
 

@@ -88,6 +88,12 @@ int noalloc_setup(int device);
 // Host-side launch wrappers (each .hip file implements its own kernels).
 
 // vl_prims.hip
+// Graph replays: *flag |= 1 when the bytes (a multiple of 4) at dev differ
+// from the host values `expected` (copied into the launch arguments).
+struct Expect8 {
+  uint32_t w[8];
+};
+void launch_expect(const void* dev, const void* expected, size_t bytes, uint32_t* flag, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
 // MBR of boxes after a transform; validates boxes (ERR_INVALID_BOX).

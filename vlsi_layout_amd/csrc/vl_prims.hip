@@ -1,4 +1,6 @@
 // vl_prims.hip — fills, bounding-box reduction and device-wide exclusive scans.
+#include <algorithm>
+
 #include "vl_internal.h"
 #include "vl_dev.h"
 
@@ -15,6 +17,13 @@ __global__ void k_fill(T* p, T v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+// Sets *flag when any of the n dwords at p differs from the expected value
+// (graph replays: a size the captured launches were planned with).
+__global__ void k_expect(const uint32_t* __restrict__ p, Expect8 ex, int n, uint32_t* flag) {
+  const int i = threadIdx.x;
+  if (i < n && p[i] != ex.w[i]) atomicOr(flag, 1u);
 }
 
 // Bounding box of the transformed boxes; flags invalid input or output boxes.
@@ -154,6 +163,17 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st) {
   size_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL((k_fill<uint64_t>), dim3(g), dim3(256), 0, st, p, v, n);
+}
+
+void launch_expect(const void* dev, const void* expected, size_t bytes, uint32_t* flag, hipStream_t st) {
+  const uint32_t* p = (const uint32_t*)dev;
+  const uint32_t* x = (const uint32_t*)expected;
+  for (size_t o = 0; o < bytes / 4; o += 8) {
+    Expect8 ex{};
+    const int n = (int)std::min<size_t>(8, bytes / 4 - o);
+    for (int i = 0; i < n; i++) ex.w[i] = x[o + i];
+    hipLaunchKernelGGL(k_expect, dim3(1), dim3(64), 0, st, p + o, ex, n, flag);
+  }
 }
 
 void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st) {
