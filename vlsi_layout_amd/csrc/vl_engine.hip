@@ -474,6 +474,9 @@ struct Outs {
   DVec<uint8_t> side_tag, stitch_tag;
   uint64_t nout = 0, nside = 0, nstitch = 0;
   long long area = 0;
+  // the current level is an AND: tiles without an overlapping pair are
+  // skipped by a pair filter before the tile kernels
+  bool and_filter = false;
   explicit Outs(vl_engine* e) : out(e), side(e), next(e), stitch(e), side_tag(e), stitch_tag(e) {}
 };
 
@@ -761,6 +764,19 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   uint64_t* tab_off = area ? nullptr : s.get<uint64_t>((size_t)ntiles);
   uint32_t* tab_n = area ? nullptr : s.get<uint32_t>(2 * (size_t)ntiles);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
+  // AND: the first tier takes only the tiles the pair filter keeps
+  uint32_t* flist = nullptr;
+  uint32_t* flist_n = nullptr;
+  // the stage's time includes the filter (its bytes are the entry reads the
+  // stage formula counts once)
+  hipEvent_t pa_first = prof_mark(e);
+  if (o.and_filter && two && !area && erode == 0) {
+    flist = s.get<uint32_t>((size_t)ntiles);
+    flist_n = s.get<uint32_t>(1);
+    if (s.oom) return fail(VL_ENOMEM, "scratch");
+    launch_fill_u32(flist_n, 0u, 1, e->st);
+    launch_and_filter(d_desc, ntiles, d_entries, flist, flist_n, tab_n, ntiles, e->st);
+  }
   // Exact grids: workgroup g of a launch takes tiles g, g+G, ... and owns one
   // region of the scratch stream (regions of the launches follow each other).
   int G[4];
@@ -807,12 +823,12 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     to.ntab = ntiles;
     to.area = d_area;
     to.err = e->d_err;
-    hipEvent_t pa = prof_mark(e);
+    hipEvent_t pa = attempt == 0 ? pa_first : prof_mark(e);
     int rbase = 0;
     for (int t = 0; t <= ntier; t++) {
       to.region_base = rbase;
-      to.list = t ? ovfl[t - 1] : nullptr;        // tiles above the previous tier's capacity
-      to.list_n = t ? small + (t - 1) : nullptr;  // (count known only on the device)
+      to.list = t ? ovfl[t - 1] : flist;          // tiles above the previous tier's capacity
+      to.list_n = t ? small + (t - 1) : flist_n;  // (count known only on the device)
       to.ovf = ovfl[t];
       to.ovf_n = small + t;
       if (t < ntier) launch_tile_wave(tiers[t], two, area, erode > 0, d_desc, ntiles, d_entries, to, G[t], e->st);
@@ -977,6 +993,7 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
   if (!desc) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
   launch_build_desc(g, start, top, desc, e->st);
   HIPCK(hipGetLastError());
+  o.and_filter = top == TOP_AND && nb > 0 && !area && erode == 0;
   return process_tiles(e, desc, (int)nt, entries, total, two_layer(top) && nb > 0, area, erode, o, 0);
 }
 

@@ -157,6 +157,11 @@ void launch_bin_count(const vl_box* a, int64_t na, Xform xra, const vl_box* b, i
                       hipStream_t st);
 void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const vl_box* b, int64_t nb, Grid g,
                         const uint64_t* start, uint32_t* cnt, vl_box* entries, hipStream_t st);
+// AND tiles: appends to list (count in *list_n, zeroed by the caller) the
+// tiles whose result may be non-empty (a clipped A box overlaps a B box, or
+// the tile is too large to test); the others get tab_n[t] = tab_n[ntab+t] = 0.
+void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, uint32_t* list, uint32_t* list_n,
+                       uint32_t* tab_n, int ntab, hipStream_t st);
 // Tile descriptors from the key bounds (start has 2*ntiles+1 entries).
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st);
 // Tile kernel: result strips of every tile.  Strips touching an internal seam

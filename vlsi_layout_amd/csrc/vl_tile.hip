@@ -224,6 +224,58 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ 
   }
 }
 
+// AND pair filter.  A tile of A AND B has an empty result unless some box of
+// A, clipped to the tile, overlaps some box of B with positive area.  A wave
+// per tile: lane j holds B box j, the A boxes are broadcast one by one with
+// readlane and tested against all B boxes at once (one ballot each), as the
+// tile kernel's own pruning does.  Tiles with such a pair, or with more than
+// a wave of boxes of either layer, are appended to `list`; the others get
+// empty result counts here and are never loaded by the tile kernels.
+__global__ __launch_bounds__(256) void k_and_filter(const TileDesc* __restrict__ desc, int ntiles,
+                                                    const vl_box* __restrict__ entries, uint32_t* __restrict__ list,
+                                                    uint32_t* __restrict__ list_n, uint32_t* __restrict__ tab_n,
+                                                    int ntab) {
+  const int lane = __lane_id();
+  const int nwaves = gridDim.x * (blockDim.x / WAVE);
+  for (int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < ntiles; t += nwaves) {
+    const int32_t* dw = (const int32_t*)(desc + t);
+    const int32_t x0 = uniform(dw[0]), y0 = uniform(dw[1]), x1 = uniform(dw[2]), y1 = uniform(dw[3]);
+    const int64_t off = uniform(desc[t].off);
+    const int nA = uniform(desc[t].nA), nB = uniform(desc[t].nB);
+    bool keep = false;
+    if (nA > 0 && nB > 0) {
+      if (nA > WAVE || nB > WAVE) {
+        keep = true;
+      } else {
+        int32_t ax1 = 0, ax2 = 0, ay1 = 0, ay2 = 0;
+        if (lane < nA) {
+          const vl_box a = entries[off + lane];
+          ax1 = max(a.x1, x0); ax2 = min(a.x2, x1); ay1 = max(a.y1, y0); ay2 = min(a.y2, y1);
+        }
+        vl_box b{0, 0, 0, 0};
+        if (lane < nB) b = entries[off + nA + lane];
+        for (int i = 0; i < nA; i++) {
+          const int32_t qx1 = __builtin_amdgcn_readlane(ax1, i), qx2 = __builtin_amdgcn_readlane(ax2, i);
+          const int32_t qy1 = __builtin_amdgcn_readlane(ay1, i), qy2 = __builtin_amdgcn_readlane(ay2, i);
+          const bool h = lane < nB && max(qx1, b.x1) < min(qx2, b.x2) && max(qy1, b.y1) < min(qy2, b.y2);
+          if (__ballot(h)) {
+            keep = true;
+            break;
+          }
+        }
+      }
+    }
+    if (lane == 0) {
+      if (keep) {
+        list[atomicAdd(list_n, 1u)] = (uint32_t)t;
+      } else {
+        tab_n[t] = 0u;
+        tab_n[ntab + t] = 0u;
+      }
+    }
+  }
+}
+
 __global__ void k_build_desc(Grid g, const uint64_t* __restrict__ start, int32_t op, TileDesc* desc) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
@@ -1156,6 +1208,12 @@ void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const
   if (na + nb > 0)
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, xsa, b, nb, g, start,
                        cnt, entries);
+}
+void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, uint32_t* list, uint32_t* list_n,
+                       uint32_t* tab_n, int ntab, hipStream_t st) {
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_and_filter, dim3(grid_for(ntiles, 4)), dim3(256), 0, st, desc, ntiles, entries, list,
+                       list_n, tab_n, ntab);
 }
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st) {
   int64_t nt = (int64_t)g.cols * g.rows;
