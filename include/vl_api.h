@@ -134,6 +134,21 @@ int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c);
  * (src/design_impl.f90:1318-1341), heal_boxes (src/polygon_fracture.f90:805) and
  * the Boost MERGE (src/boost_polygon_api.cpp:41-70). */
 int vl_merge(vl_engine* e, const vl_layer* a, vl_layer* c);
+/* C = the points covered by at least two boxes of A (the union of the
+ * positive-area intersections of every pair of A's boxes).  Replaces
+ * CalculateSingleLayerAND, the `AND` verbose operator with one input
+ * (src/design_impl.f90:385-509, src/controlparser.f90:545-549). */
+int vl_self_and(vl_engine* e, const vl_layer* a, vl_layer* c);
+/* C = rows x cols boxes tiling MBR(A), each padded by `overlap` (overlap/2
+ * on the low side, the rest on the high side) and clamped to MBR(A), in
+ * row-major order from the bottom-left, exactly as CreateGrid
+ * (src/design_impl.f90:159-275; the GRID operator passes 10, 10, 10,
+ * src/controlparser.f90:559-562).  The boxes are kept as they are (not
+ * merged).  VL_EINVAL when rows or cols < 1 or A is empty. */
+int vl_grid(vl_engine* e, const vl_layer* a, int32_t rows, int32_t cols, int32_t overlap, vl_layer* c);
+/* C = the one box MBR(A).  Replaces CreateEXTENT (src/design_impl.f90:277-293).
+ * VL_EINVAL when A is empty. */
+int vl_extent(vl_engine* e, const vl_layer* a, vl_layer* c);
 /* Box count of A (the ASSERT_ZERO check, src/controlparser.f90:524-530).
  * Returns VL_EASSERT when *count>0 and abort_on_nonzero!=0. */
 int vl_assert_zero(vl_engine* e, const vl_layer* a, int abort_on_nonzero, int64_t* count);

@@ -29,6 +29,9 @@
  *   or_and/not/or/xor       raster set algebra on tiny grids; known answer
  *                           (0,0,10,10)\(2,2,8,8) (src/test_rectboolean.f90); area identities.
  *   or_grow / or_size       raster dilation/erosion by a square; single-box closed forms.
+ *   or_self_and             raster count of boxes per cell (>= 2); identical boxes.
+ *   or_grid                 a hand-evaluated case of CreateGrid's arithmetic; area and
+ *                           tiling invariants.
  *   sorts                   library equivalence (Python sorted() with the same keys).
  */
 #include "oracle.h"
@@ -540,6 +543,72 @@ void or_morton_sort_indirect(const ob_box* b, int64_t* idx, int64_t n) {
   qsort(v, (size_t)n, sizeof(kidx), kidx_cmp);
   for (int64_t i = 0; i < n; i++) idx[i] = v[i].i + 1;   /* 1-based */
   free(v);
+}
+
+/* CalculateSingleLayerAND (design_impl.f90:385-509): for every pair i < k of */
+/* A's boxes that interact, A(i)*A(k) is kept when its area is positive       */
+/* (:459-470); the union of the fragments, canonical.  The RTree query is     */
+/* replaced by a sweep over the boxes sorted by x1.                           */
+static const ob_box* g_sort_base;
+static int idx_x1_cmp(const void* pa, const void* pb) {
+  const ob_box a = g_sort_base[*(const int64_t*)pa], b = g_sort_base[*(const int64_t*)pb];
+  if (a.x1 != b.x1) return a.x1 < b.x1 ? -1 : 1;
+  return (*(const int64_t*)pa < *(const int64_t*)pb) ? -1 : 1;
+}
+int64_t or_self_and(const ob_box* a, int64_t na, ob_box** out) {
+  bvec o = {0};
+  *out = NULL;
+  if (na <= 1) return 0;
+  int64_t* id = (int64_t*)malloc((size_t)na * sizeof(int64_t));
+  for (int64_t i = 0; i < na; i++) id[i] = i;
+  g_sort_base = a;
+  qsort(id, (size_t)na, sizeof(int64_t), idx_x1_cmp);
+  for (int64_t p = 0; p < na; p++) {
+    const ob_box A = a[id[p]];
+    for (int64_t q = p + 1; q < na; q++) {
+      const ob_box B = a[id[q]];
+      if (B.x1 >= A.x2) break;                     /* sorted by x1: no later box meets A */
+      ob_box I = {imax(A.x1, B.x1), imax(A.y1, B.y1), imin(A.x2, B.x2), imin(A.y2, B.y2)};
+      if (box_valid(I)) bv_push(&o, I);
+    }
+  }
+  free(id);
+  ob_box* fr = NULL;
+  const int64_t nf = bv_release(&o, &fr);
+  const int64_t r = or_canonical(fr, nf, out);
+  free(fr);
+  return r;
+}
+
+/* CreateGrid (design_impl.f90:159-275): rows x cols cells of MBR m, base    */
+/* size plus one unit for the first (extent mod count) cells (:204-214),      */
+/* padded by overlap/2 below / left and the rest above / right (:219-223),    */
+/* clamped to m (:253-257); row-major, rows outer (:236-262).                 */
+int64_t or_grid(ob_box m, int32_t rows, int32_t cols, int32_t overlap, ob_box** out) {
+  *out = NULL;
+  if (rows < 1 || cols < 1 || !box_valid(m)) return -1;
+  const int64_t n = (int64_t)rows * cols;
+  ob_box* v = (ob_box*)malloc((size_t)n * sizeof(ob_box));
+  const int64_t fw = (int64_t)m.x2 - m.x1, fh = (int64_t)m.y2 - m.y1;
+  const int64_t cw = fw / cols, ew = fw % cols, rh = fh / rows, eh = fh % rows;
+  const int64_t pl = overlap / 2, pr = overlap - pl;
+  int64_t k = 0;
+  for (int64_t i = 1; i <= rows; i++) {
+    const int64_t y0 = m.y1 + (i - 1) * rh + (i - 1 < eh ? i - 1 : eh);
+    const int64_t y1 = m.y1 + i * rh + (i < eh ? i : eh);
+    for (int64_t j = 1; j <= cols; j++) {
+      const int64_t x0 = m.x1 + (j - 1) * cw + (j - 1 < ew ? j - 1 : ew);
+      const int64_t x1 = m.x1 + j * cw + (j < ew ? j : ew);
+      int64_t bx1 = x0 - pl, bx2 = x1 + pr, by1 = y0 - pl, by2 = y1 + pr;
+      if (bx1 < m.x1) bx1 = m.x1;
+      if (bx2 > m.x2) bx2 = m.x2;
+      if (by1 < m.y1) by1 = m.y1;
+      if (by2 > m.y2) by2 = m.y2;
+      v[k++] = (ob_box){(int32_t)bx1, (int32_t)by1, (int32_t)bx2, (int32_t)by2};
+    }
+  }
+  *out = v;
+  return n;
 }
 
 /* ------------------------------------------------------------------------- */

@@ -51,6 +51,14 @@ int64_t or_grow(const ob_box* a, int64_t na, int32_t e, int32_t n, int32_t w, in
 /* SIZE d (Boost resize semantics restated; src/boost_polygon_api.cpp:132-135). */
 int64_t or_size(const ob_box* a, int64_t na, int32_t d, ob_box** out);
 
+/* The points covered by at least two boxes of a: the union of the positive-
+ * area intersections of pairs i < k (CalculateSingleLayerAND,
+ * src/design_impl.f90:385-509), canonical. */
+int64_t or_self_and(const ob_box* a, int64_t na, ob_box** out);
+/* CreateGrid (src/design_impl.f90:159-275): rows x cols padded cells of m,
+ * row-major; -1 when rows or cols < 1 or m is not a valid box. */
+int64_t or_grid(ob_box m, int32_t rows, int32_t cols, int32_t overlap, ob_box** out);
+
 /* Pairwise positive-area intersections (CalculateAND fragment loop,
  * src/design.f90:1168-1194), before healing. */
 int64_t or_and_fragments(const ob_box* a, int64_t na, const ob_box* b, int64_t nb, ob_box** out);

@@ -1,0 +1,96 @@
+! check_layer_ops.f90 — calls the Fortran call surface directly (test
+! helper; run by tests/test_fortran.py on a GPU host).
+!   check_layer_ops A.bin B.bin OUTDIR
+! prints the known-answer results of fracture_rectangle and
+! merge_boxes_using_scanline, and writes the results of CalculateFrameAND,
+! CalculateSingleLayerAND, CreateGrid, CreateEXTENT and CalculateGROWLayer
+! (called through an intent(in) argument) as KLBIN files in OUTDIR.
+program check_layer_ops
+  use iso_fortran_env, only: int64, int32
+  use vl_layer_ops
+  use vl_rect_boolean, only: RBox => Box, Vector_Box, fracture_rectangle
+  implicit none
+  character(len=512) :: fa, fb, dir
+  type(Layer) :: A, B, C
+  type(Vector_Box) :: overlaps, frags
+  type(Box), allocatable :: m(:)
+  integer :: k
+
+  call get_command_argument(1, fa)
+  call get_command_argument(2, fb)
+  call get_command_argument(3, dir)
+  call vl_fortran_init()
+
+  ! (0,0,10,10) minus (2,2,8,8): the four fragments around the hole
+  call overlaps%push(RBox(2_int64, 2_int64, 8_int64, 8_int64))
+  call fracture_rectangle(RBox(0_int64, 0_int64, 10_int64, 10_int64), overlaps, frags)
+  print '(A,I0)', 'FRACTURE ', frags%count
+  do k = 1, frags%count
+     print '(A,4(1X,I0))', 'F', frags%elements(k)%x1, frags%elements(k)%y1, frags%elements(k)%x2, frags%elements(k)%y2
+  end do
+  ! an empty overlap list leaves the target; a covering one removes it
+  call frags%clear()
+  call overlaps%clear()
+  call fracture_rectangle(RBox(-5_int64, -5_int64, 3_int64, 4_int64), overlaps, frags)
+  call overlaps%push(RBox(-10_int64, -10_int64, 10_int64, 10_int64))
+  call fracture_rectangle(RBox(0_int64, 0_int64, 1_int64, 1_int64), overlaps, frags)
+  print '(A,I0,4(1X,I0))', 'FRACTURE2 ', frags%count, frags%elements(1)%x1, frags%elements(1)%y1, &
+       frags%elements(1)%x2, frags%elements(1)%y2
+
+  ! (0,0,2,2) + (2,0,4,2) -> (0,0,4,2)
+  allocate(m(2))
+  m(1) = Box(0, 0, 2, 2)
+  m(2) = Box(2, 0, 4, 2)
+  call merge_boxes_using_scanline(m)
+  print '(A,I0,4(1X,I0))', 'MERGE ', size(m), m(1)%x1, m(1)%y1, m(1)%x2, m(1)%y2
+
+  call load(A, fa)
+  call load(B, fb)
+  call CalculateFrameAND(A, B, C)
+  call save(C, 'frame_and.bin')
+  call CalculateSingleLayerAND(A, C)
+  call save(C, 'self_and.bin')
+  call CreateGrid(A, C, 3, 4, 2)
+  call save(C, 'grid.bin')
+  call CreateEXTENT(A, C)
+  call save(C, 'extent.bin')
+  call grow_of(A, C)
+  call save(C, 'grow.bin')
+  print '(A)', 'CHECK DONE'
+
+contains
+
+  ! GROW of a layer the caller holds as intent(in) (src/design.f90:243)
+  subroutine grow_of(src, dst)
+    type(Layer), intent(in) :: src
+    type(Layer), intent(inout) :: dst
+    call CalculateGROWLayer(src, dst, [1_int32, 2_int32, 3_int32, 4_int32])
+  end subroutine grow_of
+
+  subroutine load(L, path)
+    type(Layer), intent(inout) :: L
+    character(len=*), intent(in) :: path
+    integer(int64) :: bytes, n
+    integer :: u
+    type(Box), allocatable :: b(:)
+    inquire(file=trim(path), size=bytes)
+    n = bytes / 16
+    allocate(b(max(n, 1_int64)))
+    open(newunit=u, file=trim(path), access='stream', form='unformatted', status='old', action='read')
+    read(u) b(1:n)
+    close(u)
+    call layer_set_boxes(L, b, n)
+  end subroutine load
+
+  subroutine save(L, name)
+    type(Layer), intent(inout) :: L
+    character(len=*), intent(in) :: name
+    integer :: u
+    call layer_to_host(L)
+    open(newunit=u, file=trim(dir)//'/'//name, access='stream', form='unformatted', status='replace', &
+         action='write')
+    if (L%n_used > 0) write(u) L%layer_boxes(1:L%n_used)
+    close(u)
+  end subroutine save
+
+end program check_layer_ops

@@ -53,6 +53,10 @@ def lib():
         L.or_grow.argtypes = [C.c_void_p, C.c_int64] + [C.c_int32] * 4 + [PB]
         L.or_size.restype = C.c_int64
         L.or_size.argtypes = [C.c_void_p, C.c_int64, C.c_int32, PB]
+        L.or_self_and.restype = C.c_int64
+        L.or_self_and.argtypes = [C.c_void_p, C.c_int64, PB]
+        L.or_grid.restype = C.c_int64
+        L.or_grid.argtypes = [OBox, C.c_int32, C.c_int32, C.c_int32, PB]
         L.or_free.argtypes = [C.c_void_p]
         L.or_event_sort.argtypes = [C.c_void_p, C.c_int64]
         L.or_morton_key.restype = C.c_uint64
@@ -147,6 +151,18 @@ def op_xor(a, b):
 
 def and_fragments(a, b):
     return _binary("or_and_fragments", a, b)
+
+
+def self_and(a):
+    return _unary("or_self_and", a)
+
+
+def grid(mbr, rows, cols, overlap):
+    p = C.c_void_p()
+    n = lib().or_grid(OBox(*[int(v) for v in mbr]), int(rows), int(cols), int(overlap), C.byref(p))
+    if n < 0:
+        raise ValueError("grid: bad arguments")
+    return _take(n, p)
 
 
 def grow(a, e, n, w, s):

@@ -35,8 +35,10 @@ def test_module_exports_reference_names():
     out = subprocess.run(["nm", "-D", FLIB], capture_output=True, text=True, check=True).stdout.lower()
     for name in ["calculateand", "calculateframenot", "calculatenot", "calculateor", "calculategrowlayer",
                  "calculateboostoperation", "mergeheallayer", "heal_boxes", "sort_event_trackers",
-                 "sortboxesdirect", "sortboxesindirect"]:
+                 "sortboxesdirect", "sortboxesindirect", "calculateframeand", "calculatesinglelayerand",
+                 "creategrid", "createextent", "merge_boxes_using_scanline"]:
         assert f"_qmvl_layer_opsp{name}" in out, name
+    assert "_qmvl_rect_booleanpfracture_rectangle" in out
 
 
 @needs_driver
@@ -140,3 +142,109 @@ dump t1:size size.bin
         assert np.array_equal(got, exp), name
     ext = read_klbin(tmp_path / "ext.bin")
     assert ext.tolist() == [[a[:, 0].min(), a[:, 1].min(), a[:, 2].max(), a[:, 3].max()]]
+
+
+CHECK = os.path.join(ROOT, "tests", "fortran", "check_layer_ops")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CHECK), reason="Fortran check program not built")
+def test_module_procedures_called_directly(tmp_path, gpu_engine):
+    """fracture_rectangle, merge_boxes_using_scanline, CalculateFrameAND,
+    CalculateSingleLayerAND, CreateGrid, CreateEXTENT and CalculateGROWLayer
+    (its input held intent(in)) called from Fortran."""
+    fa = os.path.join(GOLDEN_IN, "SED_L67_D20.bin")
+    fb = os.path.join(GOLDEN_IN, "SED_L66_D20.bin")
+    r = subprocess.run([CHECK, fa, fb, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert "CHECK DONE" in lines
+    # (0,0,10,10) minus (2,2,8,8): the reference's four fragments (src/test_rectboolean.f90)
+    assert "FRACTURE 4" in lines
+    frags = sorted(tuple(int(v) for v in ln.split()[1:]) for ln in lines if ln.startswith("F "))
+    assert frags == [(0, 0, 2, 10), (2, 0, 8, 2), (2, 8, 8, 10), (8, 0, 10, 10)]
+    assert "FRACTURE2 1 -5 -5 3 4" in lines        # nothing to cut; a covered target adds nothing
+    assert "MERGE 1 0 0 4 2" in lines              # src/test_boxmerge.f90:15-17
+    a, b = read_klbin(fa), read_klbin(fb)
+    rd = lambda n: read_klbin(tmp_path / n)
+    assert np.array_equal(O.sort_boxes(rd("frame_and.bin")), O.op_and(a, b))
+    assert np.array_equal(O.sort_boxes(rd("self_and.bin")), O.self_and(a))
+    m = (a[:, 0].min(), a[:, 1].min(), a[:, 2].max(), a[:, 3].max())
+    assert np.array_equal(rd("grid.bin"), O.grid(m, 3, 4, 2))   # bit-exact, in order
+    assert rd("extent.bin").tolist() == [list(m)]
+    assert np.array_equal(O.sort_boxes(rd("grow.bin")), O.grow(a, 1, 2, 3, 4))
+
+
+OPS_SCRIPT = """
+layer d1:a klbin {fa}
+layer d1:b klbin {fb}
+exec run t1:or   = d1:a + d1:b
+exec run t1:and  = d1:a * d1:b
+exec run t1:fnot = d1:a ~ d1:b
+exec run t1:xor  = d1:a ^ d1:b
+exec run t1:grow = d1:a GROW t1:nothing 0.002 0.0 0.001 0.003
+exec run t1:size = d1:a SIZE t1:nothing -0.001
+exec run t1:ring = t1:grow ~ d1:a
+exec run t1:chk  = t1:ring * d1:a
+exec run t1:nothing = t1:chk ASSERT_ZERO t1:nothing
+exec run t1:nothing = t1:and ASSERT_ZERO t1:nothing
+exec run t1:self = d1:a AND t1:nothing
+exec run t1:copy = t1:or COPY t1:nothing
+exec run t1:grid = d1:a GRID t1:nothing
+exec run t1:g2   = t1:grid GROW t1:nothing 0.001 0.001 0.001 0.001
+"""
+OPS_OUT = ["or", "and", "fnot", "xor", "grow", "size", "ring", "chk", "self", "copy", "grid", "g2"]
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("mode", ["immediate", "batch", "graph"])
+def test_schedules_agree_with_the_oracle(tmp_path, gpu_engine, mode):
+    """The same control file under each schedule (one operator at a time, a
+    deferred program, a deferred program captured as a HIP graph): identical
+    results, equal to the oracle, and the same ASSERT_ZERO report."""
+    fa = os.path.join(GOLDEN_IN, "SED_L67_D20.bin")
+    fb = os.path.join(GOLDEN_IN, "SED_L66_D20.bin")
+    for f in (fa, fb):
+        shutil.copy(f, tmp_path)
+    script = f"schedule {mode}\n" + OPS_SCRIPT.format(fa=os.path.basename(fa), fb=os.path.basename(fb))
+    script += "".join(f"dump t1:{n} {n}.bin\n" for n in OPS_OUT)
+    r = run_driver(tmp_path, script)
+    a, b = read_klbin(fa), read_klbin(fb)
+    exp_fail = 1 if len(O.op_and(a, b)) else 0
+    assert r.returncode == (3 if exp_fail else 0), r.stdout + r.stderr
+    assert f"ASSERT_ZERO failures {exp_fail}" in r.stdout
+    g = O.grow(a, 2, 0, 1, 3)
+    m = (a[:, 0].min(), a[:, 1].min(), a[:, 2].max(), a[:, 3].max())
+    grid = O.grid(m, 10, 10, 10)
+    expect = {"or": O.op_or(a, b), "and": O.op_and(a, b), "fnot": O.op_not(a, b), "xor": O.op_xor(a, b),
+              "grow": g, "size": O.size(a, -1), "ring": O.op_not(g, a), "chk": np.zeros((0, 4), np.int32),
+              "self": O.self_and(a), "copy": O.op_or(a, b), "g2": O.grow(grid, 1, 1, 1, 1)}
+    for name, exp in expect.items():
+        assert np.array_equal(O.sort_boxes(read_klbin(tmp_path / f"{name}.bin")), exp), (mode, name)
+    assert np.array_equal(read_klbin(tmp_path / "grid.bin"), grid), mode      # GRID keeps its boxes, in order
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_delete_and_lazy_restore(tmp_path, gpu_engine):
+    """delete releases a layer; a layer loaded from a file is read again when
+    next used (src/controlparser.f90:440-441,627-642); a deleted layer with no
+    file is an error."""
+    fa = os.path.join(GOLDEN_IN, "SED_L67_D20.bin")
+    shutil.copy(fa, tmp_path)
+    script = f"""
+layer d1:a klbin {os.path.basename(fa)}
+exec run t1:g = d1:a GROW t1:nothing 0.001 0.001 0.001 0.001
+delete d1:
+exec run t1:h = d1:a GROW t1:nothing 0.002 0.002 0.002 0.002
+dump t1:g g.bin
+dump t1:h h.bin
+"""
+    r = run_driver(tmp_path, script)
+    assert r.returncode == 0, r.stdout + r.stderr
+    a = read_klbin(fa)
+    assert np.array_equal(O.sort_boxes(read_klbin(tmp_path / "g.bin")), O.grow(a, 1, 1, 1, 1))
+    assert np.array_equal(O.sort_boxes(read_klbin(tmp_path / "h.bin")), O.grow(a, 2, 2, 2, 2))
+    r = run_driver(tmp_path, script.replace("delete d1:", "delete t1:g"))
+    assert r.returncode != 0 and "deleted" in r.stderr

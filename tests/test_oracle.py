@@ -190,3 +190,53 @@ def test_cpu_baseline_equals_oracle(op):
     else:
         got, exp = O.par_op("size", a, args=(-2,)), O.size(a, -2)
     assert np.array_equal(O.canonical(got), exp)
+
+
+def count_raster(boxes, ox, oy, W, H):
+    g = np.zeros((W, H), np.int32)
+    for x1, y1, x2, y2 in np.asarray(boxes, np.int64).reshape(-1, 4):
+        g[x1 - ox:x2 - ox, y1 - oy:y2 - oy] += 1
+    return g
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_self_and_against_cell_counts(seed):
+    """or_self_and = the cells two or more boxes cover (brute force on a grid)."""
+    rng = np.random.default_rng(300 + seed)
+    n = int(rng.integers(1, 60))
+    x = rng.integers(0, 40, n)
+    y = rng.integers(0, 40, n)
+    a = np.stack([x, y, x + rng.integers(1, 12, n), y + rng.integers(1, 12, n)], 1).astype(np.int32)
+    if seed % 2:   # exact duplicates count twice
+        a = np.concatenate([a, a[: n // 3]])
+    g = count_raster(a, 0, 0, 60, 60) >= 2
+    assert np.array_equal(O.self_and(a), strips_from_raster(g, 0, 0))
+
+
+def test_self_and_known_answers():
+    assert len(O.self_and([[0, 0, 10, 10]])) == 0
+    assert np.array_equal(O.self_and([[0, 0, 10, 10], [0, 0, 10, 10]]), [[0, 0, 10, 10]])
+    # touching boxes share no area
+    assert len(O.self_and([[0, 0, 10, 10], [10, 0, 20, 10]])) == 0
+    assert np.array_equal(O.self_and([[0, 0, 10, 10], [5, 5, 15, 15]]), [[5, 5, 10, 10]])
+
+
+def test_grid_hand_evaluated_case():
+    """CreateGrid (src/design_impl.f90:159-275) on MBR (0,0,100,50), 2 rows,
+    3 columns, overlap 10: widths 34/33/33 (100 = 3*33 + 1, the first column
+    takes the extra unit), heights 25/25, padded 5 on each side, clamped."""
+    got = O.grid((0, 0, 100, 50), 2, 3, 10)
+    exp = [[0, 0, 39, 30], [29, 0, 72, 30], [62, 0, 100, 30],
+           [0, 20, 39, 50], [29, 20, 72, 50], [62, 20, 100, 50]]
+    assert np.array_equal(got, np.array(exp, np.int32))
+
+
+@pytest.mark.parametrize("rows,cols,overlap", [(1, 1, 0), (10, 10, 10), (7, 3, 0), (4, 9, 3), (10, 10, 1)])
+def test_grid_invariants(rows, cols, overlap):
+    m = (-37, 11, 1000, 515)
+    g = O.grid(m, rows, cols, overlap)
+    assert len(g) == rows * cols
+    assert (g[:, 0] >= m[0]).all() and (g[:, 2] <= m[2]).all() and (g[:, 1] >= m[1]).all() and (g[:, 3] <= m[3]).all()
+    assert O.klee_area(g) == (m[2] - m[0]) * (m[3] - m[1])       # the cells cover the domain
+    if overlap == 0:                                              # and tile it exactly
+        assert int(((g[:, 2] - g[:, 0]).astype(np.int64) * (g[:, 3] - g[:, 1])).sum()) == O.klee_area(g)

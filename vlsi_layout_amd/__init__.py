@@ -31,6 +31,7 @@ __all__ = [
     "VLError", "Engine", "Layer", "lib", "engine",
     "CalculateAND", "CalculateFrameNOT", "CalculateNOT", "CalculateOR", "CalculateXOR",
     "CalculateGROWLayer", "CalculateBoostOperation", "MergeHealLayer", "heal_boxes",
+    "CalculateFrameAND", "CalculateSingleLayerAND", "CreateGrid", "CreateEXTENT",
     "ASSERT_ZERO", "klee_area", "canonicalize", "xor_empty",
     "device_event_sort", "thrust_sort_boxes_direct", "thrust_sort_boxes_indirect",
     "synth_layer", "synth_tile_count",
@@ -108,6 +109,9 @@ def lib():
             "vl_grow": (C.c_int, [vp, vp, i32, i32, i32, i32, vp]),
             "vl_size": (C.c_int, [vp, vp, i32, vp]),
             "vl_merge": (C.c_int, [vp, vp, vp]),
+            "vl_self_and": (C.c_int, [vp, vp, vp]),
+            "vl_grid": (C.c_int, [vp, vp, i32, i32, i32, vp]),
+            "vl_extent": (C.c_int, [vp, vp, vp]),
             "vl_assert_zero": (C.c_int, [vp, vp, C.c_int, C.POINTER(i64)]),
             "vl_klee_area": (C.c_int, [vp, vp, C.POINTER(i64)]),
             "vl_canonicalize": (C.c_int, [vp, vp, vp]),
@@ -338,6 +342,30 @@ def CalculateBoostOperation(A, B, C_out, control_parameter, control_value=0):
         _check(lib().vl_size(e, A.handle, int(control_value), C_out.handle))
         return C_out
     raise VLError(VL_EINVAL, f"unknown control parameter {control_parameter}")
+
+
+def CalculateFrameAND(A, B, C_out):
+    """C = A AND B  (AND2, src/design.f90:296-299): same point set as '*'."""
+    return _binop("vl_and", A, B, C_out)
+
+
+def CalculateSingleLayerAND(A, C_out):
+    """C = the points covered by at least two boxes of A  (the one-input `AND`,
+    src/design_impl.f90:385-509)."""
+    _check(lib().vl_self_and(A.engine.handle, A.handle, C_out.handle))
+    return C_out
+
+
+def CreateGrid(A, C_out, rows, cols, grid_overlap):
+    """C = rows x cols padded sub-boxes of MBR(A)  (src/design_impl.f90:159-275)."""
+    _check(lib().vl_grid(A.engine.handle, A.handle, int(rows), int(cols), int(grid_overlap), C_out.handle))
+    return C_out
+
+
+def CreateEXTENT(A, C_out):
+    """C = MBR(A) as one box  (src/design_impl.f90:277-293)."""
+    _check(lib().vl_extent(A.engine.handle, A.handle, C_out.handle))
+    return C_out
 
 
 def MergeHealLayer(L):

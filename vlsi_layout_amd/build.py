@@ -20,6 +20,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libvlayout.so")
 FLIB = os.path.join(PKG, "libvlayout_f.so")
 FEXE = os.path.join(PKG, "vl_magparser")
+CHECK_EXE = os.path.join(ROOT, "tests", "fortran", "check_layer_ops")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
@@ -36,7 +37,7 @@ HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["vl_prims.hip", "vl_sort.hip", "vl_tile.hip", "vl_fused.hip", "vl_stitch.hip", "vl_dense.hip", "vl_window.hip", "vl_noalloc.hip", "vl_engine.hip"]
 CPP_SOURCES = ["vl_synth.cpp"]
-F_SOURCES = ["vl_layer_ops.f90", "vl_control.f90"]
+F_SOURCES = ["vl_layer_ops.f90", "vl_rect_boolean.f90", "vl_control.f90"]
 F_MAIN = "vl_magparser.f90"
 
 
@@ -87,6 +88,14 @@ def build_fortran():
     tmp = FEXE + ".tmp"
     _run([FLANG, "-o", tmp, main_o] + objs + ["-L", PKG, "-lvlayout", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FEXE)
+    # the test helper that calls the module procedures directly
+    check_src = os.path.join(ROOT, "tests", "fortran", "check_layer_ops.f90")
+    if os.path.exists(check_src):
+        check_o = os.path.join(BUILD, "check_layer_ops.o")
+        _run([FLANG, "-O2", "-module-dir", mod, "-I", mod, "-c", check_src, "-o", check_o])
+        tmp = CHECK_EXE + ".tmp"
+        _run([FLANG, "-o", tmp, check_o] + objs + ["-L", PKG, "-lvlayout", "-Wl,-rpath," + PKG])
+        os.replace(tmp, CHECK_EXE)
     return FLIB
 
 

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(DNT) void k_dense_tile(DenseArgs g) {
   const int nxu = g.nxy[2 * i], nyu = g.nxy[2 * i + 1];
   const int ncol = nxu - 1, nrows = nyu - 1;
   const int W = (nrows + 64) >> 6;
-  const bool twol = g.two && (d.op == TOP_AND || d.op == TOP_NOT || d.op == TOP_XOR);
+  const bool twol = g.two && (d.op == TOP_AND || d.op == TOP_NOT || d.op == TOP_XOR || d.op == TOP_OVERLAP);
   const int64_t gw = (int64_t)ncol * W;
   uint64_t* MA = g.grid + g.goff[i];
   uint64_t* MB = MA + gw;
@@ -188,7 +188,14 @@ __global__ __launch_bounds__(DNT) void k_dense_tile(DenseArgs g) {
       for (int w = w1; w <= w2; w++) {
         const int lo = (w == w1) ? (cy1 & 63) : 0;
         const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
-        atomicOr((unsigned long long*)&M[(int64_t)c * W + w], (unsigned long long)bits_range(lo, hi));
+        const uint64_t bits = bits_range(lo, hi);
+        if (d.op == TOP_OVERLAP && twol) {
+          // cells this box finds covered already are covered twice
+          const uint64_t old = atomicOr((unsigned long long*)&MA[(int64_t)c * W + w], (unsigned long long)bits);
+          if (old & bits) atomicOr((unsigned long long*)&MB[(int64_t)c * W + w], (unsigned long long)(old & bits));
+        } else {
+          atomicOr((unsigned long long*)&M[(int64_t)c * W + w], (unsigned long long)bits);
+        }
       }
   }
   // the atomics land in L2: the device-scope fence also drops any stale

@@ -994,7 +994,9 @@ int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, 
   launch_build_desc(g, start, top, desc, e->st);
   HIPCK(hipGetLastError());
   o.and_filter = top == TOP_AND && nb > 0 && !area && erode == 0;
-  return process_tiles(e, desc, (int)nt, entries, total, two_layer(top) && nb > 0, area, erode, o, 0);
+  // single-layer AND rasterises into a second mask as well
+  const bool two = (two_layer(top) && nb > 0) || top == TOP_OVERLAP;
+  return process_tiles(e, desc, (int)nt, entries, total, two, area, erode, o, 0);
 }
 
 // Moves result strips into layer c.
@@ -1155,6 +1157,7 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
       break;
     case TOP_NOT:
     case TOP_UNION:
+    case TOP_OVERLAP:
       dom = ea ? Box64{0, 0, 0, 0} : ma;
       break;
     default:   // OR, XOR
@@ -1782,6 +1785,48 @@ int vl_xor(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
   return binary(e, TOP_XOR, a, b, c);
 }
 int vl_merge(vl_engine* e, const vl_layer* a, vl_layer* c) { return binary(e, TOP_UNION, a, nullptr, c); }
+
+int vl_self_and(vl_engine* e, const vl_layer* a, vl_layer* c) { return binary(e, TOP_OVERLAP, a, nullptr, c); }
+
+int vl_grid(vl_engine* e, const vl_layer* a, int32_t rows, int32_t cols, int32_t overlap, vl_layer* c) {
+  VLCK(check_layers(e, a, nullptr, c));
+  if (rows < 1 || cols < 1) return fail(VL_EINVAL, "GRID: rows = %d, cols = %d must be positive", rows, cols);
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  Box64 m;
+  VLCK(layer_mbr(e, a, &m));
+  if (a->n == 0 || m.empty()) return fail(VL_EINVAL, "GRID: the input layer has no extent");
+  const int64_t n = (int64_t)rows * cols;
+  size_t r = 0;
+  vl_box* d = (vl_box*)e->pool.get((size_t)n * sizeof(vl_box), &r);
+  if (!d) return fail(VL_ENOMEM, "GRID output of %lld boxes", (long long)n);
+  launch_grid(m.x1, m.y1, m.x2, m.y2, rows, cols, overlap, d, e->st);
+  HIPCK(hipGetLastError());
+  if (c->d) e->pool.put(c->d, c->bytes);
+  c->d = d;
+  c->bytes = r;
+  c->n = n;
+  c->mbr_ok = false;
+  c->version++;
+  return VL_OK;
+}
+
+int vl_extent(vl_engine* e, const vl_layer* a, vl_layer* c) {
+  VLCK(check_layers(e, a, nullptr, c));
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  Box64 m;
+  VLCK(layer_mbr(e, a, &m));
+  if (a->n == 0 || m.empty()) return fail(VL_EINVAL, "EXTENT: the input layer has no extent");
+  const vl_box b{(int32_t)m.x1, (int32_t)m.y1, (int32_t)m.x2, (int32_t)m.y2};
+  VLCK(layer_reserve(c, 1));   // (a's boxes are not read again: c may be a)
+  VLCK(upload(e, c->d, &b, sizeof(b)));
+  HIPCK(hipStreamSynchronize(e->st));
+  c->n = 1;
+  c->mbr_ok = false;
+  c->version++;
+  return VL_OK;
+}
 
 int vl_grow(vl_engine* e, const vl_layer* a, int32_t east, int32_t north, int32_t west, int32_t south, vl_layer* c) {
   return binary(e, TOP_UNION, a, nullptr, c, Xform{east, north, west, south});

@@ -38,6 +38,7 @@ enum TileOp : int32_t {
   TOP_NOT = 2,     // A \ B
   TOP_OR = 3,
   TOP_XOR = 4,
+  TOP_OVERLAP = 5, // cells covered by at least two boxes of A (single-layer AND)
 };
 
 // Uniform grid over a domain.  Tile (c, r) covers
@@ -95,6 +96,9 @@ struct Expect8 {
 };
 void launch_expect(const void* dev, const void* expected, size_t bytes, uint32_t* flag, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+// GRID: rows x cols boxes of the domain (CreateGrid's arithmetic).
+void launch_grid(int64_t x_lo, int64_t y_lo, int64_t x_hi, int64_t y_hi, int rows, int cols, int64_t overlap,
+                 vl_box* out, hipStream_t st);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
 // MBR of boxes after a transform; validates boxes (ERR_INVALID_BOX).
 // mbr[4] must be pre-set to (INT_MAX, INT_MAX, INT_MIN, INT_MIN).

@@ -165,6 +165,39 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st) {
   hipLaunchKernelGGL((k_fill<uint64_t>), dim3(g), dim3(256), 0, st, p, v, n);
 }
 
+// rows x cols sub-boxes of the domain (x_lo, y_lo, x_hi, y_hi), row-major
+// from the bottom-left: base width / height plus one unit for the first
+// (extent mod count) columns / rows, padded by overlap/2 to the low side and
+// overlap - overlap/2 to the high side, clamped to the domain
+// (CreateGrid, src/design_impl.f90:159-275).
+__global__ void k_grid(int64_t x_lo, int64_t y_lo, int64_t x_hi, int64_t y_hi, int rows, int cols, int64_t overlap,
+                       vl_box* out) {
+  const int64_t n = (int64_t)rows * cols;
+  const int64_t w = x_hi - x_lo, h = y_hi - y_lo;
+  const int64_t cw = w / cols, ew = w % cols, rh = h / rows, eh = h % rows;
+  const int64_t pl = overlap / 2, pr = overlap - pl;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k / cols, j = k % cols;   // row i, column j (0-based)
+    const int64_t y0 = y_lo + i * rh + min(i, eh), y1 = y_lo + (i + 1) * rh + min(i + 1, eh);
+    const int64_t x0 = x_lo + j * cw + min(j, ew), x1 = x_lo + (j + 1) * cw + min(j + 1, ew);
+    vl_box b;
+    b.x1 = (int32_t)max(x_lo, x0 - pl);
+    b.x2 = (int32_t)min(x_hi, x1 + pr);
+    b.y1 = (int32_t)max(y_lo, y0 - pl);
+    b.y2 = (int32_t)min(y_hi, y1 + pr);
+    out[k] = b;
+  }
+}
+
+void launch_grid(int64_t x_lo, int64_t y_lo, int64_t x_hi, int64_t y_hi, int rows, int cols, int64_t overlap,
+                 vl_box* out, hipStream_t st) {
+  const int64_t n = (int64_t)rows * cols;
+  if (n <= 0) return;
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_grid, dim3((unsigned)g), dim3(256), 0, st, x_lo, y_lo, x_hi, y_hi, rows, cols, overlap, out);
+}
+
 void launch_expect(const void* dev, const void* expected, size_t bytes, uint32_t* flag, hipStream_t st) {
   const uint32_t* p = (const uint32_t*)dev;
   const uint32_t* x = (const uint32_t*)expected;

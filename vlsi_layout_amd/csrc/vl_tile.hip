@@ -368,7 +368,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
     int nA = d.nA;
     int n = d.nA + d.nB;
     const int op = d.op;
-    const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR);
+    const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR || op == TOP_OVERLAP);
     if (n > NMAX || n == 0) {
       if (lane == 0) {
         if (n > NMAX) sovf[novf] = ti;
@@ -561,7 +561,14 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
         for (int w = w1; w <= w2; w++) {
           const int lo = (w == w1) ? (cy1 & 63) : 0;
           const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
-          atomicOr((unsigned long long*)&M[c * W + w], (unsigned long long)bits_range(lo, hi));
+          const uint64_t bits = bits_range(lo, hi);
+          if (TWO && op == TOP_OVERLAP) {
+            // cells this box finds covered already are covered twice
+            const uint64_t old = atomicOr((unsigned long long*)&MA[c * W + w], (unsigned long long)bits);
+            if (old & bits) atomicOr((unsigned long long*)&MX[c * W + w], (unsigned long long)(old & bits));
+          } else {
+            atomicOr((unsigned long long*)&M[c * W + w], (unsigned long long)bits);
+          }
         }
     }
     __syncthreads();
@@ -758,7 +765,7 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
     const int nA = d.nA;
     const int n = d.nA + d.nB;
     const int op = d.op;
-    const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR);
+    const bool twol = TWO && (op == TOP_AND || op == TOP_NOT || op == TOP_XOR || op == TOP_OVERLAP);
     if (n > NMAX || n == 0) {
       if (tid == 0) {
         if (n > NMAX) sovf[novf] = ti;
@@ -872,7 +879,13 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
           for (int w = w1; w <= w2; w++) {
             const int lo = (w == w1) ? (cy1 & 63) : 0;
             const int hi = (w == w2) ? ((cy2 - 1) & 63) : 63;
-            atomicOr((unsigned long long*)&col[w], (unsigned long long)bits_range(lo, hi));
+            const uint64_t bits = bits_range(lo, hi);
+            if (TWO && op == TOP_OVERLAP) {
+              const uint64_t old = atomicOr((unsigned long long*)&MA[c * W + w], (unsigned long long)bits);
+              if (old & bits) atomicOr((unsigned long long*)&MB[c * W + w], (unsigned long long)(old & bits));
+            } else {
+              atomicOr((unsigned long long*)&col[w], (unsigned long long)bits);
+            }
           }
         }
       }

@@ -46,6 +46,7 @@ __device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
     case TOP_AND: return a & b;
     case TOP_NOT: return a & ~b;
     case TOP_XOR: return a ^ b;
+    case TOP_OVERLAP: return b;   // b holds the cells a second box reached
     default: return a | b;
   }
 }

@@ -5,8 +5,17 @@
 !   exec run LHS = RHS1 <op> RHS2       <op> one of + @ * ~ ! ^ %   (:409-476)
 !   exec run LHS = RHS1 NAME RHS2|nothing [args]                    (:478-585)
 !       NAME: OR2 AND2 NOT FRAMENOT XOR COPY EXTENT ASSERT_ZERO
+!             AND (one input: the points two or more boxes cover)
+!             GRID [rows cols overlap]  (the reference's fixed 10 10 10)
 !             GROW e n w s   SIZE d    (lengths in microns, times the
 !             precision and truncated to DBU as in :566-577)
+!   flush                         run the deferred operators (:613-626 writes
+!                                 the output designs there; design files are
+!                                 out of this path)
+!   delete NAME                   release a layer, or every layer of a design
+!                                 when NAME ends in ':' (:627-642); a released
+!                                 layer loaded from a file is read again when
+!                                 next used (lazy restore, :440-441)
 ! plus the few statements a stand-alone driver needs in place of the
 ! reference's design loading (modedesign / .mag, which are out of this path):
 !   precision N                   DBU per micron (default 1000)
@@ -15,6 +24,18 @@
 !   dump NAME FILE                write the layer's boxes as KLBIN
 !   area NAME                     print the exact union area
 !   resident on|off               keep results on the device between statements
+!   schedule immediate|batch|graph
+!                                 immediate: every exec run runs when read;
+!                                 batch (default): consecutive exec run
+!                                 statements are recorded into one program
+!                                 (independent chains run concurrently,
+!                                 operator groups over one layer fuse; the
+!                                 reference notes that a graph could be built,
+!                                 :308-309) and run at the next statement that
+!                                 needs their results (dump, area, layer,
+!                                 delete, flush, an operator programs do not
+!                                 take, the end of the file); graph: the same,
+!                                 with the program captured as a HIP graph
 ! Layer names are taken whole ("d1:poly"); "nothing" and "<db>:nothing" name
 ! no layer.  '#' starts a comment.
 module vl_control
@@ -28,6 +49,8 @@ module vl_control
   type :: NamedLayer
      character(len=:), allocatable :: name
      type(Layer) :: L
+     character(len=:), allocatable :: file   ! backing KLBIN file (lazy restore)
+     logical :: released = .false.
   end type NamedLayer
 
   type(NamedLayer), allocatable, target :: table(:)
@@ -36,7 +59,47 @@ module vl_control
   integer :: assert_failures = 0
   integer :: statements_run = 0
 
+  ! deferred operators (schedule batch / graph)
+  integer :: schedule_mode = 1   ! 0 immediate, 1 batch, 2 graph
+  type(c_ptr), save :: prog = c_null_ptr
+  integer :: npend = 0
+  integer, allocatable :: pend_out(:)           ! layers the pending program writes
+  integer :: npend_out = 0
+  integer, allocatable :: pend_assert(:, :)     ! (layer, line) of pending ASSERT_ZEROs
+  integer :: npend_assert = 0
+  integer(c_int), parameter :: OP_XOR = 0, OP_OR = 1, OP_AND = 2, OP_NOT = 3, OP_MERGE = 4, OP_SIZE = 5, &
+       OP_GROW = 6
+
   interface
+     function c_program_create(e, p) bind(C, name="vl_program_create") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e
+       type(c_ptr) :: p
+       integer(c_int) :: rc
+     end function c_program_create
+     function c_program_destroy(p) bind(C, name="vl_program_destroy") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: p
+       integer(c_int) :: rc
+     end function c_program_destroy
+     function c_program_add(p, op, a, b, c, args) bind(C, name="vl_program_add") result(rc)
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: p, a, b, c
+       integer(c_int), value :: op
+       integer(c_int32_t) :: args(4)
+       integer(c_int) :: rc
+     end function c_program_add
+     function c_program_run(p) bind(C, name="vl_program_run") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: p
+       integer(c_int) :: rc
+     end function c_program_run
+     function c_program_set_graph(p, enable) bind(C, name="vl_program_set_graph") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: p
+       integer(c_int), value :: enable
+       integer(c_int) :: rc
+     end function c_program_set_graph
      function c_synth_layer(config, seed, layer, n, out) bind(C, name="vl_synth_layer") result(rc)
        import :: c_int, c_int64_t, c_ptr
        integer(c_int), value :: config, layer
@@ -72,6 +135,8 @@ contains
        do i = 1, nlayers
           call move_alloc(table(i)%name, grown(i)%name)
           call move_layer(table(i)%L, grown(i)%L)
+          if (allocated(table(i)%file)) call move_alloc(table(i)%file, grown(i)%file)
+          grown(i)%released = table(i)%released
        end do
        call move_alloc(grown, table)
     end if
@@ -99,6 +164,14 @@ contains
     do i = 1, nlayers
        if (table(i)%name == trim(name)) then
           existing = i
+          if (table(i)%released) then
+             if (.not. allocated(table(i)%file)) then
+                write(error_unit, '(A,A,A,I0)') 'ERROR: layer was deleted: ', trim(name), ' on line ', line_no
+                error stop "ERROR: deleted layer used"
+             end if
+             call load_klbin(table(i)%L, table(i)%file)   ! lazy restore
+             table(i)%released = .false.
+          end if
           return
        end if
     end do
@@ -152,25 +225,107 @@ contains
     call layer_set_boxes(L, b, n)
   end subroutine load_synth
 
-  ! EXTENT (src/controlparser.f90:538-543): the layer's bounding box.
-  subroutine extent(src, dst)
-    type(Layer), intent(inout) :: src, dst
-    type(Box) :: m(1)
-    integer(int64) :: i
-    call layer_to_host(src)
-    if (src%n_used == 0) then
-       call layer_set_boxes(dst, m, 0_int64)
-       return
-    end if
-    m(1) = src%layer_boxes(1)
-    do i = 2, src%n_used
-       m(1)%x1 = min(m(1)%x1, src%layer_boxes(i)%x1)
-       m(1)%y1 = min(m(1)%y1, src%layer_boxes(i)%y1)
-       m(1)%x2 = max(m(1)%x2, src%layer_boxes(i)%x2)
-       m(1)%y2 = max(m(1)%y2, src%layer_boxes(i)%y2)
+  ! ---- deferred operators -----------------------------------------------------
+
+  subroutine note_output(il)
+    integer, intent(in) :: il
+    integer, allocatable :: g(:)
+    integer :: k
+    do k = 1, npend_out
+       if (pend_out(k) == il) return
     end do
-    call layer_set_boxes(dst, m, 1_int64)
-  end subroutine extent
+    if (.not. allocated(pend_out)) allocate(pend_out(64))
+    if (npend_out == size(pend_out)) then
+       allocate(g(2 * size(pend_out)))
+       g(1:npend_out) = pend_out(1:npend_out)
+       call move_alloc(g, pend_out)
+    end if
+    npend_out = npend_out + 1
+    pend_out(npend_out) = il
+  end subroutine note_output
+
+  ! Records op(table(ia), table(ib)) -> table(il) into the pending program.
+  subroutine defer(op, ia, ib, il, args)
+    integer(c_int), intent(in) :: op
+    integer, intent(in) :: ia, ib, il
+    integer(c_int32_t), intent(in) :: args(4)
+    type(c_ptr) :: ha, hb, hc
+    integer(c_int32_t) :: a(4)
+    if (.not. c_associated(prog)) call vl_check(c_program_create(vl_engine_handle(), prog), "vl_program_create")
+    ha = layer_input_dev(table(ia)%L)
+    hb = c_null_ptr
+    if (ib > 0) hb = layer_input_dev(table(ib)%L)
+    hc = layer_output_dev(table(il)%L)
+    a = args
+    call vl_check(c_program_add(prog, op, ha, hb, hc, a), "vl_program_add")
+    call note_output(il)
+    npend = npend + 1
+  end subroutine defer
+
+  subroutine defer_assert(i1, line_no)
+    integer, intent(in) :: i1, line_no
+    integer, allocatable :: g(:, :)
+    if (.not. allocated(pend_assert)) allocate(pend_assert(2, 64))
+    if (npend_assert == size(pend_assert, 2)) then
+       allocate(g(2, 2 * size(pend_assert, 2)))
+       g(:, 1:npend_assert) = pend_assert(:, 1:npend_assert)
+       call move_alloc(g, pend_assert)
+    end if
+    npend_assert = npend_assert + 1
+    pend_assert(:, npend_assert) = [i1, line_no]
+  end subroutine defer_assert
+
+  subroutine report_assert(i1, line_no)
+    integer, intent(in) :: i1, line_no
+    if (LayerAssertZero(table(i1)%L) > 0) then
+       write(output_unit, '(A,I5,A)') '************ ASSERT_ZERO fail on layer. Line: ', line_no, &
+            ' **********************'
+       assert_failures = assert_failures + 1
+    end if
+  end subroutine report_assert
+
+  ! Runs the pending program (if any), then the deferred ASSERT_ZERO checks
+  ! in statement order.
+  subroutine flush_pending()
+    integer :: k
+    if (npend > 0) then
+       if (schedule_mode == 2) call vl_check(c_program_set_graph(prog, 1_c_int), "vl_program_set_graph")
+       call vl_check(c_program_run(prog), "vl_program_run")
+       do k = 1, npend_out
+          call layer_output_done(table(pend_out(k))%L)
+       end do
+    end if
+    do k = 1, npend_assert
+       call report_assert(pend_assert(1, k), pend_assert(2, k))
+    end do
+    if (c_associated(prog)) call vl_check(c_program_destroy(prog), "vl_program_destroy")
+    prog = c_null_ptr
+    npend = 0
+    npend_out = 0
+    npend_assert = 0
+  end subroutine flush_pending
+
+  ! delete NAME / delete DB: (every layer whose name starts with "DB:").
+  subroutine delete_layers(name)
+    character(len=*), intent(in) :: name
+    integer :: i, n
+    logical :: all_of_db
+    n = len_trim(name)
+    all_of_db = name(n:n) == ':'
+    do i = 1, nlayers
+       if (all_of_db) then
+          if (len(table(i)%name) < n) cycle
+          if (table(i)%name(1:n) /= name(1:n)) cycle
+       else
+          if (table(i)%name /= trim(name)) cycle
+       end if
+       call layer_release(table(i)%L)
+       if (allocated(table(i)%L%layer_boxes)) deallocate(table(i)%L%layer_boxes)
+       table(i)%L%n_used = 0
+       table(i)%L%n_alloc = 0
+       table(i)%released = .true.
+    end do
+  end subroutine delete_layers
 
   ! Splits s into blank-separated words.
   subroutine split_words(s, w, nw)
@@ -213,14 +368,17 @@ contains
     character(len=256), intent(in) :: w(:)
     integer, intent(in) :: nw, line_no
     character(len=256) :: opw
-    integer :: il, i1, i2, k
+    integer :: il, i1, i2, k, rows, cols, ovl
     integer(int32) :: iv(4)
     integer(int64) :: d
+    integer(c_int) :: code
+    logical :: later
     ! w: exec run LHS = RHS1 OP RHS2 [args]
     if (nw < 7 .or. w(4) /= '=') then
        write(error_unit, '(A,I0)') 'SYNTAX ERROR on line: ', line_no
        error stop "SYNTAX ERROR on exec run"
     end if
+    later = schedule_mode /= 0
     opw = w(6)
     i1 = existing(w(5), line_no)
     if (len_trim(opw) == 1 .and. scan(opw, '+@*%^~!') == 1) then
@@ -229,51 +387,100 @@ contains
        il = layer_index(w(3))
        select case (opw(1:1))
        case ('+')
-          call CalculateBoostOperation(table(i1)%L, table(i2)%L, table(il)%L, K_BOOST_CONTROL_OR, 0_int64)
+          code = OP_OR
        case ('*')
-          call CalculateAND(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('@')
-          call CalculateNOT(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('~')
-          call CalculateFrameNOT(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('!')
-          call CalculateBoostOperation(table(i1)%L, table(i2)%L, table(il)%L, K_BOOST_CONTROL_NOT, 0_int64)
+          code = OP_AND
        case ('^')
-          call CalculateBoostOperation(table(i1)%L, table(i2)%L, table(il)%L, K_BOOST_CONTROL_XOR, 0_int64)
+          code = OP_XOR
+       case default   ! '@', '~', '!': the same point set A \ B
+          code = OP_NOT
        end select
+       if (later) then
+          call defer(code, i1, i2, il, [0, 0, 0, 0])
+       else
+          select case (opw(1:1))
+          case ('+')
+             call CalculateBoostOperation(table(i1)%L, table(i2)%L, table(il)%L, K_BOOST_CONTROL_OR, 0_int64)
+          case ('*')
+             call CalculateAND(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('@')
+             call CalculateNOT(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('~')
+             call CalculateFrameNOT(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('!')
+             call CalculateBoostOperation(table(i1)%L, table(i2)%L, table(il)%L, K_BOOST_CONTROL_NOT, 0_int64)
+          case ('^')
+             call CalculateBoostOperation(table(i1)%L, table(i2)%L, table(il)%L, K_BOOST_CONTROL_XOR, 0_int64)
+          end select
+       end if
        statements_run = statements_run + 1
        return
     end if
     select case (trim(opw))
     case ('ASSERT_ZERO')
        if (.not. is_nothing(w(7))) error stop "ASSERT_ZERO must use nothing as second layer"
-       if (LayerAssertZero(table(i1)%L) > 0) then
-          write(output_unit, '(A,I5,A)') '************ ASSERT_ZERO fail on layer. Line: ', line_no, &
-               ' **********************'
-          assert_failures = assert_failures + 1
+       if (later) then
+          call defer_assert(i1, line_no)
+       else
+          call report_assert(i1, line_no)
        end if
     case ('OR2', 'AND2', 'NOT', 'FRAMENOT', 'XOR')
        i2 = existing(w(7), line_no)
        il = layer_index(w(3))
-       select case (trim(opw))
-       case ('OR2')
-          call CalculateOR(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('AND2')
-          call CalculateAND(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('NOT')
-          call CalculateNOT(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('FRAMENOT')
-          call CalculateFrameNOT(table(i1)%L, table(i2)%L, table(il)%L)
-       case ('XOR')
-          call CalculateXOR(table(i1)%L, table(i2)%L, table(il)%L)
-       end select
+       if (later) then
+          select case (trim(opw))
+          case ('OR2')
+             code = OP_OR
+          case ('AND2')
+             code = OP_AND
+          case ('XOR')
+             code = OP_XOR
+          case default
+             code = OP_NOT
+          end select
+          call defer(code, i1, i2, il, [0, 0, 0, 0])
+       else
+          select case (trim(opw))
+          case ('OR2')
+             call CalculateOR(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('AND2')
+             call CalculateFrameAND(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('NOT')
+             call CalculateNOT(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('FRAMENOT')
+             call CalculateFrameNOT(table(i1)%L, table(i2)%L, table(il)%L)
+          case ('XOR')
+             call CalculateXOR(table(i1)%L, table(i2)%L, table(il)%L)
+          end select
+       end if
     case ('COPY')
        il = layer_index(w(3))
-       call CopyLayer(table(il)%L, table(i1)%L)
-    case ('EXTENT')
-       if (.not. is_nothing(w(7))) error stop "EXTENT must use nothing as second layer"
+       if (later) then
+          call defer(OP_MERGE, i1, 0, il, [0, 0, 0, 0])
+       else
+          call CopyLayer(table(il)%L, table(i1)%L)
+       end if
+    case ('EXTENT', 'AND', 'GRID')
+       ! operators a program does not take: run what is pending, then this one
+       call flush_pending()
        il = layer_index(w(3))
-       call extent(table(i1)%L, table(il)%L)
+       select case (trim(opw))
+       case ('EXTENT')
+          if (.not. is_nothing(w(7))) error stop "EXTENT must use nothing as second layer"
+          call CreateEXTENT(table(i1)%L, table(il)%L)
+       case ('AND')
+          call CalculateSingleLayerAND(table(i1)%L, table(il)%L)
+       case ('GRID')
+          rows = 10
+          cols = 10
+          ovl = 10
+          if (nw >= 10) then
+             rows = nint(to_real(w(8), line_no))
+             cols = nint(to_real(w(9), line_no))
+             ovl = nint(to_real(w(10), line_no))
+          end if
+          call CreateGrid(table(i1)%L, table(il)%L, rows, cols, ovl)
+       end select
     case ('GROW')
        if (.not. is_nothing(w(7))) error stop "GROW must use nothing as second layer"
        if (nw < 11) error stop "SYNTAX ERROR: GROW needs four lengths"
@@ -281,13 +488,22 @@ contains
           iv(k) = int(real(used_precision, real64) * to_real(w(7 + k), line_no), int32)
        end do
        il = layer_index(w(3))
-       call CalculateGROWLayer(table(i1)%L, table(il)%L, iv)
+       if (later) then
+          call defer(OP_GROW, i1, 0, il, iv)
+       else
+          call CalculateGROWLayer(table(i1)%L, table(il)%L, iv)
+       end if
     case ('SIZE')
        if (.not. is_nothing(w(7))) error stop "SIZE must use nothing as second layer"
        if (nw < 8) error stop "SYNTAX ERROR: SIZE needs a length"
        d = int(to_real(w(8), line_no) * real(used_precision, real64), int64)
        il = layer_index(w(3))
-       call CalculateBoostOperation(table(i1)%L, table(i1)%L, table(il)%L, K_BOOST_CONTROL_SIZE, d)
+       if (later) then
+          if (d < -huge(0_int32) - 1_int64 .or. d > huge(0_int32)) error stop "SIZE amount out of int32 range"
+          call defer(OP_SIZE, i1, 0, il, [int(d, int32), 0, 0, 0])
+       else
+          call CalculateBoostOperation(table(i1)%L, table(i1)%L, table(il)%L, K_BOOST_CONTROL_SIZE, d)
+       end if
     case default
        write(error_unit, '(A,A,A,I0)') 'UNKNOWN primary_operator: ', trim(opw), ' on line: ', line_no
        error stop "SYNTAX ERROR on TEXT based operator"
@@ -316,10 +532,30 @@ contains
        if (p > 0) line(p:) = ' '
        call split_words(line, w, nw)
        if (nw == 0) cycle
+       ! every statement other than exec run needs the results so far
+       if (trim(w(1)) /= 'exec') call flush_pending()
        select case (trim(w(1)))
        case ('exec')
           if (nw < 2 .or. w(2) /= 'run') error stop "SYNTAX ERROR: expected exec run"
           call exec_run(w, nw, line_no)
+       case ('schedule')
+          select case (trim(w(2)))
+          case ('immediate')
+             schedule_mode = 0
+          case ('batch')
+             schedule_mode = 1
+          case ('graph')
+             schedule_mode = 2
+          case default
+             error stop "SYNTAX ERROR: schedule immediate|batch|graph"
+          end select
+       case ('flush')
+          continue
+       case ('delete')
+          if (nw < 2) error stop "SYNTAX ERROR: delete NAME"
+          call delete_layers(w(2))
+       case ('end')
+          exit
        case ('precision')
           used_precision = nint(to_real(w(2), line_no), int64)
        case ('layer')
@@ -327,6 +563,8 @@ contains
           i = layer_index(w(2))
           if (w(3) == 'klbin') then
              call load_klbin(table(i)%L, w(4))
+             table(i)%file = trim(w(4))
+             table(i)%released = .false.
           else if (w(3) == 'synth' .and. nw >= 7) then
              call load_synth(table(i)%L, nint(to_real(w(4), line_no)), int(to_real(w(5), line_no), int64), &
                   nint(to_real(w(6), line_no)), int(to_real(w(7), line_no), int64))
@@ -346,6 +584,7 @@ contains
        end select
     end do
     close(u)
+    call flush_pending()
     call vl_fortran_sync()
   end subroutine run_control_file
 

@@ -2,9 +2,12 @@
 ! the gfx950 engine (libvlayout.so) through its C ABI (include/vl_api.h).
 !
 ! Procedure names, argument order and meaning follow the reference:
-!   CalculateAND, CalculateFrameNOT, CalculateNOT, CalculateOR,
+!   CalculateAND, CalculateFrameNOT, CalculateNOT, CalculateOR, CalculateFrameAND,
 !   CalculateGROWLayer, CalculateBoostOperation, MergeHealLayer
 !                                   src/design.f90:20-31, interfaces :154-313
+!   CalculateSingleLayerAND, CreateGrid, CreateEXTENT
+!                                   src/design_impl.f90:385-509, :159-293
+!   merge_boxes_using_scanline      src/boxmerge.f90:273-304
 !   heal_boxes                      src/polygon_fracture.f90:805-808
 !   sort_event_trackers             src/event_sorter_apif.f90:30-43
 !   SortBoxesDirect/SortBoxesIndirect  src/gpu_morton_sort.f90:41-77
@@ -34,6 +37,9 @@ module vl_layer_ops
   public :: Box, XYTracker, Layer
   public :: CalculateAND, CalculateFrameNOT, CalculateNOT, CalculateOR, CalculateXOR
   public :: CalculateGROWLayer, CalculateBoostOperation, MergeHealLayer, heal_boxes
+  public :: CalculateFrameAND, CalculateSingleLayerAND, CreateGrid, CreateEXTENT
+  public :: merge_boxes_using_scanline, vl_check
+  public :: layer_input_dev, layer_output_dev, layer_output_done
   public :: sort_event_trackers, SortBoxesDirect, SortBoxesIndirect
   public :: LayerAssertZero, LayerKleeArea, LayerXorEmpty, CopyLayer
   public :: layer_set_boxes, layer_to_host, layer_boxes_changed, layer_release
@@ -141,6 +147,22 @@ module vl_layer_ops
        type(c_ptr), value :: e, a, c
        integer(c_int) :: rc
      end function c_merge
+     function c_self_and(e, a, c) bind(C, name="vl_self_and") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, c
+       integer(c_int) :: rc
+     end function c_self_and
+     function c_grid(e, a, rows, cols, overlap, c) bind(C, name="vl_grid") result(rc)
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: e, a, c
+       integer(c_int32_t), value :: rows, cols, overlap
+       integer(c_int) :: rc
+     end function c_grid
+     function c_extent(e, a, c) bind(C, name="vl_extent") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e, a, c
+       integer(c_int) :: rc
+     end function c_extent
      function c_canonicalize(e, a, c) bind(C, name="vl_canonicalize") result(rc)
        import :: c_int, c_ptr
        type(c_ptr), value :: e, a, c
@@ -233,6 +255,13 @@ contains
     write(error_unit, *)
     error stop "vl_layer_ops: engine call failed"
   end subroutine check
+
+  ! The engine status check for modules built on this one.
+  subroutine vl_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    call check(rc, what)
+  end subroutine vl_check
 
   subroutine vl_fortran_init(device)
     integer, intent(in), optional :: device
@@ -334,6 +363,37 @@ contains
     end if
   end subroutine layer_release
 
+  ! Device handle of an input the operator may not modify (intent(in)): the
+  ! layer's own device copy when it is current, else a temporary upload of
+  ! its boxes (released with release_input).  The layer itself is unchanged.
+  subroutine input_handle(A, h, temp)
+    type(Layer), intent(in), target :: A
+    type(c_ptr), intent(out) :: h
+    logical, intent(out) :: temp
+    if (A%dev_valid .and. c_associated(A%dev)) then
+       h = A%dev
+       temp = .false.
+       return
+    end if
+    call vl_fortran_init()
+    h = c_null_ptr
+    call check(c_layer_create(engine, h), "vl_layer_create")
+    temp = .true.
+    if (A%n_used > 0) then
+       if (.not. allocated(A%layer_boxes)) error stop "vl_layer_ops: layer has n_used > 0 but no boxes"
+       if (size(A%layer_boxes, kind=int64) < A%n_used) error stop "vl_layer_ops: n_used exceeds layer_boxes"
+       call check(c_layer_upload(h, c_loc(A%layer_boxes(1)), A%n_used), "vl_layer_upload")
+    else
+       call check(c_layer_upload(h, c_null_ptr, 0_c_int64_t), "vl_layer_upload")
+    end if
+  end subroutine input_handle
+
+  subroutine release_input(h, temp)
+    type(c_ptr), intent(in) :: h
+    logical, intent(in) :: temp
+    if (temp) call check(c_layer_destroy(h), "vl_layer_destroy")
+  end subroutine release_input
+
   ! The result of an operator is on the device; record it (and copy it back
   ! in eager mode).  Results are canonical, i.e. healed.
   subroutine finish_output(C)
@@ -346,6 +406,32 @@ contains
     C%layerState = LAYER_STATE_HEAL
     if (vl_eager_host) call layer_to_host(C)
   end subroutine finish_output
+
+  ! Deferred operators (a recorded program): the device handle of an input
+  ! (uploaded when its device copy is stale) ...
+  function layer_input_dev(L) result(h)
+    type(Layer), intent(inout), target :: L
+    type(c_ptr) :: h
+    call to_device(L)
+    h = L%dev
+  end function layer_input_dev
+
+  ! ... of a layer a deferred operator writes (its device copy is taken as
+  ! current from here on: the program writes it before any of its readers) ...
+  function layer_output_dev(L) result(h)
+    type(Layer), intent(inout) :: L
+    type(c_ptr) :: h
+    call ensure_handle(L)
+    L%dev_valid = .true.
+    L%host_valid = .false.
+    h = L%dev
+  end function layer_output_dev
+
+  ! ... and, once the program has run, the result recorded like any operator's.
+  subroutine layer_output_done(L)
+    type(Layer), intent(inout) :: L
+    call finish_output(L)
+  end subroutine layer_output_done
 
   subroutine binop(f, what, A, B, C)
     procedure(binop_fn) :: f
@@ -390,15 +476,69 @@ contains
     call binop(c_xor, "CalculateXOR", A, B, C)
   end subroutine CalculateXOR
 
-  ! C = A grown by ivar = (E, N, W, S)  (src/design_impl.f90:571-586)
+  ! C = A grown by ivar = (E, N, W, S)  (src/design_impl.f90:571-586; A is
+  ! intent(in) as in src/design.f90:243)
   subroutine CalculateGROWLayer(A, C, ivar)
-    type(Layer), intent(inout) :: A, C
+    type(Layer), intent(in), target :: A
+    type(Layer), intent(inout) :: C
     integer(int32), intent(in) :: ivar(4)
-    call to_device(A)
+    type(c_ptr) :: h
+    logical :: temp
+    call input_handle(A, h, temp)
     call ensure_handle(C)
-    call check(c_grow(engine, A%dev, ivar(1), ivar(2), ivar(3), ivar(4), C%dev), "CalculateGROWLayer")
+    call check(c_grow(engine, h, ivar(1), ivar(2), ivar(3), ivar(4), C%dev), "CalculateGROWLayer")
+    call release_input(h, temp)
     call finish_output(C)
   end subroutine CalculateGROWLayer
+
+  ! C = A AND B  (AND2, src/design.f90:296-299): the same point set as '*'.
+  subroutine CalculateFrameAND(A, B, C)
+    type(Layer), intent(inout) :: A, B, C
+    call binop(c_and, "CalculateFrameAND", A, B, C)
+  end subroutine CalculateFrameAND
+
+  ! C = the points covered by at least two boxes of A (the one-input AND,
+  ! src/design_impl.f90:385-509, src/controlparser.f90:545-549).
+  subroutine CalculateSingleLayerAND(input_layer_A, output_layer)
+    type(Layer), intent(inout) :: input_layer_A, output_layer
+    call to_device(input_layer_A)
+    call ensure_handle(output_layer)
+    call check(c_self_and(engine, input_layer_A%dev, output_layer%dev), "CalculateSingleLayerAND")
+    call finish_output(output_layer)
+  end subroutine CalculateSingleLayerAND
+
+  ! output_layer = rows x cols sub-boxes of MBR(input_layer), each padded by
+  ! GridOverlap and clamped to the MBR, row-major (src/design_impl.f90:159-275).
+  ! The boxes are kept as they are (the reference marks them healed so that
+  ! they are not merged).
+  subroutine CreateGrid(input_layer, output_layer, rows, cols, GridOverlap)
+    type(Layer), intent(in), target :: input_layer
+    type(Layer), intent(inout) :: output_layer
+    integer, intent(in) :: rows, cols
+    integer, intent(in) :: GridOverlap
+    type(c_ptr) :: h
+    logical :: temp
+    if (rows < 1 .or. cols < 1) error stop "CreateGrid: rows and cols must be positive"
+    call input_handle(input_layer, h, temp)
+    call ensure_handle(output_layer)
+    call check(c_grid(engine, h, int(rows, c_int32_t), int(cols, c_int32_t), int(GridOverlap, c_int32_t), &
+         output_layer%dev), "CreateGrid")
+    call release_input(h, temp)
+    call finish_output(output_layer)
+  end subroutine CreateGrid
+
+  ! output_layer = MBR(input_layer) as one box (src/design_impl.f90:277-293).
+  subroutine CreateEXTENT(input_layer, output_layer)
+    type(Layer), intent(in), target :: input_layer
+    type(Layer), intent(inout) :: output_layer
+    type(c_ptr) :: h
+    logical :: temp
+    call input_handle(input_layer, h, temp)
+    call ensure_handle(output_layer)
+    call check(c_extent(engine, h, output_layer%dev), "CreateEXTENT")
+    call release_input(h, temp)
+    call finish_output(output_layer)
+  end subroutine CreateEXTENT
 
   ! Boost-coded operators (src/design_impl.f90:931-951, src/boost_polygon_api.cpp:97-156):
   ! XOR=0, OR=1, AND=2, NOT=3, MERGE=4 (A only), SIZE=5 (A only, by control_value).
@@ -470,6 +610,33 @@ contains
     call check(c_layer_destroy(a), "vl_layer_destroy")
     call check(c_layer_destroy(c), "vl_layer_destroy")
   end subroutine heal_boxes
+
+  ! boxes -> the union of boxes as maximal strips, sorted by (x1, y1, x2, y2),
+  ! reallocated to the exact count (src/boxmerge.f90:273-304).  The reference
+  ! joins touching boxes that form a rectangle and stops on overlapping ones;
+  ! here any input is accepted and the result is the canonical decomposition
+  ! of the same point set.
+  subroutine merge_boxes_using_scanline(boxes)
+    type(Box), allocatable, intent(inout), target :: boxes(:)
+    type(Box), allocatable, target :: merged(:)
+    type(c_ptr) :: a, c
+    integer(c_int64_t) :: n
+    if (.not. allocated(boxes)) return
+    if (size(boxes) == 0) return
+    call vl_fortran_init()
+    a = c_null_ptr
+    c = c_null_ptr
+    call check(c_layer_create(engine, a), "vl_layer_create")
+    call check(c_layer_create(engine, c), "vl_layer_create")
+    call check(c_layer_upload(a, c_loc(boxes(1)), size(boxes, kind=c_int64_t)), "vl_layer_upload")
+    call check(c_canonicalize(engine, a, c), "merge_boxes_using_scanline")
+    call check(c_layer_size(c, n), "vl_layer_size")
+    allocate(merged(n))
+    if (n > 0) call check(c_layer_download(c, c_loc(merged(1)), n, n), "vl_layer_download")
+    call move_alloc(from=merged, to=boxes)
+    call check(c_layer_destroy(a), "vl_layer_destroy")
+    call check(c_layer_destroy(c), "vl_layer_destroy")
+  end subroutine merge_boxes_using_scanline
 
   ! ---- checks -----------------------------------------------------------------
 
