@@ -149,6 +149,24 @@ int vl_grid(vl_engine* e, const vl_layer* a, int32_t rows, int32_t cols, int32_t
 /* C = the one box MBR(A).  Replaces CreateEXTENT (src/design_impl.f90:277-293).
  * VL_EINVAL when A is empty. */
 int vl_extent(vl_engine* e, const vl_layer* a, vl_layer* c);
+/* Polygon numbering of A's boxes in their stored order: the PNUM step of
+ * PreprocessLayer, PerformMerge (src/pnummerge.f90:279-385).  Boxes whose
+ * closed extents meet (overlap, shared edge or shared corner) are connected;
+ * labels[i] = 0 when box i meets no other box, else the 1-based number of
+ * its connected component, components numbered in the order of their
+ * smallest box index (the reference's union-find and uf_fullreduce,
+ * src/datastructures.f90:90-171).  *ncomp = components of two or more boxes.
+ * *overlap_area = the sum over pairs i < k of their positive intersection
+ * areas; *touch_perimeter = the sum of the perimeters 2 (w + h) of the
+ * zero-area intersections of touching pairs, 0 when overlap_area > 0 (both
+ * exact here; the reference accumulates them in real64).  labels: n int64 in
+ * host or device memory. */
+int vl_pnum(vl_engine* e, const vl_layer* a, int64_t* labels, int64_t* ncomp, int64_t* overlap_area,
+            int64_t* touch_perimeter);
+/* Pairs i < k of A's boxes that overlap with positive area
+ * (ComputeInteractionsGPU, src/interaction.f90:204-310); CertifyHealing
+ * (src/design_impl.f90:35-56) holds when it is 0. */
+int vl_interaction_count(vl_engine* e, const vl_layer* a, int64_t* count);
 /* Box count of A (the ASSERT_ZERO check, src/controlparser.f90:524-530).
  * Returns VL_EASSERT when *count>0 and abort_on_nonzero!=0. */
 int vl_assert_zero(vl_engine* e, const vl_layer* a, int abort_on_nonzero, int64_t* count);

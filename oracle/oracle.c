@@ -30,6 +30,11 @@
  *                           (0,0,10,10)\(2,2,8,8) (src/test_rectboolean.f90); area identities.
  *   or_grow / or_size       raster dilation/erosion by a square; single-box closed forms.
  *   or_self_and             raster count of boxes per cell (>= 2); identical boxes.
+ *   or_pnum                 brute-force connected components of the closed-touch
+ *                           graph (all pairs, breadth-first search) with the label
+ *                           rule; known answers for edge, corner and no contact;
+ *                           areas by brute force over all pairs.
+ *   or_interactions         brute force over all pairs.
  *   or_grid                 a hand-evaluated case of CreateGrid's arithmetic; area and
  *                           tiling invariants.
  *   sorts                   library equivalence (Python sorted() with the same keys).
@@ -578,6 +583,78 @@ int64_t or_self_and(const ob_box* a, int64_t na, ob_box** out) {
   const int64_t r = or_canonical(fr, nf, out);
   free(fr);
   return r;
+}
+
+/* PerformMerge (pnummerge.f90:279-385): every pair i < k whose closed       */
+/* extents meet (:336-341, <= on both axes) is an edge; process_edges         */
+/* (:99-110) inserts both ends and merges them in the union-find of           */
+/* datastructures.f90: the larger root is hooked under the smaller (:90-112); */
+/* uf_fullreduce (:143-171) renumbers the roots 1.. in index order and leaves */
+/* never-inserted boxes at 0.  The overlap area sums the positive             */
+/* intersection areas, the perimeter the zero-area intersections' 2 (w + h),  */
+/* reset to 0 when the area is not 0 (:377-379).  A sweep over the boxes      */
+/* sorted by x1 replaces the RTree query.                                     */
+static int64_t uf_root(int64_t* p, int64_t x) {
+  while (p[x] != x) x = p[x];
+  return x;
+}
+int64_t or_pnum(const ob_box* a, int64_t n, int64_t* labels, int64_t* area, int64_t* perim) {
+  *area = 0;
+  *perim = 0;
+  if (n <= 0) return 0;
+  int64_t* par = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+  unsigned char* ins = (unsigned char*)calloc((size_t)n, 1);
+  for (int64_t i = 0; i < n; i++) par[i] = i;
+  int64_t* id = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+  for (int64_t i = 0; i < n; i++) id[i] = i;
+  g_sort_base = a;
+  qsort(id, (size_t)n, sizeof(int64_t), idx_x1_cmp);
+  int64_t ar = 0, pe = 0;
+  for (int64_t p = 0; p < n; p++) {
+    const ob_box A = a[id[p]];
+    for (int64_t q = p + 1; q < n; q++) {
+      const ob_box B = a[id[q]];
+      if (B.x1 > A.x2) break;                          /* closed: x1 == x2 still meets */
+      const int32_t ix1 = imax(A.x1, B.x1), iy1 = imax(A.y1, B.y1), ix2 = imin(A.x2, B.x2), iy2 = imin(A.y2, B.y2);
+      if (ix1 > ix2 || iy1 > iy2) continue;
+      const int64_t i = id[p], k = id[q];
+      ins[i] = ins[k] = 1;
+      const int64_t ri = uf_root(par, i), rk = uf_root(par, k);
+      if (ri < rk) par[rk] = ri; else if (rk < ri) par[ri] = rk;
+      if (ix1 < ix2 && iy1 < iy2) ar += (int64_t)(ix2 - ix1) * (iy2 - iy1);
+      else pe += 2 * ((int64_t)(ix2 - ix1) + (iy2 - iy1));
+    }
+  }
+  int64_t next = 0;
+  int64_t* map = (int64_t*)calloc((size_t)n, sizeof(int64_t));
+  for (int64_t i = 0; i < n; i++)
+    if (ins[i] && uf_root(par, i) == i) map[i] = ++next;
+  for (int64_t i = 0; i < n; i++) labels[i] = ins[i] ? map[uf_root(par, i)] : 0;
+  free(par); free(ins); free(id); free(map);
+  *area = ar;
+  *perim = ar ? 0 : pe;
+  return next;
+}
+
+/* ComputeInteractionsGPU (interaction.f90:204-310): pairs i < j with          */
+/* max(x1) < min(x2) and max(y1) < min(y2) (strict, :268-273).                */
+int64_t or_interactions(const ob_box* a, int64_t n) {
+  if (n < 2) return 0;
+  int64_t* id = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+  for (int64_t i = 0; i < n; i++) id[i] = i;
+  g_sort_base = a;
+  qsort(id, (size_t)n, sizeof(int64_t), idx_x1_cmp);
+  int64_t c = 0;
+  for (int64_t p = 0; p < n; p++) {
+    const ob_box A = a[id[p]];
+    for (int64_t q = p + 1; q < n; q++) {
+      const ob_box B = a[id[q]];
+      if (B.x1 >= A.x2) break;
+      if (imax(A.y1, B.y1) < imin(A.y2, B.y2)) c++;
+    }
+  }
+  free(id);
+  return c;
 }
 
 /* CreateGrid (design_impl.f90:159-275): rows x cols cells of MBR m, base    */

@@ -55,6 +55,13 @@ int64_t or_size(const ob_box* a, int64_t na, int32_t d, ob_box** out);
  * area intersections of pairs i < k (CalculateSingleLayerAND,
  * src/design_impl.f90:385-509), canonical. */
 int64_t or_self_and(const ob_box* a, int64_t na, ob_box** out);
+/* PerformMerge (src/pnummerge.f90:279-385): labels[i] (0 = touches nothing,
+ * else the component's 1-based number in the order of its smallest index),
+ * the component count (returned), the positive-area sum and the zero-area
+ * perimeter sum (0 when the area is not 0). */
+int64_t or_pnum(const ob_box* a, int64_t n, int64_t* labels, int64_t* area, int64_t* perim);
+/* Pairs overlapping with positive area (src/interaction.f90:204-310). */
+int64_t or_interactions(const ob_box* a, int64_t n);
 /* CreateGrid (src/design_impl.f90:159-275): rows x cols padded cells of m,
  * row-major; -1 when rows or cols < 1 or m is not a valid box. */
 int64_t or_grid(ob_box m, int32_t rows, int32_t cols, int32_t overlap, ob_box** out);

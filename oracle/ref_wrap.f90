@@ -7,12 +7,19 @@
 !   generate_trackers      src/polygon_fracture.f90:767-803
 !   scanline_fracture      src/polygon_fracture.f90:361-610
 !   calculate_union_area_fast  src/geometry.f90:1840-1927
+!   BuildRTree + PerformMerge  src/rtree.f90:55-133, src/pnummerge.f90:279-385
+!   BuildRTree + ComputeInteractionsGPU  src/interaction.f90:204-313 (built
+!                          without an offload target: the loop runs on the host)
 module vlref_wrap
   use iso_c_binding
-  use iso_fortran_env, only: int64
+  use iso_fortran_env, only: int64, real64
   use CommonModule
   use GeometryModule
   use PolygonFractureModule
+  use RTReeBuilder
+  use DataStructuresModule
+  use PNumMergeModule
+  use RTreeBuilderGPU
   implicit none
 contains
   subroutine to_boxes(n, c, b)
@@ -78,4 +85,45 @@ contains
     call to_boxes(n, cin, b)
     area = calculate_union_area_fast(b)
   end function ref_union_area
+
+  ! PerformMerge over the n boxes in their given order (n >= 1): labels(i) =
+  ! the union-find's arr(i) after uf_fullreduce (0 for untouched boxes).
+  subroutine ref_pnum(n, cin, labels, area, perim) bind(C, name="ref_pnum")
+    integer(c_int64_t), value :: n
+    integer(c_int32_t), intent(in) :: cin(4*n)
+    integer(c_int64_t), intent(out) :: labels(n)
+    real(c_double), intent(out) :: area, perim
+    type(Box), allocatable :: b(:)
+    type(RTreeNode), allocatable :: nodes(:)
+    type(UnionFind) :: uf
+    integer(kind=int64) :: root
+    real(kind=real64) :: a, p
+    allocate(b(n))
+    call to_boxes(n, cin, b)
+    allocate(nodes(CalculateTotalNodes(int(n, int64), K_LEAF_CAPACITY)))
+    root = 0
+    call BuildRTree(b, K_LEAF_CAPACITY, nodes, root)
+    call PerformMerge(uf, b, K_LEAF_CAPACITY, nodes, root, a, p)
+    labels = uf%arr(1:n)
+    area = a
+    perim = p
+  end subroutine ref_pnum
+
+  ! ComputeInteractionsGPU's count of positive-area pairs (n >= 1).
+  function ref_interactions(n, cin) result(cnt) bind(C, name="ref_interactions")
+    integer(c_int64_t), value :: n
+    integer(c_int32_t), intent(in) :: cin(4*n)
+    integer(c_int64_t) :: cnt
+    type(Box), allocatable :: b(:)
+    type(RTreeNode), allocatable :: nodes(:)
+    integer(kind=int64) :: root, nn, k
+    allocate(b(n))
+    call to_boxes(n, cin, b)
+    nn = CalculateTotalNodes(int(n, int64), K_LEAF_CAPACITY)
+    allocate(nodes(nn))
+    root = 0
+    call BuildRTree(b, K_LEAF_CAPACITY, nodes, root)
+    call ComputeInteractionsGPU(nodes, nn, b, int(n, int64), root, k)
+    cnt = k
+  end function ref_interactions
 end module vlref_wrap

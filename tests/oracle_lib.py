@@ -57,6 +57,10 @@ def lib():
         L.or_self_and.argtypes = [C.c_void_p, C.c_int64, PB]
         L.or_grid.restype = C.c_int64
         L.or_grid.argtypes = [OBox, C.c_int32, C.c_int32, C.c_int32, PB]
+        L.or_pnum.restype = C.c_int64
+        L.or_pnum.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.or_interactions.restype = C.c_int64
+        L.or_interactions.argtypes = [C.c_void_p, C.c_int64]
         L.or_free.argtypes = [C.c_void_p]
         L.or_event_sort.argtypes = [C.c_void_p, C.c_int64]
         L.or_morton_key.restype = C.c_uint64
@@ -82,6 +86,9 @@ def ref():
         L.ref_complement.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, P64]
         L.ref_union_area.restype = C.c_int64
         L.ref_union_area.argtypes = [C.c_int64, C.c_void_p]
+        L.ref_pnum.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ref_interactions.restype = C.c_int64
+        L.ref_interactions.argtypes = [C.c_int64, C.c_void_p]
         _ref = L
     return _ref
 
@@ -155,6 +162,20 @@ def and_fragments(a, b):
 
 def self_and(a):
     return _unary("or_self_and", a)
+
+
+def pnum(a):
+    """(labels, components, overlap_area, touch_perimeter) of PerformMerge."""
+    a = boxes(a)
+    labels = np.zeros(max(len(a), 1), np.int64)
+    ar, pe = C.c_int64(), C.c_int64()
+    nc = lib().or_pnum(_ptr(a), len(a), _ptr(labels), C.byref(ar), C.byref(pe))
+    return labels[: len(a)], int(nc), ar.value, pe.value
+
+
+def interactions(a):
+    a = boxes(a)
+    return int(lib().or_interactions(_ptr(a), len(a)))
 
 
 def grid(mbr, rows, cols, overlap):
@@ -248,3 +269,18 @@ def sort_boxes(a):
         return a
     order = np.lexsort((a[:, 3], a[:, 2], a[:, 1], a[:, 0]))
     return a[order]
+
+
+def ref_pnum(a):
+    """PerformMerge of the reference (n >= 1): (labels, overlap_area, overlap_perimeter) as it
+    returns them (real64 sums)."""
+    a = boxes(a)
+    labels = np.zeros(len(a), np.int64)
+    ar, pe = C.c_double(), C.c_double()
+    ref().ref_pnum(len(a), _ptr(a), _ptr(labels), C.byref(ar), C.byref(pe))
+    return labels, ar.value, pe.value
+
+
+def ref_interactions(a):
+    a = boxes(a)
+    return int(ref().ref_interactions(len(a), _ptr(a)))

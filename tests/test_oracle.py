@@ -240,3 +240,96 @@ def test_grid_invariants(rows, cols, overlap):
     assert O.klee_area(g) == (m[2] - m[0]) * (m[3] - m[1])       # the cells cover the domain
     if overlap == 0:                                              # and tile it exactly
         assert int(((g[:, 2] - g[:, 0]).astype(np.int64) * (g[:, 3] - g[:, 1])).sum()) == O.klee_area(g)
+
+
+def pnum_brute(a):
+    """PerformMerge's result by another route: every pair tested (closed
+    extents), components by breadth-first search, numbered in the order of
+    their smallest box index; boxes that touch nothing keep 0."""
+    a = np.asarray(a, np.int64).reshape(-1, 4)
+    n = len(a)
+    adj = [[] for _ in range(n)]
+    area = perim = 0
+    for i in range(n):
+        for k in range(i + 1, n):
+            ix1, iy1 = max(a[i, 0], a[k, 0]), max(a[i, 1], a[k, 1])
+            ix2, iy2 = min(a[i, 2], a[k, 2]), min(a[i, 3], a[k, 3])
+            if ix1 <= ix2 and iy1 <= iy2:
+                adj[i].append(k)
+                adj[k].append(i)
+                if ix1 < ix2 and iy1 < iy2:
+                    area += int((ix2 - ix1) * (iy2 - iy1))
+                else:
+                    perim += int(2 * ((ix2 - ix1) + (iy2 - iy1)))
+    labels = np.zeros(n, np.int64)
+    nc = 0
+    for s0 in range(n):
+        if labels[s0] or not adj[s0]:
+            continue
+        nc += 1
+        labels[s0] = nc
+        todo = [s0]
+        while todo:
+            u = todo.pop()
+            for v in adj[u]:
+                if not labels[v]:
+                    labels[v] = nc
+                    todo.append(v)
+    return labels, nc, area, 0 if area else perim
+
+
+def small_layout(rng, n, span, wmax):
+    x = rng.integers(0, span, n)
+    y = rng.integers(0, span, n)
+    return np.stack([x, y, x + rng.integers(1, wmax + 1, n), y + rng.integers(1, wmax + 1, n)], 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_pnum_against_brute_force(seed):
+    rng = np.random.default_rng(500 + seed)
+    n = int(rng.integers(1, 300))
+    a = small_layout(rng, n, 200 if seed % 3 else 60, 12)
+    if seed % 4 == 1:            # snap to a coarse grid: many shared edges and corners
+        a = (a // 4 * 4).astype(np.int32)
+        a[:, 2:] = np.maximum(a[:, 2:], a[:, :2] + 4)
+    labels, nc, area, perim = O.pnum(a)
+    bl, bnc, barea, bperim = pnum_brute(a)
+    assert np.array_equal(labels, bl) and nc == bnc and area == barea and perim == bperim
+    pos = sum(1 for i in range(n) for k in range(i + 1, n)
+              if max(a[i, 0], a[k, 0]) < min(a[i, 2], a[k, 2]) and max(a[i, 1], a[k, 1]) < min(a[i, 3], a[k, 3]))
+    assert O.interactions(a) == pos
+
+
+def test_pnum_known_answers():
+    # an edge, a corner, a gap of one unit
+    labels, nc, area, perim = O.pnum([[0, 0, 10, 10], [10, 0, 20, 10], [20, 10, 30, 20], [31, 0, 40, 5]])
+    assert labels.tolist() == [1, 1, 1, 0] and nc == 1 and area == 0 and perim == 2 * 10 + 0
+    # overlap: the area counts and the touch perimeter is reported as 0
+    labels, nc, area, perim = O.pnum([[0, 0, 10, 10], [5, 5, 15, 15], [15, 0, 20, 5], [100, 100, 110, 110]])
+    assert labels.tolist() == [1, 1, 1, 0] and nc == 1 and area == 25 and perim == 0
+    # components numbered by their smallest index, not by position
+    labels, nc, _, _ = O.pnum([[100, 0, 110, 10], [0, 0, 10, 10], [105, 5, 120, 20], [5, 5, 12, 12]])
+    assert labels.tolist() == [1, 2, 1, 2] and nc == 2
+    assert O.pnum([[0, 0, 1, 1]])[0].tolist() == [0]
+    assert O.interactions([[0, 0, 10, 10], [10, 0, 20, 10], [5, 5, 15, 15]]) == 2
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_pnum_against_reference_build(seed):
+    """Labels, area and the touch perimeter equal the reference's own
+    BuildRTree + PerformMerge, and the interaction count its
+    ComputeInteractionsGPU (flang build, oracle/_ref)."""
+    if O.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(700 + seed)
+    n = int(rng.integers(1, 3000))
+    a = small_layout(rng, n, 4000 if seed % 2 else 1500, 40)
+    if seed == 5:
+        a = (a // 8 * 8).astype(np.int32)
+        a[:, 2:] = np.maximum(a[:, 2:], a[:, :2] + 8)
+    labels, nc, area, perim = O.pnum(a)
+    rl, rarea, rperim = O.ref_pnum(a)
+    assert np.array_equal(labels, rl)
+    assert nc == int(rl.max(initial=0))
+    assert area == rarea and perim == rperim
+    assert O.interactions(a) == O.ref_interactions(a)

@@ -32,6 +32,7 @@ __all__ = [
     "CalculateAND", "CalculateFrameNOT", "CalculateNOT", "CalculateOR", "CalculateXOR",
     "CalculateGROWLayer", "CalculateBoostOperation", "MergeHealLayer", "heal_boxes",
     "CalculateFrameAND", "CalculateSingleLayerAND", "CreateGrid", "CreateEXTENT",
+    "PerformMerge", "InteractionCount", "CertifyHealing",
     "ASSERT_ZERO", "klee_area", "canonicalize", "xor_empty",
     "device_event_sort", "thrust_sort_boxes_direct", "thrust_sort_boxes_indirect",
     "synth_layer", "synth_tile_count",
@@ -112,6 +113,8 @@ def lib():
             "vl_self_and": (C.c_int, [vp, vp, vp]),
             "vl_grid": (C.c_int, [vp, vp, i32, i32, i32, vp]),
             "vl_extent": (C.c_int, [vp, vp, vp]),
+            "vl_pnum": (C.c_int, [vp, vp, vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
+            "vl_interaction_count": (C.c_int, [vp, vp, C.POINTER(i64)]),
             "vl_assert_zero": (C.c_int, [vp, vp, C.c_int, C.POINTER(i64)]),
             "vl_klee_area": (C.c_int, [vp, vp, C.POINTER(i64)]),
             "vl_canonicalize": (C.c_int, [vp, vp, vp]),
@@ -366,6 +369,30 @@ def CreateEXTENT(A, C_out):
     """C = MBR(A) as one box  (src/design_impl.f90:277-293)."""
     _check(lib().vl_extent(A.engine.handle, A.handle, C_out.handle))
     return C_out
+
+
+def PerformMerge(L):
+    """Polygon numbering of L's boxes in their stored order (src/pnummerge.f90:279-385):
+    returns (labels int64[n], components, overlap_area, touch_perimeter); labels
+    are 0 for boxes that meet no other box, else 1..components."""
+    n = L.n_used
+    labels = np.zeros(max(n, 1), np.int64)
+    nc, ar, pe = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(lib().vl_pnum(L.engine.handle, L.handle, labels.ctypes.data_as(C.c_void_p), C.byref(nc), C.byref(ar),
+                         C.byref(pe)))
+    return labels[:n], nc.value, ar.value, pe.value
+
+
+def InteractionCount(L):
+    """Pairs of L's boxes overlapping with positive area (src/interaction.f90:204-310)."""
+    v = C.c_int64()
+    _check(lib().vl_interaction_count(L.engine.handle, L.handle, C.byref(v)))
+    return v.value
+
+
+def CertifyHealing(L):
+    """True when no two boxes of L overlap with positive area (src/design_impl.f90:35-56)."""
+    return L.n_used < 2 or InteractionCount(L) == 0
 
 
 def MergeHealLayer(L):

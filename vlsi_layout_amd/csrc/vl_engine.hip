@@ -1788,6 +1788,95 @@ int vl_merge(vl_engine* e, const vl_layer* a, vl_layer* c) { return binary(e, TO
 
 int vl_self_and(vl_engine* e, const vl_layer* a, vl_layer* c) { return binary(e, TOP_OVERLAP, a, nullptr, c); }
 
+// Pairs of A's boxes whose closed extents meet, found per tile (vl_pnum.hip):
+// the box indices are binned by their closed extent (the tile range grown by
+// one unit east and north) over MBR(A) grown the same way.
+static int layer_pairs(vl_engine* e, const vl_layer* a, int mode, uint32_t* parent, uint32_t* touched,
+                       unsigned long long sums_out[3]) {
+  Box64 m;
+  VLCK(layer_mbr(e, a, &m));
+  const Box64 dom{m.x1, m.y1, std::min<int64_t>(m.x2 + 1, INT32_MAX), std::min<int64_t>(m.y2 + 1, INT32_MAX)};
+  const Grid g = make_grid(dom, a->n, e->target);
+  const int64_t nt = (int64_t)g.cols * g.rows;
+  const Xform closed{1, 1, 0, 0};
+  Scratch s(e);
+  uint64_t* start = s.get<uint64_t>(2 * (size_t)nt + 1);
+  uint32_t* cnt = s.get<uint32_t>(2 * (size_t)nt);
+  void* stmp = s.get<char>(scan_tmp_bytes(2 * (size_t)nt));
+  TileDesc* desc = s.get<TileDesc>((size_t)nt);
+  unsigned long long* sums = s.get<unsigned long long>(3);
+  if (s.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+  launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
+  launch_bin_count(a->d, a->n, closed, nullptr, 0, g, cnt, e->st);
+  scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
+  uint64_t total = 0;
+  VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
+  uint32_t* idx = s.get<uint32_t>(total + 1);
+  if (!idx) return fail(VL_ENOMEM, "pair binning (%llu entries)", (unsigned long long)total);
+  launch_bin_scatter_idx(a->d, a->n, closed, g, start, cnt, idx, e->st);
+  launch_build_desc(g, start, TOP_UNION, desc, e->st);
+  launch_fill_u64((uint64_t*)sums, 0ull, 3, e->st);
+  launch_pairs(desc, (int)nt, idx, a->d, g, mode, parent, touched, sums, e->st);
+  HIPCK(hipGetLastError());
+  return readback(e, sums_out, sums, 3 * sizeof(unsigned long long));
+}
+
+int vl_pnum(vl_engine* e, const vl_layer* a, int64_t* labels, int64_t* ncomp, int64_t* overlap_area,
+            int64_t* touch_perimeter) {
+  if (!e || !a || !ncomp || !overlap_area || !touch_perimeter || (a->n > 0 && !labels))
+    return fail(VL_EINVAL, "null argument");
+  if (a->e != e->root) return fail(VL_EINVAL, "layer belongs to another engine");
+  if (a->n > (int64_t)UINT32_MAX - 1) return fail(VL_EINVAL, "more than 2^32 - 1 boxes");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  *ncomp = *overlap_area = *touch_perimeter = 0;
+  const int64_t n = a->n;
+  if (n == 0) return VL_OK;
+  Scratch s(e);
+  uint32_t* parent = s.get<uint32_t>(n);
+  uint32_t* touched = s.get<uint32_t>(n);
+  uint32_t* root = s.get<uint32_t>(n);
+  uint32_t* is_root = s.get<uint32_t>(n + 1);
+  uint32_t* rank = s.get<uint32_t>(n + 1);
+  void* stmp = s.get<char>(scan_tmp_bytes((size_t)n));
+  const bool dev = is_device_ptr(labels);
+  int64_t* d_labels = dev ? labels : s.get<int64_t>(n);
+  if (s.oom) return fail(VL_ENOMEM, "polygon numbering scratch");
+  launch_uf_init(parent, touched, n, e->st);
+  unsigned long long sums[3] = {0, 0, 0};
+  VLCK(layer_pairs(e, a, 0, parent, touched, sums));
+  launch_uf_roots(parent, touched, root, is_root, n, e->st);
+  scan_exclusive_u32(is_root, rank, (size_t)n, rank + n, stmp, e->st);
+  launch_uf_labels(root, touched, rank, d_labels, n, e->st);
+  HIPCK(hipGetLastError());
+  uint32_t nroots = 0;
+  VLCK(readback(e, &nroots, rank + n, sizeof(nroots)));
+  if (!dev) {
+    HIPCK(hipMemcpyAsync(labels, d_labels, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+  }
+  *ncomp = nroots;
+  *overlap_area = (int64_t)sums[0];
+  // the reference reports touch perimeters only when nothing overlaps
+  *touch_perimeter = sums[0] ? 0 : (int64_t)sums[1];
+  return VL_OK;
+}
+
+int vl_interaction_count(vl_engine* e, const vl_layer* a, int64_t* count) {
+  if (!e || !a || !count) return fail(VL_EINVAL, "null argument");
+  if (a->e != e->root) return fail(VL_EINVAL, "layer belongs to another engine");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  OpTimer tm(e);
+  *count = 0;
+  if (a->n < 2) return VL_OK;
+  unsigned long long sums[3] = {0, 0, 0};
+  VLCK(layer_pairs(e, a, 1, nullptr, nullptr, sums));
+  *count = (int64_t)sums[2];
+  return VL_OK;
+}
+
 int vl_grid(vl_engine* e, const vl_layer* a, int32_t rows, int32_t cols, int32_t overlap, vl_layer* c) {
   VLCK(check_layers(e, a, nullptr, c));
   if (rows < 1 || cols < 1) return fail(VL_EINVAL, "GRID: rows = %d, cols = %d must be positive", rows, cols);

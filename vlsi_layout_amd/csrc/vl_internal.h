@@ -166,6 +166,9 @@ void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const
 // the tile is too large to test); the others get tab_n[t] = tab_n[ntab+t] = 0.
 void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, uint32_t* list, uint32_t* list_n,
                        uint32_t* tab_n, int ntab, hipStream_t st);
+// The same scatter writing box indices (idx) instead of boxes (one layer).
+void launch_bin_scatter_idx(const vl_box* a, int64_t na, Xform xra, Grid g, const uint64_t* start, uint32_t* cnt,
+                            uint32_t* idx, hipStream_t st);
 // Tile descriptors from the key bounds (start has 2*ntiles+1 entries).
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st);
 // Tile kernel: result strips of every tile.  Strips touching an internal seam
@@ -295,6 +298,18 @@ void launch_dense_tile(const TileDesc* desc, const uint32_t* list, int nlist, co
                        const uint64_t* goff, uint32_t* colcnt, const uint64_t* cofs, vl_box* strips, uint8_t* tag,
                        const uint64_t* soff, uint64_t* tab_off, uint32_t* tab_n, bool two,
                        unsigned long long* cnt, long long* area, hipStream_t st);
+
+// vl_pnum.hip — pairs of touching boxes per tile (entries = box indices);
+// mode 0: union-find of the pairs (parent, touched) and the sums
+// [0] positive intersection areas, [1] zero-area intersection perimeters;
+// mode 1: sums[2] = pairs overlapping with positive area.
+void launch_uf_init(uint32_t* parent, uint32_t* touched, int64_t n, hipStream_t st);
+void launch_pairs(const TileDesc* desc, int ntiles, const uint32_t* idx, const vl_box* boxes, Grid g, int mode,
+                  uint32_t* parent, uint32_t* touched, unsigned long long* sums, hipStream_t st);
+void launch_uf_roots(uint32_t* parent, const uint32_t* touched, uint32_t* root, uint32_t* is_root, int64_t n,
+                     hipStream_t st);
+void launch_uf_labels(const uint32_t* root, const uint32_t* touched, const uint32_t* rank, int64_t* labels, int64_t n,
+                      hipStream_t st);
 
 // vl_window.hip — window extraction from a cell-indexed layer.  Candidates
 // are the concatenation of the entry ranges [rbeg[r], rbeg[r] + roff[r+1] -

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const vl_box* __restrict__ a,
 __global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ a, int64_t na, Xform xra, Xform xsa,
                                                      const vl_box* __restrict__ b, int64_t nb, Grid g,
                                                      const uint64_t* __restrict__ start, uint32_t* __restrict__ cnt,
-                                                     vl_box* __restrict__ entries) {
+                                                     vl_box* __restrict__ entries, uint32_t* __restrict__ idx) {
   const int64_t n = na + nb;
   const int lane = __lane_id();
   const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
@@ -219,7 +219,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ 
       uint32_t old = 0;
       if (sg.is_head) old = atomicSub(&cnt[key], (uint32_t)sg.len);
       old = (uint32_t)__shfl((int)old, sg.head, WAVE);
-      if (act) entries[start[key] + old - 1u - (uint32_t)sg.rank] = stored;
+      if (act) {
+        const uint64_t pos = start[key] + old - 1u - (uint32_t)sg.rank;
+        if (idx) idx[pos] = (uint32_t)i;   // index binning: the box's position
+        else entries[pos] = stored;
+      }
     }
   }
 }
@@ -1220,7 +1224,13 @@ void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const
                         const uint64_t* start, uint32_t* cnt, vl_box* entries, hipStream_t st) {
   if (na + nb > 0)
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, xsa, b, nb, g, start,
-                       cnt, entries);
+                       cnt, entries, (uint32_t*)nullptr);
+}
+void launch_bin_scatter_idx(const vl_box* a, int64_t na, Xform xra, Grid g, const uint64_t* start, uint32_t* cnt,
+                            uint32_t* idx, hipStream_t st) {
+  if (na > 0)
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(na)), dim3(256), 0, st, a, na, xra, Xform{0, 0, 0, 0},
+                       (const vl_box*)nullptr, (int64_t)0, g, start, cnt, (vl_box*)nullptr, idx);
 }
 void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, uint32_t* list, uint32_t* list_n,
                        uint32_t* tab_n, int ntab, hipStream_t st) {
