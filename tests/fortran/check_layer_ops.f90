@@ -4,14 +4,17 @@
 ! prints the known-answer results of fracture_rectangle and
 ! merge_boxes_using_scanline, and writes the results of CalculateFrameAND,
 ! CalculateSingleLayerAND, CreateGrid, CreateEXTENT and CalculateGROWLayer
-! (called through an intent(in) argument) as KLBIN files in OUTDIR.
+! (called through an intent(in) argument) as KLBIN files in OUTDIR; then
+! PreprocessLayer, PerformMergeLayer, CertifyHealing and a Design: the
+! state bits and counts are printed, the boxes and polygon numbers written.
 program check_layer_ops
   use iso_fortran_env, only: int64, int32
   use vl_layer_ops
   use vl_rect_boolean, only: RBox => Box, Vector_Box, fracture_rectangle
   implicit none
   character(len=512) :: fa, fb, dir
-  type(Layer) :: A, B, C
+  type(Layer) :: A, B, C, P, Q, E
+  type(Design) :: d
   type(Vector_Box) :: overlaps, frags
   type(Box), allocatable :: m(:)
   integer :: k
@@ -56,6 +59,34 @@ program check_layer_ops
   call save(C, 'extent.bin')
   call grow_of(A, C)
   call save(C, 'grow.bin')
+
+  ! PreprocessLayer on the raw (overlapping) layer: heal, Morton sort, PNUM
+  print '(A,L1,1X,I0)', 'CERTIFY_RAW ', CertifyHealing(A), LayerInteractionCount(A)
+  call load(P, fa)
+  call PreprocessLayer(P)
+  print '(A,I0)', 'STATE ', P%layerState
+  print '(A,L1,1X,L1,1X,L1,1X,L1)', 'NEEDS ', NeedsHealing(P), NeedsSorting(P), NeedsRTree(P), NeedsPNum(P)
+  print '(A,L1,1X,I0)', 'CERTIFY_PRE ', CertifyHealing(P), LayerInteractionCount(P)
+  print '(A,I0,1X,F0.1,1X,F0.1)', 'PNUM ', maxval(P%pnumtable%arr), P%area, P%perimeter
+  call save(P, 'pre.bin')
+  call save_labels(P, 'pre_labels.bin')
+  ! PerformMergeLayer numbers the boxes of B after healing it
+  call load(Q, fb)
+  call PerformMergeLayer(Q)
+  call save(Q, 'pml.bin')
+  call save_labels(Q, 'pml_labels.bin')
+  print '(A,I0)', 'PML_STATE ', Q%layerState
+  ! an empty layer gets the state 31
+  call layer_set_boxes(E, m, 0_int64)
+  call PreprocessLayer(E)
+  print '(A,I0)', 'EMPTY_STATE ', E%layerState
+  ! a design holding two named layers
+  allocate(d%layers(2), d%layerNames(2))
+  d%layerNames(1) = 'metal1'
+  d%layerNames(2) = 'via1'
+  d%designName = 'check'
+  print '(A,I0,1X,I0,1X,I0)', 'DESIGN ', design_layer_index(d, 'via1'), design_layer_index(d, 'metal1'), &
+       design_layer_index(d, 'poly')
   print '(A)', 'CHECK DONE'
 
 contains
@@ -92,5 +123,15 @@ contains
     if (L%n_used > 0) write(u) L%layer_boxes(1:L%n_used)
     close(u)
   end subroutine save
+
+  subroutine save_labels(L, name)
+    type(Layer), intent(in) :: L
+    character(len=*), intent(in) :: name
+    integer :: u
+    open(newunit=u, file=trim(dir)//'/'//name, access='stream', form='unformatted', status='replace', &
+         action='write')
+    if (L%n_used > 0) write(u) L%pnumtable%arr(1:L%n_used)
+    close(u)
+  end subroutine save_labels
 
 end program check_layer_ops

@@ -36,7 +36,8 @@ def test_module_exports_reference_names():
     for name in ["calculateand", "calculateframenot", "calculatenot", "calculateor", "calculategrowlayer",
                  "calculateboostoperation", "mergeheallayer", "heal_boxes", "sort_event_trackers",
                  "sortboxesdirect", "sortboxesindirect", "calculateframeand", "calculatesinglelayerand",
-                 "creategrid", "createextent", "merge_boxes_using_scanline"]:
+                 "creategrid", "createextent", "merge_boxes_using_scanline", "preprocesslayer",
+                 "performmergelayer", "certifyhealing", "needssorting", "needspnum", "needsrtree", "needshealing"]:
         assert f"_qmvl_layer_opsp{name}" in out, name
     assert "_qmvl_rect_booleanpfracture_rectangle" in out
 
@@ -151,8 +152,9 @@ CHECK = os.path.join(ROOT, "tests", "fortran", "check_layer_ops")
 @pytest.mark.skipif(not os.path.exists(CHECK), reason="Fortran check program not built")
 def test_module_procedures_called_directly(tmp_path, gpu_engine):
     """fracture_rectangle, merge_boxes_using_scanline, CalculateFrameAND,
-    CalculateSingleLayerAND, CreateGrid, CreateEXTENT and CalculateGROWLayer
-    (its input held intent(in)) called from Fortran."""
+    CalculateSingleLayerAND, CreateGrid, CreateEXTENT, CalculateGROWLayer
+    (its input held intent(in)), PreprocessLayer, PerformMergeLayer,
+    CertifyHealing and a Design called from Fortran."""
     fa = os.path.join(GOLDEN_IN, "SED_L67_D20.bin")
     fb = os.path.join(GOLDEN_IN, "SED_L66_D20.bin")
     r = subprocess.run([CHECK, fa, fb, str(tmp_path)], capture_output=True, text=True, timeout=120)
@@ -173,6 +175,25 @@ def test_module_procedures_called_directly(tmp_path, gpu_engine):
     assert np.array_equal(rd("grid.bin"), O.grid(m, 3, 4, 2))   # bit-exact, in order
     assert rd("extent.bin").tolist() == [list(m)]
     assert np.array_equal(O.sort_boxes(rd("grow.bin")), O.grow(a, 1, 2, 3, 4))
+    # PreprocessLayer (src/design.f90:937-1083): healed, Morton-ordered, numbered
+    val = {ln.split()[0]: ln.split()[1:] for ln in lines if ln.split()}
+    inter = O.interactions(a)
+    assert val["CERTIFY_RAW"] == [("T" if inter == 0 else "F"), str(inter)]
+    assert val["STATE"] == [str(1 | 2 | 4 | 8)] and val["NEEDS"] == ["F", "F", "F", "F"]
+    assert val["CERTIFY_PRE"] == ["T", "0"]
+    pre = rd("pre.bin")
+    assert np.array_equal(O.sort_boxes(pre), O.canonical(a))
+    assert np.array_equal(O.morton_sort_direct(pre), pre)     # already in Morton order (a stable sort keeps it)
+    labels, nc, area, perim = O.pnum(pre)
+    assert np.array_equal(np.fromfile(tmp_path / "pre_labels.bin", dtype="<i8"), labels)
+    assert int(val["PNUM"][0]) == nc and float(val["PNUM"][1]) == 0.0 and area == 0
+    assert float(val["PNUM"][2]) == float(perim)
+    pml = rd("pml.bin")
+    assert np.array_equal(O.sort_boxes(pml), O.canonical(b))
+    assert np.array_equal(np.fromfile(tmp_path / "pml_labels.bin", dtype="<i8"), O.pnum(pml)[0])
+    assert val["PML_STATE"] == [str(1 | 4)]
+    assert val["EMPTY_STATE"] == ["31"]
+    assert val["DESIGN"] == ["2", "1", "0"]
 
 
 OPS_SCRIPT = """

@@ -8,6 +8,10 @@
 !   CalculateSingleLayerAND, CreateGrid, CreateEXTENT
 !                                   src/design_impl.f90:385-509, :159-293
 !   merge_boxes_using_scanline      src/boxmerge.f90:273-304
+!   PreprocessLayer, PerformMergeLayer, CertifyHealing, NeedsSorting/NeedsPNum/
+!   NeedsRTree/NeedsHealing, LAYER_STATE_*, Layer, Design
+!                                   src/design.f90:33-103, :316-337, :937-1083,
+!                                   src/design_impl.f90:22-56
 !   heal_boxes                      src/polygon_fracture.f90:805-808
 !   sort_event_trackers             src/event_sorter_apif.f90:30-43
 !   SortBoxesDirect/SortBoxesIndirect  src/gpu_morton_sort.f90:41-77
@@ -30,7 +34,7 @@
 ! may differ from the reference's).
 module vl_layer_ops
   use iso_c_binding
-  use iso_fortran_env, only: int8, int32, int64, error_unit
+  use iso_fortran_env, only: int8, int32, int64, real64, error_unit
   implicit none
   private
 
@@ -46,12 +50,22 @@ module vl_layer_ops
   public :: vl_fortran_init, vl_fortran_finalize, vl_fortran_sync, vl_engine_handle
   public :: K_BOOST_CONTROL_XOR, K_BOOST_CONTROL_OR, K_BOOST_CONTROL_AND, K_BOOST_CONTROL_NOT
   public :: K_BOOST_CONTROL_MERGE, K_BOOST_CONTROL_SIZE
-  public :: LAYER_STATE_NONE, LAYER_STATE_HEAL
+  public :: LAYER_STATE_NONE, LAYER_STATE_HEAL, LAYER_STATE_SORT, LAYER_STATE_PNUM, LAYER_STATE_RTREE
+  public :: LAYER_STATE_SLSORT, LAYER_STATE_FINAL, LAYER_STATE_MAX, LAYER_STATE_EVERYTHING
+  public :: UnionFind, Design
+  public :: NeedsSorting, NeedsPNum, NeedsRTree, NeedsHealing
+  public :: PreprocessLayer, PerformMergeLayer, CertifyHealing, LayerInteractionCount
 
   integer(int64), parameter :: K_BOOST_CONTROL_XOR = 0, K_BOOST_CONTROL_OR = 1, &
        K_BOOST_CONTROL_AND = 2, K_BOOST_CONTROL_NOT = 3, K_BOOST_CONTROL_MERGE = 4, &
        K_BOOST_CONTROL_SIZE = 5
-  integer(c_int), parameter :: LAYER_STATE_NONE = 0, LAYER_STATE_HEAL = 1
+  ! Layer state bits (src/design.f90:37-48).  Here SORT means the boxes are in
+  ! Morton order of their centres, and RTREE is set together with SORT: the
+  ! engine's spatial index is the tile binning each operator builds on the
+  ! device, so no tree is kept on the layer.
+  integer(c_int), parameter :: LAYER_STATE_NONE = 0, LAYER_STATE_HEAL = 1, LAYER_STATE_SORT = 2, &
+       LAYER_STATE_PNUM = 4, LAYER_STATE_RTREE = 8, LAYER_STATE_SLSORT = 16, LAYER_STATE_FINAL = 32, &
+       LAYER_STATE_MAX = 64, LAYER_STATE_EVERYTHING = LAYER_STATE_MAX - 1
 
   type, bind(C) :: Box
      integer(c_int32_t) :: x1 = 0, y1 = 0, x2 = 0, y2 = 0
@@ -63,15 +77,35 @@ module vl_layer_ops
      integer(c_int8_t) :: polygonNumber = 0
   end type XYTracker
 
+  ! Polygon numbers of a layer after PNUM (src/datastructures.f90:26-38):
+  ! arr(i) = 0 when box i touches no other box, else its component's number.
+  type :: UnionFind
+     integer(int64), allocatable :: arr(:)
+  end type UnionFind
+
   type :: Layer
      integer(int64) :: n_used = 0
      integer(int64) :: n_alloc = 0
      type(Box), allocatable :: layer_boxes(:)
      integer(c_int) :: layerState = LAYER_STATE_NONE
+     type(UnionFind) :: pnumtable          ! valid with LAYER_STATE_PNUM
+     real(real64) :: area = 0, perimeter = 0   ! PNUM's overlap area and touch perimeter
+     character(len=:), allocatable :: fileName
      type(c_ptr) :: dev = c_null_ptr      ! the engine's layer handle
      logical :: dev_valid = .false.        ! the device copy holds the layer
      logical :: host_valid = .true.        ! layer_boxes(1:n_used) holds the layer
   end type Layer
+
+  ! A named set of layers (src/design.f90:95-103); layers are looked up by
+  ! name with design_layer_index.
+  type :: Design
+     type(Layer), allocatable :: layers(:)
+     character(len=1024), allocatable :: layerNames(:)
+     type(Box) :: DESIGN_EXTENT
+     character(len=:), allocatable :: designName
+     character(len=:), allocatable :: fileName
+  end type Design
+  public :: design_layer_index
 
   logical, public :: vl_eager_host = .true.
   integer(c_int), public :: vl_device = 0
@@ -163,6 +197,30 @@ module vl_layer_ops
        type(c_ptr), value :: e, a, c
        integer(c_int) :: rc
      end function c_extent
+     function c_pnum(e, a, labels, ncomp, area, perim) bind(C, name="vl_pnum") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: e, a, labels
+       integer(c_int64_t) :: ncomp, area, perim
+       integer(c_int) :: rc
+     end function c_pnum
+     function c_interaction_count(e, a, cnt) bind(C, name="vl_interaction_count") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: e, a
+       integer(c_int64_t) :: cnt
+       integer(c_int) :: rc
+     end function c_interaction_count
+     function c_layer_device_ptr(l, dev) bind(C, name="vl_layer_device_ptr") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: l
+       type(c_ptr) :: dev
+       integer(c_int) :: rc
+     end function c_layer_device_ptr
+     function c_morton_sort_direct(e, boxes, n) bind(C, name="vl_morton_sort_direct") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: e, boxes
+       integer(c_int64_t), value :: n
+       integer(c_int) :: rc
+     end function c_morton_sort_direct
      function c_canonicalize(e, a, c) bind(C, name="vl_canonicalize") result(rc)
        import :: c_int, c_ptr
        type(c_ptr), value :: e, a, c
@@ -581,6 +639,131 @@ contains
     call check(c_merge(engine, L%dev, L%dev), "MergeHealLayer")
     call finish_output(L)
   end subroutine MergeHealLayer
+
+  ! ---- layer state (src/design.f90:316-337) ---------------------------------
+
+  pure logical function NeedsSorting(L)
+    type(Layer), intent(in) :: L
+    NeedsSorting = iand(L%layerState, LAYER_STATE_SORT) == 0
+  end function NeedsSorting
+
+  pure logical function NeedsPNum(L)
+    type(Layer), intent(in) :: L
+    NeedsPNum = iand(L%layerState, LAYER_STATE_PNUM) == 0
+  end function NeedsPNum
+
+  pure logical function NeedsRTree(L)
+    type(Layer), intent(in) :: L
+    NeedsRTree = iand(L%layerState, LAYER_STATE_RTREE) == 0
+  end function NeedsRTree
+
+  pure logical function NeedsHealing(L)
+    type(Layer), intent(in) :: L
+    NeedsHealing = iand(L%layerState, LAYER_STATE_HEAL) == 0
+  end function NeedsHealing
+
+  ! Morton order of the box centres, on the device copy (SortBoxesDirect's
+  ! order, src/gpu_morton_sort.f90:41-57).
+  subroutine sort_layer(L)
+    type(Layer), intent(inout) :: L
+    type(c_ptr) :: d
+    call to_device(L)
+    if (L%n_used < 2) return
+    call check(c_layer_device_ptr(L%dev, d), "vl_layer_device_ptr")
+    call check(c_morton_sort_direct(engine, d, L%n_used), "PreprocessLayer (sort)")
+    L%host_valid = .false.
+    if (vl_eager_host) call layer_to_host(L)
+  end subroutine sort_layer
+
+  ! Polygon numbers of the boxes in their current order (PerformMerge,
+  ! src/pnummerge.f90:279-385) into pnumtable, area and perimeter.
+  subroutine pnum_layer(L)
+    type(Layer), intent(inout) :: L
+    integer(int64), allocatable, target :: labels(:)
+    integer(c_int64_t) :: nc, ar, pe
+    call to_device(L)
+    allocate(labels(max(L%n_used, 1_int64)))
+    labels = 0
+    call check(c_pnum(engine, L%dev, c_loc(labels(1)), nc, ar, pe), "PerformMerge")
+    if (allocated(L%pnumtable%arr)) deallocate(L%pnumtable%arr)
+    allocate(L%pnumtable%arr(L%n_used))
+    L%pnumtable%arr = labels(1:L%n_used)
+    L%area = real(ar, real64)
+    L%perimeter = real(pe, real64)
+  end subroutine pnum_layer
+
+  ! PreprocessLayer (src/design.f90:937-1083, the PreprocessLayerSL form):
+  ! heals a layer whose HEAL bit is clear, sorts it (SORT and RTREE), and
+  ! numbers its polygons (PNUM).  An empty layer gets every state bit below
+  ! SLSORT's (31), as in the reference.
+  subroutine PreprocessLayer(L)
+    type(Layer), intent(inout) :: L
+    if (L%n_used == 0) then
+       L%layerState = 31
+       return
+    end if
+    if (NeedsHealing(L)) call MergeHealLayer(L)
+    if (NeedsSorting(L) .or. NeedsRTree(L)) then
+       call sort_layer(L)
+       L%layerState = ior(L%layerState, ior(LAYER_STATE_SORT, LAYER_STATE_RTREE))
+    end if
+    if (NeedsPNum(L)) then
+       call pnum_layer(L)
+       L%layerState = ior(L%layerState, LAYER_STATE_PNUM)
+    end if
+  end subroutine PreprocessLayer
+
+  ! PerformMergeLayer (src/design_impl.f90:22-33): heal when needed, then
+  ! number the polygons.
+  subroutine PerformMergeLayer(L)
+    type(Layer), intent(inout) :: L
+    if (NeedsHealing(L)) call MergeHealLayer(L)
+    call pnum_layer(L)
+    L%layerState = ior(L%layerState, LAYER_STATE_PNUM)
+  end subroutine PerformMergeLayer
+
+  ! Pairs of boxes overlapping with positive area (ComputeInteractionsGPU,
+  ! src/interaction.f90:204-310).
+  function LayerInteractionCount(L) result(n)
+    type(Layer), intent(in) :: L
+    integer(int64) :: n
+    type(c_ptr) :: h
+    logical :: temp
+    integer(c_int64_t) :: cnt
+    n = 0
+    if (L%n_used < 2) return
+    call input_handle(L, h, temp)
+    call check(c_interaction_count(engine, h, cnt), "ComputeInteractionsGPU")
+    call release_input(h, temp)
+    n = cnt
+  end function LayerInteractionCount
+
+  ! CertifyHealing (src/design_impl.f90:35-56): true when fewer than two
+  ! boxes, when the HEAL bit is set, or when no two boxes overlap with
+  ! positive area.  The reference answers .false. for a layer without an
+  ! R-tree; the engine needs none, so it counts the pairs instead.
+  logical function CertifyHealing(L)
+    type(Layer), intent(in) :: L
+    CertifyHealing = .true.
+    if (L%n_used < 2) return
+    if (iand(L%layerState, LAYER_STATE_HEAL) /= 0) return
+    CertifyHealing = LayerInteractionCount(L) == 0
+  end function CertifyHealing
+
+  ! 1-based index of the layer called name in d, 0 when there is none.
+  integer function design_layer_index(d, name)
+    type(Design), intent(in) :: d
+    character(len=*), intent(in) :: name
+    integer :: i
+    design_layer_index = 0
+    if (.not. allocated(d%layerNames)) return
+    do i = 1, size(d%layerNames)
+       if (trim(d%layerNames(i)) == trim(name)) then
+          design_layer_index = i
+          return
+       end if
+    end do
+  end function design_layer_index
 
   ! boxes(1:n_in) -> their union as maximal strips in boxes(1:n_out), sorted by
   ! (x1, y1, x2, y2); boxes is reallocated when the result needs more room
