@@ -78,8 +78,10 @@ int vl_engine_destroy(vl_engine* e);
  *                  one source layer in one fused tile pass; 0 runs every
  *                  operator on its own
  *   "fuse_target"  mean boxes per tile of a fused group's grid (6)
- *   "bin_mode"     1 (default): tiles binned by per-tile counters and an
- *                  atomic scatter; 0: by a stable radix sort of tile keys
+ *   "bin_mode"     2 (default): tiles binned by per-tile counters and an
+ *                  atomic scatter, one atomic per distinct tile of a wave;
+ *                  1: the same with one atomic per run of equal tiles;
+ *                  0: by a stable radix sort of tile keys
  *   "profile"      1: collect per-kernel statistics (vl_engine_kernel_stats)
  * Read-only: "fused_groups" = groups run fused by programs on this engine. */
 int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);

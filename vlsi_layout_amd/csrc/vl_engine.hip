@@ -146,7 +146,7 @@ struct vl_engine {
   int max_wg_cap = 0;            // 0 = occupancy limit
   int fuse = 1;                  // programs: fuse operator groups over one source layer
   int fuse_target = 6;           // mean boxes per tile of a fused group's grid
-  int bin_mode = 1;              // 1: atomic binning; 0: stable radix sort of (tile, layer) keys
+  int bin_mode = 2;              // 2: atomic binning, atomics grouped per wave (1: per run); 0: radix sort
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   // Strips per binned entry that a workgroup's output region is sized for
   // (plain tile stage / each output of a fused group).  Raised when a region
@@ -905,8 +905,9 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
                 Scratch& keep, vl_box** entries_out, uint64_t** start_out, uint64_t* total_out) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   const int64_t n = na + nb;
-  if (e->bin_mode == 1) {
-    // atomic binning: count per (tile, layer) key, scan, scatter
+  if (e->bin_mode >= 1) {
+    // atomic binning: count per (tile, layer) key, scan, scatter (bin_mode 2:
+    // the atomics grouped by distinct key over the wave)
     uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt + 1);
     Scratch s(e);
     uint32_t* cnt = s.get<uint32_t>(2 * (size_t)nt);
@@ -914,13 +915,13 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
     if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
     hipEvent_t pa = prof_mark(e);
     launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
-    launch_bin_count(a, na, xr, b, nb, g, cnt, e->st);
+    launch_bin_count(a, na, xr, b, nb, g, cnt, e->bin_mode == 2, e->st);
     scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
     uint64_t total = 0;
     VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
     vl_box* entries = keep.get<vl_box>(total + 1);
     if (!entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
-    launch_bin_scatter(a, na, xr, xs, b, nb, g, start, cnt, entries, e->st);
+    launch_bin_scatter(a, na, xr, xs, b, nb, g, start, cnt, entries, e->bin_mode == 2, e->st);
     HIPCK(hipGetLastError());
     hipEvent_t pb = prof_mark(e);
     // algorithmic bytes: the boxes read twice (count, scatter), 16 B written
@@ -1614,7 +1615,7 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "fuse_target out of range");
     e->fuse_target = (int)v;
   } else if (k == "bin_mode") {
-    if (v != 0 && v != 1) return fail(VL_EINVAL, "bin_mode must be 0 or 1");
+    if (v < 0 || v > 2) return fail(VL_EINVAL, "bin_mode must be 0, 1 or 2");
     e->bin_mode = (int)v;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
@@ -1807,13 +1808,13 @@ static int layer_pairs(vl_engine* e, const vl_layer* a, int mode, uint32_t* pare
   unsigned long long* sums = s.get<unsigned long long>(3);
   if (s.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
   launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
-  launch_bin_count(a->d, a->n, closed, nullptr, 0, g, cnt, e->st);
+  launch_bin_count(a->d, a->n, closed, nullptr, 0, g, cnt, e->bin_mode == 2, e->st);
   scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
   uint64_t total = 0;
   VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
   uint32_t* idx = s.get<uint32_t>(total + 1);
   if (!idx) return fail(VL_ENOMEM, "pair binning (%llu entries)", (unsigned long long)total);
-  launch_bin_scatter_idx(a->d, a->n, closed, g, start, cnt, idx, e->st);
+  launch_bin_scatter_idx(a->d, a->n, closed, g, start, cnt, idx, e->bin_mode == 2, e->st);
   launch_build_desc(g, start, TOP_UNION, desc, e->st);
   launch_fill_u64((uint64_t*)sums, 0ull, 3, e->st);
   launch_pairs(desc, (int)nt, idx, a->d, g, mode, parent, touched, sums, e->st);

@@ -156,11 +156,12 @@ void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, 
 // Atomic binning: cnt[2*tile + layer] (zeroed by the caller) += the boxes
 // meeting each tile; after an exclusive scan of cnt into start, the scatter
 // counts cnt back down to zero while placing every (box, tile) pair in its
-// key's range (entries of a key in no particular order).
+// key's range (entries of a key in no particular order).  grouped: one
+// atomic per distinct key of a wave instead of one per run of equal keys.
 void launch_bin_count(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
-                      hipStream_t st);
+                      bool grouped, hipStream_t st);
 void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const vl_box* b, int64_t nb, Grid g,
-                        const uint64_t* start, uint32_t* cnt, vl_box* entries, hipStream_t st);
+                        const uint64_t* start, uint32_t* cnt, vl_box* entries, bool grouped, hipStream_t st);
 // AND tiles: appends to list (count in *list_n, zeroed by the caller) the
 // tiles whose result may be non-empty (a clipped A box overlaps a B box, or
 // the tile is too large to test); the others get tab_n[t] = tab_n[ntab+t] = 0.
@@ -168,7 +169,7 @@ void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, 
                        uint32_t* tab_n, int ntab, hipStream_t st);
 // The same scatter writing box indices (idx) instead of boxes (one layer).
 void launch_bin_scatter_idx(const vl_box* a, int64_t na, Xform xra, Grid g, const uint64_t* start, uint32_t* cnt,
-                            uint32_t* idx, hipStream_t st);
+                            uint32_t* idx, bool grouped, hipStream_t st);
 // Tile descriptors from the key bounds (start has 2*ntiles+1 entries).
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st);
 // Tile kernel: result strips of every tile.  Strips touching an internal seam

@@ -33,22 +33,66 @@
 // kernel (capacity 255); larger tiles are split into x-sub-columns by the
 // host.  Each workgroup writes its strips into its own region; a reorder
 // pass restores tile order.
+#include <utility>
+
 #include "vl_tile_dev.h"
 
 namespace vl {
 
 namespace {
 
+// Quotients of grid offsets (below 2^32) by a tile size d >= 1 without a
+// division per box: m = floor(2^32 / d) gives floor(x * m / 2^32) = q or q - 1
+// for q = floor(x / d), and one comparison of the remainder settles it.
+struct Div32 {
+  uint32_t d, m;
+  int mode;   // 0: 2 <= d <= 2^31; 1: d == 1; 2: 2^31 < d < 2^32 (q is 0 or 1); 3: d >= 2^32 (q = 0)
+};
+
+__device__ __forceinline__ Div32 make_div(int64_t d) {
+  Div32 v{1u, 0u, 1};
+  if (d >= ((int64_t)1 << 32)) v = Div32{0u, 0u, 3};
+  else if (d > ((int64_t)1 << 31)) v = Div32{(uint32_t)d, 0u, 2};
+  else if (d >= 2) v = Div32{(uint32_t)d, (uint32_t)((1ull << 32) / (uint64_t)d), 0};
+  return v;
+}
+
+__device__ __forceinline__ uint32_t div_q(uint32_t x, const Div32& v) {
+  if (v.mode == 0) {
+    uint32_t q = __umulhi(x, v.m);
+    if (x - q * v.d >= v.d) q++;
+    return q;
+  }
+  if (v.mode == 1) return x;
+  if (v.mode == 2) return x >= v.d ? 1u : 0u;
+  return 0u;
+}
+
+// The tile sizes of grid g as reciprocals (computed once per thread).
+struct GridDiv {
+  Div32 x, y;
+};
+__device__ __forceinline__ GridDiv grid_div(const Grid& g) { return GridDiv{make_div(g.tw), make_div(g.th)}; }
+
 // Tile index range of box b grown by xr (64-bit arithmetic, clipped to the grid).
-__device__ __forceinline__ bool tile_range(const vl_box& b, const Xform& xr, const Grid& g, int& c1, int& c2,
-                                           int& r1, int& r2) {
+__device__ __forceinline__ bool tile_range(const vl_box& b, const Xform& xr, const Grid& g, const GridDiv& gd,
+                                           int& c1, int& c2, int& r1, int& r2) {
   const int64_t bx1 = max((int64_t)b.x1 - xr.w, g.x0), bx2 = min((int64_t)b.x2 + xr.e, g.x1);
   const int64_t by1 = max((int64_t)b.y1 - xr.s, g.y0), by2 = min((int64_t)b.y2 + xr.n, g.y1);
   if (bx1 >= bx2 || by1 >= by2) return false;
-  c1 = (int)((bx1 - g.x0) / g.tw);
-  c2 = (int)((bx2 - 1 - g.x0) / g.tw);
-  r1 = (int)((by1 - g.y0) / g.th);
-  r2 = (int)((by2 - 1 - g.y0) / g.th);
+  const uint64_t ox2 = (uint64_t)(bx2 - 1 - g.x0), oy2 = (uint64_t)(by2 - 1 - g.y0);
+  if (((ox2 | oy2) >> 32) == 0) {
+    // offsets below 2^32 (every grid whose domain lies in the int32 range)
+    c1 = (int)div_q((uint32_t)(bx1 - g.x0), gd.x);
+    c2 = (int)div_q((uint32_t)ox2, gd.x);
+    r1 = (int)div_q((uint32_t)(by1 - g.y0), gd.y);
+    r2 = (int)div_q((uint32_t)oy2, gd.y);
+  } else {
+    c1 = (int)((bx1 - g.x0) / g.tw);
+    c2 = (int)((int64_t)ox2 / g.tw);
+    r1 = (int)((by1 - g.y0) / g.th);
+    r2 = (int)((int64_t)oy2 / g.th);
+  }
   return true;
 }
 
@@ -72,12 +116,13 @@ __global__ __launch_bounds__(256) void k_bin_span(const vl_box* __restrict__ a, 
                                                   const vl_box* __restrict__ b, int64_t nb, Grid g,
                                                   uint32_t* __restrict__ cnt) {
   const int64_t n = na + nb;
+  const GridDiv gd = grid_div(g);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int c1, c2, r1, r2;
     const bool isA = i < na;
     const vl_box v = isA ? a[i] : b[i - na];
     uint32_t k = 0;
-    if (tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2))
+    if (tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2))
       k = (uint32_t)(c2 - c1 + 1) * (uint32_t)(r2 - r1 + 1);
     cnt[i] = k;
   }
@@ -88,11 +133,12 @@ __global__ __launch_bounds__(256) void k_bin_emit(const vl_box* __restrict__ a, 
                                                   const uint64_t* __restrict__ pos, uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ vals) {
   const int64_t n = na + nb;
+  const GridDiv gd = grid_div(g);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int c1, c2, r1, r2;
     const bool isA = i < na;
     const vl_box v = isA ? a[i] : b[i - na];
-    if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) continue;
+    if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2)) continue;
     uint64_t p = pos[i];
     const uint32_t layer = isA ? 0u : 1u;
     for (int r = r1; r <= r2; r++)
@@ -160,116 +206,290 @@ __device__ __forceinline__ Seg segment_of(bool act, uint32_t key, int lane) {
   return s;
 }
 
+// Grouped form: the active lanes of the wave are grouped by equal key, not
+// only adjacent equal keys, one group per distinct key (up to GROUP_ITERS
+// keys; lanes left after that are groups of their own).  Each group's leader
+// issues its atomic, all leaders in one instruction.  Boxes produced in
+// another grid's tile order interleave the tiles of this grid, so runs are
+// short while the wave still holds few distinct keys.
+constexpr int GROUP_ITERS = 16;
+
+struct Grp {
+  int leader;   // lane of this lane's group leader
+  int rank;     // position in the group
+  int len;      // group size (valid in every member)
+  bool is_leader;
+};
+
+__device__ __forceinline__ Grp group_of(bool act, uint32_t key, int lane) {
+  uint64_t mine = 0;   // this lane's group (its lanes)
+  uint64_t pend = __ballot(act);
+  for (int it = 0; pend && it < GROUP_ITERS; it++) {
+    const int ld = __ffsll((unsigned long long)pend) - 1;
+    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, ld);
+    const bool in = act && key == k;
+    const uint64_t grp = __ballot(in);   // every lane of key k is still pending
+    if (in) mine = grp;
+    pend &= ~grp;
+  }
+  Grp r{lane, 0, 1, act};
+  if (mine) {
+    r.leader = __ffsll((unsigned long long)mine) - 1;
+    r.rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+    r.len = __popcll(mine);
+    r.is_leader = r.leader == lane;
+  }
+  return r;
+}
+
+struct Lead {
+  int leader;   // lane of the group's leader
+  int rank;     // position in the group
+  int len;      // group size
+  bool lead;    // this lane issues the group's atomic
+};
+
+template <bool GROUPED>
+__device__ __forceinline__ Lead lead_of(bool act, uint32_t key, int lane) {
+  if constexpr (GROUPED) {
+    const Grp gr = group_of(act, key, lane);
+    return Lead{gr.leader, gr.rank, gr.len, act && gr.is_leader};
+  } else {
+    const Seg sg = segment_of(act, key, lane);
+    return Lead{sg.head, sg.rank, sg.len, sg.is_head};
+  }
+}
+
+// A wave bins BIN_U consecutive 64-box chunks per iteration: all their loads
+// are issued first, and (in the scatter) the atomics of every chunk's first
+// tile are in flight together, so the wave waits on memory once per
+// iteration rather than once per chunk.
+constexpr int BIN_U = 4;
+
+__device__ __forceinline__ uint32_t bin_key(int r, int c, const Grid& g, uint32_t layer) {
+  return (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
+}
+
+template <bool GROUPED>
 __global__ __launch_bounds__(256) void k_bin_count(const vl_box* __restrict__ a, int64_t na, Xform xra,
                                                    const vl_box* __restrict__ b, int64_t nb, Grid g,
                                                    uint32_t* __restrict__ cnt) {
   const int64_t n = na + nb;
   const int lane = __lane_id();
-  const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); w0 < n; w0 += wstride) {
-    const int64_t i = w0 + lane;
-    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
-    const bool isA = i < na;
-    if (i < n) {
-      const vl_box v = isA ? a[i] : b[i - na];
-      if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) { c2 = c1 - 1; r2 = r1 - 1; }
+  const GridDiv gd = grid_div(g);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+  for (int64_t base = wave * (WAVE * BIN_U); base < n; base += nwaves * (WAVE * BIN_U)) {
+    vl_box v[BIN_U];
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      const int64_t i = base + u * WAVE + lane;
+      v[u] = vl_box{0, 0, 0, 0};
+      if (i < n) v[u] = i < na ? a[i] : b[i - na];
     }
-    const int wc = c2 - c1 + 1, m = (c2 >= c1 && r2 >= r1) ? wc * (r2 - r1 + 1) : 0;
-    const uint32_t layer = isA ? 0u : 1u;
-    for (int j = 0; __ballot(j < m); j++) {
-      const bool act = j < m;
-      uint32_t key = 0;
-      if (act) {
-        const int r = r1 + j / wc, c = c1 + j % wc;
-        key = (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      const int64_t i = base + u * WAVE + lane;
+      const bool isA = i < na;
+      int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
+      if (i < n && !tile_range(v[u], isA ? xra : Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2)) { c2 = c1 - 1; r2 = r1 - 1; }
+      const int m = (c2 >= c1 && r2 >= r1) ? (c2 - c1 + 1) * (r2 - r1 + 1) : 0;
+      const uint32_t layer = isA ? 0u : 1u;
+      int c = c1, r = r1;   // the box's tiles in row-major order
+      for (int jt = 0; __ballot(jt < m); jt++) {
+        const bool act = jt < m;
+        const uint32_t key = act ? bin_key(r, c, g, layer) : 0u;
+        const Lead ld = lead_of<GROUPED>(act, key, lane);
+        if (ld.lead) atomicAdd(&cnt[key], (uint32_t)ld.len);
+        if (++c > c2) { c = c1; ++r; }
       }
-      const Seg sg = segment_of(act, key, lane);
-      if (sg.is_head) atomicAdd(&cnt[key], (uint32_t)sg.len);
     }
   }
 }
 
+template <bool GROUPED>
 __global__ __launch_bounds__(256) void k_bin_scatter(const vl_box* __restrict__ a, int64_t na, Xform xra, Xform xsa,
                                                      const vl_box* __restrict__ b, int64_t nb, Grid g,
                                                      const uint64_t* __restrict__ start, uint32_t* __restrict__ cnt,
                                                      vl_box* __restrict__ entries, uint32_t* __restrict__ idx) {
   const int64_t n = na + nb;
   const int lane = __lane_id();
-  const int64_t wstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); w0 < n; w0 += wstride) {
-    const int64_t i = w0 + lane;
-    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
-    const bool isA = i < na;
-    vl_box v{0, 0, 0, 0};
-    if (i < n) {
-      v = isA ? a[i] : b[i - na];
-      if (!tile_range(v, isA ? xra : Xform{0, 0, 0, 0}, g, c1, c2, r1, r2)) { c2 = c1 - 1; r2 = r1 - 1; }
+  const GridDiv gd = grid_div(g);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+  for (int64_t base = wave * (WAVE * BIN_U); base < n; base += nwaves * (WAVE * BIN_U)) {
+    vl_box v[BIN_U];
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      const int64_t i = base + u * WAVE + lane;
+      v[u] = vl_box{0, 0, 0, 0};
+      if (i < n) v[u] = i < na ? a[i] : b[i - na];
     }
-    const vl_box stored = isA ? xform(v, xsa) : v;
-    const int wc = c2 - c1 + 1, m = (c2 >= c1 && r2 >= r1) ? wc * (r2 - r1 + 1) : 0;
-    const uint32_t layer = isA ? 0u : 1u;
-    for (int j = 0; __ballot(j < m); j++) {
-      const bool act = j < m;
-      uint32_t key = 0;
-      if (act) {
-        const int r = r1 + j / wc, c = c1 + j % wc;
-        key = (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
-      }
-      const Seg sg = segment_of(act, key, lane);
-      uint32_t old = 0;
-      if (sg.is_head) old = atomicSub(&cnt[key], (uint32_t)sg.len);
-      old = (uint32_t)__shfl((int)old, sg.head, WAVE);
-      if (act) {
-        const uint64_t pos = start[key] + old - 1u - (uint32_t)sg.rank;
+    int c1[BIN_U], c2[BIN_U], r1[BIN_U], m[BIN_U];
+    uint32_t key0[BIN_U], old0[BIN_U];
+    uint64_t st0[BIN_U];
+    Lead l0[BIN_U];
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      const int64_t i = base + u * WAVE + lane;
+      const bool isA = i < na;
+      int x1 = 0, x2 = -1, y1 = 0, y2 = -1;
+      if (i < n && !tile_range(v[u], isA ? xra : Xform{0, 0, 0, 0}, g, gd, x1, x2, y1, y2)) { x2 = x1 - 1; y2 = y1 - 1; }
+      c1[u] = x1;
+      c2[u] = x2;
+      r1[u] = y1;
+      m[u] = (x2 >= x1 && y2 >= y1) ? (x2 - x1 + 1) * (y2 - y1 + 1) : 0;
+      key0[u] = m[u] > 0 ? bin_key(y1, x1, g, isA ? 0u : 1u) : 0u;
+      l0[u] = lead_of<GROUPED>(m[u] > 0, key0[u], lane);
+    }
+    // the first tile of every box: atomics and range starts of all chunks in flight
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      old0[u] = l0[u].lead ? atomicSub(&cnt[key0[u]], (uint32_t)l0[u].len) : 0u;
+      st0[u] = m[u] > 0 ? start[key0[u]] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < BIN_U; u++) {
+      const int64_t i = base + u * WAVE + lane;
+      const bool isA = i < na;
+      const vl_box stored = isA ? xform(v[u], xsa) : v[u];
+      const uint32_t old = (uint32_t)__shfl((int)old0[u], l0[u].leader, WAVE);
+      if (m[u] > 0) {
+        const uint64_t pos = st0[u] + old - 1u - (uint32_t)l0[u].rank;
         if (idx) idx[pos] = (uint32_t)i;   // index binning: the box's position
         else entries[pos] = stored;
+      }
+      // the further tiles of boxes spanning several, in row-major order
+      const uint32_t layer = isA ? 0u : 1u;
+      int c = c1[u] + 1, r = r1[u];
+      if (c > c2[u]) { c = c1[u]; ++r; }
+      for (int jt = 1; __ballot(jt < m[u]); jt++) {
+        const bool act = jt < m[u];
+        const uint32_t key = act ? bin_key(r, c, g, layer) : 0u;
+        const Lead ld = lead_of<GROUPED>(act, key, lane);
+        uint32_t o = 0;
+        if (ld.lead) o = atomicSub(&cnt[key], (uint32_t)ld.len);
+        o = (uint32_t)__shfl((int)o, ld.leader, WAVE);
+        if (act) {
+          const uint64_t pos = start[key] + o - 1u - (uint32_t)ld.rank;
+          if (idx) idx[pos] = (uint32_t)i;
+          else entries[pos] = stored;
+        }
+        if (++c > c2[u]) { c = c1[u]; ++r; }
       }
     }
   }
 }
 
 // AND pair filter.  A tile of A AND B has an empty result unless some box of
-// A, clipped to the tile, overlaps some box of B with positive area.  A wave
-// per tile: lane j holds B box j, the A boxes are broadcast one by one with
-// readlane and tested against all B boxes at once (one ballot each), as the
-// tile kernel's own pruning does.  Tiles with such a pair, or with more than
-// a wave of boxes of either layer, are appended to `list`; the others get
-// empty result counts here and are never loaded by the tile kernels.
+// A, clipped to the tile, overlaps some box of B with positive area.  Tiles
+// with such a pair, or with more than a wave of boxes of either layer, are
+// appended to `list`; the others get empty result counts here and are never
+// loaded by the tile kernels.
+//
+// Most tiles hold a few boxes of each layer, so a wave tests four tiles at
+// once, one per 16-lane row: lane s of a row holds A box s (clipped to the
+// tile) and B box s (empty boxes past the counts), and sixteen rotations of
+// the B boxes within the row (DPP row_ror) test every A box against every B
+// box, straight-line code with no memory traffic.  Tiles with more than 16
+// boxes of a layer are then tested one at a time by the whole wave (lane j
+// holds B box j, the A boxes broadcast with readlane).
+constexpr int AF_G = 16;   // lanes per tile in the grouped pass (one DPP row)
+
+template <int K>
+__device__ __forceinline__ int32_t row_ror(int32_t x) {
+  if constexpr (K == 0) return x;
+  else return __builtin_amdgcn_update_dpp(0, x, 0x120 + K, 0xF, 0xF, false);   // row_ror:K
+}
+
+// A box of this lane against the B box K lanes along its row (open overlap).
+template <int K>
+__device__ __forceinline__ bool hit_rot(int32_t ax1, int32_t ay1, int32_t ax2, int32_t ay2, const vl_box& b) {
+  const int32_t bx1 = row_ror<K>(b.x1), by1 = row_ror<K>(b.y1), bx2 = row_ror<K>(b.x2), by2 = row_ror<K>(b.y2);
+  return max(ax1, bx1) < min(ax2, bx2) && max(ay1, by1) < min(ay2, by2);
+}
+
+template <int... K>
+__device__ __forceinline__ bool hit_any(std::integer_sequence<int, K...>, int32_t ax1, int32_t ay1, int32_t ax2,
+                                        int32_t ay2, const vl_box& b) {
+  return (hit_rot<K>(ax1, ay1, ax2, ay2, b) | ...);
+}
+
+__device__ __forceinline__ bool overlap_open(int32_t ax1, int32_t ay1, int32_t ax2, int32_t ay2, const vl_box& b) {
+  return max(ax1, b.x1) < min(ax2, b.x2) && max(ay1, b.y1) < min(ay2, b.y2);
+}
+
+// Whole-wave test of one tile (nA, nB <= WAVE; arguments uniform).
+__device__ __forceinline__ bool and_tile_wave(const vl_box* __restrict__ entries, int64_t off, int nA, int nB,
+                                              int32_t x0, int32_t y0, int32_t x1, int32_t y1, int lane) {
+  int32_t ax1 = 0, ax2 = 0, ay1 = 0, ay2 = 0;
+  if (lane < nA) {
+    const vl_box a = entries[off + lane];
+    ax1 = max(a.x1, x0); ax2 = min(a.x2, x1); ay1 = max(a.y1, y0); ay2 = min(a.y2, y1);
+  }
+  vl_box b{0, 0, 0, 0};
+  if (lane < nB) b = entries[off + nA + lane];
+  for (int i = 0; i < nA; i++) {
+    const int32_t qx1 = __builtin_amdgcn_readlane(ax1, i), qx2 = __builtin_amdgcn_readlane(ax2, i);
+    const int32_t qy1 = __builtin_amdgcn_readlane(ay1, i), qy2 = __builtin_amdgcn_readlane(ay2, i);
+    if (__ballot(lane < nB && overlap_open(qx1, qy1, qx2, qy2, b))) return true;
+  }
+  return false;
+}
+
 __global__ __launch_bounds__(256) void k_and_filter(const TileDesc* __restrict__ desc, int ntiles,
                                                     const vl_box* __restrict__ entries, uint32_t* __restrict__ list,
                                                     uint32_t* __restrict__ list_n, uint32_t* __restrict__ tab_n,
                                                     int ntab) {
+  constexpr int TPW = WAVE / AF_G;   // tiles per wave
   const int lane = __lane_id();
+  const int sub = lane & (AF_G - 1), grp = lane / AF_G;
+  const int base = lane & ~(AF_G - 1);
+  const uint64_t gmask = ((1ull << AF_G) - 1ull) << base;
   const int nwaves = gridDim.x * (blockDim.x / WAVE);
-  for (int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < ntiles; t += nwaves) {
-    const int32_t* dw = (const int32_t*)(desc + t);
-    const int32_t x0 = uniform(dw[0]), y0 = uniform(dw[1]), x1 = uniform(dw[2]), y1 = uniform(dw[3]);
-    const int64_t off = uniform(desc[t].off);
-    const int nA = uniform(desc[t].nA), nB = uniform(desc[t].nB);
-    bool keep = false;
-    if (nA > 0 && nB > 0) {
-      if (nA > WAVE || nB > WAVE) {
-        keep = true;
-      } else {
-        int32_t ax1 = 0, ax2 = 0, ay1 = 0, ay2 = 0;
-        if (lane < nA) {
-          const vl_box a = entries[off + lane];
-          ax1 = max(a.x1, x0); ax2 = min(a.x2, x1); ay1 = max(a.y1, y0); ay2 = min(a.y2, y1);
-        }
-        vl_box b{0, 0, 0, 0};
-        if (lane < nB) b = entries[off + nA + lane];
-        for (int i = 0; i < nA; i++) {
-          const int32_t qx1 = __builtin_amdgcn_readlane(ax1, i), qx2 = __builtin_amdgcn_readlane(ax2, i);
-          const int32_t qy1 = __builtin_amdgcn_readlane(ay1, i), qy2 = __builtin_amdgcn_readlane(ay2, i);
-          const bool h = lane < nB && max(qx1, b.x1) < min(qx2, b.x2) && max(qy1, b.y1) < min(qy2, b.y2);
-          if (__ballot(h)) {
-            keep = true;
-            break;
-          }
-        }
-      }
+  for (int t0 = (blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * TPW; t0 < ntiles; t0 += nwaves * TPW) {
+    const int t = t0 + grp;
+    const bool tv = t < ntiles;
+    int32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0, nA = 0, nB = 0;
+    int64_t off = 0;
+    if (tv) {
+      const TileDesc d = desc[t];
+      x0 = d.x0; y0 = d.y0; x1 = d.x1; y1 = d.y1;
+      off = d.off;
+      nA = d.nA;
+      nB = d.nB;
     }
-    if (lane == 0) {
+    const bool both = tv && nA > 0 && nB > 0;
+    const bool small = nA <= AF_G && nB <= AF_G;
+    // grouped pass over the small tiles
+    const bool need = both && small;
+    int32_t ax1 = 0, ax2 = 0, ay1 = 0, ay2 = 0;
+    vl_box b{0, 0, 0, 0};
+    if (need && sub < nA) {
+      const vl_box a = entries[off + sub];
+      ax1 = max(a.x1, x0); ax2 = min(a.x2, x1); ay1 = max(a.y1, y0); ay2 = min(a.y2, y1);
+    }
+    if (need && sub < nB) b = entries[off + nA + sub];
+    // lanes past a tile's counts (and tiles not tested here) hold empty
+    // boxes, which overlap nothing
+    const bool hit = hit_any(std::make_integer_sequence<int, AF_G>{}, ax1, ay1, ax2, ay2, b);
+    bool keep = (__ballot(hit) & gmask) != 0ull;
+    // the larger tiles, one at a time with the whole wave
+    uint64_t big = __ballot(both && !small && sub == 0);
+    while (big) {
+      const int l = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int bA = __builtin_amdgcn_readlane(nA, l), bB = __builtin_amdgcn_readlane(nB, l);
+      bool k = true;
+      if (bA <= WAVE && bB <= WAVE) {
+        const int64_t boff = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(off >> 32), l) << 32) |
+                                       (uint32_t)__builtin_amdgcn_readlane((int)off, l));
+        k = and_tile_wave(entries, boff, bA, bB, __builtin_amdgcn_readlane(x0, l), __builtin_amdgcn_readlane(y0, l),
+                          __builtin_amdgcn_readlane(x1, l), __builtin_amdgcn_readlane(y1, l), lane);
+      }
+      if (base == l) keep = k;
+    }
+    if (tv && sub == 0) {
       if (keep) {
         list[atomicAdd(list_n, 1u)] = (uint32_t)t;
       } else {
@@ -1216,26 +1436,40 @@ void launch_bin_gather(const vl_box* a, int64_t na, Xform xsa, const vl_box* b, 
                        entries);
 }
 void launch_bin_count(const vl_box* a, int64_t na, Xform xra, const vl_box* b, int64_t nb, Grid g, uint32_t* cnt,
-                      hipStream_t st) {
-  if (na + nb > 0)
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, b, nb, g, cnt);
+                      bool grouped, hipStream_t st) {
+  const int64_t n = na + nb;
+  if (n <= 0) return;
+  const dim3 grid(grid_for((n + BIN_U - 1) / BIN_U));
+  if (grouped) hipLaunchKernelGGL(k_bin_count<true>, grid, dim3(256), 0, st, a, na, xra, b, nb, g, cnt);
+  else hipLaunchKernelGGL(k_bin_count<false>, grid, dim3(256), 0, st, a, na, xra, b, nb, g, cnt);
 }
 void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const vl_box* b, int64_t nb, Grid g,
-                        const uint64_t* start, uint32_t* cnt, vl_box* entries, hipStream_t st) {
-  if (na + nb > 0)
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(na + nb)), dim3(256), 0, st, a, na, xra, xsa, b, nb, g, start,
-                       cnt, entries, (uint32_t*)nullptr);
+                        const uint64_t* start, uint32_t* cnt, vl_box* entries, bool grouped, hipStream_t st) {
+  const int64_t n = na + nb;
+  if (n <= 0) return;
+  const dim3 grid(grid_for((n + BIN_U - 1) / BIN_U));
+  if (grouped)
+    hipLaunchKernelGGL(k_bin_scatter<true>, grid, dim3(256), 0, st, a, na, xra, xsa, b, nb, g, start, cnt, entries,
+                       (uint32_t*)nullptr);
+  else
+    hipLaunchKernelGGL(k_bin_scatter<false>, grid, dim3(256), 0, st, a, na, xra, xsa, b, nb, g, start, cnt, entries,
+                       (uint32_t*)nullptr);
 }
 void launch_bin_scatter_idx(const vl_box* a, int64_t na, Xform xra, Grid g, const uint64_t* start, uint32_t* cnt,
-                            uint32_t* idx, hipStream_t st) {
-  if (na > 0)
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(na)), dim3(256), 0, st, a, na, xra, Xform{0, 0, 0, 0},
+                            uint32_t* idx, bool grouped, hipStream_t st) {
+  if (na <= 0) return;
+  const dim3 grid(grid_for((na + BIN_U - 1) / BIN_U));
+  if (grouped)
+    hipLaunchKernelGGL(k_bin_scatter<true>, grid, dim3(256), 0, st, a, na, xra, Xform{0, 0, 0, 0},
+                       (const vl_box*)nullptr, (int64_t)0, g, start, cnt, (vl_box*)nullptr, idx);
+  else
+    hipLaunchKernelGGL(k_bin_scatter<false>, grid, dim3(256), 0, st, a, na, xra, Xform{0, 0, 0, 0},
                        (const vl_box*)nullptr, (int64_t)0, g, start, cnt, (vl_box*)nullptr, idx);
 }
 void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, uint32_t* list, uint32_t* list_n,
                        uint32_t* tab_n, int ntab, hipStream_t st) {
   if (ntiles > 0)
-    hipLaunchKernelGGL(k_and_filter, dim3(grid_for(ntiles, 4)), dim3(256), 0, st, desc, ntiles, entries, list,
+    hipLaunchKernelGGL(k_and_filter, dim3(grid_for(ntiles, 16)), dim3(256), 0, st, desc, ntiles, entries, list,
                        list_n, tab_n, ntab);
 }
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st) {
