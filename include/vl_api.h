@@ -82,6 +82,10 @@ int vl_engine_destroy(vl_engine* e);
  *                  atomic scatter, one atomic per distinct tile of a wave;
  *                  1: the same with one atomic per run of equal tiles;
  *                  0: by a stable radix sort of tile keys
+ *   "radix_bits"   digit width of the radix sorts (x-stitch, sort binning,
+ *                  box and event sorts): 8 (default) or 11
+ *   "workers"      programs: chains run concurrently on up to this many
+ *                  worker streams (1 to 16, default 4)
  *   "profile"      1: collect per-kernel statistics (vl_engine_kernel_stats)
  * Read-only: "fused_groups" = groups run fused by programs on this engine. */
 int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
@@ -92,9 +96,12 @@ int vl_engine_sync(vl_engine* e);
  * "erode_target" and "fuse_target" to the fastest of a few candidates, timed
  * on A NOT B (or the merge of A when b is NULL) plus GROW(A; 1), on
  * SIZE(A; -1), and on the fused ring of A (GROW(A; 1), SIZE(A; -1) and their
- * two differences with A).  The first two are returned when the pointers are
- * non-null ("fuse_target" through vl_engine_get_param).  Results of every
- * operator are independent of these parameters. */
+ * two differences with A); then "bin_mode" (2 or 1), "radix_bits" (8 or 11)
+ * and "max_wg" (0, 4096 or 2048), each timed on A NOT B, A AND B and GROW(A;
+ * 1).  A value changes only when a candidate is more than 3 % faster.  The
+ * first two are returned when the pointers are non-null (the others through
+ * vl_engine_get_param).  Results of every operator are independent of these
+ * parameters. */
 int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64_t* target,
                        int64_t* erode_target);
 

@@ -20,7 +20,7 @@ CONFIGS = [
     dict(tile_target=6, level_factor=2, bin_mode=1),         # many tiles, many seam levels; atomics per run
     dict(tile_nmax=63, tile_target=180, level_factor=4,      # most tiles on the 256-thread kernel, some
          bin_mode=0),                                        # split; binning by radix sort
-    dict(tile_nmax=127, tile_target=90, level_factor=16),
+    dict(tile_nmax=127, tile_target=90, level_factor=16, radix_bits=11),   # 11-bit radix digits
 ]
 
 

@@ -73,6 +73,25 @@ def test_program_op1_chains_match_plain(gpu_engine, nlayers, fuse):
     eng.close()
 
 
+@pytest.mark.parametrize("workers", [1, 2, 6])
+def test_program_worker_counts(gpu_engine, workers):
+    """Six OP1 chains on 1, 2 or 6 worker streams give the plain results."""
+    eng = V.Engine(0)
+    eng.set("workers", workers)
+    layers = [V.Layer(eng, V.synth_layer(V.SYN_STDCELL, 0x5EED0002, k % 4, 15000 + 2000 * k)) for k in range(6)]
+    temps = [[V.Layer(eng) for _ in range(6)] for _ in range(6)]
+    prog = V.Program(eng)
+    for L, t in zip(layers, temps):
+        op1_record(prog, L, t)
+    prog.run()
+    assert prog.assert_failures() == 0
+    for L, t in zip(layers, temps):
+        for got, exp in zip(t, op1_plain(L)):
+            assert np.array_equal(O.sort_boxes(got.boxes()), O.sort_boxes(exp.boxes()))
+    prog.close()
+    eng.close()
+
+
 def test_program_dependent_and_mixed_ops(gpu_engine):
     eng = V.Engine(0)
     rng = np.random.default_rng(41)
@@ -123,6 +142,8 @@ def test_autotune_sets_parameters_and_keeps_results(gpu_engine):
     assert chosen["tile_target"] in (12, 16, 20, 24, 32) and chosen["erode_target"] in (8, 12, 16)
     assert chosen["fuse_target"] in (4, 5, 6, 7, 8, 10) and eng.get("fuse_target") == chosen["fuse_target"]
     assert eng.get("tile_target") == chosen["tile_target"]
+    assert chosen["bin_mode"] in (1, 2) and chosen["radix_bits"] in (8, 11) and chosen["max_wg"] in (0, 2048, 4096)
+    assert all(eng.get(k) == chosen[k] for k in ("bin_mode", "radix_bits", "max_wg"))
     after = V.Layer(eng)
     V.CalculateFrameNOT(L1, L, after)
     assert np.array_equal(O.sort_boxes(before.boxes()), O.sort_boxes(after.boxes()))

@@ -147,6 +147,8 @@ struct vl_engine {
   int fuse = 1;                  // programs: fuse operator groups over one source layer
   int fuse_target = 6;           // mean boxes per tile of a fused group's grid
   int bin_mode = 2;              // 2: atomic binning, atomics grouped per wave (1: per run); 0: radix sort
+  int radix_bits = 8;            // digit width of the radix sorts (8 or 11)
+  int workers = 4;               // programs: chains run concurrently on up to this many worker streams
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   // Strips per binned entry that a workgroup's output region is sized for
   // (plain tile stage / each output of a fused group).  Raised when a region
@@ -947,7 +949,7 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
   uint32_t* vals = s2.get<uint32_t>(total + 1);
   uint32_t* keys2 = s2.get<uint32_t>(total + 1);
   uint32_t* vals2 = s2.get<uint32_t>(total + 1);
-  void* rtmp = s2.get<char>(radix_plan((int64_t)total).tmp_bytes);
+  void* rtmp = s2.get<char>(radix_plan((int64_t)total, e->radix_bits).tmp_bytes);
   vl_box* entries = keep.get<vl_box>(total + 1);
   if (s2.oom || !entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
   hipEvent_t pa = prof_mark(e);
@@ -955,7 +957,7 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
   int key_bits = 1;
   while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)(2 * nt)) key_bits++;
   hipEvent_t ps = prof_mark(e);
-  if (radix_sort_pairs_bits32(keys, vals, keys2, vals2, (int64_t)total, key_bits, rtmp, e->st)) {
+  if (radix_sort_pairs_bits32(keys, vals, keys2, vals2, (int64_t)total, key_bits, rtmp, e->st, e->radix_bits)) {
     std::swap(keys, keys2);
     std::swap(vals, vals2);
   }
@@ -1111,7 +1113,7 @@ int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
     uint32_t* vals2 = ss.get<uint32_t>(n);
     int32_t* next = ss.get<int32_t>(n);
     uint8_t* hasp = ss.get<uint8_t>(n);
-    void* rtmp = ss.get<char>(radix_plan(n).tmp_bytes);
+    void* rtmp = ss.get<char>(radix_plan(n, e->radix_bits).tmp_bytes);
     void* ptmp = ss.get<char>(partition_tmp_bytes(n));
     uint64_t* cnt = ss.get<uint64_t>(1);
     if (ss.oom) return fail(VL_ENOMEM, "stitch scratch");
@@ -1125,7 +1127,7 @@ int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
     uint64_t ns = 0;
     launch_stitch_starts(o.stitch.p, o.stitch_tag.p, n, g.x0, g.y0, yb, keys, vals, cnt, ptmp, e->st);
     VLCK(readback(e, &ns, cnt, sizeof(ns)));
-    if (radix_sort_pairs_bits(keys, vals, keys2, vals2, (int64_t)ns, xb + yb, rtmp, e->st)) {
+    if (radix_sort_pairs_bits(keys, vals, keys2, vals2, (int64_t)ns, xb + yb, rtmp, e->st, e->radix_bits)) {
       std::swap(keys, keys2);
       std::swap(vals, vals2);
     }
@@ -1617,6 +1619,12 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "bin_mode") {
     if (v < 0 || v > 2) return fail(VL_EINVAL, "bin_mode must be 0, 1 or 2");
     e->bin_mode = (int)v;
+  } else if (k == "workers") {
+    if (v < 1 || v > 16) return fail(VL_EINVAL, "workers must be 1 to 16");
+    e->workers = (int)v;
+  } else if (k == "radix_bits") {
+    if (v != 8 && v != 11) return fail(VL_EINVAL, "radix_bits must be 8 or 11");
+    e->radix_bits = (int)v;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
   }
@@ -1635,6 +1643,8 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "fuse") *v = e->fuse;
   else if (k == "fuse_target") *v = e->fuse_target;
   else if (k == "bin_mode") *v = e->bin_mode;
+  else if (k == "radix_bits") *v = e->radix_bits;
+  else if (k == "workers") *v = e->workers;
   else if (k == "fused_groups") *v = e->fused_groups;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
@@ -1999,10 +2009,10 @@ int vl_canonicalize(vl_engine* e, const vl_layer* a, vl_layer* c) {
   uint64_t* k2 = s.get<uint64_t>(n);
   uint32_t* v2 = s.get<uint32_t>(n);
   vl_box* tmpb = s.get<vl_box>(n);
-  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  void* rt = s.get<char>(radix_plan(n, e->radix_bits).tmp_bytes);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
   launch_corner_keys(c->d, k, v, n, e->st);
-  radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  radix_sort_pairs(k, v, k2, v2, n, rt, e->st, e->radix_bits);
   HIPCK(hipMemcpyAsync(tmpb, c->d, n * sizeof(vl_box), hipMemcpyDeviceToDevice, e->st));
   launch_box_gather(tmpb, v, c->d, n, e->st);
   HIPCK(hipStreamSynchronize(e->st));
@@ -2077,10 +2087,10 @@ int vl_event_sort(vl_engine* e, vl_tracker* arr, int64_t n) {
   uint64_t* k2 = s.get<uint64_t>(n);
   uint32_t* v2 = s.get<uint32_t>(n);
   vl_tracker* src = s.get<vl_tracker>(n);
-  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  void* rt = s.get<char>(radix_plan(n, e->radix_bits).tmp_bytes);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
   launch_tracker_keys(d, k, v, n, e->st);
-  radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  radix_sort_pairs(k, v, k2, v2, n, rt, e->st, e->radix_bits);
   HIPCK(hipMemcpyAsync(src, d, bytes, hipMemcpyDeviceToDevice, e->st));
   launch_tracker_gather(src, v, d, n, e->st);
   if (!dev) HIPCK(hipMemcpyAsync(arr, d, bytes, hipMemcpyDeviceToHost, e->st));
@@ -2109,10 +2119,10 @@ int vl_morton_sort_direct(vl_engine* e, vl_box* boxes, int64_t n) {
   uint64_t* k2 = s.get<uint64_t>(n);
   uint32_t* v2 = s.get<uint32_t>(n);
   vl_box* src = s.get<vl_box>(n);
-  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  void* rt = s.get<char>(radix_plan(n, e->radix_bits).tmp_bytes);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
   launch_morton_keys(d, k, v, n, e->st);
-  radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  radix_sort_pairs(k, v, k2, v2, n, rt, e->st, e->radix_bits);
   HIPCK(hipMemcpyAsync(src, d, bytes, hipMemcpyDeviceToDevice, e->st));
   launch_box_gather(src, v, d, n, e->st);
   if (!dev) HIPCK(hipMemcpyAsync(boxes, d, bytes, hipMemcpyDeviceToHost, e->st));
@@ -2140,10 +2150,10 @@ int vl_morton_sort_indirect(vl_engine* e, const vl_box* boxes, int64_t* idx, int
   uint32_t* v = s.get<uint32_t>(n);
   uint64_t* k2 = s.get<uint64_t>(n);
   uint32_t* v2 = s.get<uint32_t>(n);
-  void* rt = s.get<char>(radix_plan(n).tmp_bytes);
+  void* rt = s.get<char>(radix_plan(n, e->radix_bits).tmp_bytes);
   if (s.oom) return fail(VL_ENOMEM, "scratch");
   launch_morton_keys(d, k, v, n, e->st);
-  if (n > 1) radix_sort_pairs(k, v, k2, v2, n, rt, e->st);
+  if (n > 1) radix_sort_pairs(k, v, k2, v2, n, rt, e->st, e->radix_bits);
   launch_perm_to_index1(v, di, n, e->st);
   if (!devi) HIPCK(hipMemcpyAsync(idx, di, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, e->st));
   HIPCK(hipStreamSynchronize(e->st));
@@ -2185,7 +2195,7 @@ struct vl_program {
   std::vector<ProgOp> ops;
   int64_t failures = 0;
   std::vector<vl_engine*> workers;   // worker engines (same device, own streams)
-  int max_workers = 4;
+  int max_workers = 16;
   // graph capture (vl_program_set_graph)
   bool graph = false;
   hipGraph_t gr = nullptr;
@@ -2498,6 +2508,7 @@ static int program_workers(vl_program* p, int n) {
     w->fuse = e->fuse;
     w->fuse_target = e->fuse_target;
     w->bin_mode = e->bin_mode;
+    w->radix_bits = e->radix_bits;
     w->region_scale = std::max(w->region_scale, e->region_scale);
     w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
     (void)prof_resolve(w);
@@ -2717,7 +2728,7 @@ int vl_program_run(vl_program* p) {
   p->failures = 0;
   const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse != 0);
   const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
-  int nw = (int)std::min<size_t>(chains.size(), (size_t)p->max_workers);
+  int nw = (int)std::min<size_t>(chains.size(), (size_t)std::min(p->max_workers, p->e->workers));
   // the bounding boxes of the program's inputs (layers no operator writes)
   // are cached before concurrent chains read them
   {
@@ -3198,10 +3209,60 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
       if (outs[k]) vl_layer_destroy(outs[k]);
   }
   e->fuse_target = bft;
+  // Binning mode, radix digit width and the tile-kernel occupancy cap, each
+  // timed on the operators it affects; the current value stays unless a
+  // candidate is more than 3 % faster.
+  const int bm0 = e->bin_mode, rb0 = e->radix_bits, mw0 = e->max_wg_cap;
+  auto pick = [&](int* field, std::initializer_list<int> cands, const std::function<int(double*)>& f) {
+    const int v0 = *field;
+    int best = v0;
+    double ms0 = 1e30, bms2 = 1e30;
+    for (int v : cands) {
+      *field = v;
+      double m = 1e30;
+      for (int rep = 0; rep < 2; rep++) {
+        double acc = 0;
+        const int r = f(&acc);
+        if (r != VL_OK) {
+          *field = v0;
+          return r;
+        }
+        m = std::min(m, acc);
+      }
+      if (v == v0) ms0 = m;
+      if (m < bms2) {
+        bms2 = m;
+        best = v;
+      }
+    }
+    *field = bms2 > 0.97 * ms0 ? v0 : best;
+    return VL_OK;
+  };
+  // binary operators of A and B (or A's merge) plus A's GROW: binning, tile
+  // kernels and the x-stitch sort
+  auto ops_ab = [&](double* acc) {
+    int r = b ? vl_not(e, a, b, c) : vl_merge(e, a, c);
+    *acc += e->last_ms;
+    if (r == VL_OK && b) {
+      r = vl_and(e, a, b, c);
+      *acc += e->last_ms;
+    }
+    if (r == VL_OK) {
+      r = vl_grow(e, a, 1, 1, 1, 1, c);
+      *acc += e->last_ms;
+    }
+    return r;
+  };
+  if (rc == VL_OK) rc = pick(&e->bin_mode, {2, 1}, ops_ab);
+  if (rc == VL_OK) rc = pick(&e->radix_bits, {8, 11}, ops_ab);
+  if (rc == VL_OK) rc = pick(&e->max_wg_cap, {0, 4096, 2048}, ops_ab);
   if (rc != VL_OK) {
     e->target = t0;
     e->erode_target = et0;
     e->fuse_target = ft0;
+    e->bin_mode = bm0;
+    e->radix_bits = rb0;
+    e->max_wg_cap = mw0;
   }
   vl_layer_destroy(c);
   if (target) *target = e->target;

@@ -119,20 +119,23 @@ struct RadixPlan {
   int nblocks;
   size_t tmp_bytes;
 };
-RadixPlan radix_plan(int64_t n);
+// digit_bits: 8 or 11 (any other value means 8); the plan's scratch size
+// depends on it, so a sort takes the digit width its plan was made with.
+RadixPlan radix_plan(int64_t n, int digit_bits = 8);
 // Sorts keys/vals in place (uses alt buffers of the same size + tmp).  The
 // host reads the digit histograms once to skip passes in which all keys share
 // a digit (one synchronisation).
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
-                      int64_t n, void* tmp, hipStream_t st);
-// Same sort for keys below 2^key_bits: ceil(key_bits/8) passes, no histogram
-// pre-pass and no host synchronisation.  Returns true when the sorted pairs
-// are in keys_alt/vals_alt (odd number of passes), false when in keys/vals.
+                      int64_t n, void* tmp, hipStream_t st, int digit_bits = 8);
+// Same sort for keys below 2^key_bits: ceil(key_bits/digit_bits) passes, no
+// histogram pre-pass and no host synchronisation.  Returns true when the
+// sorted pairs are in keys_alt/vals_alt (odd number of passes), false when in
+// keys/vals.
 bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                           int key_bits, void* tmp, hipStream_t st);
+                           int key_bits, void* tmp, hipStream_t st, int digit_bits = 8);
 // The same for 32-bit keys (key_bits <= 32).
 bool radix_sort_pairs_bits32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                             int key_bits, void* tmp, hipStream_t st);
+                             int key_bits, void* tmp, hipStream_t st, int digit_bits = 8);
 void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st);
 void launch_tracker_gather(const vl_tracker* in, const uint32_t* perm, vl_tracker* out, int64_t n,
                            hipStream_t st);

@@ -1,17 +1,22 @@
 // vl_sort.hip — stable LSD radix sort of (u64 key, u32 value) pairs for gfx950.
 //
-// 8-bit digits.  One pre-pass reads the keys once and builds the histograms of
-// all eight digit positions; passes whose digit is the same for every key are
-// skipped (coordinates that use only the low bits of int32 leave the high
-// digits constant).  Each executed pass is reduce-then-scan:
+// Digits of 8 or 11 bits (an engine parameter the autotuner sets: 11-bit
+// digits take fewer passes over the keys, 8-bit ones less LDS per workgroup).
+// One pre-pass reads the keys once and builds the histograms of all digit
+// positions; passes whose digit is the same for every key are skipped
+// (coordinates that use only the low bits of int32 leave the high digits
+// constant).  Each executed pass is reduce-then-scan:
 //   upsweep   per-workgroup digit histogram of a 4096-key tile (LDS atomics)
-//   scan      exclusive scan of the digit-major [256 x tiles] histogram
-//   downsweep stable local ranking with wave64 ballots (8 ballots give the
-//             lanes that share a digit; popcount of the lower peers is the
-//             rank), per-wave digit counters in LDS, a block scan over digits,
-//             a local reorder in LDS, then a coalesced scatter: consecutive
-//             LDS slots of one digit go to consecutive global addresses.
+//   scan      exclusive scan of the digit-major [bins x tiles] histogram
+//   downsweep stable local ranking with wave64 ballots (one ballot per digit
+//             bit gives the lanes that share a digit; popcount of the lower
+//             peers is the rank), per-wave digit counters in LDS, a block
+//             scan over digits, a local reorder in LDS, then a coalesced
+//             scatter: consecutive LDS slots of one digit go to consecutive
+//             global addresses.
 // Key generation and payload gathers for the sort entry points live here too.
+#include <vector>
+
 #include "vl_internal.h"
 #include "vl_dev.h"
 
@@ -23,54 +28,66 @@ constexpr int RS_NT = 256;
 constexpr int RS_NW = RS_NT / WAVE;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_NT * RS_ITEMS;
-constexpr int RS_BINS = 256;
 
+template <int DB>
+struct Digits {
+  static constexpr int BINS = 1 << DB;
+  static constexpr int BPT = BINS / RS_NT;          // digits per thread
+  static constexpr int POS = (64 + DB - 1) / DB;    // digit positions of a 64-bit key
+  static constexpr uint32_t MASK = (uint32_t)BINS - 1u;
+  static_assert(BINS % RS_NT == 0, "whole digits per thread");
+};
+
+template <int DB>
 __global__ void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, uint32_t* ghist) {
-  __shared__ uint32_t h[8 * RS_BINS];
-  for (int i = threadIdx.x; i < 8 * RS_BINS; i += RS_NT) h[i] = 0;
+  using D = Digits<DB>;
+  __shared__ uint32_t h[D::POS * D::BINS];
+  for (int i = threadIdx.x; i < D::POS * D::BINS; i += RS_NT) h[i] = 0;
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * RS_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RS_NT) {
     uint64_t k = keys[i];
 #pragma unroll
-    for (int p = 0; p < 8; p++) atomicAdd(&h[p * RS_BINS + ((k >> (8 * p)) & 255)], 1u);
+    for (int p = 0; p < D::POS; p++) atomicAdd(&h[p * D::BINS + ((k >> (DB * p)) & D::MASK)], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * RS_BINS; i += RS_NT)
+  for (int i = threadIdx.x; i < D::POS * D::BINS; i += RS_NT)
     if (h[i]) atomicAdd(&ghist[i], h[i]);
 }
 
-template <typename K>
+template <typename K, int DB>
 __global__ __launch_bounds__(RS_NT) void k_rs_upsweep(const K* __restrict__ keys, int64_t n,
                                                        int shift, uint32_t* bhist, int nb) {
-  __shared__ uint32_t h[RS_BINS];
-  h[threadIdx.x] = 0;
+  using D = Digits<DB>;
+  __shared__ uint32_t h[D::BINS];
+  for (int d = threadIdx.x; d < D::BINS; d += RS_NT) h[d] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
   for (int k = 0; k < RS_ITEMS; k++) {
     int64_t i = base + (int64_t)k * RS_NT + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & D::MASK], 1u);
   }
   __syncthreads();
-  bhist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+  for (int d = threadIdx.x; d < D::BINS; d += RS_NT) bhist[(size_t)d * nb + blockIdx.x] = h[d];
 }
 
-template <typename K>
+template <typename K, int DB>
 __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const K* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin,
                                                          K* __restrict__ kout,
                                                          uint32_t* __restrict__ vout, int64_t n,
                                                          int shift, const uint32_t* __restrict__ goff,
                                                          int nb) {
+  using D = Digits<DB>;
   __shared__ K sk[RS_TILE];
   __shared__ uint32_t sv[RS_TILE];
-  __shared__ uint32_t wcnt[RS_NW][RS_BINS];
-  __shared__ uint32_t dstart[RS_BINS];
-  __shared__ uint32_t gbase[RS_BINS];
+  __shared__ uint32_t wcnt[RS_NW][D::BINS];
+  __shared__ uint32_t dstart[D::BINS];
+  __shared__ uint32_t gbase[D::BINS];
   __shared__ uint32_t ws[RS_NW];
   const int w = threadIdx.x / WAVE;
   const int lane = __lane_id();
-  for (int i = threadIdx.x; i < RS_NW * RS_BINS; i += RS_NT) (&wcnt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < RS_NW * D::BINS; i += RS_NT) (&wcnt[0][0])[i] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   K k[RS_ITEMS];
@@ -86,10 +103,10 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const K* __restrict__ ki
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; it++) {
-    const uint32_t d = (uint32_t)(k[it] >> shift) & 255u;
+    const uint32_t d = (uint32_t)(k[it] >> shift) & D::MASK;
     uint64_t peers = ~0ull;
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
+    for (int b = 0; b < DB; b++) {
       const bool bit = (d >> b) & 1u;
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
@@ -105,23 +122,37 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const K* __restrict__ ki
   }
   __syncthreads();
   {
-    const int d = threadIdx.x;   // RS_NT == RS_BINS
-    uint32_t run = 0;
+    // thread t owns digits t*BPT .. t*BPT+BPT-1: per-wave offsets, then a
+    // block scan of the threads' totals
+    uint32_t run[D::BPT];
+    uint32_t tsum = 0;
 #pragma unroll
-    for (int ww = 0; ww < RS_NW; ww++) {
-      uint32_t c = wcnt[ww][d];
-      wcnt[ww][d] = run;
-      run += c;
+    for (int j = 0; j < D::BPT; j++) {
+      const int d = threadIdx.x * D::BPT + j;
+      uint32_t acc = 0;
+#pragma unroll
+      for (int ww = 0; ww < RS_NW; ww++) {
+        const uint32_t c = wcnt[ww][d];
+        wcnt[ww][d] = acc;
+        acc += c;
+      }
+      run[j] = acc;
+      tsum += acc;
     }
     uint32_t tot;
-    uint32_t ex = block_excl_scan<uint32_t, RS_NT>(run, ws, &tot);
-    dstart[d] = ex;
-    gbase[d] = goff[(size_t)d * nb + blockIdx.x];
+    uint32_t ex = block_excl_scan<uint32_t, RS_NT>(tsum, ws, &tot);
+#pragma unroll
+    for (int j = 0; j < D::BPT; j++) {
+      const int d = threadIdx.x * D::BPT + j;
+      dstart[d] = ex;
+      ex += run[j];
+      gbase[d] = goff[(size_t)d * nb + blockIdx.x];
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; it++) {
-    const uint32_t d = (uint32_t)(k[it] >> shift) & 255u;
+    const uint32_t d = (uint32_t)(k[it] >> shift) & D::MASK;
     const uint32_t pos = dstart[d] + wcnt[w][d] + r[it];
     sk[pos] = k[it];
     sv[pos] = v[it];
@@ -131,7 +162,7 @@ __global__ __launch_bounds__(RS_NT) void k_rs_downsweep(const K* __restrict__ ki
   const int nvalid = rem < RS_TILE ? (int)rem : RS_TILE;
   for (int i = threadIdx.x; i < nvalid; i += RS_NT) {
     const K key = sk[i];
-    const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const uint32_t d = (uint32_t)(key >> shift) & D::MASK;
     const uint32_t g = gbase[d] + ((uint32_t)i - dstart[d]);
     kout[g] = key;
     vout[g] = sv[i];
@@ -212,47 +243,58 @@ unsigned grid_for(int64_t n) {
 
 }  // namespace
 
-RadixPlan radix_plan(int64_t n) {
+RadixPlan radix_plan(int64_t n, int digit_bits) {
   RadixPlan p;
   p.n = n;
   p.nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
-  size_t hb = (size_t)RS_BINS * (size_t)(p.nblocks > 0 ? p.nblocks : 1);
-  p.tmp_bytes = 8 * RS_BINS * sizeof(uint32_t) + 2 * hb * sizeof(uint32_t) + scan_tmp_bytes(hb) + 1024;
+  const size_t bins = digit_bits == 11 ? 2048 : 256;
+  const size_t pos = digit_bits == 11 ? 6 : 8;
+  size_t hb = bins * (size_t)(p.nblocks > 0 ? p.nblocks : 1);
+  p.tmp_bytes = pos * bins * sizeof(uint32_t) + 2 * hb * sizeof(uint32_t) + scan_tmp_bytes(hb) + 1024;
   return p;
 }
 
-void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
-                      int64_t n, void* tmp, hipStream_t st) {
-  if (n <= 1) return;
-  RadixPlan p = radix_plan(n);
+namespace {
+
+template <typename K, int DB>
+void rs_pass(const K* ks, const uint32_t* vs, K* kd, uint32_t* vd, int64_t n, int shift, uint32_t* bhist,
+             uint32_t* goff, char* stmp, int nb, hipStream_t st) {
+  const size_t hb = (size_t)Digits<DB>::BINS * nb;
+  hipLaunchKernelGGL((k_rs_upsweep<K, DB>), dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
+  scan_exclusive_u32(bhist, goff, hb, nullptr, stmp, st);
+  hipLaunchKernelGGL((k_rs_downsweep<K, DB>), dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
+}
+
+template <int DB>
+void sort_pairs_hist(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, void* tmp,
+                     hipStream_t st) {
+  using D = Digits<DB>;
+  RadixPlan p = radix_plan(n, DB);
   const int nb = p.nblocks;
-  const size_t hb = (size_t)RS_BINS * nb;
+  const size_t hb = (size_t)D::BINS * nb;
   uint32_t* ghist = (uint32_t*)tmp;
-  uint32_t* bhist = ghist + 8 * RS_BINS;
+  uint32_t* bhist = ghist + D::POS * D::BINS;
   uint32_t* goff = bhist + hb;
   char* stmp = (char*)(goff + hb);
   stmp = (char*)(((uintptr_t)stmp + 255) & ~(uintptr_t)255);
-  launch_fill_u32(ghist, 0u, 8 * RS_BINS, st);
+  launch_fill_u32(ghist, 0u, (size_t)D::POS * D::BINS, st);
   unsigned hg = (unsigned)((n + RS_NT * 16 - 1) / (RS_NT * 16));
   if (hg > 2048) hg = 2048;
-  hipLaunchKernelGGL(k_rs_hist_all, dim3(hg), dim3(RS_NT), 0, st, keys, n, ghist);
-  uint32_t h[8 * RS_BINS];
-  (void)hipMemcpyAsync(h, ghist, sizeof(h), hipMemcpyDeviceToHost, st);
+  hipLaunchKernelGGL(k_rs_hist_all<DB>, dim3(hg), dim3(RS_NT), 0, st, keys, n, ghist);
+  std::vector<uint32_t> h((size_t)D::POS * D::BINS);
+  (void)hipMemcpyAsync(h.data(), ghist, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
   (void)hipStreamSynchronize(st);
   uint64_t* ks = keys;
   uint32_t* vs = vals;
   uint64_t* kd = keys_alt;
   uint32_t* vd = vals_alt;
   int done = 0;
-  for (int pass = 0; pass < 8; pass++) {
+  for (int pass = 0; pass < D::POS; pass++) {
     bool trivial = false;
-    for (int d = 0; d < RS_BINS; d++)
-      if ((int64_t)h[pass * RS_BINS + d] == n) trivial = true;
+    for (int d = 0; d < D::BINS; d++)
+      if ((int64_t)h[(size_t)pass * D::BINS + d] == n) trivial = true;
     if (trivial) continue;
-    const int shift = 8 * pass;
-    hipLaunchKernelGGL(k_rs_upsweep<uint64_t>, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
-    scan_exclusive_u32(bhist, goff, hb, nullptr, stmp, st);
-    hipLaunchKernelGGL(k_rs_downsweep<uint64_t>, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
+    rs_pass<uint64_t, DB>(ks, vs, kd, vd, n, DB * pass, bhist, goff, stmp, nb, st);
     uint64_t* tk = ks; ks = kd; kd = tk;
     uint32_t* tv = vs; vs = vd; vd = tv;
     done++;
@@ -261,15 +303,15 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
     hipLaunchKernelGGL(k_copy_pairs<uint64_t>, dim3(grid_for(n)), dim3(256), 0, st, ks, vs, keys, vals, n);
 }
 
-namespace {
-template <typename K>
+template <typename K, int DB>
 bool sort_bits(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n, int key_bits, void* tmp,
                hipStream_t st) {
   if (n <= 1 || key_bits <= 0) return false;
-  RadixPlan p = radix_plan(n);
+  using D = Digits<DB>;
+  RadixPlan p = radix_plan(n, DB);
   const int nb = p.nblocks;
-  const size_t hb = (size_t)RS_BINS * nb;
-  uint32_t* bhist = (uint32_t*)tmp + 8 * RS_BINS;
+  const size_t hb = (size_t)D::BINS * nb;
+  uint32_t* bhist = (uint32_t*)tmp + D::POS * D::BINS;
   uint32_t* goff = bhist + hb;
   char* stmp = (char*)(goff + hb);
   stmp = (char*)(((uintptr_t)stmp + 255) & ~(uintptr_t)255);
@@ -278,26 +320,34 @@ bool sort_bits(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t
   K* kd = keys_alt;
   uint32_t* vd = vals_alt;
   bool alt = false;
-  for (int shift = 0; shift < key_bits && shift < (int)(8 * sizeof(K)); shift += 8) {
-    hipLaunchKernelGGL(k_rs_upsweep<K>, dim3(nb), dim3(RS_NT), 0, st, ks, n, shift, bhist, nb);
-    scan_exclusive_u32(bhist, goff, hb, nullptr, stmp, st);
-    hipLaunchKernelGGL(k_rs_downsweep<K>, dim3(nb), dim3(RS_NT), 0, st, ks, vs, kd, vd, n, shift, goff, nb);
+  for (int shift = 0; shift < key_bits && shift < (int)(8 * sizeof(K)); shift += DB) {
+    rs_pass<K, DB>(ks, vs, kd, vd, n, shift, bhist, goff, stmp, nb, st);
     K* tk = ks; ks = kd; kd = tk;
     uint32_t* tv = vs; vs = vd; vd = tv;
     alt = !alt;
   }
   return alt;
 }
+
 }  // namespace
 
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                      int64_t n, void* tmp, hipStream_t st, int digit_bits) {
+  if (n <= 1) return;
+  if (digit_bits == 11) sort_pairs_hist<11>(keys, vals, keys_alt, vals_alt, n, tmp, st);
+  else sort_pairs_hist<8>(keys, vals, keys_alt, vals_alt, n, tmp, st);
+}
+
 bool radix_sort_pairs_bits(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                           int key_bits, void* tmp, hipStream_t st) {
-  return sort_bits<uint64_t>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
+                           int key_bits, void* tmp, hipStream_t st, int digit_bits) {
+  if (digit_bits == 11) return sort_bits<uint64_t, 11>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
+  return sort_bits<uint64_t, 8>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
 }
 
 bool radix_sort_pairs_bits32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                             int key_bits, void* tmp, hipStream_t st) {
-  return sort_bits<uint32_t>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
+                             int key_bits, void* tmp, hipStream_t st, int digit_bits) {
+  if (digit_bits == 11) return sort_bits<uint32_t, 11>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
+  return sort_bits<uint32_t, 8>(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, st);
 }
 
 void launch_tracker_keys(const vl_tracker* t, uint64_t* keys, uint32_t* vals, int64_t n, hipStream_t st) {
