@@ -40,9 +40,12 @@ host's cores.  Its time grows faster than the box count (the frame grid has
 only nodes^(1/4) tiles per side), so a whole 10M-box layer is not sampled.
 
 Also reported in the same line: "c2_10M_total", the same ring with 10M boxes
-over the 4 layers (2.5M per layer), plain and batched; and "serving", windowed
-query latency (p50/p99) and throughput of the query server on a resident
-50M-box 4-layer design under Poisson arrivals.
+over the 4 layers (2.5M per layer), plain and batched; "c3_uneven_tiles",
+AND/NOT/OR/GROW+2 over 4096 packed tiles of uneven size (~50M boxes), plain,
+as one recorded program and as its graph, autotuned; "c4_full_chip", AND,
+NOT, GROW+1 and the exact Klee area on a 2 x 180M-box layer pair; and
+"serving", windowed query latency (p50/p99) and throughput of the query
+server on a resident 50M-box 4-layer design under Poisson arrivals.
 """
 import argparse
 import json
@@ -81,6 +84,9 @@ def parse():
     p.add_argument("--serving-boxes", type=int, default=50_000_000,
                    help="serving design size over 4 layers (0 = skip the serving measurement)")
     p.add_argument("--serving-requests", type=int, default=400)
+    p.add_argument("--no-c3", action="store_true", help="skip the 4096-uneven-tile measurement")
+    p.add_argument("--c4-boxes", type=int, default=360_000_000,
+                   help="full-chip layer pair size for the C4 measurement (0 = skip)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="first copy this many MiB device-to-device (a copy kernel of known size for "
                         "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -233,6 +239,84 @@ def serving(V, args):
             "assert_zero_failures": int((res["count"] > 0).sum()), "index_build_s": t_index}
 
 
+def c3_uneven_tiles(V, args):
+    """4096 independent tiles (box counts LogNormal(ln 5000, 1.34) clipped to
+    [100, 200000], split over two layers, tile-local coordinates in [0, 2^16)^2)
+    packed into one layer pair at a 2^17 pitch; one step = AND, NOT, OR and
+    GROW +2 of the pair, operator by operator (plain), as one recorded program
+    (batched) and as its captured graph, autotuned first."""
+    seed, ntiles, pitch = 0x5EED0003, 4096, 1 << 17
+    counts = np.array([V.synth_tile_count(seed, t) for t in range(ntiles)])
+    A_t, B_t = [], []
+    for t in range(ntiles):
+        na = int(counts[t]) // 2
+        off = np.array([(t % 64) * pitch, (t // 64) * pitch] * 2, np.int32)
+        A_t.append(V.synth_layer(V.SYN_TILES, seed, 2 * t, na) + off)
+        B_t.append(V.synth_layer(V.SYN_TILES, seed, 2 * t + 1, int(counts[t]) - na) + off)
+    A, B = np.concatenate(A_t), np.concatenate(B_t)
+    del A_t, B_t
+    eng = V.Engine(0)
+    LA, LB = V.Layer(eng, A), V.Layer(eng, B)
+    outs = [V.Layer(eng) for _ in range(4)]
+    tuned = eng.autotune(LA, LB)
+
+    def plain():
+        V.CalculateAND(LA, LB, outs[0])
+        V.CalculateFrameNOT(LA, LB, outs[1])
+        V.CalculateOR(LA, LB, outs[2])
+        V.CalculateGROWLayer(LA, outs[3], (2, 2, 2, 2))
+
+    prog = V.Program(eng)
+    prog.add(V.K_BOOST_CONTROL_AND, LA, LB, outs[0])
+    prog.add(V.K_BOOST_CONTROL_NOT, LA, LB, outs[1])
+    prog.add(V.K_BOOST_CONTROL_OR, LA, LB, outs[2])
+    prog.add(V.VL_OP_GROW, LA, None, outs[3], (2, 2, 2, 2))
+    plain()
+    ref = [o.n_used for o in outs]
+    ins = 3 * (len(A) + len(B)) + len(A)
+    res = {"tiles": ntiles, "boxes": int(len(A) + len(B)), "input_boxes_per_step": int(ins), "autotuned": tuned}
+    for name, fn, w in (("plain", plain, args.warmup), ("batched", prog.run, args.warmup), ("graph", prog.run, 1)):
+        if name == "graph":
+            prog.set_graph(True)
+        secs, _, _ = time_steps(fn, eng, args.steps, max(w, 1))
+        if [o.n_used for o in outs] != ref:
+            raise SystemExit(f"c3 {name}: output sizes differ from the plain form's")
+        res[f"{name}_ms_per_step"] = 1000.0 * secs / args.steps
+        res[f"{name}_boxes_per_s"] = ins * args.steps / secs
+    prog.close()
+    eng.close()
+    return res
+
+
+def c4_full_chip(V, args):
+    """One layer pair of 2 x (boxes/2) on [0, 2^26)^2 (70 % routing tracks on
+    a 5-DBU grid, 30 % 6x6 vias); one step = AND, NOT, GROW +1 and the exact
+    Klee area of A."""
+    seed, n = 0x5EED0004, args.c4_boxes // 2
+    a = V.synth_layer(V.SYN_CHIP, seed, 0, n)
+    b = V.synth_layer(V.SYN_CHIP, seed, 1, n)
+    eng = V.Engine(0)
+    LA, LB = V.Layer(eng, a), V.Layer(eng, b)
+    del a, b
+    AND, NOT, GROW = V.Layer(eng), V.Layer(eng), V.Layer(eng)
+    area = {}
+
+    def step():
+        V.CalculateAND(LA, LB, AND)
+        V.CalculateFrameNOT(LA, LB, NOT)
+        V.CalculateGROWLayer(LA, GROW, (1, 1, 1, 1))
+        area["A"] = V.klee_area(LA)
+
+    step()
+    if V.klee_area(AND) + V.klee_area(NOT) != area["A"]:
+        raise SystemExit("c4: area(A AND B) + area(A NOT B) != area(A)")
+    ins = 2 * (2 * n) + n + n
+    secs, _, _ = time_steps(step, eng, args.steps, args.warmup)
+    eng.close()
+    return {"boxes": 2 * n, "input_boxes_per_step": int(ins), "ms_per_step": 1000.0 * secs / args.steps,
+            "boxes_per_s": ins * args.steps / secs, "klee_area_A": int(area["A"])}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -379,6 +463,7 @@ def main():
         M = V.Layer(eng)
         V.CalculateBoostOperation(rings[k].L, rings[k].L, M, V.K_BOOST_CONTROL_MERGE)
         L = M.boxes()[: args.cpu_sample]
+        M.close()
         n = len(L)
         t0 = time.perf_counter()
         L1 = O.par_op("grow", L, args=(1, 1, 1, 1), nthreads=nthreads)
@@ -397,11 +482,17 @@ def main():
                                           f"(a band of rows of its {npl}), "
                                           f"{secs:.1f} s of CPU work on {nthreads} threads"}
         print(f"[bench] cpu baseline: {secs:.1f} s", file=sys.stderr, flush=True)
+    # the configurations below run on engines of their own: the ring's
+    # program, layers and engine (with its memory pools) go first
+    prog.close()
+    for r in rings:
+        for lay in (r.L, r.L1, r.L2, r.outer, r.inner, r.and1, r.ring):
+            lay.close()
+    eng.close()
+    del rings
     if args.c2_total:
         # the same ring on the C2 design as 10M boxes in total (2.5M per layer)
         per = args.c2_total // 4
-        prog.close()
-        eng.close()
         eng2 = V.Engine(0)
         rings2 = [Ring(V, eng2, lay[:per]) for lay in layers]
         prog2 = V.Program(eng2)
@@ -428,6 +519,14 @@ def main():
               f"graph {line['c2_10M_total']['graph_ms_per_step']:.1f} ms", file=sys.stderr, flush=True)
         prog2.close()
         eng2.close()
+    if not args.no_c3:
+        line["c3_uneven_tiles"] = c3_uneven_tiles(V, args)
+        c3 = line["c3_uneven_tiles"]
+        print(f"[bench] c3: plain {c3['plain_ms_per_step']:.1f} ms, batched {c3['batched_ms_per_step']:.1f} ms, "
+              f"graph {c3['graph_ms_per_step']:.1f} ms", file=sys.stderr, flush=True)
+    if args.c4_boxes:
+        line["c4_full_chip"] = c4_full_chip(V, args)
+        print(f"[bench] c4: {line['c4_full_chip']['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
     if args.serving_boxes:
         line["serving"] = serving(V, args)
         print(f"[bench] serving: p50 {line['serving']['p50_ms']:.2f} ms, p99 {line['serving']['p99_ms']:.2f} ms",

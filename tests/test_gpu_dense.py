@@ -133,3 +133,40 @@ def test_dense_fused_group_declines_to_plain(gpu_engine):
     assert len(and1) == 0 and len(ring) == 0
     prog.close()
     eng.close()
+
+
+def long_strips_layout(seed=77, nlong=2_000, nsmall=20_000, span_x=1_000_000, span_y=8_000):
+    """Long horizontal strips over a field of small boxes: the grid sized by the
+    box count is fine in x, so every long strip spans hundreds of tiles (the
+    binning coarsens the grid) and the coarse tiles stack more strips over one
+    x-position than a split can separate (they run on the dense kernel)."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(nlong)
+    y = (i * 4) % span_y
+    x1 = rng.integers(0, span_x // 8, nlong)
+    longs = np.stack([x1, y, x1 + rng.integers(span_x // 2, span_x, nlong), y + 2], 1)
+    small = sparse(rng, nsmall, span_x, 40)
+    small[:, 1] %= span_y
+    small[:, 3] = small[:, 1] + rng.integers(1, 40, nsmall)
+    return np.concatenate([longs, small]).astype(np.int32)
+
+
+def test_long_strips_coarsen_the_grid(gpu_engine):
+    eng = V.Engine(0)
+    a = long_strips_layout()
+    rng = np.random.default_rng(3)
+    b = sparse(rng, 1_500, 1_000_000, 300)
+    b[:, 1] %= 8_000
+    b[:, 3] = b[:, 1] + rng.integers(1, 300, len(b))
+    A, B, out = V.Layer(eng, a), V.Layer(eng, b), V.Layer(eng)
+    V.CalculateBoostOperation(A, A, out, V.K_BOOST_CONTROL_MERGE)
+    check(out, O.canonical(a), "merge")
+    V.CalculateGROWLayer(A, out, (1, 1, 1, 1))
+    check(out, O.grow(a, 1, 1, 1, 1), "grow")
+    V.CalculateBoostOperation(A, A, out, V.K_BOOST_CONTROL_SIZE, -1)
+    check(out, O.size(a, -1), "size-1")
+    for code, fn in ((V.K_BOOST_CONTROL_AND, O.op_and), (V.K_BOOST_CONTROL_NOT, O.op_not)):
+        V.CalculateBoostOperation(A, B, out, code)
+        check(out, fn(a, b), ("binary", code))
+    assert V.klee_area(A) == O.klee_area(a)
+    eng.close()

@@ -8,7 +8,6 @@ import ctypes as C
 
 import numpy as np
 import pytest
-import torch
 
 import oracle_lib as O
 import vlsi_layout_amd as V
@@ -80,12 +79,16 @@ def test_pnum_device_labels_and_small_cases(gpu_engine):
     eng = V.Engine(0)
     a = LAYOUTS["snapped"]
     A = V.Layer(eng, a)
-    d = torch.full((len(a),), -7, dtype=torch.int64, device="cuda:0")
+    # device memory for the labels: the buffer of a layer of n/2 unit boxes
+    # (16 B each), read back as int64
+    buf = V.Layer(eng, np.tile(np.array([[0, 0, 1, 1]], np.int32), ((len(a) + 1) // 2, 1)))
     nc, ar, pe = C.c_int64(), C.c_int64(), C.c_int64()
-    V._check(V.lib().vl_pnum(eng.handle, A.handle, C.c_void_p(d.data_ptr()), C.byref(nc), C.byref(ar), C.byref(pe)))
-    torch.cuda.synchronize()
+    V._check(V.lib().vl_pnum(eng.handle, A.handle, C.c_void_p(buf.device_ptr()), C.byref(nc), C.byref(ar),
+                             C.byref(pe)))
+    eng.sync()
+    got = np.ascontiguousarray(buf.boxes()).view(np.int64).reshape(-1)[: len(a)]
     ol, onc, oarea, operim = O.pnum(a)
-    assert np.array_equal(d.cpu().numpy(), ol) and (nc.value, ar.value, pe.value) == (onc, oarea, operim)
+    assert np.array_equal(got, ol) and (nc.value, ar.value, pe.value) == (onc, oarea, operim)
     for boxes in ([], [[0, 0, 5, 5]], [[0, 0, 5, 5], [5, 5, 9, 9]], [[0, 0, 5, 5], [6, 0, 9, 9]]):
         L = V.Layer(eng, boxes)
         labels, nc, area, perim = V.PerformMerge(L)

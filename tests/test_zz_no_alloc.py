@@ -17,7 +17,8 @@ H2D, D2H = 1, 2
 
 
 def hip():
-    h = C.CDLL("libamdhip64.so")
+    from conftest import hip_runtime
+    h = hip_runtime()
     h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     h.hipMalloc.restype = C.c_int
     h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]

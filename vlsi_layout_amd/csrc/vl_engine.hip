@@ -150,9 +150,9 @@ struct vl_engine {
   int radix_bits = 8;            // digit width of the radix sorts (8 or 11)
   int workers = 4;               // programs: chains run concurrently on up to this many worker streams
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
-  // Strips per binned entry that a workgroup's output region is sized for
-  // (plain tile stage / each output of a fused group).  Raised when a region
-  // overflows, so later operators of this engine do not repeat the retry.
+  // Strips per binned entry that a tile stage's scratch stream is sized for
+  // (plain tile stage / each output of a fused group).  Raised when a stream
+  // is short, so later operators of this engine do not repeat the retry.
   double region_scale = 4.0;
   double fuse_region_scale = 4.0;
   double last_ms = 0.0;
@@ -280,7 +280,7 @@ struct DVec {
 // Host reads of device values.  Each segment is copied to its host address;
 // `check`: in capture mode, a replay compares the device value with the
 // logged one (segments whose value may differ between correct runs, such as
-// the largest region fill, are not checked).  One stream wait per call.
+// a strip stream's demand, are not checked).  One stream wait per call.
 struct RB {
   void* host;
   const void* dev;
@@ -470,6 +470,23 @@ Grid make_grid(Box64 dom, int64_t nboxes, int target, int64_t max_tiles = (int64
   return g;
 }
 
+// Entries a binning may make before its grid is coarsened: boxes that span
+// many tiles (long strips of a complement, say, over a fine grid) would
+// otherwise multiply the entries far beyond the boxes.
+// (operators on ordinary layouts make 1 to 2.5 entries per box)
+uint64_t bin_entry_limit(int64_t n) { return 4 * (uint64_t)n + ((uint64_t)1 << 16); }
+
+// Grid g with tiles twice as wide and tall (none smaller than the domain).
+bool coarsen_grid(Grid& g) {
+  if (g.cols <= 1 && g.rows <= 1) return false;
+  const int64_t W = g.x1 - g.x0, H = g.y1 - g.y0;
+  g.tw = std::min<int64_t>(2 * g.tw, W);
+  g.th = std::min<int64_t>(2 * g.th, H);
+  g.cols = (int32_t)((W + g.tw - 1) / g.tw);
+  g.rows = (int32_t)((H + g.th - 1) / g.th);
+  return true;
+}
+
 // Output streams of one operator.
 struct Outs {
   DVec<vl_box> out, side, next, stitch;
@@ -481,6 +498,10 @@ struct Outs {
   bool and_filter = false;
   explicit Outs(vl_engine* e) : out(e), side(e), next(e), stitch(e), side_tag(e), stitch_tag(e) {}
 };
+
+// Strips per chunk of a tile stage's scratch stream (a workgroup takes a new
+// chunk when its current one cannot hold the next tile's strips).
+constexpr uint64_t STRIP_CHUNK = 1024;
 
 bool two_layer(int top) { return top == TOP_AND || top == TOP_NOT || top == TOP_XOR; }
 
@@ -586,6 +607,23 @@ int split_tiles(vl_engine* e, const TileDesc* d_desc, const uint32_t* d_ovf, uin
   scan_pair_counts(cA, cB, off, ns, off + ns, stmp, e->st);
   uint64_t total = 0;
   VLCK(readback(e, &total, off + ns, sizeof(total)));
+  {
+    // boxes spanning many sub-columns (long strips) would multiply the
+    // entries: those tiles go to the dense path whole instead
+    uint64_t nin = 0;
+    for (const TileDesc& p : parents) nin += (uint64_t)p.nA + p.nB;
+    if (total > bin_entry_limit((int64_t)nin)) {
+      if (!dense)
+        return fail(VL_EINTERNAL, "splitting %u tiles would make %llu entries of %llu", novf,
+                    (unsigned long long)total, (unsigned long long)nin);
+      for (uint32_t i = 0; i < novf; i++) {
+        if (first[i + 1] == first[i]) continue;   // already listed as dense
+        dense->push_back(ovf[i]);
+        dense_par->push_back(parents[i]);
+      }
+      return VL_OK;
+    }
+  }
   vl_box* sub_entries = s.get<vl_box>(total + 1);
   if (!sub_entries) return fail(VL_ENOMEM, "scratch");
   launch_split_scatter(d_par, (int)novf, d_first, d_sub, ns, off, cA, cB, fA, fB, d_entries, sub_entries, halo,
@@ -758,7 +796,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     if (e->nmax == 127) tiers[ntier++] = 127;
   }
   uint32_t* small = s.get<uint32_t>(4);   // overflow count of tier t at [t], of the block kernel at [ntier]
-  unsigned long long* cnt = s.get<unsigned long long>(3);   // [0] strips [1] seam strips [2] max region fill
+  unsigned long long* cnt = s.get<unsigned long long>(3);   // [0] strips [1] seam strips [2] stream bump counter
   uint32_t* ovfl[4];
   for (int t = 0; t <= ntier; t++) ovfl[t] = s.get<uint32_t>(ntiles);
   uint32_t* ovf2 = ovfl[ntier];
@@ -779,8 +817,9 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     launch_fill_u32(flist_n, 0u, 1, e->st);
     launch_and_filter(d_desc, ntiles, d_entries, flist, flist_n, tab_n, ntiles, e->st);
   }
-  // Exact grids: workgroup g of a launch takes tiles g, g+G, ... and owns one
-  // region of the scratch stream (regions of the launches follow each other).
+  // Exact grids: workgroup g of a launch takes tiles g, g+G, ... and takes
+  // chunks of the scratch stream as it needs them (nregions bounds the chunks
+  // left partly used).
   int G[4];
   int nregions = 0;
   for (int t = 0; t < ntier; t++) {
@@ -796,17 +835,16 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   uint64_t total = 0, nsideT = 0;
   uint32_t nov = 0;
   const double t_pt = g_debug ? now_ms() : 0.0;
-  // region size: a few times the mean strips per workgroup of the first tier.
-  // A region that overflows makes the launch run again with bigger regions:
-  // the measured largest fill doubled (the tiles of the later tiers come from
-  // an overflow list whose order depends on timing, so the fills of a rerun
-  // can differ from the first run's).
-  uint64_t region = (uint64_t)(e->region_scale * (double)nentries / G[0]) + 1024;
+  // The scratch stream: the expected strips (region_scale per entry, learnt
+  // from earlier operators) plus one chunk per workgroup (the most a
+  // workgroup leaves unused).  A short stream makes the launch run again with
+  // the demand the stream's bump counter measured, plus a quarter (the
+  // chunks a rerun leaves unused depend on scheduling).
+  uint64_t cap = (uint64_t)(e->region_scale * (double)nentries) + (uint64_t)nregions * STRIP_CHUNK + 1024;
   for (int attempt = 0; attempt < 4; attempt++) {
     if (!area) {
-      const size_t need = (size_t)nregions * region;
-      VLCK(scr.ensure(need, 0));
-      VLCK(scrT.ensure(need, 0));
+      VLCK(scr.ensure(cap, 0));
+      VLCK(scrT.ensure(cap, 0));
     }
     launch_fill_u32(small, 0u, 4, e->st);
     launch_fill_u64((uint64_t*)cnt, 0ull, 3, e->st);
@@ -815,27 +853,24 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     TileOut to;
     to.strips = scr.p;
     to.tag = scrT.p;
-    to.region_cap = area ? 0 : region;
-    to.region_base = 0;
+    to.stream_cap = area ? 0 : cap;
+    to.chunk = STRIP_CHUNK;
+    to.next = cnt + 2;
     to.erode = erode;
     to.cnt = cnt;
-    to.maxuse = cnt + 2;
     to.tab_off = tab_off;
     to.tab_n = tab_n;
     to.ntab = ntiles;
     to.area = d_area;
     to.err = e->d_err;
     hipEvent_t pa = attempt == 0 ? pa_first : prof_mark(e);
-    int rbase = 0;
     for (int t = 0; t <= ntier; t++) {
-      to.region_base = rbase;
       to.list = t ? ovfl[t - 1] : flist;          // tiles above the previous tier's capacity
       to.list_n = t ? small + (t - 1) : flist_n;  // (count known only on the device)
       to.ovf = ovfl[t];
       to.ovf_n = small + t;
       if (t < ntier) launch_tile_wave(tiers[t], two, area, erode > 0, d_desc, ntiles, d_entries, to, G[t], e->st);
       else launch_tile_block(two, area, erode > 0, d_desc, ntiles, d_entries, to, G[t], e->st);
-      rbase += G[t];
     }
     HIPCK(hipGetLastError());
     hipEvent_t pb = prof_mark(e);
@@ -845,7 +880,7 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
       long long area;
     } rb;
     {
-      // the largest region fill (cnt[2]) depends on scheduling: not checked
+      // the stream's demand (cnt[2]) depends on scheduling: not checked
       const RB r[5] = {{rb.cnt, cnt, 16, true},
                        {rb.cnt + 2, cnt + 2, 8, false},
                        {&rb.err, e->d_err, 4, true},
@@ -863,9 +898,9 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
     }
     if (rb.err & ERR_OVERFLOW_CAP) {
       if (attempt == 3) return fail(VL_EINTERNAL, "output capacity still short after resize");
-      region = std::max<uint64_t>(2 * region, 2 * rb.cnt[2] + 64);
-      e->region_scale = std::max(e->region_scale, 1.5 * (double)rb.cnt[2] * G[0] / std::max<double>(1.0, nentries));
-      if (g_debug) fprintf(stderr, "[vl]  region overflow: retry with %llu strips/region\n", (unsigned long long)region);
+      cap = std::max<uint64_t>(2 * cap, rb.cnt[2] + rb.cnt[2] / 4 + (uint64_t)nregions * STRIP_CHUNK + 1024);
+      e->region_scale = std::max(e->region_scale, 1.25 * (double)rb.cnt[2] / std::max<double>(1.0, nentries));
+      if (g_debug) fprintf(stderr, "[vl]  strip stream short: retry with %llu strips\n", (unsigned long long)cap);
       continue;
     }
     break;
@@ -899,28 +934,42 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   return VL_OK;
 }
 
+constexpr int BIN_TOO_MANY = 1003;   // bin_entries without coarsening: the grid would need it
+
 // Bins up to two inputs into grid g: entries of tile t (layer A, then B, each
 // in input order) at entries[start[2t] .. start[2t+2]).  Layer A boxes are
 // binned by their extent grown by xr and stored grown by xs.  The arrays are
-// allocated from `keep` and live as long as it does.
-int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xform xr, const vl_box* b, int64_t nb,
-                Scratch& keep, vl_box** entries_out, uint64_t** start_out, uint64_t* total_out) {
-  const int64_t nt = (int64_t)g.cols * g.rows;
+// allocated from `keep` and live as long as it does.  When the boxes would
+// make more than bin_entry_limit entries, g is coarsened (coarsen true: the
+// caller's grid changes) or BIN_TOO_MANY is returned.
+int bin_entries(vl_engine* e, Grid& g, const vl_box* a, int64_t na, Xform xs, Xform xr, const vl_box* b, int64_t nb,
+                Scratch& keep, vl_box** entries_out, uint64_t** start_out, uint64_t* total_out,
+                bool coarsen = true) {
+  const int64_t nt0 = (int64_t)g.cols * g.rows;
   const int64_t n = na + nb;
   if (e->bin_mode >= 1) {
     // atomic binning: count per (tile, layer) key, scan, scatter (bin_mode 2:
     // the atomics grouped by distinct key over the wave)
-    uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt + 1);
+    uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt0 + 1);
     Scratch s(e);
-    uint32_t* cnt = s.get<uint32_t>(2 * (size_t)nt);
-    void* stmp = s.get<char>(scan_tmp_bytes(2 * (size_t)nt));
-    if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
+    uint32_t* cnt = s.get<uint32_t>(2 * (size_t)nt0);
+    void* stmp = s.get<char>(scan_tmp_bytes(2 * (size_t)nt0));
+    if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt0);
     hipEvent_t pa = prof_mark(e);
-    launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
-    launch_bin_count(a, na, xr, b, nb, g, cnt, e->bin_mode == 2, e->st);
-    scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
     uint64_t total = 0;
-    VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
+    int64_t nt = nt0;
+    for (;;) {   // coarser grids have fewer tiles: the arrays of the first fit
+      nt = (int64_t)g.cols * g.rows;
+      launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
+      launch_bin_count(a, na, xr, b, nb, g, cnt, e->bin_mode == 2, e->st);
+      scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
+      VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
+      if (total <= bin_entry_limit(n)) break;
+      if (!coarsen) return BIN_TOO_MANY;
+      if (g_debug) fprintf(stderr, "[vl]  %llu entries for %lld boxes: coarser grid\n", (unsigned long long)total,
+                           (long long)n);
+      if (!coarsen_grid(g)) break;
+    }
     vl_box* entries = keep.get<vl_box>(total + 1);
     if (!entries) return fail(VL_ENOMEM, "binning scratch (%llu entries)", (unsigned long long)total);
     launch_bin_scatter(a, na, xr, xs, b, nb, g, start, cnt, entries, e->bin_mode == 2, e->st);
@@ -938,12 +987,18 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
   uint32_t* cnt = s.get<uint32_t>((size_t)n + 1);
   uint64_t* pos = s.get<uint64_t>((size_t)n + 1);
   void* stmp = s.get<char>(scan_tmp_bytes((size_t)n + 1));
-  uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt + 1);
-  if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
-  launch_bin_span(a, na, xr, b, nb, g, cnt, e->st);
-  scan_u32_to_u64(cnt, pos, (size_t)n, pos + n, stmp, e->st);
+  uint64_t* start = keep.get<uint64_t>(2 * (size_t)nt0 + 1);
+  if (s.oom || keep.oom) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt0);
   uint64_t total = 0;
-  VLCK(readback(e, &total, pos + n, sizeof(total)));
+  for (;;) {
+    launch_bin_span(a, na, xr, b, nb, g, cnt, e->st);
+    scan_u32_to_u64(cnt, pos, (size_t)n, pos + n, stmp, e->st);
+    VLCK(readback(e, &total, pos + n, sizeof(total)));
+    if (total <= bin_entry_limit(n)) break;
+    if (!coarsen) return BIN_TOO_MANY;
+    if (!coarsen_grid(g)) break;
+  }
+  const int64_t nt = (int64_t)g.cols * g.rows;
   Scratch s2(e);
   uint32_t* keys = s2.get<uint32_t>(total + 1);
   uint32_t* vals = s2.get<uint32_t>(total + 1);
@@ -984,14 +1039,15 @@ int bin_entries(vl_engine* e, Grid g, const vl_box* a, int64_t na, Xform xs, Xfo
 // Bins up to two inputs into grid g and processes the tiles.  Layer A boxes
 // are binned by their extent grown by xr and stored grown by xs; erode > 0
 // selects the erosion tile kernel (xr is then the erosion halo, xs zero).
-int bin_and_process(vl_engine* e, Grid g, int top, const vl_box* a, int64_t na, Xform xs, Xform xr,
+// g may come back coarser (bin_entries): callers go on with it.
+int bin_and_process(vl_engine* e, Grid& g, int top, const vl_box* a, int64_t na, Xform xs, Xform xr,
                     const vl_box* b, int64_t nb, bool area, int32_t erode, Outs& o) {
-  const int64_t nt = (int64_t)g.cols * g.rows;
   Scratch keep(e);
   vl_box* entries = nullptr;
   uint64_t* start = nullptr;
   uint64_t total = 0;
   VLCK(bin_entries(e, g, a, na, xs, xr, b, nb, keep, &entries, &start, &total));
+  const int64_t nt = (int64_t)g.cols * g.rows;
   TileDesc* desc = keep.get<TileDesc>(nt);
   if (!desc) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
   launch_build_desc(g, start, top, desc, e->st);
@@ -1260,10 +1316,11 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     G[t] = std::max(1, std::min(ntiles, mw));
     nregions += G[t];
   }
-  // region size per output: a few strips per entry of a first-tier
+  // stream size per output: the expected strips plus one chunk per
   // workgroup; regrown as in the single-operator tile stage
-  uint64_t region[FUSE_MAX_OUT];
-  for (int k = 0; k < nout; k++) region[k] = (uint64_t)(e->fuse_region_scale * (double)nentries / G[0]) + 1024;
+  uint64_t scap[FUSE_MAX_OUT];
+  for (int k = 0; k < nout; k++)
+    scap[k] = (uint64_t)(e->fuse_region_scale * (double)nentries) + (uint64_t)nregions * STRIP_CHUNK + 1024;
   std::unique_ptr<Scratch> rs;
   vl_box* scr[FUSE_MAX_OUT] = {};
   uint8_t* scrT[FUSE_MAX_OUT] = {};
@@ -1273,9 +1330,8 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
   for (int attempt = 0; attempt < 4; attempt++) {
     rs.reset(new Scratch(e));
     for (int k = 0; k < nout; k++) {
-      const size_t need = (size_t)nregions * region[k];
-      scr[k] = rs->get<vl_box>(need);
-      scrT[k] = rs->get<uint8_t>(need);
+      scr[k] = rs->get<vl_box>(scap[k]);
+      scrT[k] = rs->get<uint8_t>(scap[k]);
     }
     if (rs->oom) return fail(VL_ENOMEM, "fused strip scratch");
     launch_fill_u32(small, 0u, 4, e->st);
@@ -1289,26 +1345,24 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
       fo.tab_off[k] = tab_off + (size_t)k * ntiles;
       fo.tab_n[k] = tab_n + (size_t)k * 2 * ntiles;
       fo.cnt[k] = cnt + 3 * k;
-      fo.region_cap[k] = region[k];
+      fo.stream_cap[k] = scap[k];
     }
+    fo.chunk = STRIP_CHUNK;
     fo.ntab = ntiles;
     fo.err = e->d_err;
     hipEvent_t pa = prof_mark(e);
-    int rbase = 0;
     for (int t = 0; t < NT; t++) {
-      fo.region_base = rbase;
       fo.list = t ? ovfl[t - 1] : nullptr;        // tiles above the previous tier's capacity
       fo.list_n = t ? small + (t - 1) : nullptr;
       fo.ovf = ovfl[t];
       fo.ovf_n = small + t;
       launch_tile_fused(tiers[t], d_desc, ntiles, d_entries, fs, fo, G[t], e->st);
-      rbase += G[t];
     }
     HIPCK(hipGetLastError());
     hipEvent_t pb = prof_mark(e);
     uint32_t err = 0;
     {
-      // per output: strips and seam strips checked, the largest region fill not
+      // per output: strips and seam strips checked, the stream's demand not
       RB r[2 * FUSE_MAX_OUT + 2];
       int nr = 0;
       for (int k = 0; k < FUSE_MAX_OUT; k++) {
@@ -1327,12 +1381,13 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     if (err & ERR_OVERFLOW_CAP) {
       if (attempt == 3) return fail(VL_EINTERNAL, "fused output capacity still short after resize");
       for (int k = 0; k < nout; k++) {
-        if (hc[3 * k + 2] <= region[k]) continue;
-        region[k] = std::max<uint64_t>(2 * region[k], 2 * hc[3 * k + 2] + 64);
-        e->fuse_region_scale = std::max(e->fuse_region_scale,
-                                        1.5 * (double)hc[3 * k + 2] * G[0] / std::max<double>(1.0, nentries));
+        const uint64_t demand = hc[3 * k + 2];
+        if (demand <= scap[k]) continue;
+        scap[k] = std::max<uint64_t>(2 * scap[k], demand + demand / 4 + (uint64_t)nregions * STRIP_CHUNK + 1024);
+        e->fuse_region_scale =
+            std::max(e->fuse_region_scale, 1.25 * (double)demand / std::max<double>(1.0, nentries));
       }
-      if (g_debug) fprintf(stderr, "[vl]  fused region overflow: retry\n");
+      if (g_debug) fprintf(stderr, "[vl]  fused strip stream short: retry\n");
       continue;
     }
     break;
@@ -1437,8 +1492,13 @@ int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer*
     uint64_t* start = nullptr;
     uint64_t total = 0;
     const int32_t R = fs.halo;
-    VLCK(bin_entries(e, g, S->d, S->n, Xform{0, 0, 0, 0}, Xform{R, R, R, R}, nullptr, 0, keep, &entries, &start,
-                     &total));
+    // a fused grid keeps its tile windows (the kernel's coordinate range
+    // depends on them): a layer whose boxes span too many tiles is declined
+    Grid gb = g;
+    const int brc = bin_entries(e, gb, S->d, S->n, Xform{0, 0, 0, 0}, Xform{R, R, R, R}, nullptr, 0, keep, &entries,
+                                &start, &total, false);
+    if (brc == BIN_TOO_MANY) return FUSE_DECLINED;
+    VLCK(brc);
     TileDesc* desc = keep.get<TileDesc>(nt);
     if (!desc) return fail(VL_ENOMEM, "scratch for %lld tiles", (long long)nt);
     launch_build_desc(g, start, TOP_UNION, desc, e->st);

@@ -29,10 +29,21 @@ namespace vl {
 
 namespace {
 
-constexpr int FUSE_OVF_BUF = 32;   // overflow tiles buffered in LDS per workgroup
+constexpr int FUSE_OVF_BUF = 16;   // overflow tiles buffered in LDS per workgroup
 // resident-workgroup cap of the fused grid: 24 one-wave workgroups per CU
 // (6 per SIMD) measured best on the 64-slot tier, ahead of 20, 28 and 32
 constexpr int FUSED_WAVES_PER_CU = 24;
+
+// arr[q] for a wave-uniform q, by selects over the kernel argument's array
+// (no copy in LDS)
+template <typename T>
+__device__ __forceinline__ T pick_out(const T (&arr)[FUSE_MAX_OUT], int q) {
+  T v = arr[0];
+#pragma unroll
+  for (int k = 1; k < FUSE_MAX_OUT; k++)
+    if (k == q) v = arr[k];
+  return v;
+}
 
 // Value of truth table tt (bit s | m0 << 1 | m1 << 2) on base words s, m0, m1
 // (bit 0, the value outside every mask, is taken as 0).  Shannon expansion on
@@ -96,7 +107,12 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   __shared__ uint8_t* s_tag[FUSE_MAX_OUT];
   __shared__ uint64_t* s_toff[FUSE_MAX_OUT];
   __shared__ uint32_t* s_tn[FUSE_MAX_OUT];
-  __shared__ uint64_t s_cap[FUSE_MAX_OUT], s_used[FUSE_MAX_OUT], s_side[FUSE_MAX_OUT];
+  __shared__ uint64_t s_used[FUSE_MAX_OUT], s_side[FUSE_MAX_OUT];
+  // each output's current chunk of its stream: next free strip, chunk end
+  // (the LDS footprint sets how many workgroups a CU holds: 12.5 KB on the
+  // 128-slot tier is 13 per CU)
+  __shared__ uint64_t s_cpos[FUSE_MAX_OUT], s_cend[FUSE_MAX_OUT];
+  __shared__ uint32_t s_short;
   __shared__ uint32_t s_tt[FUSE_MAX_OUT];
   static_assert(TILE_BATCH * 10 <= WAVE, "one descriptor dword per lane");
 
@@ -114,11 +130,12 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
       s_tag[k] = o.tag[k];
       s_toff[k] = o.tab_off[k];
       s_tn[k] = o.tab_n[k];
-      s_cap[k] = o.region_cap[k];
       s_used[k] = 0;
       s_side[k] = 0;
+      s_cpos[k] = s_cend[k] = 0;
       s_tt[k] = fs.tt[k];
     }
+    s_short = 0u;
   }
   uint32_t novf = 0;   // entries in sovf (uniform)
   // Per-slot constants of the coordinate step (slot e = lane*K + k holds
@@ -521,10 +538,19 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           }
         }
         const uint32_t T = wave_sum(myT);
-        const uint64_t cap = uniform(s_cap[q]);
+        // the tile's range: from this workgroup's chunk of the output's stream,
+        // or a new chunk of max(chunk, T) strips
+        uint64_t cpos = uniform(s_cpos[q]), cend = uniform(s_cend[q]);
+        if (cpos + T > cend) {
+          const uint64_t want = (uint64_t)T > o.chunk ? (uint64_t)T : o.chunk;
+          unsigned long long nb = 0;
+          if (lane == 0) nb = atomicAdd(pick_out(o.cnt, q) + 2, (unsigned long long)want);
+          cpos = read_lane((uint64_t)nb, 0);
+          cend = cpos + want;
+        }
         const uint64_t used = uniform(s_used[q]);
-        const uint64_t base = (uint64_t)(o.region_base + blockIdx.x) * cap + used;
-        const bool ok = used + T <= cap;
+        const uint64_t base = cpos;
+        const bool ok = base + T <= pick_out(o.stream_cap, q);
         vl_box* const strips = uniform(s_strips[q]);
         uint8_t* const tags = uniform(s_tag[q]);
         if (lane == 0) gstore(s_toff[q] + ti, base);
@@ -601,6 +627,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
           gstore(tn + o.ntab + ti, nside);
           s_used[q] = used + T;
           s_side[q] += nside;
+          s_cpos[q] = cpos + T;
+          s_cend[q] = cend;
+          if (!ok) s_short = 1u;
         }
         __syncthreads();
       }
@@ -622,10 +651,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         if (k == lane) cnt = o.cnt[k];
       atomicAdd(&cnt[0], (unsigned long long)used);
       if (s_side[lane]) atomicAdd(&cnt[1], (unsigned long long)s_side[lane]);
-      atomicMax(&cnt[2], (unsigned long long)used);
-      if (used > s_cap[lane]) atomicOr(o.err, ERR_OVERFLOW_CAP);
     }
   }
+  if (lane == 0 && s_short) atomicOr(o.err, ERR_OVERFLOW_CAP);
 }
 
 template <int P>

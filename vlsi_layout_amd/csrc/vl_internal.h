@@ -179,21 +179,24 @@ void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc
 // go to side[] with a tag byte; the rest go to out[].  Tiles with more boxes
 // than the kernel's capacity are listed in ovf[] (count in *ovf_n).
 // Workgroup g of a launch takes tiles g, g+G, g+2G, ... (no shared ticket
-// counter) and writes all strips of its tiles into its own region
-// [ (region_base+g)*region_cap, ... ) of the scratch stream, so no output
-// atomics are needed per tile.  Each tile's range is recorded in per-tile
-// tables; each strip carries a tag byte (0 = final, SEAM_* bits = seam piece).
-// A reorder pass then copies the strips in tile order into the final and side
-// streams, so results do not depend on scheduling.  A region that would
-// overflow sets ERR_OVERFLOW_CAP; maxuse tells the host the size to retry with.
+// counter).  A tile's strips are one range of the scratch stream, taken from
+// its workgroup's current chunk; a workgroup whose chunk cannot hold the next
+// tile takes a new chunk of max(chunk, strips) from the stream's bump counter
+// *next (one atomic per chunk, not per tile), so uneven tiles cost at most
+// one partly used chunk per workgroup.  Each tile's range is recorded in
+// per-tile tables; each strip carries a tag byte (0 = final, SEAM_* bits =
+// seam piece).  A reorder pass then copies the strips in tile order into the
+// final and side streams, so results do not depend on scheduling.  Ranges
+// past stream_cap are not written and set ERR_OVERFLOW_CAP; *next then holds
+// the size to retry with.
 struct TileOut {
   vl_box* strips;              // scratch stream
   uint8_t* tag;                // tag per strip
-  uint64_t region_cap;         // strips per workgroup region
-  int region_base;             // first region index of this launch
+  uint64_t stream_cap;         // strips the scratch stream holds
+  uint64_t chunk;              // strips per chunk
+  unsigned long long* next;    // bump counter of the stream (zeroed by the host)
   int32_t erode;               // erosion kernels: erosion distance (tile windows grown by it)
   unsigned long long* cnt;     // [0] strips written, [1] seam strips (one add per workgroup)
-  unsigned long long* maxuse;  // largest region fill (atomicMax, one per workgroup)
   uint64_t* tab_off;           // per tile: range start
   uint32_t* tab_n;             // per tile: [t] final count; tab_n[ntab + t] side count
   int ntab;                    // table stride (number of descriptors)
@@ -205,7 +208,7 @@ struct TileOut {
   uint32_t* err;
 };
 // Wave-per-tile kernel (capacity nmax = 31, 63 or 127 boxes; erosion: 15 or
-// 31), launched with exactly wg workgroups (wg regions of the scratch stream).
+// 31), launched with exactly wg workgroups.
 // With o.list set it takes the tiles of that list.  Erosion mode
 // computes the erosion of layer A by the square of half-side o.erode inside
 // each tile from the boxes meeting the tile grown by o.erode.
@@ -236,15 +239,15 @@ struct FusedSpec {
   uint32_t tt[FUSE_MAX_OUT];      // truth table per output (bit 0 must be 0)
 };
 // Per-output scratch streams and tables (same meaning as TileOut's, one set
-// per output; each workgroup owns region region_base + g of every output).
+// per output; a workgroup takes chunks of every output's stream).
 struct FusedOut {
   vl_box* strips[FUSE_MAX_OUT];
   uint8_t* tag[FUSE_MAX_OUT];
   uint64_t* tab_off[FUSE_MAX_OUT];
   uint32_t* tab_n[FUSE_MAX_OUT];           // [t] final, [ntab + t] seam count
-  unsigned long long* cnt[FUSE_MAX_OUT];   // [0] strips, [1] seam strips, [2] largest region fill
-  uint64_t region_cap[FUSE_MAX_OUT];
-  int region_base;
+  unsigned long long* cnt[FUSE_MAX_OUT];   // [0] strips, [1] seam strips, [2] the stream's bump counter
+  uint64_t stream_cap[FUSE_MAX_OUT];
+  uint64_t chunk;
   int ntab;
   const uint32_t* list;                    // optional tile list (as TileOut)
   const uint32_t* list_n;

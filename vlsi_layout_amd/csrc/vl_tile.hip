@@ -31,8 +31,8 @@
 // row's end boundaries, and every S mark is paired with the first E mark to
 // its right in its row.  Tiles above its capacity run on the 256-thread
 // kernel (capacity 255); larger tiles are split into x-sub-columns by the
-// host.  Each workgroup writes its strips into its own region; a reorder
-// pass restores tile order.
+// host.  Each workgroup writes its strips into chunks of a scratch stream; a
+// reorder pass restores tile order.
 #include <utility>
 
 #include "vl_tile_dev.h"
@@ -543,8 +543,9 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
 
   const int lane = threadIdx.x;
   const int nt = o.list_n ? (int)*o.list_n : ntiles;
-  const uint64_t rbase = (uint64_t)(o.region_base + blockIdx.x) * o.region_cap;
+  uint64_t cbase = 0, cused = 0, ccap = 0;   // this workgroup's current chunk (uniform)
   uint64_t used = 0, side_total = 0;
+  bool short_cap = false;   // a range fell past the stream's capacity (uniform)
   uint32_t novf = 0;        // entries in sovf (uniform)
   long long area_acc = 0;   // area mode: this workgroup's share (uniform)
   // Workgroup g takes batches of TILE_BATCH consecutive tiles (list entries)
@@ -874,8 +875,18 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
       }
       continue;
     }
-    const uint64_t base = rbase + used;
-    const bool ok = used + T <= o.region_cap;
+    if (cused + T > ccap) {   // a new chunk
+      const uint64_t want = T > o.chunk ? (uint64_t)T : o.chunk;
+      unsigned long long nb = 0;
+      if (lane == 0) nb = atomicAdd(o.next, (unsigned long long)want);
+      cbase = read_lane((uint64_t)nb, 0);
+      cused = 0;
+      ccap = want;
+    }
+    const uint64_t base = cbase + cused;
+    const bool ok = base + T <= o.stream_cap;
+    short_cap = short_cap || !ok;
+    cused += T;
     used += T;
     if (lane == 0) o.tab_off[ti] = base;
     // 4. strips.  Lane l owns boundary j = 64*kb + l and walks its S marks:
@@ -951,8 +962,7 @@ __global__ __launch_bounds__(64) void k_tile_w(const TileDesc* __restrict__ desc
   if (lane == 0 && !AREA && used) {
     atomicAdd(&o.cnt[0], (unsigned long long)used);
     if (side_total) atomicAdd(&o.cnt[1], (unsigned long long)side_total);
-    atomicMax(o.maxuse, (unsigned long long)used);
-    if (used > o.region_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
+    if (short_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
   }
 }
 
@@ -979,8 +989,10 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
   __shared__ uint32_t sovf[OVF_BUF];
   const int tid = threadIdx.x;
   const int nt = o.list_n ? (int)*o.list_n : ntiles;
-  const uint64_t rbase = (uint64_t)(o.region_base + blockIdx.x) * o.region_cap;
+  __shared__ unsigned long long s_chunk;
+  uint64_t cbase = 0, cused = 0, ccap = 0;   // this workgroup's current chunk (uniform)
   uint64_t used = 0, side_total = 0;   // uniform across the workgroup
+  bool short_cap = false;
   uint32_t novf = 0;
   long long area_acc = 0;
   for (int tk = blockIdx.x; tk < nt; tk += gridDim.x) {
@@ -1180,10 +1192,22 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
       continue;
     }
 
-    // one range per tile in the workgroup's region: final strips first, then seam strips
-    const uint64_t b0 = rbase + used;
-    const bool ok = used + F + S <= o.region_cap;
-    used += F + S;
+    // one range per tile, from the workgroup's chunk: final strips first, then seam strips
+    const uint64_t T = F + S;
+    if (T > 0 && cused + T > ccap) {   // a new chunk
+      const uint64_t want = T > o.chunk ? T : o.chunk;
+      if (tid == 0) s_chunk = atomicAdd(o.next, (unsigned long long)want);
+      __syncthreads();
+      cbase = s_chunk;
+      cused = 0;
+      ccap = want;
+      __syncthreads();
+    }
+    const uint64_t b0 = cbase + cused;
+    const bool ok = b0 + T <= o.stream_cap;
+    short_cap = short_cap || !ok;
+    cused += T;
+    used += T;
     side_total += S;
     if (tid == 0) {
       gstore(o.tab_off + ti, b0);
@@ -1236,8 +1260,7 @@ __global__ __launch_bounds__(NT) void k_tile_b(const TileDesc* __restrict__ desc
   if (tid == 0 && !AREA && used) {
     atomicAdd(&o.cnt[0], (unsigned long long)used);
     if (side_total) atomicAdd(&o.cnt[1], (unsigned long long)side_total);
-    atomicMax(o.maxuse, (unsigned long long)used);
-    if (used > o.region_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
+    if (short_cap) atomicOr(o.err, ERR_OVERFLOW_CAP);
   }
 }
 
