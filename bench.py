@@ -422,8 +422,12 @@ def main():
     d = hs[dom]
     achieved = d["gbs"]
     per_launch = d["bytes_per_step"] / max(d["launches_per_step"], 1e-9)
+    # HBM traffic per launch of the same stage on this workload, from the
+    # committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counters cannot be
+    # read inside the timed run)
     traffic = None
-    if npl == 10_000_000 and not args.no_autotune:
+    traffic_src = f"profiles/{dom}_traffic_stdcell_10M_per_layer.md"
+    if npl == 10_000_000:
         traffic = profile_figure(f"{dom}_traffic_stdcell_10M_per_layer.md", "traffic_bytes_per_launch")
     line = {
         "metric": METRIC,
@@ -450,6 +454,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": STAGES[dom], "stage": dom, "form": rform,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes_per_launch": per_launch, "traffic": traffic,
+                     "traffic_source": traffic_src if traffic is not None else None,
                      "compulsory_io_frac": result[head]["compulsory_io_gbs"] / HBM_PEAK_GBS},
         "generation_s": t_gen,
     }
