@@ -176,6 +176,60 @@ int vl_pnum(vl_engine* e, const vl_layer* a, int64_t* labels, int64_t* ncomp, in
  * (ComputeInteractionsGPU, src/interaction.f90:204-310); CertifyHealing
  * (src/design_impl.f90:35-56) holds when it is 0. */
 int vl_interaction_count(vl_engine* e, const vl_layer* a, int64_t* count);
+
+/* ---- layer files ------------------------------------------------------------ */
+/* KLBIN: a raw array of boxes (x1, y1, x2, y2 as int32, 16 bytes each).
+ * LoadKLBin (src/kldata.f90:14-63): a missing or empty file is VL_EINVAL,
+ * bytes past the last whole box are not read, every box must be valid. */
+int vl_layer_load_klbin(vl_engine* e, vl_layer* l, const char* path);
+/* WriteKLBin (src/kldata.f90:64-97): replaces the file; an empty layer writes
+ * an empty file. */
+int vl_layer_save_klbin(vl_engine* e, const vl_layer* l, const char* path);
+/* SNAP container (SaveLayerToSnap + CompressBoxesToStream +
+ * SaveCompressedStreamToDisk, src/diskdesign_impl.f90:59-81,
+ * src/box_codec.f90:670-764, src/serialization.f90:15-56): the layer's boxes
+ * in stored order, 65536 per chunk, codec-encoded and compressed with method
+ * 1 (Snappy), 2 (zlib) or 3 (zstd, level 20; needs libzstd.so.1 at run
+ * time).  layer_properties are the LAYER_STATE_* bits written to the header;
+ * the R-tree bit (8) is cleared and the PNUM bit (4) is set exactly when pnum
+ * (n int64 polygon numbers, host memory) is given, which then follow the
+ * header.  The reference sorts a layer for its R-tree before saving; the file
+ * holds the same boxes in this layer's order. */
+int vl_layer_save_snap(vl_engine* e, const vl_layer* l, const char* path, int32_t method, int32_t layer_properties,
+                       const int64_t* pnum);
+/* The same file from n host boxes (host only: no GPU is used; every box must
+ * be valid). */
+int vl_snap_write_boxes(const vl_box* boxes, int64_t n, const char* path, int32_t method, int32_t layer_properties,
+                        const int64_t* pnum);
+/* Reads a SNAP header: properties, box and chunk counts, compression method
+ * (an empty file reads as all zeros). */
+int vl_snap_info(const char* path, int32_t* layer_properties, int64_t* total_boxes, int64_t* chunks,
+                 int32_t* method);
+/* RestoreCompressedStreamFromDisk + DecompressStreamToBoxes
+ * (src/serialization.f90:58-130, src/box_codec.f90:769-891): chunks are
+ * decompressed on the host and the box codec is decoded on the GPU into the
+ * layer.  An empty file is an empty layer.  *layer_properties (may be null)
+ * receives the header's LAYER_STATE_* bits masked to 63 (RestoreSnapToLayer).
+ * When the file holds polygon numbers and pnum is not null they are copied
+ * to pnum (VL_ENOSPC when pnum_cap < boxes).  Malformed files (header, chunk
+ * sizes, decompression, a codec record past its chunk, a dictionary index
+ * out of range, a box count that does not match) are VL_EINVAL; decoded
+ * boxes must be valid (VL_EINVAL). */
+int vl_layer_load_snap(vl_engine* e, vl_layer* l, const char* path, int32_t* layer_properties, int64_t* pnum,
+                       int64_t pnum_cap);
+/* RestoreSnapToLayer's dispatch (src/diskdesign_impl.f90:83-114): a path
+ * containing ".bin" is KLBIN, anything else SNAP. */
+int vl_layer_load_file(vl_engine* e, vl_layer* l, const char* path, int32_t* layer_properties);
+/* The box codec alone (CompressBoxesUsingCodec, src/box_codec.f90:565-666):
+ * encodes n <= 65536 boxes as one chunk into out (VL_ENOSPC with *out_n set
+ * when cap is too small).  Host only. */
+int vl_codec_encode(const vl_box* boxes, int64_t n, uint8_t* out, int64_t cap, int64_t* out_n);
+/* GPU decode of uncompressed codec chunks (DecompressBoxesUsingCodec,
+ * src/box_codec.f90:447-536) into `out`: raw holds the chunks back to back
+ * (host memory), chunk_bytes[k] / chunk_boxes[k] their sizes.  `out` must
+ * belong to e. */
+int vl_codec_decode(vl_engine* e, const uint8_t* raw, const int64_t* chunk_bytes, const int64_t* chunk_boxes,
+                    int64_t nchunks, vl_layer* out);
 /* Box count of A (the ASSERT_ZERO check, src/controlparser.f90:524-530).
  * Returns VL_EASSERT when *count>0 and abort_on_nonzero!=0. */
 int vl_assert_zero(vl_engine* e, const vl_layer* a, int abort_on_nonzero, int64_t* count);

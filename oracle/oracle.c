@@ -35,6 +35,10 @@
  *                           rule; known answers for edge, corner and no contact;
  *                           areas by brute force over all pairs.
  *   or_interactions         brute force over all pairs.
+ *   or_codec_encode/decode  known answers derived by hand from the format
+ *                           (every record form: new value, index, current,
+ *                           square, zero and negative deltas); decode of
+ *                           encode is the identity, also across int32 wraps.
  *   or_grid                 a hand-evaluated case of CreateGrid's arithmetic; area and
  *                           tiling invariants.
  *   sorts                   library equivalence (Python sorted() with the same keys).
@@ -655,6 +659,146 @@ int64_t or_interactions(const ob_box* a, int64_t n) {
   }
   free(id);
   return c;
+}
+
+/* The SNAP box codec, one chunk (box_codec.f90:447-666).  Per box an info   */
+/* byte: bit 4 square (h = w), bit 5/6 x1/y1 moved (zigzag varint of the     */
+/* int32 delta from the previous box), bit 0 width not the current           */
+/* dictionary entry (then bit 1: new value as a varint, appended to the      */
+/* dictionary; else its 1-based int16 index), bits 2/3 the same for the      */
+/* height (not present for squares).  Dictionaries, cursor and current       */
+/* indices start empty / 0 per chunk; varints are 7-bit little-endian groups */
+/* with a continuation bit (box_codec.f90:342-442).  Arithmetic on x, y, w   */
+/* and h is int32 with wrap-around, as the reference's int32 code does.      */
+static int64_t zz_enc(int32_t v) { int64_t t = v; return t < 0 ? (-t) * 2 - 1 : t * 2; }
+static int32_t zz_dec(uint64_t v) { return (int32_t)(uint32_t)((v >> 1) ^ (0 - (v & 1))); }
+static int64_t put_varint(uint8_t* o, int64_t p, uint64_t v) {
+  for (;;) {
+    const uint8_t b = (uint8_t)(v & 127u);
+    v >>= 7;
+    if (!v) { o[p++] = b; return p; }
+    o[p++] = (uint8_t)(b | 128u);
+  }
+}
+static int64_t get_varint(const uint8_t* in, int64_t p, int64_t len, uint64_t* v) {
+  uint64_t r = 0;
+  int sh = 0;
+  for (;;) {
+    if (p >= len || sh > 63) return -1;
+    const uint8_t b = in[p++];
+    r |= (uint64_t)(b & 127u) << sh;
+    if (!(b & 128u)) break;
+    sh += 7;
+  }
+  *v = r;
+  return p;
+}
+static int64_t dict_find(const int32_t* d, int64_t n, int32_t v) {
+  for (int64_t i = 0; i < n; i++)
+    if (d[i] == v) return i + 1;
+  return -1;
+}
+int64_t or_codec_encode(const ob_box* b, int64_t n, uint8_t* out, int64_t cap) {
+  int32_t* uw = (int32_t*)malloc((size_t)(n + 1) * sizeof(int32_t));
+  int32_t* uh = (int32_t*)malloc((size_t)(n + 1) * sizeof(int32_t));
+  int64_t nw = 0, nh = 0, cw = 0, chh = 0, p = 0;
+  int32_t cx = 0, cy = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (p + 48 > cap) { p = -1; break; }
+    const int32_t w = (int32_t)((uint32_t)b[i].x2 - (uint32_t)b[i].x1);
+    const int32_t h = (int32_t)((uint32_t)b[i].y2 - (uint32_t)b[i].y1);
+    uint8_t info = 0;
+    const int sq = w == h;
+    if (sq) info |= 16;
+    const int32_t dx = (int32_t)((uint32_t)b[i].x1 - (uint32_t)cx);
+    if (dx) { info |= 32; cx = b[i].x1; }
+    const int32_t dy = (int32_t)((uint32_t)b[i].y1 - (uint32_t)cy);
+    if (dy) { info |= 64; cy = b[i].y1; }
+    int64_t wi = dict_find(uw, nw, w);
+    if (wi != cw || nw == 0) {
+      info |= 1;
+      if (wi <= 0) { info |= 2; uw[nw++] = w; wi = nw; }
+      cw = wi;
+    }
+    int64_t hi = 0;
+    if (!sq) {
+      hi = dict_find(uh, nh, h);
+      if (hi != chh || nh == 0) {
+        info |= 4;
+        if (hi <= 0) { info |= 8; uh[nh++] = h; hi = nh; }
+        chh = hi;
+      }
+    }
+    out[p++] = info;
+    if (info & 32) p = put_varint(out, p, (uint64_t)zz_enc(dx));
+    if (info & 64) p = put_varint(out, p, (uint64_t)zz_enc(dy));
+    if (info & 1) {
+      if (info & 2) p = put_varint(out, p, (uint64_t)(int64_t)w);
+      else { out[p++] = (uint8_t)(wi & 255); out[p++] = (uint8_t)((wi >> 8) & 255); }
+    }
+    if (!sq && (info & 4)) {
+      if (info & 8) p = put_varint(out, p, (uint64_t)(int64_t)h);
+      else { out[p++] = (uint8_t)(hi & 255); out[p++] = (uint8_t)((hi >> 8) & 255); }
+    }
+  }
+  free(uw); free(uh);
+  return p;
+}
+int or_codec_decode(const uint8_t* in, int64_t len, int64_t n, ob_box* out) {
+  int32_t* uw = (int32_t*)malloc((size_t)(n + 1) * sizeof(int32_t));
+  int32_t* uh = (int32_t*)malloc((size_t)(n + 1) * sizeof(int32_t));
+  int64_t nw = 0, nh = 0, cw = 0, chh = 0, p = 0;
+  int32_t cx = 0, cy = 0;
+  int rc = 0;
+  for (int64_t i = 0; i < n && !rc; i++) {
+    if (p >= len) { rc = 1; break; }
+    const uint8_t info = in[p++];
+    const int sq = (info >> 4) & 1;
+    uint64_t v;
+    if (info & 32) { if ((p = get_varint(in, p, len, &v)) < 0) { rc = 1; break; } cx = (int32_t)((uint32_t)cx + (uint32_t)zz_dec(v)); }
+    if (info & 64) { if ((p = get_varint(in, p, len, &v)) < 0) { rc = 1; break; } cy = (int32_t)((uint32_t)cy + (uint32_t)zz_dec(v)); }
+    int32_t w, h;
+    if (info & 1) {
+      if (info & 2) {
+        if ((p = get_varint(in, p, len, &v)) < 0) { rc = 1; break; }
+        uw[nw++] = (int32_t)(uint32_t)v;
+        cw = nw;
+      } else {
+        if (p + 2 > len) { rc = 1; break; }
+        const int16_t k = (int16_t)(in[p] | (in[p + 1] << 8));
+        p += 2;
+        if (k < 1 || k > nw) { rc = 1; break; }
+        cw = k;
+      }
+    }
+    if (cw < 1) { rc = 1; break; }
+    w = uw[cw - 1];
+    if (sq) {
+      h = w;
+    } else {
+      if (info & 4) {
+        if (info & 8) {
+          if ((p = get_varint(in, p, len, &v)) < 0) { rc = 1; break; }
+          uh[nh++] = (int32_t)(uint32_t)v;
+          chh = nh;
+        } else {
+          if (p + 2 > len) { rc = 1; break; }
+          const int16_t k = (int16_t)(in[p] | (in[p + 1] << 8));
+          p += 2;
+          if (k < 1 || k > nh) { rc = 1; break; }
+          chh = k;
+        }
+      }
+      if (chh < 1) { rc = 1; break; }
+      h = uh[chh - 1];
+    }
+    out[i].x1 = cx; out[i].y1 = cy;
+    out[i].x2 = (int32_t)((uint32_t)cx + (uint32_t)w);
+    out[i].y2 = (int32_t)((uint32_t)cy + (uint32_t)h);
+  }
+  if (!rc && p != len) rc = 2;   /* bytes left over */
+  free(uw); free(uh);
+  return rc;
 }
 
 /* CreateGrid (design_impl.f90:159-275): rows x cols cells of MBR m, base    */

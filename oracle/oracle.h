@@ -62,6 +62,11 @@ int64_t or_self_and(const ob_box* a, int64_t na, ob_box** out);
 int64_t or_pnum(const ob_box* a, int64_t n, int64_t* labels, int64_t* area, int64_t* perim);
 /* Pairs overlapping with positive area (src/interaction.f90:204-310). */
 int64_t or_interactions(const ob_box* a, int64_t n);
+/* SNAP box codec of one chunk (src/box_codec.f90:447-666): encode returns
+ * the bytes written (-1 when cap is short; 48 per box always suffice),
+ * decode 0 on success. */
+int64_t or_codec_encode(const ob_box* b, int64_t n, uint8_t* out, int64_t cap);
+int or_codec_decode(const uint8_t* in, int64_t len, int64_t n, ob_box* out);
 /* CreateGrid (src/design_impl.f90:159-275): rows x cols padded cells of m,
  * row-major; -1 when rows or cols < 1 or m is not a valid box. */
 int64_t or_grid(ob_box m, int32_t rows, int32_t cols, int32_t overlap, ob_box** out);

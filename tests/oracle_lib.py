@@ -61,6 +61,10 @@ def lib():
         L.or_pnum.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.or_interactions.restype = C.c_int64
         L.or_interactions.argtypes = [C.c_void_p, C.c_int64]
+        L.or_codec_encode.restype = C.c_int64
+        L.or_codec_encode.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
+        L.or_codec_decode.restype = C.c_int
+        L.or_codec_decode.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
         L.or_free.argtypes = [C.c_void_p]
         L.or_event_sort.argtypes = [C.c_void_p, C.c_int64]
         L.or_morton_key.restype = C.c_uint64
@@ -171,6 +175,23 @@ def pnum(a):
     ar, pe = C.c_int64(), C.c_int64()
     nc = lib().or_pnum(_ptr(a), len(a), _ptr(labels), C.byref(ar), C.byref(pe))
     return labels[: len(a)], int(nc), ar.value, pe.value
+
+
+def codec_encode(a):
+    """SNAP codec bytes of one chunk of boxes (src/box_codec.f90:565-666)."""
+    a = boxes(a)
+    out = np.zeros(48 * len(a) + 16, np.uint8)
+    n = lib().or_codec_encode(_ptr(a), len(a), _ptr(out), len(out))
+    assert n >= 0
+    return out[:n].tobytes()
+
+
+def codec_decode(data, n):
+    """Boxes of one SNAP codec chunk of n boxes, or None when malformed."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+    out = np.zeros((max(n, 1), 4), np.int32)
+    rc = lib().or_codec_decode(_ptr(buf), len(data), n, _ptr(out))
+    return None if rc else out[:n]
 
 
 def interactions(a):

@@ -333,3 +333,44 @@ def test_pnum_against_reference_build(seed):
     assert nc == int(rl.max(initial=0))
     assert area == rarea and perim == rperim
     assert O.interactions(a) == O.ref_interactions(a)
+
+
+# (0,0,10,10): square, no move, new width 10           -> 13 0a
+# (5,0,8,4): x moved +5 (zigzag 10), new w 3, new h 4  -> 2f 0a 03 04
+# (5,-1,15,9): y moved -1 (zigzag 1), square, w 10 is
+#              entry 1 (not the current entry 2)        -> 51 01 01 00
+# (7,-1,17,5): x moved +2 (zigzag 4), w 10 current,
+#              new h 6                                  -> 2c 04 06
+CODEC_KNOWN = ([[0, 0, 10, 10], [5, 0, 8, 4], [5, -1, 15, 9], [7, -1, 17, 5]],
+               bytes([0x13, 0x0A, 0x2F, 0x0A, 0x03, 0x04, 0x51, 0x01, 0x01, 0x00, 0x2C, 0x04, 0x06]))
+
+
+def test_codec_known_answer():
+    b, data = CODEC_KNOWN
+    assert O.codec_encode(b) == data
+    assert O.codec_decode(data, 4).tolist() == b
+    assert O.codec_decode(data[:-1], 4) is None          # truncated
+    assert O.codec_decode(data + b"\x00", 4) is None    # trailing byte
+    # an index past the dictionary is rejected
+    assert O.codec_decode(bytes([0x13, 0x0A, 0x21, 0x02, 0x05, 0x00]), 2) is None
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_codec_round_trip(seed):
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(1, 5000))
+    if seed == 4:    # extreme coordinates: deltas and widths wrap int32
+        x = rng.integers(-2**31, 2**31 - 1, n)
+        w = rng.integers(1, 2**31 - 1, n)
+        x1 = np.minimum(x, 2**31 - 1 - w)
+        a = np.stack([x1, x1 // 3, x1 + w, x1 // 3 + 1], 1).astype(np.int32)
+    else:            # a few widths/heights (dictionary hits), squares, moves of every sign
+        x = rng.integers(-5000, 5000, n)
+        y = rng.integers(-5000, 5000, n)
+        w = rng.choice([1, 2, 5, 8, 40, 333], n)
+        h = np.where(rng.random(n) < 0.3, w, rng.choice([1, 3, 8, 600], n))
+        a = np.stack([x, y, x + w, y + h], 1).astype(np.int32)
+        a[rng.random(n) < 0.2] = a[0]          # repeats: zero deltas
+    data = O.codec_encode(a)
+    assert np.array_equal(O.codec_decode(data, n), a)
+

@@ -115,6 +115,15 @@ def lib():
             "vl_extent": (C.c_int, [vp, vp, vp]),
             "vl_pnum": (C.c_int, [vp, vp, vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
             "vl_interaction_count": (C.c_int, [vp, vp, C.POINTER(i64)]),
+            "vl_layer_load_klbin": (C.c_int, [vp, vp, C.c_char_p]),
+            "vl_layer_save_klbin": (C.c_int, [vp, vp, C.c_char_p]),
+            "vl_layer_save_snap": (C.c_int, [vp, vp, C.c_char_p, i32, i32, vp]),
+            "vl_snap_write_boxes": (C.c_int, [vp, i64, C.c_char_p, i32, i32, vp]),
+            "vl_snap_info": (C.c_int, [C.c_char_p, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]),
+            "vl_layer_load_snap": (C.c_int, [vp, vp, C.c_char_p, C.POINTER(i32), vp, i64]),
+            "vl_layer_load_file": (C.c_int, [vp, vp, C.c_char_p, C.POINTER(i32)]),
+            "vl_codec_encode": (C.c_int, [vp, i64, vp, i64, C.POINTER(i64)]),
+            "vl_codec_decode": (C.c_int, [vp, vp, vp, vp, i64, vp]),
             "vl_assert_zero": (C.c_int, [vp, vp, C.c_int, C.POINTER(i64)]),
             "vl_klee_area": (C.c_int, [vp, vp, C.POINTER(i64)]),
             "vl_canonicalize": (C.c_int, [vp, vp, vp]),
@@ -395,6 +404,95 @@ def InteractionCount(L):
 def CertifyHealing(L):
     """True when no two boxes of L overlap with positive area (src/design_impl.f90:35-56)."""
     return L.n_used < 2 or InteractionCount(L) == 0
+
+
+COMPRESSION_METHOD_SNAPPY, COMPRESSION_METHOD_ZLIB, COMPRESSION_METHOD_ZSTD = 1, 2, 3
+
+
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+def LoadKLBin(path, L):
+    """Replace L by the boxes of a KLBIN file (src/kldata.f90:14-63)."""
+    _check(lib().vl_layer_load_klbin(L.engine.handle, L.handle, _path(path)))
+    return L
+
+
+def WriteKLBin(path, L):
+    """Write L's boxes as a KLBIN file (src/kldata.f90:64-97)."""
+    _check(lib().vl_layer_save_klbin(L.engine.handle, L.handle, _path(path)))
+
+
+def SaveLayerToSnap(L, path, method=COMPRESSION_METHOD_SNAPPY, layer_properties=0, pnum=None):
+    """Write L as a SNAP file (src/diskdesign_impl.f90:59-81); pnum: optional
+    int64 polygon numbers, one per box, stored after the header."""
+    ptr = None
+    if pnum is not None:
+        pnum = np.ascontiguousarray(pnum, dtype=np.int64)
+        if pnum.shape != (L.n_used,):
+            raise VLError(VL_EINVAL, f"pnum must hold {L.n_used} values")
+        ptr = pnum.ctypes.data_as(C.c_void_p)
+    _check(lib().vl_layer_save_snap(L.engine.handle, L.handle, _path(path), int(method), int(layer_properties),
+                                    ptr))
+
+
+def snap_write_boxes(boxes, path, method=COMPRESSION_METHOD_SNAPPY, layer_properties=0, pnum=None):
+    """SNAP file from host boxes (no GPU), as SaveLayerToSnap writes it."""
+    b = as_boxes(boxes)
+    ptr = None
+    if pnum is not None:
+        pnum = np.ascontiguousarray(pnum, dtype=np.int64)
+        if pnum.shape != (len(b),):
+            raise VLError(VL_EINVAL, f"pnum must hold {len(b)} values")
+        ptr = pnum.ctypes.data_as(C.c_void_p)
+    _check(lib().vl_snap_write_boxes(b.ctypes.data_as(C.c_void_p), len(b), _path(path), int(method),
+                                     int(layer_properties), ptr))
+
+
+def SnapInfo(path):
+    """(layer_properties, total_boxes, chunks, method) of a SNAP file's header."""
+    pr, nb, nc, m = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int32()
+    _check(lib().vl_snap_info(_path(path), C.byref(pr), C.byref(nb), C.byref(nc), C.byref(m)))
+    return pr.value, nb.value, nc.value, m.value
+
+
+def RestoreSnapToLayer(L, path, with_pnum=False):
+    """Replace L by a SNAP or KLBIN file's boxes (src/diskdesign_impl.f90:83-114;
+    a path containing ".bin" is KLBIN).  Returns the layer properties, or
+    (properties, pnum) with with_pnum (pnum None when the file has none)."""
+    pr = C.c_int32()
+    if not with_pnum:
+        _check(lib().vl_layer_load_file(L.engine.handle, L.handle, _path(path), C.byref(pr)))
+        return pr.value
+    props, total, _, _ = SnapInfo(path)
+    pn = np.zeros(max(total, 1), np.int64) if props & 4 else None
+    _check(lib().vl_layer_load_snap(L.engine.handle, L.handle, _path(path), C.byref(pr),
+                                    None if pn is None else pn.ctypes.data_as(C.c_void_p),
+                                    0 if pn is None else len(pn)))
+    return pr.value, (None if pn is None else pn[:total])
+
+
+def codec_encode(boxes):
+    """Box codec bytes of one chunk (<= 65536 boxes; src/box_codec.f90:565-666)."""
+    b = as_boxes(boxes)
+    cap = 41 * len(b) + 16
+    out = np.zeros(cap, np.uint8)
+    n = C.c_int64()
+    _check(lib().vl_codec_encode(b.ctypes.data_as(C.c_void_p), len(b), out.ctypes.data_as(C.c_void_p), cap,
+                                 C.byref(n)))
+    return out[:n.value].tobytes()
+
+
+def codec_decode(chunks, counts, out):
+    """GPU decode of uncompressed codec chunks (bytes objects) holding
+    counts[k] boxes each into layer `out` (src/box_codec.f90:447-536)."""
+    raw = np.frombuffer(b"".join(chunks) or b"\0", np.uint8)
+    cb = np.array([len(c) for c in chunks] or [0], np.int64)
+    nb = np.array(list(counts) or [0], np.int64)
+    _check(lib().vl_codec_decode(out.engine.handle, raw.ctypes.data_as(C.c_void_p), cb.ctypes.data_as(C.c_void_p),
+                                 nb.ctypes.data_as(C.c_void_p), len(chunks), out.handle))
+    return out
 
 
 def MergeHealLayer(L):

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["vl_prims.hip", "vl_sort.hip", "vl_tile.hip", "vl_fused.hip", "vl_stitch.hip", "vl_dense.hip", "vl_window.hip", "vl_pnum.hip", "vl_noalloc.hip", "vl_engine.hip"]
+HIP_SOURCES = ["vl_prims.hip", "vl_sort.hip", "vl_tile.hip", "vl_fused.hip", "vl_stitch.hip", "vl_dense.hip", "vl_window.hip", "vl_pnum.hip", "vl_snap.hip", "vl_io.hip", "vl_noalloc.hip", "vl_engine.hip"]
 CPP_SOURCES = ["vl_synth.cpp"]
 F_SOURCES = ["vl_layer_ops.f90", "vl_rect_boolean.f90", "vl_control.f90"]
 F_MAIN = "vl_magparser.f90"
@@ -64,7 +64,8 @@ def build_hip(verbose=False):
                 print(o)
     objs = [os.path.join(BUILD, s + ".o") for s in HIP_SOURCES + CPP_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    # zlib for SNAP files (zstd is opened at run time when a file needs it)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz", "-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

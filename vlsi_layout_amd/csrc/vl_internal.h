@@ -82,6 +82,33 @@ class EngineLock {
   bool locked_ = false;
 };
 
+// vl_snap.hip: GPU decode of SNAP codec chunks (concatenated raw codec bytes;
+// coff: nchunks+1 byte offsets, roff: nchunks+1 record offsets, boff:
+// nchunks+1 walk-block offsets).  Scratch: len (total_raw), exit_e and cnt
+// (nblocks*41), entry and rbase (nblocks), start/rx/ry/widx/hidx/dictw/dicth
+// (records).  *err (zeroed by the caller) != 0: malformed stream.
+struct SnapDecode {
+  const uint8_t* raw;
+  int64_t total_raw;
+  const int64_t* coff;
+  const int64_t* roff;
+  const int64_t* boff;
+  int nchunks;
+  int64_t nblocks;
+  uint8_t* len;
+  uint8_t* exit_e;
+  uint16_t* cnt;
+  uint8_t* entry;
+  int64_t* rbase;
+  int64_t* start;
+  int32_t *rx, *ry, *dictw, *dicth;
+  uint32_t *widx, *hidx;
+  vl_box* out;
+  uint32_t* err;
+};
+int64_t snap_blocks(int64_t raw_bytes);   // walk blocks of one chunk
+void launch_snap_decode(const SnapDecode& d, hipStream_t st);
+
 // vl_noalloc.hip: the static scratch's device address on `device` (current).
 int noalloc_setup(int device);
 
