@@ -37,7 +37,8 @@ def test_module_exports_reference_names():
                  "calculateboostoperation", "mergeheallayer", "heal_boxes", "sort_event_trackers",
                  "sortboxesdirect", "sortboxesindirect", "calculateframeand", "calculatesinglelayerand",
                  "creategrid", "createextent", "merge_boxes_using_scanline", "preprocesslayer",
-                 "performmergelayer", "certifyhealing", "needssorting", "needspnum", "needsrtree", "needshealing"]:
+                 "performmergelayer", "certifyhealing", "needssorting", "needspnum", "needsrtree", "needshealing",
+                 "restoresnaptolayer", "savelayertosnap", "loadklbin", "writeklbin", "clearlayer"]:
         assert f"_qmvl_layer_opsp{name}" in out, name
     assert "_qmvl_rect_booleanpfracture_rectangle" in out
 

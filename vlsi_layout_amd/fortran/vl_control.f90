@@ -9,15 +9,34 @@
 !             GRID [rows cols overlap]  (the reference's fixed 10 10 10)
 !             GROW e n w s   SIZE d    (lengths in microns, times the
 !             precision and truncated to DBU as in :566-577)
-!   flush                         run the deferred operators (:613-626 writes
-!                                 the output designs there; design files are
-!                                 out of this path)
+!   decl design D = FILE.mag      an input design (:257-271): every layer
+!   exec modedesign D = FILE.mag  section of the .mag file becomes layer D:NAME,
+!                                 from its rect lines or the KLBIN / SNAP /
+!                                 DSNAP file it names (src/magparser.f90:
+!                                 108-290); each layer is then numbered and,
+!                                 when its boxes overlap, healed (:358-418)
+!   decl output O = FILE.mag      an output design (:240-256): its sections
+!                                 name the files flush writes (KLBIN for .bin,
+!                                 SNAP for .snap, src/magparser.f90:510-564)
+!   decl memory M = nothing       a design of intermediate layers (:272-289)
+!   exec init DB:NAME             an empty layer (:310-340)
+!   var temp_folder | temporary_layers | abort_on_assert_zero | abort_on_xor |
+!       debug_verbosity VALUE     (:180-235): a layer created by exec run or
+!                                 exec init in a declared design is backed by
+!                                 temp_folder//DB//'_MTL_'//NAME;
+!                                 temporary_layers 2 spills memory designs to
+!                                 those files at flush and reads a missing
+!                                 layer file as an empty layer
+!   program NAME, systeminfo, info  accepted (messages only)
+!   flush                         run the deferred operators, then write the
+!                                 output designs' non-empty layers to their
+!                                 files and release them (:613-626)
 !   delete NAME                   release a layer, or every layer of a design
 !                                 when NAME ends in ':' (:627-642); a released
-!                                 layer loaded from a file is read again when
-!                                 next used (lazy restore, :440-441)
-! plus the few statements a stand-alone driver needs in place of the
-! reference's design loading (modedesign / .mag, which are out of this path):
+!                                 layer with a backing file is read again when
+!                                 next used (lazy restore, :440-441); delete D
+!                                 of a declared design drops its layers
+! plus the statements a stand-alone driver adds:
 !   precision N                   DBU per micron (default 1000)
 !   layer NAME klbin FILE         raw little-endian int32 boxes (src/kldata.f90)
 !   layer NAME synth CFG SEED K N seeded synthetic layer (include/vl_synth.h)
@@ -37,7 +56,8 @@
 !                                 take, the end of the file); graph: the same,
 !                                 with the program captured as a HIP graph
 ! Layer names are taken whole ("d1:poly"); "nothing" and "<db>:nothing" name
-! no layer.  '#' starts a comment.
+! no layer.  '#' starts a comment.  Unknown statements are reported and
+! skipped, as the reference does (:658-659).
 module vl_control
   use iso_c_binding
   use iso_fortran_env, only: int32, int64, real64, output_unit, error_unit
@@ -49,12 +69,25 @@ module vl_control
   type :: NamedLayer
      character(len=:), allocatable :: name
      type(Layer) :: L
-     character(len=:), allocatable :: file   ! backing KLBIN file (lazy restore)
+     character(len=:), allocatable :: file   ! backing file (lazy restore, flush)
+     logical :: file_is_klbin = .false.       ! read with `layer NAME klbin`
      logical :: released = .false.
+     integer :: db = 0                        ! declared design, 0 for none
   end type NamedLayer
+
+  ! Declared designs (src/design.f90:86-88 directions)
+  integer, parameter :: DIR_INPUT = 0, DIR_OUTPUT = 1, DIR_MEMORY = 2, DIR_DELETED = -1
+  type :: DeclaredDesign
+     character(len=:), allocatable :: name
+     character(len=:), allocatable :: file
+     integer :: direction = DIR_INPUT
+  end type DeclaredDesign
 
   type(NamedLayer), allocatable, target :: table(:)
   integer :: nlayers = 0
+  type(DeclaredDesign), allocatable :: designs(:)
+  integer :: ndesigns = 0
+  character(len=:), allocatable :: temp_folder
   integer(int64) :: used_precision = 1000
   integer :: assert_failures = 0
   integer :: statements_run = 0
@@ -136,7 +169,9 @@ contains
           call move_alloc(table(i)%name, grown(i)%name)
           call move_layer(table(i)%L, grown(i)%L)
           if (allocated(table(i)%file)) call move_alloc(table(i)%file, grown(i)%file)
+          grown(i)%file_is_klbin = table(i)%file_is_klbin
           grown(i)%released = table(i)%released
+          grown(i)%db = table(i)%db
        end do
        call move_alloc(grown, table)
     end if
@@ -150,6 +185,9 @@ contains
     dst%n_used = src%n_used
     dst%n_alloc = src%n_alloc
     if (allocated(src%layer_boxes)) call move_alloc(src%layer_boxes, dst%layer_boxes)
+    if (allocated(src%pnumtable%arr)) call move_alloc(src%pnumtable%arr, dst%pnumtable%arr)
+    dst%area = src%area
+    dst%perimeter = src%perimeter
     dst%layerState = src%layerState
     dst%dev = src%dev
     dst%dev_valid = src%dev_valid
@@ -169,7 +207,11 @@ contains
                 write(error_unit, '(A,A,A,I0)') 'ERROR: layer was deleted: ', trim(name), ' on line ', line_no
                 error stop "ERROR: deleted layer used"
              end if
-             call load_klbin(table(i)%L, table(i)%file)   ! lazy restore
+             if (table(i)%file_is_klbin) then       ! lazy restore
+                call load_klbin(table(i)%L, table(i)%file)
+             else
+                call RestoreSnapToLayer(table(i)%L, table(i)%file)
+             end if
              table(i)%released = .false.
           end if
           return
@@ -327,12 +369,257 @@ contains
     end do
   end subroutine delete_layers
 
+  ! ---- designs ----------------------------------------------------------------
+
+  integer function design_index(name)
+    character(len=*), intent(in) :: name
+    integer :: k
+    design_index = 0
+    do k = 1, ndesigns
+       if (designs(k)%direction == DIR_DELETED) cycle
+       if (designs(k)%name == trim(name)) then
+          design_index = k
+          return
+       end if
+    end do
+  end function design_index
+
+  ! Declared design of a layer name "DB:NAME" (0 when DB is not declared).
+  integer function design_of(name)
+    character(len=*), intent(in) :: name
+    integer :: p
+    design_of = 0
+    p = index(name, ':')
+    if (p > 1) design_of = design_index(name(1:p-1))
+  end function design_of
+
+  logical function layer_exists(name)
+    character(len=*), intent(in) :: name
+    integer :: i
+    layer_exists = .false.
+    do i = 1, nlayers
+       if (table(i)%name == trim(name)) then
+          layer_exists = .true.
+          return
+       end if
+    end do
+  end function layer_exists
+
+  integer function declare_design(name, file, direction)
+    character(len=*), intent(in) :: name, file
+    integer, intent(in) :: direction
+    type(DeclaredDesign), allocatable :: g(:)
+    integer :: k
+    if (design_index(name) > 0) then
+       write(error_unit, '(A,A)') 'ERROR: Duplicate db handle seen: ', trim(name)
+       error stop "ERROR: duplicate design"
+    end if
+    if (.not. allocated(designs)) allocate(designs(8))
+    if (ndesigns == size(designs)) then
+       allocate(g(2 * size(designs)))
+       do k = 1, ndesigns
+          call move_alloc(designs(k)%name, g(k)%name)
+          if (allocated(designs(k)%file)) call move_alloc(designs(k)%file, g(k)%file)
+          g(k)%direction = designs(k)%direction
+       end do
+       call move_alloc(g, designs)
+    end if
+    ndesigns = ndesigns + 1
+    designs(ndesigns)%name = trim(name)
+    designs(ndesigns)%file = trim(file)
+    designs(ndesigns)%direction = direction
+    declare_design = ndesigns
+  end function declare_design
+
+  ! A layer created in declared design k is backed by
+  ! temp_folder//DB//'_MTL_'//NAME (src/controlparser.f90:322-330,363-373).
+  subroutine back_new_layer(i, k)
+    integer, intent(in) :: i, k
+    integer :: p
+    character(len=:), allocatable :: lname
+    p = index(table(i)%name, ':')
+    lname = table(i)%name(p+1:)
+    table(i)%db = k
+    table(i)%file_is_klbin = .false.
+    if (lname == 'nothing') then
+       table(i)%file = 'MTL_NOTHING'
+    else
+       table(i)%file = temp_folder//designs(k)%name//'_MTL_'//lname
+    end if
+  end subroutine back_new_layer
+
+  ! The layer an exec run writes (:341-384): created when new (backed as
+  ! above in a declared design); a released layer is replaced by the result.
+  integer function lhs_index(name)
+    character(len=*), intent(in) :: name
+    integer :: k
+    logical :: fresh
+    fresh = .not. layer_exists(name)
+    lhs_index = layer_index(name)
+    table(lhs_index)%released = .false.
+    k = design_of(name)
+    if (fresh .and. k > 0) call back_new_layer(lhs_index, k)
+  end function lhs_index
+
+  ! exec init DB:NAME (:310-340): an empty layer of a declared design.
+  subroutine exec_init(name, line_no)
+    character(len=*), intent(in) :: name
+    integer, intent(in) :: line_no
+    integer :: i, k
+    k = design_of(name)
+    if (k == 0) then
+       write(error_unit, '(A,A,A,I0)') 'DB: ', trim(name), ' not found. Syntax ERROR: line: ', line_no
+       error stop "Syntax ERROR"
+    end if
+    i = layer_index(name)
+    call back_new_layer(i, k)
+    call ClearLayer(table(i)%L)
+    table(i)%released = .false.
+  end subroutine exec_init
+
+  ! addBoxToLayer (src/magparser.f90:493-507): appends one valid box.
+  subroutine add_box(L, b)
+    type(Layer), intent(inout) :: L
+    type(Box), intent(in) :: b
+    type(Box), allocatable :: g(:)
+    if (.not. (b%x1 < b%x2 .and. b%y1 < b%y2)) then
+       write(error_unit, '(A,4I12)') 'Box ', b%x1, b%y1, b%x2, b%y2
+       error stop "ERROR: box in valid"
+    end if
+    if (.not. L%host_valid) call layer_to_host(L)
+    if (.not. allocated(L%layer_boxes)) allocate(L%layer_boxes(4))
+    if (L%n_used >= size(L%layer_boxes, kind=int64)) then
+       allocate(g(2 * size(L%layer_boxes, kind=int64)))
+       g(1:L%n_used) = L%layer_boxes(1:L%n_used)
+       call move_alloc(g, L%layer_boxes)
+    end if
+    L%n_used = L%n_used + 1
+    L%layer_boxes(L%n_used) = b
+    L%n_alloc = size(L%layer_boxes, kind=int64)
+    call layer_boxes_changed(L)
+  end subroutine add_box
+
+  ! Reads design k's .mag file (parseMagicLayoutFile, src/magparser.f90:
+  ! 108-290): each section "<< NAME >>" (but labels and end) is layer
+  ! DB:NAME; rect lines add boxes to it; KLBIN / SNAP / DSNAP lines name its
+  ! backing file, read now for an input design.  "<< end >>" ends the file;
+  ! magscale, rlabel and other lines are skipped.  An input design's layers
+  ! are then numbered and, when their boxes overlap, healed (:358-418).
+  subroutine load_mag(k)
+    integer, intent(in) :: k
+    character(len=4096) :: line
+    character(len=256) :: dummy
+    character(len=:), allocatable :: section
+    integer :: u, ios, cur, i, n
+    integer(int32) :: x1, y1, x2, y2
+    open(newunit=u, file=designs(k)%file, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       write(error_unit, '(A,A)') 'ERROR: cannot open design file ', designs(k)%file
+       error stop "ERROR: design file"
+    end if
+    cur = 0
+    do
+       read(u, '(A)', iostat=ios) line
+       if (ios /= 0) exit
+       if (len_trim(line) == 0 .or. line(1:1) == '#') cycle
+       if (line(1:8) == 'magscale') cycle
+       if (line(1:9) == '<< end >>') exit
+       if (line(1:2) == '<<') then
+          n = len_trim(line)
+          section = trim(adjustl(line(4:max(3, n - 2))))
+          if (section == 'labels' .or. section == 'end') cycle
+          cur = layer_index(designs(k)%name//':'//section)
+          table(cur)%db = k
+          table(cur)%released = .false.
+          cycle
+       end if
+       if (cur == 0) cycle
+       if (line(1:4) == 'rect') then
+          read(line, *, iostat=ios) dummy, x1, y1, x2, y2
+          if (ios /= 0) then
+             write(error_unit, '(A,A)') 'ERROR: bad rect line in ', designs(k)%file
+             error stop "ERROR: design file"
+          end if
+          call add_box(table(cur)%L, Box(x1, y1, x2, y2))
+       else if (line(1:5) == 'KLBIN' .or. line(1:5) == 'DSNAP') then
+          call attach_file(cur, k, trim(adjustl(line(7:))))
+       else if (line(1:4) == 'SNAP') then
+          call attach_file(cur, k, trim(adjustl(line(6:))))
+       end if
+    end do
+    close(u)
+    if (designs(k)%direction /= DIR_INPUT) return
+    do i = 1, nlayers
+       if (table(i)%db == k .and. table(i)%L%n_used > 0) call PNumHealLayer(table(i)%L)
+    end do
+  end subroutine load_mag
+
+  subroutine attach_file(i, k, f)
+    integer, intent(in) :: i, k
+    character(len=*), intent(in) :: f
+    table(i)%file = f
+    table(i)%file_is_klbin = .false.
+    if (designs(k)%direction == DIR_INPUT) call RestoreSnapToLayer(table(i)%L, f)
+  end subroutine attach_file
+
+  ! flush (:613-626, writeMagicLayoutFile src/magparser.f90:510-564): every
+  ! non-empty layer of an output design goes to its file (SNAP when the name
+  ! ends in .snap, KLBIN when it ends in .bin, anything else is an error),
+  ! and with temporary_layers 2 every non-empty layer of a memory design to
+  ! its SNAP file; each written layer is released (read again when used).
+  subroutine flush_designs()
+    integer :: i, k, n
+    character(len=:), allocatable :: f
+    do i = 1, nlayers
+       k = table(i)%db
+       if (k == 0) cycle
+       if (designs(k)%direction /= DIR_OUTPUT .and. &
+            .not. (designs(k)%direction == DIR_MEMORY .and. temporary_layers == 2)) cycle
+       if (table(i)%released .or. table(i)%L%n_used == 0) cycle
+       if (.not. allocated(table(i)%file)) then
+          write(error_unit, '(A,A)') 'Layer not assigned backing store: ', table(i)%name
+          error stop "ERROR: layer backing store name not allocated"
+       end if
+       f = table(i)%file
+       n = len(f)
+       if (designs(k)%direction == DIR_MEMORY) then
+          call SaveLayerToSnap(table(i)%L, f, int(K_COMPRESSION_METHOD_TO_USE))
+       else if (n >= 5 .and. f(max(1, n - 4):n) == '.snap') then
+          call SaveLayerToSnap(table(i)%L, f, int(K_COMPRESSION_METHOD_TO_USE))
+       else if (n >= 4 .and. f(max(1, n - 3):n) == '.bin') then
+          call layer_to_host(table(i)%L)
+          call WriteKLBin(f, table(i)%L%layer_boxes, table(i)%L%n_used)
+       else
+          write(error_unit, '(A,A)') 'ERROR: File format not supported: ', f
+          error stop "DBOUT only supports KLBIN, edit the output MAG file to fix"
+       end if
+       call ClearLayer(table(i)%L)
+       table(i)%released = .true.
+    end do
+  end subroutine flush_designs
+
+  ! delete D of a declared design (:627-642, DeleteDesign): its layers are
+  ! dropped; using one afterwards is an error.
+  subroutine delete_design(k)
+    integer, intent(in) :: k
+    integer :: i
+    do i = 1, nlayers
+       if (table(i)%db /= k) cycle
+       call ClearLayer(table(i)%L)
+       if (allocated(table(i)%file)) deallocate(table(i)%file)
+       table(i)%released = .true.
+       table(i)%db = 0
+    end do
+    designs(k)%direction = DIR_DELETED
+  end subroutine delete_design
+
   ! Splits s into blank-separated words.
   subroutine split_words(s, w, nw)
     character(len=*), intent(in) :: s
     character(len=256), intent(out) :: w(:)
     integer, intent(out) :: nw
     integer :: i, j, n
+    w = ' '
     nw = 0
     n = len_trim(s)
     i = 1
@@ -384,7 +671,7 @@ contains
     if (len_trim(opw) == 1 .and. scan(opw, '+@*%^~!') == 1) then
        if (opw(1:1) == '%') return
        i2 = existing(w(7), line_no)
-       il = layer_index(w(3))
+       il = lhs_index(w(3))
        select case (opw(1:1))
        case ('+')
           code = OP_OR
@@ -426,7 +713,7 @@ contains
        end if
     case ('OR2', 'AND2', 'NOT', 'FRAMENOT', 'XOR')
        i2 = existing(w(7), line_no)
-       il = layer_index(w(3))
+       il = lhs_index(w(3))
        if (later) then
           select case (trim(opw))
           case ('OR2')
@@ -454,7 +741,7 @@ contains
           end select
        end if
     case ('COPY')
-       il = layer_index(w(3))
+       il = lhs_index(w(3))
        if (later) then
           call defer(OP_MERGE, i1, 0, il, [0, 0, 0, 0])
        else
@@ -463,7 +750,7 @@ contains
     case ('EXTENT', 'AND', 'GRID')
        ! operators a program does not take: run what is pending, then this one
        call flush_pending()
-       il = layer_index(w(3))
+       il = lhs_index(w(3))
        select case (trim(opw))
        case ('EXTENT')
           if (.not. is_nothing(w(7))) error stop "EXTENT must use nothing as second layer"
@@ -487,7 +774,7 @@ contains
        do k = 1, 4
           iv(k) = int(real(used_precision, real64) * to_real(w(7 + k), line_no), int32)
        end do
-       il = layer_index(w(3))
+       il = lhs_index(w(3))
        if (later) then
           call defer(OP_GROW, i1, 0, il, iv)
        else
@@ -497,7 +784,7 @@ contains
        if (.not. is_nothing(w(7))) error stop "SIZE must use nothing as second layer"
        if (nw < 8) error stop "SYNTAX ERROR: SIZE needs a length"
        d = int(to_real(w(8), line_no) * real(used_precision, real64), int64)
-       il = layer_index(w(3))
+       il = lhs_index(w(3))
        if (later) then
           if (d < -huge(0_int32) - 1_int64 .or. d > huge(0_int32)) error stop "SIZE amount out of int32 range"
           call defer(OP_SIZE, i1, 0, il, [int(d, int32), 0, 0, 0])
@@ -518,6 +805,7 @@ contains
     character(len=256) :: w(16)
     integer :: u, ios, nw, line_no, p, i
     integer(int64) :: a
+    if (.not. allocated(temp_folder)) temp_folder = ''
     open(newunit=u, file=trim(path), status='old', action='read', iostat=ios)
     if (ios /= 0) then
        write(error_unit, *) 'ERROR: cannot open control file ', trim(path)
@@ -533,11 +821,52 @@ contains
        call split_words(line, w, nw)
        if (nw == 0) cycle
        ! every statement other than exec run needs the results so far
-       if (trim(w(1)) /= 'exec') call flush_pending()
+       if (trim(w(1)) /= 'exec' .or. trim(w(2)) /= 'run') call flush_pending()
        select case (trim(w(1)))
        case ('exec')
-          if (nw < 2 .or. w(2) /= 'run') error stop "SYNTAX ERROR: expected exec run"
-          call exec_run(w, nw, line_no)
+          select case (trim(w(2)))
+          case ('run')
+             call exec_run(w, nw, line_no)
+          case ('init')
+             if (nw < 3) error stop "SYNTAX ERROR: exec init DB:NAME"
+             call exec_init(w(3), line_no)
+          case ('modedesign')
+             if (nw < 5 .or. w(4) /= '=') error stop "SYNTAX ERROR: exec modedesign D = FILE"
+             call load_mag(declare_design(w(3), w(5), DIR_INPUT))
+          case default
+             write(error_unit, '(A,I0)') 'Syntax ERROR: line: ', line_no
+             error stop "SYNTAX ERROR: exec run | init | modedesign"
+          end select
+       case ('decl')
+          if (nw < 5 .or. w(4) /= '=') then
+             write(error_unit, '(A,I0)') 'Syntax ERROR: line: ', line_no
+             error stop "Syntax ERROR."
+          end if
+          select case (trim(w(2)))
+          case ('design')
+             call load_mag(declare_design(w(3), w(5), DIR_INPUT))
+          case ('output')
+             call load_mag(declare_design(w(3), w(5), DIR_OUTPUT))
+          case ('memory')
+             if (trim(w(5)) /= 'nothing') error stop "MEMORY DB must have backing store nothing"
+             i = declare_design(w(3), 'nothing', DIR_MEMORY)
+          case default
+             write(error_unit, '(A,I0)') 'Syntax ERROR: line: ', line_no
+             error stop "Syntax ERROR."
+          end select
+       case ('var')
+          select case (trim(w(2)))
+          case ('temp_folder')
+             temp_folder = trim(w(3))
+          case ('temporary_layers')
+             temporary_layers = nint(to_real(w(3), line_no))
+          case default   ! abort_on_assert_zero, abort_on_xor, debug_verbosity: accepted
+             continue
+          end select
+       case ('program')
+          write(output_unit, '(A,A)') '--- Starting Program: ', trim(w(2))
+       case ('systeminfo', 'info')
+          continue
        case ('schedule')
           select case (trim(w(2)))
           case ('immediate')
@@ -550,10 +879,14 @@ contains
              error stop "SYNTAX ERROR: schedule immediate|batch|graph"
           end select
        case ('flush')
-          continue
+          call flush_designs()
        case ('delete')
           if (nw < 2) error stop "SYNTAX ERROR: delete NAME"
-          call delete_layers(w(2))
+          if (design_index(w(2)) > 0) then
+             call delete_design(design_index(w(2)))
+          else
+             call delete_layers(w(2))
+          end if
        case ('end')
           exit
        case ('precision')
@@ -564,6 +897,7 @@ contains
           if (w(3) == 'klbin') then
              call load_klbin(table(i)%L, w(4))
              table(i)%file = trim(w(4))
+             table(i)%file_is_klbin = .true.
              table(i)%released = .false.
           else if (w(3) == 'synth' .and. nw >= 7) then
              call load_synth(table(i)%L, nint(to_real(w(4), line_no)), int(to_real(w(5), line_no), int64), &
@@ -579,8 +913,7 @@ contains
        case ('resident')
           vl_eager_host = trim(w(2)) /= 'on'
        case default
-          write(error_unit, '(A,A,A,I0)') 'SYNTAX ERROR: unknown statement ', trim(w(1)), ' on line ', line_no
-          error stop "SYNTAX ERROR"
+          write(output_unit, '(A,A,A,I0)') 'Unknown command: ', trim(w(1)), ' on line ', line_no
        end select
     end do
     close(u)

@@ -106,6 +106,17 @@ module vl_layer_ops
      character(len=:), allocatable :: fileName
   end type Design
   public :: design_layer_index
+  public :: RestoreSnapToLayer, SaveLayerToSnap, LoadKLBin, WriteKLBin, ClearLayer, PNumHealLayer
+  public :: COMPRESSION_METHOD_SNAPPY, COMPRESSION_METHOD_ZLIB, COMPRESSION_METHOD_ZSTD
+
+  ! Layer files (src/commonmodule.f90:15-21, src/box_codec.f90): the
+  ! compression method of SNAP files written at flush (1 Snappy, the
+  ! reference's default), and temporary_layers 2, under which a missing layer
+  ! file reads as an empty layer (src/diskdesign_impl.f90:87-97).
+  integer(c_int32_t), parameter :: COMPRESSION_METHOD_SNAPPY = 1, COMPRESSION_METHOD_ZLIB = 2, &
+       COMPRESSION_METHOD_ZSTD = 3
+  integer, public :: temporary_layers = 1
+  integer(c_int32_t), public :: K_COMPRESSION_METHOD_TO_USE = COMPRESSION_METHOD_SNAPPY
 
   logical, public :: vl_eager_host = .true.
   integer(c_int), public :: vl_device = 0
@@ -232,6 +243,40 @@ module vl_layer_ops
        integer(c_int64_t) :: area
        integer(c_int) :: rc
      end function c_klee_area
+     function c_layer_load_klbin(e, l, path) bind(C, name="vl_layer_load_klbin") result(rc)
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: e, l
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int) :: rc
+     end function c_layer_load_klbin
+     function c_layer_save_klbin(e, l, path) bind(C, name="vl_layer_save_klbin") result(rc)
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: e, l
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int) :: rc
+     end function c_layer_save_klbin
+     function c_layer_save_snap(e, l, path, method, props, pnum) bind(C, name="vl_layer_save_snap") result(rc)
+       import :: c_int, c_ptr, c_char, c_int32_t
+       type(c_ptr), value :: e, l, pnum
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: method, props
+       integer(c_int) :: rc
+     end function c_layer_save_snap
+     function c_layer_load_snap(e, l, path, props, pnum, cap) bind(C, name="vl_layer_load_snap") result(rc)
+       import :: c_int, c_ptr, c_char, c_int32_t, c_int64_t
+       type(c_ptr), value :: e, l, pnum
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t) :: props
+       integer(c_int64_t), value :: cap
+       integer(c_int) :: rc
+     end function c_layer_load_snap
+     function c_snap_info(path, props, total, chunks, method) bind(C, name="vl_snap_info") result(rc)
+       import :: c_int, c_char, c_int32_t, c_int64_t
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t) :: props, method
+       integer(c_int64_t) :: total, chunks
+       integer(c_int) :: rc
+     end function c_snap_info
      function c_xor_empty(e, a, b, empty) bind(C, name="vl_xor_empty") result(rc)
        import :: c_int, c_ptr
        type(c_ptr), value :: e, a, b
@@ -894,5 +939,152 @@ contains
     call vl_fortran_init()
     call thrust_sort_boxes_indirect(c_loc(Boxes(1)), c_loc(SortedIndices(1)), N)
   end subroutine SortBoxesIndirect
+
+  ! The polygon pass of a design load (parseMagicLayoutFile,
+  ! src/magparser.f90:358-418): number the polygons; a layer whose boxes
+  ! overlap is healed and numbered again.  The layer ends HEAL and PNUM.
+  subroutine PNumHealLayer(L)
+    type(Layer), intent(inout) :: L
+    if (L%n_used == 0) return
+    call pnum_layer(L)
+    if (L%area > 0) then
+       if (CertifyHealing(L)) error stop "ERROR: INCORRECT GPU solution detected"
+       call MergeHealLayer(L)
+       call pnum_layer(L)
+       if (L%area > 0) error stop 'UNION failed.'
+    end if
+    L%layerState = ior(L%layerState, ior(LAYER_STATE_PNUM, LAYER_STATE_HEAL))
+  end subroutine PNumHealLayer
+
+  ! ---- layer files ----------------------------------------------------------
+
+  ! ClearLayer (src/design.f90:1279-1287): no boxes, every state bit set.
+  subroutine ClearLayer(L)
+    type(Layer), intent(inout) :: L
+    call layer_release(L)
+    if (allocated(L%layer_boxes)) deallocate(L%layer_boxes)
+    if (allocated(L%pnumtable%arr)) deallocate(L%pnumtable%arr)
+    L%n_used = 0
+    L%n_alloc = 0
+    L%host_valid = .true.
+    L%dev_valid = .false.
+    L%layerState = LAYER_STATE_EVERYTHING
+  end subroutine ClearLayer
+
+  ! RestoreSnapToLayer (src/diskdesign_impl.f90:83-114): a name holding
+  ! ".bin" is a KLBIN file, anything else a SNAP file, read straight into the
+  ! layer's device copy (SNAP chunks are decompressed on the host and their
+  ! box codec decoded on the GPU).  With temporary_layers > 1 a missing file
+  ! is an empty layer.  A SNAP file's state bits (masked to
+  ! LAYER_STATE_EVERYTHING) become the layer's, and its polygon numbers, when
+  ! present, fill pnumtable.  The reference then builds an R-tree (sorting
+  ! the boxes); the engine needs none, so the file's order is kept.
+  subroutine RestoreSnapToLayer(input_layer, snap_filename)
+    type(Layer), intent(inout), target :: input_layer
+    character(len=*), intent(in) :: snap_filename
+    character(kind=c_char, len=:), allocatable :: cpath
+    integer(c_int32_t) :: props, method
+    integer(c_int64_t) :: total, nch, n
+    logical :: exists
+    inquire(file=trim(snap_filename), exist=exists)
+    if (temporary_layers > 1 .and. .not. exists) then
+       call ClearLayer(input_layer)
+       return
+    end if
+    cpath = trim(snap_filename)//c_null_char
+    call ClearLayer(input_layer)
+    call ensure_handle(input_layer)
+    if (index(snap_filename, '.bin') /= 0) then
+       call check(c_layer_load_klbin(engine, input_layer%dev, cpath), "LoadKLBin")
+       input_layer%layerState = LAYER_STATE_NONE
+    else
+       call check(c_snap_info(cpath, props, total, nch, method), "RestoreSnapToLayer")
+       if (iand(props, LAYER_STATE_PNUM) /= 0 .and. total > 0) then
+          allocate(input_layer%pnumtable%arr(total))
+          call check(c_layer_load_snap(engine, input_layer%dev, cpath, props, c_loc(input_layer%pnumtable%arr(1)), &
+               total), "RestoreSnapToLayer")
+       else
+          call check(c_layer_load_snap(engine, input_layer%dev, cpath, props, c_null_ptr, 0_c_int64_t), &
+               "RestoreSnapToLayer")
+       end if
+       input_layer%layerState = iand(props, LAYER_STATE_EVERYTHING)
+    end if
+    call check(c_layer_size(input_layer%dev, n), "vl_layer_size")
+    input_layer%n_used = n
+    input_layer%dev_valid = .true.
+    input_layer%host_valid = .false.
+    if (vl_eager_host) call layer_to_host(input_layer)
+  end subroutine RestoreSnapToLayer
+
+  ! SaveLayerToSnap (src/diskdesign_impl.f90:59-81): the layer's boxes in a
+  ! SNAP file; its state bits without LAYER_STATE_RTREE go to the header,
+  ! its polygon numbers after it when LAYER_STATE_PNUM is set.  The
+  ! reference sorts the layer for its R-tree first; here the stored order is
+  ! written.
+  subroutine SaveLayerToSnap(input_layer, snap_filename, method_to_use)
+    type(Layer), intent(inout), target :: input_layer
+    character(len=*), intent(in) :: snap_filename
+    integer, intent(in) :: method_to_use
+    type(c_ptr) :: pn
+    integer(c_int32_t) :: props
+    if (method_to_use /= COMPRESSION_METHOD_SNAPPY .and. method_to_use /= COMPRESSION_METHOD_ZLIB .and. &
+         method_to_use /= COMPRESSION_METHOD_ZSTD) then
+       write(error_unit, *) 'ERROR: unknown compression method: ', method_to_use, ' requested.'
+       error stop
+    end if
+    call to_device(input_layer)
+    props = iand(input_layer%layerState, not(LAYER_STATE_RTREE))
+    pn = c_null_ptr
+    if (iand(props, LAYER_STATE_PNUM) /= 0 .and. allocated(input_layer%pnumtable%arr) .and. input_layer%n_used > 0) then
+       if (size(input_layer%pnumtable%arr, kind=int64) == input_layer%n_used) pn = c_loc(input_layer%pnumtable%arr(1))
+    end if
+    call check(c_layer_save_snap(engine, input_layer%dev, trim(snap_filename)//c_null_char, &
+         int(method_to_use, c_int32_t), props, pn), "SaveLayerToSnap")
+  end subroutine SaveLayerToSnap
+
+  ! LoadKLBin (src/kldata.f90:14-63): boxes of a raw 16-byte-per-box file;
+  ! an empty or missing file and an invalid box stop the program.
+  subroutine LoadKLBin(fileName, boxes)
+    character(len=*), intent(in) :: fileName
+    type(Box), allocatable, intent(out) :: boxes(:)
+    integer(int64) :: file_bytes, total_boxes, i
+    integer :: u, ios
+    inquire(file=trim(fileName), size=file_bytes)
+    if (file_bytes <= 0) then
+       write(error_unit, *) 'Error: KLBIN File ', trim(fileName), ' is empty or does not exist.'
+       error stop
+    end if
+    total_boxes = file_bytes / 16
+    allocate(boxes(total_boxes))
+    open(newunit=u, file=trim(fileName), access='stream', form='unformatted', status='old', action='read', &
+         iostat=ios)
+    if (ios /= 0) error stop "Error: Could not open the binary file"
+    if (total_boxes > 0) read(u, iostat=ios) boxes
+    close(u)
+    if (ios /= 0) error stop "Error occurred while reading the data array."
+    do i = 1, total_boxes
+       if (.not. (boxes(i)%x1 < boxes(i)%x2 .and. boxes(i)%y1 < boxes(i)%y2)) error stop "INVALID BOX detected on input"
+    end do
+  end subroutine LoadKLBin
+
+  ! WriteKLBin (src/kldata.f90:64-97): boxes(1:total_boxes), each valid.
+  subroutine WriteKLBin(fileName, boxes, total_boxes)
+    character(len=*), intent(in) :: fileName
+    type(Box), intent(in) :: boxes(:)
+    integer(int64), intent(in) :: total_boxes
+    integer(int64) :: i
+    integer :: u, ios
+    do i = 1, total_boxes
+       if (.not. (boxes(i)%x1 < boxes(i)%x2 .and. boxes(i)%y1 < boxes(i)%y2)) then
+          write(error_unit, *) 'Box: ', i, ' WRONG.'
+          error stop "INVALID BOX"
+       end if
+    end do
+    open(newunit=u, file=trim(fileName), access='stream', form='unformatted', status='replace', action='write', &
+         iostat=ios)
+    if (ios /= 0) error stop "Error: Could not open the binary file for writing."
+    if (total_boxes > 0) write(u) boxes(1:total_boxes)
+    close(u)
+  end subroutine WriteKLBin
 
 end module vl_layer_ops
