@@ -43,9 +43,11 @@ Also reported in the same line: "c2_10M_total", the same ring with 10M boxes
 over the 4 layers (2.5M per layer), plain and batched; "c3_uneven_tiles",
 AND/NOT/OR/GROW+2 over 4096 packed tiles of uneven size (~50M boxes), plain,
 as one recorded program and as its graph, autotuned; "c4_full_chip", AND,
-NOT, GROW+1 and the exact Klee area on a 2 x 180M-box layer pair; and
-"serving", windowed query latency (p50/p99) and throughput of the query
-server on a resident 50M-box 4-layer design under Poisson arrivals.
+NOT, GROW+1 and the exact Klee area on a 2 x 180M-box layer pair;
+"design_io", loading the 10M-box met1 layer from a SNAP file (host Snappy +
+GPU codec decode) and from KLBIN; and "serving", windowed query latency
+(p50/p99) and throughput of the query server on a resident 50M-box 4-layer
+design under Poisson arrivals.
 """
 import argparse
 import json
@@ -87,6 +89,8 @@ def parse():
     p.add_argument("--no-c3", action="store_true", help="skip the 4096-uneven-tile measurement")
     p.add_argument("--c4-boxes", type=int, default=360_000_000,
                    help="full-chip layer pair size for the C4 measurement (0 = skip)")
+    p.add_argument("--design-io-boxes", type=int, default=10_000_000,
+                   help="boxes of the layer-file measurement (0 skips it)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="first copy this many MiB device-to-device (a copy kernel of known size for "
                         "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -317,6 +321,78 @@ def c4_full_chip(V, args):
             "boxes_per_s": ins * args.steps / secs, "klee_area_A": int(area["A"])}
 
 
+def design_io(V, args):
+    """Layer files: the benchmark design's met1 layer (10M boxes by default)
+    written as a Snappy SNAP file (the reference's flush format) and as KLBIN,
+    then read back into HBM.  One SNAP step = one RestoreSnapToLayer: file
+    read, Snappy decompression of the 65536-box chunks on the host cores,
+    upload of the codec bytes, GPU codec decode straight into the layer and
+    box validation.  The GPU decode alone is timed by its HIP events
+    (kernel statistic "snap"; algorithmic bytes: the codec stream read once
+    plus 16 B per box written).  cpu: the oracle's sequential codec decode
+    (the reference's DecompressBoxesUsingCodec loop) of the first 16 chunks
+    on one core."""
+    import shutil
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    n = args.design_io_boxes
+    boxes = V.synth_layer(V.SYN_STDCELL, SEED, 2, n)
+    eng = V.Engine(0)
+    L = V.Layer(eng, boxes)
+    d = tempfile.mkdtemp(prefix="vl_io_")
+    try:
+        snap, kl = os.path.join(d, "met1.snap"), os.path.join(d, "met1.bin")
+        t0 = time.perf_counter()
+        V.SaveLayerToSnap(L, snap, V.COMPRESSION_METHOD_SNAPPY)
+        t_save = time.perf_counter() - t0
+        V.WriteKLBin(kl, L)
+        L.close()
+        R = V.Layer(eng)
+
+        def timed(fn):
+            for _ in range(args.warmup):
+                fn()
+            eng.sync()
+            eng.set("profile", 1)
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            eng.sync()
+            secs = time.perf_counter() - t
+            st = eng.kernel_stats("snap")
+            eng.set("profile", 0)
+            return secs / args.steps, st
+
+        s_snap, (nl, ms, by) = timed(lambda: V.RestoreSnapToLayer(R, snap))
+        if not np.array_equal(R.boxes(), boxes):
+            raise SystemExit("design_io: the SNAP file did not load back to the layer")
+        s_kl, _ = timed(lambda: V.LoadKLBin(kl, R))
+        _, total, nch, _ = V.SnapInfo(snap)
+        dec_ms = ms / args.steps
+        dec_gbs = (by / args.steps) / (dec_ms * 1e-3) / 1e9
+        chunks = [boxes[k * 65536:(k + 1) * 65536] for k in range(min(16, nch))]
+        raws = [V.codec_encode(c) for c in chunks]
+        t = time.perf_counter()
+        for c, raw in zip(chunks, raws):
+            if O.codec_decode(raw, len(c)) is None:
+                raise SystemExit("design_io: oracle decode failed")
+        cpu_s = time.perf_counter() - t
+        res = {"boxes": int(total), "chunks": int(nch), "snap_bytes": os.path.getsize(snap),
+               "codec_bytes_per_box": (by / args.steps - 16.0 * total) / total,
+               "save_snap_s": t_save,
+               "load_snap_ms": 1000.0 * s_snap, "load_snap_boxes_per_s": total / s_snap,
+               "load_klbin_ms": 1000.0 * s_kl, "load_klbin_boxes_per_s": total / s_kl,
+               "gpu_decode_ms": dec_ms, "gpu_decode_boxes_per_s": total / (dec_ms * 1e-3),
+               "gpu_decode_gbs": dec_gbs, "gpu_decode_hbm_frac": dec_gbs / HBM_PEAK_GBS,
+               "gpu_decode_launches": nl / args.steps,
+               "cpu_decode_boxes_per_s": sum(len(c) for c in chunks) / cpu_s, "cpu_decode_cores": 1}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    eng.close()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -532,6 +608,12 @@ def main():
     if args.c4_boxes:
         line["c4_full_chip"] = c4_full_chip(V, args)
         print(f"[bench] c4: {line['c4_full_chip']['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
+    if args.design_io_boxes:
+        line["design_io"] = design_io(V, args)
+        dio = line["design_io"]
+        print(f"[bench] design_io: SNAP load {dio['load_snap_ms']:.1f} ms "
+              f"(GPU decode {dio['gpu_decode_ms']:.2f} ms, {dio['gpu_decode_gbs']:.0f} GB/s), "
+              f"KLBIN load {dio['load_klbin_ms']:.1f} ms", file=sys.stderr, flush=True)
     if args.serving_boxes:
         line["serving"] = serving(V, args)
         print(f"[bench] serving: p50 {line['serving']['p50_ms']:.2f} ms, p99 {line['serving']['p99_ms']:.2f} ms",

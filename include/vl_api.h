@@ -374,11 +374,13 @@ int vl_engine_last_time_ms(vl_engine* e, double* ms);
  * tiers and the workgroup kernel launched back to back), "fused" (the fused
  * tile kernel's tiers of a program's operator group), "bin" (tile binning:
  * span, emit, radix sort, bounds, gather), "sort" (the radix passes inside
- * the binning).  *launches counts kernel launches (sort: digit passes) and
- * *bytes is their algorithmic byte count (tile and fused: 16 B per entry
- * read + 16 B per strip written + 48 B per tile of descriptor and range
- * tables; sort: 16 B per entry per pass; bin: 32 B per input box + 8 B per
- * entry emitted + the sort + 32 B per entry gathered). */
+ * the binning), "snap" (the GPU box-codec decode of SNAP loads and
+ * vl_codec_decode: its five kernels back to back).  *launches counts kernel
+ * launches (sort: digit passes) and *bytes is their algorithmic byte count
+ * (tile and fused: 16 B per entry read + 16 B per strip written + 48 B per
+ * tile of descriptor and range tables; sort: 16 B per entry per pass; bin:
+ * 32 B per input box + 8 B per entry emitted + the sort + 32 B per entry
+ * gathered; snap: the codec bytes read once + 16 B per box written). */
 int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms,
                            double* bytes);
 

@@ -163,7 +163,7 @@ struct vl_engine {
     double bytes = 0.0;
   };
   bool prof = false;
-  KStat kst[4];                  // KS_* below
+  KStat kst[5];                  // KS_* below
   // Event pairs of profiled stages whose durations are read later (at the
   // next statistics query or the end of a program run), so that collecting
   // statistics adds no host waits to the stream.
@@ -191,7 +191,7 @@ struct vl_engine {
 
 enum { CAP_EAGER = 0, CAP_RECORD = 1, CAP_CAPTURE = 2 };
 
-enum { KS_TILE = 0, KS_BIN = 1, KS_SORT = 2, KS_FUSED = 3, KS_N = 4 };
+enum { KS_TILE = 0, KS_BIN = 1, KS_SORT = 2, KS_FUSED = 3, KS_SNAP = KSTAT_SNAP, KS_N = 5 };
 
 struct vl_layer {
   vl_engine* e = nullptr;
@@ -1567,6 +1567,11 @@ vl_engine* default_engine() {
 
 // ===========================================================================
 // C ABI
+hipEvent_t vl::engine_prof_mark(vl_engine* e) { return prof_mark(e); }
+void vl::engine_prof_add(vl_engine* e, hipEvent_t a, hipEvent_t b, int kind, int64_t launches, double bytes) {
+  prof_add(e, a, b, kind, launches, bytes);
+}
+
 extern "C" {
 
 const char* vl_last_error(void) { return g_err.c_str(); }
@@ -1719,7 +1724,8 @@ int vl_engine_sync(vl_engine* e) {
 int vl_engine_kernel_stats(vl_engine* e, const char* kernel, int64_t* launches, double* total_ms, double* bytes) {
   if (!e || !kernel || !launches || !total_ms || !bytes) return fail(VL_EINVAL, "null argument");
   std::string k(kernel);
-  int i = k == "tile" ? KS_TILE : k == "bin" ? KS_BIN : k == "sort" ? KS_SORT : k == "fused" ? KS_FUSED : -1;
+  int i = k == "tile" ? KS_TILE : k == "bin" ? KS_BIN : k == "sort" ? KS_SORT : k == "fused" ? KS_FUSED
+        : k == "snap" ? KS_SNAP : -1;
   if (i < 0) return fail(VL_EINVAL, "unknown kernel '%s'", kernel);
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   VLCK(prof_resolve(e));

@@ -82,6 +82,13 @@ class EngineLock {
   bool locked_ = false;
 };
 
+// Kernel statistics of stages outside vl_engine.hip (vl_engine_kernel_stats
+// "snap"): marks are events on the engine stream (null when the engine does
+// not collect statistics); the caller holds the engine lock.
+enum { KSTAT_SNAP = 4 };
+hipEvent_t engine_prof_mark(vl_engine* e);
+void engine_prof_add(vl_engine* e, hipEvent_t a, hipEvent_t b, int kind, int64_t launches, double bytes);
+
 // vl_snap.hip: GPU decode of SNAP codec chunks (concatenated raw codec bytes;
 // coff: nchunks+1 byte offsets, roff: nchunks+1 record offsets, boff:
 // nchunks+1 walk-block offsets).  Scratch: len (total_raw), exit_e and cnt

@@ -455,7 +455,11 @@ int snap_decode_to_layer(vl_engine* e, vl_layer* l, const uint8_t* raw, const st
   s.boff = dboff;
   s.nchunks = nch;
   s.nblocks = nblocks;
+  hipEvent_t pa = engine_prof_mark(e);
   launch_snap_decode(s, st);
+  hipEvent_t pb = engine_prof_mark(e);
+  // algorithmic bytes: the codec stream read once, 16 B written per box
+  engine_prof_add(e, pa, pb, KSTAT_SNAP, 5, (double)total_raw + 16.0 * (double)nrec);
   if (hipGetLastError() != hipSuccess) return set_error(VL_EHIP, "decode launch");
   uint32_t err = 0;
   if (hipMemcpyAsync(&err, s.err, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
