@@ -11,12 +11,13 @@ Throughput = input boxes of all operators of a step (N_A+N_B for binary ops,
 N for GROW/SIZE) / wall time of the step, with the engine stream synchronised
 on both sides of the timed region.  "plain" runs one operator per call with
 the default tile budgets; "batched" runs the same six operators per layer as
-one recorded program after the autotuner has picked the tile budgets: GROW,
-SIZE and the two NOTs of a layer are one fused group (the layer binned once,
-the four outputs computed per tile from shared masks), the two ANDs run on the
-materialised outer / inner layers (so the ASSERT_ZEROs check what the fused
-group produced), and the four layer chains run concurrently on worker
-streams.  "graph" (the headline) is that program captured as a HIP graph
+one recorded program after the autotuner has picked the tile budgets and
+whether groups fuse: GROW, SIZE and the two NOTs of a layer are one fused
+group (the layer binned once, the four outputs computed per tile from shared
+masks) unless the autotuner finds the four single operators faster, the two
+ANDs run on the materialised outer / inner layers (so the ASSERT_ZEROs check
+what the group produced), and the four layer chains run concurrently on
+worker streams.  "graph" (the headline) is that program captured as a HIP graph
 (vl_program_set_graph): the first warm-up run records the sizes, captures
 the work and launches it, and the timed steps are replays of the graph
 (no host round trips; every replay checks its sizes on the device).  All

@@ -75,8 +75,9 @@ int vl_engine_destroy(vl_engine* e);
  *                  0: by complement, grow and difference
  *   "max_wg"       cap on the tile-kernel grid (0 = occupancy limit)
  *   "fuse"         programs: 1 (default) evaluates groups of operators over
- *                  one source layer in one fused tile pass; 0 runs every
- *                  operator on its own
+ *                  one source layer in one fused tile pass (up to two
+ *                  morphological images per group); 2 the same with at most
+ *                  one image per group; 0 runs every operator on its own
  *   "fuse_target"  mean boxes per tile of a fused group's grid (6)
  *   "bin_mode"     2 (default): tiles binned by per-tile counters and an
  *                  atomic scatter, one atomic per distinct tile of a wave;
@@ -96,9 +97,10 @@ int vl_engine_sync(vl_engine* e);
  * "erode_target" and "fuse_target" to the fastest of a few candidates, timed
  * on A NOT B (or the merge of A when b is NULL) plus GROW(A; 1), on
  * SIZE(A; -1), and on the fused ring of A (GROW(A; 1), SIZE(A; -1) and their
- * two differences with A); then "bin_mode" (2 or 1), "radix_bits" (8 or 11)
- * and "max_wg" (0, 4096 or 2048), each timed on A NOT B, A AND B and GROW(A;
- * 1).  A value changes only when a candidate is more than 3 % faster.  The
+ * two differences with A), then "fuse" (1 or 0: that ring as one fused group
+ * against the same four operators one at a time); then "bin_mode" (2 or 1),
+ * "radix_bits" (8 or 11) and "max_wg" (0, 4096 or 2048), each timed on A NOT
+ * B, A AND B and GROW(A; 1).  A value changes only when a candidate is more than 3 % faster.  The
  * first two are returned when the pointers are non-null (the others through
  * vl_engine_get_param).  Results of every operator are independent of these
  * parameters. */

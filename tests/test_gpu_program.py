@@ -73,6 +73,33 @@ def test_program_op1_chains_match_plain(gpu_engine, nlayers, fuse):
     eng.close()
 
 
+def test_program_one_image_groups(gpu_engine):
+    """fuse 2: groups of at most one morphological image.  The ring recorded
+    as GROW, its NOT, SIZE, its NOT is two groups per layer; every output is
+    the plain one."""
+    eng = V.Engine(0)
+    eng.set("fuse", 2)
+    layers = [V.Layer(eng, V.synth_layer(V.SYN_STDCELL, 0x5EED0002, k, 25000)) for k in range(2)]
+    prog = V.Program(eng)
+    temps = []
+    for L in layers:
+        g, s, outer, inner = (V.Layer(eng) for _ in range(4))
+        prog.add(V.VL_OP_GROW, L, None, g, (1, 1, 1, 1))
+        prog.add(V.K_BOOST_CONTROL_NOT, g, L, outer)
+        prog.add(V.K_BOOST_CONTROL_SIZE, L, None, s, (-1,))
+        prog.add(V.K_BOOST_CONTROL_NOT, L, s, inner)
+        temps.append((g, s, outer, inner))
+    prog.run()
+    assert eng.get("fused_groups") == 4
+    for L, t in zip(layers, temps):
+        for got, exp in zip(t, op1_plain(L)[:4]):
+            assert np.array_equal(O.sort_boxes(got.boxes()), O.sort_boxes(exp.boxes()))
+    with pytest.raises(V.VLError):
+        eng.set("fuse", 3)
+    prog.close()
+    eng.close()
+
+
 @pytest.mark.parametrize("workers", [1, 2, 6])
 def test_program_worker_counts(gpu_engine, workers):
     """Six OP1 chains on 1, 2 or 6 worker streams give the plain results."""
@@ -143,7 +170,8 @@ def test_autotune_sets_parameters_and_keeps_results(gpu_engine):
     assert chosen["fuse_target"] in (4, 5, 6, 7, 8, 10) and eng.get("fuse_target") == chosen["fuse_target"]
     assert eng.get("tile_target") == chosen["tile_target"]
     assert chosen["bin_mode"] in (1, 2) and chosen["radix_bits"] in (8, 11) and chosen["max_wg"] in (0, 2048, 4096)
-    assert all(eng.get(k) == chosen[k] for k in ("bin_mode", "radix_bits", "max_wg"))
+    assert chosen["fuse"] in (0, 1)
+    assert all(eng.get(k) == chosen[k] for k in ("bin_mode", "radix_bits", "max_wg", "fuse"))
     after = V.Layer(eng)
     V.CalculateFrameNOT(L1, L, after)
     assert np.array_equal(O.sort_boxes(before.boxes()), O.sort_boxes(after.boxes()))

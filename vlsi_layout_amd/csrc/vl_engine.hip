@@ -1677,7 +1677,8 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "erode_kernel") {
     e->erode_kernel = v != 0;
   } else if (k == "fuse") {
-    e->fuse = v != 0;
+    if (v < 0 || v > 2) return fail(VL_EINVAL, "fuse must be 0, 1 or 2");
+    e->fuse = (int)v;
   } else if (k == "fuse_target") {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "fuse_target out of range");
     e->fuse_target = (int)v;
@@ -2376,10 +2377,11 @@ struct GroupBuilder {
     return false;
   }
   // Image index of a morphological operator of S (added if new), -1 if full.
+  int max_images = 2;
   int morph(int32_t kind, const int32_t a[4]) {
     for (int k = 0; k < fs.nmorph; k++)
       if (fs.kind[k] == kind && memcmp(fs.arg[k], a, 4 * sizeof(int32_t)) == 0) return k;
-    if (fs.nmorph == 2) return -1;
+    if (fs.nmorph >= max_images) return -1;
     const int k = fs.nmorph++;
     fs.kind[k] = kind;
     memcpy(fs.arg[k], a, 4 * sizeof(int32_t));
@@ -2448,14 +2450,20 @@ struct GroupBuilder {
     if (o.op != VL_OP_GROW && o.op != VL_OP_SIZE && o.op != VL_OP_MERGE) return false;
     src = o.a;
     if (add(o)) return true;
+    const int mi = max_images;
     *this = GroupBuilder();
+    max_images = mi;
     return false;
   }
 };
 
-static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, bool fuse) {
+// fuse: 0 no groups, 1 groups of up to two morphological images, 2 groups of
+// at most one (smaller compressed grids per tile, one more binning of S).
+static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int fuse) {
   std::vector<ProgStep> steps;
+  const int max_images = fuse == 2 ? 1 : 2;
   GroupBuilder g;
+  g.max_images = max_images;
   auto single = [&](int i) {
     ProgStep st;
     st.ops.push_back(i);
@@ -2479,6 +2487,7 @@ static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, boo
       for (int i : g.ops) single(i);
     }
     g = GroupBuilder();
+    g.max_images = max_images;
   };
   for (int i = 0; i < (int)ops.size(); i++) {
     if (fuse && g.add(ops[i])) {
@@ -2792,7 +2801,7 @@ int vl_program_run(vl_program* p) {
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
   p->failures = 0;
-  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse != 0);
+  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse);
   const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
   int nw = (int)std::min<size_t>(chains.size(), (size_t)std::min(p->max_workers, p->e->workers));
   // the bounding boxes of the program's inputs (layers no operator writes)
@@ -3183,9 +3192,10 @@ extern "C" {
 // timing candidates on the caller's layers: A NOT B (or the merge of A) plus
 // GROW(A; 1) for the operator grid, SIZE(A; -1) for the erosion grid, and the
 // fused ring of A (GROW(A; 1), SIZE(A; -1), GROW(A; 1) NOT A, A NOT
-// SIZE(A; -1)) for the fused grid; each candidate is timed twice (device time,
-// HIP events) and the faster run counts.  Results never depend on these
-// parameters, only run time.
+// SIZE(A; -1)) for the fused grid, then whether programs fuse groups at all
+// ("fuse": the same ring as four single operators against the fused group);
+// each candidate is timed twice (device time, HIP events) and the faster run
+// counts.  Results never depend on these parameters, only run time.
 int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64_t* target, int64_t* erode_target) {
   if (!e || !a) return fail(VL_EINVAL, "null argument");
   std::lock_guard<std::recursive_mutex> lk(e->mu);
@@ -3235,7 +3245,7 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
   // (the current value stays unless a candidate is more than 3 % faster:
   // neighbouring budgets differ by less than the run-to-run noise)
   const int ftargets[] = {4, 5, 6, 7, 8, 10};
-  const int ft0 = e->fuse_target;
+  const int ft0 = e->fuse_target, fu0 = e->fuse;
   int bft = ft0;
   double ft0_ms = 1e30;
   {
@@ -3258,23 +3268,52 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
       if (rc != VL_OK) break;
       e->fuse_target = ft;
       double m = 0;
+      bool declined = false;
       rc = best_of([&]() {
         HIPCK(hipSetDevice(e->device));
         OpTimer tm(e);
         const int r = run_fused(e, a, fs, outs);
-        return r == FUSE_DECLINED ? VL_OK : r;   // a declined candidate times nothing useful
+        if (r == FUSE_DECLINED) declined = true;
+        return r == FUSE_DECLINED ? VL_OK : r;
       }, &m);
+      if (declined) m = 1e30;   // a declined candidate times nothing useful
       if (rc == VL_OK && ft == ft0) ft0_ms = m;
       if (rc == VL_OK && m < bms) {
         bms = m;
         bft = ft;
       }
     }
-    if (bms > 0.97 * ft0_ms) bft = ft0;
+    if (bms > 0.97 * ft0_ms) {
+      bft = ft0;
+      bms = ft0_ms;
+    }
+    e->fuse_target = bft;
+    // fused groups at all: the same ring as four single operators (GROW,
+    // SIZE and the two differences) against the fused group at its best
+    // budget; the current mode stays unless the other is more than 3 % faster
+    if (rc == VL_OK) {
+      double mu = 1e30;
+      for (int rep = 0; rep < 2 && rc == VL_OK; rep++) {
+        double acc = 0;
+        rc = vl_grow(e, a, 1, 1, 1, 1, outs[0]);
+        acc += e->last_ms;
+        if (rc == VL_OK) rc = vl_size(e, a, -1, outs[1]);
+        acc += e->last_ms;
+        if (rc == VL_OK) rc = vl_not(e, outs[0], a, outs[2]);
+        acc += e->last_ms;
+        if (rc == VL_OK) rc = vl_not(e, a, outs[1], outs[3]);
+        acc += e->last_ms;
+        mu = std::min(mu, acc);
+      }
+      if (rc == VL_OK) {
+        const bool fused_now = e->fuse != 0;
+        const double cur = fused_now ? bms : mu, alt = fused_now ? mu : bms;
+        if (alt < 0.97 * cur) e->fuse = fused_now ? 0 : 1;
+      }
+    }
     for (int k = 0; k < 4; k++)
       if (outs[k]) vl_layer_destroy(outs[k]);
   }
-  e->fuse_target = bft;
   // Binning mode, radix digit width and the tile-kernel occupancy cap, each
   // timed on the operators it affects; the current value stays unless a
   // candidate is more than 3 % faster.
@@ -3329,6 +3368,7 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
     e->bin_mode = bm0;
     e->radix_bits = rb0;
     e->max_wg_cap = mw0;
+    e->fuse = fu0;
   }
   vl_layer_destroy(c);
   if (target) *target = e->target;
