@@ -391,3 +391,101 @@ def test_fused_op1_ring_full_c1_layer_against_oracle(gpu_engine):
     assert len(and1) == 0 and len(ring) == 0
     prog.close()
     eng.close()
+
+
+def _check_and_cases(rng):
+    """(A, B) pairs for ANDs read only by ASSERT_ZERO: empty and non-empty
+    results, contacts that are not overlaps, boxes spanning many tiles."""
+    cases = {}
+    a = rand_boxes(rng, 3000, 1 << 16, 300)
+    cases["random_overlap"] = (a, rand_boxes(rng, 2000, 1 << 16, 300))
+    # a 5-DBU grid of 5x5 cells: A the even cells, B the odd ones (edge and
+    # corner contacts only, the AND is empty)
+    ii, jj = np.meshgrid(np.arange(120), np.arange(120))
+    cells = np.stack([ii.ravel() * 5, jj.ravel() * 5, ii.ravel() * 5 + 5, jj.ravel() * 5 + 5], 1).astype(np.int32)
+    even = ((ii + jj).ravel() % 2) == 0
+    cases["checkerboard"] = (cells[even], cells[~even])
+    # the same with one B cell grown by one unit: one overlap of area 1..2
+    b = cells[~even].copy()
+    b[777] += np.array([-1, -1, 1, 1], np.int32)
+    cases["checkerboard_one_overlap"] = (cells[even], b)
+    # long horizontal wires over sparse vias between them (empty), and one via
+    # moved onto a wire
+    wires = np.array([[0, 100 * k, 1 << 20, 100 * k + 8] for k in range(200)], np.int32)
+    vias = np.array([[rng.integers(0, 1 << 20), 100 * k + 20, 0, 0] for k in range(199)], np.int32)
+    vias[:, 2] = vias[:, 0] + 6
+    vias[:, 3] = vias[:, 1] + 6
+    cases["wires_vias"] = (wires, vias)
+    v2 = vias.copy()
+    v2[123, 1] = 100 * 123 + 5
+    v2[123, 3] = v2[123, 1] + 6
+    cases["wires_via_on_wire"] = (wires, v2)
+    cases["disjoint_extents"] = (a, a + np.int32(1 << 20))
+    cases["empty_b"] = (a, np.zeros((0, 4), np.int32))
+    return cases
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_check_and_matches_oracle(gpu_engine, graph):
+    """An AND read only by ASSERT_ZERO is planned as a semi-join emptiness test
+    (the smaller input binned, the other probed) and computed in full only
+    when some pair overlaps: the result is the oracle's AND either way, the
+    failure count says whether it is empty, and the plan without the test
+    gives the same layers."""
+    rng = np.random.default_rng(1234)
+    cases = _check_and_cases(rng)
+    for name, (a, b) in cases.items():
+        exp = O.op_and(a, b)
+        for check in (1, 0):
+            eng = V.Engine(0)
+            eng.set("and_check", check)
+            A, B = V.Layer(eng, a), V.Layer(eng, b)
+            C, D = V.Layer(eng), V.Layer(eng)
+            prog = V.Program(eng)
+            prog.add(V.K_BOOST_CONTROL_AND, A, B, C)
+            prog.add(-1, C)
+            prog.add(V.K_BOOST_CONTROL_AND, B, A, D)   # the other operand order
+            prog.add(-1, D)
+            if graph:
+                prog.set_graph(True)
+            for rep in range(3 if graph else 1):
+                prog.run()
+                assert prog.assert_failures() == 2 * int(len(exp) > 0), (name, check, rep)
+                assert np.array_equal(O.sort_boxes(C.boxes()), exp), (name, check, rep)
+                assert np.array_equal(O.sort_boxes(D.boxes()), exp), (name, check, rep)
+            prog.close()
+            eng.close()
+
+
+def test_check_and_plan_only_for_assert_reads(gpu_engine):
+    """An AND whose result feeds another operator is computed in full (the
+    emptiness plan is for checks); a re-written layer ends the reads that
+    count."""
+    rng = np.random.default_rng(99)
+    a = rand_boxes(rng, 2000, 1 << 15, 200)
+    b = rand_boxes(rng, 2000, 1 << 15, 200)
+    eng = V.Engine(0)
+    A, B, C, D = V.Layer(eng, a), V.Layer(eng, b), V.Layer(eng), V.Layer(eng)
+    prog = V.Program(eng)
+    prog.add(V.K_BOOST_CONTROL_AND, A, B, C)
+    prog.add(-1, C)
+    prog.add(V.K_BOOST_CONTROL_OR, C, A, D)
+    prog.run()
+    exp = O.op_and(a, b)
+    assert prog.assert_failures() == int(len(exp) > 0)
+    assert np.array_equal(O.sort_boxes(C.boxes()), exp)
+    assert np.array_equal(O.sort_boxes(D.boxes()), O.op_or(exp, a))
+    prog.close()
+    # C asserted, then written again and read by an operator
+    E = V.Layer(eng)
+    prog = V.Program(eng)
+    prog.add(V.K_BOOST_CONTROL_AND, A, B, C)
+    prog.add(-1, C)
+    prog.add(V.K_BOOST_CONTROL_OR, A, B, C)
+    prog.add(V.K_BOOST_CONTROL_NOT, C, A, E)
+    prog.run()
+    assert prog.assert_failures() == int(len(exp) > 0)
+    assert np.array_equal(O.sort_boxes(C.boxes()), O.op_or(a, b))
+    assert np.array_equal(O.sort_boxes(E.boxes()), O.op_not(b, a))
+    prog.close()
+    eng.close()

@@ -149,6 +149,7 @@ struct vl_engine {
   int bin_mode = 2;              // 2: atomic binning, atomics grouped per wave (1: per run); 0: radix sort
   int radix_bits = 8;            // digit width of the radix sorts (8 or 11)
   int workers = 4;               // programs: chains run concurrently on up to this many worker streams
+  int and_check = 1;             // programs: ANDs read only by ASSERT_ZERO are tested for emptiness first
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   // Strips per binned entry that a tile stage's scratch stream is sized for
   // (plain tile stage / each output of a fused group).  Raised when a stream
@@ -1532,6 +1533,50 @@ int binary(vl_engine* e, int top, const vl_layer* a, const vl_layer* b, vl_layer
   return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr, 0, &ka, b ? &kb : nullptr);
 }
 
+// C = A AND B planned for a result that is expected to be empty (an AND read
+// only by ASSERT_ZERO, a DRC check): the smaller input is binned alone and
+// every box of the other is tested against the binned boxes of the tiles it
+// meets (k_and_probe).  No positive-area overlap: C is empty and nothing else
+// runs.  Otherwise the full AND computes C, so C is A AND B either way.
+int and_check(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
+  VLCK(check_layers(e, a, b, c));
+  if (!b) return fail(VL_EINVAL, "null layer");
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  HIPCK(hipSetDevice(e->device));
+  {
+    OpTimer tm(e);
+    Box64 ka, kb;
+    VLCK(layer_mbr(e, a, &ka));
+    VLCK(layer_mbr(e, b, &kb));
+    const Box64 dom{std::max(ka.x1, kb.x1), std::max(ka.y1, kb.y1), std::min(ka.x2, kb.x2), std::min(ka.y2, kb.y2)};
+    if (a->n == 0 || b->n == 0 || dom.empty()) return set_empty(e, c);
+    const vl_layer* build = a->n <= b->n ? a : b;
+    const vl_layer* probe = build == a ? b : a;
+    Grid g = make_grid(dom, build->n, e->target);
+    Scratch keep(e);
+    vl_box* entries = nullptr;
+    uint64_t* start = nullptr;
+    uint64_t total = 0;
+    VLCK(bin_entries(e, g, build->d, build->n, Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0, keep, &entries,
+                     &start, &total));
+    uint32_t* hit = keep.get<uint32_t>(1);
+    if (!hit) return fail(VL_ENOMEM, "scratch");
+    hipEvent_t pa = prof_mark(e);
+    launch_fill_u32(hit, 0u, 1, e->st);
+    launch_and_probe(probe->d, probe->n, g, entries, start, hit, e->st);
+    HIPCK(hipGetLastError());
+    hipEvent_t pb = prof_mark(e);
+    // algorithmic bytes: every probe box and every binned entry read once
+    prof_add(e, pa, pb, KS_TILE, 1, 16.0 * (double)probe->n + 16.0 * (double)total);
+    uint32_t h = 0;
+    VLCK(readback(e, &h, hit, sizeof(h)));
+    if (g_debug) fprintf(stderr, "[vl] and_check build=%lld probe=%lld entries=%llu hit=%u\n", (long long)build->n,
+                         (long long)probe->n, (unsigned long long)total, h);
+    if (h == 0) return set_empty(e, c);
+  }
+  return vl_and(e, a, b, c);
+}
+
 bool is_device_ptr(const void* p) {
   hipPointerAttribute_t at;
   memset(&at, 0, sizeof(at));
@@ -1691,6 +1736,8 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "radix_bits") {
     if (v != 8 && v != 11) return fail(VL_EINVAL, "radix_bits must be 8 or 11");
     e->radix_bits = (int)v;
+  } else if (k == "and_check") {
+    e->and_check = v != 0;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
   }
@@ -1711,6 +1758,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "bin_mode") *v = e->bin_mode;
   else if (k == "radix_bits") *v = e->radix_bits;
   else if (k == "workers") *v = e->workers;
+  else if (k == "and_check") *v = e->and_check;
   else if (k == "fused_groups") *v = e->fused_groups;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
@@ -2345,6 +2393,7 @@ int vl_program_add(vl_program* p, int op, const vl_layer* a, const vl_layer* b, 
 struct ProgStep {
   std::vector<int> ops;
   bool fused = false;
+  bool check = false;   // a single AND read only by ASSERT_ZERO: and_check
   const vl_layer* src = nullptr;
   FusedSpec fs{};
   vl_layer* outs[FUSE_MAX_OUT] = {};
@@ -2459,7 +2508,7 @@ struct GroupBuilder {
 
 // fuse: 0 no groups, 1 groups of up to two morphological images, 2 groups of
 // at most one (smaller compressed grids per tile, one more binning of S).
-static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int fuse) {
+static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int fuse, bool check_ands) {
   std::vector<ProgStep> steps;
   const int max_images = fuse == 2 ? 1 : 2;
   GroupBuilder g;
@@ -2502,6 +2551,25 @@ static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int
     single(i);
   }
   flush();
+  // ANDs whose result is read only by ASSERT_ZERO until the layer is written
+  // again (checks expected to pass) are planned as emptiness tests first
+  if (check_ands) {
+    for (ProgStep& st : steps) {
+      if (st.fused || st.ops.size() != 1 || ops[st.ops[0]].op != VL_OP_AND) continue;
+      const int i = st.ops[0];
+      const vl_layer* c = ops[i].c;
+      bool only_asserts = true, read = false;
+      for (int j = i + 1; j < (int)ops.size() && only_asserts; j++) {
+        const ProgOp& o = ops[j];
+        if (o.a == c || o.b == c) {
+          if (o.op == -1) read = true;
+          else only_asserts = false;
+        }
+        if (o.c == c) break;
+      }
+      st.check = only_asserts && read;
+    }
+  }
   return steps;
 }
 
@@ -2522,6 +2590,10 @@ static int run_step(vl_engine* e, const std::vector<ProgOp>& ops, const ProgStep
     }
     if (rc != FUSE_DECLINED) return rc;
     g_err.clear();
+  }
+  if (st.check) {
+    const ProgOp& o = ops[st.ops[0]];
+    return and_check(e, o.a, o.b, o.c);
   }
   for (int i : st.ops) VLCK(run_prog_op(e, ops[i], failures));
   return VL_OK;
@@ -2801,7 +2873,7 @@ int vl_program_run(vl_program* p) {
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
   p->failures = 0;
-  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse);
+  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse, e->and_check != 0);
   const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
   int nw = (int)std::min<size_t>(chains.size(), (size_t)std::min(p->max_workers, p->e->workers));
   // the bounding boxes of the program's inputs (layers no operator writes)

@@ -204,6 +204,11 @@ void launch_bin_scatter(const vl_box* a, int64_t na, Xform xra, Xform xsa, const
 // the tile is too large to test); the others get tab_n[t] = tab_n[ntab+t] = 0.
 void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, uint32_t* list, uint32_t* list_n,
                        uint32_t* tab_n, int ntab, hipStream_t st);
+// AND emptiness by semi-join: entries/start hold one input binned alone into g
+// (layer-A keys); *hit (cleared by the caller) is set to 1 when some box of
+// p[0..n) overlaps a binned box with positive area.
+void launch_and_probe(const vl_box* p, int64_t n, Grid g, const vl_box* entries, const uint64_t* start,
+                      uint32_t* hit, hipStream_t st);
 // The same scatter writing box indices (idx) instead of boxes (one layer).
 void launch_bin_scatter_idx(const vl_box* a, int64_t na, Xform xra, Grid g, const uint64_t* start, uint32_t* cnt,
                             uint32_t* idx, bool grouped, hipStream_t st);

@@ -500,6 +500,42 @@ __global__ __launch_bounds__(256) void k_and_filter(const TileDesc* __restrict__
   }
 }
 
+// Semi-join test of an AND: the boxes of one input were binned alone into g
+// (tile t's boxes at entries[start[2t] .. start[2t+1])); a lane takes a box
+// of the other input and tests it against the binned boxes of every tile its
+// interior meets.  A positive-area overlap of any pair lies inside a tile
+// both boxes meet, so *hit stays 0 exactly when the AND is empty.  Inputs in
+// spatial order keep neighbouring lanes on the same tiles (cache hits).
+constexpr int PROBE_U = 4;   // binned boxes loaded per inner iteration
+
+__global__ __launch_bounds__(256) void k_and_probe(const vl_box* __restrict__ p, int64_t n, Grid g,
+                                                   const vl_box* __restrict__ entries,
+                                                   const uint64_t* __restrict__ start, uint32_t* __restrict__ hit) {
+  const GridDiv gd = grid_div(g);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool any = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const vl_box q = p[i];
+    int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
+    if (any || !tile_range(q, Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2)) continue;
+    for (int r = r1; r <= r2 && !any; r++) {
+      for (int c = c1; c <= c2 && !any; c++) {
+        const uint64_t t = (uint64_t)r * (uint64_t)g.cols + (uint64_t)c;
+        const uint64_t s0 = start[2 * t], s1 = start[2 * t + 1];
+        for (uint64_t j = s0; j < s1; j += PROBE_U) {
+          vl_box b[PROBE_U];
+#pragma unroll
+          for (int u = 0; u < PROBE_U; u++) b[u] = j + u < s1 ? entries[j + u] : vl_box{0, 0, 0, 0};
+#pragma unroll
+          for (int u = 0; u < PROBE_U; u++)
+            any |= max(q.x1, b[u].x1) < min(q.x2, b[u].x2) && max(q.y1, b[u].y1) < min(q.y2, b[u].y2);
+        }
+      }
+    }
+  }
+  if (__ballot(any) != 0ull && __lane_id() == 0) gstore(hit, 1u);
+}
+
 __global__ void k_build_desc(Grid g, const uint64_t* __restrict__ start, int32_t op, TileDesc* desc) {
   const int64_t nt = (int64_t)g.cols * g.rows;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
@@ -1494,6 +1530,11 @@ void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, 
   if (ntiles > 0)
     hipLaunchKernelGGL(k_and_filter, dim3(grid_for(ntiles, 16)), dim3(256), 0, st, desc, ntiles, entries, list,
                        list_n, tab_n, ntab);
+}
+void launch_and_probe(const vl_box* p, int64_t n, Grid g, const vl_box* entries, const uint64_t* start,
+                      uint32_t* hit, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_and_probe, dim3(grid_for(n)), dim3(256), 0, st, p, n, g, entries, start, hit);
 }
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st) {
   int64_t nt = (int64_t)g.cols * g.rows;
