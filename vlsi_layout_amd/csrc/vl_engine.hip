@@ -150,6 +150,7 @@ struct vl_engine {
   int radix_bits = 8;            // digit width of the radix sorts (8 or 11)
   int workers = 4;               // programs: chains run concurrently on up to this many worker streams
   int and_check = 1;             // programs: ANDs read only by ASSERT_ZERO are tested for emptiness first
+  int check_target = 6;          // mean binned boxes per tile of that test's grid
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   // Strips per binned entry that a tile stage's scratch stream is sized for
   // (plain tile stage / each output of a fused group).  Raised when a stream
@@ -1552,7 +1553,7 @@ int and_check(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
     if (a->n == 0 || b->n == 0 || dom.empty()) return set_empty(e, c);
     const vl_layer* build = a->n <= b->n ? a : b;
     const vl_layer* probe = build == a ? b : a;
-    Grid g = make_grid(dom, build->n, e->target);
+    Grid g = make_grid(dom, build->n, e->check_target);
     Scratch keep(e);
     vl_box* entries = nullptr;
     uint64_t* start = nullptr;
@@ -1738,6 +1739,9 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
     e->radix_bits = (int)v;
   } else if (k == "and_check") {
     e->and_check = v != 0;
+  } else if (k == "check_target") {
+    if (v < 1 || v > 4096) return fail(VL_EINVAL, "check_target out of range");
+    e->check_target = (int)v;
   } else {
     return fail(VL_EINVAL, "unknown parameter '%s'", key);
   }
@@ -1759,6 +1763,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "radix_bits") *v = e->radix_bits;
   else if (k == "workers") *v = e->workers;
   else if (k == "and_check") *v = e->and_check;
+  else if (k == "check_target") *v = e->check_target;
   else if (k == "fused_groups") *v = e->fused_groups;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
@@ -2656,6 +2661,7 @@ static int program_workers(vl_program* p, int n) {
     w->fuse_target = e->fuse_target;
     w->bin_mode = e->bin_mode;
     w->radix_bits = e->radix_bits;
+    w->check_target = e->check_target;
     w->region_scale = std::max(w->region_scale, e->region_scale);
     w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
     (void)prof_resolve(w);

@@ -505,8 +505,14 @@ __global__ __launch_bounds__(256) void k_and_filter(const TileDesc* __restrict__
 // of the other input and tests it against the binned boxes of every tile its
 // interior meets.  A positive-area overlap of any pair lies inside a tile
 // both boxes meet, so *hit stays 0 exactly when the AND is empty.  Inputs in
-// spatial order keep neighbouring lanes on the same tiles (cache hits).
-constexpr int PROBE_U = 4;   // binned boxes loaded per inner iteration
+// spatial order keep neighbouring lanes on the same tiles (cache hits); the
+// binned boxes of a tile are loaded PROBE_U at a time (one memory round trip
+// for most tiles) and the lane's next box is loaded before the walk.
+constexpr int PROBE_U = 8;
+
+__device__ __forceinline__ bool overlaps(const vl_box& q, const vl_box& b) {
+  return max(q.x1, b.x1) < min(q.x2, b.x2) && max(q.y1, b.y1) < min(q.y2, b.y2);
+}
 
 __global__ __launch_bounds__(256) void k_and_probe(const vl_box* __restrict__ p, int64_t n, Grid g,
                                                    const vl_box* __restrict__ entries,
@@ -514,8 +520,11 @@ __global__ __launch_bounds__(256) void k_and_probe(const vl_box* __restrict__ p,
   const GridDiv gd = grid_div(g);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   bool any = false;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const vl_box q = p[i];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  vl_box nq = i < n ? p[i] : vl_box{0, 0, 0, 0};
+  for (; i < n; i += stride) {
+    const vl_box q = nq;
+    if (i + stride < n) nq = p[i + stride];
     int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
     if (any || !tile_range(q, Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2)) continue;
     for (int r = r1; r <= r2 && !any; r++) {
@@ -527,8 +536,7 @@ __global__ __launch_bounds__(256) void k_and_probe(const vl_box* __restrict__ p,
 #pragma unroll
           for (int u = 0; u < PROBE_U; u++) b[u] = j + u < s1 ? entries[j + u] : vl_box{0, 0, 0, 0};
 #pragma unroll
-          for (int u = 0; u < PROBE_U; u++)
-            any |= max(q.x1, b[u].x1) < min(q.x2, b[u].x2) && max(q.y1, b[u].y1) < min(q.y2, b[u].y2);
+          for (int u = 0; u < PROBE_U; u++) any |= overlaps(q, b[u]);
         }
       }
     }
