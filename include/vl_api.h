@@ -334,8 +334,10 @@ int vl_program_graph_stats(vl_program* p, int64_t* captures, int64_t* replays, i
 /* ---- serving: windowed queries against resident layers --------------------- */
 /* No counterpart in the reference (its control file runs one operator at a
  * time).  A server indexes its layers once (boxes binned into a uniform cell
- * grid, cell starts kept on the host too) and answers windowed DRC-style
- * queries  R = GROW_s(A ∩ W) AND (B ∩ W)  with continuous batching: requests
+ * grid whose cells are numbered in Morton order, cell starts kept on the host
+ * too) and answers windowed DRC-style queries  R = GROW_s(A ∩ W) AND (B ∩ W)
+ * (R is tested for emptiness before it is computed in full, as for an AND
+ * read only by ASSERT_ZERO) with continuous batching: requests
  * are released at their arrival times, admitted in arrival order whenever a
  * slot is free and the in-flight memory estimate stays within the HBM budget
  * (a request is always admitted when nothing else is in flight), and run
@@ -352,7 +354,7 @@ typedef struct vl_query {
 typedef struct vl_query_result {
   int32_t status;             /* VL_OK or the failure code */
   int32_t slot;               /* slot that ran it */
-  int64_t in_boxes;           /* boxes of A ∩ W plus B ∩ W (a box counts once per index cell it meets) */
+  int64_t in_boxes;           /* boxes of A plus boxes of B with positive area inside W, each counted once */
   int64_t count;              /* boxes of R (canonical maximal strips): 0 = ASSERT_ZERO holds */
   uint64_t hash;              /* order-independent hash of R's boxes */
   double start_ms, end_ms;    /* admission and completion, relative to the start of the run */

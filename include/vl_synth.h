@@ -19,6 +19,11 @@
  *   VL_SYN_CHIP     full-chip routing: [0,2^26)^2, 70% tracks (w=8, length
  *                   log-uniform 10..1e5, 5-DBU grid; layer 0 vertical, 1
  *                   horizontal), 30% 6x6 vias.
+ *   VL_SYN_ROUTE    spacing-check design: layers 0/1 the wires (w=8, length
+ *                   log-uniform 20..5000) of the even/odd horizontal tracks of
+ *                   a 14-DBU pitch (6 DBU between the two layers), 2/3 the
+ *                   same vertically, over [0,2^20); one wire in 4096 jogs 2 DBU
+ *                   towards its neighbour track (a 5-DBU spacing violation).
  */
 #ifndef VL_SYNTH_H
 #define VL_SYNTH_H
@@ -33,6 +38,7 @@ extern "C" {
 #define VL_SYN_STDCELL 2
 #define VL_SYN_TILES   3
 #define VL_SYN_CHIP    4
+#define VL_SYN_ROUTE   5
 
 /* Writes exactly n boxes of the given layer into out. */
 int vl_synth_layer(int config, uint64_t seed, int layer, int64_t n, vl_box* out);

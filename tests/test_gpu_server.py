@@ -119,3 +119,44 @@ def test_bad_query_reports_status(design):
     assert res["status"].tolist() == [0, V.VL_EINVAL, V.VL_EINVAL]
     assert (res["count"][0], res["hash"][0]) == expected(boxes, q[0])
     srv.close()
+
+
+@pytest.fixture(scope="module")
+def route_design(gpu_engine):
+    eng = V.Engine(0)
+    boxes = [V.synth_layer(V.SYN_ROUTE, 0x5EED0006, k, 60000) for k in range(4)]
+    layers = [V.Layer(eng, b) for b in boxes]
+    yield eng, boxes, layers
+    eng.close()
+
+
+def test_spacing_queries_on_routing_design(route_design):
+    """DRC spacing queries between the two layers of one metal (neighbouring
+    tracks 6 DBU apart, a few wires jogged to 4 DBU): every result equals the
+    oracle's, in_boxes counts each box with area inside the window once (the
+    index keeps one copy of a box that spans several cells), checks at 1 and
+    2 DBU always pass, and some at 5 DBU fail (the jogs)."""
+    eng, boxes, layers = route_design
+    rng = np.random.default_rng(11)
+    allb = np.concatenate(boxes)
+    X0, Y0, X1, Y1 = allb[:, 0].min(), allb[:, 1].min(), allb[:, 2].max(), allb[:, 3].max()
+    n = 48
+    q = np.zeros(n, V.QUERY_DTYPE)
+    for i in range(n):
+        sx = int(rng.uniform(0.05, 0.6) * (X1 - X0))
+        sy = int(rng.uniform(0.05, 0.6) * (Y1 - Y0))
+        x = int(rng.integers(X0 - sx // 4, X1 - sx // 2))
+        y = int(rng.integers(Y0 - sy // 4, Y1 - sy // 2))
+        la = int(rng.integers(0, 4))
+        q[i] = (x, y, x + sx, y + sy, la, la ^ 1, (1, 2, 5)[i % 3], 0, 0.3 * i)
+    srv = V.Server(eng, layers, slots=4)
+    rc, res = srv.run(q)
+    assert rc == 0 and (res["status"] == 0).all()
+    for i in range(n):
+        w = (q["x1"][i], q["y1"][i], q["x2"][i], q["y2"][i])
+        assert res["in_boxes"][i] == len(clip(boxes[q["layer_a"][i]], w)) + len(clip(boxes[q["layer_b"][i]], w)), i
+        cnt, h = expected(boxes, q[i])
+        assert res["count"][i] == cnt and res["hash"][i] == h, i
+    assert (res["count"][q["spacing"] < 5] == 0).all()
+    assert (res["count"][q["spacing"] == 5] > 0).any()
+    srv.close()

@@ -50,7 +50,7 @@ VL_OP_GROW = 6
 
 VL_OK, VL_EINVAL, VL_ENOSPC, VL_ENOMEM, VL_EHIP, VL_EASSERT, VL_EINTERNAL = 0, 1, 2, 3, 4, 5, 7
 
-SYN_UNIFORM, SYN_STDCELL, SYN_TILES, SYN_CHIP = 1, 2, 3, 4
+SYN_UNIFORM, SYN_STDCELL, SYN_TILES, SYN_CHIP, SYN_ROUTE = 1, 2, 3, 4, 5
 
 # vl_event_sort_noalloc's static scratch bound (include/vl_api.h)
 VL_NOALLOC_MAX_TRACKERS = 3500000

@@ -947,7 +947,7 @@ constexpr int BIN_TOO_MANY = 1003;   // bin_entries without coarsening: the grid
 int bin_entries(vl_engine* e, Grid& g, const vl_box* a, int64_t na, Xform xs, Xform xr, const vl_box* b, int64_t nb,
                 Scratch& keep, vl_box** entries_out, uint64_t** start_out, uint64_t* total_out,
                 bool coarsen = true) {
-  const int64_t nt0 = (int64_t)g.cols * g.rows;
+  const int64_t nt0 = grid_tiles(g);
   const int64_t n = na + nb;
   if (e->bin_mode >= 1) {
     // atomic binning: count per (tile, layer) key, scan, scatter (bin_mode 2:
@@ -961,13 +961,13 @@ int bin_entries(vl_engine* e, Grid& g, const vl_box* a, int64_t na, Xform xs, Xf
     uint64_t total = 0;
     int64_t nt = nt0;
     for (;;) {   // coarser grids have fewer tiles: the arrays of the first fit
-      nt = (int64_t)g.cols * g.rows;
+      nt = grid_tiles(g);
       launch_fill_u32(cnt, 0u, 2 * (size_t)nt, e->st);
       launch_bin_count(a, na, xr, b, nb, g, cnt, e->bin_mode == 2, e->st);
       scan_u32_to_u64(cnt, start, 2 * (size_t)nt, start + 2 * nt, stmp, e->st);
       VLCK(readback(e, &total, start + 2 * nt, sizeof(total)));
       if (total <= bin_entry_limit(n)) break;
-      if (!coarsen) return BIN_TOO_MANY;
+      if (!coarsen || g.zb) return BIN_TOO_MANY;
       if (g_debug) fprintf(stderr, "[vl]  %llu entries for %lld boxes: coarser grid\n", (unsigned long long)total,
                            (long long)n);
       if (!coarsen_grid(g)) break;
@@ -997,10 +997,10 @@ int bin_entries(vl_engine* e, Grid& g, const vl_box* a, int64_t na, Xform xs, Xf
     scan_u32_to_u64(cnt, pos, (size_t)n, pos + n, stmp, e->st);
     VLCK(readback(e, &total, pos + n, sizeof(total)));
     if (total <= bin_entry_limit(n)) break;
-    if (!coarsen) return BIN_TOO_MANY;
+    if (!coarsen || g.zb) return BIN_TOO_MANY;
     if (!coarsen_grid(g)) break;
   }
-  const int64_t nt = (int64_t)g.cols * g.rows;
+  const int64_t nt = grid_tiles(g);
   Scratch s2(e);
   uint32_t* keys = s2.get<uint32_t>(total + 1);
   uint32_t* vals = s2.get<uint32_t>(total + 1);
@@ -1534,48 +1534,107 @@ int binary(vl_engine* e, int top, const vl_layer* a, const vl_layer* b, vl_layer
   return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr, 0, &ka, b ? &kb : nullptr);
 }
 
-// C = A AND B planned for a result that is expected to be empty (an AND read
-// only by ASSERT_ZERO, a DRC check): the smaller input is binned alone and
-// every box of the other is tested against the binned boxes of the tiles it
-// meets (k_and_probe).  No positive-area overlap: C is empty and nothing else
-// runs.  Otherwise the full AND computes C, so C is A AND B either way.
+// One input of a semi-join: boxes and the transform applied to them.
+struct Side {
+  const vl_box* d;
+  int64_t n;
+  Xform x;
+};
+
+// The boxes of p (transformed) that overlap a box of the binned input
+// (entries/start over grid g) with positive area, compacted, transformed,
+// into *out (allocated from keep); *m = their number.
+int probe_reduce(vl_engine* e, const Side& p, const Grid& g, const vl_box* entries, const uint64_t* start,
+                 Scratch& keep, vl_box** out, uint64_t* m) {
+  Scratch s(e);
+  uint32_t* hit = s.get<uint32_t>(1);
+  uint32_t* mark = s.get<uint32_t>((size_t)p.n + 1);
+  uint32_t* pos = s.get<uint32_t>((size_t)p.n + 1);
+  void* stmp = s.get<char>(scan_tmp_bytes((size_t)p.n + 1));
+  if (s.oom) return fail(VL_ENOMEM, "semi-join scratch");
+  launch_fill_u32(hit, 0u, 1, e->st);
+  launch_and_probe(p.d, p.n, p.x, g, entries, start, hit, mark, e->st);
+  scan_exclusive_u32(mark, pos, (size_t)p.n, pos + p.n, stmp, e->st);
+  uint32_t k = 0;
+  VLCK(readback(e, &k, pos + p.n, sizeof(k)));
+  *m = k;
+  *out = keep.get<vl_box>((size_t)k + 1);
+  if (!*out) return fail(VL_ENOMEM, "semi-join output");
+  launch_compact_marked(p.d, p.n, p.x, mark, pos, *out, e->st);
+  HIPCK(hipGetLastError());
+  return VL_OK;
+}
+
+// C = A AND B (A's boxes transformed by A.x: a GROW of A's union) planned for
+// a result that is expected to be empty (an AND read only by ASSERT_ZERO, a
+// DRC check); ma, mb: bounding boxes of the transformed inputs.  The smaller
+// input is binned alone and every box of the other is tested against the
+// binned boxes of the tiles it meets (k_and_probe).  No positive-area
+// overlap: C is empty and nothing else runs.  Otherwise the inputs are
+// reduced to the boxes that take part in an overlap (the probed boxes that
+// overlap a binned one, then the binned input's boxes that overlap one of
+// those) and the AND of the reduced inputs, which is the same point set, is
+// computed in full: a failing check costs a few passes over the inputs plus
+// an AND of the violating boxes.  The caller holds the engine.
+int and_check_core(vl_engine* e, const Side& A, const Side& B, const Box64& ma, const Box64& mb, vl_layer* c) {
+  const Box64 dom{std::max(ma.x1, mb.x1), std::max(ma.y1, mb.y1), std::min(ma.x2, mb.x2), std::min(ma.y2, mb.y2)};
+  if (A.n == 0 || B.n == 0 || dom.empty()) return set_empty(e, c);
+  const bool build_a = A.n <= B.n;
+  const Side& build = build_a ? A : B;
+  const Side& probe = build_a ? B : A;
+  Grid g = make_grid(dom, build.n, e->check_target);
+  Scratch keep(e);
+  vl_box* entries = nullptr;
+  uint64_t* start = nullptr;
+  uint64_t total = 0;
+  VLCK(bin_entries(e, g, build.d, build.n, build.x, build.x, nullptr, 0, keep, &entries, &start, &total));
+  uint32_t* hit = keep.get<uint32_t>(1);
+  if (!hit) return fail(VL_ENOMEM, "scratch");
+  hipEvent_t pa = prof_mark(e);
+  launch_fill_u32(hit, 0u, 1, e->st);
+  launch_and_probe(probe.d, probe.n, probe.x, g, entries, start, hit, nullptr, e->st);
+  HIPCK(hipGetLastError());
+  hipEvent_t pb = prof_mark(e);
+  // algorithmic bytes: every probe box and every binned entry read once
+  prof_add(e, pa, pb, KS_TILE, 1, 16.0 * (double)probe.n + 16.0 * (double)total);
+  uint32_t h = 0;
+  VLCK(readback(e, &h, hit, sizeof(h)));
+  if (g_debug) fprintf(stderr, "[vl] and_check build=%lld probe=%lld entries=%llu hit=%u\n", (long long)build.n,
+                       (long long)probe.n, (unsigned long long)total, h);
+  if (h == 0) return set_empty(e, c);
+  // semi-join reduction in both directions (the reduced boxes carry their
+  // transform)
+  vl_box* p2 = nullptr;
+  uint64_t np2 = 0;
+  VLCK(probe_reduce(e, probe, g, entries, start, keep, &p2, &np2));
+  Grid g2 = make_grid(dom, (int64_t)np2, e->check_target);
+  vl_box* entries2 = nullptr;
+  uint64_t* start2 = nullptr;
+  uint64_t total2 = 0;
+  VLCK(bin_entries(e, g2, p2, (int64_t)np2, Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0, keep, &entries2,
+                   &start2, &total2));
+  vl_box* b2 = nullptr;
+  uint64_t nb2 = 0;
+  VLCK(probe_reduce(e, build, g2, entries2, start2, keep, &b2, &nb2));
+  if (g_debug) fprintf(stderr, "[vl] and_check reduced to %llu x %llu boxes\n", (unsigned long long)np2,
+                       (unsigned long long)nb2);
+  const vl_box* ra = build_a ? b2 : p2;
+  const vl_box* rb = build_a ? p2 : b2;
+  const int64_t na = (int64_t)(build_a ? nb2 : np2), nb = (int64_t)(build_a ? np2 : nb2);
+  return run_op(e, TOP_AND, ra, na, Xform{0, 0, 0, 0}, rb, nb, c, nullptr);
+}
+
+// and_check_core on two layers (no transform; the layers' cached bounding boxes).
 int and_check(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
   VLCK(check_layers(e, a, b, c));
   if (!b) return fail(VL_EINVAL, "null layer");
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
-  {
-    OpTimer tm(e);
-    Box64 ka, kb;
-    VLCK(layer_mbr(e, a, &ka));
-    VLCK(layer_mbr(e, b, &kb));
-    const Box64 dom{std::max(ka.x1, kb.x1), std::max(ka.y1, kb.y1), std::min(ka.x2, kb.x2), std::min(ka.y2, kb.y2)};
-    if (a->n == 0 || b->n == 0 || dom.empty()) return set_empty(e, c);
-    const vl_layer* build = a->n <= b->n ? a : b;
-    const vl_layer* probe = build == a ? b : a;
-    Grid g = make_grid(dom, build->n, e->check_target);
-    Scratch keep(e);
-    vl_box* entries = nullptr;
-    uint64_t* start = nullptr;
-    uint64_t total = 0;
-    VLCK(bin_entries(e, g, build->d, build->n, Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0, keep, &entries,
-                     &start, &total));
-    uint32_t* hit = keep.get<uint32_t>(1);
-    if (!hit) return fail(VL_ENOMEM, "scratch");
-    hipEvent_t pa = prof_mark(e);
-    launch_fill_u32(hit, 0u, 1, e->st);
-    launch_and_probe(probe->d, probe->n, g, entries, start, hit, e->st);
-    HIPCK(hipGetLastError());
-    hipEvent_t pb = prof_mark(e);
-    // algorithmic bytes: every probe box and every binned entry read once
-    prof_add(e, pa, pb, KS_TILE, 1, 16.0 * (double)probe->n + 16.0 * (double)total);
-    uint32_t h = 0;
-    VLCK(readback(e, &h, hit, sizeof(h)));
-    if (g_debug) fprintf(stderr, "[vl] and_check build=%lld probe=%lld entries=%llu hit=%u\n", (long long)build->n,
-                         (long long)probe->n, (unsigned long long)total, h);
-    if (h == 0) return set_empty(e, c);
-  }
-  return vl_and(e, a, b, c);
+  OpTimer tm(e);
+  Box64 ka, kb;
+  VLCK(layer_mbr(e, a, &ka));
+  VLCK(layer_mbr(e, b, &kb));
+  return and_check_core(e, Side{a->d, a->n, Xform{0, 0, 0, 0}}, Side{b->d, b->n, Xform{0, 0, 0, 0}}, ka, kb, c);
 }
 
 bool is_device_ptr(const void* p) {
@@ -2976,7 +3035,10 @@ int vl_program_assert_count(vl_program* p, int64_t* failures) {
 // ---- query server --------------------------------------------------------------
 namespace {
 
-// A layer binned into a uniform cell grid; cell starts also on the host.
+// A layer binned into a uniform cell grid whose cells are numbered in Morton
+// (Z) order, so a window's cells form few long runs of entries and nearby
+// cells sit together in memory (row-major numbering when the Morton square
+// would be far larger than the grid); cell starts also on the host.
 struct CellIndex {
   Grid g{};
   vl_box* entries = nullptr;
@@ -2986,7 +3048,30 @@ struct CellIndex {
   Scratch* keep = nullptr;
 };
 
-// Entry ranges of the cell rows a window covers (one contiguous range per row).
+// Morton-number runs [z0, z1) of the cells of columns c1..c2 and rows r1..r2:
+// the quadtree over the 2^level square at (r0, c0), numbers from zbase, is
+// taken whole when the window holds it and split into its four quadrants when
+// the window cuts it.  Runs come out in increasing order; adjacent ones merge.
+void zruns(int level, uint32_t zbase, int r0, int c0, int c1, int c2, int r1, int r2,
+           std::vector<std::pair<uint32_t, uint32_t>>& out) {
+  const int size = 1 << level;
+  if (c0 > c2 || c0 + size - 1 < c1 || r0 > r2 || r0 + size - 1 < r1) return;
+  if (c0 >= c1 && c0 + size - 1 <= c2 && r0 >= r1 && r0 + size - 1 <= r2) {
+    const uint32_t z1 = zbase + (uint32_t)size * (uint32_t)size;
+    if (!out.empty() && out.back().second == zbase) out.back().second = z1;
+    else out.push_back({zbase, z1});
+    return;
+  }
+  const int h = size >> 1;
+  const uint32_t q = (uint32_t)h * (uint32_t)h;
+  zruns(level - 1, zbase, r0, c0, c1, c2, r1, r2, out);
+  zruns(level - 1, zbase + q, r0, c0 + h, c1, c2, r1, r2, out);
+  zruns(level - 1, zbase + 2 * q, r0 + h, c0, c1, c2, r1, r2, out);
+  zruns(level - 1, zbase + 3 * q, r0 + h, c0 + h, c1, c2, r1, r2, out);
+}
+
+// Entry ranges of the cells a window covers: one per Morton run (one per cell
+// row with row-major numbering).
 void window_ranges(const CellIndex& ix, const vl_box& w, std::vector<uint64_t>& rbeg, std::vector<uint64_t>& roff) {
   rbeg.clear();
   roff.assign(1, 0);
@@ -2997,13 +3082,21 @@ void window_ranges(const CellIndex& ix, const vl_box& w, std::vector<uint64_t>& 
   if (x1 >= x2 || y1 >= y2) return;
   const int c1 = (int)((x1 - g.x0) / g.tw), c2 = (int)((x2 - 1 - g.x0) / g.tw);
   const int r1 = (int)((y1 - g.y0) / g.th), r2 = (int)((y2 - 1 - g.y0) / g.th);
-  for (int r = r1; r <= r2; r++) {
-    const int64_t t1 = (int64_t)r * g.cols + c1, t2 = (int64_t)r * g.cols + c2;
-    const uint64_t b = ix.start[2 * t1], e = ix.start[2 * t2 + 2];
+  auto add = [&](uint64_t b, uint64_t e) {
     if (e > b) {
       rbeg.push_back(b);
       roff.push_back(roff.back() + (e - b));
     }
+  };
+  if (g.zb) {
+    std::vector<std::pair<uint32_t, uint32_t>> runs;
+    zruns(g.zb, 0u, 0, 0, c1, c2, r1, r2, runs);
+    for (const auto& z : runs) add(ix.start[2 * (uint64_t)z.first], ix.start[2 * (uint64_t)z.second]);
+    return;
+  }
+  for (int r = r1; r <= r2; r++) {
+    const int64_t t1 = (int64_t)r * g.cols + c1, t2 = (int64_t)r * g.cols + c2;
+    add(ix.start[2 * t1], ix.start[2 * t2 + 2]);
   }
 }
 
@@ -3024,7 +3117,7 @@ int window_extract(vl_engine* e, const CellIndex& ix, const vl_box& w, vl_layer*
   if (s.oom) return fail(VL_ENOMEM, "window scratch");
   HIPCK(hipMemcpyAsync(d_rbeg, rbeg.data(), nr * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
   HIPCK(hipMemcpyAsync(d_roff, roff.data(), (nr + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, e->st));
-  launch_window_clip(ix.entries, d_rbeg, d_roff, nr, n, w, clip, keep, e->st);
+  launch_window_clip(ix.entries, d_rbeg, d_roff, nr, n, w, ix.d_start, ix.g, clip, keep, e->st);
   scan_exclusive_u32(keep, pos, n, pos + n, stmp, e->st);
   uint32_t m = 0;
   VLCK(readback(e, &m, pos + n, sizeof(m)));
@@ -3069,15 +3162,25 @@ int run_query(vl_server* sv, vl_engine* w, const vl_query& q, vl_query_result& r
     return fail(VL_EINVAL, "query layer index out of range");
   if (q.spacing < 0) return fail(VL_EINVAL, "negative spacing");
   if (q.window.x1 >= q.window.x2 || q.window.y1 >= q.window.y2) return fail(VL_EINVAL, "invalid window");
-  vl_layer A, B, G, R;
-  A.e = B.e = G.e = R.e = sv->e;
+  vl_layer A, B, R;
+  A.e = B.e = R.e = sv->e;
   int rc = window_extract(w, sv->index[q.layer_a], q.window, &A);
   if (rc == VL_OK) rc = window_extract(w, sv->index[q.layer_b], q.window, &B);
   r.in_boxes = A.n + B.n;
   const int32_t s = q.spacing;
-  if (rc == VL_OK) rc = run_op(w, TOP_UNION, A.d, A.n, Xform{s, s, s, s}, nullptr, 0, &G, nullptr);
-  if (rc == VL_OK) rc = run_op(w, TOP_AND, G.d, G.n, Xform{0, 0, 0, 0}, B.d, B.n, &R, nullptr);
+  // R feeds ASSERT_ZERO: the grown boxes of A are tested against B for an
+  // overlap first (GROW of a union is the union of the grown boxes), and R is
+  // computed only from the boxes that take part in one
+  Box64 ma, mb;
+  if (rc == VL_OK && A.n > 0 && B.n > 0) rc = mbr2(w, A.d, A.n, Xform{s, s, s, s}, B.d, B.n, &ma, &mb);
   if (rc == VL_OK) {
+    if (A.n == 0 || B.n == 0) rc = set_empty(w, &R);
+    else rc = and_check_core(w, Side{A.d, A.n, Xform{s, s, s, s}}, Side{B.d, B.n, Xform{0, 0, 0, 0}}, ma, mb, &R);
+  }
+  if (rc == VL_OK && R.n == 0) {
+    r.count = 0;
+    r.hash = 0;   // the hash of no boxes
+  } else if (rc == VL_OK) {
     r.count = R.n;
     Scratch hs(w);
     unsigned long long* h = hs.get<unsigned long long>(1);
@@ -3093,7 +3196,6 @@ int run_query(vl_server* sv, vl_engine* w, const vl_query& q, vl_query_result& r
   }
   release_layer(w, A);
   release_layer(w, B);
-  release_layer(w, G);
   release_layer(w, R);
   return rc;
 }
@@ -3127,12 +3229,26 @@ int vl_server_create(vl_engine* e, const vl_layer* const* layers, int nlayers, i
     int rc = layer_mbr(e, l, &m);
     if (rc != VL_OK) return cleanup(rc);
     ix.g = make_grid(m, l->n, 64);   // about 64 boxes per cell
+    {
+      int zb = 0;
+      while ((1 << zb) < std::max(ix.g.cols, ix.g.rows)) zb++;
+      // Morton numbering when its square stays within 8x the cells (and 2^30 keys)
+      if (zb <= 14 && ((int64_t)1 << (2 * zb)) <= 8 * (int64_t)ix.g.cols * ix.g.rows) ix.g.zb = std::max(zb, 1);
+    }
     ix.keep = new Scratch(e);
     uint64_t total = 0;
     rc = bin_entries(e, ix.g, l->d, l->n, Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0, *ix.keep, &ix.entries,
-                     &ix.d_start, &total);
+                     &ix.d_start, &total, ix.g.zb == 0);
+    if (rc == BIN_TOO_MANY) {   // boxes spanning many cells: row-major cells, coarsened
+      g_err.clear();
+      delete ix.keep;
+      ix.keep = new Scratch(e);
+      ix.g = make_grid(m, l->n, 64);
+      rc = bin_entries(e, ix.g, l->d, l->n, Xform{0, 0, 0, 0}, Xform{0, 0, 0, 0}, nullptr, 0, *ix.keep, &ix.entries,
+                       &ix.d_start, &total);
+    }
     if (rc != VL_OK) return cleanup(rc);
-    const size_t ns = 2 * (size_t)ix.g.cols * ix.g.rows + 1;
+    const size_t ns = 2 * (size_t)grid_tiles(ix.g) + 1;
     ix.start.resize(ns);
     if (hipMemcpyAsync(ix.start.data(), ix.d_start, ns * sizeof(uint64_t), hipMemcpyDeviceToHost, e->st) !=
             hipSuccess ||
@@ -3150,6 +3266,7 @@ int vl_server_create(vl_engine* e, const vl_layer* const* layers, int nlayers, i
     w->level_factor = e->level_factor;
     w->erode_kernel = e->erode_kernel;
     w->bin_mode = e->bin_mode;
+    w->check_target = e->check_target;
     sv->slots.push_back(w);
   }
   *out = sv;

@@ -42,12 +42,33 @@ enum TileOp : int32_t {
 };
 
 // Uniform grid over a domain.  Tile (c, r) covers
-// [x0 + c*tw, min(x0+(c+1)*tw, x1)) x [y0 + r*th, min(...)).
+// [x0 + c*tw, min(x0+(c+1)*tw, x1)) x [y0 + r*th, min(...)).  Tiles are
+// numbered row by row (r * cols + c), or, with zb > 0, in Morton (Z) order
+// over a 2^zb x 2^zb square of tiles (the bits of c and r interleaved, c in
+// the even bits): then 4^zb numbers, those outside cols x rows unused.
 struct Grid {
   int64_t x0, y0, x1, y1;   // domain (int64 so widths never overflow)
   int64_t tw, th;
   int32_t cols, rows;
+  int32_t zb = 0;
 };
+
+// Tile numbers of grid g (the size of its per-tile arrays).
+inline int64_t grid_tiles(const Grid& g) { return g.zb ? (int64_t)1 << (2 * g.zb) : (int64_t)g.cols * g.rows; }
+
+// Morton number of tile (c, r): bit k of c at bit 2k, bit k of r at bit 2k+1.
+__host__ __device__ inline uint32_t morton_spread16(uint32_t v) {
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__host__ __device__ inline uint32_t tile_number(int r, int c, const Grid& g) {
+  return g.zb ? (morton_spread16((uint32_t)c) | (morton_spread16((uint32_t)r) << 1))
+              : (uint32_t)r * (uint32_t)g.cols + (uint32_t)c;
+}
 
 // Per-box transform applied while binning (GROW): x1-=w, y1-=s, x2+=e, y2+=n.
 struct Xform {
@@ -206,9 +227,13 @@ void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, 
                        uint32_t* tab_n, int ntab, hipStream_t st);
 // AND emptiness by semi-join: entries/start hold one input binned alone into g
 // (layer-A keys); *hit (cleared by the caller) is set to 1 when some box of
-// p[0..n) overlaps a binned box with positive area.
-void launch_and_probe(const vl_box* p, int64_t n, Grid g, const vl_box* entries, const uint64_t* start,
-                      uint32_t* hit, hipStream_t st);
+// p[0..n), transformed by xp, overlaps a binned box with positive area; mark
+// (may be null): per box of p, 1 when it overlaps one, else 0.
+void launch_and_probe(const vl_box* p, int64_t n, Xform xp, Grid g, const vl_box* entries, const uint64_t* start,
+                      uint32_t* hit, uint32_t* mark, hipStream_t st);
+// out[pos[i]] = p[i] transformed by xp for every i with mark[i] != 0.
+void launch_compact_marked(const vl_box* p, int64_t n, Xform xp, const uint32_t* mark, const uint32_t* pos,
+                           vl_box* out, hipStream_t st);
 // The same scatter writing box indices (idx) instead of boxes (one layer).
 void launch_bin_scatter_idx(const vl_box* a, int64_t na, Xform xra, Grid g, const uint64_t* start, uint32_t* cnt,
                             uint32_t* idx, bool grouped, hipStream_t st);
@@ -360,8 +385,11 @@ void launch_uf_labels(const uint32_t* root, const uint32_t* touched, const uint3
 // vl_window.hip — window extraction from a cell-indexed layer.  Candidates
 // are the concatenation of the entry ranges [rbeg[r], rbeg[r] + roff[r+1] -
 // roff[r]); each is clipped to w (clip[i]) and flagged when it keeps area.
+// A box binned into several cells is kept only in the cell holding the
+// lower-left corner of its part inside w (start: the index's cell starts,
+// layer-A keys of grid g), so every box of the window is returned once.
 void launch_window_clip(const vl_box* entries, const uint64_t* rbeg, const uint64_t* roff, int nr, uint64_t n,
-                        vl_box w, vl_box* clip, uint32_t* keep, hipStream_t st);
+                        vl_box w, const uint64_t* start, Grid g, vl_box* clip, uint32_t* keep, hipStream_t st);
 // out[pos[i]] = clip[i] for the flagged candidates.
 void launch_window_compact(const vl_box* clip, const uint32_t* keep, const uint32_t* pos, uint64_t n, vl_box* out,
                            hipStream_t st);

@@ -138,6 +138,31 @@ void gen_chip(uint64_t seed, int layer, int64_t n, vl_box* out) {
   }
 }
 
+// Routing tracks for DRC spacing checks: layers 0 and 1 are the met1 wires of
+// the even and odd horizontal tracks, layers 2 and 3 the met2 wires of the
+// even and odd vertical tracks.  Track pitch 14 DBU, wire width 8, so wires of
+// neighbouring tracks (one layer of a pair against the other) are 6 DBU apart;
+// along a track, wires of length log-uniform 20..5000 with gaps of 10..100,
+// over [0, 2^20).  One wire in 4096 is a jog moved 2 DBU towards the next
+// track (4 DBU from it): a spacing check of the pair at 5 DBU fails there, at
+// 1 and 2 DBU nowhere.
+void gen_route(uint64_t seed, int layer, int64_t n, vl_box* out) {
+  const int64_t PITCH = 14, WIDTH = 8, SPAN = (int64_t)1 << 20;
+  const bool vertical = layer >= 2;
+  int64_t i = 0;
+  for (int64_t k = 0; i < n; k++) {
+    const int64_t track = 2 * k + (layer & 1);
+    Rng r(mix(seed, 40 + (uint64_t)layer, (uint64_t)k));
+    for (int64_t pos = r.below(100); pos < SPAN && i < n;) {
+      const int64_t L = r.logu(20, 5000);
+      const int64_t off = track * PITCH + (r.below(4096) == 0 ? 2 : 0);
+      out[i++] = vertical ? vl_box{(int32_t)off, (int32_t)pos, (int32_t)(off + WIDTH), (int32_t)(pos + L)}
+                          : vl_box{(int32_t)pos, (int32_t)off, (int32_t)(pos + L), (int32_t)(off + WIDTH)};
+      pos += L + 10 + r.below(91);
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int vl_synth_layer(int config, uint64_t seed, int layer, int64_t n, vl_box* out) {
@@ -147,6 +172,10 @@ extern "C" int vl_synth_layer(int config, uint64_t seed, int layer, int64_t n, v
     case VL_SYN_STDCELL: gen_stdcell(seed, layer, n, out); return VL_OK;
     case VL_SYN_TILES: gen_tile(seed, layer, n, out); return VL_OK;
     case VL_SYN_CHIP: gen_chip(seed, layer, n, out); return VL_OK;
+    case VL_SYN_ROUTE:
+      if (layer > 3) return VL_EINVAL;
+      gen_route(seed, layer, n, out);
+      return VL_OK;
     default: return VL_EINVAL;
   }
 }

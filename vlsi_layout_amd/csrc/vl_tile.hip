@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_bin_emit(const vl_box* __restrict__ a, 
     const uint32_t layer = isA ? 0u : 1u;
     for (int r = r1; r <= r2; r++)
       for (int c = c1; c <= c2; c++) {
-        keys[p] = (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
+        keys[p] = (tile_number(r, c, g) << 1) | layer;
         vals[p] = (uint32_t)i;
         p++;
       }
@@ -267,7 +267,7 @@ __device__ __forceinline__ Lead lead_of(bool act, uint32_t key, int lane) {
 constexpr int BIN_U = 4;
 
 __device__ __forceinline__ uint32_t bin_key(int r, int c, const Grid& g, uint32_t layer) {
-  return (((uint32_t)r * (uint32_t)g.cols + (uint32_t)c) << 1) | layer;
+  return (tile_number(r, c, g) << 1) | layer;
 }
 
 template <bool GROUPED>
@@ -504,44 +504,59 @@ __global__ __launch_bounds__(256) void k_and_filter(const TileDesc* __restrict__
 // (tile t's boxes at entries[start[2t] .. start[2t+1])); a lane takes a box
 // of the other input and tests it against the binned boxes of every tile its
 // interior meets.  A positive-area overlap of any pair lies inside a tile
-// both boxes meet, so *hit stays 0 exactly when the AND is empty.  Inputs in
-// spatial order keep neighbouring lanes on the same tiles (cache hits); the
-// binned boxes of a tile are loaded PROBE_U at a time (one memory round trip
-// for most tiles) and the lane's next box is loaded before the walk.
+// both boxes meet, so *hit stays 0 exactly when the AND is empty.  With mark,
+// mark[i] says whether box i overlaps some binned box (the semi-join
+// reduction); without, a lane stops testing after its first overlap.  Inputs
+// in spatial order keep neighbouring lanes on the same tiles (cache hits);
+// the binned boxes of a tile are loaded PROBE_U at a time (one memory round
+// trip for most tiles) and the lane's next box is loaded before the walk.
 constexpr int PROBE_U = 8;
 
 __device__ __forceinline__ bool overlaps(const vl_box& q, const vl_box& b) {
   return max(q.x1, b.x1) < min(q.x2, b.x2) && max(q.y1, b.y1) < min(q.y2, b.y2);
 }
 
-__global__ __launch_bounds__(256) void k_and_probe(const vl_box* __restrict__ p, int64_t n, Grid g,
+__global__ __launch_bounds__(256) void k_and_probe(const vl_box* __restrict__ p, int64_t n, Xform xp, Grid g,
                                                    const vl_box* __restrict__ entries,
-                                                   const uint64_t* __restrict__ start, uint32_t* __restrict__ hit) {
+                                                   const uint64_t* __restrict__ start, uint32_t* __restrict__ hit,
+                                                   uint32_t* __restrict__ mark) {
   const GridDiv gd = grid_div(g);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  bool any = false;
+  bool found = false;   // some box of this lane overlaps
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   vl_box nq = i < n ? p[i] : vl_box{0, 0, 0, 0};
   for (; i < n; i += stride) {
-    const vl_box q = nq;
+    const vl_box q = xform(nq, xp);
     if (i + stride < n) nq = p[i + stride];
+    bool h = false;
     int c1 = 0, c2 = -1, r1 = 0, r2 = -1;
-    if (any || !tile_range(q, Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2)) continue;
-    for (int r = r1; r <= r2 && !any; r++) {
-      for (int c = c1; c <= c2 && !any; c++) {
-        const uint64_t t = (uint64_t)r * (uint64_t)g.cols + (uint64_t)c;
-        const uint64_t s0 = start[2 * t], s1 = start[2 * t + 1];
-        for (uint64_t j = s0; j < s1; j += PROBE_U) {
-          vl_box b[PROBE_U];
+    if ((mark || !found) && tile_range(q, Xform{0, 0, 0, 0}, g, gd, c1, c2, r1, r2)) {
+      for (int r = r1; r <= r2 && !h; r++) {
+        for (int c = c1; c <= c2 && !h; c++) {
+          const uint64_t t = (uint64_t)r * (uint64_t)g.cols + (uint64_t)c;
+          const uint64_t s0 = start[2 * t], s1 = start[2 * t + 1];
+          for (uint64_t j = s0; j < s1; j += PROBE_U) {
+            vl_box b[PROBE_U];
 #pragma unroll
-          for (int u = 0; u < PROBE_U; u++) b[u] = j + u < s1 ? entries[j + u] : vl_box{0, 0, 0, 0};
+            for (int u = 0; u < PROBE_U; u++) b[u] = j + u < s1 ? entries[j + u] : vl_box{0, 0, 0, 0};
 #pragma unroll
-          for (int u = 0; u < PROBE_U; u++) any |= overlaps(q, b[u]);
+            for (int u = 0; u < PROBE_U; u++) h |= overlaps(q, b[u]);
+          }
         }
       }
     }
+    if (mark) mark[i] = h ? 1u : 0u;
+    found |= h;
   }
-  if (__ballot(any) != 0ull && __lane_id() == 0) gstore(hit, 1u);
+  if (__ballot(found) != 0ull && __lane_id() == 0) gstore(hit, 1u);
+}
+
+// out[pos[i]] = p[i] transformed by xp, for the marked boxes.
+__global__ __launch_bounds__(256) void k_compact_marked(const vl_box* __restrict__ p, int64_t n, Xform xp,
+                                                        const uint32_t* __restrict__ mark,
+                                                        const uint32_t* __restrict__ pos, vl_box* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (mark[i]) out[pos[i]] = xform(p[i], xp);
 }
 
 __global__ void k_build_desc(Grid g, const uint64_t* __restrict__ start, int32_t op, TileDesc* desc) {
@@ -1539,10 +1554,15 @@ void launch_and_filter(const TileDesc* desc, int ntiles, const vl_box* entries, 
     hipLaunchKernelGGL(k_and_filter, dim3(grid_for(ntiles, 16)), dim3(256), 0, st, desc, ntiles, entries, list,
                        list_n, tab_n, ntab);
 }
-void launch_and_probe(const vl_box* p, int64_t n, Grid g, const vl_box* entries, const uint64_t* start,
-                      uint32_t* hit, hipStream_t st) {
+void launch_and_probe(const vl_box* p, int64_t n, Xform xp, Grid g, const vl_box* entries, const uint64_t* start,
+                      uint32_t* hit, uint32_t* mark, hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_and_probe, dim3(grid_for(n)), dim3(256), 0, st, p, n, g, entries, start, hit);
+    hipLaunchKernelGGL(k_and_probe, dim3(grid_for(n)), dim3(256), 0, st, p, n, xp, g, entries, start, hit, mark);
+}
+void launch_compact_marked(const vl_box* p, int64_t n, Xform xp, const uint32_t* mark, const uint32_t* pos,
+                           vl_box* out, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_compact_marked, dim3(grid_for(n)), dim3(256), 0, st, p, n, xp, mark, pos, out);
 }
 void launch_build_desc(Grid g, const uint64_t* start, int32_t op, TileDesc* desc, hipStream_t st) {
   int64_t nt = (int64_t)g.cols * g.rows;

@@ -2,12 +2,13 @@
 // order-independent hash of a box set (used by the query server).
 //
 // A layer index is the layer's boxes binned into a uniform grid of cells
-// (the tile binning of vl_tile.hip): entries in cell order plus the start of
-// every cell.  A window query gathers the entry ranges of the cell rows the
-// window covers (one contiguous range per row), clips every candidate to the
+// (the tile binning of vl_tile.hip, cells in Morton order): entries in cell
+// order plus the start of every cell.  A window query gathers the entry
+// ranges of the runs of cells the window covers, clips every candidate to the
 // window and compacts the ones that keep a positive area.  A box spanning
-// several cells is returned once per cell: duplicates do not change any
-// union-based result.
+// several cells has a copy in each; only the copy in the cell holding the
+// lower-left corner of the box's part inside the window is kept, so the
+// window's boxes come out once each.
 #include "vl_internal.h"
 #include "vl_dev.h"
 
@@ -28,17 +29,24 @@ __device__ __forceinline__ int range_of(const uint64_t* off, int nr, uint64_t i)
 __global__ __launch_bounds__(256) void k_window_clip(const vl_box* __restrict__ entries,
                                                      const uint64_t* __restrict__ rbeg,
                                                      const uint64_t* __restrict__ roff, int nr, uint64_t n,
-                                                     vl_box w, vl_box* __restrict__ clip,
-                                                     uint32_t* __restrict__ keep) {
+                                                     vl_box w, const uint64_t* __restrict__ start, Grid g,
+                                                     vl_box* __restrict__ clip, uint32_t* __restrict__ keep) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const int r = range_of(roff, nr, i);
-    const vl_box b = entries[rbeg[r] + (i - roff[r])];
+    const uint64_t e = rbeg[r] + (i - roff[r]);
+    const vl_box b = entries[e];
     vl_box c;
     c.x1 = max(b.x1, w.x1);
     c.y1 = max(b.y1, w.y1);
     c.x2 = min(b.x2, w.x2);
     c.y2 = min(b.y2, w.y2);
-    const bool ok = c.x1 < c.x2 && c.y1 < c.y2;
+    bool ok = c.x1 < c.x2 && c.y1 < c.y2;
+    if (ok) {
+      // the cell of (c.x1, c.y1): inside the domain, since the box is
+      const int cc = (int)(((int64_t)c.x1 - g.x0) / g.tw), rr = (int)(((int64_t)c.y1 - g.y0) / g.th);
+      const uint64_t t = tile_number(rr, cc, g);
+      ok = start[2 * t] <= e && e < start[2 * t + 1];
+    }
     clip[i] = c;
     keep[i] = ok ? 1u : 0u;
   }
@@ -89,10 +97,10 @@ unsigned grid_of(uint64_t n) {
 }  // namespace
 
 void launch_window_clip(const vl_box* entries, const uint64_t* rbeg, const uint64_t* roff, int nr, uint64_t n,
-                        vl_box w, vl_box* clip, uint32_t* keep, hipStream_t st) {
+                        vl_box w, const uint64_t* start, Grid g, vl_box* clip, uint32_t* keep, hipStream_t st) {
   if (n > 0)
-    hipLaunchKernelGGL(k_window_clip, dim3(grid_of(n)), dim3(256), 0, st, entries, rbeg, roff, nr, n, w, clip,
-                       keep);
+    hipLaunchKernelGGL(k_window_clip, dim3(grid_of(n)), dim3(256), 0, st, entries, rbeg, roff, nr, n, w, start, g,
+                       clip, keep);
 }
 
 void launch_window_compact(const vl_box* clip, const uint32_t* keep, const uint32_t* pos, uint64_t n, vl_box* out,
