@@ -210,6 +210,11 @@ struct vl_layer {
   // layer clears it).  Operators read it instead of reducing the boxes again.
   mutable bool mbr_ok = false;
   mutable int64_t mbr[4] = {0, 0, 0, 0};
+  // A box known to contain every box of the layer (not necessarily the
+  // smallest), valid while version == bound_version: set by producers that
+  // know it without a reduction (a fused group's outputs lie in its domain).
+  uint64_t bound_version = ~0ull;
+  int64_t bound[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -1292,6 +1297,16 @@ int layer_mbr(vl_engine* e, const vl_layer* l, Box64* m) {
   return VL_OK;
 }
 
+// A box containing every box of the layer: the cached bounding box, else a
+// producer's bound for the current version, else the bounding box reduced now.
+int layer_extent(vl_engine* e, const vl_layer* l, Box64* m) {
+  if (!l->mbr_ok && l->bound_version == l->version) {
+    *m = Box64{l->bound[0], l->bound[1], l->bound[2], l->bound[3]};
+    return VL_OK;
+  }
+  return layer_mbr(e, l, m);
+}
+
 // ---- fused operator groups ------------------------------------------------------
 // Level 0 of a fused group: the fused tile kernel's two tiers (20 and 41 boxes
 // per window), then x-sub-column splits of the tiles above the second tier's
@@ -1514,7 +1529,12 @@ int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer*
   }
   if (g_debug) fprintf(stderr, "[vl] fused level0 n=%lld outputs=%d %.2f ms\n", (long long)S->n, nout,
                        now_ms() - t_start);
-  for (int k = 0; k < nout; k++) VLCK(finish_op(e, g, *outs[k], c[k], t_start));
+  for (int k = 0; k < nout; k++) {
+    VLCK(finish_op(e, g, *outs[k], c[k], t_start));
+    // every output lies in the group's domain
+    c[k]->bound[0] = dom.x1; c[k]->bound[1] = dom.y1; c[k]->bound[2] = dom.x2; c[k]->bound[3] = dom.y2;
+    c[k]->bound_version = c[k]->version;
+  }
   return VL_OK;
 }
 
@@ -1631,9 +1651,11 @@ int and_check(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
   OpTimer tm(e);
+  // a box containing each input is enough here (the test's grid only has to
+  // cover every overlap)
   Box64 ka, kb;
-  VLCK(layer_mbr(e, a, &ka));
-  VLCK(layer_mbr(e, b, &kb));
+  VLCK(layer_extent(e, a, &ka));
+  VLCK(layer_extent(e, b, &kb));
   return and_check_core(e, Side{a->d, a->n, Xform{0, 0, 0, 0}}, Side{b->d, b->n, Xform{0, 0, 0, 0}}, ka, kb, c);
 }
 
@@ -2811,7 +2833,10 @@ static std::vector<const vl_layer*> program_layers(const vl_program* p) {
 
 static void forget_output_mbrs(vl_program* p) {
   for (const ProgOp& o : p->ops)
-    if (o.c) o.c->mbr_ok = false;
+    if (o.c) {
+      o.c->mbr_ok = false;
+      o.c->bound_version = ~0ull;
+    }
 }
 
 // Launches the captured graph and reports whether every size it was planned
