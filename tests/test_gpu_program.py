@@ -489,3 +489,118 @@ def test_check_and_plan_only_for_assert_reads(gpu_engine):
     assert np.array_equal(O.sort_boxes(E.boxes()), O.op_not(b, a))
     prog.close()
     eng.close()
+
+
+# Groups over a source S and a second layer T (the group's layer image):
+# inputs name S, T or an earlier step's output by index.
+GROUPS2 = {
+    # the uneven-tile batch's step
+    "c3": [(V.K_BOOST_CONTROL_AND, "S", "T", None), (V.K_BOOST_CONTROL_NOT, "S", "T", None),
+           (V.K_BOOST_CONTROL_OR, "S", "T", None), (V.VL_OP_GROW, "S", None, (2, 2, 2, 2))],
+    # T as first operand, XOR, an erosion of S against T
+    "mixed": [(V.K_BOOST_CONTROL_XOR, "S", "T", None), (V.K_BOOST_CONTROL_NOT, "T", "S", None),
+              (V.K_BOOST_CONTROL_SIZE, "S", None, (-2,)), (V.K_BOOST_CONTROL_AND, 2, "T", None),
+              (V.K_BOOST_CONTROL_OR, 0, 3, None)],
+    # a GROW first, then T joins: GROW(S) NOT T, T AND S
+    "grow_first": [(V.VL_OP_GROW, "S", None, (1, 3, 0, 2)), (V.K_BOOST_CONTROL_NOT, 0, "T", None),
+                   (V.K_BOOST_CONTROL_AND, "T", "S", None)],
+}
+
+
+def run_group2(eng, S, T, steps, graph=False, reps=1):
+    outs = [V.Layer(eng) for _ in steps]
+    prog = V.Program(eng)
+    pick = lambda x: S if x == "S" else T if x == "T" else outs[x]
+    for (op, a, b, args), o in zip(steps, outs):
+        prog.add(op, pick(a), None if b is None else pick(b), o, args)
+    prog.add(-1, outs[-1])
+    if graph:
+        prog.set_graph(True)
+    snaps = []
+    for _ in range(reps):
+        prog.run()
+        snaps.append([O.sort_boxes(o.boxes()) for o in outs])
+    fails = prog.assert_failures()
+    prog.close()
+    return outs, fails, snaps
+
+
+def oracle_group2(s, t, steps):
+    res = []
+    pick = lambda x: s if x == "S" else t if x == "T" else res[x]
+    for op, a, b, args in steps:
+        res.append(oracle_apply(op, pick(a), None if b is None else pick(b), args))
+    return res
+
+
+def two_layers(kind, rng):
+    if kind == "random":
+        return rand_boxes(rng, 2500, 20000, 600), rand_boxes(rng, 2000, 20000, 600)
+    if kind == "dense":   # uneven-tile-batch density: uniform in 2^16, sizes up to 500
+        w = lambda n: np.exp(rng.uniform(0, np.log(501), n)).astype(np.int64)
+        mk = lambda n: np.stack([x := rng.integers(0, 1 << 14, n), y := rng.integers(0, 1 << 14, n),
+                                 x + w(n), y + w(n)], 1).astype(np.int32)
+        return mk(3000), mk(3000)
+    if kind == "grid5":   # 5-DBU grid, touching and overlapping boxes
+        return (rand_boxes(rng, 3000, 400, 6) * 5).astype(np.int32), (rand_boxes(rng, 3000, 400, 6) * 5).astype(np.int32)
+    # stdcell poly and met1 of the same rows
+    return V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 0, 12000), V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 2, 12000)
+
+
+@pytest.mark.parametrize("group", sorted(GROUPS2))
+@pytest.mark.parametrize("kind", ["random", "dense", "grid5", "stdcell"])
+def test_two_layer_groups_match_oracle(gpu_engine, group, kind):
+    """A group over a source S and a second layer T runs as one fused tile
+    pass (T's boxes are each tile's second-layer entries, rasterised into the
+    group's layer image) and every output equals the oracle's."""
+    rng = np.random.default_rng(hash((group, kind)) & 0xFFFF)
+    s, t = two_layers(kind, rng)
+    eng = V.Engine(0)
+    S, T = V.Layer(eng, s), V.Layer(eng, t)
+    steps = GROUPS2[group]
+    got, fails, _ = run_group2(eng, S, T, steps)
+    assert eng.get("fused_groups") == 1, (group, kind)
+    exp = oracle_group2(s, t, steps)
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), x), (group, kind, k)
+    assert fails == (1 if len(exp[-1]) else 0)
+    eng.close()
+
+
+@pytest.mark.parametrize("target", [2, 20, 60])
+def test_two_layer_groups_tiers_and_splits(gpu_engine, target):
+    """Two-layer groups on grids below and above the tiers' capacities (empty
+    windows, second and third tiers, x-sub-column splits), against the
+    oracle; an empty S falls back to single operators."""
+    rng = np.random.default_rng(700 + target)
+    s, t = two_layers("dense", rng)
+    eng = V.Engine(0)
+    eng.set("fuse_target", target)
+    S, T = V.Layer(eng, s), V.Layer(eng, t)
+    got, _, _ = run_group2(eng, S, T, GROUPS2["c3"])
+    exp = oracle_group2(s, t, GROUPS2["c3"])
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), x), (target, k)
+    E = V.Layer(eng)
+    before = eng.get("fused_groups")
+    got, _, _ = run_group2(eng, E, T, GROUPS2["c3"])
+    assert eng.get("fused_groups") == before
+    exp = oracle_group2(np.zeros((0, 4), np.int32), t, GROUPS2["c3"])
+    for k, (g, x) in enumerate(zip(got, exp)):
+        assert np.array_equal(O.sort_boxes(g.boxes()), x), ("empty S", k)
+    eng.close()
+
+
+def test_two_layer_group_graph_replays(gpu_engine):
+    """The uneven-tile step captured as a graph: every replay equals the
+    oracle."""
+    rng = np.random.default_rng(77)
+    s, t = two_layers("random", rng)
+    eng = V.Engine(0)
+    S, T = V.Layer(eng, s), V.Layer(eng, t)
+    _, _, snaps = run_group2(eng, S, T, GROUPS2["c3"], graph=True, reps=4)
+    exp = oracle_group2(s, t, GROUPS2["c3"])
+    for rep, snap in enumerate(snaps):
+        for k, (g, x) in enumerate(zip(snap, exp)):
+            assert np.array_equal(g, x), (rep, k)
+    eng.close()

@@ -221,13 +221,14 @@ class Engine:
         return Layer(self, boxes)
 
     def autotune(self, a, b=None):
-        """Picks tile_target, erode_target, fuse_target, fuse, bin_mode, radix_bits and max_wg by
-        timing candidates on layers a (and b); returns the chosen values."""
+        """Picks tile_target, erode_target, fuse_target, fuse, fuse2 (with b), bin_mode, radix_bits
+        and max_wg by timing candidates on layers a (and b); returns the chosen values."""
         t, et = C.c_int64(), C.c_int64()
         _check(lib().vl_engine_autotune(self._h, a.handle, b.handle if b is not None else None, C.byref(t),
                                         C.byref(et)))
         return {"tile_target": t.value, "erode_target": et.value, "fuse_target": self.get("fuse_target"),
-                "fuse": self.get("fuse"), "bin_mode": self.get("bin_mode"), "radix_bits": self.get("radix_bits"),
+                "fuse": self.get("fuse"), "fuse2": self.get("fuse2"), "bin_mode": self.get("bin_mode"),
+                "radix_bits": self.get("radix_bits"),
                 "max_wg": self.get("max_wg")}
 
 

@@ -145,6 +145,7 @@ struct vl_engine {
   int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
   int fuse = 1;                  // programs: fuse operator groups over one source layer
+  int fuse2 = 1;                 // programs: groups may take a second layer as an image
   int fuse_target = 6;           // mean boxes per tile of a fused group's grid
   int bin_mode = 2;              // 2: atomic binning, atomics grouped per wave (1: per run); 0: radix sort
   int radix_bits = 8;            // digit width of the radix sorts (8 or 11)
@@ -1441,7 +1442,10 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
   if (nov > 0) {
     // a sub-column still above the largest tier holds more boxes over one
     // x-position than further x-splits separate: the group runs operator by
-    // operator instead (the single-operator path has the dense kernel)
+    // operator instead (the single-operator path has the dense kernel).
+    // Rows are not split: every tile of a grid row must share its row
+    // borders, or a strip crossing a column border would reach the seam
+    // levels on one side and the x-stitch on the other.
     if (depth >= 1) return fail(VL_EINTERNAL, "%u fused sub-columns above the largest tier", nov);
     Scratch ss(e);
     SplitTiles sp;
@@ -1457,8 +1461,11 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
 constexpr int FUSE_DECLINED = 1000;
 
 // Outputs c[k] = fs.tt[k](S, morphological images of S) of a fused group.
-int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer* const* c_in) {
+int run_fused(vl_engine* e, const vl_layer* S, const vl_layer* T, const FusedSpec& fs_in, vl_layer* const* c_in) {
   if (S->n == 0) {
+    // with a second layer, outputs such as S OR T are not empty: the
+    // operators run one by one
+    if (T) return FUSE_DECLINED;
     for (int k = 0; k < fs_in.nout; k++) set_empty(e, c_in[k]);
     return VL_OK;
   }
@@ -1485,11 +1492,16 @@ int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer*
       de = std::max<int64_t>(de, fs.arg[k][0]);
     }
   }
-  const Box64 dom{m.x1 - gw, m.y1 - gs, m.x2 + ge, m.y2 + gn};
+  Box64 dom{m.x1 - gw, m.y1 - gs, m.x2 + ge, m.y2 + gn};
+  if (T && T->n > 0) {   // outputs may cover parts of the second layer outside S
+    Box64 mt;
+    VLCK(layer_mbr(e, T, &mt));
+    dom = Box64{std::min(dom.x1, mt.x1), std::min(dom.y1, mt.y1), std::max(dom.x2, mt.x2), std::max(dom.y2, mt.y2)};
+  }
   const int64_t h = fs.halo;
   if (dom.x1 - h < INT32_MIN || dom.y1 - h < INT32_MIN || dom.x2 + h > INT32_MAX || dom.y2 + h > INT32_MAX)
     return FUSE_DECLINED;
-  const Grid g = make_grid(dom, S->n, e->fuse_target);
+  const Grid g = make_grid(dom, S->n + (T ? T->n : 0), e->fuse_target);
   // windows must stay small against the tiles (each box meets a few windows),
   // and below 2^24 - 1 DBU wide and tall (the kernel sorts coordinates relative
   // to the window corner in 24 bits)
@@ -1512,8 +1524,10 @@ int run_fused(vl_engine* e, const vl_layer* S, const FusedSpec& fs_in, vl_layer*
     // a fused grid keeps its tile windows (the kernel's coordinate range
     // depends on them): a layer whose boxes span too many tiles is declined
     Grid gb = g;
-    const int brc = bin_entries(e, gb, S->d, S->n, Xform{0, 0, 0, 0}, Xform{R, R, R, R}, nullptr, 0, keep, &entries,
-                                &start, &total, false);
+    // S's boxes by their extent grown by the halo (its images reach that far),
+    // the second layer's by their extent (its image is the layer itself)
+    const int brc = bin_entries(e, gb, S->d, S->n, Xform{0, 0, 0, 0}, Xform{R, R, R, R}, T ? T->d : nullptr,
+                                T ? T->n : 0, keep, &entries, &start, &total, false);
     if (brc == BIN_TOO_MANY) return FUSE_DECLINED;
     VLCK(brc);
     TileDesc* desc = keep.get<TileDesc>(nt);
@@ -1806,6 +1820,8 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
   } else if (k == "fuse") {
     if (v < 0 || v > 2) return fail(VL_EINVAL, "fuse must be 0, 1 or 2");
     e->fuse = (int)v;
+  } else if (k == "fuse2") {
+    e->fuse2 = v != 0;
   } else if (k == "fuse_target") {
     if (v < 2 || v > 4096) return fail(VL_EINVAL, "fuse_target out of range");
     e->fuse_target = (int)v;
@@ -1840,6 +1856,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "erode_kernel") *v = e->erode_kernel;
   else if (k == "fuse") *v = e->fuse;
   else if (k == "fuse_target") *v = e->fuse_target;
+  else if (k == "fuse2") *v = e->fuse2;
   else if (k == "bin_mode") *v = e->bin_mode;
   else if (k == "radix_bits") *v = e->radix_bits;
   else if (k == "workers") *v = e->workers;
@@ -2481,19 +2498,25 @@ struct ProgStep {
   bool fused = false;
   bool check = false;   // a single AND read only by ASSERT_ZERO: and_check
   const vl_layer* src = nullptr;
+  const vl_layer* src2 = nullptr;   // the group's second layer (an image of kind MORPH_LAYER)
   FusedSpec fs{};
   vl_layer* outs[FUSE_MAX_OUT] = {};
 };
 
-// Greedy grouping of consecutive operators over one source layer S: the group
-// starts at a GROW, SIZE or MERGE of S and takes every following operator
-// whose inputs are S or earlier group outputs (GROW with non-negative
-// arguments and SIZE only on S itself, at most two distinct morphological
-// images, at most FUSE_MAX_OUT outputs, each written once and never S) and
-// every ASSERT_ZERO of such a layer.  Each output is a truth table over
-// (S, image 0, image 1); S is 0xAA, image 0 is 0xCC, image 1 is 0xF0.
+// Greedy grouping of consecutive operators over one source layer S and at
+// most one second layer T: the group starts at a GROW, SIZE or MERGE of S, or
+// at a Boolean operator of two layers S and T, and takes every following
+// operator whose inputs are S, T or earlier group outputs (GROW with
+// non-negative arguments and SIZE only on S itself, at most two images in
+// all, counting T, at most FUSE_MAX_OUT outputs, each written once and
+// never S or T) and every ASSERT_ZERO of such a layer.  Each output is a
+// truth table over (S, image 0, image 1); S is 0xAA, image 0 is 0xCC, image 1
+// is 0xF0.  The OP1 ring of a layer is one group (S and two morphological
+// images); AND, NOT, OR of a layer pair and a GROW of the first (the
+// uneven-tile batch) is one group (S, T and one grown image).
 struct GroupBuilder {
   const vl_layer* src = nullptr;
+  const vl_layer* src2 = nullptr;
   FusedSpec fs{};
   std::vector<std::pair<vl_layer*, uint32_t>> nodes;
   std::vector<int> ops;
@@ -2503,6 +2526,13 @@ struct GroupBuilder {
     if (l == src) {
       *tt = 0xAAu;
       return true;
+    }
+    if (l && l == src2) {
+      for (int k = 0; k < fs.nmorph; k++)
+        if (fs.kind[k] == MORPH_LAYER) {
+          *tt = k ? 0xF0u : 0xCCu;
+          return true;
+        }
     }
     for (const auto& nd : nodes)
       if (nd.first == l) {
@@ -2524,14 +2554,30 @@ struct GroupBuilder {
     fs.halo = std::max(fs.halo, reach);
     return k;
   }
+  // The image of second layer l (added if there is none yet and a slot is free).
+  bool two_layers = true;
+  bool layer_image(const vl_layer* l, uint32_t* tt) {
+    if (src2) return tt_of(l, tt);
+    if (!two_layers) return false;
+    if (!l || l == src || fs.nmorph >= max_images) return false;
+    for (const auto& nd : nodes)
+      if (nd.first == l) return false;
+    const int k = fs.nmorph++;
+    fs.kind[k] = MORPH_LAYER;
+    memset(fs.arg[k], 0, sizeof(fs.arg[k]));
+    src2 = l;
+    *tt = k ? 0xF0u : 0xCCu;
+    return true;
+  }
   bool add(const ProgOp& o) {
     if (!src) return false;
     uint32_t ta = 0, tb = 0, tt = 0;
     if (o.op == -1) return tt_of(o.a, &ta);
-    if (!o.c || o.c == src || (int)nodes.size() == FUSE_MAX_OUT) return false;
+    if (!o.c || o.c == src || (src2 && o.c == src2) || (int)nodes.size() == FUSE_MAX_OUT) return false;
     for (const auto& nd : nodes)
       if (nd.first == o.c) return false;
     const FusedSpec saved = fs;
+    const vl_layer* const saved2 = src2;
     switch (o.op) {
       case VL_OP_GROW: {
         if (o.a != src || o.args[0] < 0 || o.args[1] < 0 || o.args[2] < 0 || o.args[3] < 0) return false;
@@ -2560,21 +2606,26 @@ struct GroupBuilder {
       case VL_OP_AND:
       case VL_OP_NOT:
       case VL_OP_OR:
-      case VL_OP_XOR:
-        if (!tt_of(o.a, &ta) || !tt_of(o.b, &tb)) return false;
+      case VL_OP_XOR: {
+        const bool ka = tt_of(o.a, &ta), kb = tt_of(o.b, &tb);
+        // one operand outside the group: the group's second layer
+        const bool ok = (ka && kb) || (ka && !kb && layer_image(o.b, &tb)) || (!ka && kb && layer_image(o.a, &ta));
         tt = o.op == VL_OP_AND ? (ta & tb) : o.op == VL_OP_NOT ? (ta & ~tb) : o.op == VL_OP_OR ? (ta | tb) : (ta ^ tb);
         tt &= 0xFFu;
         // An operator whose result is empty for every S (the OP1 ring's
         // ANDs of a ring with its own source) is a consistency check of the
         // group's outputs: it stays out of the group and runs on the
         // materialised layers, so the check can fail when they are wrong.
-        if (tt == 0u) {
+        if (!ok || o.c == src2 || tt == 0u) {
           fs = saved;
+          src2 = saved2;
           return false;
         }
         break;
+      }
       default:
         fs = saved;
+        src2 = saved2;
         return false;
     }
     nodes.push_back({o.c, tt});
@@ -2582,23 +2633,31 @@ struct GroupBuilder {
     return true;
   }
   bool start(const ProgOp& o) {
-    if (o.op != VL_OP_GROW && o.op != VL_OP_SIZE && o.op != VL_OP_MERGE) return false;
+    const bool binop = o.op == VL_OP_AND || o.op == VL_OP_NOT || o.op == VL_OP_OR || o.op == VL_OP_XOR;
+    if (o.op != VL_OP_GROW && o.op != VL_OP_SIZE && o.op != VL_OP_MERGE && !(two_layers && binop && o.b && o.b != o.a))
+      return false;
     src = o.a;
     if (add(o)) return true;
     const int mi = max_images;
+    const bool tl = two_layers;
     *this = GroupBuilder();
     max_images = mi;
+    two_layers = tl;
     return false;
   }
 };
 
 // fuse: 0 no groups, 1 groups of up to two morphological images, 2 groups of
 // at most one (smaller compressed grids per tile, one more binning of S).
-static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int fuse, bool check_ands) {
+// fuse2: groups may take a second layer (they run fused even with fuse 0,
+// which then only keeps groups without one from fusing).
+static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int fuse, bool fuse2, bool check_ands) {
   std::vector<ProgStep> steps;
   const int max_images = fuse == 2 ? 1 : 2;
+  const bool grouping = fuse || fuse2;
   GroupBuilder g;
   g.max_images = max_images;
+  g.two_layers = fuse2;
   auto single = [&](int i) {
     ProgStep st;
     st.ops.push_back(i);
@@ -2606,11 +2665,12 @@ static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int
   };
   auto flush = [&]() {
     if (g.ops.empty()) return;
-    if (g.nprod >= 2) {
+    if (g.nprod >= 2 && (fuse || g.src2)) {
       ProgStep st;
       st.ops = g.ops;
       st.fused = true;
       st.src = g.src;
+      st.src2 = g.src2;
       st.fs = g.fs;
       st.fs.nout = (int32_t)g.nodes.size();
       for (size_t k = 0; k < g.nodes.size(); k++) {
@@ -2623,14 +2683,15 @@ static std::vector<ProgStep> compile_program(const std::vector<ProgOp>& ops, int
     }
     g = GroupBuilder();
     g.max_images = max_images;
+    g.two_layers = fuse2;
   };
   for (int i = 0; i < (int)ops.size(); i++) {
-    if (fuse && g.add(ops[i])) {
+    if (grouping && g.add(ops[i])) {
       g.ops.push_back(i);
       continue;
     }
     flush();
-    if (fuse && g.start(ops[i])) {
+    if (grouping && g.start(ops[i])) {
       g.ops.push_back(i);
       continue;
     }
@@ -2666,7 +2727,7 @@ static int run_step(vl_engine* e, const std::vector<ProgOp>& ops, const ProgStep
     {
       std::lock_guard<std::recursive_mutex> lk(e->mu);
       HIPCK(hipSetDevice(e->device));
-      rc = run_fused(e, st.src, st.fs, st.outs);
+      rc = run_fused(e, st.src, st.src2, st.fs, st.outs);
     }
     if (rc == VL_OK) {
       e->fused_groups++;
@@ -2739,6 +2800,7 @@ static int program_workers(vl_program* p, int n) {
     w->erode_kernel = e->erode_kernel;
     w->max_wg_cap = e->max_wg_cap;
     w->fuse = e->fuse;
+    w->fuse2 = e->fuse2;
     w->fuse_target = e->fuse_target;
     w->bin_mode = e->bin_mode;
     w->radix_bits = e->radix_bits;
@@ -2963,7 +3025,7 @@ int vl_program_run(vl_program* p) {
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   HIPCK(hipSetDevice(e->device));
   p->failures = 0;
-  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse, e->and_check != 0);
+  const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse, e->fuse2 != 0, e->and_check != 0);
   const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
   int nw = (int)std::min<size_t>(chains.size(), (size_t)std::min(p->max_workers, p->e->workers));
   // the bounding boxes of the program's inputs (layers no operator writes)
@@ -3465,7 +3527,7 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
   // (the current value stays unless a candidate is more than 3 % faster:
   // neighbouring budgets differ by less than the run-to-run noise)
   const int ftargets[] = {4, 5, 6, 7, 8, 10};
-  const int ft0 = e->fuse_target, fu0 = e->fuse;
+  const int ft0 = e->fuse_target, fu0 = e->fuse, fu20 = e->fuse2;
   int bft = ft0;
   double ft0_ms = 1e30;
   {
@@ -3492,7 +3554,7 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
       rc = best_of([&]() {
         HIPCK(hipSetDevice(e->device));
         OpTimer tm(e);
-        const int r = run_fused(e, a, fs, outs);
+        const int r = run_fused(e, a, nullptr, fs, outs);
         if (r == FUSE_DECLINED) declined = true;
         return r == FUSE_DECLINED ? VL_OK : r;
       }, &m);
@@ -3529,6 +3591,43 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
         const bool fused_now = e->fuse != 0;
         const double cur = fused_now ? bms : mu, alt = fused_now ? mu : bms;
         if (alt < 0.97 * cur) e->fuse = fused_now ? 0 : 1;
+      }
+    }
+    // two-layer groups: AND, NOT and OR of A and B in one fused pass against
+    // the three single operators (the current mode stays unless the other
+    // is more than 3 % faster; a declined group counts as slower)
+    if (rc == VL_OK && b && b->n > 0) {
+      FusedSpec f2{};
+      f2.nmorph = 1;
+      f2.kind[0] = MORPH_LAYER;
+      f2.nout = 3;
+      f2.tt[0] = 0xAAu & 0xCCu;
+      f2.tt[1] = 0xAAu & ~0xCCu & 0xFFu;
+      f2.tt[2] = 0xAAu | 0xCCu;
+      double mf = 1e30, ms = 1e30;
+      bool declined = false;
+      rc = best_of([&]() {
+        HIPCK(hipSetDevice(e->device));
+        OpTimer tm(e);
+        const int r = run_fused(e, a, b, f2, outs);
+        if (r == FUSE_DECLINED) declined = true;
+        return r == FUSE_DECLINED ? VL_OK : r;
+      }, &mf);
+      if (declined) mf = 1e30;
+      for (int rep = 0; rep < 2 && rc == VL_OK; rep++) {
+        double acc = 0;
+        rc = vl_and(e, a, b, outs[0]);
+        acc += e->last_ms;
+        if (rc == VL_OK) rc = vl_not(e, a, b, outs[1]);
+        acc += e->last_ms;
+        if (rc == VL_OK) rc = vl_or(e, a, b, outs[2]);
+        acc += e->last_ms;
+        ms = std::min(ms, acc);
+      }
+      if (rc == VL_OK) {
+        const bool now = e->fuse2 != 0;
+        const double cur = now ? mf : ms, alt = now ? ms : mf;
+        if (alt < 0.97 * cur) e->fuse2 = now ? 0 : 1;
       }
     }
     for (int k = 0; k < 4; k++)
@@ -3589,6 +3688,7 @@ int vl_engine_autotune(vl_engine* e, const vl_layer* a, const vl_layer* b, int64
     e->radix_bits = rb0;
     e->max_wg_cap = mw0;
     e->fuse = fu0;
+    e->fuse2 = fu20;
   }
   vl_layer_destroy(c);
   if (target) *target = e->target;

@@ -121,7 +121,9 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   const int nout = fs.nout;
   const int nmorph = fs.nmorph;
   const int32_t R = fs.halo;
-  const bool g0 = nmorph > 0 && fs.kind[0] == MORPH_GROW, g1 = nmorph > 1 && fs.kind[1] == MORPH_GROW;
+  // images rasterised from their slots: grown images and the second layer
+  const bool g0 = nmorph > 0 && (fs.kind[0] == MORPH_GROW || fs.kind[0] == MORPH_LAYER);
+  const bool g1 = nmorph > 1 && (fs.kind[1] == MORPH_GROW || fs.kind[1] == MORPH_LAYER);
   const bool e0 = nmorph > 0 && fs.kind[0] == MORPH_ERODE, e1 = nmorph > 1 && fs.kind[1] == MORPH_ERODE;
   if (lane == 0) {
 #pragma unroll
@@ -143,7 +145,12 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
   // edge of a grown image (raw edge plus offset, clamped to the window), bit
   // 2 = edge of an eroded image (clipped edge plus offset, clamped to the
   // tile), bits 3-4 = which border the slot holds in the border group (window
-  // low / high, tile low / high).  Other slots hold the clipped edge.
+  // low / high, tile low / high), bit 6 = edge of the second layer's image
+  // (the clipped edge of a second-layer box).  Other slots hold the clipped
+  // edge.  A second-layer box (the tile's entries past nA) fills only its
+  // second-layer image slots; a first-layer box every slot but those.  A slot
+  // a box does not fill holds the window corner in both edges (an empty
+  // image).
   uint32_t cflag[K];
   int32_t coffx[K], coffy[K];
 #pragma unroll
@@ -163,6 +170,8 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         f |= 2u;
         ox = hi ? ae : -aw;
         oy = hi ? an : -as;
+      } else if (kind == MORPH_LAYER) {
+        f |= 64u;
       } else {   // erosion by ae: edges shifted inwards
         f |= 4u;
         ox = oy = hi ? -ae : ae;
@@ -249,11 +258,12 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         const bool grown = f & 2u, eroded = f & 4u;
         const int32_t cx = clamp64((int64_t)(grown ? rx : kx) + coffx[k], eroded ? d.x0 : wx0, eroded ? d.x1 : wx1);
         const int32_t cy = clamp64((int64_t)(grown ? ry : ky) + coffy[k], eroded ? d.y0 : wy0, eroded ? d.y1 : wy1);
-        const uint32_t bsel = f >> 3;
+        const uint32_t bsel = (f >> 3) & 3u;
         const int32_t bxv = bsel == 0 ? wx0 : bsel == 1 ? wx1 : bsel == 2 ? d.x0 : d.x1;
         const int32_t byv = bsel == 0 ? wy0 : bsel == 1 ? wy1 : bsel == 2 ? d.y0 : d.y1;
-        vx[k] = i < n ? cx : (i == n ? bxv : INT32_MAX);
-        vy[k] = i < n ? cy : (i == n ? byv : INT32_MAX);
+        const bool nul = (i >= d.nA) != ((f & 64u) != 0u);   // a slot this box does not fill
+        vx[k] = i < n ? (nul ? wx0 : cx) : (i == n ? bxv : INT32_MAX);
+        vy[k] = i < n ? (nul ? wy0 : cy) : (i == n ? byv : INT32_MAX);
       }
       const int cnt = CPB * n + CPB;
       // Sort keys: the coordinate relative to the window corner (below 2^24:

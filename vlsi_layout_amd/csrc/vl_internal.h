@@ -293,10 +293,13 @@ constexpr int TILE_BLOCK_NMAX = 255;
 // those base values.  Tile windows are grown by `halo` (at least every
 // morph's reach), so each output is exact inside the tile.
 constexpr int FUSE_MAX_OUT = 8;
-enum : int32_t { MORPH_GROW = 1, MORPH_ERODE = 2 };
+// Images of a fused group besides its source S: a GROW of S, an erosion of
+// S, or a second input layer (its boxes are the tiles' layer-B entries; at
+// most one image is a layer).
+enum : int32_t { MORPH_GROW = 1, MORPH_ERODE = 2, MORPH_LAYER = 3 };
 struct FusedSpec {
   int32_t nmorph;                 // 0..2
-  int32_t kind[2];                // MORPH_GROW / MORPH_ERODE
+  int32_t kind[2];                // MORPH_GROW / MORPH_ERODE / MORPH_LAYER
   int32_t arg[2][4];              // GROW: e, n, w, s (all >= 0); ERODE: d > 0 in arg[k][0]
   int32_t halo;
   int32_t nout;                   // 1..FUSE_MAX_OUT
