@@ -28,9 +28,12 @@ The roofline figure is for the kernel stage with the largest device time in
 the timed step: algorithmic bytes of its launches (tile / fused stage: 16 B
 per entry read, 16 B per strip written, 48 B per tile; sort: 16 B per entry
 per pass) / their summed duration from HIP events on the stream they run on,
-against the MI355X's 8 TB/s HBM peak; for the graph form they are read from
-the eager batched form's timed steps (the same kernels and sizes: events
-recorded by graph replays give no usable elapsed times).
+against the MI355X's 8 TB/s HBM peak.  For the batched and graph forms the
+durations come from one more untimed step of the eager program with its
+chains on one worker stream (the kernel alone on the GPU: on four streams
+every launch shares the GPU with the other chains' kernels, and its events
+time the sharing); events recorded by graph replays give no usable elapsed
+times, and the eager program launches the same kernels with the same sizes.
 "compulsory_io_frac" is 16 B per input and output box of every operator /
 step time, against the same peak.
 
@@ -493,6 +496,20 @@ def main():
             result[m]["graph"] = gs
             if gs["fallbacks"] or gs["replays"] < args.steps:
                 raise SystemExit(f"graph form: the timed steps were not all replays ({gs})")
+        if m == "batched":
+            # Kernel figures of the program with its chains on one worker
+            # stream (one more, untimed step): on four streams each launch
+            # shares the GPU with the other chains' kernels, and its events
+            # time that sharing, not the kernel.
+            workers = eng.get("workers")
+            eng.set("workers", 1)
+            eng.set("profile", 1)
+            if batched_step():
+                raise SystemExit("OP1 ring assertion failed in the one-stream batched step")
+            eng.sync()
+            result[m]["stages_one_stream"] = stage_stats(eng, 1)
+            eng.set("profile", 0)
+            eng.set("workers", workers)
         print(f"[bench] {m}: {ms:.1f} ms/step", file=sys.stderr, flush=True)
 
     head = "graph" if "graph" in result else "batched" if "batched" in result else "plain"
@@ -504,9 +521,11 @@ def main():
     hs = result[rform]["stages"]
     # the roofline names the kernel stage with the largest device time in the
     # timed step (the fused kernel in the batched form, the tile stage in the
-    # plain one); its achieved rate = algorithmic bytes / summed duration
+    # plain one); its achieved rate = algorithmic bytes / summed duration of
+    # its launches, for the batched form from the one-stream step (the kernel
+    # alone on the GPU)
     dom = max(("fused", "tile", "sort"), key=lambda k: hs[k]["ms_per_step"])
-    d = hs[dom]
+    d = result[rform].get("stages_one_stream", hs)[dom]
     achieved = d["gbs"]
     per_launch = d["bytes_per_step"] / max(d["launches_per_step"], 1e-9)
     # HBM traffic per launch of the same stage on this workload, from the
@@ -538,7 +557,9 @@ def main():
         "autotuned": tuned,
         "batched_over_plain": (result[head]["boxes_per_s"] / result["plain"]["boxes_per_s"]
                                if head != "plain" and "plain" in result else None),
-        "roofline": {"bound": "hbm", "kernel": STAGES[dom], "stage": dom, "form": rform,
+        "roofline": {"bound": "hbm", "kernel": STAGES[dom], "stage": dom,
+                     "form": rform + (" (one worker stream)" if rform == "batched" else ""),
+                     "ms_per_launch": d["ms_per_step"] / max(d["launches_per_step"], 1e-9),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes_per_launch": per_launch, "traffic": traffic,
                      "traffic_source": traffic_src if traffic is not None else None,
