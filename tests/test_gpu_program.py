@@ -288,14 +288,15 @@ def test_fused_groups_match_plain(gpu_engine, group, layer):
     eng.close()
 
 
-@pytest.mark.parametrize("n", [9, 10, 11, 12, 19, 20, 21, 22])
+@pytest.mark.parametrize("n", [8, 9, 10, 11, 12, 19, 20, 21, 22, 40, 41, 42])
 @pytest.mark.parametrize("group", sorted(GROUPS))
 def test_fused_row_word_borders(gpu_engine, group, n):
     """n strips three units tall, eight apart, plus a bar meeting every row,
     in one window: with the grown and eroded edges each strip brings about
     six grid lines, so the windows of this sweep sit on the 64-, 128- and
-    256-slot tiers with row counts on both sides of the 64- and 128-row word
-    borders and runs ending on the grid's last row."""
+    256-slot tiers (10, 20 and 42 boxes, the last two grid lines of a full
+    64- or 256-slot grid being the tile's borders) with row counts on both
+    sides of the 64- and 128-row word borders, up to the grid's last row."""
     eng = V.Engine(0)
     eng.set("fuse_target", 60)
     rng = np.random.default_rng(2000 + n)

@@ -1309,7 +1309,7 @@ int layer_extent(vl_engine* e, const vl_layer* l, Box64* m) {
 }
 
 // ---- fused operator groups ------------------------------------------------------
-// Level 0 of a fused group: the fused tile kernel's two tiers (20 and 41 boxes
+// Level 0 of a fused group: the fused tile kernel's tiers (10, 20 and 42 boxes
 // per window), then x-sub-column splits of the tiles above the second tier's
 // capacity (windows grown by the halo), recursively.
 int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box* d_entries,

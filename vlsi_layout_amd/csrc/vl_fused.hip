@@ -21,8 +21,8 @@
 // The compressed grid holds, per box, x1 and x2 clipped to the window and the
 // two edges of each morphological image (grown edges clipped to the window,
 // eroded edges x1+d, x2-d clamped to the tile): 6 coordinates per box and
-// axis, so a 64-slot grid takes 9 boxes, a 128-slot grid 20 and a 256-slot
-// grid 41.
+// axis, plus the window's and the tile's borders, so a 64-slot grid takes 10
+// boxes, a 128-slot grid 20 and a 256-slot grid 42.
 #include "vl_tile_dev.h"
 
 namespace vl {
@@ -74,7 +74,10 @@ template <int P>
 __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc, int ntiles,
                                                const vl_box* __restrict__ entries, FusedSpec fs, FusedOut o) {
   constexpr int CPB = 6;               // coordinates per box and axis
-  constexpr int NMAX = P / CPB - 1;    // the last slot group holds the window and tile borders
+  // the slot group after the boxes holds the window and tile borders: four
+  // slots (components 0..3; components 4 and 5 repeat the window's, and a
+  // grid the boxes fill to within four slots has no room for them)
+  constexpr int NMAX = (P - 4) / CPB;
   constexpr int K = P / WAVE;          // slots per lane
   constexpr int NCOL = P - 1;
   constexpr int NW = P / WAVE;         // words per column

@@ -324,9 +324,9 @@ struct FusedOut {
 };
 // Capacities (boxes per tile window) of the fused kernel's tiers (64-, 128-
 // and 256-slot grids of 6 coordinates per box and axis).
-constexpr int FUSE_NMAX_TINY = 9;
+constexpr int FUSE_NMAX_TINY = 10;
 constexpr int FUSE_NMAX_SMALL = 20;
-constexpr int FUSE_NMAX_LARGE = 41;
+constexpr int FUSE_NMAX_LARGE = 42;
 void launch_tile_fused(int nmax, const TileDesc* desc, int ntiles, const vl_box* entries, const FusedSpec& fs,
                        const FusedOut& o, int wg, hipStream_t st);
 int tile_fused_max_wg(int nmax);
