@@ -509,6 +509,7 @@ GROUPS2 = {
 
 
 def run_group2(eng, S, T, steps, graph=False, reps=1):
+    eng.set("fuse2", 1)
     outs = [V.Layer(eng) for _ in steps]
     prog = V.Program(eng)
     pick = lambda x: S if x == "S" else T if x == "T" else outs[x]

@@ -108,4 +108,20 @@ def test_c4_full_chip_properties(gpu_engine):
         got = layer.boxes()
         assert np.array_equal(O.canonical(clip(meets(got, w), w)), exp), name
         del got
+    # the same three operators as one program with two-layer groups on: the
+    # group's output streams would not fit next to each other, so it runs
+    # operator by operator (on worker streams that may free the root
+    # engine's cached buffers) and gives the same layers
+    eng.set("fuse2", 1)
+    P = [V.Layer(eng) for _ in range(3)]
+    prog = V.Program(eng)
+    prog.add(V.K_BOOST_CONTROL_AND, LA, LB, P[0])
+    prog.add(V.K_BOOST_CONTROL_NOT, LA, LB, P[1])
+    prog.add(V.VL_OP_GROW, LA, None, P[2], (1, 1, 1, 1))
+    prog.run()
+    for got, ref, name in zip(P, (AND, NOT, GROW), ("and", "not", "grow")):
+        assert got.n_used == ref.n_used, name
+        assert V.klee_area(got) == V.klee_area(ref), name
+    assert V.xor_empty(P[2], GROW)
+    prog.close()
     eng.close()

@@ -77,11 +77,18 @@ int vl::set_error(int code, const char* fmt, ...) {
   } while (0)
 
 // ---------------------------------------------------------------------------
-// device memory pool (stream-ordered reuse: every engine launch uses one stream)
+// device memory pool (stream-ordered reuse: every engine launch uses one
+// stream).  When the device is full, the pool frees its cached buffers and,
+// for a worker engine, those of the engine that created it (`fallback`,
+// idle while its workers run a program): worker streams running next to
+// each other must not fail where one stream would not.
 struct Pool {
   std::map<size_t, std::vector<void*>> free_;
   size_t live = 0;
+  size_t cached = 0;   // bytes held in free_
   bool no_trim = false;
+  Pool* fallback = nullptr;
+  std::mutex m;   // the fallback's cache is freed from a worker's thread
   static size_t round(size_t b) {
     size_t r = 256;
     while (r < b) r <<= 1;
@@ -90,18 +97,26 @@ struct Pool {
   void* get(size_t b, size_t* rounded) {
     size_t r = round(b ? b : 1);
     *rounded = r;
+    std::unique_lock<std::mutex> lk(m);
     auto it = free_.find(r);
     if (it != free_.end() && !it->second.empty()) {
       void* p = it->second.back();
       it->second.pop_back();
+      cached -= r;
       return p;
     }
     void* p = nullptr;
     if (hipMalloc(&p, r) != hipSuccess) {
       (void)hipGetLastError();
       if (no_trim) return nullptr;   // buffers may be referenced by a graph being captured
-      trim();
-      if (hipMalloc(&p, r) != hipSuccess) {
+      trim_locked();
+      bool ok = hipMalloc(&p, r) == hipSuccess;
+      if (!ok && fallback) {
+        (void)hipGetLastError();
+        fallback->trim();
+        ok = hipMalloc(&p, r) == hipSuccess;
+      }
+      if (!ok) {
         (void)hipGetLastError();
         return nullptr;
       }
@@ -110,15 +125,33 @@ struct Pool {
     return p;
   }
   void put(void* p, size_t r) {
-    if (p) free_[r].push_back(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(m);
+    free_[r].push_back(p);
+    cached += r;
   }
   void trim() {
+    std::lock_guard<std::mutex> lk(m);
+    trim_locked();
+  }
+  void trim_locked() {
     for (auto& kv : free_)
       for (void* p : kv.second) {
         (void)hipFree(p);
         live -= kv.first;
       }
     free_.clear();
+    cached = 0;
+  }
+  // Device bytes an allocation can count on: free memory plus this pool's cache.
+  size_t available() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      (void)hipGetLastError();
+      return SIZE_MAX;
+    }
+    std::lock_guard<std::mutex> lk(m);
+    return fr + cached;
   }
 };
 
@@ -145,7 +178,7 @@ struct vl_engine {
   int erode_kernel = 1;          // SIZE(-d) by the per-tile erosion kernel (0: by complement, grow, difference)
   int max_wg_cap = 0;            // 0 = occupancy limit
   int fuse = 1;                  // programs: fuse operator groups over one source layer
-  int fuse2 = 1;                 // programs: groups may take a second layer as an image
+  int fuse2 = 0;                 // programs: groups may take a second layer as an image (set by the autotuner)
   int fuse_target = 6;           // mean boxes per tile of a fused group's grid
   int bin_mode = 2;              // 2: atomic binning, atomics grouped per wave (1: per run); 0: radix sort
   int radix_bits = 8;            // digit width of the radix sorts (8 or 11)
@@ -942,6 +975,9 @@ int process_tiles(vl_engine* e, const TileDesc* d_desc, int ntiles, const vl_box
   return VL_OK;
 }
 
+// Status of run_fused when the group cannot run fused here (nothing written;
+// the caller runs the group's operators one by one).
+constexpr int FUSE_DECLINED = 1000;
 constexpr int BIN_TOO_MANY = 1003;   // bin_entries without coarsening: the grid would need it
 
 // Bins up to two inputs into grid g: entries of tile t (layer A, then B, each
@@ -1335,10 +1371,20 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
     nregions += G[t];
   }
   // stream size per output: the expected strips plus one chunk per
-  // workgroup; regrown as in the single-operator tile stage
+  // workgroup; regrown as in the single-operator tile stage.  The streams of
+  // all outputs exist at once: when they would take more than half of the
+  // device memory left (a full-chip layer), the group runs operator by
+  // operator instead, which holds one output's at a time.
   uint64_t scap[FUSE_MAX_OUT];
   for (int k = 0; k < nout; k++)
     scap[k] = (uint64_t)(e->fuse_region_scale * (double)nentries) + (uint64_t)nregions * STRIP_CHUNK + 1024;
+  const double budget = 0.5 * (double)e->pool.available();
+  auto fits = [&]() {
+    double need = 0;
+    for (int k = 0; k < nout; k++) need += (double)scap[k] * (sizeof(vl_box) + 1);
+    return need <= budget;
+  };
+  if (e->cap.mode == CAP_EAGER && !fits()) return FUSE_DECLINED;
   std::unique_ptr<Scratch> rs;
   vl_box* scr[FUSE_MAX_OUT] = {};
   uint8_t* scrT[FUSE_MAX_OUT] = {};
@@ -1405,6 +1451,7 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
         e->fuse_region_scale =
             std::max(e->fuse_region_scale, 1.25 * (double)demand / std::max<double>(1.0, nentries));
       }
+      if (e->cap.mode == CAP_EAGER && !fits()) return FUSE_DECLINED;
       if (g_debug) fprintf(stderr, "[vl]  fused strip stream short: retry\n");
       continue;
     }
@@ -1456,9 +1503,6 @@ int process_tiles_fused(vl_engine* e, const TileDesc* d_desc, int ntiles, const 
   return VL_OK;
 }
 
-// Status of run_fused when the group cannot run fused here (nothing written;
-// the caller runs the group's operators one by one).
-constexpr int FUSE_DECLINED = 1000;
 
 // Outputs c[k] = fs.tt[k](S, morphological images of S) of a fused group.
 int run_fused(vl_engine* e, const vl_layer* S, const vl_layer* T, const FusedSpec& fs_in, vl_layer* const* c_in) {
@@ -2735,6 +2779,13 @@ static int run_step(vl_engine* e, const std::vector<ProgOp>& ops, const ProgStep
         if (ops[i].op == -1 && ops[i].a->n > 0) (*failures)++;
       return VL_OK;
     }
+    // a group whose scratch does not fit (its output streams are sized for
+    // every output at once) runs operator by operator, which needs less
+    if (rc == VL_ENOMEM && e->cap.mode == CAP_EAGER) {
+      if (g_debug) fprintf(stderr, "[vl] fused group out of memory, run by operator: %s\n", g_err.c_str());
+      e->pool.trim();
+      rc = FUSE_DECLINED;
+    }
     if (rc != FUSE_DECLINED) return rc;
     g_err.clear();
   }
@@ -2789,6 +2840,7 @@ static int program_workers(vl_program* p, int n) {
     vl_engine* w = nullptr;
     VLCK(engine_new(e->device, &w));
     w->root = e;
+    w->pool.fallback = &e->pool;
     p->workers.push_back(w);
   }
   for (int k = 0; k < n; k++) {
@@ -3347,6 +3399,7 @@ int vl_server_create(vl_engine* e, const vl_layer* const* layers, int nlayers, i
     int rc = engine_new(e->device, &w);
     if (rc != VL_OK) return cleanup(rc);
     w->root = e;
+    w->pool.fallback = &e->pool;
     w->nmax = e->nmax;
     w->target = e->target;
     w->erode_target = e->erode_target;
