@@ -1207,31 +1207,22 @@ int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
   if (o.nstitch > 0) {
     const int64_t n = (int64_t)o.nstitch;
     Scratch ss(e);
-    uint64_t* keys = ss.get<uint64_t>(n);
-    uint32_t* vals = ss.get<uint32_t>(n);
-    uint64_t* keys2 = ss.get<uint64_t>(n);
-    uint32_t* vals2 = ss.get<uint32_t>(n);
+    const uint64_t slots = stitch_table_slots(n);
+    unsigned long long* tk = ss.get<unsigned long long>(slots);
+    uint32_t* tv = ss.get<uint32_t>(slots);
     int32_t* next = ss.get<int32_t>(n);
     uint8_t* hasp = ss.get<uint8_t>(n);
-    void* rtmp = ss.get<char>(radix_plan(n, e->radix_bits).tmp_bytes);
     void* ptmp = ss.get<char>(partition_tmp_bytes(n));
     uint64_t* cnt = ss.get<uint64_t>(1);
     if (ss.oom) return fail(VL_ENOMEM, "stitch scratch");
-    // starts sorted by (x1, y1) relative to the domain: xb + yb key bits
+    // corner keys (x1, y1) relative to the domain: y in the low yb bits
     auto bits_for = [](int64_t v) {
       int b = 1;
       while (b < 63 && (v >> b) != 0) b++;
       return b;
     };
-    const int yb = bits_for(g.y1 - g.y0), xb = bits_for(g.x1 - g.x0);
-    uint64_t ns = 0;
-    launch_stitch_starts(o.stitch.p, o.stitch_tag.p, n, g.x0, g.y0, yb, keys, vals, cnt, ptmp, e->st);
-    VLCK(readback(e, &ns, cnt, sizeof(ns)));
-    if (radix_sort_pairs_bits(keys, vals, keys2, vals2, (int64_t)ns, xb + yb, rtmp, e->st, e->radix_bits)) {
-      std::swap(keys, keys2);
-      std::swap(vals, vals2);
-    }
-    launch_stitch_link(o.stitch.p, o.stitch_tag.p, n, keys, vals, (int64_t)ns, g.x0, g.y0, yb, next, hasp, e->st);
+    const int yb = bits_for(g.y1 - g.y0);
+    launch_stitch_link(o.stitch.p, o.stitch_tag.p, n, g.x0, g.y0, yb, tk, tv, next, hasp, e->st);
     VLCK(o.out.ensure(o.nout + (uint64_t)n, o.nout));
     launch_stitch_emit(o.stitch.p, n, next, hasp, o.out.p, o.nout, ptmp, cnt, e->st);
     uint64_t heads = 0;

@@ -406,16 +406,14 @@ void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl
                            hipStream_t st);
 size_t partition_tmp_bytes(int64_t n);
 // X-stitch of strips whose left/right ends sit on internal x-seams.
-// Starts (strips on a left seam): keys relative to the domain origin (x0, y0;
-// y in the low yb bits) and strip indices, compacted; *count = their number.
-// tmp holds partition_tmp_bytes(n).
-void launch_stitch_starts(const vl_box* s, const uint8_t* tag, int64_t n, int64_t x0, int64_t y0, int yb,
-                          uint64_t* keys, uint32_t* vals, uint64_t* count, void* tmp, hipStream_t st);
-// Links every strip ending on a right seam to the start at its end corner
-// with the same rows (skeys/svals: the ns starts sorted by key).
-void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, const uint64_t* skeys,
-                        const uint32_t* svals, int64_t ns, int64_t x0, int64_t y0, int yb, int32_t* next,
-                        uint8_t* has_prev, hipStream_t st);
+// Links every strip ending on a right seam to the strip starting on a left
+// seam at its lower-right corner with the same rows (next[i], has_prev[j]),
+// through a hash table of the starts (tk: stitch_table_slots(n) keys, tv as
+// many indices) keyed relative to the domain origin (x0, y0; y in the low yb
+// bits).
+uint64_t stitch_table_slots(int64_t n);
+void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, int64_t x0, int64_t y0, int yb,
+                        unsigned long long* tk, uint32_t* tv, int32_t* next, uint8_t* has_prev, hipStream_t st);
 void launch_stitch_emit(const vl_box* s, int64_t n, const int32_t* next, const uint8_t* has_prev,
                         vl_box* out, uint64_t out_base, void* tmp, uint64_t* count, hipStream_t st);
 

@@ -5,10 +5,11 @@
 // level with coarser rows.  Strips that only touch internal column borders
 // (SEAM_L/R) carry an exact global run and only need joining along x: a strip
 // ending at x=s and a strip starting at x=s with the same [y1,y2) are the same
-// maximal strip (runs at one x are disjoint, so (s, y1) names at most one
-// start).  Starts are radix-sorted by (x1,y1); every end looks up its
-// successor by binary search; chain heads walk their successors and emit the
-// joined strip.
+// maximal strip (the pieces are disjoint, so the corner (s, y1) names at most
+// one start).  Starts go into an open-addressing hash table keyed by their
+// lower-left corner (64-bit CAS, linear probing); every end looks up the
+// start at its lower-right corner; chain heads walk their successors and emit
+// the joined strip.  The result does not depend on the table's layout.
 #include "vl_internal.h"
 #include "vl_dev.h"
 
@@ -60,46 +61,56 @@ __device__ __forceinline__ uint64_t rel_key(int32_t x, int32_t y, int64_t x0, in
   return ((uint64_t)((int64_t)x - x0) << yb) | (uint64_t)((int64_t)y - y0);
 }
 
-// Starts are the strips on a left seam: flag, then (after a scan) their keys
-// and indices at the scanned positions, so only starts are sorted.
-__global__ void k_start_flags(const uint8_t* __restrict__ tag, int64_t n, uint32_t* flag) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    flag[i] = (tag[i] & SEAM_L) ? 1u : 0u;
+// Hash table of the starts: 2^k slots (at least twice the strips), key
+// HASH_EMPTY = free (no corner key reaches it: x - x0 < 2^32 above y - y0).
+constexpr uint64_t HASH_EMPTY = ~0ull;
+
+__device__ __forceinline__ uint64_t hash_slot(uint64_t k, uint64_t mask) {
+  k ^= k >> 31;
+  k *= 0x9E3779B97F4A7C15ull;
+  k ^= k >> 29;
+  return k & mask;
 }
 
-__global__ void k_start_keys(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n,
-                             const uint32_t* __restrict__ pre, int64_t x0, int64_t y0, int yb, uint64_t* keys,
-                             uint32_t* vals) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    if (tag[i] & SEAM_L) {
-      keys[pre[i]] = rel_key(s[i].x1, s[i].y1, x0, y0, yb);
-      vals[pre[i]] = (uint32_t)i;
+__global__ void k_hash_starts(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n, int64_t x0,
+                              int64_t y0, int yb, unsigned long long* __restrict__ tk, uint32_t* __restrict__ tv,
+                              uint64_t mask) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (!(tag[i] & SEAM_L)) continue;
+    const unsigned long long key = rel_key(s[i].x1, s[i].y1, x0, y0, yb);
+    uint64_t slot = hash_slot(key, mask);
+    for (uint64_t probe = 0; probe <= mask; probe++) {   // fewer starts than slots: ends at a free slot
+      const unsigned long long prev = atomicCAS(&tk[slot], (unsigned long long)HASH_EMPTY, key);
+      if (prev == HASH_EMPTY || prev == key) {
+        tv[slot] = (uint32_t)i;
+        break;
+      }
+      slot = (slot + 1) & mask;
     }
+  }
 }
 
-__global__ void k_start_count(const uint32_t* pre, const uint8_t* tag, int64_t n, uint64_t* count) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *count = n ? (uint64_t)pre[n - 1] + ((tag[n - 1] & SEAM_L) ? 1u : 0u) : 0u;
-}
-
-__global__ void k_stitch_link(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n,
-                              const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t ns,
-                              int64_t x0, int64_t y0, int yb, int32_t* next, uint8_t* has_prev) {
+__global__ void k_hash_link(const vl_box* __restrict__ s, const uint8_t* __restrict__ tag, int64_t n, int64_t x0,
+                            int64_t y0, int yb, const unsigned long long* __restrict__ tk,
+                            const uint32_t* __restrict__ tv, uint64_t mask, int32_t* next, uint8_t* has_prev) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int32_t nx = -1;
     if (tag[i] & SEAM_R) {
       const vl_box b = s[i];
-      const uint64_t k = rel_key(b.x2, b.y1, x0, y0, yb);
-      int64_t lo = 0, hi = ns;
-      while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (skeys[mid] < k) lo = mid + 1; else hi = mid;
-      }
-      if (lo < ns && skeys[lo] == k) {
-        const uint32_t j = svals[lo];
-        if (s[j].y2 == b.y2) {
-          nx = (int32_t)j;
-          has_prev[j] = 1;
+      const unsigned long long key = rel_key(b.x2, b.y1, x0, y0, yb);
+      uint64_t slot = hash_slot(key, mask);
+      for (uint64_t probe = 0; probe <= mask; probe++) {
+        const unsigned long long k = tk[slot];
+        if (k == HASH_EMPTY) break;
+        if (k == key) {
+          const uint32_t j = tv[slot];
+          if (s[j].y2 == b.y2) {
+            nx = (int32_t)j;
+            has_prev[j] = 1;
+          }
+          break;
         }
+        slot = (slot + 1) & mask;
       }
     }
     next[i] = nx;
@@ -155,27 +166,20 @@ void launch_partition_side(const vl_box* side, const uint8_t* tag, int64_t n, vl
   hipLaunchKernelGGL(k_part_counts, dim3(1), dim3(64), 0, st, pre, tag, n, counts);
 }
 
-void launch_stitch_starts(const vl_box* s, const uint8_t* tag, int64_t n, int64_t x0, int64_t y0, int yb,
-                          uint64_t* keys, uint32_t* vals, uint64_t* count, void* tmp, hipStream_t st) {
-  if (n <= 0) {
-    launch_fill_u64(count, 0ull, 1, st);
-    return;
-  }
-  uint32_t* flag = (uint32_t*)tmp;
-  uint32_t* pre = flag + n;
-  char* stmp = (char*)(((uintptr_t)(pre + n) + 255) & ~(uintptr_t)255);
-  hipLaunchKernelGGL(k_start_flags, dim3(grid_for(n)), dim3(256), 0, st, tag, n, flag);
-  scan_exclusive_u32(flag, pre, (size_t)n, nullptr, stmp, st);
-  hipLaunchKernelGGL(k_start_keys, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, pre, x0, y0, yb, keys, vals);
-  hipLaunchKernelGGL(k_start_count, dim3(1), dim3(64), 0, st, pre, tag, n, count);
+uint64_t stitch_table_slots(int64_t n) {
+  uint64_t h = 1024;
+  while (h < 2 * (uint64_t)(n > 0 ? n : 1)) h <<= 1;
+  return h;
 }
 
-void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, const uint64_t* skeys,
-                        const uint32_t* svals, int64_t ns, int64_t x0, int64_t y0, int yb, int32_t* next,
-                        uint8_t* has_prev, hipStream_t st) {
+void launch_stitch_link(const vl_box* s, const uint8_t* tag, int64_t n, int64_t x0, int64_t y0, int yb,
+                        unsigned long long* tk, uint32_t* tv, int32_t* next, uint8_t* has_prev, hipStream_t st) {
   if (n <= 0) return;
+  const uint64_t slots = stitch_table_slots(n);
+  launch_fill_u64((uint64_t*)tk, HASH_EMPTY, slots, st);
   hipLaunchKernelGGL(k_zero_u8, dim3(grid_for(n)), dim3(256), 0, st, has_prev, n);
-  hipLaunchKernelGGL(k_stitch_link, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, skeys, svals, ns, x0, y0, yb,
+  hipLaunchKernelGGL(k_hash_starts, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, x0, y0, yb, tk, tv, slots - 1);
+  hipLaunchKernelGGL(k_hash_link, dim3(grid_for(n)), dim3(256), 0, st, s, tag, n, x0, y0, yb, tk, tv, slots - 1,
                      next, has_prev);
 }
 
