@@ -185,6 +185,7 @@ struct vl_engine {
   int workers = 4;               // programs: chains run concurrently on up to this many worker streams
   int and_check = 1;             // programs: ANDs read only by ASSERT_ZERO are tested for emptiness first
   int check_target = 6;          // mean binned boxes per tile of that test's grid
+  int tile_shape = 32;           // tiles shaped like the boxes, up to this aspect (0: square tiles)
   int64_t fused_groups = 0;      // groups of operators run fused so far (programs)
   // Strips per binned entry that a tile stage's scratch stream is sized for
   // (plain tile stage / each output of a fused group).  Raised when a stream
@@ -244,6 +245,7 @@ struct vl_layer {
   // layer clears it).  Operators read it instead of reducing the boxes again.
   mutable bool mbr_ok = false;
   mutable int64_t mbr[4] = {0, 0, 0, 0};
+  mutable double sum_w = 0.0, sum_h = 0.0;   // summed box widths and heights (valid with mbr)
   // A box known to contain every box of the layer (not necessarily the
   // smallest), valid while version == bound_version: set by producers that
   // know it without a reduction (a fused group's outputs lie in its domain).
@@ -470,36 +472,57 @@ struct Box64 {
 };
 
 // Bounding boxes of A (transformed) and B; validates boxes.
-int mbr2(vl_engine* e, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb, Box64* ma, Box64* mb) {
+// sums (may be null): summed widths and heights of A's (transformed) boxes,
+// then of B's.
+int mbr2(vl_engine* e, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb, Box64* ma, Box64* mb,
+         double* sums = nullptr) {
   Scratch s(e);
   int32_t* d = s.get<int32_t>(8);
-  if (!d) return fail(VL_ENOMEM, "scratch");
+  unsigned long long* ds = s.get<unsigned long long>(4);
+  if (!d || !ds) return fail(VL_ENOMEM, "scratch");
   // (INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN) twice, by fill kernels
   for (int k = 0; k < 8; k += 4) {
     launch_fill_u32((uint32_t*)d + k, (uint32_t)INT32_MAX, 2, e->st);
     launch_fill_u32((uint32_t*)d + k + 2, (uint32_t)INT32_MIN, 2, e->st);
   }
+  launch_fill_u64((uint64_t*)ds, 0ull, 4, e->st);
   VLCK(clear_err(e));
-  launch_mbr(a, na, xa, d, e->d_err, e->st);
-  launch_mbr(b, nb, Xform{0, 0, 0, 0}, d + 4, e->d_err, e->st);
+  launch_mbr(a, na, xa, d, e->d_err, e->st, ds);
+  launch_mbr(b, nb, Xform{0, 0, 0, 0}, d + 4, e->d_err, e->st, ds + 2);
   int32_t h[8];
+  unsigned long long hs[4];
   uint32_t err = 0;
-  const RB r[2] = {{h, d, sizeof(h), true}, {&err, e->d_err, 4, true}};
-  VLCK(sync_read(e, r, 2));
+  const RB r[3] = {{h, d, sizeof(h), true}, {&err, e->d_err, 4, true}, {hs, ds, sizeof(hs), true}};
+  VLCK(sync_read(e, r, 3));
   if (err & ERR_INVALID_BOX)
     return fail(VL_EINVAL, "invalid box: input has x1>=x2 or y1>=y2, or a transformed box became invalid");
   *ma = Box64{h[0], h[1], h[2], h[3]};
   *mb = Box64{h[4], h[5], h[6], h[7]};
+  if (sums)
+    for (int k = 0; k < 4; k++) sums[k] = (double)hs[k];
   return VL_OK;
 }
 
-Grid make_grid(Box64 dom, int64_t nboxes, int target, int64_t max_tiles = (int64_t)1 << 26) {
+// Tile height / width for boxes of summed widths sw and heights sh: their
+// mean aspect, within [1/cap, cap] (cap 0: square tiles).  A tile shaped
+// like the boxes is crossed by the fewest of them (for tiles of a given area,
+// the expected tiles a box meets, (1 + w/tw)(1 + h/th), is least at
+// th/tw = h/w on average), so the grid holds the fewest duplicate entries
+// and seam pieces: vertical wires get tall tiles, horizontal wires wide ones.
+double tile_shape(double sw, double sh, int cap) {
+  if (cap <= 0 || !(sw > 0) || !(sh > 0)) return 1.0;
+  return std::min((double)cap, std::max(1.0 / cap, sh / sw));
+}
+
+// Uniform grid of about nboxes / target tiles over dom, each tile about
+// `shape` times as tall as it is wide.
+Grid make_grid(Box64 dom, int64_t nboxes, int target, double shape = 1.0, int64_t max_tiles = (int64_t)1 << 26) {
   Grid g;
   g.x0 = dom.x1; g.y0 = dom.y1; g.x1 = dom.x2; g.y1 = dom.y2;
   const int64_t W = dom.x2 - dom.x1, H = dom.y2 - dom.y1;
   double nt = std::max(1.0, (double)nboxes / std::max(1, target));
   nt = std::min(nt, (double)max_tiles);
-  double aspect = (double)W / (double)H;
+  double aspect = (double)W * shape / (double)H;
   int64_t cols = std::max<int64_t>(1, (int64_t)llround(std::sqrt(nt * aspect)));
   cols = std::min(cols, W);
   int64_t rows = std::max<int64_t>(1, (int64_t)std::ceil(nt / (double)cols));
@@ -1238,7 +1261,7 @@ int finish_op(vl_engine* e, Grid g, Outs& o, vl_layer* c, double t_start) {
 
 int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const vl_box* b, int64_t nb,
            vl_layer* c, long long* area_out, int32_t erode = 0, const Box64* ka = nullptr,
-           const Box64* kb = nullptr) {
+           const Box64* kb = nullptr, double shape = 1.0) {
   if (!two_layer(top) && top != TOP_OR) { b = nullptr; nb = 0; kb = nullptr; }
   Box64 ma, mb;
   VLCK(op_mbrs(e, a, na, xa, b, nb, ka, kb, &ma, &mb));
@@ -1267,7 +1290,7 @@ int run_op(vl_engine* e, int top, const vl_box* a, int64_t na, Xform xa, const v
   }
   const bool area = area_out != nullptr;
   Outs o(e);
-  Grid g = make_grid(dom, na + nb, erode > 0 ? e->erode_target : e->target);
+  Grid g = make_grid(dom, na + nb, erode > 0 ? e->erode_target : e->target, shape);
   const double t_start = g_debug ? now_ms() : 0.0;
   const Xform xr = erode > 0 ? Xform{erode, erode, erode, erode} : xa;
   VLCK(bin_and_process(e, g, top, a, na, xa, xr, b, nb, area, erode, o));
@@ -1317,8 +1340,11 @@ int snapshot(vl_engine* e, const vl_layer* l, Scratch& s, const vl_box** out) {
 int layer_mbr(vl_engine* e, const vl_layer* l, Box64* m) {
   if (!l->mbr_ok) {
     Box64 ma{INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN}, mb;
-    if (l->n > 0) VLCK(mbr2(e, l->d, l->n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb));
+    double sums[4] = {0, 0, 0, 0};
+    if (l->n > 0) VLCK(mbr2(e, l->d, l->n, Xform{0, 0, 0, 0}, nullptr, 0, &ma, &mb, sums));
     l->mbr[0] = ma.x1; l->mbr[1] = ma.y1; l->mbr[2] = ma.x2; l->mbr[3] = ma.y2;
+    l->sum_w = sums[0];
+    l->sum_h = sums[1];
     l->mbr_ok = true;
   }
   *m = Box64{l->mbr[0], l->mbr[1], l->mbr[2], l->mbr[3]};
@@ -1536,7 +1562,16 @@ int run_fused(vl_engine* e, const vl_layer* S, const vl_layer* T, const FusedSpe
   const int64_t h = fs.halo;
   if (dom.x1 - h < INT32_MIN || dom.y1 - h < INT32_MIN || dom.x2 + h > INT32_MAX || dom.y2 + h > INT32_MAX)
     return FUSE_DECLINED;
-  const Grid g = make_grid(dom, S->n + (T ? T->n : 0), e->fuse_target);
+  double shape = 1.0;
+  if (e->tile_shape) {
+    double sw = S->sum_w, sh = S->sum_h;
+    if (T && T->mbr_ok) {
+      sw += T->sum_w;
+      sh += T->sum_h;
+    }
+    shape = tile_shape(sw, sh, e->tile_shape);
+  }
+  const Grid g = make_grid(dom, S->n + (T ? T->n : 0), e->fuse_target, shape);
   // windows must stay small against the tiles (each box meets a few windows),
   // and below 2^24 - 1 DBU wide and tall (the kernel sorts coordinates relative
   // to the window corner in 24 bits)
@@ -1600,7 +1635,13 @@ int binary(vl_engine* e, int top, const vl_layer* a, const vl_layer* b, vl_layer
   const vl_box* pb = b ? b->d : nullptr;
   if (c == a) VLCK(snapshot(e, a, s, &pa));
   if (b && c == b) VLCK(snapshot(e, b, s, &pb));
-  return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr, 0, &ka, b ? &kb : nullptr);
+  // tiles shaped like the inputs' boxes (B's count only where it is binned)
+  double shape = 1.0;
+  if (e->tile_shape) {
+    const bool with_b = b && (two_layer(top) || top == TOP_OR);
+    shape = tile_shape(a->sum_w + (with_b ? b->sum_w : 0.0), a->sum_h + (with_b ? b->sum_h : 0.0), e->tile_shape);
+  }
+  return run_op(e, top, pa, a->n, xa, pb, b ? b->n : 0, c, nullptr, 0, &ka, b ? &kb : nullptr, shape);
 }
 
 // One input of a semi-join: boxes and the transform applied to them.
@@ -1871,6 +1912,9 @@ int vl_engine_set_param(vl_engine* e, const char* key, int64_t v) {
     e->radix_bits = (int)v;
   } else if (k == "and_check") {
     e->and_check = v != 0;
+  } else if (k == "tile_shape") {
+    if (v < 0 || v > 1024) return fail(VL_EINVAL, "tile_shape out of range (0..1024)");
+    e->tile_shape = (int)v;
   } else if (k == "check_target") {
     if (v < 1 || v > 4096) return fail(VL_EINVAL, "check_target out of range");
     e->check_target = (int)v;
@@ -1897,6 +1941,7 @@ int vl_engine_get_param(vl_engine* e, const char* key, int64_t* v) {
   else if (k == "workers") *v = e->workers;
   else if (k == "and_check") *v = e->and_check;
   else if (k == "check_target") *v = e->check_target;
+  else if (k == "tile_shape") *v = e->tile_shape;
   else if (k == "fused_groups") *v = e->fused_groups;
   else return fail(VL_EINVAL, "unknown parameter '%s'", key);
   return VL_OK;
@@ -2202,9 +2247,10 @@ int vl_size(vl_engine* e, const vl_layer* a, int32_t d, vl_layer* c) {
   if (c == a) VLCK(snapshot(e, a, s, &pa));
   // Per-tile erosion when the tiles of the operator's grid are at least 4d
   // wide and tall (each box then meets only a few tile windows).
-  const Grid g = make_grid(ma, a->n, e->erode_target);
+  const double shape = tile_shape(a->sum_w, a->sum_h, e->tile_shape);
+  const Grid g = make_grid(ma, a->n, e->erode_target, shape);
   if (e->erode_kernel && g.tw >= 4 * (int64_t)r && g.th >= 4 * (int64_t)r) {
-    const int rc = run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r, &ma);
+    const int rc = run_op(e, TOP_UNION, pa, a->n, Xform{0, 0, 0, 0}, nullptr, 0, c, nullptr, r, &ma, nullptr, shape);
     if (rc != ERODE_FALLBACK) return rc;
     g_err.clear();   // a tile no erosion kernel holds: the composition below
   }
@@ -2848,6 +2894,7 @@ static int program_workers(vl_program* p, int n) {
     w->bin_mode = e->bin_mode;
     w->radix_bits = e->radix_bits;
     w->check_target = e->check_target;
+    w->tile_shape = e->tile_shape;
     w->region_scale = std::max(w->region_scale, e->region_scale);
     w->fuse_region_scale = std::max(w->fuse_region_scale, e->fuse_region_scale);
     (void)prof_resolve(w);

@@ -156,8 +156,10 @@ void launch_grid(int64_t x_lo, int64_t y_lo, int64_t x_hi, int64_t y_hi, int row
                  vl_box* out, hipStream_t st);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
 // MBR of boxes after a transform; validates boxes (ERR_INVALID_BOX).
-// mbr[4] must be pre-set to (INT_MAX, INT_MAX, INT_MIN, INT_MIN).
-void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st);
+// mbr[4] must be pre-set to (INT_MAX, INT_MAX, INT_MIN, INT_MIN).  sums (may
+// be null, zeroed by the caller): += summed widths, heights of the boxes.
+void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st,
+                unsigned long long* sums = nullptr);
 // Exclusive scans.  tmp must hold scan_tmp_bytes(n) bytes.  total may be null.
 size_t scan_tmp_bytes(size_t n);
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, void* tmp,

@@ -27,9 +27,13 @@ __global__ void k_expect(const uint32_t* __restrict__ p, Expect8 ex, int n, uint
 }
 
 // Bounding box of the transformed boxes; flags invalid input or output boxes.
-__global__ void k_mbr(const vl_box* __restrict__ b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err) {
+// sums (may be null): += the boxes' summed widths and heights (sums[0], [1]).
+__global__ void k_mbr(const vl_box* __restrict__ b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err,
+                      unsigned long long* sums) {
   __shared__ int32_t s[4][SCAN_NT / WAVE];
+  __shared__ unsigned long long sw[2][SCAN_NT / WAVE];
   int32_t x1 = INT32_MAX, y1 = INT32_MAX, x2 = INT32_MIN, y2 = INT32_MIN;
+  unsigned long long tw = 0, th = 0;
   uint32_t bad = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -43,7 +47,11 @@ __global__ void k_mbr(const vl_box* __restrict__ b, int64_t n, Xform xf, int32_t
     y1 = min(y1, (int32_t)max(c1, (int64_t)INT32_MIN));
     x2 = max(x2, (int32_t)min(a2, (int64_t)INT32_MAX));
     y2 = max(y2, (int32_t)min(c2, (int64_t)INT32_MAX));
+    tw += a2 > a1 ? (unsigned long long)(a2 - a1) : 0ull;
+    th += c2 > c1 ? (unsigned long long)(c2 - c1) : 0ull;
   }
+  tw = wave_sum(tw);
+  th = wave_sum(th);
   for (int d = WAVE / 2; d > 0; d >>= 1) {
     x1 = min(x1, __shfl_xor(x1, d, WAVE));
     y1 = min(y1, __shfl_xor(y1, d, WAVE));
@@ -54,14 +62,23 @@ __global__ void k_mbr(const vl_box* __restrict__ b, int64_t n, Xform xf, int32_t
     if (__lane_id() == 0) atomicOr(err, ERR_INVALID_BOX);
   }
   const int w = threadIdx.x / WAVE;
-  if (__lane_id() == 0) { s[0][w] = x1; s[1][w] = y1; s[2][w] = x2; s[3][w] = y2; }
+  if (__lane_id() == 0) {
+    s[0][w] = x1; s[1][w] = y1; s[2][w] = x2; s[3][w] = y2;
+    sw[0][w] = tw; sw[1][w] = th;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < (int)(blockDim.x / WAVE); i++) {
       x1 = min(x1, s[0][i]); y1 = min(y1, s[1][i]); x2 = max(x2, s[2][i]); y2 = max(y2, s[3][i]);
+      tw += sw[0][i];
+      th += sw[1][i];
     }
     atomicMin(&mbr[0], x1); atomicMin(&mbr[1], y1);
     atomicMax(&mbr[2], x2); atomicMax(&mbr[3], y2);
+    if (sums) {
+      atomicAdd(&sums[0], tw);
+      atomicAdd(&sums[1], th);
+    }
   }
 }
 
@@ -209,11 +226,12 @@ void launch_expect(const void* dev, const void* expected, size_t bytes, uint32_t
   }
 }
 
-void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st) {
+void launch_mbr(const vl_box* b, int64_t n, Xform xf, int32_t* mbr, uint32_t* err, hipStream_t st,
+                unsigned long long* sums) {
   if (n <= 0) return;
   int64_t g = (n + SCAN_NT - 1) / SCAN_NT;
   if (g > 1024) g = 1024;   // one set of 4 atomics per workgroup on shared addresses
-  hipLaunchKernelGGL(k_mbr, dim3((unsigned)g), dim3(SCAN_NT), 0, st, b, n, xf, mbr, err);
+  hipLaunchKernelGGL(k_mbr, dim3((unsigned)g), dim3(SCAN_NT), 0, st, b, n, xf, mbr, err, sums);
 }
 
 size_t scan_tmp_bytes(size_t n) {
