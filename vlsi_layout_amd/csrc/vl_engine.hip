@@ -251,6 +251,7 @@ struct vl_layer {
   // know it without a reduction (a fused group's outputs lie in its domain).
   uint64_t bound_version = ~0ull;
   int64_t bound[4] = {0, 0, 0, 0};
+  double shape_hint = 1.0;   // tile height / width suited to the boxes (with the bound)
 };
 
 namespace {
@@ -1361,6 +1362,15 @@ int layer_extent(vl_engine* e, const vl_layer* l, Box64* m) {
   return layer_mbr(e, l, m);
 }
 
+// Tile shape suited to a layer's boxes: from its summed widths and heights
+// when its bounding box is cached, else a producer's hint for its current
+// version, else square.
+double layer_shape(const vl_engine* e, const vl_layer* l) {
+  if (l->mbr_ok) return tile_shape(l->sum_w, l->sum_h, e->tile_shape);
+  if (l->bound_version == l->version && e->tile_shape > 0) return l->shape_hint;
+  return 1.0;
+}
+
 // ---- fused operator groups ------------------------------------------------------
 // Level 0 of a fused group: the fused tile kernel's tiers (10, 20 and 42 boxes
 // per window), then x-sub-column splits of the tiles above the second tier's
@@ -1618,6 +1628,7 @@ int run_fused(vl_engine* e, const vl_layer* S, const vl_layer* T, const FusedSpe
     // every output lies in the group's domain
     c[k]->bound[0] = dom.x1; c[k]->bound[1] = dom.y1; c[k]->bound[2] = dom.x2; c[k]->bound[3] = dom.y2;
     c[k]->bound_version = c[k]->version;
+    c[k]->shape_hint = shape;   // outputs follow the outlines of the source's boxes
   }
   return VL_OK;
 }
@@ -1686,13 +1697,14 @@ int probe_reduce(vl_engine* e, const Side& p, const Grid& g, const vl_box* entri
 // those) and the AND of the reduced inputs, which is the same point set, is
 // computed in full: a failing check costs a few passes over the inputs plus
 // an AND of the violating boxes.  The caller holds the engine.
-int and_check_core(vl_engine* e, const Side& A, const Side& B, const Box64& ma, const Box64& mb, vl_layer* c) {
+int and_check_core(vl_engine* e, const Side& A, const Side& B, const Box64& ma, const Box64& mb, vl_layer* c,
+                   double shape_a = 1.0, double shape_b = 1.0) {
   const Box64 dom{std::max(ma.x1, mb.x1), std::max(ma.y1, mb.y1), std::min(ma.x2, mb.x2), std::min(ma.y2, mb.y2)};
   if (A.n == 0 || B.n == 0 || dom.empty()) return set_empty(e, c);
   const bool build_a = A.n <= B.n;
   const Side& build = build_a ? A : B;
   const Side& probe = build_a ? B : A;
-  Grid g = make_grid(dom, build.n, e->check_target);
+  Grid g = make_grid(dom, build.n, e->check_target, build_a ? shape_a : shape_b);
   Scratch keep(e);
   vl_box* entries = nullptr;
   uint64_t* start = nullptr;
@@ -1717,7 +1729,7 @@ int and_check_core(vl_engine* e, const Side& A, const Side& B, const Box64& ma, 
   vl_box* p2 = nullptr;
   uint64_t np2 = 0;
   VLCK(probe_reduce(e, probe, g, entries, start, keep, &p2, &np2));
-  Grid g2 = make_grid(dom, (int64_t)np2, e->check_target);
+  Grid g2 = make_grid(dom, (int64_t)np2, e->check_target, build_a ? shape_b : shape_a);
   vl_box* entries2 = nullptr;
   uint64_t* start2 = nullptr;
   uint64_t total2 = 0;
@@ -1746,7 +1758,8 @@ int and_check(vl_engine* e, const vl_layer* a, const vl_layer* b, vl_layer* c) {
   Box64 ka, kb;
   VLCK(layer_extent(e, a, &ka));
   VLCK(layer_extent(e, b, &kb));
-  return and_check_core(e, Side{a->d, a->n, Xform{0, 0, 0, 0}}, Side{b->d, b->n, Xform{0, 0, 0, 0}}, ka, kb, c);
+  return and_check_core(e, Side{a->d, a->n, Xform{0, 0, 0, 0}}, Side{b->d, b->n, Xform{0, 0, 0, 0}}, ka, kb, c,
+                        layer_shape(e, a), layer_shape(e, b));
 }
 
 bool is_device_ptr(const void* p) {
