@@ -89,9 +89,15 @@ struct Pool {
   bool no_trim = false;
   Pool* fallback = nullptr;
   std::mutex m;   // the fallback's cache is freed from a worker's thread
+  // Size classes: powers of two up to 1 MiB, then four per power of two
+  // (a large buffer wastes at most a quarter of its size, not a half).
   static size_t round(size_t b) {
     size_t r = 256;
     while (r < b) r <<= 1;
+    if (r <= ((size_t)1 << 20)) return r;
+    const size_t q = r >> 3;   // r/2 < b <= r: the smallest of 5/8, 6/8, 7/8 and 8/8 of r that holds b
+    for (size_t k = 5; k < 8; k++)
+      if (k * q >= b) return k * q;
     return r;
   }
   void* get(size_t b, size_t* rounded) {
@@ -2515,6 +2521,7 @@ struct vl_program {
   std::vector<void*> pinned;                                   // sources of captured uploads
   std::vector<std::pair<const vl_layer*, uint64_t>> versions;  // layer versions the graph holds for
   int64_t cap_failures = 0, cap_fused = 0;                     // per replay
+  int graph_nw = 0;                                            // worker streams the graph was captured on
   int64_t captures = 0, replays = 0, fallbacks = 0;
 };
 static int run_prog_op(vl_engine* e, const ProgOp& p, int64_t* failures) {
@@ -3110,6 +3117,7 @@ static int graph_record_capture(vl_program* p, const std::vector<ProgStep>& step
   if (crc == VL_OK && ok) {
     for (const vl_layer* l : program_layers(p)) p->versions.push_back({l, l->version});
     p->captures++;
+    p->graph_nw = nw;
     return VL_OK;
   }
   // the capture replaced the outputs' storage: compute them again eagerly
@@ -3131,6 +3139,43 @@ int vl_program_run(vl_program* p) {
   const std::vector<ProgStep> steps = compile_program(p->ops, e->fuse, e->fuse2 != 0, e->and_check != 0);
   const std::vector<std::vector<int>> chains = program_chains(p->ops, steps);
   int nw = (int)std::min<size_t>(chains.size(), (size_t)std::min(p->max_workers, p->e->workers));
+  // Chains run side by side only as far as their scratch fits in device
+  // memory: an operator holds up to about 300 bytes per input box at its
+  // peak (tile entries, strip streams, outputs, pool rounding), and a
+  // chain's operators work on layers about as large as the chain's inputs
+  // (the layers no operator of the program writes, whose sizes do not change
+  // from run to run).  The chains with the largest inputs count first.
+  if (nw > 1) {
+    auto written = [&](const vl_layer* l) {
+      for (const ProgOp& o : p->ops)
+        if (o.c == l) return true;
+      return false;
+    };
+    std::vector<double> est;
+    for (const auto& ch : chains) {
+      double in = 0;
+      std::vector<const vl_layer*> seen;
+      for (int si : ch)
+        for (int k : steps[si].ops)
+          for (const vl_layer* l : {p->ops[k].a, p->ops[k].b})
+            if (l && !written(l) && std::find(seen.begin(), seen.end(), l) == seen.end()) {
+              seen.push_back(l);
+              in += (double)l->n;
+            }
+      est.push_back(300.0 * in);
+    }
+    std::sort(est.begin(), est.end(), std::greater<double>());
+    const double avail = 0.7 * (double)e->pool.available();
+    double acc = 0;
+    int fit = 0;
+    for (int k = 0; k < nw; k++) {
+      acc += est[k];
+      if (k > 0 && acc > avail) break;
+      fit++;
+    }
+    if (fit < nw && g_debug) fprintf(stderr, "[vl] program: %d of %d worker streams fit in device memory\n", fit, nw);
+    nw = std::max(1, fit);
+  }
   // the bounding boxes of the program's inputs (layers no operator writes)
   // are cached before concurrent chains read them
   {
@@ -3152,8 +3197,10 @@ int vl_program_run(vl_program* p) {
   int rc = VL_OK;
   if (p->graph && !chains.empty()) {
     // a captured program runs on worker engines only (their memory pools
-    // hold the graph's buffers), one per chain up to max_workers
+    // hold the graph's buffers), one per chain up to max_workers; a replay
+    // checks the streams the graph was captured on
     nw = std::max(nw, 1);
+    if (p->gexec) nw = p->graph_nw;
     rc = program_workers(p, nw);
     bool replayed = false;
     if (rc == VL_OK && p->gexec) {
