@@ -606,3 +606,37 @@ def test_two_layer_group_graph_replays(gpu_engine):
         for k, (g, x) in enumerate(zip(snap, exp)):
             assert np.array_equal(g, x), (rep, k)
     eng.close()
+
+
+@pytest.mark.parametrize("shape", [0, 4, 32, 1024])
+def test_tile_shape_keeps_results(gpu_engine, shape):
+    """Grids shaped like the boxes (tall tiles for vertical wires, wide ones
+    for horizontal wires, up to the tile_shape factor; 0 = square): every
+    plain operator, the erosion, a fused ring and a check give the oracle's
+    result on wire layers of both orientations."""
+    rng = np.random.default_rng(500 + shape)
+    vert = V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 3, 15000)            # met2: 8 x L vertical
+    horiz = V.synth_layer(V.SYN_STDCELL, 0x5EED0002, 2, 15000)           # met1: L x 8 horizontal
+    x = rng.integers(0, 1 << 18, 3000)
+    y = rng.integers(0, 1 << 18, 3000)
+    tall = np.stack([x, y, x + 2, y + rng.integers(500, 20000, 3000)], 1).astype(np.int32)   # 2-DBU slivers
+    eng = V.Engine(0)
+    eng.set("tile_shape", shape)
+    for name, a, b in (("met2/met1", vert, horiz), ("slivers/met2", tall, vert)):
+        A, B = V.Layer(eng, a), V.Layer(eng, b)
+        for op, fn in ((V.K_BOOST_CONTROL_AND, O.op_and), (V.K_BOOST_CONTROL_NOT, O.op_not),
+                       (V.K_BOOST_CONTROL_OR, O.op_or), (V.K_BOOST_CONTROL_XOR, O.op_xor)):
+            C = V.Layer(eng)
+            V.CalculateBoostOperation(A, B, C, op)
+            assert np.array_equal(O.sort_boxes(C.boxes()), fn(a, b)), (name, op, shape)
+        G, S = V.Layer(eng), V.Layer(eng)
+        V.CalculateGROWLayer(A, G, (2, 1, 0, 3))
+        assert np.array_equal(O.sort_boxes(G.boxes()), O.grow(a, 2, 1, 0, 3)), (name, shape)
+        V.CalculateBoostOperation(A, A, S, V.K_BOOST_CONTROL_SIZE, -1)
+        assert np.array_equal(O.sort_boxes(S.boxes()), O.size(a, -1)), (name, shape)
+        got, fails = run_group(eng, A, GROUPS["op1"])
+        exp = oracle_group(a, GROUPS["op1"])
+        for k, (g, e) in enumerate(zip(got, exp)):
+            assert np.array_equal(O.sort_boxes(g.boxes()), e), (name, shape, k)
+        assert fails == 0
+    eng.close()
