@@ -87,6 +87,13 @@ int vl_engine_destroy(vl_engine* e);
  *                  box and event sorts): 8 (default) or 11
  *   "workers"      programs: chains run concurrently on up to this many
  *                  worker streams (1 to 16, default 4)
+ *   "fuse2"        programs: 1 lets a fused group take a second layer as an
+ *                  image (default 0; set by the autotuner)
+ *   "and_check"    programs: 1 (default) tests an AND read only by
+ *                  ASSERT_ZERO for emptiness instead of computing it
+ *   "check_target" mean binned boxes per tile of that test's grid (6)
+ *   "tile_shape"   tiles shaped like the boxes, up to this aspect (32;
+ *                  0: square tiles)
  *   "profile"      1: collect per-kernel statistics (vl_engine_kernel_stats)
  * Read-only: "fused_groups" = groups run fused by programs on this engine. */
 int vl_engine_set_param(vl_engine* e, const char* key, int64_t value);
