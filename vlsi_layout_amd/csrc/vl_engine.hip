@@ -3015,8 +3015,11 @@ static void forget_output_mbrs(vl_program* p) {
 // with held (*ok).
 static int graph_replay(vl_program* p, int nw, bool* ok) {
   vl_engine* e = p->e;
+  const double tl = g_debug ? now_ms() : 0.0;
   HIPCK(hipGraphLaunch(p->gexec, e->st));
+  if (g_debug) fprintf(stderr, "[vl] graph launch returned after %.2f ms\n", now_ms() - tl);
   HIPCK(hipStreamSynchronize(e->st));
+  if (g_debug) fprintf(stderr, "[vl] graph replay done after %.2f ms\n", now_ms() - tl);
   uint32_t flag = 0;
   *ok = true;
   for (int k = 0; k < nw; k++) {
