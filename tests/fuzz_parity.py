@@ -1,14 +1,17 @@
 """Randomized parity sweep on a GPU (run by hand; pytest does not collect it):
 
-    python tests/fuzz_parity.py SECONDS SEED MAX_BOXES
+    python tests/fuzz_parity.py SECONDS SEED MAX_BOXES [server]
 
 Draws cases until SECONDS have passed: two layers of random families (the
 generators of test_gpu_mixed_layouts: wires of both orientations, buses,
 clusters, standard-cell and routing layers) of up to MAX_BOXES boxes each,
 random tile, fused, check and erosion budgets, then the same checks as that
 test (AND, NOT, OR, XOR, GROW, SIZE, and the ring program with a checked AND,
-eager and as a replayed graph) against the oracle.  Prints each case's seed
-before running it; exits 1 if any case differs."""
+eager and as a replayed graph) against the oracle.  With `server`, each case
+is instead a query server over four such layers answering random windows
+(degenerate, outside the layers, covering them) with spacings 0..8, every
+result's count and order-independent hash against the oracle.  Prints each
+case's seed before running it; exits 1 if any case differs."""
 import os
 import sys
 import time
@@ -21,6 +24,7 @@ sys.path.insert(0, TESTS)
 import oracle_lib as O  # noqa: E402
 import vlsi_layout_amd as V  # noqa: E402
 from test_gpu_mixed_layouts import KINDS, layout, ring_program, same  # noqa: E402
+from test_gpu_server import expected  # noqa: E402
 
 
 def run_case(eng, s, nmax):
@@ -77,17 +81,59 @@ def run_case(eng, s, nmax):
     return bad
 
 
+def run_server_case(eng, s, nmax):
+    r = np.random.default_rng(s)
+    kinds = [KINDS[r.integers(len(KINDS))] for _ in range(4)]
+    boxes = [layout(r, k, int(r.integers(50, nmax))) for k in kinds]
+    print(f"seed {s}: server over {kinds} {[len(b) for b in boxes]}", flush=True)
+    layers = [V.Layer(eng, b) for b in boxes]
+    allb = np.concatenate(boxes)
+    X0, Y0, X1, Y1 = (int(v) for v in (allb[:, 0].min(), allb[:, 1].min(), allb[:, 2].max(), allb[:, 3].max()))
+    n = 24
+    q = np.zeros(n, V.QUERY_DTYPE)
+    t = 0.0
+    for i in range(n):
+        form = int(r.integers(5))
+        if form == 0:      # tiny
+            x, y = int(r.integers(X0, X1)), int(r.integers(Y0, Y1))
+            w = (x, y, x + int(r.integers(1, 4)), y + int(r.integers(1, 4)))
+        elif form == 1:    # the whole extent and more
+            w = (X0 - 10, Y0 - 10, X1 + 10, Y1 + 10)
+        elif form == 2:    # outside the layers
+            w = (X1 + 5, Y1 + 5, X1 + 50, Y1 + 50)
+        else:
+            sx = max(1, int(r.uniform(0.01, 0.6) * (X1 - X0)))
+            sy = max(1, int(r.uniform(0.01, 0.6) * (Y1 - Y0)))
+            x, y = int(r.integers(X0 - sx // 2, X1)), int(r.integers(Y0 - sy // 2, Y1))
+            w = (x, y, x + sx, y + sy)
+        q[i] = (*w, int(r.integers(4)), int(r.integers(4)), int(r.choice([0, 1, 2, 3, 5, 8])), 0, t)
+        t += float(r.exponential(0.3))
+    srv = V.Server(eng, layers, slots=int(r.choice([1, 2, 4])))
+    rc, res = srv.run(q)
+    bad = []
+    if rc != 0 or (res["status"] != 0).any():
+        bad.append(f"status rc={rc} {res['status'].tolist()}")
+    else:
+        for i in range(n):
+            cnt, h = expected(boxes, q[i])
+            if res["count"][i] != cnt or res["hash"][i] != h:
+                bad.append(f"query {i} {q[i]}: count {res['count'][i]} want {cnt}")
+    srv.close()
+    return bad
+
+
 def main():
     seconds = float(sys.argv[1]) if len(sys.argv) > 1 else 240.0
     seed = int(sys.argv[2]) if len(sys.argv) > 2 else 1
     nmax = int(sys.argv[3]) if len(sys.argv) > 3 else 30000
+    case_fn = run_server_case if len(sys.argv) > 4 and sys.argv[4] == "server" else run_case
     end = time.time() + seconds
     rng = np.random.default_rng(seed)
     eng = V.Engine(0)
     cases = fails = 0
     while time.time() < end:
         cases += 1
-        bad = run_case(eng, int(rng.integers(1, 1 << 31)), nmax)
+        bad = case_fn(eng, int(rng.integers(1, 1 << 31)), nmax)
         if bad:
             fails += 1
             print("  MISMATCH:", bad, flush=True)
