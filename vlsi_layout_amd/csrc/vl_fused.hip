@@ -70,8 +70,12 @@ __device__ __forceinline__ uint64_t tt_eval_common(uint32_t tt, uint64_t s, uint
   }
 }
 
+// The 64-slot tier asks for 6 waves per SIMD (at most 80 VGPRs, no spills):
+// its shared strip pass holds every output's transposed end marks, and
+// without the bound the compiler takes 86 VGPRs (5 waves per SIMD), which
+// costs more than the pass saves.
 template <int P>
-__global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc, int ntiles,
+__global__ __launch_bounds__(64, P == 64 ? 6 : 1) void k_tile_f(const TileDesc* __restrict__ desc, int ntiles,
                                                const vl_box* __restrict__ entries, FusedSpec fs, FusedOut o) {
   constexpr int CPB = 6;               // coordinates per box and axis
   // the slot group after the boxes holds the window and tile borders: four
@@ -490,8 +494,153 @@ __global__ __launch_bounds__(64) void k_tile_f(const TileDesc* __restrict__ desc
         }
         __syncthreads();
       }
+      bool shared_pass = false;
+      if constexpr (NW == 1) {
+        if (onepass) {
+          // One-word grids, up to four outputs: each output's marks, stream
+          // range and transposed end marks as below, but the strip starts of
+          // all outputs go into one table (output q's entries contiguous, in
+          // output order), and one pass over the table writes every output's
+          // strips: entry g of output q goes to position off[q] + g, where
+          // off[q] = its range's base - the table index of its first entry
+          // (the same positions, hence the same order, as one pass per
+          // output).  The table is written out whenever it cannot take a
+          // full round (a start per lane).
+          shared_pass = true;
+          constexpr int Q = EROW / WAVE;
+          uint16_t* const stab = erow;   // (j << 8) | (q << 6) | b
+          uint64_t reQ[NQ];              // transposed end marks of each output
+          uint32_t TQ[NQ], sideQ[NQ];
+          uint64_t off[NQ];
+          uint32_t okmask = 0;           // outputs whose range fits their stream
+          uint32_t tab = 0;              // entries in the table (uniform)
+          uint64_t gbase = 0;            // table index of stab[0] over the tile
+#pragma unroll
+          for (int q = 0; q < NQ; q++) {
+            reQ[q] = 0ull;
+            TQ[q] = 0u;
+            sideQ[q] = 0u;
+            off[q] = 0ull;
+          }
+          auto flush = [&]() {
+            __syncthreads();
+            for (uint32_t g0 = 0; g0 < tab; g0 += WAVE) {
+              const uint32_t g = g0 + (uint32_t)lane;
+              const bool has = g < tab;
+              const uint32_t ent = has ? (uint32_t)stab[g] : 0u;
+              const int j = (int)(ent >> 8), qq = (int)((ent >> 6) & 3u), b = (int)(ent & 63u);
+              uint64_t rr[1] = {0ull};
+              uint64_t po = 0ull;
+#pragma unroll
+              for (int q = 0; q < NQ; q++) {
+                if (q >= nout) break;
+                const uint64_t v = lane_read64(reQ[q], b);   // every lane active
+                if (qq == q) {
+                  rr[0] = v;
+                  po = off[q];
+                }
+              }
+              uint32_t tg = 0u;
+              if (has) {
+                const int k = next_bit_above<1>(rr, j, ncol);
+                const int a = run_start_w<1>(MTQ[qq] + j, 1, b);
+                tg = ((a == rowT0 && (flags & SEAM_B)) ? SEAM_B : 0u) |
+                     ((b == rowT1 && (flags & SEAM_T)) ? SEAM_T : 0u) |
+                     ((j == colT0 && (flags & SEAM_L)) ? SEAM_L : 0u) |
+                     ((k == colT1 && (flags & SEAM_R)) ? SEAM_R : 0u);
+                if ((okmask >> qq) & 1u) {
+                  const uint64_t p = po + gbase + g;
+                  gstore_box(s_strips[qq] + p, ux[j], uy[a], ux[k], uy[b]);
+                  gstore(s_tag[qq] + p, (uint8_t)tg);
+                }
+              }
+#pragma unroll
+              for (int q = 0; q < NQ; q++)
+                sideQ[q] += (uint32_t)__popcll(__ballot(tg != 0u && qq == q));
+            }
+            gbase += tab;
+            tab = 0;
+            __syncthreads();
+          };
+#pragma unroll
+          for (int q = 0; q < NQ; q++) {
+            if (q >= nout) break;
+            if (!((anymask >> q) & 1u)) {
+              if (lane == 0) {
+                gstore(s_tn[q] + ti, 0u);
+                gstore(s_tn[q] + o.ntab + ti, 0u);
+              }
+              continue;
+            }
+            const uint64_t* MO = MTQ[q];
+            uint64_t Sr = 0ull, Er = 0ull;
+            uint32_t myT = 0;
+            if (lane <= ncol) {
+              const uint64_t* cur = (lane < ncol) ? MO + lane : nullptr;
+              const uint64_t* prv = (lane > 0) ? MO + (lane - 1) : nullptr;
+              uint64_t D[1];
+              change_mask<1>(cur, prv, 1, D);
+              ends_meeting<1>(cur, 1, D, &Sr);
+              ends_meeting<1>(prv, 1, D, &Er);
+              myT = (uint32_t)__popcll(Er);
+            }
+            const uint32_t T = wave_sum(myT);
+            uint64_t cpos = uniform(s_cpos[q]), cend = uniform(s_cend[q]);
+            if (cpos + T > cend) {
+              const uint64_t want = (uint64_t)T > o.chunk ? (uint64_t)T : o.chunk;
+              unsigned long long nb = 0;
+              if (lane == 0) nb = atomicAdd(pick_out(o.cnt, q) + 2, (unsigned long long)want);
+              cpos = read_lane((uint64_t)nb, 0);
+              cend = cpos + want;
+            }
+            const bool ok = cpos + T <= pick_out(o.stream_cap, q);
+            if (ok) okmask |= 1u << q;
+            const uint64_t used = uniform(s_used[q]);
+            if (lane == 0) {
+              gstore(s_toff[q] + ti, cpos);
+              s_used[q] = used + T;
+              s_cpos[q] = cpos + T;
+              s_cend[q] = cend;
+              if (!ok) s_short = 1u;
+            }
+            TQ[q] = T;
+            reQ[q] = transpose64(Er, lane);
+            off[q] = cpos - (gbase + tab);
+            // the starts of output q: lane l owns boundary l, its S marks are
+            // the rows b just above the runs of the strips starting there
+            uint64_t m = Sr;
+            while (__ballot(m != 0ull)) {
+              if (tab + WAVE > (uint32_t)EROW) flush();
+#pragma unroll
+              for (int r = 0; r < Q; r++) {
+                if (tab + WAVE > (uint32_t)EROW) break;
+                const uint64_t act = __ballot(m != 0ull);
+                if (!act) break;
+                if (m) {
+                  const uint32_t t = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                  m &= m - 1;
+                  stab[tab + lanes_below(act)] = (uint16_t)(((uint32_t)lane << 8) | ((uint32_t)q << 6) | t);
+                }
+                tab += (uint32_t)__popcll(act);
+              }
+            }
+          }
+          if (tab) flush();
+          if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+              if (q >= nout) break;
+              if (!((anymask >> q) & 1u)) continue;
+              gstore(s_tn[q] + ti, TQ[q] - sideQ[q]);
+              gstore(s_tn[q] + o.ntab + ti, sideQ[q]);
+              s_side[q] += sideQ[q];
+            }
+          }
+          __syncthreads();
+        }
+      }
 #pragma unroll 1
-      for (int q = 0; q < nout; q++) {
+      for (int q = 0; q < (shared_pass ? 0 : nout); q++) {
         uint32_t* const tn = uniform(s_tn[q]);
         const uint64_t* MO = MT;
         if (onepass) {
