@@ -5,7 +5,8 @@
 Draws cases until SECONDS have passed: two layers of random families (the
 generators of test_gpu_mixed_layouts: wires of both orientations, buses,
 clusters, standard-cell and routing layers) of up to MAX_BOXES boxes each,
-random tile, fused, check and erosion budgets, then the same checks as that
+random tile, fused, check and erosion budgets, fused groups over one or
+two layers, then the same checks as that
 test (AND, NOT, OR, XOR, GROW, SIZE, and the ring program with a checked AND,
 eager and as a replayed graph) against the oracle.  With `server`, each case
 is instead a query server over four such layers answering random windows
@@ -34,7 +35,7 @@ def run_case(eng, s, nmax):
     b = layout(r, kb, int(r.integers(50, nmax)))
     params = {"tile_shape": int(r.choice([0, 4, 32])), "fuse_target": int(r.choice([3, 6, 9])),
               "check_target": int(r.choice([2, 6, 12])), "tile_target": int(r.choice([6, 20, 40])),
-              "erode_target": int(r.choice([6, 12]))}
+              "erode_target": int(r.choice([6, 12])), "fuse2": int(r.integers(2))}
     for k, v in params.items():
         eng.set(k, v)
     print(f"seed {s}: {ka}({len(a)}) x {kb}({len(b)}), {params}", flush=True)
